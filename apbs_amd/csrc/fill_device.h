@@ -1,0 +1,118 @@
+// fill_device.h -- coefficient maps, boundary values, energies and forces on
+// the device (Vpmg_fillco, bcCalc, focusFillBound, Vpmg_*Energy, Vpmg_*Force).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+namespace pmg {
+
+// Grid geometry of one Vpmg (pmgp->xmin.. as set by Vpmg_fillco).
+struct Geo {
+  int nx, ny, nz;
+  double hx, hy, hz;
+  double xmin, ymin, zmin, xmax, ymax, zmax;
+};
+
+// Uniform cell hash over points (atoms or SAS points).  Cells hold point
+// indices in ascending order, so a k-way merge over neighbouring cells visits
+// the points in index order.
+struct Cells {
+  double lo[3];
+  double cs;       // cell edge (A)
+  int dim[3];
+  const int* start;  // dim0*dim1*dim2 + 1
+  const int* list;
+};
+
+struct AtomsDev {
+  int n;
+  const double* pos;     // 3n
+  const double* rad;
+  const double* chg;
+  const double* partID;  // n
+};
+
+// ------------------------------------------------------------------ charges
+// chgm spl2 (vpmg.c:5123-5248) / spl0 (vpmg.c:4986-5089), charge already
+// scaled by zmagic/(hx hy hz) per atom order of the reference.
+void charge_map(hipStream_t s, const Geo& g, const AtomsDev& A, const Cells& C, int spl2,
+                double zmagic, int bcfl_lenient, double* charge);
+
+// ------------------------------------------------------------- coefficients
+// srfm spl2 (vpmg.c:4631-4854): kappa, epsx/y/z in [0,1] windows, then the
+// linear map to epsp..epsw and the ion mask.
+void coef_spline(hipStream_t s, const Geo& g, const AtomsDev& A, const Cells& C, double win,
+                 double irad, double ionmask, double epsw, double epsp, double* kappa,
+                 double* epsx, double* epsy, double* epsz);
+
+// SAS points of every atom (Vacc_atomSurf, vacc.c:866-922): flags[a*nref+q]
+// = 1 when the point rad*ref+apos is outside every other inflated atom.
+void sas_flags(hipStream_t s, const AtomsDev& A, const Cells& C, const double* ref, int nref,
+               double probe, unsigned char* flags);
+// compact flagged points to xyz (order of (atom, ref) index); returns count
+// on the host (synchronises).
+int sas_compact(hipStream_t s, const AtomsDev& A, const double* ref, int nref, double probe,
+                const unsigned char* flags, double* xyz, int* counter, int cap);
+
+// markSphere (vpmg.c:6444-6491) for a set of spheres: mode 0 atoms (radius
+// r_a + add, strict on-mesh test, dielectric), mode 1 atoms (r_a + add,
+// inflated test, ion accessibility), mode 2 points of radius add (SAS).
+// which: 0/1/2 = x/y/z shifted grid, 3 = unshifted.
+void mark_spheres(hipStream_t s, const Geo& g, int npts, const double* pos, const double* rad,
+                  double add, int mode, int which, double val, double* arr);
+// 9-point harmonic smoothing (vpmg.c:4500-4628); a1..a3 are the unsmoothed maps.
+void mol_smooth(hipStream_t s, const Geo& g, const double* a1, const double* a2, const double* a3,
+                double* epsx, double* epsy, double* epsz);
+
+// ------------------------------------------------------------- boundaries
+// bcflnew (vpmg.c:3289-3367): Debye-Hueckel sum over all atoms on the faces.
+void bc_mdh(hipStream_t s, const Geo& g, const double* xf, const double* yf, const double* zf,
+            const AtomsDev& A, double pre1, double xkappa, double* gx, double* gy, double* gz);
+// focusFillBound (vpmg.c:1914-2414): trilinear values of the old solution.
+struct OldGrid {
+  int nx, ny, nz;
+  double hx, hy, hz, xmin, ymin, zmin, xmax, ymax, zmax;
+  const double* u;
+};
+void focus_bound(hipStream_t s, const Geo& g, double xminN, double yminN, double zminN,
+                 double xmaxN, double ymaxN, double zmaxN, const OldGrid& o, double* gx,
+                 double* gy, double* gz, int* bad);
+
+// --------------------------------------------------------------- partition
+// Vpmg_setPart grid part (vpmg.c:720-870) with all bflags 0; invert = 1
+// applies extEnergy's flip (vpmg.c:2491-2509).
+void set_part(hipStream_t s, const Geo& g, double xcen0, double ycen0, double zcen0,
+              const double lo[3], const double hi[3], int invert, double* pvec);
+
+// ---------------------------------------------------------------- energies
+// Each writes the raw sum to out (device); callers apply h^3/zmagic.
+void energy_qf_volume(hipStream_t s, size_t n, const double* pvec, const double* u,
+                      const double* charge, double* part, double* out);
+void energy_diel(hipStream_t s, const Geo& g, const double* pvec, const double* u,
+                 const double* ex, const double* ey, const double* ez, double* part,
+                 double* out);
+// nonlinear: sum pvec kappa zks2 c_j (exp(-q_j u) - 1); linear: sum pvec zkappa2 kappa u^2
+struct IonBulk {
+  int nion;
+  double q[10], c[10];
+};
+void energy_qm(hipStream_t s, size_t n, int nonlin, const double* pvec, const double* kappa,
+               const double* u, double zks2, double zkappa2, const IonBulk& ions, double* part,
+               double* out, int* nchop);
+// spl0 point energy: per atom trilinear u at the atom times charge*partID
+void energy_qf_point(hipStream_t s, const Geo& g, const AtomsDev& A, const double* u,
+                     double* part, double* out);
+
+// ------------------------------------------------------------------ forces
+// Per-atom force components (kT/A), out[3*a..3*a+2]; rc[a] = 1/0 like the
+// reference's return codes.
+void force_qf(hipStream_t s, const Geo& g, const AtomsDev& A, int spl2, const double* u,
+              double* out);
+void force_ib(hipStream_t s, const Geo& g, const AtomsDev& A, double win, double irad,
+              double zkappa2, double izmagic, int nonlin, double ionstr, const IonBulk& ions,
+              const double* kappa, const double* u, double* out);
+void force_db(hipStream_t s, const Geo& g, const AtomsDev& A, double win, double srad,
+              double epsp, double epsw, double izmagic, const double* epsx, const double* epsy,
+              const double* epsz, const double* u, double* out, int* rc);
+
+}  // namespace pmg

@@ -1,0 +1,983 @@
+// fill_device.hip -- coefficient maps, boundary values, energies and forces.
+//
+// Order-sensitive maps (spline charges, the multiplicative spline dielectric
+// and kappa windows) are gathers: each grid point visits the atoms that reach
+// it in ascending atom index, which is the order in which the reference's
+// per-atom scatter loops (src/mg/vpmg.c:4631-4854, 4986-5248) update that
+// point.  The result is deterministic and rounds like the reference.
+#include "fill_device.h"
+
+#include <cmath>
+
+namespace pmg {
+namespace {
+
+constexpr int kB = 256;
+constexpr int kParts = 1024;
+constexpr double VSMALL = 1.0e-9;
+constexpr double VPMGSMALL = 1e-12;
+
+__host__ __device__ inline size_t IX(int i, int j, int k, int nx, int ny) {
+  return ((size_t)k * ny + j) * nx + i;
+}
+
+inline unsigned nb(size_t n) {
+  size_t b = (n + kB - 1) / kB;
+  return (unsigned)(b == 0 ? 1 : b);
+}
+
+// ------------------------------------------------------- ordered cell merge
+struct Merge {
+  int cur[27], end[27];
+  int nc;
+};
+
+__device__ inline void merge_init(Merge& m, const Cells& C, double x, double y, double z) {
+  const int cx = (int)floor((x - C.lo[0]) / C.cs);
+  const int cy = (int)floor((y - C.lo[1]) / C.cs);
+  const int cz = (int)floor((z - C.lo[2]) / C.cs);
+  m.nc = 0;
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        const int a = cx + dx, b = cy + dy, c = cz + dz;
+        if (a < 0 || b < 0 || c < 0 || a >= C.dim[0] || b >= C.dim[1] || c >= C.dim[2]) continue;
+        const int id = a + C.dim[0] * (b + C.dim[1] * c);
+        const int s = C.start[id], e = C.start[id + 1];
+        if (s < e) {
+          m.cur[m.nc] = s;
+          m.end[m.nc] = e;
+          m.nc++;
+        }
+      }
+}
+
+// next point index in ascending order, -1 when exhausted
+__device__ inline int merge_next(Merge& m, const int* list) {
+  int best = -1, bi = -1;
+  for (int c = 0; c < m.nc; c++) {
+    if (m.cur[c] < m.end[c]) {
+      const int v = list[m.cur[c]];
+      if (best < 0 || v < best) {
+        best = v;
+        bi = c;
+      }
+    }
+  }
+  if (bi >= 0) m.cur[bi]++;
+  return best;
+}
+
+// ----------------------------------------------------------------- splines
+__device__ inline double bspline2(double x) {  // vpmg.c:5091-5104
+  double m2m, m2, m3;
+  if ((x >= 0.0) && (x <= 2.0)) m2m = 1.0 - fabs(x - 1.0); else m2m = 0.0;
+  if ((x >= 1.0) && (x <= 3.0)) m2 = 1.0 - fabs(x - 2.0); else m2 = 0.0;
+  if ((x >= 0.0) && (x <= 3.0)) m3 = 0.5 * x * m2m + 0.5 * (3.0 - x) * m2; else m3 = 0.0;
+  return m3;
+}
+
+__device__ inline double dbspline2(double x) {  // vpmg.c:5106-5121
+  double m2m, m2;
+  if ((x >= 0.0) && (x <= 2.0)) m2m = 1.0 - fabs(x - 1.0); else m2m = 0.0;
+  if ((x >= 1.0) && (x <= 3.0)) m2 = 1.0 - fabs(x - 2.0); else m2 = 0.0;
+  return m2m - m2;
+}
+
+__device__ inline double vfchi(int iint, double iflt) { return 1.5 + ((double)(iint) - (iflt)); }
+
+__device__ inline int imin2(int a, int b) { return a <= b ? a : b; }
+__device__ inline int imax2(int a, int b) { return a >= b ? a : b; }
+
+// ----------------------------------------------------------------- charges
+__global__ void __launch_bounds__(kB)
+    k_charge(Geo g, AtomsDev A, Cells C, int spl2, double zmagic, double* charge) {
+  const int t = blockIdx.x * kB + threadIdx.x;
+  const int k = blockIdx.y;
+  const int i = t % g.nx, j = t / g.nx;
+  if (j >= g.ny) return;
+  const double x = g.xmin + i * g.hx, y = g.ymin + j * g.hy, z = g.zmin + k * g.hz;
+  Merge m;
+  merge_init(m, C, x, y, z);
+  double sum = 0.0;
+  for (int a = merge_next(m, C.list); a >= 0; a = merge_next(m, C.list)) {
+    const double* ap = A.pos + 3 * a;
+    const double fi = (ap[0] - g.xmin) / g.hx, fj = (ap[1] - g.ymin) / g.hy,
+                 fk = (ap[2] - g.zmin) / g.hz;
+    const double c = A.chg[a] * zmagic / (g.hx * g.hy * g.hz);
+    if (spl2) {
+      if ((ap[0] <= (g.xmin - g.hx)) || (ap[0] >= (g.xmax + g.hx)) || (ap[1] <= (g.ymin - g.hy)) ||
+          (ap[1] >= (g.ymax + g.hy)) || (ap[2] <= (g.zmin - g.hz)) || (ap[2] >= (g.zmax + g.hz)))
+        continue;
+      const int ip2 = imin2((int)ceil(fi) + 1, g.nx - 1), im2 = imax2((int)floor(fi) - 1, 0);
+      const int jp2 = imin2((int)ceil(fj) + 1, g.ny - 1), jm2 = imax2((int)floor(fj) - 1, 0);
+      const int kp2 = imin2((int)ceil(fk) + 1, g.nz - 1), km2 = imax2((int)floor(fk) - 1, 0);
+      if (i < im2 || i > ip2 || j < jm2 || j > jp2 || k < km2 || k > kp2) continue;
+      const double mx = bspline2(vfchi(i, fi)), my = bspline2(vfchi(j, fj)),
+                   mz = bspline2(vfchi(k, fk));
+      sum += (c * mx * my * mz);
+    } else {
+      if ((ap[0] <= g.xmin) || (ap[0] >= g.xmax) || (ap[1] <= g.ymin) || (ap[1] >= g.ymax) ||
+          (ap[2] <= g.zmin) || (ap[2] >= g.zmax))
+        continue;
+      const int ihi = (int)ceil(fi), ilo = (int)floor(fi), jhi = (int)ceil(fj),
+                jlo = (int)floor(fj), khi = (int)ceil(fk), klo = (int)floor(fk);
+      const double dx = fi - (double)(ilo), dy = fj - (double)(jlo), dz = fk - (double)(klo);
+      // the eight statements of vpmg.c:5078-5085, in order
+      if (i == ihi && j == jhi && k == khi) sum += (dx * dy * dz * c);
+      if (i == ihi && j == jlo && k == khi) sum += (dx * (1.0 - dy) * dz * c);
+      if (i == ihi && j == jhi && k == klo) sum += (dx * dy * (1.0 - dz) * c);
+      if (i == ihi && j == jlo && k == klo) sum += (dx * (1.0 - dy) * (1.0 - dz) * c);
+      if (i == ilo && j == jhi && k == khi) sum += ((1.0 - dx) * dy * dz * c);
+      if (i == ilo && j == jlo && k == khi) sum += ((1.0 - dx) * (1.0 - dy) * dz * c);
+      if (i == ilo && j == jhi && k == klo) sum += ((1.0 - dx) * dy * (1.0 - dz) * c);
+      if (i == ilo && j == jlo && k == klo) sum += ((1.0 - dx) * (1.0 - dy) * (1.0 - dz) * c);
+    }
+  }
+  charge[IX(i, j, k, g.nx, g.ny)] = sum;
+}
+
+// ------------------------------------------------------------ spline coefs
+__global__ void __launch_bounds__(kB)
+    k_coef_spline(Geo g, AtomsDev A, Cells C, double win, double irad, double ionmask, double epsw,
+                  double epsp, double* kappa, double* epsx, double* epsy, double* epsz) {
+  const int t = blockIdx.x * kB + threadIdx.x;
+  const int k = blockIdx.y;
+  const int i = t % g.nx, j = t / g.nx;
+  if (j >= g.ny) return;
+  const double w2i = 1.0 / (win * win), w3i = 1.0 / (win * win * win);
+  const double hx = g.hx, hy = g.hy, hzed = g.hz;
+  double ka = 1.0, ex = 1.0, ey = 1.0, ez = 1.0;
+  Merge m;
+  merge_init(m, C, g.xmin + i * hx, g.ymin + j * hy, g.zmin + k * hzed);
+  for (int a = merge_next(m, C.list); a >= 0; a = merge_next(m, C.list)) {
+    const double* ap = A.pos + 3 * a;
+    const double arad = A.rad[a];
+    if ((ap[0] <= g.xmin) || (ap[0] >= g.xmax) || (ap[1] <= g.ymin) || (ap[1] >= g.ymax) ||
+        (ap[2] <= g.zmin) || (ap[2] >= g.zmax))
+      continue;
+    if (!(arad > VPMGSMALL)) continue;
+    const double p0 = ap[0] - g.xmin, p1 = ap[1] - g.ymin, p2 = ap[2] - g.zmin;
+    const double itot = irad + arad + win, itot2 = itot * itot;
+    const double ictot = fmax(0.0, (irad + arad - win)), ictot2 = ictot * ictot;
+    const double stot = arad + win, stot2 = stot * stot;
+    const double sctot = fmax(0.0, (arad - win)), sctot2 = sctot * sctot;
+    const double rtot = itot >= stot ? itot : stot;
+    const double dx = rtot + 0.5 * hx, dy = rtot + 0.5 * hy, dz = rtot + 0.5 * hzed;
+    const int imin = imax2(0, (int)floor((p0 - dx) / hx)), imax = imin2(g.nx - 1, (int)ceil((p0 + dx) / hx));
+    const int jmin = imax2(0, (int)floor((p1 - dy) / hy)), jmax = imin2(g.ny - 1, (int)ceil((p1 + dy) / hy));
+    const int kmin = imax2(0, (int)floor((p2 - dz) / hzed)), kmax = imin2(g.nz - 1, (int)ceil((p2 + dz) / hzed));
+    if (i < imin || i > imax || j < jmin || j > jmax || k < kmin || k > kmax) continue;
+    const double dx2 = (p0 - hx * i) * (p0 - hx * i);
+    const double dy2 = (p1 - hy * j) * (p1 - hy * j);
+    const double dz2 = (p2 - k * hzed) * (p2 - k * hzed);
+    if (ka > VPMGSMALL) {
+      const double d2 = dz2 + dy2 + dx2;
+      if (d2 <= ictot2) ka = 0.0;
+      if ((d2 < itot2) && (d2 > ictot2)) {
+        const double dist = sqrt(d2), sm = dist - (arad + irad) + win, sm2 = sm * sm;
+        ka *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+      }
+    }
+    if (ex > VPMGSMALL) {
+      const double q = p0 - (i + 0.5) * hx;
+      const double d2 = dz2 + dy2 + q * q;
+      if (d2 >= stot2) ex *= 1.0;
+      if (d2 <= sctot2) ex = 0.0;
+      if ((d2 > sctot2) && (d2 < stot2)) {
+        const double dist = sqrt(d2), sm = dist - arad + win, sm2 = sm * sm;
+        ex *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+      }
+    }
+    if (ey > VPMGSMALL) {
+      const double q = p1 - (j + 0.5) * hy;
+      const double d2 = dz2 + dx2 + q * q;
+      if (d2 >= stot2) ey *= 1.0;
+      if (d2 <= sctot2) ey = 0.0;
+      if ((d2 > sctot2) && (d2 < stot2)) {
+        const double dist = sqrt(d2), sm = dist - arad + win, sm2 = sm * sm;
+        ey *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+      }
+    }
+    if (ez > VPMGSMALL) {
+      const double q = p2 - (k + 0.5) * hzed;
+      const double d2 = dy2 + dx2 + q * q;
+      if (d2 >= stot2) ez *= 1.0;
+      if (d2 <= sctot2) ez = 0.0;
+      if ((d2 > sctot2) && (d2 < stot2)) {
+        const double dist = sqrt(d2), sm = dist - arad + win, sm2 = sm * sm;
+        ez *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+      }
+    }
+  }
+  const size_t p = IX(i, j, k, g.nx, g.ny);
+  kappa[p] = ionmask * ka;
+  epsx[p] = (epsw - epsp) * ex + epsp;
+  epsy[p] = (epsw - epsp) * ey + epsp;
+  epsz[p] = (epsw - epsp) * ez + epsp;
+}
+
+// ------------------------------------------------------------ SAS / markSphere
+__global__ void __launch_bounds__(kB)
+    k_sas_flags(AtomsDev A, Cells C, const double* ref, int nref, double probe,
+                unsigned char* flags) {
+  const long t = (long)blockIdx.x * kB + threadIdx.x;
+  if (t >= (long)A.n * nref) return;
+  const int a = (int)(t / nref), q = (int)(t % nref);
+  const double arad = A.rad[a];
+  if (arad < VSMALL) {
+    flags[t] = 0;
+    return;
+  }
+  const double rad = arad + probe;
+  const double* ap = A.pos + 3 * a;
+  const double px = rad * (ref[3 * q]) + ap[0];
+  const double py = rad * (ref[3 * q + 1]) + ap[1];
+  const double pz = rad * (ref[3 * q + 2]) + ap[2];
+  Merge m;
+  merge_init(m, C, px, py, pz);
+  unsigned char ok = 1;
+  for (int b = merge_next(m, C.list); b >= 0; b = merge_next(m, C.list)) {
+    if (b == a) continue;
+    const double* bp = A.pos + 3 * b;
+    const double d2 = (px - bp[0]) * (px - bp[0]) + (py - bp[1]) * (py - bp[1]) +
+                      (pz - bp[2]) * (pz - bp[2]);
+    const double rr = A.rad[b] + probe;
+    if (d2 < rr * rr) {
+      ok = 0;
+      break;
+    }
+  }
+  flags[t] = ok;
+}
+
+__global__ void k_sas_compact(AtomsDev A, const double* ref, int nref, double probe,
+                              const unsigned char* flags, double* xyz, int* counter, int cap) {
+  const long t = (long)blockIdx.x * kB + threadIdx.x;
+  if (t >= (long)A.n * nref) return;
+  if (!flags[t]) return;
+  const int a = (int)(t / nref), q = (int)(t % nref);
+  const double rad = A.rad[a] + probe;
+  const double* ap = A.pos + 3 * a;
+  const int slot = atomicAdd(counter, 1);
+  if (slot >= cap) return;
+  xyz[3 * slot] = rad * (ref[3 * q]) + ap[0];
+  xyz[3 * slot + 1] = rad * (ref[3 * q + 1]) + ap[1];
+  xyz[3 * slot + 2] = rad * (ref[3 * q + 2]) + ap[2];
+}
+
+// markSphere (vpmg.c:6444-6491) as a scatter: one thread per sphere writes
+// val at every grid point of the sphere.  Every writer of a point writes the
+// same constant, so the result does not depend on thread order.
+// mode 0: atom spheres r_a + add with the strict on-mesh test (dielectric),
+// mode 1: atom spheres r_a + add with the inflated on-mesh test (ions),
+// mode 2: SAS points with radius add (no mesh test).
+__global__ void __launch_bounds__(kB)
+    k_mark(Geo g, int npts, const double* pos, const double* rad, double add, int mode, double xs,
+           double ys, double zs, double val, double* arr) {
+  const int a = blockIdx.x * kB + threadIdx.x;
+  if (a >= npts) return;
+  const double* tp = pos + 3 * a;
+  double rtot = add;
+  if (mode != 2) {
+    const double ar = rad[a];
+    if (mode == 0) {
+      if ((tp[0] <= g.xmin) || (tp[0] >= g.xmax) || (tp[1] <= g.ymin) || (tp[1] >= g.ymax) ||
+          (tp[2] <= g.zmin) || (tp[2] >= g.zmax))
+        return;
+      if (!(ar > VSMALL)) return;
+    } else {
+      if (!(ar > VSMALL)) return;
+      if ((tp[0] < (g.xmin - add - ar)) || (tp[0] > (g.xmax + add + ar)) ||
+          (tp[1] < (g.ymin - add - ar)) || (tp[1] > (g.ymax + add + ar)) ||
+          (tp[2] < (g.zmin - add - ar)) || (tp[2] > (g.zmax + add + ar)))
+        return;
+    }
+    rtot = ar + add;
+  }
+  const double posx = tp[0] - xs, posy = tp[1] - ys, posz = tp[2] - zs;
+  const double rtot2 = rtot * rtot;
+  const double xr = rtot + 0.5 * g.hx, yr = rtot + 0.5 * g.hy, zr = rtot + 0.5 * g.hz;
+  const int imin = imax2(0, (int)ceil((posx - xr) / g.hx));
+  const int jmin = imax2(0, (int)ceil((posy - yr) / g.hy));
+  const int kmin = imax2(0, (int)ceil((posz - zr) / g.hz));
+  const int imax = imin2(g.nx - 1, (int)floor((posx + xr) / g.hx));
+  const int jmax = imin2(g.ny - 1, (int)floor((posy + yr) / g.hy));
+  const int kmax = imin2(g.nz - 1, (int)floor((posz + zr) / g.hz));
+  double fi = imin;
+  for (int i = imin; i <= imax; i++, fi += 1.) {
+    const double dx2 = (posx - g.hx * fi) * (posx - g.hx * fi);
+    double fj = jmin;
+    for (int j = jmin; j <= jmax; j++, fj += 1.) {
+      const double dy2 = (posy - g.hy * fj) * (posy - g.hy * fj);
+      if ((dx2 + dy2) > rtot2) continue;
+      double fk = kmin;
+      for (int k = kmin; k <= kmax; k++, fk += 1.) {
+        const double dz2 = (posz - g.hz * fk) * (posz - g.hz * fk);
+        if ((dz2 + dy2 + dx2) <= rtot2) arr[IX(i, j, k, g.nx, g.ny)] = val;
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kB)
+    k_mol_smooth(Geo g, const double* a1, const double* a2, const double* a3, double* epsx,
+                 double* epsy, double* epsz) {
+  const int t = blockIdx.x * kB + threadIdx.x;
+  const int k = blockIdx.y;
+  const int i = t % g.nx, j = t / g.nx;
+  if (j >= g.ny) return;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  double frac;
+  int np;
+  frac = 1.0 / a1[IX(i, j, k, nx, ny)];
+  frac += 1.0 / a2[IX(i, j, k, nx, ny)];
+  frac += 1.0 / a3[IX(i, j, k, nx, ny)];
+  np = 3;
+  if (j > 0) { frac += 1.0 / a2[IX(i, j - 1, k, nx, ny)]; np += 1; }
+  if (k > 0) { frac += 1.0 / a3[IX(i, j, k - 1, nx, ny)]; np += 1; }
+  if (i < nx - 1) {
+    frac += 1.0 / a2[IX(i + 1, j, k, nx, ny)];
+    frac += 1.0 / a3[IX(i + 1, j, k, nx, ny)];
+    np += 2;
+    if (j > 0) { frac += 1.0 / a2[IX(i + 1, j - 1, k, nx, ny)]; np += 1; }
+    if (k > 0) { frac += 1.0 / a3[IX(i + 1, j, k - 1, nx, ny)]; np += 1; }
+  }
+  epsx[IX(i, j, k, nx, ny)] = np / frac;
+
+  frac = 1.0 / a2[IX(i, j, k, nx, ny)];
+  frac += 1.0 / a1[IX(i, j, k, nx, ny)];
+  frac += 1.0 / a3[IX(i, j, k, nx, ny)];
+  np = 3;
+  if (i > 0) { frac += 1.0 / a1[IX(i - 1, j, k, nx, ny)]; np += 1; }
+  if (k > 0) { frac += 1.0 / a3[IX(i, j, k - 1, nx, ny)]; np += 1; }
+  if (j < ny - 1) {
+    frac += 1.0 / a1[IX(i, j + 1, k, nx, ny)];
+    frac += 1.0 / a3[IX(i, j + 1, k, nx, ny)];
+    np += 2;
+    if (i > 0) { frac += 1.0 / a1[IX(i - 1, j + 1, k, nx, ny)]; np += 1; }
+    if (k > 0) { frac += 1.0 / a3[IX(i, j + 1, k - 1, nx, ny)]; np += 1; }
+  }
+  epsy[IX(i, j, k, nx, ny)] = np / frac;
+
+  frac = 1.0 / a3[IX(i, j, k, nx, ny)];
+  frac += 1.0 / a1[IX(i, j, k, nx, ny)];
+  frac += 1.0 / a2[IX(i, j, k, nx, ny)];
+  np = 3;
+  if (i > 0) { frac += 1.0 / a1[IX(i - 1, j, k, nx, ny)]; np += 1; }
+  if (j > 0) { frac += 1.0 / a2[IX(i, j - 1, k, nx, ny)]; np += 1; }
+  if (k < nz - 1) {
+    frac += 1.0 / a1[IX(i, j, k + 1, nx, ny)];
+    frac += 1.0 / a2[IX(i, j, k + 1, nx, ny)];
+    np += 2;
+    if (i > 0) { frac += 1.0 / a1[IX(i - 1, j, k + 1, nx, ny)]; np += 1; }
+    if (j > 0) { frac += 1.0 / a2[IX(i, j - 1, k + 1, nx, ny)]; np += 1; }
+  }
+  epsz[IX(i, j, k, nx, ny)] = np / frac;
+}
+
+// -------------------------------------------------------------- boundaries
+// face f: 0 x-low, 1 x-high, 2 y-low, 3 y-high, 4 z-low, 5 z-high
+__device__ inline void face_point(const Geo& g, int f, int t, int& i, int& j, int& k) {
+  if (f < 2) {
+    j = t % g.ny;
+    k = t / g.ny;
+    i = f == 0 ? 0 : g.nx - 1;
+  } else if (f < 4) {
+    i = t % g.nx;
+    k = t / g.nx;
+    j = f == 2 ? 0 : g.ny - 1;
+  } else {
+    i = t % g.nx;
+    j = t / g.nx;
+    k = f == 4 ? 0 : g.nz - 1;
+  }
+}
+
+__device__ inline double* face_slot(const Geo& g, int f, int i, int j, int k, double* gx, double* gy,
+                                    double* gz) {
+  if (f < 2) return gx + j + (size_t)g.ny * k + (f == 1 ? (size_t)g.ny * g.nz : 0);
+  if (f < 4) return gy + i + (size_t)g.nx * k + (f == 3 ? (size_t)g.nx * g.nz : 0);
+  return gz + i + (size_t)g.nx * j + (f == 5 ? (size_t)g.nx * g.ny : 0);
+}
+
+__device__ inline int face_count(const Geo& g, int f) {
+  return f < 2 ? g.ny * g.nz : (f < 4 ? g.nx * g.nz : g.nx * g.ny);
+}
+
+__global__ void __launch_bounds__(kB)
+    k_bc_mdh(Geo g, const double* xf, const double* yf, const double* zf, AtomsDev A, double pre1,
+             double xkappa, double* gx, double* gy, double* gz) {
+  const int f = blockIdx.y;
+  const int t = blockIdx.x * kB + threadIdx.x;
+  if (t >= face_count(g, f)) return;
+  int i, j, k;
+  face_point(g, f, t, i, j, k);
+  const double px = xf[i], py = yf[j], pz = zf[k];
+  const double ec = 1.6021773e-19;  // Vunit_ec, src/generic/vunit.h:80
+  double val = 0.0;
+  for (int a = 0; a < A.n; a++) {
+    const double* ap = A.pos + 3 * a;
+    const double q = ec * A.chg[a];
+    const double dist = sqrt((px - ap[0]) * (px - ap[0]) + (py - ap[1]) * (py - ap[1]) +
+                             (pz - ap[2]) * (pz - ap[2]));
+    if (xkappa > VSMALL)
+      val += pre1 * (q / dist) * exp(-xkappa * (dist - A.rad[a])) / (1 + xkappa * A.rad[a]);
+    else
+      val += pre1 * (q / dist);
+  }
+  *face_slot(g, f, i, j, k, gx, gy, gz) = val;
+}
+
+__device__ inline double trilin(const OldGrid& o, double x, double y, double z, int* bad) {
+  const double VS = VSMALL;
+  const bool ok = (x >= (o.xmin - VS)) && (y >= (o.ymin - VS)) && (z >= (o.zmin - VS)) &&
+                  (x <= (o.xmax + VS)) && (y <= (o.ymax + VS)) && (z <= (o.zmax + VS));
+  if (!ok) {
+    atomicExch(bad, 1);
+    return 0.0;
+  }
+  const double fi = (x - o.xmin) / o.hx, fj = (y - o.ymin) / o.hy, fk = (z - o.zmin) / o.hz;
+  int ihi = (int)ceil(fi); if (ihi > (o.nx - 1)) ihi = o.nx - 1;
+  int ilo = (int)floor(fi); if (ilo < 0) ilo = 0;
+  int jhi = (int)ceil(fj); if (jhi > (o.ny - 1)) jhi = o.ny - 1;
+  int jlo = (int)floor(fj); if (jlo < 0) jlo = 0;
+  int khi = (int)ceil(fk); if (khi > (o.nz - 1)) khi = o.nz - 1;
+  int klo = (int)floor(fk); if (klo < 0) klo = 0;
+  const double dx = fi - (double)(ilo), dy = fj - (double)(jlo), dz = fk - (double)(klo);
+  const double* d = o.u;
+  const int nx = o.nx, ny = o.ny;
+  return dx * dy * dz * (d[IX(ihi, jhi, khi, nx, ny)]) + dx * (1.0 - dy) * dz * (d[IX(ihi, jlo, khi, nx, ny)]) +
+         dx * dy * (1.0 - dz) * (d[IX(ihi, jhi, klo, nx, ny)]) +
+         dx * (1.0 - dy) * (1.0 - dz) * (d[IX(ihi, jlo, klo, nx, ny)]) +
+         (1.0 - dx) * dy * dz * (d[IX(ilo, jhi, khi, nx, ny)]) +
+         (1.0 - dx) * (1.0 - dy) * dz * (d[IX(ilo, jlo, khi, nx, ny)]) +
+         (1.0 - dx) * dy * (1.0 - dz) * (d[IX(ilo, jhi, klo, nx, ny)]) +
+         (1.0 - dx) * (1.0 - dy) * (1.0 - dz) * (d[IX(ilo, jlo, klo, nx, ny)]);
+}
+
+__global__ void __launch_bounds__(kB)
+    k_focus(Geo g, double xminN, double yminN, double zminN, double xmaxN, double ymaxN,
+            double zmaxN, OldGrid o, double* gx, double* gy, double* gz, int* bad) {
+  const int f = blockIdx.y;
+  const int t = blockIdx.x * kB + threadIdx.x;
+  if (t >= face_count(g, f)) return;
+  int i, j, k;
+  face_point(g, f, t, i, j, k);
+  double x, y, z;
+  if (f < 2) {
+    x = f == 0 ? xminN : xmaxN;
+    y = yminN + j * g.hy;
+    z = zminN + k * g.hz;
+  } else if (f < 4) {
+    x = xminN + i * g.hx;
+    y = f == 2 ? yminN : ymaxN;
+    z = zminN + k * g.hz;
+  } else {
+    x = xminN + i * g.hx;
+    y = yminN + j * g.hy;
+    z = f == 4 ? zminN : zmaxN;
+  }
+  *face_slot(g, f, i, j, k, gx, gy, gz) = trilin(o, x, y, z, bad);
+}
+
+// -------------------------------------------------------------- partition
+__device__ inline double part_axis(double x, double lo, double hi, double h) {
+  double ok;
+  if ((x < (hi - h / 2)) && (x > (lo + h / 2))) ok = 1.0;
+  else if (fabs(x - lo) < VPMGSMALL) ok = 1.0;
+  else if (fabs(x - hi) < VPMGSMALL) ok = 1.0;
+  else if ((x > (hi + h / 2)) || (x < (lo - h / 2))) ok = 0.0;
+  else {
+    const double x0 = fmax(x - h / 2, lo), x1 = fmin(x + h / 2, hi);
+    ok = fabs(x1 - x0) / h;
+    if (ok < 0.0 && fabs(ok) < VPMGSMALL) ok = 0.0;
+    if (ok > 1.0 && fabs(ok - 1.0) < VPMGSMALL) ok = 1.0;
+  }
+  return ok;
+}
+
+__global__ void __launch_bounds__(kB)
+    k_set_part(Geo g, double xm, double ym, double zm, double lo0, double lo1, double lo2,
+               double hi0, double hi1, double hi2, int invert, double* pvec) {
+  const int t = blockIdx.x * kB + threadIdx.x;
+  const int k = blockIdx.y;
+  const int i = t % g.nx, j = t / g.nx;
+  if (j >= g.ny) return;
+  const double xok = part_axis(i * g.hx + xm, lo0, hi0, g.hx);
+  const double yok = part_axis(j * g.hy + ym, lo1, hi1, g.hy);
+  const double zok = part_axis(k * g.hz + zm, lo2, hi2, g.hz);
+  const double v0 = xok * yok * zok;
+  double v = (fabs(v0) < VPMGSMALL) ? 0.0 : v0;
+  if (invert) {
+    if (v > VSMALL) v = 1.0;
+    v = (1 - v) * 1.0;
+  }
+  pvec[IX(i, j, k, g.nx, g.ny)] = v;
+}
+
+// ---------------------------------------------------------------- energies
+__device__ inline double bsum(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(kB) k_fin(const double* part, double* out) {
+  __shared__ double sh[kB];
+  double a = 0.0;
+  for (int t = threadIdx.x; t < kParts; t += kB) a += part[t];
+  double s = bsum(a, sh);
+  if (threadIdx.x == 0) out[0] = s;
+}
+
+__global__ void __launch_bounds__(kB)
+    k_e_qf(size_t n, const double* pvec, const double* u, const double* charge, double* part) {
+  __shared__ double sh[kB];
+  double acc = 0.0;
+  for (size_t t = (size_t)blockIdx.x * kB + threadIdx.x; t < n; t += (size_t)kParts * kB)
+    acc += (pvec[t] * u[t] * charge[t]);
+  double s = bsum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kB)
+    k_e_diel(Geo g, const double* pv, const double* u, const double* ex, const double* ey,
+             const double* ez, double* part) {
+  __shared__ double sh[kB];
+  const int nx = g.nx, ny = g.ny;
+  const long ni = nx - 1, nij = ni * (ny - 1), n = nij * (g.nz - 1);
+  double acc = 0.0;
+  for (long t = (long)blockIdx.x * kB + threadIdx.x; t < n; t += (long)kParts * kB) {
+    const int i = (int)(t % ni), j = (int)((t / ni) % (ny - 1)), k = (int)(t / nij);
+    const size_t p = IX(i, j, k, nx, ny);
+    const double pvx = 0.5 * (pv[p] + pv[p + 1]);
+    const double pvy = 0.5 * (pv[p] + pv[p + nx]);
+    const double pvz = 0.5 * (pv[p] + pv[p + (size_t)nx * ny]);
+    const double dxu = (u[p] - u[p + 1]) / g.hx, dyu = (u[p] - u[p + nx]) / g.hy,
+                 dzu = (u[p] - u[p + (size_t)nx * ny]) / g.hz;
+    const double nrgx = ex[p] * pvx * (dxu * dxu);
+    const double nrgy = ey[p] * pvy * (dyu * dyu);
+    const double nrgz = ez[p] * pvz * (dzu * dzu);
+    acc += (nrgx + nrgy + nrgz);
+  }
+  double s = bsum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__device__ inline double capexp(double x, int* ichop) {  // vcap.c:59-73
+  if (x > 85.0) { *ichop = 1; return exp(85.0); }
+  if (x < -85.0) { *ichop = 1; return exp(-85.0); }
+  *ichop = 0;
+  return exp(x);
+}
+
+__global__ void __launch_bounds__(kB)
+    k_e_qm(size_t n, int nonlin, const double* pvec, const double* kappa, const double* u,
+           double zks2, double zkappa2, IonBulk I, double* part, int* nchop) {
+  __shared__ double sh[kB];
+  double acc = 0.0;
+  int chops = 0;
+  for (size_t t = (size_t)blockIdx.x * kB + threadIdx.x; t < n; t += (size_t)kParts * kB) {
+    if (pvec[t] * kappa[t] > VSMALL) {
+      if (nonlin) {
+        for (int s = 0; s < I.nion; s++) {
+          int ic;
+          acc += (pvec[t] * kappa[t] * zks2 * I.c[s] * (capexp(-I.q[s] * u[t], &ic) - 1.0));
+          chops += ic;
+        }
+      } else {
+        acc += (pvec[t] * zkappa2 * kappa[t] * (u[t] * u[t]));
+      }
+    }
+  }
+  if (chops) atomicAdd(nchop, chops);
+  double s = bsum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// one thread per atom, partial per atom, then fixed-order sum
+__global__ void __launch_bounds__(kB)
+    k_e_qf_point(Geo g, AtomsDev A, const double* u, double* per_atom) {
+  const int a = blockIdx.x * kB + threadIdx.x;
+  if (a >= A.n) return;
+  const double* ap = A.pos + 3 * a;
+  const int nx = g.nx, ny = g.ny;
+  const double fi = (ap[0] - g.xmin) / g.hx, fj = (ap[1] - g.ymin) / g.hy, fk = (ap[2] - g.zmin) / g.hz;
+  const int ihi = (int)ceil(fi), ilo = (int)floor(fi), jhi = (int)ceil(fj), jlo = (int)floor(fj);
+  const int khi = (int)ceil(fk), klo = (int)floor(fk);
+  double e = 0.0;
+  if (A.partID[a] > 0 && (ihi < g.nx) && (jhi < g.ny) && (khi < g.nz) && (ilo >= 0) &&
+      (jlo >= 0) && (klo >= 0)) {
+    const double dx = fi - (double)(ilo), dy = fj - (double)(jlo), dz = fk - (double)(klo);
+    const double uval =
+        dx * dy * dz * u[IX(ihi, jhi, khi, nx, ny)] + dx * (1.0 - dy) * dz * u[IX(ihi, jlo, khi, nx, ny)] +
+        dx * dy * (1.0 - dz) * u[IX(ihi, jhi, klo, nx, ny)] +
+        dx * (1.0 - dy) * (1.0 - dz) * u[IX(ihi, jlo, klo, nx, ny)] +
+        (1.0 - dx) * dy * dz * u[IX(ilo, jhi, khi, nx, ny)] +
+        (1.0 - dx) * (1.0 - dy) * dz * u[IX(ilo, jlo, khi, nx, ny)] +
+        (1.0 - dx) * dy * (1.0 - dz) * u[IX(ilo, jhi, klo, nx, ny)] +
+        (1.0 - dx) * (1.0 - dy) * (1.0 - dz) * u[IX(ilo, jlo, klo, nx, ny)];
+    e = (uval * A.chg[a] * A.partID[a]);
+  }
+  per_atom[a] = e;
+}
+
+// sequential sum over atoms in index order (one thread): same order as the
+// reference's loop (vpmg.c:1740-1785)
+__global__ void k_seq_sum(const double* v, int n, double* out) {
+  if (threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int a = 0; a < n; a++) s += v[a];
+  out[0] = s;
+}
+
+// ------------------------------------------------------------------ forces
+__global__ void k_force_qf(Geo g, AtomsDev A, int spl2, const double* u, double* out) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= A.n) return;
+  const double* ap = A.pos + 3 * a;
+  const int nx = g.nx, ny = g.ny;
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  const double charge = A.chg[a];
+  if (A.partID[a] != 0) {
+    if (spl2) {  // qfForceSpline2 (vpmg.c:6043-6154)
+      if (!((ap[0] <= (g.xmin + g.hx)) || (ap[0] >= (g.xmax - g.hx)) || (ap[1] <= (g.ymin + g.hy)) ||
+            (ap[1] >= (g.ymax - g.hy)) || (ap[2] <= (g.zmin + g.hz)) || (ap[2] >= (g.zmax - g.hz)))) {
+        const double fi = (ap[0] - g.xmin) / g.hx, fj = (ap[1] - g.ymin) / g.hy,
+                     fk = (ap[2] - g.zmin) / g.hz;
+        const int ip2 = imin2((int)ceil(fi) + 1, g.nx - 1), im2 = imax2((int)floor(fi) - 1, 0);
+        const int jp2 = imin2((int)ceil(fj) + 1, g.ny - 1), jm2 = imax2((int)floor(fj) - 1, 0);
+        const int kp2 = imin2((int)ceil(fk) + 1, g.nz - 1), km2 = imax2((int)floor(fk) - 1, 0);
+        for (int ii = im2; ii <= ip2; ii++) {
+          const double mx = bspline2(vfchi(ii, fi)), dmx = dbspline2(vfchi(ii, fi));
+          for (int jj = jm2; jj <= jp2; jj++) {
+            const double my = bspline2(vfchi(jj, fj)), dmy = dbspline2(vfchi(jj, fj));
+            for (int kk = km2; kk <= kp2; kk++) {
+              const double mz = bspline2(vfchi(kk, fk)), dmz = dbspline2(vfchi(kk, fk));
+              const double uu = u[IX(ii, jj, kk, nx, ny)];
+              f0 += (charge * dmx * my * mz * uu) / g.hx;
+              f1 += (charge * mx * dmy * mz * uu) / g.hy;
+              f2 += (charge * mx * my * dmz * uu) / g.hz;
+            }
+          }
+        }
+      }
+    } else if (!((ap[0] <= g.xmin) || (ap[0] >= g.xmax) || (ap[1] <= g.ymin) ||
+                 (ap[1] >= g.ymax) || (ap[2] <= g.zmin) || (ap[2] >= g.zmax))) {
+      // qfForceSpline1 (vpmg.c:5899-6041)
+      const double fi = (ap[0] - g.xmin) / g.hx, fj = (ap[1] - g.ymin) / g.hy, fk = (ap[2] - g.zmin) / g.hz;
+      const int ihi = (int)ceil(fi), ilo = (int)floor(fi), jhi = (int)ceil(fj), jlo = (int)floor(fj);
+      const int khi = (int)ceil(fk), klo = (int)floor(fk);
+      const double dx = fi - (double)(ilo), dy = fj - (double)(jlo), dz = fk - (double)(klo);
+#define U(a_, b_, c_) u[IX(a_, b_, c_, nx, ny)]
+      if ((dx > VPMGSMALL) && (fabs(1.0 - dx) > VPMGSMALL))
+        f0 = -charge * (dy * dz * U(ihi, jhi, khi) + dy * (1 - dz) * U(ihi, jhi, klo) +
+                        (1 - dy) * dz * U(ihi, jlo, khi) + (1 - dy) * (1 - dz) * U(ihi, jlo, klo) -
+                        dy * dz * U(ilo, jhi, khi) - dy * (1 - dz) * U(ilo, jhi, klo) -
+                        (1 - dy) * dz * U(ilo, jlo, khi) - (1 - dy) * (1 - dz) * U(ilo, jlo, klo)) / g.hx;
+      if ((dy > VPMGSMALL) && (fabs(1.0 - dy) > VPMGSMALL))
+        f1 = -charge * (dx * dz * U(ihi, jhi, khi) + dx * (1 - dz) * U(ihi, jhi, klo) -
+                        dx * dz * U(ihi, jlo, khi) - dx * (1 - dz) * U(ihi, jlo, klo) +
+                        (1 - dx) * dz * U(ilo, jhi, khi) + (1 - dx) * (1 - dz) * U(ilo, jhi, klo) -
+                        (1 - dx) * dz * U(ilo, jlo, khi) - (1 - dx) * (1 - dz) * U(ilo, jlo, klo)) / g.hy;
+      if ((dz > VPMGSMALL) && (fabs(1.0 - dz) > VPMGSMALL))
+        f2 = -charge * (dy * dx * U(ihi, jhi, khi) - dy * dx * U(ihi, jhi, klo) +
+                        (1 - dy) * dx * U(ihi, jlo, khi) - (1 - dy) * dx * U(ihi, jlo, klo) +
+                        dy * (1 - dx) * U(ilo, jhi, khi) - dy * (1 - dx) * U(ilo, jhi, klo) +
+                        (1 - dy) * (1 - dx) * U(ilo, jlo, khi) - (1 - dy) * (1 - dx) * U(ilo, jlo, klo)) / g.hz;
+#undef U
+    }
+  }
+  out[3 * a] = f0;
+  out[3 * a + 1] = f1;
+  out[3 * a + 2] = f2;
+}
+
+// Vacc_splineAccGradAtomNorm (vacc.c:314-373)
+__device__ inline void spline_grad(const double* c, double win, double infrad, const double* ap,
+                                   double aradius, double* grad) {
+  const double w2i = 1.0 / (win * win), w3i = 1.0 / (win * win * win);
+  grad[0] = grad[1] = grad[2] = 0.0;
+  if (aradius > 0.0) {
+    const double arad = aradius + infrad;
+    const double dist = sqrt((ap[0] - c[0]) * (ap[0] - c[0]) + (ap[1] - c[1]) * (ap[1] - c[1]) +
+                             (ap[2] - c[2]) * (ap[2] - c[2]));
+    if (dist < (arad - win)) return;
+    if (dist > (arad + win)) return;
+    if ((fabs(dist - (arad - win)) < VSMALL) || (fabs(dist - (arad + win)) < VSMALL)) return;
+    const double sm = dist - arad + win, sm2 = sm * sm;
+    const double mychi = 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+    const double mygrad = 1.5 * sm * w2i - 0.75 * sm2 * w3i;
+    for (int i = 0; i < 3; i++) grad[i] = -(mygrad / mychi) * ((c[i] - ap[i]) / dist);
+  }
+}
+
+// Ion-boundary force (vpmg.c:5440-5603): one block per atom.  The box is
+// walked by a single thread per atom in the reference's i, j, k order so
+// the sum rounds the same way.
+__global__ void k_force_ib(Geo g, AtomsDev A, double win, double irad, double zkappa2,
+                           double izmagic, int nonlin, double ionstr, IonBulk I,
+                           const double* kappa, const double* u, double* out) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= A.n) return;
+  const double* ap = A.pos + 3 * a;
+  const double arad = A.rad[a];
+  double f[3] = {0.0, 0.0, 0.0};
+  const double hx = g.hx, hy = g.hy, hzed = g.hz;
+  if (A.partID[a] != 0 && !(zkappa2 < VPMGSMALL) &&
+      !((ap[0] <= g.xmin) || (ap[0] >= g.xmax) || (ap[1] <= g.ymin) || (ap[1] >= g.ymax) ||
+        (ap[2] <= g.zmin) || (ap[2] >= g.zmax))) {
+    const double p0 = ap[0] - g.xmin, p1 = ap[1] - g.ymin, p2 = ap[2] - g.zmin;
+    const double rtot = (irad + arad + win), rtot2 = rtot * rtot;
+    const double dx = rtot + 0.5 * hx;
+    const int imin = imax2(0, (int)ceil((p0 - dx) / hx));
+    const int imax = imin2(g.nx - 1, (int)floor((p0 + dx) / hx));
+    for (int i = imin; i <= imax; i++) {
+      const double dx2 = (p0 - hx * i) * (p0 - hx * i);
+      const double dy = (rtot2 > dx2) ? sqrt(rtot2 - dx2) + 0.5 * hy : 0.5 * hy;
+      const int jmin = imax2(0, (int)ceil((p1 - dy) / hy));
+      const int jmax = imin2(g.ny - 1, (int)floor((p1 + dy) / hy));
+      for (int j = jmin; j <= jmax; j++) {
+        const double dy2 = (p1 - hy * j) * (p1 - hy * j);
+        const double dz = (rtot2 > (dx2 + dy2)) ? sqrt(rtot2 - dx2 - dy2) + 0.5 * hzed : 0.5 * hzed;
+        const int kmin = imax2(0, (int)ceil((p2 - dz) / hzed));
+        const int kmax = imin2(g.nz - 1, (int)floor((p2 + dz) / hzed));
+        for (int k = kmin; k <= kmax; k++) {
+          const double dz2 = (k * hzed - p2) * (k * hzed - p2);
+          if ((dz2 + dy2 + dx2) <= rtot2) {
+            const double gp[3] = {i * hx + g.xmin, j * hy + g.ymin, k * hzed + g.zmin};
+            double tg[3];
+            spline_grad(gp, win, irad, ap, arad, tg);
+            const size_t p = IX(i, j, k, g.nx, g.ny);
+            double fmag;
+            if (nonlin) {
+              fmag = 0.0;
+              for (int s = 0; s < I.nion; s++) {
+                int ic;
+                fmag += (kappa[p]) * I.c[s] * (capexp(-I.q[s] * u[p], &ic) - 1.0) / ionstr;
+              }
+            } else {
+              fmag = (u[p] * u[p]) * (kappa[p]);
+            }
+            f[0] += (zkappa2 * fmag * tg[0]);
+            f[1] += (zkappa2 * fmag * tg[1]);
+            f[2] += (zkappa2 * fmag * tg[2]);
+          }
+        }
+      }
+    }
+    for (int d = 0; d < 3; d++) f[d] = f[d] * 0.5 * hx * hy * hzed * izmagic;
+  }
+  out[3 * a] = f[0];
+  out[3 * a + 1] = f[1];
+  out[3 * a + 2] = f[2];
+}
+
+// Dielectric-boundary force (vpmg.c:5605-5860), one thread per atom, box in
+// the reference's loop order.
+__global__ void k_force_db(Geo g, AtomsDev A, double win, double srad, double epsp, double epsw,
+                           double izmagic, const double* epsx, const double* epsy,
+                           const double* epsz, const double* u, double* out, int* rcs) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= A.n) return;
+  const double* ap = A.pos + 3 * a;
+  const double arad = A.rad[a];
+  const int nx = g.nx, ny = g.ny;
+  const double hx = g.hx, hy = g.hy, hzed = g.hz;
+  double F[3] = {0.0, 0.0, 0.0};
+  int rc = 1;
+  const double rtot = (arad + win + srad);
+  bool go = A.partID[a] != 0 && !(fabs(epsp - epsw) < VPMGSMALL) &&
+            !((ap[0] <= g.xmin + rtot) || (ap[0] >= g.xmax - rtot) || (ap[1] <= g.ymin + rtot) ||
+              (ap[1] >= g.ymax - rtot) || (ap[2] <= g.zmin + rtot) || (ap[2] >= g.zmax - rtot));
+  int imin = 0, imax = -1, jmin = 0, jmax = -1, kmin = 0, kmax = -1;
+  if (go) {
+    const double p0 = ap[0] - g.xmin, p1 = ap[1] - g.ymin, p2 = ap[2] - g.zmin;
+    imin = (int)floor((p0 - rtot) / hx);
+    imax = (int)ceil((p0 + rtot) / hx);
+    jmin = (int)floor((p1 - rtot) / hy);
+    jmax = (int)ceil((p1 + rtot) / hy);
+    kmin = (int)floor((p2 - rtot) / hzed);
+    kmax = (int)ceil((p2 + rtot) / hzed);
+    if (imin < 1 || imax > (nx - 2) || jmin < 1 || jmax > (ny - 2) || kmin < 1 || kmax > (g.nz - 2)) {
+      rc = 0;
+      go = false;
+    }
+  }
+  if (go) {
+    const double deps = (epsw - epsp), depsi = 1.0 / deps;
+    for (int i = imin; i <= imax; i++)
+      for (int j = jmin; j <= jmax; j++)
+        for (int k = kmin; k <= kmax; k++) {
+          double gp[3], dHx[3], dHy[3], dHz[3], dHxm[3], dHym[3], dHzm[3];
+          gp[0] = (i + 0.5) * hx + g.xmin; gp[1] = j * hy + g.ymin; gp[2] = k * hzed + g.zmin;
+          const double Hx = (epsx[IX(i, j, k, nx, ny)] - epsp) * depsi;
+          spline_grad(gp, win, 0., ap, arad, dHx);
+          for (int l = 0; l < 3; l++) dHx[l] *= Hx;
+          gp[0] = i * hx + g.xmin; gp[1] = (j + 0.5) * hy + g.ymin; gp[2] = k * hzed + g.zmin;
+          const double Hy = (epsy[IX(i, j, k, nx, ny)] - epsp) * depsi;
+          spline_grad(gp, win, 0., ap, arad, dHy);
+          for (int l = 0; l < 3; l++) dHy[l] *= Hy;
+          gp[0] = i * hx + g.xmin; gp[1] = j * hy + g.ymin; gp[2] = (k + 0.5) * hzed + g.zmin;
+          const double Hz = (epsz[IX(i, j, k, nx, ny)] - epsp) * depsi;
+          spline_grad(gp, win, 0., ap, arad, dHz);
+          for (int l = 0; l < 3; l++) dHz[l] *= Hz;
+          gp[0] = (i - 0.5) * hx + g.xmin; gp[1] = j * hy + g.ymin; gp[2] = k * hzed + g.zmin;
+          const double Hxm = (epsx[IX(i - 1, j, k, nx, ny)] - epsp) * depsi;
+          spline_grad(gp, win, 0., ap, arad, dHxm);
+          for (int l = 0; l < 3; l++) dHxm[l] *= Hxm;
+          gp[0] = i * hx + g.xmin; gp[1] = (j - 0.5) * hy + g.ymin; gp[2] = k * hzed + g.zmin;
+          const double Hym = (epsy[IX(i, j - 1, k, nx, ny)] - epsp) * depsi;
+          spline_grad(gp, win, 0., ap, arad, dHym);
+          for (int l = 0; l < 3; l++) dHym[l] *= Hym;
+          gp[0] = i * hx + g.xmin; gp[1] = j * hy + g.ymin; gp[2] = (k - 0.5) * hzed + g.zmin;
+          const double Hzm = (epsz[IX(i, j, k - 1, nx, ny)] - epsp) * depsi;
+          spline_grad(gp, win, 0., ap, arad, dHzm);
+          for (int l = 0; l < 3; l++) dHzm[l] *= Hzm;
+          const size_t p = IX(i, j, k, nx, ny);
+          const double fmag = u[p];
+          for (int l = 0; l < 3; l++) {
+            const double tg = (dHx[l] * (u[p + 1] - u[p]) + dHxm[l] * (u[p - 1] - u[p])) / (hx * hx) +
+                              (dHy[l] * (u[p + nx] - u[p]) + dHym[l] * (u[p - nx] - u[p])) / (hy * hy) +
+                              (dHz[l] * (u[p + (size_t)nx * ny] - u[p]) +
+                               dHzm[l] * (u[p - (size_t)nx * ny] - u[p])) / (hzed * hzed);
+            F[l] += (fmag * tg);
+          }
+        }
+    for (int d = 0; d < 3; d++) F[d] = -F[d] * hx * hy * hzed * deps * 0.5 * izmagic;
+  }
+  out[3 * a] = F[0];
+  out[3 * a + 1] = F[1];
+  out[3 * a + 2] = F[2];
+  rcs[a] = rc;
+}
+
+}  // namespace
+
+// ================================================================ wrappers
+
+static dim3 grid_all(const Geo& g) {
+  return dim3((unsigned)(((long)g.nx * g.ny + kB - 1) / kB), (unsigned)g.nz);
+}
+
+void charge_map(hipStream_t s, const Geo& g, const AtomsDev& A, const Cells& C, int spl2,
+                double zmagic, int, double* charge) {
+  k_charge<<<grid_all(g), kB, 0, s>>>(g, A, C, spl2, zmagic, charge);
+}
+
+void coef_spline(hipStream_t s, const Geo& g, const AtomsDev& A, const Cells& C, double win,
+                 double irad, double ionmask, double epsw, double epsp, double* kappa,
+                 double* epsx, double* epsy, double* epsz) {
+  k_coef_spline<<<grid_all(g), kB, 0, s>>>(g, A, C, win, irad, ionmask, epsw, epsp, kappa, epsx,
+                                           epsy, epsz);
+}
+
+void sas_flags(hipStream_t s, const AtomsDev& A, const Cells& C, const double* ref, int nref,
+               double probe, unsigned char* flags) {
+  const long n = (long)A.n * nref;
+  k_sas_flags<<<(unsigned)((n + kB - 1) / kB), kB, 0, s>>>(A, C, ref, nref, probe, flags);
+}
+
+int sas_compact(hipStream_t s, const AtomsDev& A, const double* ref, int nref, double probe,
+                const unsigned char* flags, double* xyz, int* counter, int cap) {
+  const long n = (long)A.n * nref;
+  hipMemsetAsync(counter, 0, sizeof(int), s);
+  k_sas_compact<<<(unsigned)((n + kB - 1) / kB), kB, 0, s>>>(A, ref, nref, probe, flags, xyz,
+                                                              counter, cap);
+  int h = 0;
+  hipMemcpyAsync(&h, counter, sizeof(int), hipMemcpyDeviceToHost, s);
+  hipStreamSynchronize(s);
+  return h;
+}
+
+void mark_spheres(hipStream_t s, const Geo& g, int npts, const double* pos, const double* rad,
+                  double add, int mode, int which, double val, double* arr) {
+  if (npts <= 0) return;
+  const double xs = g.xmin + (which == 0 ? 0.5 * g.hx : 0.0);
+  const double ys = g.ymin + (which == 1 ? 0.5 * g.hy : 0.0);
+  const double zs = g.zmin + (which == 2 ? 0.5 * g.hz : 0.0);
+  k_mark<<<(npts + kB - 1) / kB, kB, 0, s>>>(g, npts, pos, rad, add, mode, xs, ys, zs, val, arr);
+}
+
+void mol_smooth(hipStream_t s, const Geo& g, const double* a1, const double* a2, const double* a3,
+                double* epsx, double* epsy, double* epsz) {
+  k_mol_smooth<<<grid_all(g), kB, 0, s>>>(g, a1, a2, a3, epsx, epsy, epsz);
+}
+
+void bc_mdh(hipStream_t s, const Geo& g, const double* xf, const double* yf, const double* zf,
+            const AtomsDev& A, double pre1, double xkappa, double* gx, double* gy, double* gz) {
+  int mx = g.ny * g.nz;
+  if (g.nx * g.nz > mx) mx = g.nx * g.nz;
+  if (g.nx * g.ny > mx) mx = g.nx * g.ny;
+  k_bc_mdh<<<dim3((mx + kB - 1) / kB, 6), kB, 0, s>>>(g, xf, yf, zf, A, pre1, xkappa, gx, gy, gz);
+}
+
+void focus_bound(hipStream_t s, const Geo& g, double xminN, double yminN, double zminN,
+                 double xmaxN, double ymaxN, double zmaxN, const OldGrid& o, double* gx,
+                 double* gy, double* gz, int* bad) {
+  int mx = g.ny * g.nz;
+  if (g.nx * g.nz > mx) mx = g.nx * g.nz;
+  if (g.nx * g.ny > mx) mx = g.nx * g.ny;
+  k_focus<<<dim3((mx + kB - 1) / kB, 6), kB, 0, s>>>(g, xminN, yminN, zminN, xmaxN, ymaxN, zmaxN, o,
+                                                     gx, gy, gz, bad);
+}
+
+void set_part(hipStream_t s, const Geo& g, double xcen0, double ycen0, double zcen0,
+              const double lo[3], const double hi[3], int invert, double* pvec) {
+  k_set_part<<<grid_all(g), kB, 0, s>>>(g, xcen0, ycen0, zcen0, lo[0], lo[1], lo[2], hi[0], hi[1],
+                                        hi[2], invert, pvec);
+}
+
+void energy_qf_volume(hipStream_t s, size_t n, const double* pvec, const double* u,
+                      const double* charge, double* part, double* out) {
+  k_e_qf<<<kParts, kB, 0, s>>>(n, pvec, u, charge, part);
+  k_fin<<<1, kB, 0, s>>>(part, out);
+}
+
+void energy_diel(hipStream_t s, const Geo& g, const double* pvec, const double* u,
+                 const double* ex, const double* ey, const double* ez, double* part, double* out) {
+  k_e_diel<<<kParts, kB, 0, s>>>(g, pvec, u, ex, ey, ez, part);
+  k_fin<<<1, kB, 0, s>>>(part, out);
+}
+
+void energy_qm(hipStream_t s, size_t n, int nonlin, const double* pvec, const double* kappa,
+               const double* u, double zks2, double zkappa2, const IonBulk& ions, double* part,
+               double* out, int* nchop) {
+  hipMemsetAsync(nchop, 0, sizeof(int), s);
+  k_e_qm<<<kParts, kB, 0, s>>>(n, nonlin, pvec, kappa, u, zks2, zkappa2, ions, part, nchop);
+  k_fin<<<1, kB, 0, s>>>(part, out);
+}
+
+void energy_qf_point(hipStream_t s, const Geo& g, const AtomsDev& A, const double* u, double* part,
+                     double* out) {
+  k_e_qf_point<<<(A.n + kB - 1) / kB, kB, 0, s>>>(g, A, u, part);
+  k_seq_sum<<<1, 64, 0, s>>>(part, A.n, out);
+}
+
+void force_qf(hipStream_t s, const Geo& g, const AtomsDev& A, int spl2, const double* u,
+              double* out) {
+  k_force_qf<<<(A.n + 63) / 64, 64, 0, s>>>(g, A, spl2, u, out);
+}
+
+void force_ib(hipStream_t s, const Geo& g, const AtomsDev& A, double win, double irad,
+              double zkappa2, double izmagic, int nonlin, double ionstr, const IonBulk& ions,
+              const double* kappa, const double* u, double* out) {
+  k_force_ib<<<(A.n + 63) / 64, 64, 0, s>>>(g, A, win, irad, zkappa2, izmagic, nonlin, ionstr, ions,
+                                            kappa, u, out);
+}
+
+void force_db(hipStream_t s, const Geo& g, const AtomsDev& A, double win, double srad, double epsp,
+              double epsw, double izmagic, const double* epsx, const double* epsy,
+              const double* epsz, const double* u, double* out, int* rc) {
+  k_force_db<<<(A.n + 63) / 64, 64, 0, s>>>(g, A, win, srad, epsp, epsw, izmagic, epsx, epsy, epsz,
+                                            u, out, rc);
+}
+
+}  // namespace pmg

@@ -1,0 +1,110 @@
+// pmg_device.h -- launch wrappers for the PMG multigrid kernels (HIP, gfx950).
+//
+// Grids are (nx, ny, nz), x fastest, index p = (k*ny + j)*nx + i, 0-based;
+// the Dirichlet boundary is the outer layer, interior is 1..n-2 per axis.
+// Operator arrays follow the reference layout of src/pmgc/gsd.c:74-87:
+//   0 oC, 1 oE, 2 oN, 3 uC, 4 oNE, 5 oNW, 6 uE, 7 uW, 8 uN, 9 uS,
+//   10 uNE, 11 uNW, 12 uSE, 13 uSW.
+// The fine level holds 0..3 (7-point), coarse levels all 14 (27-point).
+// D is the smoothing/residual diagonal oC + cc, precomputed once per operator
+// so every use rounds exactly like the reference's inline (oC + cc).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+namespace pmg {
+
+constexpr int kMaxIon = 10;
+
+template <class T>
+struct Ops {
+  const T* a[14];  // nullptr beyond index 3 on 7-point levels
+};
+
+struct Ions {
+  int nion;
+  double q[kMaxIon];
+  double sconc[kMaxIon];  // bulk concentrations scaled by 1/(2 I)
+};
+
+// Reduction scratch: kParts partial sums per slot, results in dev scalars.
+constexpr int kParts = 1024;
+
+// ---------------------------------------------------------------- smoothers
+template <class T>
+void gsrb7(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
+           T* x, int nu, int iadjoint);
+template <class T>
+void gsrb27(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
+            T* x, int nu, int iadjoint);
+
+// ------------------------------------------------------- residual / matvec
+template <class T>
+void resid(hipStream_t s, int st, int nx, int ny, int nz, const Ops<T>& A, const T* D,
+           const T* fc, const T* x, T* r);
+// nonlinear residual on the fine 7-point level: f + sum a x - oC x - N(x)
+void nresid7(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* cc,
+             const double* fc, const double* x, double* r, const Ions& ions);
+// cprime = N'(x) (Vdc_vecpmg), full array
+void dcvec(hipStream_t s, size_t n, const double* cc, const double* x, double* out,
+           const Ions& ions);
+
+// ------------------------------------------------------------- transfers
+template <class T>
+void restrict3(hipStream_t s, int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
+               const T* in, T* out);
+// xf += damp[0] * P xc  (interior of the fine grid)
+template <class T>
+void prolong_add(hipStream_t s, int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
+                 const T* xc, T* xf, const double* damp);
+
+// ------------------------------------------------------------ reductions
+// Each writes one double to out (device), deterministically.
+template <class T>
+void nrm1(hipStream_t s, int nx, int ny, int nz, const T* x, double* part, double* out);
+// out[0] = <x, w0> / <x, A x>   (Hackbusch-Reusken damping, 27-point A)
+template <class T>
+void damp_factor(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D,
+                 const T* x, const T* w0, double* part, double* out);
+
+// ------------------------------------------------------------ vector kit
+template <class T>
+void fill(hipStream_t s, T* x, size_t n, T v);
+template <class T>
+void axpy_interior(hipStream_t s, int nx, int ny, int nz, double a, const T* x, T* y);
+// y = x + a*z on the interior (boundary of y untouched)
+void xpay_interior(hipStream_t s, int nx, int ny, int nz, const double* x, double a,
+                   const double* z, double* y);
+template <class T>
+void copy_interior(hipStream_t s, int nx, int ny, int nz, const T* x, T* y);
+// y = a * x over the whole array
+void scale(hipStream_t s, size_t n, double a, const double* x, double* y);
+template <class T>
+void add_diag(hipStream_t s, size_t n, const T* oC, const T* cc, T* D);
+template <class Tin, class Tout>
+void convert(hipStream_t s, size_t n, const Tin* in, Tout* out);
+void set_faces(hipStream_t s, int nx, int ny, int nz, double* u, const double* gx,
+               const double* gy, const double* gz);
+
+// --------------------------------------------------------------- operators
+void buildA_fv(hipStream_t s, int nx, int ny, int nz, const double* xf, const double* yf,
+               const double* zf, const double* gx, const double* gy, const double* gz,
+               const double* a1, const double* a2, const double* a3, const double* ccf,
+               const double* fcf, double* oC, double* cc, double* fc, double* oE, double* oN,
+               double* uC);
+// Galerkin A_c = P^T A P (without the Helmholtz term); ac holds 14 arrays.
+void galerkin(hipStream_t s, int st, int nxf, int nyf, int nzf, const Ops<double>& Af, int nxc,
+              int nyc, int nzc, double* const ac[14]);
+
+// ----------------------------------------------------- coarsest level (dense)
+// Assemble the dense (N x N) operator of the interior of a 27-point level with
+// diagonal D, then invert it in place (Gauss-Jordan, SPD, one workgroup).
+void dense_assemble(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* D,
+                    double* M, int N);
+void dense_invert(hipStream_t s, double* M, double* work, int N);
+// x(interior) = Minv * f(interior); boundary untouched
+template <class T>
+void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int N, const T* f,
+                 T* x);
+
+}  // namespace pmg

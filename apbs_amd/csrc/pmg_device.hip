@@ -1,0 +1,865 @@
+// pmg_device.hip -- PMG multigrid kernels for CDNA4 (gfx950).
+//
+// The fine 7-point kernels evaluate exactly the reference's expressions in the
+// reference's order (src/pmgc/gsd.c:95-165, matvecd.c:461-496) and are built
+// without FMA contraction, so one sweep is bitwise identical to the reference
+// arithmetic.  Coarse 27-point levels are smoothed with an 8-colour
+// Gauss-Seidel ordering (parity of i, j, k) instead of the reference's
+// in-place lexicographic sweep (gsd.c:230-327), whose dependencies are serial.
+#include "pmg_device.h"
+
+#include <cmath>
+
+namespace pmg {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__host__ __device__ inline size_t IX(int i, int j, int k, int nx, int ny) {
+  return ((size_t)k * ny + j) * nx + i;
+}
+
+inline dim3 grid_interior(int nx, int ny, int nz) {
+  long cnt = (long)(nx - 2) * (ny - 2);
+  return dim3((unsigned)((cnt + kBlock - 1) / kBlock), (unsigned)(nz - 2), 1);
+}
+
+// ------------------------------------------------------------------ 7-point
+// fc + north + south + east + west + down + up, left to right.
+template <class T>
+__device__ inline T acc7(T init, const T* __restrict__ oE, const T* __restrict__ oN,
+                         const T* __restrict__ uC, const T* __restrict__ x, size_t p, int nx,
+                         size_t nxy) {
+  T s = init;
+  s += oN[p] * x[p + nx];
+  s += oN[p - nx] * x[p - nx];
+  s += oE[p] * x[p + 1];
+  s += oE[p - 1] * x[p - 1];
+  s += uC[p - nxy] * x[p - nxy];
+  s += uC[p] * x[p + nxy];
+  return s;
+}
+
+// One colour of the red-black sweep: points with (i+j+k)&1 == par (0-based).
+// The reference's "red" points (1-based i+j+k even) have par == 1.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_gsrb7(int nx, int ny, int nz, const T* __restrict__ oE, const T* __restrict__ oN,
+            const T* __restrict__ uC, const T* __restrict__ D, const T* __restrict__ fc, T* x,
+            int par) {
+  const int npair = (nx - 1) / 2;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int k = blockIdx.y + 1;
+  const int ip = t % npair;
+  const int j = 1 + t / npair;
+  if (j > ny - 2) return;
+  const int i = 1 + (((1 + j + k) ^ par) & 1) + 2 * ip;
+  if (i > nx - 2) return;
+  const size_t nxy = (size_t)nx * ny;
+  const size_t p = IX(i, j, k, nx, ny);
+  x[p] = acc7(fc[p], oE, oN, uC, x, p, nx, nxy) / D[p];
+}
+
+// ----------------------------------------------------------------- 27-point
+template <class T>
+struct P27 {
+  T o, u, d;
+};
+
+template <class T>
+__device__ inline P27<T> part27(const T* const* a, const T* __restrict__ x, size_t p, int nx,
+                                size_t nxy) {
+  const T *oE = a[1], *oN = a[2], *uC = a[3], *oNE = a[4], *oNW = a[5], *uE = a[6], *uW = a[7],
+          *uN = a[8], *uS = a[9], *uNE = a[10], *uNW = a[11], *uSE = a[12], *uSW = a[13];
+  P27<T> r;
+  T o = oN[p] * x[p + nx];
+  o += oN[p - nx] * x[p - nx];
+  o += oE[p] * x[p + 1];
+  o += oE[p - 1] * x[p - 1];
+  o += oNE[p] * x[p + nx + 1];
+  o += oNW[p] * x[p + nx - 1];
+  o += oNW[p - nx + 1] * x[p - nx + 1];
+  o += oNE[p - nx - 1] * x[p - nx - 1];
+  const size_t q = p + nxy;
+  T u = uC[p] * x[q];
+  u += uN[p] * x[q + nx];
+  u += uS[p] * x[q - nx];
+  u += uE[p] * x[q + 1];
+  u += uW[p] * x[q - 1];
+  u += uNE[p] * x[q + nx + 1];
+  u += uNW[p] * x[q + nx - 1];
+  u += uSE[p] * x[q - nx + 1];
+  u += uSW[p] * x[q - nx - 1];
+  const size_t m = p - nxy;
+  T d = uC[m] * x[m];
+  d += uS[m + nx] * x[m + nx];
+  d += uN[m - nx] * x[m - nx];
+  d += uW[m + 1] * x[m + 1];
+  d += uE[m - 1] * x[m - 1];
+  d += uSW[m + nx + 1] * x[m + nx + 1];
+  d += uSE[m + nx - 1] * x[m + nx - 1];
+  d += uNW[m - nx + 1] * x[m - nx + 1];
+  d += uNE[m - nx - 1] * x[m - nx - 1];
+  r.o = o;
+  r.u = u;
+  r.d = d;
+  return r;
+}
+
+template <class T>
+struct Ops27 {
+  const T* a[14];
+};
+
+// One colour of the 8-colour sweep: 0-based parities (c&1, (c>>1)&1, (c>>2)&1).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_gsrb27(int nx, int ny, int nz, Ops27<T> A, const T* __restrict__ D,
+             const T* __restrict__ fc, T* x, int c) {
+  const int ci = c & 1, cj = (c >> 1) & 1, ck = (c >> 2) & 1;
+  const int i0 = ci ? 1 : 2, j0 = cj ? 1 : 2, k0 = ck ? 1 : 2;
+  const int ni = (nx - 2 - i0) / 2 + 1, nj = (ny - 2 - j0) / 2 + 1;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int k = k0 + 2 * (int)blockIdx.y;
+  if (k > nz - 2 || ni <= 0 || nj <= 0) return;
+  const int ti = t % ni, tj = t / ni;
+  if (tj >= nj) return;
+  const int i = i0 + 2 * ti, j = j0 + 2 * tj;
+  if (i > nx - 2 || j > ny - 2) return;
+  const size_t nxy = (size_t)nx * ny;
+  const size_t p = IX(i, j, k, nx, ny);
+  P27<T> s = part27<T>(A.a, x, p, nx, nxy);
+  x[p] = (fc[p] + (s.o + s.u + s.d)) / D[p];
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_resid7(int nx, int ny, int nz, const T* __restrict__ oE, const T* __restrict__ oN,
+             const T* __restrict__ uC, const T* __restrict__ D, const T* __restrict__ fc,
+             const T* __restrict__ x, T* __restrict__ r) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int k = blockIdx.y + 1;
+  const int i = 1 + t % (nx - 2), j = 1 + t / (nx - 2);
+  if (j > ny - 2) return;
+  const size_t nxy = (size_t)nx * ny;
+  const size_t p = IX(i, j, k, nx, ny);
+  r[p] = acc7(fc[p], oE, oN, uC, x, p, nx, nxy) - D[p] * x[p];
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_resid27(int nx, int ny, int nz, Ops27<T> A, const T* __restrict__ D,
+              const T* __restrict__ fc, const T* __restrict__ x, T* __restrict__ r) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int k = blockIdx.y + 1;
+  const int i = 1 + t % (nx - 2), j = 1 + t / (nx - 2);
+  if (j > ny - 2) return;
+  const size_t nxy = (size_t)nx * ny;
+  const size_t p = IX(i, j, k, nx, ny);
+  P27<T> s = part27<T>(A.a, x, p, nx, nxy);
+  r[p] = fc[p] + s.o + s.u + s.d - D[p] * x[p];
+}
+
+// --------------------------------------------------------------- reductions
+__device__ inline double block_sum(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_nrm1_part(int nx, int ny, int nz, const T* __restrict__ x, double* part) {
+  __shared__ double sh[kBlock];
+  const long ni = nx - 2, nij = ni * (ny - 2), n = nij * (nz - 2);
+  double acc = 0.0;
+  for (long t = (long)blockIdx.x * kBlock + threadIdx.x; t < n; t += (long)kParts * kBlock) {
+    const int i = 1 + (int)(t % ni), j = 1 + (int)((t / ni) % (ny - 2)), k = 1 + (int)(t / nij);
+    acc += fabs((double)x[IX(i, j, k, nx, ny)]);
+  }
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// out[0] = sum(part[0..kParts-1]); ratio: out[0] = sum(a)/sum(b)
+__global__ void __launch_bounds__(kBlock) k_finish(const double* part, double* out, int ratio) {
+  __shared__ double sh[kBlock];
+  double a = 0.0, b = 0.0;
+  for (int t = threadIdx.x; t < kParts; t += kBlock) {
+    a += part[t];
+    if (ratio) b += part[kParts + t];
+  }
+  double sa = block_sum(a, sh);
+  double sb = ratio ? block_sum(b, sh) : 0.0;
+  if (threadIdx.x == 0) {
+    if (ratio) {
+      out[0] = sa / sb;
+      out[1] = sa;
+      out[2] = sb;
+    } else {
+      out[0] = sa;
+    }
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_damp_part(int nx, int ny, int nz, Ops27<T> A, const T* __restrict__ D,
+                const T* __restrict__ x, const T* __restrict__ w0, double* part) {
+  __shared__ double sh[kBlock];
+  const long ni = nx - 2, nij = ni * (ny - 2), n = nij * (nz - 2);
+  const size_t nxy = (size_t)nx * ny;
+  double num = 0.0, den = 0.0;
+  for (long t = (long)blockIdx.x * kBlock + threadIdx.x; t < n; t += (long)kParts * kBlock) {
+    const int i = 1 + (int)(t % ni), j = 1 + (int)((t / ni) % (ny - 2)), k = 1 + (int)(t / nij);
+    const size_t p = IX(i, j, k, nx, ny);
+    P27<T> s = part27<T>(A.a, x, p, nx, nxy);
+    const T ax = (-s.o) + (-s.u) + (-s.d) + D[p] * x[p];
+    num += (double)x[p] * (double)w0[p];
+    den += (double)x[p] * (double)ax;
+  }
+  double sn = block_sum(num, sh);
+  double sd = block_sum(den, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = sn;
+    part[kParts + blockIdx.x] = sd;
+  }
+}
+
+// ---------------------------------------------------------------- transfers
+// Restriction P^T (matvecd.c:801-913) with weights 1, 1/2, 1/4, 1/8.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_restrict(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc, const T* __restrict__ in,
+               T* __restrict__ out) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int K = blockIdx.y + 1;
+  const int I = 1 + t % (nxc - 2), J = 1 + t / (nxc - 2);
+  if (J > nyc - 2) return;
+  const T W1 = T(1.0), W2 = T(0.5), W4 = T(0.25), W8 = T(0.125);
+  const int ii = 2 * I, jj = 2 * J, kk = 2 * K;
+  const size_t sx = 1, sy = nxf, sz = (size_t)nxf * nyf;
+  const size_t c = IX(ii, jj, kk, nxf, nyf);
+  T o = W1 * in[c];
+  o += W2 * in[c + sy];
+  o += W2 * in[c - sy];
+  o += W2 * in[c + sx];
+  o += W2 * in[c - sx];
+  o += W4 * in[c + sx + sy];
+  o += W4 * in[c - sx + sy];
+  o += W4 * in[c + sx - sy];
+  o += W4 * in[c - sx - sy];
+  const size_t u0 = c + sz;
+  T u = W2 * in[u0];
+  u += W4 * in[u0 + sy];
+  u += W4 * in[u0 - sy];
+  u += W4 * in[u0 + sx];
+  u += W4 * in[u0 - sx];
+  u += W8 * in[u0 + sx + sy];
+  u += W8 * in[u0 - sx + sy];
+  u += W8 * in[u0 + sx - sy];
+  u += W8 * in[u0 - sx - sy];
+  const size_t d0 = c - sz;
+  T d = W2 * in[d0];
+  d += W4 * in[d0 + sy];
+  d += W4 * in[d0 - sy];
+  d += W4 * in[d0 + sx];
+  d += W4 * in[d0 - sx];
+  d += W8 * in[d0 + sx + sy];
+  d += W8 * in[d0 - sx + sy];
+  d += W8 * in[d0 + sx - sy];
+  d += W8 * in[d0 - sx - sy];
+  out[IX(I, J, K, nxc, nyc)] = o + u + d;
+}
+
+// Trilinear prolongation and correction x_f += damp * P x_c (matvecd.c:935-1076
+// followed by Vxaxpy, mgcsd.c:485-487).  Coarse boundary points carry weight 0.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_prolong_add(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf, const T* __restrict__ xc,
+                  T* __restrict__ xf, const double* __restrict__ damp) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int k = blockIdx.y + 1;
+  const int i = 1 + t % (nxf - 2), j = 1 + t / (nxf - 2);
+  if (j > nyf - 2) return;
+  const int ii = i >> 1, jj = j >> 1, kk = k >> 1;
+  const int bx = i & 1, by = j & 1, bz = k & 1;
+  const T W2 = T(0.5), W4 = T(0.25), W8 = T(0.125);
+  auto in = [&](int a, int b, int c) -> T {
+    return (a > 0 && a < nxc - 1 && b > 0 && b < nyc - 1 && c > 0 && c < nzc - 1) ? T(1.0) : T(0.0);
+  };
+  auto X = [&](int a, int b, int c) -> T { return xc[IX(a, b, c, nxc, nyc)]; };
+  T v;
+  const int sel = bx | (by << 1) | (bz << 2);
+  switch (sel) {
+    case 0:
+      v = X(ii, jj, kk);
+      break;
+    case 1:
+      v = W2 * in(ii, jj, kk) * X(ii, jj, kk) + W2 * in(ii + 1, jj, kk) * X(ii + 1, jj, kk);
+      break;
+    case 2:
+      v = W2 * in(ii, jj, kk) * X(ii, jj, kk) + W2 * in(ii, jj + 1, kk) * X(ii, jj + 1, kk);
+      break;
+    case 4:
+      v = W2 * in(ii, jj, kk) * X(ii, jj, kk) + W2 * in(ii, jj, kk + 1) * X(ii, jj, kk + 1);
+      break;
+    case 3:
+      v = W4 * in(ii, jj, kk) * X(ii, jj, kk) + W4 * in(ii + 1, jj, kk) * X(ii + 1, jj, kk) +
+          W4 * in(ii, jj + 1, kk) * X(ii, jj + 1, kk) +
+          W4 * in(ii + 1, jj + 1, kk) * X(ii + 1, jj + 1, kk);
+      break;
+    case 5:
+      v = W4 * in(ii, jj, kk) * X(ii, jj, kk) + W4 * in(ii + 1, jj, kk) * X(ii + 1, jj, kk) +
+          W4 * in(ii, jj, kk + 1) * X(ii, jj, kk + 1) +
+          W4 * in(ii + 1, jj, kk + 1) * X(ii + 1, jj, kk + 1);
+      break;
+    case 6:
+      v = W4 * in(ii, jj, kk) * X(ii, jj, kk) + W4 * in(ii, jj + 1, kk) * X(ii, jj + 1, kk) +
+          W4 * in(ii, jj, kk + 1) * X(ii, jj, kk + 1) +
+          W4 * in(ii, jj + 1, kk + 1) * X(ii, jj + 1, kk + 1);
+      break;
+    default:
+      v = W8 * in(ii, jj, kk) * X(ii, jj, kk) + W8 * in(ii + 1, jj, kk) * X(ii + 1, jj, kk) +
+          W8 * in(ii, jj + 1, kk) * X(ii, jj + 1, kk) +
+          W8 * in(ii + 1, jj + 1, kk) * X(ii + 1, jj + 1, kk) +
+          W8 * in(ii, jj, kk + 1) * X(ii, jj, kk + 1) +
+          W8 * in(ii + 1, jj, kk + 1) * X(ii + 1, jj, kk + 1) +
+          W8 * in(ii, jj + 1, kk + 1) * X(ii, jj + 1, kk + 1) +
+          W8 * in(ii + 1, jj + 1, kk + 1) * X(ii + 1, jj + 1, kk + 1);
+      break;
+  }
+  const size_t p = IX(i, j, k, nxf, nyf);
+  xf[p] = xf[p] + T(damp[0]) * v;
+}
+
+// ------------------------------------------------------------- vector kit
+template <class T>
+__global__ void k_fill(T* x, size_t n, T v) {
+  for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (size_t)gridDim.x * kBlock)
+    x[t] = v;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_axpy_int(int nx, int ny, int nz, double a, const T* __restrict__ x, T* __restrict__ y) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int k = blockIdx.y + 1;
+  const int i = 1 + t % (nx - 2), j = 1 + t / (nx - 2);
+  if (j > ny - 2) return;
+  const size_t p = IX(i, j, k, nx, ny);
+  y[p] += T(a) * x[p];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_xpay_int(int nx, int ny, int nz, const double* __restrict__ x, double a,
+               const double* __restrict__ z, double* __restrict__ y) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int k = blockIdx.y + 1;
+  const int i = 1 + t % (nx - 2), j = 1 + t / (nx - 2);
+  if (j > ny - 2) return;
+  const size_t p = IX(i, j, k, nx, ny);
+  double v = x[p];
+  v += a * z[p];
+  y[p] = v;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_copy_int(int nx, int ny, int nz, const T* __restrict__ x, T* __restrict__ y) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int k = blockIdx.y + 1;
+  const int i = 1 + t % (nx - 2), j = 1 + t / (nx - 2);
+  if (j > ny - 2) return;
+  const size_t p = IX(i, j, k, nx, ny);
+  y[p] = x[p];
+}
+
+template <class T>
+__global__ void k_add_diag(size_t n, const T* __restrict__ oC, const T* __restrict__ cc,
+                           T* __restrict__ D) {
+  for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (size_t)gridDim.x * kBlock)
+    D[t] = oC[t] + cc[t];
+}
+
+__global__ void k_scale(size_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
+  for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (size_t)gridDim.x * kBlock)
+    y[t] = a * x[t];
+}
+
+template <class Tin, class Tout>
+__global__ void k_convert(size_t n, const Tin* __restrict__ in, Tout* __restrict__ out) {
+  for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (size_t)gridDim.x * kBlock)
+    out[t] = (Tout)in[t];
+}
+
+// VfboundPMG (mikpckd.c:209-256): x faces, then y faces, then z faces, so
+// edge points take the value of the last face written.
+__global__ void k_faces(int nx, int ny, int nz, double* u, const double* gx, const double* gy,
+                        const double* gz, int pass) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (pass == 0) {
+    if (t >= ny * nz) return;
+    const int j = t % ny, k = t / ny;
+    u[IX(0, j, k, nx, ny)] = gx[j + ny * k];
+    u[IX(nx - 1, j, k, nx, ny)] = gx[j + ny * k + (size_t)ny * nz];
+  } else if (pass == 1) {
+    if (t >= nx * nz) return;
+    const int i = t % nx, k = t / nx;
+    u[IX(i, 0, k, nx, ny)] = gy[i + nx * k];
+    u[IX(i, ny - 1, k, nx, ny)] = gy[i + nx * k + (size_t)nx * nz];
+  } else {
+    if (t >= nx * ny) return;
+    const int i = t % nx, j = t / nx;
+    u[IX(i, j, 0, nx, ny)] = gz[i + nx * j];
+    u[IX(i, j, nz - 1, nx, ny)] = gz[i + nx * j + (size_t)nx * ny];
+  }
+}
+
+// ---------------------------------------------------------------- nonlinear
+// sum over species of (-c_s q_s) cc exp(clamp(-q_s u)) (mypdec.c:145-223)
+__device__ inline double ncvec(double coef, double u, const Ions& I) {
+  double out = 0;
+  for (int s = 0; s < I.nion; s++) {
+    const double zcf2 = -1.0 * I.sconc[s] * I.q[s];
+    const double zu2 = -1.0 * I.q[s];
+    double az = fabs(zcf2 * coef);
+    const double am_zero = (1.0e-20 <= az ? 1.0e-20 : az) * 1.0e20;
+    double a = zu2 * u;
+    double mn = (a <= 0.0) ? a : 0.0;
+    const double am_neg = (mn >= -85.0) ? mn : -85.0;
+    double mx = (a >= 0.0) ? a : 0.0;
+    const double am_pos = (mx <= 85.0) ? mx : 85.0;
+    const double arg = am_zero * (am_neg + am_pos);
+    out = out + zcf2 * coef * exp(arg);
+  }
+  return out;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_nresid7(int nx, int ny, int nz, const double* __restrict__ oC, const double* __restrict__ oE,
+              const double* __restrict__ oN, const double* __restrict__ uC,
+              const double* __restrict__ cc, const double* __restrict__ fc,
+              const double* __restrict__ x, double* __restrict__ r, Ions ions) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int k = blockIdx.y + 1;
+  const int i = 1 + t % (nx - 2), j = 1 + t / (nx - 2);
+  if (j > ny - 2) return;
+  const size_t nxy = (size_t)nx * ny;
+  const size_t p = IX(i, j, k, nx, ny);
+  const double w = ncvec(cc[p], x[p], ions);
+  r[p] = acc7(fc[p], oE, oN, uC, x, p, nx, nxy) - oC[p] * x[p] - w;
+}
+
+__global__ void k_dcvec(size_t n, const double* __restrict__ cc, const double* __restrict__ x,
+                        double* __restrict__ out, Ions I) {
+  for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (size_t)gridDim.x * kBlock) {
+    double o = 0.0;
+    const double coef = cc[t], u = x[t];
+    for (int s = 0; s < I.nion; s++) {
+      const double zcf2 = I.sconc[s] * I.q[s] * I.q[s];
+      const double zu2 = -1.0 * I.q[s];
+      double az = fabs(zcf2 * coef);
+      const double am_zero = (1.0e-20 <= az ? 1.0e-20 : az) * 1.0e20;
+      double a = zu2 * u;
+      double mn = (a <= 0.0) ? a : 0.0;
+      const double am_neg = (mn >= -85.0) ? mn : -85.0;
+      double mx = (a >= 0.0) ? a : 0.0;
+      const double am_pos = (mx <= 85.0) ? mx : 85.0;
+      const double arg = am_zero * (am_neg + am_pos);
+      o += zcf2 * coef * exp(arg);
+    }
+    out[t] = o;
+  }
+}
+
+// ---------------------------------------------------------------- operators
+// VbuildA_fv (buildAd.c:105-281), face terms folded into fc in the order
+// east, north, up, west, south, down.
+__global__ void __launch_bounds__(kBlock)
+    k_buildA(int nx, int ny, int nz, const double* xf, const double* yf, const double* zf,
+             const double* gx, const double* gy, const double* gz, const double* a1,
+             const double* a2, const double* a3, const double* ccf, const double* fcf,
+             double* oC, double* cc, double* fc, double* oE, double* oN, double* uC) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int k = blockIdx.y + 1;
+  const int i = 1 + t % (nx - 2), j = 1 + t / (nx - 2);
+  if (j > ny - 2) return;
+  const double diag = 1.0;
+  const double hzm1 = zf[k] - zf[k - 1], hz = zf[k + 1] - zf[k];
+  const double hym1 = yf[j] - yf[j - 1], hy = yf[j + 1] - yf[j];
+  const double hxm1 = xf[i] - xf[i - 1], hx = xf[i + 1] - xf[i];
+  const double c_oE = diag * (hym1 + hy) * (hzm1 + hz) / (4.0 * hx);
+  const double c_oEm1 = diag * (hym1 + hy) * (hzm1 + hz) / (4.0 * hxm1);
+  const double c_oN = diag * (hxm1 + hx) * (hzm1 + hz) / (4.0 * hy);
+  const double c_oNm1 = diag * (hxm1 + hx) * (hzm1 + hz) / (4.0 * hym1);
+  const double c_uC = diag * (hxm1 + hx) * (hym1 + hy) / (4.0 * hz);
+  const double c_uCm1 = diag * (hxm1 + hx) * (hym1 + hy) / (4.0 * hzm1);
+  const double c_fc = diag * (hxm1 + hx) * (hym1 + hy) * (hzm1 + hz) / 8.0;
+  const size_t nxy = (size_t)nx * ny;
+  const size_t p = IX(i, j, k, nx, ny);
+  double f = c_fc * fcf[p];
+  cc[p] = c_fc * ccf[p];
+  double d = c_oE * a1[p];
+  d += c_oEm1 * a1[p - 1];
+  d += c_oN * a2[p];
+  d += c_oNm1 * a2[p - nx];
+  d += c_uC * a3[p];
+  d += c_uCm1 * a3[p - nxy];
+  oC[p] = d;
+  if (i != nx - 2) {
+    oE[p] = c_oE * a1[p];
+  } else {
+    oE[p] = 0.0;
+    f += c_oE * a1[p] * gx[j + ny * k + (size_t)ny * nz];
+  }
+  if (j != ny - 2) {
+    oN[p] = c_oN * a2[p];
+  } else {
+    oN[p] = 0.0;
+    f += c_oN * a2[p] * gy[i + nx * k + (size_t)nx * nz];
+  }
+  if (k != nz - 2) {
+    uC[p] = c_uC * a3[p];
+  } else {
+    uC[p] = 0.0;
+    f += c_uC * a3[p] * gz[i + nx * j + (size_t)nx * ny];
+  }
+  if (i == 1) f += c_oEm1 * a1[p - 1] * gx[j + ny * k];
+  if (j == 1) f += c_oNm1 * a2[p - nx] * gy[i + nx * k];
+  if (k == 1) f += c_uCm1 * a3[p - nxy] * gz[i + nx * j];
+  fc[p] = f;
+}
+
+// forward offsets of the 14 stored coefficients
+__constant__ int c_fwd[14][3] = {{0, 0, 0},  {1, 0, 0},  {0, 1, 0},  {0, 0, 1}, {1, 1, 0},
+                                 {-1, 1, 0}, {1, 0, 1},  {-1, 0, 1}, {0, 1, 1}, {0, -1, 1},
+                                 {1, 1, 1},  {-1, 1, 1}, {1, -1, 1}, {-1, -1, 1}};
+
+__device__ inline double pweight(int dx, int dy, int dz) {
+  if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) return 0.0;
+  const int m = abs(dx) + abs(dy) + abs(dz);
+  return m == 0 ? 1.0 : (m == 1 ? 0.5 : (m == 2 ? 0.25 : 0.125));
+}
+
+// A(f, f+s) of the fine operator without cc (same search order as the
+// oracle's restatement: first matching stored coefficient, forward then back).
+__device__ inline double fine_entry(int st, const Ops27<double>& A, int nxf, int nyf, int fi,
+                                    int fj, int fk, int sx, int sy, int sz) {
+  if (sx == 0 && sy == 0 && sz == 0) return A.a[0][IX(fi, fj, fk, nxf, nyf)];
+  const int ns = st == 7 ? 4 : 14;
+  for (int s = 1; s < ns; s++) {
+    if (c_fwd[s][0] == sx && c_fwd[s][1] == sy && c_fwd[s][2] == sz)
+      return -A.a[s][IX(fi, fj, fk, nxf, nyf)];
+    if (c_fwd[s][0] == -sx && c_fwd[s][1] == -sy && c_fwd[s][2] == -sz)
+      return -A.a[s][IX(fi + sx, fj + sy, fk + sz, nxf, nyf)];
+  }
+  return 0.0;
+}
+
+struct OutPtrs {
+  double* a[14];
+};
+
+// One thread per (coarse interior point, stored coefficient).
+__global__ void __launch_bounds__(kBlock)
+    k_galerkin(int st, int nxf, int nyf, int nzf, Ops27<double> Af, int nxc, int nyc, int nzc,
+               OutPtrs out) {
+  const long ni = nxc - 2, nj = nyc - 2, nk = nzc - 2;
+  const long tot = ni * nj * nk * 14;
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= tot) return;
+  const int s = (int)(t % 14);
+  const long q = t / 14;
+  const int I = 1 + (int)(q % ni), J = 1 + (int)((q / ni) % nj), K = 1 + (int)(q / (ni * nj));
+  const int ex = c_fwd[s][0], ey = c_fwd[s][1], ez = c_fwd[s][2];
+  const int I2 = I + ex, J2 = J + ey, K2 = K + ez;
+  const size_t pc = IX(I, J, K, nxc, nyc);
+  if (I2 < 1 || I2 > nxc - 2 || J2 < 1 || J2 > nyc - 2 || K2 < 1 || K2 > nzc - 2) {
+    out.a[s][pc] = 0.0;
+    return;
+  }
+  const int f0x = 2 * I, f0y = 2 * J, f0z = 2 * K;
+  const int g0x = 2 * I2, g0y = 2 * J2, g0z = 2 * K2;
+  double sum = 0.0;
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        const int fx = f0x + dx, fy = f0y + dy, fz = f0z + dz;
+        const double p1 = pweight(dx, dy, dz);
+        double tt = 0.0;
+        for (int sz = -1; sz <= 1; sz++)
+          for (int sy = -1; sy <= 1; sy++)
+            for (int sx = -1; sx <= 1; sx++) {
+              if (st == 7 && abs(sx) + abs(sy) + abs(sz) > 1) continue;
+              const double p2 = pweight(fx + sx - g0x, fy + sy - g0y, fz + sz - g0z);
+              if (p2 == 0.0) continue;
+              tt += fine_entry(st, Af, nxf, nyf, fx, fy, fz, sx, sy, sz) * p2;
+            }
+        sum += p1 * tt;
+      }
+  out.a[s][pc] = (s == 0) ? sum : -sum;
+}
+
+// ------------------------------------------------------------ dense coarse
+__global__ void k_dense_assemble(int nx, int ny, int nz, Ops27<double> A, const double* D,
+                                 double* M, int N) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= N) return;
+  const int ni = nx - 2, nj = ny - 2;
+  const int i = r % ni + 1, j = (r / ni) % nj + 1, k = r / (ni * nj) + 1;
+  for (int c = 0; c < N; c++) M[(size_t)r * N + c] = 0.0;
+  M[(size_t)r * N + r] = D[IX(i, j, k, nx, ny)];
+  // couplings to all 26 neighbours: forward ones from this point's
+  // coefficients, backward ones from the neighbour's
+  for (int s = 1; s < 14; s++) {
+    for (int dir = -1; dir <= 1; dir += 2) {
+      const int i2 = i + dir * c_fwd[s][0], j2 = j + dir * c_fwd[s][1], k2 = k + dir * c_fwd[s][2];
+      if (i2 < 1 || i2 > nx - 2 || j2 < 1 || j2 > ny - 2 || k2 < 1 || k2 > nz - 2) continue;
+      const int c = (i2 - 1) + ni * ((j2 - 1) + nj * (k2 - 1));
+      const double v = dir > 0 ? -A.a[s][IX(i, j, k, nx, ny)] : -A.a[s][IX(i2, j2, k2, nx, ny)];
+      M[(size_t)r * N + c] = v;
+    }
+  }
+}
+
+// In-place Gauss-Jordan inverse of an SPD matrix, one workgroup.
+__global__ void __launch_bounds__(1024) k_dense_invert(double* M, double* col, int N) {
+  for (int k = 0; k < N; k++) {
+    // save column k, then scale pivot row
+    for (int i = threadIdx.x; i < N; i += blockDim.x) col[i] = M[(size_t)i * N + k];
+    __syncthreads();
+    const double piv = col[k];
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+      const double v = (j == k) ? 1.0 : M[(size_t)k * N + j];
+      M[(size_t)k * N + j] = v / piv;
+    }
+    __syncthreads();
+    for (long e = threadIdx.x; e < (long)N * N; e += blockDim.x) {
+      const int i = (int)(e / N), j = (int)(e % N);
+      if (i == k) continue;
+      const double f = col[i];
+      const double base = (j == k) ? 0.0 : M[(size_t)i * N + j];
+      M[(size_t)i * N + j] = base - f * M[(size_t)k * N + j];
+    }
+    __syncthreads();
+  }
+}
+
+template <class T>
+__global__ void k_dense_solve(int nx, int ny, int nz, const double* Minv, int N, const T* f,
+                              T* x) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= N) return;
+  const int ni = nx - 2, nj = ny - 2;
+  double s = 0.0;
+  for (int c = 0; c < N; c++) {
+    const int i = c % ni + 1, j = (c / ni) % nj + 1, k = c / (ni * nj) + 1;
+    s += Minv[(size_t)r * N + c] * (double)f[IX(i, j, k, nx, ny)];
+  }
+  const int i = r % ni + 1, j = (r / ni) % nj + 1, k = r / (ni * nj) + 1;
+  x[IX(i, j, k, nx, ny)] = (T)s;
+}
+
+inline unsigned nblocks(size_t n) {
+  size_t b = (n + kBlock - 1) / kBlock;
+  if (b > 65535) b = 65535;
+  if (b == 0) b = 1;
+  return (unsigned)b;
+}
+
+template <class T>
+Ops27<T> to27(const Ops<T>& A) {
+  Ops27<T> o;
+  for (int s = 0; s < 14; s++) o.a[s] = A.a[s];
+  return o;
+}
+
+}  // namespace
+
+// =================================================================== wrappers
+
+template <class T>
+void gsrb7(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc, T* x,
+           int nu, int iadjoint) {
+  const int npair = (nx - 1) / 2;
+  const long cnt = (long)npair * (ny - 2);
+  dim3 g((unsigned)((cnt + kBlock - 1) / kBlock), (unsigned)(nz - 2));
+  for (int it = 0; it < nu; it++) {
+    // red (par 1) then black (par 0); swapped for the adjoint sweep
+    const int first = iadjoint ? 0 : 1;
+    k_gsrb7<T><<<g, kBlock, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, x, first);
+    k_gsrb7<T><<<g, kBlock, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, x, 1 - first);
+  }
+}
+
+template <class T>
+void gsrb27(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc, T* x,
+            int nu, int iadjoint) {
+  const Ops27<T> o = to27(A);
+  for (int it = 0; it < nu; it++)
+    for (int cc = 0; cc < 8; cc++) {
+      const int c = iadjoint ? 7 - cc : cc;
+      const int i0 = (c & 1) ? 1 : 2, j0 = ((c >> 1) & 1) ? 1 : 2, k0 = ((c >> 2) & 1) ? 1 : 2;
+      const int ni = (nx - 2 - i0) / 2 + 1, nj = (ny - 2 - j0) / 2 + 1, nk = (nz - 2 - k0) / 2 + 1;
+      if (ni <= 0 || nj <= 0 || nk <= 0) continue;
+      dim3 g((unsigned)(((long)ni * nj + kBlock - 1) / kBlock), (unsigned)nk);
+      k_gsrb27<T><<<g, kBlock, 0, s>>>(nx, ny, nz, o, D, fc, x, c);
+    }
+}
+
+template <class T>
+void resid(hipStream_t s, int st, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
+           const T* x, T* r) {
+  dim3 g = grid_interior(nx, ny, nz);
+  if (st == 7)
+    k_resid7<T><<<g, kBlock, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, x, r);
+  else
+    k_resid27<T><<<g, kBlock, 0, s>>>(nx, ny, nz, to27(A), D, fc, x, r);
+}
+
+void nresid7(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* cc,
+             const double* fc, const double* x, double* r, const Ions& ions) {
+  k_nresid7<<<grid_interior(nx, ny, nz), kBlock, 0, s>>>(nx, ny, nz, A.a[0], A.a[1], A.a[2],
+                                                         A.a[3], cc, fc, x, r, ions);
+}
+
+void dcvec(hipStream_t s, size_t n, const double* cc, const double* x, double* out,
+           const Ions& ions) {
+  k_dcvec<<<nblocks(n), kBlock, 0, s>>>(n, cc, x, out, ions);
+}
+
+template <class T>
+void restrict3(hipStream_t s, int nxf, int nyf, int nzf, int nxc, int nyc, int nzc, const T* in,
+               T* out) {
+  k_restrict<T><<<grid_interior(nxc, nyc, nzc), kBlock, 0, s>>>(nxf, nyf, nzf, nxc, nyc, nzc, in,
+                                                                out);
+}
+
+template <class T>
+void prolong_add(hipStream_t s, int nxc, int nyc, int nzc, int nxf, int nyf, int nzf, const T* xc,
+                 T* xf, const double* damp) {
+  k_prolong_add<T><<<grid_interior(nxf, nyf, nzf), kBlock, 0, s>>>(nxc, nyc, nzc, nxf, nyf, nzf,
+                                                                   xc, xf, damp);
+}
+
+template <class T>
+void nrm1(hipStream_t s, int nx, int ny, int nz, const T* x, double* part, double* out) {
+  k_nrm1_part<T><<<kParts, kBlock, 0, s>>>(nx, ny, nz, x, part);
+  k_finish<<<1, kBlock, 0, s>>>(part, out, 0);
+}
+
+template <class T>
+void damp_factor(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* x,
+                 const T* w0, double* part, double* out) {
+  k_damp_part<T><<<kParts, kBlock, 0, s>>>(nx, ny, nz, to27(A), D, x, w0, part);
+  k_finish<<<1, kBlock, 0, s>>>(part, out, 1);
+}
+
+template <class T>
+void fill(hipStream_t s, T* x, size_t n, T v) {
+  k_fill<T><<<nblocks(n), kBlock, 0, s>>>(x, n, v);
+}
+
+template <class T>
+void axpy_interior(hipStream_t s, int nx, int ny, int nz, double a, const T* x, T* y) {
+  k_axpy_int<T><<<grid_interior(nx, ny, nz), kBlock, 0, s>>>(nx, ny, nz, a, x, y);
+}
+
+void xpay_interior(hipStream_t s, int nx, int ny, int nz, const double* x, double a,
+                   const double* z, double* y) {
+  k_xpay_int<<<grid_interior(nx, ny, nz), kBlock, 0, s>>>(nx, ny, nz, x, a, z, y);
+}
+
+template <class T>
+void copy_interior(hipStream_t s, int nx, int ny, int nz, const T* x, T* y) {
+  k_copy_int<T><<<grid_interior(nx, ny, nz), kBlock, 0, s>>>(nx, ny, nz, x, y);
+}
+
+template <class T>
+void add_diag(hipStream_t s, size_t n, const T* oC, const T* cc, T* D) {
+  k_add_diag<T><<<nblocks(n), kBlock, 0, s>>>(n, oC, cc, D);
+}
+
+void scale(hipStream_t s, size_t n, double a, const double* x, double* y) {
+  k_scale<<<nblocks(n), kBlock, 0, s>>>(n, a, x, y);
+}
+
+template <class Tin, class Tout>
+void convert(hipStream_t s, size_t n, const Tin* in, Tout* out) {
+  k_convert<Tin, Tout><<<nblocks(n), kBlock, 0, s>>>(n, in, out);
+}
+
+void set_faces(hipStream_t s, int nx, int ny, int nz, double* u, const double* gx,
+               const double* gy, const double* gz) {
+  k_faces<<<nblocks((size_t)ny * nz), kBlock, 0, s>>>(nx, ny, nz, u, gx, gy, gz, 0);
+  k_faces<<<nblocks((size_t)nx * nz), kBlock, 0, s>>>(nx, ny, nz, u, gx, gy, gz, 1);
+  k_faces<<<nblocks((size_t)nx * ny), kBlock, 0, s>>>(nx, ny, nz, u, gx, gy, gz, 2);
+}
+
+void buildA_fv(hipStream_t s, int nx, int ny, int nz, const double* xf, const double* yf,
+               const double* zf, const double* gx, const double* gy, const double* gz,
+               const double* a1, const double* a2, const double* a3, const double* ccf,
+               const double* fcf, double* oC, double* cc, double* fc, double* oE, double* oN,
+               double* uC) {
+  k_buildA<<<grid_interior(nx, ny, nz), kBlock, 0, s>>>(nx, ny, nz, xf, yf, zf, gx, gy, gz, a1,
+                                                        a2, a3, ccf, fcf, oC, cc, fc, oE, oN, uC);
+}
+
+void galerkin(hipStream_t s, int st, int nxf, int nyf, int nzf, const Ops<double>& Af, int nxc,
+              int nyc, int nzc, double* const ac[14]) {
+  OutPtrs o;
+  for (int k = 0; k < 14; k++) o.a[k] = ac[k];
+  const long tot = (long)(nxc - 2) * (nyc - 2) * (nzc - 2) * 14;
+  k_galerkin<<<(unsigned)((tot + kBlock - 1) / kBlock), kBlock, 0, s>>>(st, nxf, nyf, nzf, to27(Af),
+                                                                        nxc, nyc, nzc, o);
+}
+
+void dense_assemble(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* D,
+                    double* M, int N) {
+  k_dense_assemble<<<(N + kBlock - 1) / kBlock, kBlock, 0, s>>>(nx, ny, nz, to27(A), D, M, N);
+}
+
+void dense_invert(hipStream_t s, double* M, double* work, int N) {
+  k_dense_invert<<<1, 1024, 0, s>>>(M, work, N);
+}
+
+template <class T>
+void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int N, const T* f,
+                 T* x) {
+  k_dense_solve<T><<<(N + kBlock - 1) / kBlock, kBlock, 0, s>>>(nx, ny, nz, Minv, N, f, x);
+}
+
+// ------------------------------------------------------ explicit instances
+#define INST(T)                                                                                 \
+  template void gsrb7<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*, T*, int, \
+                         int);                                                                  \
+  template void gsrb27<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*, T*, int, \
+                          int);                                                                 \
+  template void resid<T>(hipStream_t, int, int, int, int, const Ops<T>&, const T*, const T*,     \
+                         const T*, T*);                                                         \
+  template void restrict3<T>(hipStream_t, int, int, int, int, int, int, const T*, T*);          \
+  template void prolong_add<T>(hipStream_t, int, int, int, int, int, int, const T*, T*,         \
+                               const double*);                                                  \
+  template void nrm1<T>(hipStream_t, int, int, int, const T*, double*, double*);                \
+  template void damp_factor<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*,   \
+                               const T*, double*, double*);                                     \
+  template void fill<T>(hipStream_t, T*, size_t, T);                                            \
+  template void axpy_interior<T>(hipStream_t, int, int, int, double, const T*, T*);             \
+  template void copy_interior<T>(hipStream_t, int, int, int, const T*, T*);                     \
+  template void add_diag<T>(hipStream_t, size_t, const T*, const T*, T*);                       \
+  template void dense_solve<T>(hipStream_t, int, int, int, const double*, int, const T*, T*);
+INST(double)
+INST(float)
+#undef INST
+template void convert<double, float>(hipStream_t, size_t, const double*, float*);
+template void convert<float, double>(hipStream_t, size_t, const float*, double*);
+
+}  // namespace pmg

@@ -1,0 +1,172 @@
+// testhooks.cpp -- single-kernel entry points for kernel-by-kernel parity
+// tests (host arrays in, host arrays out).  Not part of the Vpmg API.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "pmg_device.h"
+
+#define TCHECK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) {                                                                \
+      fprintf(stderr, "pmgtest: HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, \
+              __LINE__);                                                                   \
+      abort();                                                                             \
+    }                                                                                      \
+  } while (0)
+
+namespace {
+double* up(const double* h, size_t n) {
+  double* d = nullptr;
+  TCHECK(hipMalloc(&d, (n ? n : 1) * sizeof(double)));
+  if (h) TCHECK(hipMemcpy(d, h, n * sizeof(double), hipMemcpyHostToDevice));
+  return d;
+}
+void down(double* h, const double* d, size_t n) {
+  TCHECK(hipMemcpy(h, d, n * sizeof(double), hipMemcpyDeviceToHost));
+}
+struct OpsHolder {
+  std::vector<double*> ptr;
+  pmg::Ops<double> ops{};
+  OpsHolder(const double* a, size_t n, int st) {
+    const int na = st == 7 ? 4 : 14;
+    for (int s = 0; s < 14; s++) ops.a[s] = nullptr;
+    for (int s = 0; s < na; s++) {
+      ptr.push_back(up(a + (size_t)s * n, n));
+      ops.a[s] = ptr.back();
+    }
+  }
+  ~OpsHolder() {
+    for (double* p : ptr) TCHECK(hipFree(p));
+  }
+};
+}  // namespace
+
+extern "C" {
+
+// x <- nu sweeps (7-point red-black or 27-point 8-colour), D = oC + cc
+int pmgtest_gsrb(int nx, int ny, int nz, int st, const double* a, const double* cc,
+                 const double* fc, double* x, int nu, int iadjoint) {
+  const size_t n = (size_t)nx * ny * nz;
+  OpsHolder A(a, n, st);
+  double *dcc = up(cc, n), *dfc = up(fc, n), *dx = up(x, n), *dD = up(nullptr, n);
+  pmg::add_diag<double>(0, n, A.ops.a[0], dcc, dD);
+  if (st == 7)
+    pmg::gsrb7<double>(0, nx, ny, nz, A.ops, dD, dfc, dx, nu, iadjoint);
+  else
+    pmg::gsrb27<double>(0, nx, ny, nz, A.ops, dD, dfc, dx, nu, iadjoint);
+  TCHECK(hipDeviceSynchronize());
+  down(x, dx, n);
+  TCHECK(hipFree(dcc));
+  TCHECK(hipFree(dfc));
+  TCHECK(hipFree(dx));
+  TCHECK(hipFree(dD));
+  return 1;
+}
+
+int pmgtest_resid(int nx, int ny, int nz, int st, const double* a, const double* cc,
+                  const double* fc, const double* x, double* r) {
+  const size_t n = (size_t)nx * ny * nz;
+  OpsHolder A(a, n, st);
+  double *dcc = up(cc, n), *dfc = up(fc, n), *dx = up(x, n), *dD = up(nullptr, n), *dr = up(r, n);
+  pmg::add_diag<double>(0, n, A.ops.a[0], dcc, dD);
+  pmg::resid<double>(0, st, nx, ny, nz, A.ops, dD, dfc, dx, dr);
+  TCHECK(hipDeviceSynchronize());
+  down(r, dr, n);
+  for (double* p : {dcc, dfc, dx, dD, dr}) TCHECK(hipFree(p));
+  return 1;
+}
+
+int pmgtest_restrict(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc, const double* in,
+                     double* out) {
+  const size_t nf = (size_t)nxf * nyf * nzf, nc = (size_t)nxc * nyc * nzc;
+  double *di = up(in, nf), *dout = up(out, nc);
+  pmg::restrict3<double>(0, nxf, nyf, nzf, nxc, nyc, nzc, di, dout);
+  TCHECK(hipDeviceSynchronize());
+  down(out, dout, nc);
+  TCHECK(hipFree(di));
+  TCHECK(hipFree(dout));
+  return 1;
+}
+
+// xf_out = xf + damp * P xc on the fine interior
+int pmgtest_prolong(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf, const double* xc,
+                    double* xf, double damp) {
+  const size_t nf = (size_t)nxf * nyf * nzf, nc = (size_t)nxc * nyc * nzc;
+  double *dc = up(xc, nc), *df = up(xf, nf), *dd = up(&damp, 1);
+  pmg::prolong_add<double>(0, nxc, nyc, nzc, nxf, nyf, nzf, dc, df, dd);
+  TCHECK(hipDeviceSynchronize());
+  down(xf, df, nf);
+  for (double* p : {dc, df, dd}) TCHECK(hipFree(p));
+  return 1;
+}
+
+int pmgtest_galerkin(int nxf, int nyf, int nzf, int st, const double* af, int nxc, int nyc,
+                     int nzc, double* ac) {
+  const size_t nf = (size_t)nxf * nyf * nzf, nc = (size_t)nxc * nyc * nzc;
+  OpsHolder A(af, nf, st);
+  double* out[14];
+  for (int s = 0; s < 14; s++) {
+    out[s] = up(nullptr, nc);
+    TCHECK(hipMemset(out[s], 0, nc * sizeof(double)));
+  }
+  pmg::galerkin(0, st, nxf, nyf, nzf, A.ops, nxc, nyc, nzc, out);
+  TCHECK(hipDeviceSynchronize());
+  for (int s = 0; s < 14; s++) {
+    down(ac + (size_t)s * nc, out[s], nc);
+    TCHECK(hipFree(out[s]));
+  }
+  return 1;
+}
+
+int pmgtest_buildA(int nx, int ny, int nz, const double* xf, const double* yf, const double* zf,
+                   const double* gx, const double* gy, const double* gz, const double* a1,
+                   const double* a2, const double* a3, const double* ccf, const double* fcf,
+                   double* oC, double* cc, double* fc, double* oE, double* oN, double* uC) {
+  const size_t n = (size_t)nx * ny * nz;
+  double *dxf = up(xf, nx), *dyf = up(yf, ny), *dzf = up(zf, nz);
+  double *dgx = up(gx, 2 * (size_t)ny * nz), *dgy = up(gy, 2 * (size_t)nx * nz),
+         *dgz = up(gz, 2 * (size_t)nx * ny);
+  double *d1 = up(a1, n), *d2 = up(a2, n), *d3 = up(a3, n), *dccf = up(ccf, n), *dfcf = up(fcf, n);
+  double *doC = up(oC, n), *dcc = up(cc, n), *dfc = up(fc, n), *doE = up(oE, n), *doN = up(oN, n),
+         *duC = up(uC, n);
+  pmg::buildA_fv(0, nx, ny, nz, dxf, dyf, dzf, dgx, dgy, dgz, d1, d2, d3, dccf, dfcf, doC, dcc, dfc,
+                 doE, doN, duC);
+  TCHECK(hipDeviceSynchronize());
+  down(oC, doC, n);
+  down(cc, dcc, n);
+  down(fc, dfc, n);
+  down(oE, doE, n);
+  down(oN, doN, n);
+  down(uC, duC, n);
+  for (double* p : {dxf, dyf, dzf, dgx, dgy, dgz, d1, d2, d3, dccf, dfcf, doC, dcc, dfc, doE, doN, duC})
+    TCHECK(hipFree(p));
+  return 1;
+}
+
+// nonlinear term pieces: out_r = nonlinear residual (7-point), out_d = N'(x)
+int pmgtest_nonlinear(int nx, int ny, int nz, const double* a, const double* cc, const double* fc,
+                      const double* x, int nion, const double* q, const double* sconc,
+                      double* out_r, double* out_d) {
+  const size_t n = (size_t)nx * ny * nz;
+  OpsHolder A(a, n, 7);
+  pmg::Ions I{};
+  I.nion = nion;
+  for (int i = 0; i < nion; i++) {
+    I.q[i] = q[i];
+    I.sconc[i] = sconc[i];
+  }
+  double *dcc = up(cc, n), *dfc = up(fc, n), *dx = up(x, n), *dr = up(out_r, n), *dd = up(out_d, n);
+  pmg::nresid7(0, nx, ny, nz, A.ops, dcc, dfc, dx, dr, I);
+  pmg::dcvec(0, n, dcc, dx, dd, I);
+  TCHECK(hipDeviceSynchronize());
+  down(out_r, dr, n);
+  down(out_d, dd, n);
+  for (double* p : {dcc, dfc, dx, dr, dd}) TCHECK(hipFree(p));
+  return 1;
+}
+
+}  // extern "C"

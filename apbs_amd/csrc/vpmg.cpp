@@ -1,0 +1,1894 @@
+// vpmg.cpp -- host side of libapbs_pmg.so: the Vpmg C API on one MI355X.
+//
+// Solve path (src/pmgc/mgdrvd.c, newdrvd.c, mgcsd.c, newtond.c):
+//   fine operator VbuildA_fv -> Galerkin coarse operators -> V-cycles with
+//   Hackbusch-Reusken damping, or damped inexact Newton with V-cycle inner
+//   solves.  All grids live in HBM; the host only reads back stopping norms.
+// Deviations from the reference schedule (the fixed point is the same):
+//   * coarse 27-point levels use 8-colour Gauss-Seidel;
+//   * coarsening continues below the reference's nlev while the grid still
+//     halves, and the coarsest level is solved exactly with a dense inverse
+//     of the full operator (the reference uses banded LINPACK on oC alone
+//     for lpbe and 100 CG iterations for npbe).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/apbs_pmg.h"
+#include "fill_device.h"
+#include "pmg_device.h"
+
+#define HIPCHECK(x)                                                                      \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) {                                                              \
+      fprintf(stderr, "libapbs_pmg: HIP error %s at %s:%d\n", hipGetErrorString(e_),     \
+              __FILE__, __LINE__);                                                       \
+      abort();                                                                           \
+    }                                                                                    \
+  } while (0)
+
+namespace {
+
+// constants of src/generic/vhal.h:377-458, src/generic/vunit.h:68-116 and the
+// MALOC values VSMALL / VPI
+constexpr double VSMALL = 1.0e-9;
+constexpr double VPMGSMALL = 1e-12;
+constexpr double VPI = 3.14159265358979323846;
+constexpr double Vunit_ec = 1.6021773e-19;
+constexpr double Vunit_kb = 1.3806581e-23;
+constexpr double Vunit_eps0 = 8.8541878e-12;
+constexpr double VREDFRAC = 0.25;
+constexpr int VMGNLEV = 4;
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+template <class T>
+T* dalloc(size_t n) {
+  T* p = nullptr;
+  if (n == 0) n = 1;
+  HIPCHECK(hipMalloc(&p, n * sizeof(T)));
+  return p;
+}
+
+template <class T>
+void dfree(T*& p) {
+  if (p) HIPCHECK(hipFree(p));
+  p = nullptr;
+}
+
+int coarsen(int n) { return (n - 1) / 2 + 1; }
+
+}  // namespace
+
+// =================================================================== Valist
+
+extern "C" Valist* Valist_ctor(void) {
+  Valist* v = (Valist*)calloc(1, sizeof(Valist));
+  return v;
+}
+
+extern "C" void Valist_dtor(Valist** thee) {
+  if (thee && *thee) {
+    free((*thee)->atoms);
+    free(*thee);
+    *thee = nullptr;
+  }
+}
+
+extern "C" int Valist_getStatistics(Valist* thee) {
+  if (!thee || thee->number <= 0) return 0;
+  Vatom* a0 = &thee->atoms[0];
+  for (int d = 0; d < 3; d++) thee->mincrd[d] = thee->maxcrd[d] = a0->position[d];
+  thee->maxrad = a0->radius;
+  thee->charge = 0.0;
+  for (int i = 0; i < thee->number; i++) {
+    Vatom* a = &thee->atoms[i];
+    for (int d = 0; d < 3; d++) {
+      if (a->position[d] < thee->mincrd[d]) thee->mincrd[d] = a->position[d];
+      if (a->position[d] > thee->maxcrd[d]) thee->maxcrd[d] = a->position[d];
+    }
+    if (a->radius > thee->maxrad) thee->maxrad = a->radius;
+    thee->charge = thee->charge + a->charge;
+  }
+  for (int d = 0; d < 3; d++) thee->center[d] = 0.5 * (thee->maxcrd[d] + thee->mincrd[d]);
+  return 1;
+}
+
+extern "C" int Valist_setAtoms(Valist* thee, int n, const double* xyz, const double* radius,
+                               const double* charge) {
+  if (!thee || n <= 0) return 0;
+  free(thee->atoms);
+  thee->atoms = (Vatom*)calloc((size_t)n, sizeof(Vatom));
+  thee->number = n;
+  for (int i = 0; i < n; i++) {
+    Vatom* a = &thee->atoms[i];
+    for (int d = 0; d < 3; d++) a->position[d] = xyz[3 * i + d];
+    a->radius = radius[i];
+    a->charge = charge[i];
+    a->partID = 1.0;
+    a->id = i;
+  }
+  return Valist_getStatistics(thee);
+}
+
+extern "C" int Valist_readPQRFile(Valist* thee, const char* path) {
+  FILE* fh = fopen(path, "r");
+  if (!fh) return 0;
+  std::vector<double> xyz, r, q;
+  char line[1024];
+  while (fgets(line, sizeof line, fh)) {
+    if (strncmp(line, "ATOM", 4) && strncmp(line, "HETATM", 6)) continue;
+    std::vector<std::string> tok;
+    char* save = nullptr;
+    for (char* t = strtok_r(line, " \t\r\n", &save); t; t = strtok_r(nullptr, " \t\r\n", &save))
+      tok.emplace_back(t);
+    if (tok.size() < 6) continue;
+    size_t m = tok.size();
+    xyz.push_back(atof(tok[m - 5].c_str()));
+    xyz.push_back(atof(tok[m - 4].c_str()));
+    xyz.push_back(atof(tok[m - 3].c_str()));
+    q.push_back(atof(tok[m - 2].c_str()));
+    r.push_back(atof(tok[m - 1].c_str()));
+  }
+  fclose(fh);
+  if (q.empty()) return 0;
+  return Valist_setAtoms(thee, (int)q.size(), xyz.data(), r.data(), q.data());
+}
+
+extern "C" int Valist_getNumberAtoms(Valist* thee) { return thee ? thee->number : 0; }
+extern "C" Vatom* Valist_getAtom(Valist* thee, int i) {
+  return (thee && i >= 0 && i < thee->number) ? &thee->atoms[i] : nullptr;
+}
+
+// ===================================================================== Vpbe
+
+struct sVpbe {
+  Valist* alist;
+  double soluteCenter[3], soluteRadius, soluteXlen, soluteYlen, soluteZlen, soluteCharge;
+  int numIon;
+  double ionConc[APBS_PMG_MAXION], ionRadii[APBS_PMG_MAXION], ionQ[APBS_PMG_MAXION];
+  double bulkIonicStrength, maxIonRadius;
+  double T, soluteDiel, solventDiel, solventRadius, sdens;
+  double xkappa, deblen, zkappa2, zmagic;
+  double clistMaxRadius;  // Vclist max radius (vpbe.c:427-431)
+};
+
+// Vpbe_ctor2 (vpbe.c:264-465)
+extern "C" Vpbe* Vpbe_ctor(Valist* alist, int ionNum, double* ionConc, double* ionRadii,
+                           double* ionQ, double T, double soluteDiel, double solventDiel,
+                           double solventRadius, int /*focusFlag*/, double sdens,
+                           double /*z_mem*/, double /*L*/, double /*membraneDiel*/,
+                           double /*V*/) {
+  if (!alist || alist->number <= 0) return nullptr;
+  const double N_A = 6.022045000e+23, e_c = 4.803242384e-10, k_B = 1.380662000e-16;
+  const double pi = 4. * atan(1.);
+  sVpbe* t = (sVpbe*)calloc(1, sizeof(sVpbe));
+  t->alist = alist;
+  for (int d = 0; d < 3; d++) t->soluteCenter[d] = alist->center[d];
+  double radius = 0;
+  Vatom* a0 = &alist->atoms[0];
+  double xmin = a0->position[0], xmax = xmin, ymin = a0->position[1], ymax = ymin,
+         zmin = a0->position[2], zmax = zmin, charge = 0;
+  for (int i = 0; i < alist->number; i++) {
+    Vatom* a = &alist->atoms[i];
+    double r = a->radius, x = a->position[0], y = a->position[1], z = a->position[2];
+    if ((x + r) > xmax) xmax = x + r;
+    if ((x - r) < xmin) xmin = x - r;
+    if ((y + r) > ymax) ymax = y + r;
+    if ((y - r) < ymin) ymin = y - r;
+    if ((z + r) > zmax) zmax = z + r;
+    if ((z - r) < zmin) zmin = z - r;
+    double dx = x - t->soluteCenter[0], dy = y - t->soluteCenter[1], dz = z - t->soluteCenter[2];
+    double dist = sqrt(dx * dx + dy * dy + dz * dz) + r;
+    if (dist > radius) radius = dist;
+    charge += a->charge;
+  }
+  t->soluteRadius = radius;
+  t->soluteXlen = xmax - xmin;
+  t->soluteYlen = ymax - ymin;
+  t->soluteZlen = zmax - zmin;
+  t->soluteCharge = charge;
+  if (ionNum >= APBS_PMG_MAXION) {
+    free(t);
+    return nullptr;
+  }
+  t->numIon = ionNum;
+  double netCharge = 0.0;
+  for (int i = 0; i < ionNum; i++) {
+    t->ionConc[i] = ionConc[i];
+    t->ionRadii[i] = ionRadii[i];
+    if (ionRadii[i] > t->maxIonRadius) t->maxIonRadius = ionRadii[i];
+    t->ionQ[i] = ionQ[i];
+    t->bulkIonicStrength += (0.5 * ionConc[i] * ionQ[i] * ionQ[i]);
+    netCharge += (ionConc[i] * ionQ[i]);
+  }
+  if (fabs(netCharge) > VSMALL) {  // counterion imbalance (vpbe.c:362-373)
+    fprintf(stderr, "Vpbe_ctor: counterion charge imbalance (%g M)\n", netCharge);
+    free(t);
+    return nullptr;
+  }
+  if (T == 0.0) {
+    free(t);
+    return nullptr;
+  }
+  t->T = T;
+  t->soluteDiel = soluteDiel;
+  t->solventDiel = solventDiel;
+  t->solventRadius = solventRadius;
+  t->sdens = sdens;
+  if (t->bulkIonicStrength == 0.) {
+    t->xkappa = t->deblen = t->zkappa2 = 0.;
+  } else {
+    t->xkappa = sqrt(t->bulkIonicStrength * 1.0e-16 *
+                     ((8.0 * pi * N_A * e_c * e_c) / (1000.0 * t->solventDiel * k_B * T)));
+    t->deblen = 1. / t->xkappa;
+    t->zkappa2 = t->solventDiel * t->xkappa * t->xkappa;
+  }
+  t->zmagic = ((4.0 * pi * e_c * e_c) / (k_B * t->T)) * 1.0e+8;
+  t->clistMaxRadius =
+      (t->maxIonRadius > t->solventRadius ? t->maxIonRadius : t->solventRadius) + 0.5;
+  return t;
+}
+
+extern "C" void Vpbe_dtor(Vpbe** thee) {
+  if (thee && *thee) {
+    free(*thee);
+    *thee = nullptr;
+  }
+}
+extern "C" double Vpbe_getBulkIonicStrength(Vpbe* t) { return t->bulkIonicStrength; }
+extern "C" double Vpbe_getMaxIonRadius(Vpbe* t) { return t->maxIonRadius; }
+extern "C" double Vpbe_getTemperature(Vpbe* t) { return t->T; }
+extern "C" double Vpbe_getSoluteDiel(Vpbe* t) { return t->soluteDiel; }
+extern "C" double Vpbe_getSolventDiel(Vpbe* t) { return t->solventDiel; }
+extern "C" double Vpbe_getSolventRadius(Vpbe* t) { return t->solventRadius; }
+extern "C" double Vpbe_getXkappa(Vpbe* t) { return t->xkappa; }
+extern "C" double Vpbe_getZkappa2(Vpbe* t) { return t->zkappa2; }
+extern "C" double Vpbe_getZmagic(Vpbe* t) { return t->zmagic; }
+extern "C" double Vpbe_getSoluteRadius(Vpbe* t) { return t->soluteRadius; }
+extern "C" double Vpbe_getSoluteCharge(Vpbe* t) { return t->soluteCharge; }
+extern "C" double* Vpbe_getSoluteCenter(Vpbe* t) { return t->soluteCenter; }
+extern "C" Valist* Vpbe_getValist(Vpbe* t) { return t->alist; }
+extern "C" int Vpbe_getIons(Vpbe* t, int* nion, double ionConc[APBS_PMG_MAXION],
+                            double ionRadii[APBS_PMG_MAXION], double ionQ[APBS_PMG_MAXION]) {
+  *nion = t->numIon;
+  for (int i = 0; i < t->numIon; i++) {
+    ionConc[i] = t->ionConc[i];
+    ionRadii[i] = t->ionRadii[i];
+    ionQ[i] = t->ionQ[i];
+  }
+  return *nion;
+}
+
+// =================================================================== MGparm
+
+extern "C" MGparm* MGparm_ctor(MGparm_CalcType type) {
+  MGparm* m = (MGparm*)calloc(1, sizeof(MGparm));
+  m->type = type;
+  m->nlev = VMGNLEV;
+  m->chgs = VCM_CHARGE;
+  m->parsed = 1;
+  return m;
+}
+
+extern "C" void MGparm_dtor(MGparm** thee) {
+  if (thee && *thee) {
+    free(*thee);
+    *thee = nullptr;
+  }
+}
+
+// nlev / dime fix-up of MGparm_check (mgparm.c:265-320)
+extern "C" int MGparm_check(MGparm* thee) {
+  if (!thee || !thee->parsed || !thee->setdime || !thee->setchgm) return 0;
+  if (thee->type == MCT_MANUAL || thee->type == MCT_DUMMY) {
+    if (!thee->setgrid && !thee->setglen) return 0;
+    if (thee->setgrid && thee->setglen) return 0;
+    if (!thee->setgcent) return 0;
+  }
+  if (thee->type == MCT_AUTO || thee->type == MCT_PARALLEL) {
+    if (!thee->setcglen || !thee->setfglen || !thee->setcgcent || !thee->setfgcent) return 0;
+  }
+  int tdime[3], tnlev[3];
+  if (thee->type != MCT_DUMMY) {
+    for (int i = 0; i < 3; i++) {
+      int ti = thee->dime[i] - 1;
+      if (ti == (int)pow(2, thee->nlev + 1)) {
+        tnlev[i] = thee->nlev;
+        tdime[i] = thee->dime[i];
+      } else {
+        tdime[i] = thee->dime[i];
+        ti = tdime[i] - 1;
+        tnlev[i] = 0;
+        while (ti % 2 == 0 && ti > 0) {
+          tnlev[i]++;
+          ti = (int)ceil(0.5 * ti);
+        }
+        tnlev[i]--;
+        if ((tdime[i] > 65) && (tnlev[i] < VMGNLEV)) {
+          ti = (int)(tdime[i] / pow(2., (VMGNLEV + 1)));
+          if (ti < 1) ti = 1;
+          tdime[i] = ti * (int)(pow(2., (VMGNLEV + 1))) + 1;
+          tnlev[i] = 4;
+        }
+      }
+    }
+  } else {
+    for (int i = 0; i < 3; i++) {
+      tnlev[i] = thee->nlev;
+      tdime[i] = thee->dime[i];
+    }
+  }
+  int nlev = std::min(std::min(tnlev[0], tnlev[1]), tnlev[2]);
+  thee->nlev = nlev;
+  if (nlev <= 0) return 0;
+  for (int i = 0; i < 3; i++) thee->dime[i] = tdime[i];
+  if (thee->type == MCT_MANUAL || thee->type == MCT_DUMMY) {
+    // NOsh_setupCalcMGMANUAL (nosh.c:1787-1799)
+    if (!thee->setgrid)
+      for (int i = 0; i < 3; i++) thee->grid[i] = thee->glen[i] / ((double)(thee->dime[i] - 1));
+    if (!thee->setglen)
+      for (int i = 0; i < 3; i++) thee->glen[i] = thee->grid[i] * ((double)(thee->dime[i] - 1));
+  }
+  return 1;
+}
+
+// NOsh_setupCalcMGAUTO (nosh.c:1828-2101)
+extern "C" int MGparm_setupAuto(MGparm* elec, MGparm** out, int maxcalc) {
+  if (!elec || !out) return 0;
+  double cgrid[3], fgrid[3], redrat[3];
+  int tn[3];
+  for (int j = 0; j < 3; j++) {
+    cgrid[j] = (elec->cglen[j]) / ((double)(elec->dime[j] - 1));
+    fgrid[j] = (elec->fglen[j]) / ((double)(elec->dime[j] - 1));
+  }
+  for (int j = 0; j < 3; j++) {
+    if (fgrid[j] / cgrid[j] < VREDFRAC) {
+      double redfrac = fgrid[j] / cgrid[j];
+      double td = log(redfrac) / log(VREDFRAC);
+      tn[j] = (int)ceil(td) + 1;
+    } else {
+      tn[j] = 2;
+    }
+  }
+  int nfocus = std::max(std::max(tn[0], tn[1]), tn[2]);
+  if (nfocus > maxcalc) return 0;
+  for (int j = 0; j < 3; j++) redrat[j] = pow((fgrid[j] / cgrid[j]), 1.0 / ((double)nfocus - 1.0));
+  for (int f = 0; f < nfocus; f++) {
+    MGparm* c = out[f];
+    MGparm* p = f > 0 ? out[f - 1] : nullptr;
+    *c = *elec;
+    c->type = MCT_MANUAL;
+    for (int j = 0; j < 3; j++) {
+      if (f == 0) {
+        c->grid[j] = cgrid[j];
+        c->glen[j] = elec->cglen[j];
+      } else {
+        c->grid[j] = redrat[j] * (p->grid[j]);
+        c->glen[j] = redrat[j] * (p->glen[j]);
+      }
+    }
+    c->setgrid = c->setglen = 1;
+    if (f == 0) {
+      c->cmeth = elec->ccmeth;
+      for (int j = 0; j < 3; j++) c->center[j] = elec->ccenter[j];
+    } else if (f == nfocus - 1) {
+      c->cmeth = elec->fcmeth;
+      for (int j = 0; j < 3; j++) c->center[j] = elec->fcenter[j];
+    } else {
+      c->cmeth = MCM_FOCUS;
+      for (int j = 0; j < 3; j++) c->center[j] = elec->fcenter[j];
+    }
+    if (f != 0) {
+      for (int j = 0; j < 3; j++) {
+        int dofix = 0;
+        double minf = c->center[j] - 0.5 * (c->glen[j]);
+        double minc = p->center[j] - 0.5 * (p->glen[j]);
+        double d = minc - minf;
+        if (d >= VSMALL) {
+          if (f == nfocus - 1) return 0;
+          c->center[j] += d;
+          dofix = 1;
+        }
+        double maxf = c->center[j] + 0.5 * (c->glen[j]);
+        double maxc = p->center[j] + 0.5 * (p->glen[j]);
+        d = maxf - maxc;
+        if (d >= VSMALL) {
+          if (f == nfocus - 1 || dofix) return 0;
+          c->center[j] -= d;
+        }
+      }
+    }
+    for (int j = 0; j < 6; j++) c->partDisjOwnSide[j] = 0;
+    for (int j = 0; j < 3; j++) {
+      c->partDisjCenter[j] = 0;
+      c->partDisjLength[j] = c->glen[j];
+    }
+    c->setgcent = 1;
+  }
+  return nfocus;
+}
+
+// ==================================================================== Vpmgp
+
+extern "C" void Vpmgp_makeCoarse(int numLevel, int nxOld, int nyOld, int nzOld, int* nxNew,
+                                 int* nyNew, int* nzNew) {
+  *nxNew = nxOld;
+  *nyNew = nyOld;
+  *nzNew = nzOld;
+  for (int l = 0; l < numLevel; l++) {
+    *nxNew = coarsen(*nxNew);
+    *nyNew = coarsen(*nyNew);
+    *nzNew = coarsen(*nzNew);
+  }
+}
+
+// Vpmgp_size (vpmgp.c:196-291): the reference's workspace accounting, kept
+// because callers read these fields.
+extern "C" void Vpmgp_size(Vpmgp* t) {
+  t->nf = t->nx * t->ny * t->nz;
+  t->narr = t->nf;
+  int nxf = t->nx, nyf = t->ny, nzf = t->nz;
+  t->nxc = t->nx;
+  t->nyc = t->ny;
+  t->nzc = t->nz;
+  for (int l = 2; l <= t->nlev; l++) {
+    Vpmgp_makeCoarse(1, nxf, nyf, nzf, &t->nxc, &t->nyc, &t->nzc);
+    nxf = t->nxc;
+    nyf = t->nyc;
+    nzf = t->nzc;
+    t->narr += nxf * nyf * nzf;
+  }
+  t->nc = t->nxc * t->nyc * t->nzc;
+  t->narrc = t->narr - t->nf;
+  int n_band = 0;
+  if (t->mgsolv == 1) {
+    int num_band = 1 + (t->nxc - 2) * (t->nyc - 2) + (t->nxc - 2) + 1;
+    int nc_band = (t->nxc - 2) * (t->nyc - 2) * (t->nzc - 2);
+    n_band = nc_band * num_band;
+  }
+  t->n_rpc = 100 * (t->nlev + 1);
+  t->nrwk = 2 * (size_t)t->narr + (size_t)4 * t->nf + (size_t)(27 + 14) * t->narrc + n_band + t->n_rpc;
+  t->n_iz = 50 * (t->nlev + 1);
+  t->n_ipc = 100 * (t->nlev + 1);
+  t->niwk = t->n_iz + t->n_ipc;
+}
+
+// Vpmgp_ctor2 (vpmgp.c:95-170)
+extern "C" Vpmgp* Vpmgp_ctor(MGparm* mgparm) {
+  if (!mgparm) return nullptr;
+  Vpmgp* t = (Vpmgp*)calloc(1, sizeof(Vpmgp));
+  t->nx = mgparm->dime[0];
+  t->ny = mgparm->dime[1];
+  t->nz = mgparm->dime[2];
+  t->hx = mgparm->grid[0];
+  t->hy = mgparm->grid[1];
+  t->hzed = mgparm->grid[2];
+  t->xlen = ((double)(mgparm->dime[0] - 1)) * mgparm->grid[0];
+  t->ylen = ((double)(mgparm->dime[1] - 1)) * mgparm->grid[1];
+  t->zlen = ((double)(mgparm->dime[2] - 1)) * mgparm->grid[2];
+  t->nlev = mgparm->nlev;
+  t->nonlin = mgparm->nonlintype;
+  t->meth = mgparm->method;
+  t->ipkey = (t->nonlin == NONLIN_LPBE) ? IPKEY_LPBE
+                                        : (t->nonlin == NONLIN_SMPBE ? IPKEY_SMPBE : IPKEY_NPBE);
+  t->errtol = mgparm->setetol ? mgparm->etol : 1.0e-6;
+  t->itmax = 200;
+  t->istop = 1;
+  t->iinfo = 1;
+  t->bcfl = BCFL_SDH;
+  t->mgcoar = 2;
+  t->nu1 = 2;
+  t->nu2 = 2;
+  t->omegal = 19.4e-1;
+  t->omegan = 9.0e-1;
+  t->ipcon = 3;
+  t->irite = 8;
+  t->mgsmoo = 1;
+  t->mgsolv = (t->nonlin == NONLIN_NPBE || t->nonlin == NONLIN_SMPBE) ? 0 : 1;
+  for (int d = 0; d < 3; d++) (&t->xcent)[d] = mgparm->center[d];
+  return t;
+}
+
+extern "C" void Vpmgp_dtor(Vpmgp** thee) {
+  if (thee && *thee) {
+    free(*thee);
+    *thee = nullptr;
+  }
+}
+
+// ============================================================ device state
+
+namespace {
+
+struct DLevel {
+  int nx = 0, ny = 0, nz = 0, st = 7;
+  size_t n = 0;
+  double* a[14] = {};
+  double *cc = nullptr, *fc = nullptr, *D = nullptr, *x = nullptr, *w0 = nullptr;
+  double *cprime = nullptr, *xtmp = nullptr;
+  // FP32 copies for the mixed-precision correction cycles
+  float* af[14] = {};
+  float *Df = nullptr, *xfl = nullptr, *w0f = nullptr;
+
+  pmg::Ops<double> ops() const {
+    pmg::Ops<double> o;
+    for (int s = 0; s < 14; s++) o.a[s] = a[s];
+    return o;
+  }
+  pmg::Ops<float> opsf() const {
+    pmg::Ops<float> o;
+    for (int s = 0; s < 14; s++) o.a[s] = af[s];
+    return o;
+  }
+};
+
+struct DevCells {
+  int* start = nullptr;
+  int* list = nullptr;
+  pmg::Cells c{};
+  void release() {
+    dfree(start);
+    dfree(list);
+  }
+};
+
+struct PmgDev {
+  hipStream_t s = nullptr;
+  int nx = 0, ny = 0, nz = 0;
+  size_t n = 0;
+  // fine maps
+  double *epsx = nullptr, *epsy = nullptr, *epsz = nullptr, *kappa = nullptr,
+         *charge = nullptr, *u = nullptr, *pvec = nullptr, *ccf = nullptr;
+  double *gx = nullptr, *gy = nullptr, *gz = nullptr, *xf = nullptr, *yf = nullptr,
+         *zf = nullptr;
+  // atoms
+  int natoms = 0;
+  double *apos = nullptr, *arad = nullptr, *achg = nullptr, *apart = nullptr;
+  std::vector<double> hpart;  // host partID copy
+  // hierarchy
+  int nlev_ref = 0, ntot = 0;
+  DLevel L[24];
+  bool built = false;
+  double *w1 = nullptr, *F = nullptr, *wn1 = nullptr, *wn2 = nullptr, *wn3 = nullptr,
+         *nlw = nullptr;
+  float* w1f = nullptr;
+  double *part = nullptr, *scal = nullptr;
+  double* hscal = nullptr;  // pinned
+  double *dense = nullptr, *dwork = nullptr;
+  int dn = 0;
+  // forces cache
+  double *fqf = nullptr, *fib = nullptr, *fdb = nullptr;
+  int* frc = nullptr;
+  double* atmp = nullptr;  // per-atom scratch
+  bool forces_valid = false;
+  std::vector<double> hqf, hib, hdb;
+  std::vector<int> hrc;
+  // options / stats
+  int precision = 0;
+  double mixed_tol = -1.0;
+  int mirror = 1;
+  double stats[8] = {};
+  pmg::Geo geo{};
+};
+
+PmgDev* DEV(Vpmg* p) { return (PmgDev*)p->dev; }
+
+double read_scalar(PmgDev* d, const double* dptr) {
+  HIPCHECK(hipMemcpyAsync(d->hscal, dptr, sizeof(double), hipMemcpyDeviceToHost, d->s));
+  HIPCHECK(hipStreamSynchronize(d->s));
+  return d->hscal[0];
+}
+
+// Cell list over points with ascending indices per cell (counting sort), for
+// gathers that must visit points in index order.
+DevCells build_cells(hipStream_t s, const double* hpos, int n, double reach) {
+  DevCells dc;
+  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+  for (int a = 0; a < n; a++)
+    for (int d = 0; d < 3; d++) {
+      mn[d] = std::min(mn[d], hpos[3 * a + d]);
+      mx[d] = std::max(mx[d], hpos[3 * a + d]);
+    }
+  double cs = reach * 1.001 + 1e-6;
+  if (cs < 0.25) cs = 0.25;
+  // keep the table bounded for very sparse/large sets
+  for (;;) {
+    long cells = 1;
+    for (int d = 0; d < 3; d++) cells *= (long)((mx[d] - (mn[d] - cs)) / cs) + 3;
+    if (cells <= 8l * 1024 * 1024) break;
+    cs *= 1.5;
+  }
+  dc.c.cs = cs;
+  for (int d = 0; d < 3; d++) {
+    dc.c.lo[d] = mn[d] - cs;
+    dc.c.dim[d] = (int)((mx[d] - dc.c.lo[d]) / cs) + 2;
+  }
+  const long nc = (long)dc.c.dim[0] * dc.c.dim[1] * dc.c.dim[2];
+  std::vector<int> start(nc + 1, 0), list(std::max(n, 1)), cid(std::max(n, 1));
+  for (int a = 0; a < n; a++) {
+    int ic[3];
+    for (int d = 0; d < 3; d++) ic[d] = (int)floor((hpos[3 * a + d] - dc.c.lo[d]) / cs);
+    cid[a] = ic[0] + dc.c.dim[0] * (ic[1] + dc.c.dim[1] * ic[2]);
+    start[cid[a] + 1]++;
+  }
+  for (long i = 0; i < nc; i++) start[i + 1] += start[i];
+  std::vector<int> fill(nc, 0);
+  for (int a = 0; a < n; a++) list[start[cid[a]] + fill[cid[a]]++] = a;
+  dc.start = dalloc<int>(nc + 1);
+  dc.list = dalloc<int>(std::max(n, 1));
+  HIPCHECK(hipMemcpyAsync(dc.start, start.data(), (nc + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemcpyAsync(dc.list, list.data(), std::max(n, 1) * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  dc.c.start = dc.start;
+  dc.c.list = dc.list;
+  return dc;
+}
+
+// VaccSurf_refSphere (vacc.c:924-978); computed with the host libm, as the
+// reference does.
+std::vector<double> ref_sphere(int npts) {
+  std::vector<double> pts;
+  double frac = ((double)npts) / 4.0;
+  int ntheta = (int)floor(sqrt(VPI * frac) + 0.5);
+  double dtheta = VPI / ((double)ntheta);
+  int nphimax = 2 * ntheta;
+  for (int it = 0; it < ntheta; it++) {
+    double theta = dtheta * ((double)it);
+    double sintheta = sin(theta), costheta = cos(theta);
+    int nphi = (int)floor(sintheta * nphimax + 0.5);
+    if (nphi != 0) {
+      double dphi = 2 * VPI / ((double)nphi);
+      for (int ip = 0; ip < nphi; ip++) {
+        double phi = dphi * ((double)ip);
+        pts.push_back(cos(phi) * sintheta);
+        pts.push_back(sin(phi) * sintheta);
+        pts.push_back(costheta);
+      }
+    }
+  }
+  return pts;
+}
+
+pmg::AtomsDev atoms_dev(PmgDev* d) {
+  pmg::AtomsDev A;
+  A.n = d->natoms;
+  A.pos = d->apos;
+  A.rad = d->arad;
+  A.chg = d->achg;
+  A.partID = d->apart;
+  return A;
+}
+
+void upload_part(PmgDev* d) {
+  HIPCHECK(hipMemcpyAsync(d->apart, d->hpart.data(), d->natoms * sizeof(double),
+                          hipMemcpyHostToDevice, d->s));
+}
+
+// --------------------------------------------------------------- hierarchy
+
+void free_hierarchy(PmgDev* d) {
+  for (int l = 0; l < d->ntot; l++) {
+    DLevel& L = d->L[l];
+    for (int s = 0; s < 14; s++) {
+      dfree(L.a[s]);
+      dfree(L.af[s]);
+    }
+    dfree(L.cc);
+    dfree(L.fc);
+    dfree(L.D);
+    if (l > 0) dfree(L.x);
+    L.x = nullptr;
+    dfree(L.w0);
+    dfree(L.cprime);
+    dfree(L.xtmp);
+    dfree(L.Df);
+    dfree(L.xfl);
+    dfree(L.w0f);
+  }
+  dfree(d->w1);
+  dfree(d->w1f);
+  dfree(d->F);
+  dfree(d->wn1);
+  dfree(d->wn2);
+  dfree(d->wn3);
+  dfree(d->nlw);
+  dfree(d->dense);
+  dfree(d->dwork);
+  d->ntot = 0;
+  d->built = false;
+}
+
+// Level sizes: the reference's nlev (Vbuildstr, mgsubd.c:261-342) extended
+// while every axis still halves exactly and the coarsest interior exceeds 64.
+int plan_levels(int nx, int ny, int nz, int nlev, int dims[][3]) {
+  int cx = nx, cy = ny, cz = nz, nt = 0;
+  dims[nt][0] = cx;
+  dims[nt][1] = cy;
+  dims[nt][2] = cz;
+  nt++;
+  for (int l = 1; l < nlev; l++) {
+    cx = coarsen(cx);
+    cy = coarsen(cy);
+    cz = coarsen(cz);
+    dims[nt][0] = cx;
+    dims[nt][1] = cy;
+    dims[nt][2] = cz;
+    nt++;
+  }
+  while (((cx - 1) % 2 == 0) && ((cy - 1) % 2 == 0) && ((cz - 1) % 2 == 0) && coarsen(cx) >= 3 &&
+         coarsen(cy) >= 3 && coarsen(cz) >= 3 && (cx - 2) * (cy - 2) * (cz - 2) > 64 && nt < 23) {
+    cx = coarsen(cx);
+    cy = coarsen(cy);
+    cz = coarsen(cz);
+    dims[nt][0] = cx;
+    dims[nt][1] = cy;
+    dims[nt][2] = cz;
+    nt++;
+  }
+  return nt;
+}
+
+void alloc_hierarchy(PmgDev* d, int nlev, bool newton, bool mixed) {
+  free_hierarchy(d);
+  int dims[24][3];
+  d->nlev_ref = nlev;
+  d->ntot = plan_levels(d->nx, d->ny, d->nz, nlev, dims);
+  for (int l = 0; l < d->ntot; l++) {
+    DLevel& L = d->L[l];
+    L.nx = dims[l][0];
+    L.ny = dims[l][1];
+    L.nz = dims[l][2];
+    L.n = (size_t)L.nx * L.ny * L.nz;
+    L.st = l == 0 ? 7 : 27;
+    const int na = l == 0 ? 4 : 14;
+    for (int s = 0; s < na; s++) L.a[s] = dalloc<double>(L.n);
+    L.cc = dalloc<double>(L.n);
+    L.fc = dalloc<double>(L.n);
+    L.D = dalloc<double>(L.n);
+    L.x = l == 0 ? d->u : dalloc<double>(L.n);
+    L.w0 = dalloc<double>(L.n);
+    pmg::fill<double>(d->s, L.w0, L.n, 0.0);
+    if (l > 0) pmg::fill<double>(d->s, L.x, L.n, 0.0);
+    if (newton) {
+      L.cprime = dalloc<double>(L.n);
+      L.xtmp = dalloc<double>(L.n);
+      pmg::fill<double>(d->s, L.xtmp, L.n, 0.0);
+    }
+    if (mixed) {
+      for (int s = 0; s < na; s++) L.af[s] = dalloc<float>(L.n);
+      L.Df = dalloc<float>(L.n);
+      L.xfl = dalloc<float>(L.n);
+      L.w0f = dalloc<float>(L.n);
+      pmg::fill<float>(d->s, L.xfl, L.n, 0.0f);
+      pmg::fill<float>(d->s, L.w0f, L.n, 0.0f);
+    }
+  }
+  d->w1 = dalloc<double>(d->n);
+  pmg::fill<double>(d->s, d->w1, d->n, 0.0);
+  if (mixed) {
+    d->w1f = dalloc<float>(d->n);
+    pmg::fill<float>(d->s, d->w1f, d->n, 0.0f);
+  }
+  d->F = dalloc<double>(d->n);
+  pmg::fill<double>(d->s, d->F, d->n, 0.0);
+  if (newton) {
+    d->wn1 = dalloc<double>(d->n);
+    d->wn2 = dalloc<double>(d->n);
+    d->wn3 = dalloc<double>(d->n);
+    d->nlw = dalloc<double>(d->n);
+    pmg::fill<double>(d->s, d->wn1, d->n, 0.0);
+    pmg::fill<double>(d->s, d->wn2, d->n, 0.0);
+    pmg::fill<double>(d->s, d->wn3, d->n, 0.0);
+  }
+  const DLevel& C = d->L[d->ntot - 1];
+  d->dn = (C.nx - 2) * (C.ny - 2) * (C.nz - 2);
+  d->dense = dalloc<double>((size_t)d->dn * d->dn);
+  d->dwork = dalloc<double>(d->dn);
+}
+
+// Vbuildops (mgsubd.c:61-258): fine FV operator, Galerkin coarse operators,
+// restricted Helmholtz term and source.
+void build_operators(PmgDev* d) {
+  DLevel& F = d->L[0];
+  pmg::buildA_fv(d->s, d->nx, d->ny, d->nz, d->xf, d->yf, d->zf, d->gx, d->gy, d->gz, d->epsx,
+                 d->epsy, d->epsz, d->ccf, d->charge, F.a[0], F.cc, F.fc, F.a[1], F.a[2], F.a[3]);
+  for (int l = 1; l < d->ntot; l++) {
+    DLevel& P = d->L[l - 1];
+    DLevel& C = d->L[l];
+    pmg::galerkin(d->s, P.st, P.nx, P.ny, P.nz, P.ops(), C.nx, C.ny, C.nz, C.a);
+    pmg::restrict3<double>(d->s, P.nx, P.ny, P.nz, C.nx, C.ny, C.nz, P.cc, C.cc);
+    pmg::restrict3<double>(d->s, P.nx, P.ny, P.nz, C.nx, C.ny, C.nz, P.fc, C.fc);
+  }
+}
+
+// D_l = oC_l + c_l for the given per-level Helmholtz arrays, then the dense
+// inverse of the coarsest level.
+void set_diagonals(PmgDev* d, bool use_cprime) {
+  for (int l = 0; l < d->ntot; l++) {
+    DLevel& L = d->L[l];
+    pmg::add_diag<double>(d->s, L.n, L.a[0], use_cprime ? L.cprime : L.cc, L.D);
+  }
+  const DLevel& C = d->L[d->ntot - 1];
+  pmg::dense_assemble(d->s, C.nx, C.ny, C.nz, C.ops(), C.D, d->dense, d->dn);
+  pmg::dense_invert(d->s, d->dense, d->dwork, d->dn);
+}
+
+void sync_fp32(PmgDev* d) {
+  for (int l = 0; l < d->ntot; l++) {
+    DLevel& L = d->L[l];
+    const int na = l == 0 ? 4 : 14;
+    for (int s = 0; s < na; s++) pmg::convert<double, float>(d->s, L.n, L.a[s], L.af[s]);
+    pmg::convert<double, float>(d->s, L.n, L.D, L.Df);
+  }
+}
+
+// ------------------------------------------------------------------ V-cycle
+// One V-cycle of Vmvcs (mgcsd.c:315-523) on per-level arrays.  X[l] are the
+// per-level solutions (X[0] fine), f the fine right-hand side, w1 a fine-size
+// scratch buffer, part/scal reduction scratch.
+template <class T>
+struct CycleArgs {
+  T* X[24];
+  const T* Dl[24];
+  pmg::Ops<T> A[24];
+  T* w0[24];
+  const T* f;
+  T* w1;
+};
+
+template <class T>
+void vcycle(PmgDev* d, CycleArgs<T>& c) {
+  hipStream_t s = d->s;
+  const int nt = d->ntot;
+  DLevel* L = d->L;
+  pmg::gsrb7<T>(s, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], 2, 0);
+  pmg::resid<T>(s, 7, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], c.w1);
+  for (int l = 1; l < nt; l++) {
+    pmg::restrict3<T>(s, L[l - 1].nx, L[l - 1].ny, L[l - 1].nz, L[l].nx, L[l].ny, L[l].nz, c.w1,
+                      c.w0[l]);
+    if (l != nt - 1) {
+      pmg::fill<T>(s, c.X[l], L[l].n, T(0));
+      pmg::gsrb27<T>(s, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l], 2, 0);
+      pmg::resid<T>(s, 27, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l], c.w1);
+    }
+  }
+  const DLevel& C = L[nt - 1];
+  pmg::dense_solve<T>(s, C.nx, C.ny, C.nz, d->dense, d->dn, c.w0[nt - 1], c.X[nt - 1]);
+  for (int l = nt - 2; l >= 0; l--) {
+    const DLevel& Cl = L[l + 1];
+    pmg::damp_factor<T>(s, Cl.nx, Cl.ny, Cl.nz, c.A[l + 1], c.Dl[l + 1], c.X[l + 1], c.w0[l + 1],
+                        d->part, d->scal + 4);
+    pmg::prolong_add<T>(s, Cl.nx, Cl.ny, Cl.nz, L[l].nx, L[l].ny, L[l].nz, c.X[l + 1], c.X[l],
+                        d->scal + 4);
+    if (l == 0)
+      pmg::gsrb7<T>(s, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], 2, 1);
+    else
+      pmg::gsrb27<T>(s, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l], 2, 1);
+  }
+}
+
+CycleArgs<double> args64(PmgDev* d, double* const* X, const double* f) {
+  CycleArgs<double> c;
+  for (int l = 0; l < d->ntot; l++) {
+    c.X[l] = X[l];
+    c.Dl[l] = d->L[l].D;
+    c.A[l] = d->L[l].ops();
+    c.w0[l] = d->L[l].w0;
+  }
+  c.f = f;
+  c.w1 = d->w1;
+  return c;
+}
+
+// Vmvcs with istop 0 (absolute) or 1 (relative to ||f||1); returns cycles
+int mvcs64(PmgDev* d, double* const* X, const double* f, int istop, int itmax, double errtol,
+           double* relres) {
+  const DLevel& F = d->L[0];
+  double rsden = 1.0;
+  if (istop == 1) {
+    pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, f, d->part, d->scal);
+    rsden = read_scalar(d, d->scal);
+  }
+  if (rsden == 0.0) rsden = 1.0;
+  CycleArgs<double> c = args64(d, X, f);
+  int iters = 0;
+  double rsnrm = rsden;
+  do {
+    vcycle<double>(d, c);
+    iters++;
+    pmg::resid<double>(d->s, 7, F.nx, F.ny, F.nz, c.A[0], c.Dl[0], f, X[0], d->w1);
+    pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, d->w1, d->part, d->scal);
+    rsnrm = read_scalar(d, d->scal);
+  } while (iters < itmax && (rsnrm / rsden) > errtol);
+  if (relres) *relres = rsnrm / rsden;
+  return iters;
+}
+
+// Mixed precision: FP32 V-cycles on the correction equation inside an FP64
+// defect-correction loop (fine residual, update and stopping test in FP64).
+int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int itmax,
+                double errtol, double* relres, int* outer) {
+  const DLevel& F = d->L[0];
+  double rsden = 1.0;
+  if (istop == 1) {
+    pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, f, d->part, d->scal);
+    rsden = read_scalar(d, d->scal);
+  }
+  if (rsden == 0.0) rsden = 1.0;
+  CycleArgs<float> c;
+  for (int l = 0; l < d->ntot; l++) {
+    c.X[l] = d->L[l].xfl;
+    c.Dl[l] = d->L[l].Df;
+    c.A[l] = d->L[l].opsf();
+    c.w0[l] = d->L[l].w0f;
+  }
+  c.w1 = d->w1f;
+  // r (FP64) -> FP32 rhs stored in L[0].w0f
+  c.f = d->L[0].w0f;
+  const pmg::Ops<double> A0 = F.ops();
+  int it = 0;
+  double rsnrm = rsden;
+  pmg::resid<double>(d->s, 7, F.nx, F.ny, F.nz, A0, F.D, f, X64[0], d->w1);
+  pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, d->w1, d->part, d->scal);
+  rsnrm = read_scalar(d, d->scal);
+  while (it < itmax && (rsnrm / rsden) > errtol) {
+    pmg::convert<double, float>(d->s, F.n, d->w1, d->L[0].w0f);
+    pmg::fill<float>(d->s, d->L[0].xfl, F.n, 0.0f);
+    vcycle<float>(d, c);
+    // X64 += (double) e
+    pmg::convert<float, double>(d->s, F.n, d->L[0].xfl, d->F);
+    pmg::axpy_interior<double>(d->s, F.nx, F.ny, F.nz, 1.0, d->F, X64[0]);
+    it++;
+    pmg::resid<double>(d->s, 7, F.nx, F.ny, F.nz, A0, F.D, f, X64[0], d->w1);
+    pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, d->w1, d->part, d->scal);
+    rsnrm = read_scalar(d, d->scal);
+  }
+  if (relres) *relres = rsnrm / rsden;
+  if (outer) *outer = it;
+  return it;
+}
+
+pmg::Ions ions_scaled(Vpbe* pbe) {
+  pmg::Ions I{};
+  I.nion = pbe->numIon;
+  double zks2 = (pbe->bulkIonicStrength > 0.0) ? 0.5 / pbe->bulkIonicStrength : 0.0;
+  for (int i = 0; i < I.nion; i++) {
+    I.q[i] = pbe->ionQ[i];
+    I.sconc[i] = zks2 * pbe->ionConc[i];  // vpmg.c:340-377
+  }
+  return I;
+}
+
+// Vnewton (newtond.c:162-548) with Vgetjac (:550-598)
+void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out) {
+  PmgDev* d = DEV(thee);
+  DLevel* L = d->L;
+  const DLevel& F = L[0];
+  const int nx = F.nx, ny = F.ny, nz = F.nz;
+  const pmg::Ions I = ions_scaled(thee->pbe);
+  const pmg::Ops<double> A0 = F.ops();
+  hipStream_t s = d->s;
+  // rsden = ||F(0)||1
+  pmg::fill<double>(s, d->wn1, d->n, 0.0);
+  pmg::nresid7(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->wn2, I);
+  pmg::nrm1<double>(s, nx, ny, nz, d->wn2, d->part, d->scal);
+  double rsden = read_scalar(d, d->scal);
+  if (rsden == 0.0) rsden = 1.0;
+  pmg::nresid7(s, nx, ny, nz, A0, F.cc, F.fc, d->u, d->F, I);
+  pmg::nrm1<double>(s, nx, ny, nz, d->F, d->part, d->scal);
+  double xnorm_old = read_scalar(d, d->scal);
+  double xnorm_new = xnorm_old, xnorm_med = 0.0, rsnrm = rsden;
+  int idamp = 1, iters = 0, inner = 0;
+  const double errtol = thee->pmgp->errtol;
+  const int itmax = thee->pmgp->itmax;
+  double* X[24];
+  for (;;) {
+    iters++;
+    // Jacobian diagonal on all levels
+    pmg::dcvec(s, d->n, F.cc, d->u, L[0].cprime, I);
+    for (int l = 1; l < d->ntot; l++)
+      pmg::restrict3<double>(s, L[l - 1].nx, L[l - 1].ny, L[l - 1].nz, L[l].nx, L[l].ny, L[l].nz,
+                             L[l - 1].cprime, L[l].cprime);
+    set_diagonals(d, true);
+    const double errtol_s = std::min(0.9 * xnorm_old, 1.0 * pow(xnorm_old, 2.0));
+    for (int l = 0; l < d->ntot; l++) {
+      X[l] = L[l].xtmp;
+      pmg::fill<double>(s, X[l], L[l].n, 0.0);
+    }
+    double rr = 0;
+    int its;
+    if (d->precision == 1) {
+      sync_fp32(d);
+      int outer = 0;
+      mixed_solve(d, X, d->F, 0, 1000, errtol_s, &rr, &outer);
+      its = outer;
+    } else {
+      its = mvcs64(d, X, d->F, 0, 1000, errtol_s, &rr);
+    }
+    inner += its;
+    double* xtmp = L[0].xtmp;
+    if (idamp == 1) {
+      pmg::xpay_interior(s, nx, ny, nz, d->u, 1.0, xtmp, d->wn1);
+      pmg::nresid7(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->F, I);
+      pmg::nrm1<double>(s, nx, ny, nz, d->F, d->part, d->scal);
+      xnorm_new = read_scalar(d, d->scal);
+      double damp = 1.0;
+      int iter_d = 0, mode = 0;
+      while (iter_d < 10) {
+        if (mode == 0) {
+          if (xnorm_new < xnorm_old) mode = 1;
+        } else if (xnorm_new > xnorm_med) {
+          break;
+        }
+        pmg::copy_interior<double>(s, nx, ny, nz, d->wn1, d->wn2);
+        pmg::copy_interior<double>(s, nx, ny, nz, d->F, d->wn3);
+        xnorm_med = xnorm_new;
+        damp = damp / 2.0;
+        pmg::xpay_interior(s, nx, ny, nz, d->u, damp, xtmp, d->wn1);
+        pmg::nresid7(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->F, I);
+        pmg::nrm1<double>(s, nx, ny, nz, d->F, d->part, d->scal);
+        xnorm_new = read_scalar(d, d->scal);
+        iter_d++;
+      }
+      pmg::copy_interior<double>(s, nx, ny, nz, d->wn2, d->u);
+      pmg::copy_interior<double>(s, nx, ny, nz, d->wn3, d->F);
+      xnorm_new = xnorm_med;
+      xnorm_old = xnorm_new;
+      if ((iter_d - 1) == 0) idamp = 0;
+    } else {
+      pmg::axpy_interior<double>(s, nx, ny, nz, 1.0, xtmp, d->u);
+      pmg::nresid7(s, nx, ny, nz, A0, F.cc, F.fc, d->u, d->F, I);
+      pmg::nrm1<double>(s, nx, ny, nz, d->F, d->part, d->scal);
+      xnorm_new = read_scalar(d, d->scal);
+      xnorm_old = xnorm_new;
+    }
+    rsnrm = xnorm_new;
+    if ((rsnrm / rsden) <= errtol) break;
+    if (iters >= itmax) break;
+  }
+  *relres_out = rsnrm / rsden;
+  *iters_out = iters;
+  *inner_out = inner;
+}
+
+void ensure_forces(Vpmg* thee);
+
+}  // namespace
+
+// ===================================================================== Vpmg
+
+extern "C" int Vpmg_ctor2(Vpmg* thee, Vpmgp* pmgp, Vpbe* pbe, int focusFlag, Vpmg* pmgOLD,
+                          MGparm* mgparm, PBEparm_calcEnergy energyFlag);
+
+extern "C" Vpmg* Vpmg_ctor(Vpmgp* pmgp, Vpbe* pbe, int focusFlag, Vpmg* pmgOLD, MGparm* mgparm,
+                           PBEparm_calcEnergy energyFlag) {
+  Vpmg* t = (Vpmg*)calloc(1, sizeof(Vpmg));
+  if (!Vpmg_ctor2(t, pmgp, pbe, focusFlag, pmgOLD, mgparm, energyFlag)) {
+    free(t);
+    return nullptr;
+  }
+  return t;
+}
+
+static void ext_energy(Vpmg* thee, Vpmg* old, PBEparm_calcEnergy, double* partMin,
+                       double* partMax, int* bflags);
+static int focus_fill_bound(Vpmg* thee, Vpmg* old);
+
+extern "C" int Vpmg_ctor2(Vpmg* thee, Vpmgp* pmgp, Vpbe* pbe, int focusFlag, Vpmg* pmgOLD,
+                          MGparm* mgparm, PBEparm_calcEnergy energyFlag) {
+  if (!thee || !pmgp || !pbe || !mgparm) return 0;
+  thee->pmgp = pmgp;
+  thee->pbe = pbe;
+  Vpmgp_size(pmgp);
+  if ((pmgp->nonlin == NONLIN_NPBE || pmgp->nonlin == NONLIN_SMPBE) && pmgp->meth == VSOL_Newton)
+    pmgp->nrwk += 2 * (size_t)pmgp->nf;
+  PmgDev* d = new PmgDev();
+  thee->dev = d;
+  HIPCHECK(hipStreamCreateWithFlags(&d->s, hipStreamNonBlocking));
+  d->nx = pmgp->nx;
+  d->ny = pmgp->ny;
+  d->nz = pmgp->nz;
+  d->n = (size_t)d->nx * d->ny * d->nz;
+  const size_t n = d->n;
+  d->epsx = dalloc<double>(n);
+  d->epsy = dalloc<double>(n);
+  d->epsz = dalloc<double>(n);
+  d->kappa = dalloc<double>(n);
+  d->charge = dalloc<double>(n);
+  d->u = dalloc<double>(n);
+  d->pvec = dalloc<double>(n);
+  d->ccf = dalloc<double>(n);
+  pmg::fill<double>(d->s, d->u, n, 0.0);
+  d->gx = dalloc<double>(2 * (size_t)d->ny * d->nz);
+  d->gy = dalloc<double>(2 * (size_t)d->nx * d->nz);
+  d->gz = dalloc<double>(2 * (size_t)d->nx * d->ny);
+  d->xf = dalloc<double>(d->nx);
+  d->yf = dalloc<double>(d->ny);
+  d->zf = dalloc<double>(d->nz);
+  d->part = dalloc<double>(2 * pmg::kParts);
+  d->scal = dalloc<double>(32);
+  HIPCHECK(hipHostMalloc(&d->hscal, 32 * sizeof(double), hipHostMallocDefault));
+  Valist* al = pbe->alist;
+  d->natoms = al->number;
+  std::vector<double> pos(3 * (size_t)d->natoms), rad(d->natoms), chg(d->natoms);
+  for (int a = 0; a < d->natoms; a++) {
+    for (int k = 0; k < 3; k++) pos[3 * a + k] = al->atoms[a].position[k];
+    rad[a] = al->atoms[a].radius;
+    chg[a] = al->atoms[a].charge;
+  }
+  d->apos = dalloc<double>(3 * (size_t)d->natoms);
+  d->arad = dalloc<double>(d->natoms);
+  d->achg = dalloc<double>(d->natoms);
+  d->apart = dalloc<double>(d->natoms);
+  d->atmp = dalloc<double>(d->natoms);
+  HIPCHECK(hipMemcpyAsync(d->apos, pos.data(), pos.size() * sizeof(double), hipMemcpyHostToDevice, d->s));
+  HIPCHECK(hipMemcpyAsync(d->arad, rad.data(), rad.size() * sizeof(double), hipMemcpyHostToDevice, d->s));
+  HIPCHECK(hipMemcpyAsync(d->achg, chg.data(), chg.size() * sizeof(double), hipMemcpyHostToDevice, d->s));
+  d->hpart.assign(d->natoms, 1.0);
+  // host arrays callers dereference (vpmg.h:116-193)
+  thee->u = (double*)calloc(n, sizeof(double));
+  thee->pvec = (double*)calloc(n, sizeof(double));
+  thee->rwork = (double*)calloc(n, sizeof(double));
+  if (focusFlag && pmgOLD) {
+    if (pmgp->bcfl != BCFL_FOCUS) pmgp->bcfl = BCFL_FOCUS;
+    if (!focus_fill_bound(thee, pmgOLD)) {
+      fprintf(stderr, "Vpmg_ctor2: new mesh not contained in the old mesh\n");
+      return 0;
+    }
+    if (energyFlag != PCE_NO) {
+      double partMin[3], partMax[3];
+      for (int j = 0; j < 3; j++) {
+        if (mgparm->type == MCT_PARALLEL) {
+          partMin[j] = mgparm->partDisjCenter[j] - 0.5 * mgparm->partDisjLength[j];
+          partMax[j] = mgparm->partDisjCenter[j] + 0.5 * mgparm->partDisjLength[j];
+        } else {
+          partMin[j] = mgparm->center[j] - 0.5 * mgparm->glen[j];
+          partMax[j] = mgparm->center[j] + 0.5 * mgparm->glen[j];
+        }
+      }
+      ext_energy(thee, pmgOLD, energyFlag, partMin, partMax, mgparm->partDisjOwnSide);
+    }
+  } else {
+    thee->extQmEnergy = thee->extDiEnergy = thee->extQfEnergy = 0.0;
+  }
+  thee->chargeSrc = mgparm->chgs;
+  Vpmg_unsetPart(thee);
+  thee->filled = 0;
+  Vpmg_dtor(&pmgOLD);
+  return 1;
+}
+
+extern "C" void Vpmg_dtor2(Vpmg* thee) {
+  if (!thee) return;
+  PmgDev* d = DEV(thee);
+  if (d) {
+    HIPCHECK(hipStreamSynchronize(d->s));
+    free_hierarchy(d);
+    dfree(d->epsx);
+    dfree(d->epsy);
+    dfree(d->epsz);
+    dfree(d->kappa);
+    dfree(d->charge);
+    dfree(d->u);
+    dfree(d->pvec);
+    dfree(d->ccf);
+    dfree(d->gx);
+    dfree(d->gy);
+    dfree(d->gz);
+    dfree(d->xf);
+    dfree(d->yf);
+    dfree(d->zf);
+    dfree(d->apos);
+    dfree(d->arad);
+    dfree(d->achg);
+    dfree(d->apart);
+    dfree(d->atmp);
+    dfree(d->part);
+    dfree(d->scal);
+    dfree(d->fqf);
+    dfree(d->fib);
+    dfree(d->fdb);
+    dfree(d->frc);
+    if (d->hscal) HIPCHECK(hipHostFree(d->hscal));
+    HIPCHECK(hipStreamDestroy(d->s));
+    delete d;
+    thee->dev = nullptr;
+  }
+  free(thee->u);
+  free(thee->pvec);
+  free(thee->rwork);
+  free(thee->epsx);
+  free(thee->epsy);
+  free(thee->epsz);
+  free(thee->kappa);
+  free(thee->charge);
+  thee->u = thee->pvec = thee->rwork = nullptr;
+  thee->epsx = thee->epsy = thee->epsz = thee->kappa = thee->charge = nullptr;
+}
+
+extern "C" void Vpmg_dtor(Vpmg** thee) {
+  if (thee && *thee) {
+    Vpmg_dtor2(*thee);
+    free(*thee);
+    *thee = nullptr;
+  }
+}
+
+static pmg::Geo geo_of(Vpmg* thee) {
+  pmg::Geo g;
+  Vpmgp* p = thee->pmgp;
+  g.nx = p->nx;
+  g.ny = p->ny;
+  g.nz = p->nz;
+  g.hx = p->hx;
+  g.hy = p->hy;
+  g.hz = p->hzed;
+  g.xmin = p->xmin;
+  g.ymin = p->ymin;
+  g.zmin = p->zmin;
+  g.xmax = p->xmax;
+  g.ymax = p->ymax;
+  g.zmax = p->zmax;
+  return g;
+}
+
+// Vpmg_fillco (vpmg.c:5250-5415)
+extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
+                           Vchrg_Meth chargeMeth, int useDielXMap, Vgrid*, int useDielYMap,
+                           Vgrid*, int useDielZMap, Vgrid*, int useKappaMap, Vgrid*,
+                           int usePotMap, Vgrid*, int useChargeMap, Vgrid*) {
+  if (!thee) return 0;
+  if (useDielXMap || useDielYMap || useDielZMap || useKappaMap || usePotMap || useChargeMap) {
+    fprintf(stderr, "Vpmg_fillco: external maps are not supported\n");
+    return 0;
+  }
+  if (chargeMeth == VCM_BSPL4 || surfMeth == VSM_SPLINE3 || surfMeth == VSM_SPLINE4) {
+    fprintf(stderr, "Vpmg_fillco: chgm spl4 / srfm spl4 are not supported\n");
+    return 0;
+  }
+  const double t0 = now();
+  PmgDev* d = DEV(thee);
+  Vpmgp* p = thee->pmgp;
+  Vpbe* pbe = thee->pbe;
+  thee->surfMeth = surfMeth;
+  thee->splineWin = splineWin;
+  thee->chargeMeth = chargeMeth;
+  const double ionstr = pbe->bulkIonicStrength, epsw = pbe->solventDiel, epsp = pbe->soluteDiel;
+  p->xmin = p->xcent - (p->xlen / 2.0);
+  p->ymin = p->ycent - (p->ylen / 2.0);
+  p->zmin = p->zcent - (p->zlen / 2.0);
+  p->xmax = p->xcent + (p->xlen / 2.0);
+  p->ymax = p->ycent + (p->ylen / 2.0);
+  p->zmax = p->zcent + (p->zlen / 2.0);
+  const int islap = (ionstr < VPMGSMALL) && (fabs(epsp - epsw) < VPMGSMALL);
+  std::vector<double> xf(d->nx), yf(d->ny), zf(d->nz);
+  for (int i = 0; i < d->nx; i++) xf[i] = p->xmin + i * p->hx;
+  for (int i = 0; i < d->ny; i++) yf[i] = p->ymin + i * p->hy;
+  for (int i = 0; i < d->nz; i++) zf[i] = p->zmin + i * p->hzed;
+  HIPCHECK(hipMemcpyAsync(d->xf, xf.data(), d->nx * sizeof(double), hipMemcpyHostToDevice, d->s));
+  HIPCHECK(hipMemcpyAsync(d->yf, yf.data(), d->ny * sizeof(double), hipMemcpyHostToDevice, d->s));
+  HIPCHECK(hipMemcpyAsync(d->zf, zf.data(), d->nz * sizeof(double), hipMemcpyHostToDevice, d->s));
+  const pmg::Geo g = geo_of(thee);
+  d->geo = g;
+  const pmg::AtomsDev A = atoms_dev(d);
+  Valist* al = pbe->alist;
+  std::vector<double> hpos(3 * (size_t)d->natoms), hrad(d->natoms);
+  double rmax = 0;
+  for (int a = 0; a < d->natoms; a++) {
+    for (int k = 0; k < 3; k++) hpos[3 * a + k] = al->atoms[a].position[k];
+    hrad[a] = al->atoms[a].radius;
+    rmax = std::max(rmax, hrad[a]);
+  }
+  const double hmax = std::max(std::max(p->hx, p->hy), p->hzed);
+  // charges: every atom reaches at most 2 grid spacings per axis
+  {
+    DevCells cells = build_cells(d->s, hpos.data(), d->natoms, 2.0 * hmax + 1e-9);
+    pmg::charge_map(d->s, g, A, cells.c, chargeMeth == VCM_BSPL2, pbe->zmagic, 0, d->charge);
+    HIPCHECK(hipStreamSynchronize(d->s));
+    cells.release();
+  }
+  if (!islap) {
+    const double irad = pbe->maxIonRadius;
+    const double ionmask = (ionstr > VPMGSMALL) ? 1.0 : 0.0;
+    if (surfMeth == VSM_SPLINE) {
+      const double reach = std::max(irad + rmax + splineWin, rmax + splineWin) + 1.5 * hmax;
+      DevCells cells = build_cells(d->s, hpos.data(), d->natoms, reach);
+      pmg::coef_spline(d->s, g, A, cells.c, splineWin, irad, ionmask, epsw, epsp, d->kappa,
+                       d->epsx, d->epsy, d->epsz);
+      HIPCHECK(hipStreamSynchronize(d->s));
+      cells.release();
+    } else {
+      // fillcoCoefMolDiel (vpmg.c:4335-4344) then fillcoCoefMolIon
+      const double srad = pbe->solventRadius;
+      double* maps[3] = {d->epsx, d->epsy, d->epsz};
+      for (int w = 0; w < 3; w++) {
+        pmg::fill<double>(d->s, maps[w], d->n, epsw);
+        pmg::mark_spheres(d->s, g, d->natoms, d->apos, d->arad, srad, 0, w, epsp, maps[w]);
+      }
+      if (srad > VSMALL) {
+        double maxrad = rmax + pbe->clistMaxRadius;
+        int nsphere = (int)ceil(4.0 * VPI * maxrad * maxrad * pbe->sdens);
+        std::vector<double> ref = ref_sphere(nsphere);
+        const int nref = (int)(ref.size() / 3);
+        double* dref = dalloc<double>(ref.size());
+        HIPCHECK(hipMemcpyAsync(dref, ref.data(), ref.size() * sizeof(double),
+                                hipMemcpyHostToDevice, d->s));
+        DevCells cells = build_cells(d->s, hpos.data(), d->natoms, rmax + srad);
+        const size_t tot = (size_t)d->natoms * nref;
+        unsigned char* flags = dalloc<unsigned char>(tot);
+        pmg::sas_flags(d->s, A, cells.c, dref, nref, srad, flags);
+        double* sas = dalloc<double>(3 * tot);
+        int* counter = dalloc<int>(1);
+        int nsas = pmg::sas_compact(d->s, A, dref, nref, srad, flags, sas, counter, (int)tot);
+        for (int w = 0; w < 3; w++)
+          pmg::mark_spheres(d->s, g, nsas, sas, nullptr, srad, 2, w, epsw, maps[w]);
+        HIPCHECK(hipStreamSynchronize(d->s));
+        dfree(flags);
+        dfree(sas);
+        dfree(counter);
+        dfree(dref);
+        cells.release();
+      }
+      if (surfMeth == VSM_MOLSMOOTH) {
+        double* a1 = dalloc<double>(d->n);
+        double* a2 = dalloc<double>(d->n);
+        double* a3 = dalloc<double>(d->n);
+        HIPCHECK(hipMemcpyAsync(a1, d->epsx, d->n * sizeof(double), hipMemcpyDeviceToDevice, d->s));
+        HIPCHECK(hipMemcpyAsync(a2, d->epsy, d->n * sizeof(double), hipMemcpyDeviceToDevice, d->s));
+        HIPCHECK(hipMemcpyAsync(a3, d->epsz, d->n * sizeof(double), hipMemcpyDeviceToDevice, d->s));
+        pmg::mol_smooth(d->s, g, a1, a2, a3, d->epsx, d->epsy, d->epsz);
+        HIPCHECK(hipStreamSynchronize(d->s));
+        dfree(a1);
+        dfree(a2);
+        dfree(a3);
+      }
+      pmg::fill<double>(d->s, d->kappa, d->n, ionmask);
+      if (!(ionstr < VPMGSMALL))
+        pmg::mark_spheres(d->s, g, d->natoms, d->apos, d->arad, irad, 1, 3, 0.0, d->kappa);
+    }
+  } else {
+    pmg::fill<double>(d->s, d->kappa, d->n, 0.0);
+    pmg::fill<double>(d->s, d->epsx, d->n, epsp);
+    pmg::fill<double>(d->s, d->epsy, d->n, epsp);
+    pmg::fill<double>(d->s, d->epsz, d->n, epsp);
+  }
+  if (p->bcfl != BCFL_FOCUS) {
+    pmg::fill<double>(d->s, d->gx, 2 * (size_t)d->ny * d->nz, 0.0);
+    pmg::fill<double>(d->s, d->gy, 2 * (size_t)d->nx * d->nz, 0.0);
+    pmg::fill<double>(d->s, d->gz, 2 * (size_t)d->nx * d->ny, 0.0);
+    if (p->bcfl == BCFL_MDH) {
+      const double pre1 =
+          ((Vunit_ec) / (4 * VPI * Vunit_eps0 * epsw * Vunit_kb * pbe->T)) * (1.0e10);
+      pmg::bc_mdh(d->s, g, d->xf, d->yf, d->zf, A, pre1, pbe->xkappa, d->gx, d->gy, d->gz);
+    } else if (p->bcfl != BCFL_ZERO) {
+      fprintf(stderr, "Vpmg_fillco: bcfl %d is not supported (zero, mdh, focus are)\n",
+              (int)p->bcfl);
+      return 0;
+    }
+  }
+  HIPCHECK(hipStreamSynchronize(d->s));
+  thee->filled = 1;
+  d->forces_valid = false;
+  d->stats[5] = now() - t0;
+  return 1;
+}
+
+// Vpmg_solve (vpmg.c:401-570)
+extern "C" int Vpmg_solve(Vpmg* thee) {
+  if (!thee || !thee->filled) return 0;
+  PmgDev* d = DEV(thee);
+  Vpmgp* p = thee->pmgp;
+  const double t0 = now();
+  const bool newton = (p->meth == VSOL_Newton);
+  if (!newton && p->meth != VSOL_MG) {
+    fprintf(stderr, "Vpmg_solve: solver method %d is not supported\n", p->meth);
+    return 0;
+  }
+  const double zkappa2 = thee->pbe->zkappa2;
+  // ccf = zkappa2 * kappa, or 0 (vpmg.c:437-447)
+  if (zkappa2 > VPMGSMALL)
+    pmg::scale(d->s, d->n, zkappa2, d->kappa, d->ccf);
+  else
+    pmg::fill<double>(d->s, d->ccf, d->n, 0.0);
+  alloc_hierarchy(d, p->nlev, newton, d->precision == 1);
+  const double tb = now();
+  build_operators(d);
+  pmg::fill<double>(d->s, d->u, d->n, 0.0);
+  double relres = 0;
+  int iters = 0, inner = 0, outer = 0;
+  if (newton) {
+    newton_solve(thee, &relres, &iters, &inner);
+  } else {
+    set_diagonals(d, false);
+    double* X[24];
+    for (int l = 0; l < d->ntot; l++) X[l] = d->L[l].x;
+    HIPCHECK(hipStreamSynchronize(d->s));
+    d->stats[6] = now() - tb;
+    if (d->precision == 1) {
+      sync_fp32(d);
+      const double tol = d->mixed_tol > 0 ? d->mixed_tol : p->errtol;
+      iters = mixed_solve(d, X, d->L[0].fc, 1, p->itmax * 4, tol, &relres, &outer);
+    } else {
+      iters = mvcs64(d, X, d->L[0].fc, 1, p->itmax, p->errtol, &relres);
+    }
+  }
+  pmg::set_faces(d->s, d->nx, d->ny, d->nz, d->u, d->gx, d->gy, d->gz);
+  if (d->mirror)
+    HIPCHECK(hipMemcpyAsync(thee->u, d->u, d->n * sizeof(double), hipMemcpyDeviceToHost, d->s));
+  HIPCHECK(hipStreamSynchronize(d->s));
+  d->stats[0] = iters;
+  d->stats[1] = inner;
+  d->stats[2] = d->ntot;
+  d->stats[3] = relres;
+  d->stats[4] = now() - t0;
+  d->stats[7] = outer;
+  d->forces_valid = false;
+  return 1;
+}
+
+// ================================================================ focusing
+
+// focusFillBound (vpmg.c:1914-2414): Dirichlet faces from the old solution
+static int focus_fill_bound(Vpmg* thee, Vpmg* old) {
+  PmgDev* d = DEV(thee);
+  PmgDev* o = DEV(old);
+  Vpmgp* p = thee->pmgp;
+  Vpmgp* q = old->pmgp;
+  double xminN = p->xcent - ((double)(p->nx - 1) * p->hx) / 2.0;
+  double xmaxN = p->xcent + ((double)(p->nx - 1) * p->hx) / 2.0;
+  double yminN = p->ycent - ((double)(p->ny - 1) * p->hy) / 2.0;
+  double ymaxN = p->ycent + ((double)(p->ny - 1) * p->hy) / 2.0;
+  double zminN = p->zcent - ((double)(p->nz - 1) * p->hzed) / 2.0;
+  double zmaxN = p->zcent + ((double)(p->nz - 1) * p->hzed) / 2.0;
+  pmg::OldGrid og;
+  og.nx = q->nx;
+  og.ny = q->ny;
+  og.nz = q->nz;
+  og.hx = q->hx;
+  og.hy = q->hy;
+  og.hz = q->hzed;
+  og.xmin = q->xcent - ((double)(q->nx - 1) * q->hx) / 2.0;
+  og.xmax = q->xcent + ((double)(q->nx - 1) * q->hx) / 2.0;
+  og.ymin = q->ycent - ((double)(q->ny - 1) * q->hy) / 2.0;
+  og.ymax = q->ycent + ((double)(q->ny - 1) * q->hy) / 2.0;
+  og.zmin = q->zcent - ((double)(q->nz - 1) * q->hzed) / 2.0;
+  og.zmax = q->zcent + ((double)(q->nz - 1) * q->hzed) / 2.0;
+  og.u = o->u;
+  if (fabs(og.xmin - xminN) < VSMALL) xminN = og.xmin;
+  if (fabs(og.xmax - xmaxN) < VSMALL) xmaxN = og.xmax;
+  if (fabs(og.ymin - yminN) < VSMALL) yminN = og.ymin;
+  if (fabs(og.ymax - ymaxN) < VSMALL) ymaxN = og.ymax;
+  if (fabs(og.zmin - zminN) < VSMALL) zminN = og.zmin;
+  if (fabs(og.zmax - zmaxN) < VSMALL) zmaxN = og.zmax;
+  if ((xmaxN > og.xmax) || (ymaxN > og.ymax) || (zmaxN > og.zmax) || (og.xmin > xminN) ||
+      (og.ymin > yminN) || (og.zmin > zminN))
+    return 0;
+  HIPCHECK(hipStreamSynchronize(o->s));
+  pmg::Geo g{};
+  g.nx = p->nx;
+  g.ny = p->ny;
+  g.nz = p->nz;
+  g.hx = p->hx;
+  g.hy = p->hy;
+  g.hz = p->hzed;
+  int* bad = dalloc<int>(1);
+  HIPCHECK(hipMemsetAsync(bad, 0, sizeof(int), d->s));
+  pmg::focus_bound(d->s, g, xminN, yminN, zminN, xmaxN, ymaxN, zmaxN, og, d->gx, d->gy, d->gz, bad);
+  int hbad = 0;
+  HIPCHECK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, d->s));
+  HIPCHECK(hipStreamSynchronize(d->s));
+  dfree(bad);
+  return hbad ? 0 : 1;
+}
+
+// Vpmg_setPart (vpmg.c:627-870), atom part on the host, grid part on the
+// device when no partition side is shared (sequential focusing), else on the
+// host with the reference's arithmetic.
+static double part_axis_h(double x, double lo, double hi, double h, int bl, int bh) {
+  double ok;
+  if ((x < (hi - h / 2)) && (x > (lo + h / 2))) ok = 1.0;
+  else if ((fabs(x - lo) < VPMGSMALL) && (bl == 0)) ok = 1.0;
+  else if ((fabs(x - lo) < VPMGSMALL) && (bl == 1)) ok = 0.5;
+  else if ((fabs(x - hi) < VPMGSMALL) && (bh == 0)) ok = 1.0;
+  else if ((fabs(x - hi) < VPMGSMALL) && (bh == 1)) ok = 0.5;
+  else if ((x > (hi + h / 2)) || (x < (lo - h / 2))) ok = 0.0;
+  else {
+    double x0 = std::max(x - h / 2, lo), x1 = std::min(x + h / 2, hi);
+    ok = fabs(x1 - x0) / h;
+    if (ok < 0.0 && fabs(ok) < VPMGSMALL) ok = 0.0;
+    if (ok > 1.0 && fabs(ok - 1.0) < VPMGSMALL) ok = 1.0;
+  }
+  return ok;
+}
+
+static double atom_axis_h(double x, double lo, double hi, int bl, int bh) {
+  if ((x < hi) && (x > lo)) return 1;
+  if ((fabs(x - lo) < VPMGSMALL) && (bl == 0)) return 1;
+  if ((fabs(x - lo) < VPMGSMALL) && (bl == 1)) return 0.5;
+  if ((fabs(x - hi) < VPMGSMALL) && (bh == 0)) return 1;
+  if ((fabs(x - hi) < VPMGSMALL) && (bh == 1)) return 0.5;
+  return 0;
+}
+
+// bflags order of src/generic/vhal.h VAPBS_{RIGHT,FRONT,UP,LEFT,BACK,DOWN} = 0..5
+static void set_part_impl(Vpmg* thee, const double lo[3], const double hi[3], const int bflags[6],
+                          int invert) {
+  PmgDev* d = DEV(thee);
+  Vpmgp* p = thee->pmgp;
+  const int bR = bflags[0], bF = bflags[1], bU = bflags[2], bL = bflags[3], bB = bflags[4],
+            bD = bflags[5];
+  Valist* al = thee->pbe->alist;
+  for (int a = 0; a < d->natoms; a++) {
+    const double* x = al->atoms[a].position;
+    double v = atom_axis_h(x[0], lo[0], hi[0], bL, bR) * atom_axis_h(x[1], lo[1], hi[1], bB, bF) *
+               atom_axis_h(x[2], lo[2], hi[2], bD, bU);
+    d->hpart[a] = v;
+    al->atoms[a].partID = v;
+  }
+  const double xm = p->xcent - 0.5 * p->hx * (p->nx - 1);
+  const double ym = p->ycent - 0.5 * p->hy * (p->ny - 1);
+  const double zm = p->zcent - 0.5 * p->hzed * (p->nz - 1);
+  const bool shared = bR || bF || bU || bL || bB || bD;
+  if (!shared) {
+    pmg::Geo g{};
+    g.nx = p->nx;
+    g.ny = p->ny;
+    g.nz = p->nz;
+    g.hx = p->hx;
+    g.hy = p->hy;
+    g.hz = p->hzed;
+    pmg::set_part(d->s, g, xm, ym, zm, lo, hi, invert, d->pvec);
+  } else {
+    std::vector<double> pv(d->n);
+    for (int i = 0; i < p->nx; i++) {
+      double xok = part_axis_h(i * p->hx + xm, lo[0], hi[0], p->hx, bL, bR);
+      for (int j = 0; j < p->ny; j++) {
+        double yok = part_axis_h(j * p->hy + ym, lo[1], hi[1], p->hy, bB, bF);
+        for (int k = 0; k < p->nz; k++) {
+          double zok = part_axis_h(k * p->hzed + zm, lo[2], hi[2], p->hzed, bD, bU);
+          double v = xok * yok * zok;
+          pv[((size_t)k * p->ny + j) * p->nx + i] = (fabs(v) < VPMGSMALL) ? 0.0 : v;
+        }
+      }
+    }
+    HIPCHECK(hipMemcpyAsync(d->pvec, pv.data(), d->n * sizeof(double), hipMemcpyHostToDevice, d->s));
+  }
+  if (invert) {
+    for (int a = 0; a < d->natoms; a++) {
+      double v = d->hpart[a];
+      if (v > VSMALL) v = 1.0;
+      v = (1 - v) * 1.0;
+      d->hpart[a] = v;
+      al->atoms[a].partID = v;
+    }
+  }
+  upload_part(d);
+  HIPCHECK(hipMemcpyAsync(thee->pvec, d->pvec, d->n * sizeof(double), hipMemcpyDeviceToHost, d->s));
+  HIPCHECK(hipStreamSynchronize(d->s));
+  d->forces_valid = false;
+}
+
+extern "C" void Vpmg_setPart(Vpmg* thee, double lowerCorner[3], double upperCorner[3],
+                             int bflags[6]) {
+  set_part_impl(thee, lowerCorner, upperCorner, bflags, 0);
+}
+
+extern "C" void Vpmg_unsetPart(Vpmg* thee) {
+  PmgDev* d = DEV(thee);
+  pmg::fill<double>(d->s, d->pvec, d->n, 1.0);
+  Valist* al = thee->pbe->alist;
+  for (int a = 0; a < d->natoms; a++) {
+    d->hpart[a] = 1.0;
+    al->atoms[a].partID = 1.0;
+  }
+  upload_part(d);
+  for (size_t i = 0; i < d->n; i++) thee->pvec[i] = 1.0;
+  HIPCHECK(hipStreamSynchronize(d->s));
+  d->forces_valid = false;
+}
+
+// extEnergy (vpmg.c:2416-2545): energies of the old solve on the part of its
+// domain outside the new one.
+static void ext_energy(Vpmg* thee, Vpmg* old, PBEparm_calcEnergy, double* partMin,
+                       double* partMax, int* bflags) {
+  Vpmgp* p = thee->pmgp;
+  double lo[3], hi[3];
+  lo[0] = p->xcent - ((double)(p->nx - 1) * p->hx) / 2.0;
+  hi[0] = p->xcent + ((double)(p->nx - 1) * p->hx) / 2.0;
+  lo[1] = p->ycent - ((double)(p->ny - 1) * p->hy) / 2.0;
+  hi[1] = p->ycent + ((double)(p->ny - 1) * p->hy) / 2.0;
+  lo[2] = p->zcent - ((double)(p->nz - 1) * p->hzed) / 2.0;
+  hi[2] = p->zcent + ((double)(p->nz - 1) * p->hzed) / 2.0;
+  const bool shared = bflags[0] || bflags[1] || bflags[2] || bflags[3] || bflags[4] || bflags[5];
+  if (shared) {
+    fprintf(stderr, "extEnergy: parallel-focusing partitions are not supported\n");
+  }
+  (void)partMin;
+  (void)partMax;
+  set_part_impl(old, lo, hi, bflags, 1);
+  thee->extQmEnergy = Vpmg_qmEnergy(old, 1);
+  thee->extQfEnergy = Vpmg_qfEnergy(old, 1);
+  thee->extDiEnergy = Vpmg_dielEnergy(old, 1);
+  Vpmg_unsetPart(old);
+}
+
+// ================================================================ energies
+
+extern "C" double Vpmg_qfEnergy(Vpmg* thee, int extFlag) {
+  PmgDev* d = DEV(thee);
+  Vpmgp* p = thee->pmgp;
+  double e;
+  if (thee->chargeMeth == VCM_BSPL2) {  // Vpmg_qfEnergyVolume (vpmg.c:1861-1891)
+    pmg::energy_qf_volume(d->s, d->n, d->pvec, d->u, d->charge, d->part, d->scal);
+    e = read_scalar(d, d->scal);
+    e = e * p->hx * p->hy * p->hzed / thee->pbe->zmagic;
+    if (extFlag == 1) e += thee->extQfEnergy;
+  } else {  // Vpmg_qfEnergyPoint (vpmg.c:1704-1789)
+    pmg::energy_qf_point(d->s, d->geo, atoms_dev(d), d->u, d->atmp, d->scal);
+    e = read_scalar(d, d->scal);
+    if (extFlag) e += thee->extQfEnergy;
+  }
+  return e;
+}
+
+extern "C" double Vpmg_dielEnergy(Vpmg* thee, int extFlag) {
+  PmgDev* d = DEV(thee);
+  Vpmgp* p = thee->pmgp;
+  if (!thee->filled) return 0.0;
+  pmg::energy_diel(d->s, d->geo, d->pvec, d->u, d->epsx, d->epsy, d->epsz, d->part, d->scal);
+  double e = read_scalar(d, d->scal);
+  e = 0.5 * e * p->hx * p->hy * p->hzed;
+  e = e / thee->pbe->zmagic;
+  if (extFlag == 1) e += thee->extDiEnergy;
+  return e;
+}
+
+extern "C" double Vpmg_qmEnergy(Vpmg* thee, int extFlag) {
+  PmgDev* d = DEV(thee);
+  Vpmgp* p = thee->pmgp;
+  Vpbe* pbe = thee->pbe;
+  if (pbe->zkappa2 < VSMALL) return 0.0;
+  const double zks2 = 0.5 * pbe->zkappa2 / pbe->bulkIonicStrength;
+  pmg::IonBulk I{};
+  I.nion = pbe->numIon;
+  for (int i = 0; i < I.nion; i++) {
+    I.q[i] = pbe->ionQ[i];
+    I.c[i] = pbe->ionConc[i];
+  }
+  int* nchop = dalloc<int>(1);
+  pmg::energy_qm(d->s, d->n, p->nonlin ? 1 : 0, d->pvec, d->kappa, d->u, zks2, pbe->zkappa2, I,
+                 d->part, d->scal, nchop);
+  double e = read_scalar(d, d->scal);
+  int hchop = 0;
+  HIPCHECK(hipMemcpy(&hchop, nchop, sizeof(int), hipMemcpyDeviceToHost));
+  dfree(nchop);
+  if (p->nonlin && hchop > 0) {
+    // the reference aborts here (vpmg.c:1468-1472)
+    fprintf(stderr, "Vpmg_qmEnergy: chopped EXP %d times; large potential values\n", hchop);
+    return NAN;
+  }
+  if (!p->nonlin) e = 0.5 * e;
+  e = e * p->hx * p->hy * p->hzed;
+  e = e / pbe->zmagic;
+  if (extFlag == 1) e += thee->extQmEnergy;
+  return e;
+}
+
+extern "C" double Vpmg_energy(Vpmg* thee, int extFlag) {
+  if ((thee->pmgp->nonlin) && (thee->pbe->bulkIonicStrength > 0.)) {
+    double qm = Vpmg_qmEnergy(thee, extFlag);
+    double qf = Vpmg_qfEnergy(thee, extFlag);
+    double di = Vpmg_dielEnergy(thee, extFlag);
+    return qf - di - qm;
+  }
+  return 0.5 * Vpmg_qfEnergy(thee, extFlag);
+}
+
+// Vpmg_qfAtomEnergy (vpmg.c:1791-1859): trilinear u at the atom
+extern "C" double Vpmg_qfAtomEnergy(Vpmg* thee, Vatom* atom) {
+  PmgDev* d = DEV(thee);
+  Vpmgp* p = thee->pmgp;
+  if (!thee->u) return 0.0;
+  if (!d->mirror)
+    HIPCHECK(hipMemcpy(thee->u, d->u, d->n * sizeof(double), hipMemcpyDeviceToHost));
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  const double xmin = p->xmin, ymin = p->ymin, zmin = p->zmin;
+  const double fi = (atom->position[0] - xmin) / p->hx, fj = (atom->position[1] - ymin) / p->hy,
+               fk = (atom->position[2] - zmin) / p->hzed;
+  const int ihi = (int)ceil(fi), ilo = (int)floor(fi), jhi = (int)ceil(fj), jlo = (int)floor(fj),
+            khi = (int)ceil(fk), klo = (int)floor(fk);
+  if (!(atom->partID > 0)) return 0.0;
+  if (!((ihi < nx) && (jhi < ny) && (khi < nz) && (ilo >= 0) && (jlo >= 0) && (klo >= 0))) return 0.0;
+  const double dx = fi - ilo, dy = fj - jlo, dz = fk - klo;
+  auto U = [&](int i, int j, int k) { return thee->u[((size_t)k * ny + j) * nx + i]; };
+  const double uval = dx * dy * dz * U(ihi, jhi, khi) + dx * (1.0 - dy) * dz * U(ihi, jlo, khi) +
+                      dx * dy * (1.0 - dz) * U(ihi, jhi, klo) +
+                      dx * (1.0 - dy) * (1.0 - dz) * U(ihi, jlo, klo) +
+                      (1.0 - dx) * dy * dz * U(ilo, jhi, khi) +
+                      (1.0 - dx) * (1.0 - dy) * dz * U(ilo, jlo, khi) +
+                      (1.0 - dx) * dy * (1.0 - dz) * U(ilo, jhi, klo) +
+                      (1.0 - dx) * (1.0 - dy) * (1.0 - dz) * U(ilo, jlo, klo);
+  return uval * atom->charge * atom->partID;
+}
+
+// ================================================================== forces
+
+namespace {
+void ensure_forces(Vpmg* thee) {
+  PmgDev* d = DEV(thee);
+  if (d->forces_valid) return;
+  const int na = d->natoms;
+  if (!d->fqf) {
+    d->fqf = dalloc<double>(3 * (size_t)na);
+    d->fib = dalloc<double>(3 * (size_t)na);
+    d->fdb = dalloc<double>(3 * (size_t)na);
+    d->frc = dalloc<int>(na);
+  }
+  Vpbe* pbe = thee->pbe;
+  const pmg::AtomsDev A = atoms_dev(d);
+  pmg::force_qf(d->s, d->geo, A, thee->chargeMeth == VCM_BSPL2, d->u, d->fqf);
+  pmg::IonBulk I{};
+  I.nion = pbe->numIon;
+  for (int i = 0; i < I.nion; i++) {
+    I.q[i] = pbe->ionQ[i];
+    I.c[i] = pbe->ionConc[i];
+  }
+  pmg::force_ib(d->s, d->geo, A, thee->splineWin, pbe->maxIonRadius, pbe->zkappa2,
+                1.0 / pbe->zmagic, thee->pmgp->nonlin ? 1 : 0, pbe->bulkIonicStrength, I,
+                d->kappa, d->u, d->fib);
+  pmg::force_db(d->s, d->geo, A, thee->splineWin, pbe->solventRadius, pbe->soluteDiel,
+                pbe->solventDiel, 1.0 / pbe->zmagic, d->epsx, d->epsy, d->epsz, d->u, d->fdb,
+                d->frc);
+  d->hqf.resize(3 * (size_t)na);
+  d->hib.resize(3 * (size_t)na);
+  d->hdb.resize(3 * (size_t)na);
+  d->hrc.resize(na);
+  HIPCHECK(hipMemcpyAsync(d->hqf.data(), d->fqf, 3 * na * sizeof(double), hipMemcpyDeviceToHost, d->s));
+  HIPCHECK(hipMemcpyAsync(d->hib.data(), d->fib, 3 * na * sizeof(double), hipMemcpyDeviceToHost, d->s));
+  HIPCHECK(hipMemcpyAsync(d->hdb.data(), d->fdb, 3 * na * sizeof(double), hipMemcpyDeviceToHost, d->s));
+  HIPCHECK(hipMemcpyAsync(d->hrc.data(), d->frc, na * sizeof(int), hipMemcpyDeviceToHost, d->s));
+  HIPCHECK(hipStreamSynchronize(d->s));
+  d->forces_valid = true;
+}
+}  // namespace
+
+extern "C" int Vpmg_qfForce(Vpmg* thee, double* force, int atomID, Vchrg_Meth chgm) {
+  force[0] = force[1] = force[2] = 0.0;
+  if (chgm == VCM_BSPL4) return 0;
+  PmgDev* d = DEV(thee);
+  if (atomID < 0 || atomID >= d->natoms) return 0;
+  ensure_forces(thee);
+  for (int k = 0; k < 3; k++) force[k] = d->hqf[3 * atomID + k];
+  return 1;
+}
+
+static bool spline_srfm(Vsurf_Meth s) {
+  return s == VSM_SPLINE || s == VSM_SPLINE3 || s == VSM_SPLINE4;
+}
+
+extern "C" int Vpmg_ibForce(Vpmg* thee, double* force, int atomID, Vsurf_Meth srfm) {
+  force[0] = force[1] = force[2] = 0.0;
+  if (!spline_srfm(srfm)) return 0;  // vpmg.c:5471-5477
+  PmgDev* d = DEV(thee);
+  if (atomID < 0 || atomID >= d->natoms) return 0;
+  ensure_forces(thee);
+  for (int k = 0; k < 3; k++) force[k] = d->hib[3 * atomID + k];
+  return 1;
+}
+
+extern "C" int Vpmg_dbForce(Vpmg* thee, double* dbForce, int atomID, Vsurf_Meth srfm) {
+  dbForce[0] = dbForce[1] = dbForce[2] = 0.0;
+  if (!thee->filled) return 0;
+  if (!spline_srfm(srfm)) return 0;  // vpmg.c:5638-5644
+  PmgDev* d = DEV(thee);
+  if (atomID < 0 || atomID >= d->natoms) return 0;
+  ensure_forces(thee);
+  for (int k = 0; k < 3; k++) dbForce[k] = d->hdb[3 * atomID + k];
+  return d->hrc[atomID];
+}
+
+// Vpmg_force (vpmg.c:5417-5438)
+extern "C" int Vpmg_force(Vpmg* thee, double* force, int atomID, Vsurf_Meth srfm,
+                          Vchrg_Meth chgm) {
+  double qf[3] = {0, 0, 0}, db[3] = {0, 0, 0}, ib[3] = {0, 0, 0};
+  int rc = 1;
+  rc = rc && Vpmg_dbForce(thee, db, atomID, srfm);
+  rc = rc && Vpmg_ibForce(thee, ib, atomID, srfm);
+  rc = rc && Vpmg_qfForce(thee, qf, atomID, chgm);
+  for (int k = 0; k < 3; k++) force[k] = qf[k] + db[k] + ib[k];
+  return rc;
+}
+
+// ================================================================== arrays
+
+extern "C" int Vpmg_copyArray(Vpmg* thee, int which, double* out, size_t n) {
+  PmgDev* d = DEV(thee);
+  const double* src = nullptr;
+  size_t need = d->n;
+  switch (which) {
+    case 0: src = d->u; break;
+    case 1: src = d->epsx; break;
+    case 2: src = d->epsy; break;
+    case 3: src = d->epsz; break;
+    case 4: src = d->kappa; break;
+    case 5: src = d->charge; break;
+    case 6: src = d->pvec; break;
+    case 7: src = d->gx; need = 2 * (size_t)d->ny * d->nz; break;
+    case 8: src = d->gy; need = 2 * (size_t)d->nx * d->nz; break;
+    case 9: src = d->gz; need = 2 * (size_t)d->nx * d->ny; break;
+    default: return 0;
+  }
+  if (n < need) return 0;
+  HIPCHECK(hipMemcpyAsync(out, src, need * sizeof(double), hipMemcpyDeviceToHost, d->s));
+  HIPCHECK(hipStreamSynchronize(d->s));
+  return 1;
+}
+
+extern "C" int Vpmg_fillArray(Vpmg* thee, double* vec, Vdata_Type type, double, Vhal_PBEType,
+                              void*) {
+  PmgDev* d = DEV(thee);
+  if (!thee->filled) return 0;
+  switch (type) {
+    case VDT_CHARGE: {
+      if (!Vpmg_copyArray(thee, 5, vec, d->n)) return 0;
+      const double zm = thee->pbe->zmagic;
+      for (size_t i = 0; i < d->n; i++) vec[i] = vec[i] / zm;
+      return 1;
+    }
+    case VDT_DIELX: return Vpmg_copyArray(thee, 1, vec, d->n);
+    case VDT_DIELY: return Vpmg_copyArray(thee, 2, vec, d->n);
+    case VDT_DIELZ: return Vpmg_copyArray(thee, 3, vec, d->n);
+    case VDT_KAPPA: return Vpmg_copyArray(thee, 4, vec, d->n);
+    case VDT_POT: return Vpmg_copyArray(thee, 0, vec, d->n);
+    default:
+      fprintf(stderr, "Vpmg_fillArray: data type %d is not supported\n", (int)type);
+      return 0;
+  }
+}
+
+extern "C" unsigned long int Vpmg_memChk(Vpmg* thee) {
+  if (!thee || !thee->dev) return 0;
+  PmgDev* d = DEV(thee);
+  size_t bytes = 10 * d->n * sizeof(double);
+  for (int l = 0; l < d->ntot; l++)
+    bytes += (size_t)(d->L[l].st == 7 ? 8 : 18) * d->L[l].n * sizeof(double);
+  return (unsigned long)bytes;
+}
+
+// ============================================================= extensions
+
+extern "C" int Vpmg_setPrecision(Vpmg* thee, int mode, double tol) {
+  if (!thee || (mode != 0 && mode != 1)) return 0;
+  PmgDev* d = DEV(thee);
+  d->precision = mode;
+  d->mixed_tol = tol;
+  return 1;
+}
+
+extern "C" int Vpmg_setHostMirror(Vpmg* thee, int on) {
+  if (!thee) return 0;
+  DEV(thee)->mirror = on ? 1 : 0;
+  return 1;
+}
+
+extern "C" int Vpmg_getStats(Vpmg* thee, double* out, int n) {
+  if (!thee) return 0;
+  PmgDev* d = DEV(thee);
+  int m = std::min(n, 8);
+  for (int i = 0; i < m; i++) out[i] = d->stats[i];
+  return m;
+}
+
+extern "C" const char* apbs_pmg_version(void) { return "apbs_pmg 0.1 (gfx950)"; }

@@ -1,0 +1,281 @@
+/*
+ * apbs_pmg.h -- C ABI of the MI355X PMG Poisson-Boltzmann solver
+ * (libapbs_pmg.so).
+ *
+ * The entry points keep the names, argument order, argument meaning and
+ * return conventions of APBS's Vpmg / Vpmgp API (src/mg/vpmg.h,
+ * src/mg/vpmgp.h of Electrostatics/apbs 3.4.1), so the pmgc path of an APBS
+ * caller (src/routines.c initMG/solveMG/energyMG/forceMG) binds to this
+ * library instead.  Each declaration cites the reference interface it
+ * replaces.  Plain pointers and sizes only; all device memory is owned by the
+ * library.  Return codes: 1 = success, 0 = failure, as in the reference.
+ *
+ * The physics objects the Vpmg constructor borrows (Valist, Vpbe, MGparm)
+ * are this library's own types with the fields the PMG path reads; their
+ * constructors keep the reference signatures where the reference's own
+ * arguments are plain data.
+ */
+#ifndef APBS_PMG_H
+#define APBS_PMG_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ enums
+ * Values follow src/generic/vhal.h:83-295, src/generic/mgparm.h:77-83 and
+ * src/generic/pbeparm.h:82-100. */
+typedef enum { VSOL_CGMG = 0, VSOL_Newton = 1, VSOL_MG = 2, VSOL_CG = 3, VSOL_SOR = 4,
+               VSOL_RBGS = 5, VSOL_WJ = 6, VSOL_Richardson = 7, VSOL_CGMGAqua = 8,
+               VSOL_NewtonAqua = 9 } Vsol_Meth;
+typedef enum { VSM_MOL = 0, VSM_MOLSMOOTH = 1, VSM_SPLINE = 2, VSM_SPLINE3 = 3,
+               VSM_SPLINE4 = 4 } Vsurf_Meth;
+typedef enum { PBE_LPBE = 0, PBE_NPBE = 1, PBE_LRPBE = 2, PBE_NRPBE = 3,
+               PBE_SMPBE = 4 } Vhal_PBEType;
+typedef enum { NONLIN_LPBE = 0, NONLIN_NPBE = 1, NONLIN_SMPBE = 2, NONLIN_LPBEAQUA = 3,
+               NONLIN_NPBEAQUA = 4 } Vhal_NonlinType;
+typedef enum { IPKEY_SMPBE = -2, IPKEY_LPBE = -1, IPKEY_NPBE = 0 } Vhal_IPKEYType;
+typedef enum { BCFL_ZERO = 0, BCFL_SDH = 1, BCFL_MDH = 2, BCFL_UNUSED = 3, BCFL_FOCUS = 4,
+               BCFL_MEM = 5, BCFL_MAP = 6 } Vbcfl;
+typedef enum { VCM_TRIL = 0, VCM_BSPL2 = 1, VCM_BSPL4 = 2 } Vchrg_Meth;
+typedef enum { VCM_CHARGE = 0, VCM_PERMANENT = 1, VCM_INDUCED = 2,
+               VCM_NLINDUCED = 3 } Vchrg_Src;
+typedef enum { VDT_CHARGE = 0, VDT_POT = 1, VDT_ATOMPOT = 2, VDT_SMOL = 3, VDT_SSPL = 4,
+               VDT_VDW = 5, VDT_IVDW = 6, VDT_LAP = 7, VDT_EDENS = 8, VDT_NDENS = 9,
+               VDT_QDENS = 10, VDT_DIELX = 11, VDT_DIELY = 12, VDT_DIELZ = 13,
+               VDT_KAPPA = 14 } Vdata_Type;
+typedef enum { MCT_MANUAL = 0, MCT_AUTO = 1, MCT_PARALLEL = 2, MCT_DUMMY = 3,
+               MCT_NONE = 4 } MGparm_CalcType;
+typedef enum { MCM_POINT = 0, MCM_MOLECULE = 1, MCM_FOCUS = 2 } MGparm_CentMeth;
+typedef enum { PCE_NO = 0, PCE_TOTAL = 1, PCE_COMPS = 2 } PBEparm_calcEnergy;
+
+#define APBS_PMG_MAXION 10   /* src/generic/vhal.h:377 MAXION */
+
+typedef struct sVmem Vmem;   /* opaque; unused by this library */
+typedef struct sVgrid Vgrid; /* opaque; external maps are not supported */
+
+/* ------------------------------------------------------------------ atoms
+ * struct sVatom (src/generic/vatom.h:74-90) without the WCA/name fields. */
+typedef struct sVatom {
+    double position[3];
+    double radius;
+    double charge;
+    double partID;
+    int id;
+} Vatom;
+
+/* struct sValist (src/generic/valist.h:78-89) */
+typedef struct sValist {
+    int number;
+    double center[3];
+    double mincrd[3];
+    double maxcrd[3];
+    double maxrad;
+    double charge;
+    Vatom *atoms;
+} Valist;
+
+/* Valist_ctor / Valist_dtor (src/generic/valist.c:122-186) */
+Valist *Valist_ctor(void);
+void Valist_dtor(Valist **thee);
+/* Replaces Valist_readPQR(thee, param, sock) (src/generic/valist.c:606-723),
+ * which reads through a MALOC socket; reads the whitespace PQR records of a
+ * file path.  Returns 1/0. */
+int Valist_readPQRFile(Valist *thee, const char *path);
+/* No reference counterpart: fills the list from plain arrays (xyz: 3n). */
+int Valist_setAtoms(Valist *thee, int n, const double *xyz, const double *radius,
+                    const double *charge);
+/* Valist_getStatistics (src/generic/valist.c:869-921) */
+int Valist_getStatistics(Valist *thee);
+int Valist_getNumberAtoms(Valist *thee);        /* valist.h:173 */
+Vatom *Valist_getAtom(Valist *thee, int i);      /* valist.h:181 */
+
+/* ------------------------------------------------------------------- Vpbe */
+typedef struct sVpbe Vpbe;
+/* Vpbe_ctor (src/generic/vpbe.c:246-262, body :264-465).  The cell list and
+ * accessibility object of the reference are built on the device when needed. */
+Vpbe *Vpbe_ctor(Valist *alist, int ionNum, double *ionConc, double *ionRadii, double *ionQ,
+                double T, double soluteDiel, double solventDiel, double solventRadius,
+                int focusFlag, double sdens, double z_mem, double L, double membraneDiel,
+                double V);
+void Vpbe_dtor(Vpbe **thee);
+double Vpbe_getBulkIonicStrength(Vpbe *thee);   /* vpbe.h:268 */
+double Vpbe_getMaxIonRadius(Vpbe *thee);        /* vpbe.h:277 */
+double Vpbe_getTemperature(Vpbe *thee);         /* vpbe.h:284 */
+double Vpbe_getSoluteDiel(Vpbe *thee);          /* vpbe.h:291 */
+double Vpbe_getSolventDiel(Vpbe *thee);         /* vpbe.h:316 */
+double Vpbe_getSolventRadius(Vpbe *thee);       /* vpbe.h:323 */
+double Vpbe_getXkappa(Vpbe *thee);              /* vpbe.h:330 */
+double Vpbe_getZkappa2(Vpbe *thee);             /* vpbe.h:344 */
+double Vpbe_getZmagic(Vpbe *thee);              /* vpbe.h:351 */
+double Vpbe_getSoluteRadius(Vpbe *thee);        /* vpbe.h:358 */
+double Vpbe_getSoluteCharge(Vpbe *thee);        /* vpbe.h:393 */
+double *Vpbe_getSoluteCenter(Vpbe *thee);       /* vpbe.h:300 */
+Valist *Vpbe_getValist(Vpbe *thee);             /* vpbe.h:251 */
+int Vpbe_getIons(Vpbe *thee, int *nion, double ionConc[APBS_PMG_MAXION],
+                 double ionRadii[APBS_PMG_MAXION], double ionQ[APBS_PMG_MAXION]);
+
+/* ----------------------------------------------------------------- MGparm
+ * The fields of struct sMGparm (src/generic/mgparm.h:112-183) that the PMG
+ * path and the mg-manual / mg-auto set-up read. */
+typedef struct sMGparm {
+    MGparm_CalcType type;
+    int parsed;
+    int dime[3];  int setdime;
+    int nlev;     int setnlev;
+    double etol;  int setetol;
+    double grid[3];   int setgrid;
+    double glen[3];   int setglen;
+    MGparm_CentMeth cmeth;   double center[3];  int centmol;  int setgcent;
+    double cglen[3];  int setcglen;
+    double fglen[3];  int setfglen;
+    MGparm_CentMeth ccmeth;  double ccenter[3]; int ccentmol; int setcgcent;
+    MGparm_CentMeth fcmeth;  double fcenter[3]; int fcentmol; int setfgcent;
+    double partDisjCenter[3];
+    double partDisjLength[3];
+    int partDisjOwnSide[6];
+    Vhal_NonlinType nonlintype;  int setnonlin;
+    Vsol_Meth method;            int setmethod;
+    int useAqua;                 int setUseAqua;
+    Vchrg_Meth chgm;  int setchgm;
+    Vchrg_Src chgs;
+} MGparm;
+
+MGparm *MGparm_ctor(MGparm_CalcType type);      /* src/generic/mgparm.c:100 */
+void MGparm_dtor(MGparm **thee);                /* src/generic/mgparm.c:163 */
+/* MGparm_check (src/generic/mgparm.c:182-330): validates and derives nlev
+ * and dime; returns 1 (success) / 0. */
+int MGparm_check(MGparm *thee);
+/* NOsh_setupCalcMGAUTO (src/generic/nosh.c:1828-2101): expands an mg-auto
+ * ELEC block into its focusing chain; writes up to maxcalc mg-manual MGparm
+ * copies to out (the caller allocates them with MGparm_ctor) and returns
+ * the number of levels, 0 on failure.  No reference counterpart as a
+ * separate call: the reference appends to the NOsh calculation list. */
+int MGparm_setupAuto(MGparm *elec, MGparm **out, int maxcalc);
+
+/* ------------------------------------------------------------------ Vpmgp
+ * struct sVpmgp, field for field (src/mg/vpmgp.h:80-198). */
+typedef struct sVpmgp {
+    int nx, ny, nz, nlev;
+    double hx, hy, hzed;
+    int nonlin;
+    int nxc, nyc, nzc, nf, nc, narrc, n_rpc, n_iz, n_ipc;
+    size_t nrwk;
+    int niwk, narr, ipkey;
+    double xcent, ycent, zcent, errtol;
+    int itmax, istop, iinfo;
+    Vbcfl bcfl;
+    int key, iperf, meth, mgkey, nu1, nu2, mgsmoo, mgprol, mgcoar, mgsolv, mgdisc;
+    double omegal, omegan;
+    int irite, ipcon;
+    double xlen, ylen, zlen, xmin, ymin, zmin, xmax, ymax, zmax;
+} Vpmgp;
+
+Vpmgp *Vpmgp_ctor(MGparm *mgparm);              /* src/mg/vpmgp.c:73-93 */
+void Vpmgp_dtor(Vpmgp **thee);                  /* src/mg/vpmgp.c:178 */
+void Vpmgp_size(Vpmgp *thee);                   /* src/mg/vpmgp.c:196-291 */
+void Vpmgp_makeCoarse(int numLevel, int nxOld, int nyOld, int nzOld, int *nxNew,
+                      int *nyNew, int *nzNew);  /* src/mg/vpmgp.c:313-334 */
+
+/* ------------------------------------------------------------------- Vpmg
+ * struct sVpmg: the reference field order up to chargeMap
+ * (src/mg/vpmg.h:116-193), which callers dereference directly (pmgp, pbe,
+ * u, pvec, rwork, vmem).  u, pvec and rwork are host arrays of nx*ny*nz
+ * doubles; u is mirrored from the device after Vpmg_solve and pvec after
+ * Vpmg_setPart.  epsx..charge are mirrored on demand by Vpmg_fillArray and
+ * are NULL until then.  The arrays the reference uses only as PMG
+ * workspace (iparm..tcf, xf..gzcf) are NULL.  Device state hangs off the
+ * trailing private field. */
+typedef struct sVpmg {
+    Vmem *vmem;
+    Vpmgp *pmgp;
+    Vpbe *pbe;
+    double *epsx, *epsy, *epsz, *kappa, *pot, *charge;
+    int *iparm;
+    double *rparm;
+    int *iwork;
+    double *rwork;
+    double *a1cf, *a2cf, *a3cf, *ccf, *fcf, *tcf;
+    double *u;
+    double *xf, *yf, *zf;
+    double *gxcf, *gycf, *gzcf;
+    double *pvec;
+    double extDiEnergy, extQmEnergy, extQfEnergy, extNpEnergy;
+    Vsurf_Meth surfMeth;
+    double splineWin;
+    Vchrg_Meth chargeMeth;
+    Vchrg_Src chargeSrc;
+    int filled;
+    int useDielXMap;  Vgrid *dielXMap;
+    int useDielYMap;  Vgrid *dielYMap;
+    int useDielZMap;  Vgrid *dielZMap;
+    int useKappaMap;  Vgrid *kappaMap;
+    int usePotMap;    Vgrid *potMap;
+    int useChargeMap; Vgrid *chargeMap;
+    void *dev;   /* private: device state */
+} Vpmg;
+
+/* Vpmg_ctor / Vpmg_ctor2 (src/mg/vpmg.h:226-242, vpmg.c:141-399).  Borrows
+ * pmgp and pbe; consumes and frees pmgOLD (focusing), like vpmg.c:396. */
+Vpmg *Vpmg_ctor(Vpmgp *parms, Vpbe *pbe, int focusFlag, Vpmg *pmgOLD, MGparm *mgparm,
+                PBEparm_calcEnergy energyFlag);
+int Vpmg_ctor2(Vpmg *thee, Vpmgp *parms, Vpbe *pbe, int focusFlag, Vpmg *pmgOLD,
+               MGparm *mgparm, PBEparm_calcEnergy energyFlag);
+void Vpmg_dtor(Vpmg **thee);                    /* src/mg/vpmg.h:260 */
+void Vpmg_dtor2(Vpmg *thee);                    /* src/mg/vpmg.h:270 */
+/* Vpmg_fillco (src/mg/vpmg.h:281-298, vpmg.c:5250-5415).  External maps
+ * (use*Map != 0) are not supported and return 0. */
+int Vpmg_fillco(Vpmg *thee, Vsurf_Meth surfMeth, double splineWin, Vchrg_Meth chargeMeth,
+                int useDielXMap, Vgrid *dielXMap, int useDielYMap, Vgrid *dielYMap,
+                int useDielZMap, Vgrid *dielZMap, int useKappaMap, Vgrid *kappaMap,
+                int usePotMap, Vgrid *potMap, int useChargeMap, Vgrid *chargeMap);
+/* Vpmg_solve (src/mg/vpmg.h:306, vpmg.c:401-570): 0 if not filled. */
+int Vpmg_solve(Vpmg *thee);
+/* Energies in kT (src/mg/vpmg.h:334-445, vpmg.c:1248-1891) */
+double Vpmg_energy(Vpmg *thee, int extFlag);
+double Vpmg_qfEnergy(Vpmg *thee, int extFlag);
+double Vpmg_qmEnergy(Vpmg *thee, int extFlag);
+double Vpmg_dielEnergy(Vpmg *thee, int extFlag);
+double Vpmg_qfAtomEnergy(Vpmg *thee, Vatom *atom);   /* vpmg.h:386 */
+/* Forces in kT/A (src/mg/vpmg.h:482-546, vpmg.c:5417-6154) */
+int Vpmg_force(Vpmg *thee, double *force, int atomID, Vsurf_Meth srfm, Vchrg_Meth chgm);
+int Vpmg_qfForce(Vpmg *thee, double *force, int atomID, Vchrg_Meth chgm);
+int Vpmg_ibForce(Vpmg *thee, double *force, int atomID, Vsurf_Meth srfm);
+int Vpmg_dbForce(Vpmg *thee, double *dbForce, int atomID, Vsurf_Meth srfm);
+/* Partitions (src/mg/vpmg.h:553-569, vpmg.c:627-890) */
+void Vpmg_setPart(Vpmg *thee, double lowerCorner[3], double upperCorner[3], int bflags[6]);
+void Vpmg_unsetPart(Vpmg *thee);
+/* Vpmg_fillArray (src/mg/vpmg.h:576, vpmg.c:892): VDT_POT, VDT_CHARGE,
+ * VDT_DIELX/Y/Z, VDT_KAPPA; pbeparm is unused and may be NULL. */
+int Vpmg_fillArray(Vpmg *thee, double *vec, Vdata_Type type, double parm,
+                   Vhal_PBEType pbetype, void *pbeparm);
+unsigned long int Vpmg_memChk(Vpmg *thee);     /* src/mg/vpmg.h:208 */
+
+/* ------------------------------------------------------ extensions
+ * No reference counterpart. */
+
+/* Precision of the multigrid solve: 0 = FP64 throughout (default, the
+ * reference arithmetic); 1 = FP32 V-cycles (smoother, transfers, coarse
+ * operators) inside an FP64 defect-correction loop whose fine residual,
+ * coarsest solve, Newton update and energy/force sums stay FP64.  tol is the
+ * relative FP64 residual the mixed loop stops at (<= 0: pmgp->errtol). */
+int Vpmg_setPrecision(Vpmg *thee, int mode, double tol);
+/* Whether Vpmg_solve copies u back to the host array (default 1). */
+int Vpmg_setHostMirror(Vpmg *thee, int on);
+/* Solver statistics of the last solve: [0] V-cycles (linear) or Newton
+ * steps, [1] total inner V-cycles, [2] levels used, [3] final relative
+ * residual, [4] solve time (s), [5] fillco time (s), [6] operator-build time
+ * (s), [7] mixed-precision outer iterations.  Returns the count written. */
+int Vpmg_getStats(Vpmg *thee, double *out, int n);
+/* Copy a fine-grid device array to host: 0 u, 1 epsx, 2 epsy, 3 epsz,
+ * 4 kappa, 5 charge, 6 pvec, 7 gxcf (2*ny*nz), 8 gycf, 9 gzcf. */
+int Vpmg_copyArray(Vpmg *thee, int which, double *out, size_t n);
+/* Library version string. */
+const char *apbs_pmg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

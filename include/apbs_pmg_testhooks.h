@@ -1,0 +1,41 @@
+/*
+ * apbs_pmg_testhooks.h -- single-kernel entry points of libapbs_pmg.so used
+ * by the kernel-by-kernel parity tests.  Host arrays in, host arrays out; no
+ * counterpart in the reference (they expose the stencil kernels that the
+ * reference runs inside Vmvcs / Vbuildops).
+ */
+#ifndef APBS_PMG_TESTHOOKS_H
+#define APBS_PMG_TESTHOOKS_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* Vgsrb7x / 27-point 8-colour sweeps (src/pmgc/gsd.c:95-340); a packs 4
+ * (stencil 7) or 14 (stencil 27) operator arrays of nx*ny*nz each. */
+int pmgtest_gsrb(int nx, int ny, int nz, int stencil, const double *a, const double *cc,
+                 const double *fc, double *x, int nu, int iadjoint);
+/* Vmresid7_1s / Vmresid27_1s (src/pmgc/matvecd.c:461-596) */
+int pmgtest_resid(int nx, int ny, int nz, int stencil, const double *a, const double *cc,
+                  const double *fc, const double *x, double *r);
+/* Vrestrc2 (src/pmgc/matvecd.c:801-913) */
+int pmgtest_restrict(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc, const double *in,
+                     double *out);
+/* VinterpPMG2 + Vxaxpy (src/pmgc/matvecd.c:935-1076, mgcsd.c:485-487) */
+int pmgtest_prolong(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf, const double *xc,
+                    double *xf, double damp);
+/* VbuildG_7 / VbuildG_27 (src/pmgc/buildGd.c:450-3115) */
+int pmgtest_galerkin(int nxf, int nyf, int nzf, int stencil, const double *af, int nxc, int nyc,
+                     int nzc, double *ac);
+/* VbuildA_fv (src/pmgc/buildAd.c:105-281) */
+int pmgtest_buildA(int nx, int ny, int nz, const double *xf, const double *yf, const double *zf,
+                   const double *gx, const double *gy, const double *gz, const double *a1,
+                   const double *a2, const double *a3, const double *ccf, const double *fcf,
+                   double *oC, double *cc, double *fc, double *oE, double *oN, double *uC);
+/* Vnmresid7_1s with Vc_vecpmg, and Vdc_vecpmg (src/pmgc/matvecd.c:633-674,
+ * src/pmgc/mypdec.c:145-223, 366-433) */
+int pmgtest_nonlinear(int nx, int ny, int nz, const double *a, const double *cc,
+                      const double *fc, const double *x, int nion, const double *q,
+                      const double *sconc, double *out_r, double *out_d);
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,87 @@
+"""The C-ABI library loads here (no GPU) and exports every symbol that
+include/*.h declares; no compute calls."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "apbs_amd", "libapbs_pmg.so")
+
+
+def declared_symbols():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text):
+            name = m.group(1)
+            if name in ("if", "sizeof", "return", "defined"):
+                continue
+            # declarations: identifier followed by '(' at top level of a prototype
+            names.add(name)
+    # keep only names that look like API functions
+    return sorted(n for n in names if re.match(r"^(Vpmg|Vpmgp|Vpbe|Valist|MGparm|apbs_pmg|pmgtest)_", n))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        pytest.skip("library not built")
+    return ctypes.CDLL(LIB)
+
+
+def test_library_loads_and_exports_all_declared(lib):
+    decl = declared_symbols()
+    assert len(decl) >= 60
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    missing = [d for d in decl if d not in exported]
+    assert not missing, missing
+
+
+def test_python_binding_covers_the_abi():
+    import apbs_amd
+    decl = {d for d in declared_symbols() if not d.startswith("pmgtest_")}
+    assert decl <= set(apbs_amd.EXPORTED), sorted(decl - set(apbs_amd.EXPORTED))
+
+
+def test_host_only_entry_points(lib):
+    """Entry points that touch no device memory can run without a GPU."""
+    import apbs_amd
+    L = apbs_amd.lib()
+    assert L.apbs_pmg_version().startswith(b"apbs_pmg")
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    L.Vpmgp_makeCoarse(2, 161, 97, 65, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    assert (a.value, b.value, c.value) == (41, 25, 17)
+    m = L.MGparm_ctor(apbs_amd.MCT_MANUAL)
+    m.contents.dime[:] = [161, 161, 161]
+    m.contents.setdime = 1
+    m.contents.grid[:] = [0.4, 0.4, 0.4]
+    m.contents.setgrid = 1
+    m.contents.setgcent = 1
+    m.contents.setchgm = 1
+    assert L.MGparm_check(m) == 1
+    # mgparm.c:265-320: 161 = 5*2^5+1 -> nlev 4 (not the 5 a user may ask for)
+    assert m.contents.nlev == 4
+    L.MGparm_dtor(ctypes.byref(m))
+
+
+def test_mgauto_chain_matches_restatement():
+    """MGparm_setupAuto (library) against the test restatement of
+    NOsh_setupCalcMGAUTO for the born example (cglen 50, fglen 12, 65^3)."""
+    import numpy as np
+    from apbs_amd import driver
+    import orc_driver as od
+    chain = driver.auto_chain([65] * 3, [50] * 3, [12] * 3, [0, 0, 0], [0.5, -0.25, 1.0])
+    ref = od.auto_chain([65] * 3, [50] * 3, [12] * 3, [0, 0, 0], [0.5, -0.25, 1.0])
+    assert len(chain) == len(ref) == 3
+    for m, (grid, glen, center) in zip(chain, ref):
+        assert np.allclose(m.contents.grid[:], grid, rtol=0, atol=0)
+        assert np.allclose(m.contents.glen[:], glen, rtol=0, atol=0)
+        assert np.allclose(m.contents.center[:], center, rtol=0, atol=0)

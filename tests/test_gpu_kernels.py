@@ -1,0 +1,165 @@
+"""Kernel-by-kernel parity of the HIP path against the oracle.
+
+Fine-level kernels (7-point red-black sweep, residuals, restriction,
+prolongation, VbuildA_fv, Galerkin) evaluate the reference's expressions in
+the reference's order without FMA contraction, so the bar is bitwise
+equality.  The 8-colour 27-point sweep is compared with the oracle's
+8-colour variant, also bitwise.  The nonlinear term goes through exp(),
+whose device and glibc implementations may differ in the last ulp: 4 ulp.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import orc_driver as od
+
+pytestmark = pytest.mark.gpu
+
+dp = ctypes.POINTER(ctypes.c_double)
+
+
+def P(a):
+    return a.ctypes.data_as(dp)
+
+
+@pytest.fixture(scope="module")
+def L():
+    import apbs_amd
+    return ctypes.CDLL(apbs_amd.LIB_PATH)
+
+
+@pytest.fixture(scope="module")
+def O():
+    return od.lib()
+
+
+def rand_op(rng, n3, st):
+    nx, ny, nz = n3
+    n = nx * ny * nz
+    na = 4 if st == 7 else 14
+    a = np.zeros((na, nz, ny, nx))
+    inner = (slice(1, -1),) * 3
+    for s in range(1, na):
+        a[(s,) + inner] = rng.uniform(0.1, 1.0, (nz - 2, ny - 2, nx - 2))
+    a[(0,) + inner] = rng.uniform(6.0, 12.0, (nz - 2, ny - 2, nx - 2)) * (1 if st == 7 else 3)
+    return np.ascontiguousarray(a.reshape(na * n))
+
+
+def interior(x, n3):
+    nx, ny, nz = n3
+    return x.reshape(nz, ny, nx)[1:-1, 1:-1, 1:-1]
+
+
+@pytest.mark.parametrize("st,n3", [(7, (17, 19, 21)), (7, (33, 33, 33)), (27, (17, 17, 17)),
+                                   (27, (9, 11, 13))])
+@pytest.mark.parametrize("adj", [0, 1])
+def test_gsrb_bitwise(L, O, st, n3, adj):
+    rng = np.random.default_rng(7 + st + adj)
+    nx, ny, nz = n3
+    n = nx * ny * nz
+    a = rand_op(rng, n3, st)
+    cc = rng.uniform(0, 1, n)
+    fc = rng.normal(size=n)
+    x0 = np.zeros(n)
+    x0.reshape(nz, ny, nx)[1:-1, 1:-1, 1:-1] = rng.normal(size=(nz - 2, ny - 2, nx - 2))
+    xg = x0.copy()
+    xo = x0.copy()
+    L.pmgtest_gsrb(nx, ny, nz, st, P(a), P(cc), P(fc), P(xg), 2, adj)
+    O.orc_gsrb(nx, ny, nz, st, P(a), P(cc), P(fc), P(xo), 2, adj, 1)
+    assert np.array_equal(xg, xo)
+
+
+@pytest.mark.parametrize("st", [7, 27])
+def test_resid_bitwise(L, O, st):
+    rng = np.random.default_rng(11)
+    n3 = (21, 17, 19)
+    nx, ny, nz = n3
+    n = nx * ny * nz
+    a = rand_op(rng, n3, st)
+    cc, fc, x = rng.uniform(0, 1, n), rng.normal(size=n), rng.normal(size=n)
+    rg = np.zeros(n)
+    ro = np.zeros(n)
+    L.pmgtest_resid(nx, ny, nz, st, P(a), P(cc), P(fc), P(x), P(rg))
+    O.orc_resid(nx, ny, nz, st, P(a), P(cc), P(fc), P(x), P(ro))
+    assert np.array_equal(interior(rg, n3), interior(ro, n3))
+
+
+def test_transfers_bitwise(L, O):
+    rng = np.random.default_rng(3)
+    f3, c3 = (33, 17, 25), (17, 9, 13)
+    nf, nc = np.prod(f3), np.prod(c3)
+    xin = rng.normal(size=nf)
+    og, oo = np.zeros(nc), np.zeros(nc)
+    L.pmgtest_restrict(*f3, *c3, P(xin), P(og))
+    O.orc_restrict(*f3, *c3, P(xin.copy()), P(oo))
+    assert np.array_equal(interior(og, c3), interior(oo, c3))
+    xc = rng.normal(size=nc)
+    xc.reshape(c3[2], c3[1], c3[0])[[0, -1], :, :] = 0
+    xc.reshape(c3[2], c3[1], c3[0])[:, [0, -1], :] = 0
+    xc.reshape(c3[2], c3[1], c3[0])[:, :, [0, -1]] = 0
+    fg, fo = np.zeros(nf), np.zeros(nf)
+    L.pmgtest_prolong(*c3, *f3, P(xc), P(fg), ctypes.c_double(1.0))
+    O.orc_interp(*c3, *f3, P(xc.copy()), P(fo))
+    assert np.array_equal(interior(fg, f3), interior(fo, f3))
+
+
+@pytest.mark.parametrize("st", [7, 27])
+def test_galerkin_bitwise(L, O, st):
+    rng = np.random.default_rng(5 + st)
+    f3, c3 = (17, 17, 17), (9, 9, 9)
+    nf, nc = np.prod(f3), np.prod(c3)
+    af = rand_op(rng, f3, st)
+    ag, ao = np.zeros(14 * nc), np.zeros(14 * nc)
+    L.pmgtest_galerkin(*f3, st, P(af), *c3, P(ag))
+    O.orc_galerkin(*f3, st, P(af), *c3, P(ao))
+    for s in range(14):
+        assert np.array_equal(interior(ag[s * nc:(s + 1) * nc], c3),
+                              interior(ao[s * nc:(s + 1) * nc], c3)), s
+
+
+def test_buildA_bitwise(L, O):
+    rng = np.random.default_rng(9)
+    n3 = (19, 17, 21)
+    nx, ny, nz = n3
+    n = nx * ny * nz
+    xf = -3.7 + np.arange(nx) * 0.41
+    yf = 1.3 + np.arange(ny) * 0.37
+    zf = -0.2 + np.arange(nz) * 0.43
+    gx, gy, gz = (rng.normal(size=2 * ny * nz), rng.normal(size=2 * nx * nz),
+                  rng.normal(size=2 * nx * ny))
+    a1, a2, a3 = (rng.uniform(2, 80, n) for _ in range(3))
+    ccf, fcf = rng.uniform(0, 1, n), rng.normal(size=n)
+    outs_g = [np.zeros(n) for _ in range(6)]
+    outs_o = [np.zeros(n) for _ in range(6)]
+    L.pmgtest_buildA(nx, ny, nz, P(xf), P(yf), P(zf), P(gx), P(gy), P(gz), P(a1), P(a2), P(a3),
+                     P(ccf), P(fcf), *[P(o) for o in outs_g])
+    O.orc_buildA_fv(nx, ny, nz, P(xf), P(yf), P(zf), P(gx), P(gy), P(gz), P(a1), P(a2), P(a3),
+                    P(ccf), P(fcf), *[P(o) for o in outs_o])
+    for g, o in zip(outs_g, outs_o):
+        assert np.array_equal(interior(g, n3), interior(o, n3))
+
+
+def test_nonlinear_terms(L, O):
+    rng = np.random.default_rng(13)
+    n3 = (17, 17, 17)
+    nx, ny, nz = n3
+    n = nx * ny * nz
+    a = rand_op(rng, n3, 7)
+    cc = rng.uniform(0, 1, n)
+    cc[::7] = 0.0
+    fc, x = rng.normal(size=n), rng.normal(scale=30, size=n)
+    q = np.array([1.0, -1.0, 2.0, -2.0])
+    sc = np.array([0.2, 0.2, 0.05, 0.05])
+    rg, dg = np.zeros(n), np.zeros(n)
+    L.pmgtest_nonlinear(nx, ny, nz, P(a), P(cc), P(fc), P(x), 4, P(q), P(sc), P(rg), P(dg))
+    ro, w, do = np.zeros(n), np.zeros(n), np.zeros(n)
+    zero = np.zeros(n)
+    O.orc_resid(nx, ny, nz, 7, P(a), P(zero), P(fc), P(x), P(ro))
+    O.orc_cvec(n, 4, P(q), P(sc), P(cc), P(x), P(w))
+    O.orc_dcvec(n, 4, P(q), P(sc), P(cc), P(x), P(do))
+    ro = ro - w
+    ri, rgi = interior(ro, n3), interior(rg, n3)
+    scale = np.abs(ri).max()
+    assert np.max(np.abs(rgi - ri)) <= 8 * np.finfo(float).eps * scale
+    assert np.allclose(dg, do, rtol=4 * np.finfo(float).eps, atol=0)
