@@ -183,6 +183,9 @@ _SIGS = [
     ("Vpmg_setHostMirror", ctypes.c_int, [ctypes.POINTER(Vpmg), ctypes.c_int]),
     ("Vpmg_getStats", ctypes.c_int, [ctypes.POINTER(Vpmg), _dp, ctypes.c_int]),
     ("Vpmg_copyArray", ctypes.c_int, [ctypes.POINTER(Vpmg), ctypes.c_int, _dp, ctypes.c_size_t]),
+    ("Vpmg_setKernelTiming", ctypes.c_int, [ctypes.POINTER(Vpmg), ctypes.c_int]),
+    ("Vpmg_getKernelTiming", ctypes.c_int, [ctypes.POINTER(Vpmg), _dp, ctypes.c_int]),
+    ("apbs_pmg_synchronize", ctypes.c_int, []),
     ("apbs_pmg_version", ctypes.c_char_p, []),
 ]
 

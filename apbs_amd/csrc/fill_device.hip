@@ -218,37 +218,106 @@ __global__ void __launch_bounds__(kB)
 }
 
 // ------------------------------------------------------------ SAS / markSphere
+// Accessibility of the SAS points of one atom (Vacc_atomSurf with
+// ivdwAccExclus, vacc.c:78-128, 866-922): a point is kept unless another atom
+// b has |pt - b|^2 < (r_b + probe)^2.  One workgroup per atom: the atoms that
+// can exclude any point of this atom's sphere (|a - b| < r_a + r_b + 2 probe,
+// plus a margin) are gathered once into LDS, then every point is tested
+// against that list.  The test is a boolean "any", so the list order does not
+// matter; the arithmetic of each test is the oracle's.
+constexpr int kSasCap = 1024;
+
+__device__ inline bool sas_free_global(const AtomsDev& A, const Cells& C, int self, double px,
+                                       double py, double pz, double probe) {
+  const int cx = (int)floor((px - C.lo[0]) / C.cs);
+  const int cy = (int)floor((py - C.lo[1]) / C.cs);
+  const int cz = (int)floor((pz - C.lo[2]) / C.cs);
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        const int a = cx + dx, b = cy + dy, c = cz + dz;
+        if (a < 0 || b < 0 || c < 0 || a >= C.dim[0] || b >= C.dim[1] || c >= C.dim[2]) continue;
+        const int id = a + C.dim[0] * (b + C.dim[1] * c);
+        for (int q = C.start[id]; q < C.start[id + 1]; q++) {
+          const int n = C.list[q];
+          if (n == self) continue;
+          const double* bp = A.pos + 3 * n;
+          const double d2 = (px - bp[0]) * (px - bp[0]) + (py - bp[1]) * (py - bp[1]) +
+                            (pz - bp[2]) * (pz - bp[2]);
+          const double rr = A.rad[n] + probe;
+          if (d2 < rr * rr) return false;
+        }
+      }
+  return true;
+}
+
 __global__ void __launch_bounds__(kB)
     k_sas_flags(AtomsDev A, Cells C, const double* ref, int nref, double probe,
                 unsigned char* flags) {
-  const long t = (long)blockIdx.x * kB + threadIdx.x;
-  if (t >= (long)A.n * nref) return;
-  const int a = (int)(t / nref), q = (int)(t % nref);
+  __shared__ double sx[kSasCap], sy[kSasCap], sz[kSasCap], sr2[kSasCap];
+  __shared__ int cnt;
+  const int a = blockIdx.x;
+  const int tid = threadIdx.x;
+  unsigned char* fl = flags + (size_t)a * nref;
   const double arad = A.rad[a];
   if (arad < VSMALL) {
-    flags[t] = 0;
+    for (int q = tid; q < nref; q += kB) fl[q] = 0;
     return;
   }
-  const double rad = arad + probe;
+  if (tid == 0) cnt = 0;
+  __syncthreads();
   const double* ap = A.pos + 3 * a;
-  const double px = rad * (ref[3 * q]) + ap[0];
-  const double py = rad * (ref[3 * q + 1]) + ap[1];
-  const double pz = rad * (ref[3 * q + 2]) + ap[2];
-  Merge m;
-  merge_init(m, C, px, py, pz);
-  unsigned char ok = 1;
-  for (int b = merge_next(m, C.list); b >= 0; b = merge_next(m, C.list)) {
-    if (b == a) continue;
-    const double* bp = A.pos + 3 * b;
-    const double d2 = (px - bp[0]) * (px - bp[0]) + (py - bp[1]) * (py - bp[1]) +
-                      (pz - bp[2]) * (pz - bp[2]);
-    const double rr = A.rad[b] + probe;
-    if (d2 < rr * rr) {
-      ok = 0;
-      break;
+  const double ax = ap[0], ay = ap[1], az = ap[2];
+  const int cx = (int)floor((ax - C.lo[0]) / C.cs);
+  const int cy = (int)floor((ay - C.lo[1]) / C.cs);
+  const int cz = (int)floor((az - C.lo[2]) / C.cs);
+  // 125 cells (+-2 around the atom's cell), one per thread
+  if (tid < 125) {
+    const int x = cx + tid % 5 - 2, y = cy + (tid / 5) % 5 - 2, z = cz + tid / 25 - 2;
+    if (x >= 0 && y >= 0 && z >= 0 && x < C.dim[0] && y < C.dim[1] && z < C.dim[2]) {
+      const int id = x + C.dim[0] * (y + C.dim[1] * z);
+      for (int q = C.start[id]; q < C.start[id + 1]; q++) {
+        const int b = C.list[q];
+        if (b == a) continue;
+        const double* bp = A.pos + 3 * b;
+        const double rr = A.rad[b] + probe;
+        const double lim = arad + probe + rr + 1e-6;
+        const double d2 = (ax - bp[0]) * (ax - bp[0]) + (ay - bp[1]) * (ay - bp[1]) +
+                          (az - bp[2]) * (az - bp[2]);
+        if (d2 < lim * lim) {
+          const int slot = atomicAdd(&cnt, 1);
+          if (slot < kSasCap) {
+            sx[slot] = bp[0];
+            sy[slot] = bp[1];
+            sz[slot] = bp[2];
+            sr2[slot] = rr * rr;
+          }
+        }
+      }
     }
   }
-  flags[t] = ok;
+  __syncthreads();
+  const int n = cnt;
+  const double rad = arad + probe;
+  for (int q = tid; q < nref; q += kB) {
+    const double px = rad * (ref[3 * q]) + ax;
+    const double py = rad * (ref[3 * q + 1]) + ay;
+    const double pz = rad * (ref[3 * q + 2]) + az;
+    unsigned char ok = 1;
+    if (n <= kSasCap) {
+      for (int b = 0; b < n; b++) {
+        const double d2 = (px - sx[b]) * (px - sx[b]) + (py - sy[b]) * (py - sy[b]) +
+                          (pz - sz[b]) * (pz - sz[b]);
+        if (d2 < sr2[b]) {
+          ok = 0;
+          break;
+        }
+      }
+    } else {
+      ok = sas_free_global(A, C, a, px, py, pz, probe) ? 1 : 0;
+    }
+    fl[q] = ok;
+  }
 }
 
 __global__ void k_sas_compact(AtomsDev A, const double* ref, int nref, double probe,
@@ -881,8 +950,8 @@ void coef_spline(hipStream_t s, const Geo& g, const AtomsDev& A, const Cells& C,
 
 void sas_flags(hipStream_t s, const AtomsDev& A, const Cells& C, const double* ref, int nref,
                double probe, unsigned char* flags) {
-  const long n = (long)A.n * nref;
-  k_sas_flags<<<(unsigned)((n + kB - 1) / kB), kB, 0, s>>>(A, C, ref, nref, probe, flags);
+  if (A.n <= 0) return;
+  k_sas_flags<<<(unsigned)A.n, kB, 0, s>>>(A, C, ref, nref, probe, flags);
 }
 
 int sas_compact(hipStream_t s, const AtomsDev& A, const double* ref, int nref, double probe,

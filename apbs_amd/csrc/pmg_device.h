@@ -34,6 +34,8 @@ constexpr int kParts = 1024;
 template <class T>
 void gsrb7(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
            T* x, int nu, int iadjoint);
+// kernel launches one gsrb7 call of nu sweeps makes
+int gsrb7_launches(int nu);
 template <class T>
 void gsrb27(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
             T* x, int nu, int iadjoint);

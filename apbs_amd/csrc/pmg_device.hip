@@ -543,6 +543,9 @@ __constant__ int c_fwd[14][3] = {{0, 0, 0},  {1, 0, 0},  {0, 1, 0},  {0, 0, 1}, 
                                  {-1, 1, 0}, {1, 0, 1},  {-1, 0, 1}, {0, 1, 1}, {0, -1, 1},
                                  {1, 1, 1},  {-1, 1, 1}, {1, -1, 1}, {-1, -1, 1}};
 
+// stored direction of each offset (sx,sy,sz) at (sz+1)*9+(sy+1)*3+(sx+1): +s forward, -s backward
+__constant__ int c_ent[27] = {-10, -8, -11, -6, -3, -7, -12, -9, -13, -4, -2, -5, -1, 0, 1, 5, 2, 4, 13, 9, 12, 7, 3, 6, 11, 8, 10};
+
 __device__ inline double pweight(int dx, int dy, int dz) {
   if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) return 0.0;
   const int m = abs(dx) + abs(dy) + abs(dz);
@@ -554,14 +557,12 @@ __device__ inline double pweight(int dx, int dy, int dz) {
 __device__ inline double fine_entry(int st, const Ops27<double>& A, int nxf, int nyf, int fi,
                                     int fj, int fk, int sx, int sy, int sz) {
   if (sx == 0 && sy == 0 && sz == 0) return A.a[0][IX(fi, fj, fk, nxf, nyf)];
-  const int ns = st == 7 ? 4 : 14;
-  for (int s = 1; s < ns; s++) {
-    if (c_fwd[s][0] == sx && c_fwd[s][1] == sy && c_fwd[s][2] == sz)
-      return -A.a[s][IX(fi, fj, fk, nxf, nyf)];
-    if (c_fwd[s][0] == -sx && c_fwd[s][1] == -sy && c_fwd[s][2] == -sz)
-      return -A.a[s][IX(fi + sx, fj + sy, fk + sz, nxf, nyf)];
-  }
-  return 0.0;
+  // each offset is exactly one stored direction, forward (+) or backward (-)
+  const int e = c_ent[(sz + 1) * 9 + (sy + 1) * 3 + (sx + 1)];
+  const int s = e > 0 ? e : -e;
+  if (s >= (st == 7 ? 4 : 14)) return 0.0;
+  if (e > 0) return -A.a[s][IX(fi, fj, fk, nxf, nyf)];
+  return -A.a[s][IX(fi + sx, fj + sy, fk + sz, nxf, nyf)];
 }
 
 struct OutPtrs {
@@ -588,19 +589,24 @@ __global__ void __launch_bounds__(kBlock)
   }
   const int f0x = 2 * I, f0y = 2 * J, f0z = 2 * K;
   const int g0x = 2 * I2, g0y = 2 * J2, g0z = 2 * K2;
+  // The terms with a non-zero second weight pweight(f + s - g0) are those
+  // with |f + s - g0| <= 1 per axis; visiting only that range (in the same
+  // order) adds exactly the terms the full 27-point scan adds.
   double sum = 0.0;
   for (int dz = -1; dz <= 1; dz++)
     for (int dy = -1; dy <= 1; dy++)
       for (int dx = -1; dx <= 1; dx++) {
         const int fx = f0x + dx, fy = f0y + dy, fz = f0z + dz;
         const double p1 = pweight(dx, dy, dz);
+        const int sx0 = max(-1, g0x - 1 - fx), sx1 = min(1, g0x + 1 - fx);
+        const int sy0 = max(-1, g0y - 1 - fy), sy1 = min(1, g0y + 1 - fy);
+        const int sz0 = max(-1, g0z - 1 - fz), sz1 = min(1, g0z + 1 - fz);
         double tt = 0.0;
-        for (int sz = -1; sz <= 1; sz++)
-          for (int sy = -1; sy <= 1; sy++)
-            for (int sx = -1; sx <= 1; sx++) {
+        for (int sz = sz0; sz <= sz1; sz++)
+          for (int sy = sy0; sy <= sy1; sy++)
+            for (int sx = sx0; sx <= sx1; sx++) {
               if (st == 7 && abs(sx) + abs(sy) + abs(sz) > 1) continue;
               const double p2 = pweight(fx + sx - g0x, fy + sy - g0y, fz + sz - g0z);
-              if (p2 == 0.0) continue;
               tt += fine_entry(st, Af, nxf, nyf, fx, fy, fz, sx, sy, sz) * p2;
             }
         sum += p1 * tt;
@@ -653,6 +659,34 @@ __global__ void __launch_bounds__(1024) k_dense_invert(double* M, double* col, i
   }
 }
 
+// The same elimination with the matrix held in LDS (N*N + N doubles).
+__global__ void __launch_bounds__(1024) k_dense_invert_lds(double* Mg, int N) {
+  extern __shared__ double sm[];
+  double* M = sm;
+  double* col = sm + (size_t)N * N;
+  for (int e = threadIdx.x; e < N * N; e += blockDim.x) M[e] = Mg[e];
+  __syncthreads();
+  for (int k = 0; k < N; k++) {
+    for (int i = threadIdx.x; i < N; i += blockDim.x) col[i] = M[i * N + k];
+    __syncthreads();
+    const double piv = col[k];
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+      const double v = (j == k) ? 1.0 : M[k * N + j];
+      M[k * N + j] = v / piv;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+      const int i = e / N, j = e % N;
+      if (i == k) continue;
+      const double f = col[i];
+      const double base = (j == k) ? 0.0 : M[i * N + j];
+      M[i * N + j] = base - f * M[k * N + j];
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < N * N; e += blockDim.x) Mg[e] = M[e];
+}
+
 template <class T>
 __global__ void k_dense_solve(int nx, int ny, int nz, const double* Minv, int N, const T* f,
                               T* x) {
@@ -699,6 +733,8 @@ void gsrb7(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, c
     k_gsrb7<T><<<g, kBlock, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, x, 1 - first);
   }
 }
+
+int gsrb7_launches(int nu) { return 2 * nu; }
 
 template <class T>
 void gsrb27(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc, T* x,
@@ -828,7 +864,11 @@ void dense_assemble(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A,
 }
 
 void dense_invert(hipStream_t s, double* M, double* work, int N) {
-  k_dense_invert<<<1, 1024, 0, s>>>(M, work, N);
+  const size_t lds = ((size_t)N * N + N) * sizeof(double);
+  if (lds <= 64 * 1024)
+    k_dense_invert_lds<<<1, 1024, lds, s>>>(M, N);
+  else
+    k_dense_invert<<<1, 1024, 0, s>>>(M, work, N);
 }
 
 template <class T>

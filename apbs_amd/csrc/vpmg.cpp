@@ -559,11 +559,13 @@ struct PmgDev {
   std::vector<double> hpart;  // host partID copy
   // hierarchy
   int nlev_ref = 0, ntot = 0;
+  bool h_newton = false, h_mixed = false;
   DLevel L[24];
   bool built = false;
   double *w1 = nullptr, *F = nullptr, *wn1 = nullptr, *wn2 = nullptr, *wn3 = nullptr,
          *nlw = nullptr;
   float* w1f = nullptr;
+  double* ecorr = nullptr;  // FP64 image of the FP32 correction (mixed solves)
   double *part = nullptr, *scal = nullptr;
   double* hscal = nullptr;  // pinned
   double *dense = nullptr, *dwork = nullptr;
@@ -581,6 +583,16 @@ struct PmgDev {
   int mirror = 1;
   double stats[8] = {};
   pmg::Geo geo{};
+  // kernel timing: HIP event pairs recorded on the object's stream around
+  // fine-level smoother calls (kind 0) and whole V-cycles (kind 1)
+  int timing = 0;
+  std::vector<hipEvent_t> ev;
+  struct Mark {
+    int e, kind, launches;
+  };
+  std::vector<Mark> marks;
+  int ev_next = 0;    // next free event of the pool
+  double kt[6] = {};  // ms0, launches0, ms1, count1, word bytes, fine points
 };
 
 PmgDev* DEV(Vpmg* p) { return (PmgDev*)p->dev; }
@@ -589,6 +601,37 @@ double read_scalar(PmgDev* d, const double* dptr) {
   HIPCHECK(hipMemcpyAsync(d->hscal, dptr, sizeof(double), hipMemcpyDeviceToHost, d->s));
   HIPCHECK(hipStreamSynchronize(d->s));
   return d->hscal[0];
+}
+
+int tbegin(PmgDev* d) {
+  if (!d->timing) return -1;
+  const int e = d->ev_next;
+  d->ev_next += 2;
+  while ((int)d->ev.size() < e + 2) {
+    hipEvent_t h;
+    HIPCHECK(hipEventCreate(&h));
+    d->ev.push_back(h);
+  }
+  HIPCHECK(hipEventRecord(d->ev[e], d->s));
+  return e;
+}
+
+void tend(PmgDev* d, int e, int kind, int launches) {
+  if (e < 0) return;
+  HIPCHECK(hipEventRecord(d->ev[e + 1], d->s));
+  d->marks.push_back({e, kind, launches});
+}
+
+// Accumulate the recorded intervals (the stream must be idle).
+void tcollect(PmgDev* d) {
+  for (const auto& m : d->marks) {
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, d->ev[m.e], d->ev[m.e + 1]));
+    d->kt[2 * m.kind] += ms;
+    d->kt[2 * m.kind + 1] += m.launches;
+  }
+  d->marks.clear();
+  d->ev_next = 0;
 }
 
 // Cell list over points with ascending indices per cell (counting sort), for
@@ -699,6 +742,7 @@ void free_hierarchy(PmgDev* d) {
   }
   dfree(d->w1);
   dfree(d->w1f);
+  dfree(d->ecorr);
   dfree(d->F);
   dfree(d->wn1);
   dfree(d->wn2);
@@ -741,7 +785,12 @@ int plan_levels(int nx, int ny, int nz, int nlev, int dims[][3]) {
 }
 
 void alloc_hierarchy(PmgDev* d, int nlev, bool newton, bool mixed) {
+  // A hierarchy of the same shape is reused: every array in it is written
+  // before it is read within a solve.
+  if (d->ntot > 0 && d->nlev_ref == nlev && d->h_newton == newton && d->h_mixed == mixed) return;
   free_hierarchy(d);
+  d->h_newton = newton;
+  d->h_mixed = mixed;
   int dims[24][3];
   d->nlev_ref = nlev;
   d->ntot = plan_levels(d->nx, d->ny, d->nz, nlev, dims);
@@ -767,7 +816,7 @@ void alloc_hierarchy(PmgDev* d, int nlev, bool newton, bool mixed) {
       pmg::fill<double>(d->s, L.xtmp, L.n, 0.0);
     }
     if (mixed) {
-      for (int s = 0; s < na; s++) L.af[s] = dalloc<float>(L.n);
+      for (int s = 1; s < na; s++) L.af[s] = dalloc<float>(L.n);  // oC is folded into Df
       L.Df = dalloc<float>(L.n);
       L.xfl = dalloc<float>(L.n);
       L.w0f = dalloc<float>(L.n);
@@ -780,6 +829,8 @@ void alloc_hierarchy(PmgDev* d, int nlev, bool newton, bool mixed) {
   if (mixed) {
     d->w1f = dalloc<float>(d->n);
     pmg::fill<float>(d->s, d->w1f, d->n, 0.0f);
+    d->ecorr = dalloc<double>(d->n);
+    pmg::fill<double>(d->s, d->ecorr, d->n, 0.0);
   }
   d->F = dalloc<double>(d->n);
   pmg::fill<double>(d->s, d->F, d->n, 0.0);
@@ -825,11 +876,15 @@ void set_diagonals(PmgDev* d, bool use_cprime) {
   pmg::dense_invert(d->s, d->dense, d->dwork, d->dn);
 }
 
-void sync_fp32(PmgDev* d) {
+// FP32 images of the level operators (ops: the stencil arrays, which do not
+// change within a solve) and of the diagonals D (which change with every
+// Newton Jacobian).
+void sync_fp32(PmgDev* d, bool ops) {
   for (int l = 0; l < d->ntot; l++) {
     DLevel& L = d->L[l];
     const int na = l == 0 ? 4 : 14;
-    for (int s = 0; s < na; s++) pmg::convert<double, float>(d->s, L.n, L.a[s], L.af[s]);
+    if (ops)
+      for (int s = 1; s < na; s++) pmg::convert<double, float>(d->s, L.n, L.a[s], L.af[s]);
     pmg::convert<double, float>(d->s, L.n, L.D, L.Df);
   }
 }
@@ -853,7 +908,12 @@ void vcycle(PmgDev* d, CycleArgs<T>& c) {
   hipStream_t s = d->s;
   const int nt = d->ntot;
   DLevel* L = d->L;
+  d->kt[4] = sizeof(T);
+  d->kt[5] = (double)L[0].n;
+  const int ecyc = tbegin(d);
+  int e0 = tbegin(d);
   pmg::gsrb7<T>(s, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], 2, 0);
+  tend(d, e0, 0, pmg::gsrb7_launches(2));
   pmg::resid<T>(s, 7, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], c.w1);
   for (int l = 1; l < nt; l++) {
     pmg::restrict3<T>(s, L[l - 1].nx, L[l - 1].ny, L[l - 1].nz, L[l].nx, L[l].ny, L[l].nz, c.w1,
@@ -872,11 +932,14 @@ void vcycle(PmgDev* d, CycleArgs<T>& c) {
                         d->part, d->scal + 4);
     pmg::prolong_add<T>(s, Cl.nx, Cl.ny, Cl.nz, L[l].nx, L[l].ny, L[l].nz, c.X[l + 1], c.X[l],
                         d->scal + 4);
-    if (l == 0)
+    if (l == 0) {
+      e0 = tbegin(d);
       pmg::gsrb7<T>(s, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], 2, 1);
-    else
+      tend(d, e0, 0, pmg::gsrb7_launches(2));
+    } else
       pmg::gsrb27<T>(s, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l], 2, 1);
   }
+  tend(d, ecyc, 1, 1);
 }
 
 CycleArgs<double> args64(PmgDev* d, double* const* X, const double* f) {
@@ -948,8 +1011,8 @@ int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int i
     pmg::fill<float>(d->s, d->L[0].xfl, F.n, 0.0f);
     vcycle<float>(d, c);
     // X64 += (double) e
-    pmg::convert<float, double>(d->s, F.n, d->L[0].xfl, d->F);
-    pmg::axpy_interior<double>(d->s, F.nx, F.ny, F.nz, 1.0, d->F, X64[0]);
+    pmg::convert<float, double>(d->s, F.n, d->L[0].xfl, d->ecorr);
+    pmg::axpy_interior<double>(d->s, F.nx, F.ny, F.nz, 1.0, d->ecorr, X64[0]);
     it++;
     pmg::resid<double>(d->s, 7, F.nx, F.ny, F.nz, A0, F.D, f, X64[0], d->w1);
     pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, d->w1, d->part, d->scal);
@@ -1010,7 +1073,7 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
     double rr = 0;
     int its;
     if (d->precision == 1) {
-      sync_fp32(d);
+      sync_fp32(d, iters == 1);
       int outer = 0;
       mixed_solve(d, X, d->F, 0, 1000, errtol_s, &rr, &outer);
       its = outer;
@@ -1202,6 +1265,7 @@ extern "C" void Vpmg_dtor2(Vpmg* thee) {
     dfree(d->fdb);
     dfree(d->frc);
     if (d->hscal) HIPCHECK(hipHostFree(d->hscal));
+    for (hipEvent_t h : d->ev) HIPCHECK(hipEventDestroy(h));
     HIPCHECK(hipStreamDestroy(d->s));
     delete d;
     thee->dev = nullptr;
@@ -1417,7 +1481,7 @@ extern "C" int Vpmg_solve(Vpmg* thee) {
     HIPCHECK(hipStreamSynchronize(d->s));
     d->stats[6] = now() - tb;
     if (d->precision == 1) {
-      sync_fp32(d);
+      sync_fp32(d, true);
       const double tol = d->mixed_tol > 0 ? d->mixed_tol : p->errtol;
       iters = mixed_solve(d, X, d->L[0].fc, 1, p->itmax * 4, tol, &relres, &outer);
     } else {
@@ -1428,6 +1492,7 @@ extern "C" int Vpmg_solve(Vpmg* thee) {
   if (d->mirror)
     HIPCHECK(hipMemcpyAsync(thee->u, d->u, d->n * sizeof(double), hipMemcpyDeviceToHost, d->s));
   HIPCHECK(hipStreamSynchronize(d->s));
+  tcollect(d);
   d->stats[0] = iters;
   d->stats[1] = inner;
   d->stats[2] = d->ntot;
@@ -1890,5 +1955,25 @@ extern "C" int Vpmg_getStats(Vpmg* thee, double* out, int n) {
   for (int i = 0; i < m; i++) out[i] = d->stats[i];
   return m;
 }
+
+extern "C" int Vpmg_setKernelTiming(Vpmg* thee, int on) {
+  if (!thee) return 0;
+  PmgDev* d = DEV(thee);
+  d->timing = on ? 1 : 0;
+  for (double& v : d->kt) v = 0.0;
+  d->marks.clear();
+  d->ev_next = 0;
+  return 1;
+}
+
+extern "C" int Vpmg_getKernelTiming(Vpmg* thee, double* out, int n) {
+  if (!thee) return 0;
+  PmgDev* d = DEV(thee);
+  int m = std::min(n, 6);
+  for (int i = 0; i < m; i++) out[i] = d->kt[i];
+  return m;
+}
+
+extern "C" int apbs_pmg_synchronize(void) { return hipDeviceSynchronize() == hipSuccess; }
 
 extern "C" const char* apbs_pmg_version(void) { return "apbs_pmg 0.1 (gfx950)"; }

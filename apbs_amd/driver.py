@@ -164,6 +164,16 @@ class Calc:
                 "relres": out[3], "t_solve": out[4], "t_fillco": out[5], "t_build": out[6],
                 "outer": int(out[7])}
 
+    def kernel_timing(self, on=True):
+        """Start (zeroing the accumulators) or stop HIP-event kernel timing."""
+        lib().Vpmg_setKernelTiming(self.pmg, 1 if on else 0)
+
+    def kernel_times(self):
+        out = np.zeros(6)
+        lib().Vpmg_getKernelTiming(self.pmg, _dp(out), 6)
+        return {"smoother_ms": out[0], "smoother_launches": int(out[1]), "vcycle_ms": out[2],
+                "vcycles": int(out[3]), "word": int(out[4]), "points": int(out[5])}
+
     def release(self):
         L = lib()
         if getattr(self, "pmg", None):

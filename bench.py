@@ -1,0 +1,225 @@
+"""Benchmark: PB solves/s and p50/p99 latency, mixed precision vs FP64, at 161^3 npbe.
+
+Workload (the metric's configuration): nonlinear PB, Newton-multigrid, grid
+161^3 at 0.4 A, a seeded 5,000-atom globule in a 28x24x20 A ellipsoid with net
+charge -10 e, pdie 4, sdie 78.54, srfm smol, chgm spl2, bcfl mdh, 0.10 M and
+0.05 M monovalent salt.
+
+One step = one complete PB calculation on the device-resident Vpmg object:
+Vpmg_fillco (coefficient maps from the atom list) + Vpmg_solve (Newton
+multigrid to etol 1e-6) + Vpmg_energy.  The potential stays in HBM (no host
+mirror).  The atom list (120 KB) is uploaded inside fillco; that copy is part
+of the step.
+
+The JSON line reports the mixed-precision path as `value` (FP32 V-cycles in
+an FP64 defect-correction loop), the FP64 path beside it, the energy and
+potential error of mixed against FP64, the roofline of the dominant kernel
+(the fine-level red-black smoother) from HIP events recorded on the solver's
+stream during the timed steps, and the oracle (the FP64 CPU restatement of the
+reference) timed on a bounded sample on the host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "npbe161"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default=WORKLOAD)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--modes", default="fp64,mixed")
+    return ap.parse_args()
+
+
+def run_mode(mol, cfg, precision, steps, warmup):
+    from apbs_amd import driver, lib
+    L = lib()
+    pbe = dict(cfg["pbe"])
+    center = mol.center
+    m = driver.manual_mgparm([cfg["dime"]] * 3, center, grid=[cfg["h"]] * 3, chgm=pbe["chgm"])
+    calc = driver.Calc(mol, m, pbe, precision=precision, mixed_tol=-1.0, mirror=False)
+
+    def step():
+        calc.fillco()
+        calc.solve()
+        return calc.energy(1)
+
+    for _ in range(warmup):
+        step()
+    L.apbs_pmg_synchronize()
+    calc.kernel_timing(True)
+    lat = []
+    L.apbs_pmg_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        e = step()
+        lat.append(time.perf_counter() - ts)
+    L.apbs_pmg_synchronize()
+    total = time.perf_counter() - t0
+    kt = calc.kernel_times()
+    calc.kernel_timing(False)
+    stats = calc.stats()
+    lat_ms = np.array(lat) * 1e3
+    res = {
+        "solves_per_s": steps / total,
+        "ms_per_solve": total / steps * 1e3,
+        "p50_ms": float(np.percentile(lat_ms, 50)),
+        "p99_ms": float(np.percentile(lat_ms, 99)),
+        "energy_kT": e,
+        "newton_steps": stats["iters"],
+        "inner_cycles": stats["inner"],
+        "levels": stats["nlev"],
+        "relres": stats["relres"],
+        "kernel": kt,
+    }
+    return res, calc
+
+
+def smoother_roofline(kt):
+    """Fine-level red-black smoother: algorithmic bytes 7 words per point per
+    full sweep (D = oC + cc, oE, oN, uC, f, x in, x out); one colour launch is
+    half a sweep."""
+    if kt["smoother_launches"] == 0:
+        return None
+    per_launch_ms = kt["smoother_ms"] / kt["smoother_launches"]
+    bytes_per_launch = 3.5 * kt["word"] * kt["points"]
+    achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_gsrb7<%s>" % ("float" if kt["word"] == 4 else "double"),
+            "bytes_per_launch": bytes_per_launch, "avg_launch_us": per_launch_ms * 1e3}
+
+
+def vcycle_bandwidth(kt, cfg, levels):
+    """Achieved GB/s per V-cycle against the algorithmic V-cycle bytes
+    w * (44 N0 + 106 * sum of the intermediate levels' points); the coarsest
+    level (solved by a dense inverse) is excluded."""
+    if kt["vcycles"] == 0:
+        return None
+    n = cfg["dime"]
+    dims = [n]
+    for _ in range(levels - 1):
+        dims.append((dims[-1] - 1) // 2 + 1)
+    inter = sum(d ** 3 for d in dims[1:-1])
+    words = 44 * n ** 3 + 106 * inter
+    ms = kt["vcycle_ms"] / kt["vcycles"]
+    return {"ms_per_vcycle": ms, "algorithmic_GB": words * kt["word"] / 1e9,
+            "GBs": words * kt["word"] / (ms * 1e-3) / 1e9, "levels": dims}
+
+
+def cpu_baseline(cfg, newton_steps, sample_steps=3):
+    """The oracle (FP64 CPU restatement of the reference's pmgc path, one
+    host thread) on a bounded sample of the same workload: Vpmg_fillco in
+    full, then the first `sample_steps` Newton steps of the solve.  The solve
+    time is scaled to the full Newton step count of the FP64 GPU solve (the
+    same damped-inexact-Newton algorithm, one inner V-cycle per step at this
+    size)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc_driver as od
+    from apbs_amd import synth
+    atoms = synth.make(WORKLOAD)
+    c = od.mol_center(atoms[0])
+    pbe = dict(cfg["pbe"], newton_itmax=sample_steps)
+    nlev, dime = od.check_nlev([cfg["dime"]] * 3)
+    p = od.make_params(dime, [cfg["h"]] * 3, c, nlev, pbe)
+    o = od.OrcCalc(p, atoms)
+    t = time.perf_counter()
+    o.fillco()
+    t_fill = time.perf_counter() - t
+    t = time.perf_counter()
+    o.solve()
+    t_part = time.perf_counter() - t
+    done = o.stats()["iters"]
+    t_solve = t_part * newton_steps / done
+    total = t_fill + t_solve
+    return {"value": round(1.0 / total, 5), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"{WORKLOAD}: oracle fillco in full ({t_fill:.1f} s) + the first {done} "
+                      f"Newton steps ({t_part:.1f} s), solve time scaled to {newton_steps} "
+                      f"steps; one host thread"}
+
+
+def main():
+    a = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU only; a multi-GPU figure is not measured "
+              "(use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    from apbs_amd import driver, synth
+    cfg = synth.CONFIGS[a.workload]
+    xyz, q, r = synth.make(a.workload)
+    mol = driver.Molecule(xyz, q, r)
+    modes = a.modes.split(",")
+    out = {}
+    calcs = {}
+    for mode in modes:
+        res, calc = run_mode(mol, cfg, 1 if mode == "mixed" else 0, a.steps, a.warmup)
+        out[mode] = res
+        calcs[mode] = calc
+        print(f"# {mode}: {res['solves_per_s']:.2f} solves/s, p50 {res['p50_ms']:.2f} ms, "
+              f"p99 {res['p99_ms']:.2f} ms, newton {res['newton_steps']}, "
+              f"inner {res['inner_cycles']}, kernel {res['kernel']}", file=sys.stderr, flush=True)
+    head = "mixed" if "mixed" in out else modes[0]
+    err = None
+    if "mixed" in out and "fp64" in out:
+        e64, emx = out["fp64"]["energy_kT"], out["mixed"]["energy_kT"]
+        u64 = calcs["fp64"].array("u")
+        umx = calcs["mixed"].array("u")
+        d = umx - u64
+        err = {"energy_rel": abs(emx - e64) / abs(e64),
+               "potential_rel_L2": float(np.linalg.norm(d) / np.linalg.norm(u64)),
+               "potential_rel_Linf": float(np.abs(d).max() / np.abs(u64).max())}
+    kt = out[head]["kernel"]
+    line = {
+        "metric": "solves/s + p50/p99 latency, mixed vs FP64 PMG at 161^3 npbe; HBM GB/s",
+        "value": round(out[head]["solves_per_s"], 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(out[head]["ms_per_solve"], 3),
+        "p50_ms": round(out[head]["p50_ms"], 3),
+        "p99_ms": round(out[head]["p99_ms"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if head == "mixed" else "f64",
+        "data": "synthetic (seeded globule)",
+        "config": {"workload": f"{a.workload}: npbe 161^3 at 0.4 A, 5000 atoms, smol, spl2, mdh",
+                   "path": head, "step": "fillco + solve + energy"},
+        "modes": {k: {kk: v[kk] for kk in ("solves_per_s", "ms_per_solve", "p50_ms", "p99_ms",
+                                           "newton_steps", "inner_cycles", "levels", "relres",
+                                           "energy_kT")} for k, v in out.items()},
+        "mixed_vs_fp64": err,
+        "speedup_mixed_over_fp64": (out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+                                    if "mixed" in out and "fp64" in out else None),
+        "roofline": smoother_roofline(kt),
+        "roofline_fp64": smoother_roofline(out["fp64"]["kernel"]) if "fp64" in out else None,
+        "vcycle": vcycle_bandwidth(kt, cfg, out[head]["levels"]),
+    }
+    for c in calcs.values():
+        c.release()
+    if not a.no_cpu_baseline:
+        steps = out["fp64"]["newton_steps"] if "fp64" in out else out[head]["newton_steps"]
+        line["cpu_baseline"] = cpu_baseline(cfg, steps)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -55,6 +55,8 @@ typedef struct orc_params {
      * exactly with a dense Cholesky factorisation. */
     int coarse_smoother;
     int deep_coarse;
+    /* > 0: stop Newton after this many steps (a timing sample, not a solve) */
+    int newton_itmax;
 } orc_params;
 
 typedef struct orc_pmg orc_pmg;

@@ -69,6 +69,7 @@ typedef struct omg {
     int nion; double q[ORC_MAXION], sconc[ORC_MAXION];
     double errtol;
     int itmax, nu1, nu2;
+    int newton_itmax;
     int iters, inner_total; double relres;
 } omg;
 

@@ -991,7 +991,7 @@ int omg_solve_newton(omg *g, double *u)
         rsnrm = xnorm_new;
         (void)xnorm_den;
         if ((rsnrm / rsden) <= g->errtol) break;
-        if (iters >= g->itmax) break;
+        if (iters >= g->newton_itmax) break;
     }
     g->iters = iters;
     g->inner_total = inner_total;

@@ -829,6 +829,7 @@ int orc_solve(orc_pmg *m)
     g->deep = m->p.deep_coarse;
     g->errtol = (m->p.etol > 0) ? m->p.etol : 1.0e-6;
     g->itmax = 200; g->nu1 = 2; g->nu2 = 2;
+    g->newton_itmax = m->p.newton_itmax > 0 ? m->p.newton_itmax : g->itmax;
     /* Vpmg_ctor2 scales the bulk concentrations by 1/(2 I) (vpmg.c:340-377) */
     double zks2 = (m->ionstr > 0.0) ? 0.5 / m->ionstr : 0.0;
     g->nion = m->p.nion;

@@ -56,6 +56,7 @@ class OrcParams(ctypes.Structure):
         ("etol", ctypes.c_double),
         ("coarse_smoother", ctypes.c_int),
         ("deep_coarse", ctypes.c_int),
+        ("newton_itmax", ctypes.c_int),
     ]
 
 
@@ -221,6 +222,7 @@ def make_params(dime, grid, center, nlev, pbe):
     p.etol = pbe.get("etol", 0.0)
     p.coarse_smoother = pbe.get("coarse_smoother", 0)
     p.deep_coarse = pbe.get("deep_coarse", 0)
+    p.newton_itmax = pbe.get("newton_itmax", 0)
     return p
 
 

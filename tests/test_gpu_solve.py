@@ -1,0 +1,201 @@
+"""Full Vpmg calculations on the GPU (through the C ABI) against the oracle
+and against the reference's own golden energies.
+
+Parity protocol.  The GPU solver orders its coarse-level Gauss-Seidel sweeps
+by 8 colours and coarsens below the reference's level count, solving the
+coarsest grid exactly; the reference sweeps coarse levels lexicographically
+and stops at nlev with a banded / CG coarse solve.  Both converge to the same
+discrete solution, so:
+  * at a tight tolerance (etol 1e-12) both solves are converged and the GPU
+    potential and energy match the oracle's reference schedule to 1e-9
+    relative;
+  * against the oracle run with the GPU's own schedule (coarse_smoother 1,
+    deep_coarse 1) at the default etol, the iteration counts are equal and
+    energies agree to 1e-11;
+  * the reference's golden energies (computed at etol 1e-6) are met to 1e-6.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import orc_driver as od
+from apbs_amd import driver, synth
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+G = json.load(open(os.path.join(HERE, "golden", "reference_values.json")))
+
+
+def rel(a, b):
+    return abs(a - b) / abs(b)
+
+
+def pot_err(ug, uo):
+    """(relative L2, relative Linf) of ug against uo."""
+    d = ug - uo
+    return (np.linalg.norm(d) / np.linalg.norm(uo), np.abs(d).max() / np.abs(uo).max())
+
+
+def small_mol(n, radius, seed, net=None):
+    xyz, q, r, _ = synth.globule(n, (radius,) * 3, seed, net_charge=net)
+    return xyz, q, r
+
+
+SALT = [(1, 0.15, 2.0), (-1, 0.15, 2.0)]
+
+
+def pbe_of(srfm, chgm, **kw):
+    p = dict(pdie=2.0, sdie=78.54, srfm=srfm, chgm=chgm, srad=1.4, swin=0.3, sdens=10.0,
+             temp=298.15, bcfl="mdh", ions=SALT)
+    p.update(kw)
+    return p
+
+
+# ------------------------------------------------------------ coefficient maps
+
+@pytest.mark.parametrize("srfm,chgm", [("mol", "spl0"), ("smol", "spl2"), ("spl2", "spl2"),
+                                       ("smol", "spl0")])
+def test_fillco_maps(srfm, chgm):
+    """Vpmg_fillco: dielectric face maps, kappa and charge maps equal the
+    oracle's bitwise; MDH boundary values to 8 ulp of the largest value
+    (device exp/sqrt against glibc)."""
+    atoms = small_mol(80, 6.0, 11)
+    c = od.mol_center(atoms[0])
+    pbe = pbe_of(srfm, chgm)
+    mol = driver.Molecule(*atoms)
+    g = driver.Calc(mol, driver.manual_mgparm([41] * 3, c, grid=[0.45] * 3, chgm=chgm), pbe)
+    g.fillco()
+    o = od.OrcCalc(od.make_params([41] * 3, [0.45] * 3, c, 4, pbe), atoms)
+    o.fillco()
+    for name in ("epsx", "epsy", "epsz", "kappa", "charge"):
+        a, b = g.array(name), o.array(name)
+        assert np.array_equal(a, b), (name, np.abs(a - b).max())
+    for a, b in zip(g.boundary(), o.boundary()):
+        assert np.abs(a - b).max() <= 8 * np.finfo(float).eps * np.abs(b).max()
+
+
+# -------------------------------------------------------------- golden energies
+
+@pytest.mark.parametrize("srfm,key", [("smol", "born_smol_auto"), ("mol", "born_mol_auto")])
+def test_born_focusing_golden(srfm, key):
+    """examples/born/apbs-{s}mol-auto.in on the GPU: every focusing level's
+    energy against tests/test_cases.cfg."""
+    atoms = od.read_pqr(os.path.join(HERE, "golden", "born_ion.pqr"))
+    c = od.mol_center(atoms[0])
+    mol = driver.Molecule(*atoms)
+    last = {}
+    for state, sdie in (("solvated_levels", 78.54), ("reference_levels", 1.0)):
+        pbe = dict(pdie=1.0, sdie=sdie, srfm=srfm, chgm="spl2", srad=1.4, swin=0.3,
+                   sdens=10.0, temp=298.15, bcfl="mdh")
+        e, _ = driver.run_auto(mol, [65] * 3, [50] * 3, [12] * 3, c, c, pbe)
+        for got, want in zip(e, G[key][state]):
+            assert rel(got, want) < 1e-6, (state, got, want)
+        last[state] = e[-1]
+    dg = last["solvated_levels"] - last["reference_levels"]
+    assert rel(dg, G[key]["difference"]) < 1e-4
+
+
+@pytest.mark.parametrize("srfm", ["smol", "mol"])
+def test_solv_manual_golden(srfm):
+    """examples/solv/apbs-{s}mol.in on the GPU (chgm spl0, no salt)."""
+    out = []
+    for name in ("methanol", "methoxide"):
+        atoms = od.read_pqr(os.path.join(HERE, "golden", f"{name}.pqr"))
+        c = od.mol_center(atoms[0])
+        mol = driver.Molecule(*atoms)
+        for sdie in (78.0, 1.0):
+            pbe = dict(pdie=2.0, sdie=sdie, srfm=srfm, chgm="spl0", srad=0.0, swin=0.3,
+                       sdens=10.0, temp=300.0, bcfl="mdh", ions=[(1, 0.0, 2.0), (-1, 0.0, 2.0)])
+            e, _ = driver.run_manual(mol, [65] * 3, c, pbe, grid=[0.25] * 3)
+            out.append(e)
+    for got, want in zip(out, G["solv_" + srfm]["energies"]):
+        assert rel(got, want) < 1e-6, (got, want)
+
+
+# ------------------------------------------------------------ oracle parity
+
+def _pair(atoms, dime, h, pbe, orc_extra=None):
+    c = od.mol_center(atoms[0])
+    mol = driver.Molecule(*atoms)
+    eg, g = driver.run_manual(mol, [dime] * 3, c, pbe, grid=[h] * 3)
+    po = dict(pbe)
+    po.update(orc_extra or {})
+    eo, o = od.run_manual(atoms, [dime] * 3, c, po, grid=[h] * 3)
+    return eg, g, eo, o
+
+
+@pytest.mark.parametrize("npbe", [False, True])
+def test_tight_parity_reference_schedule(npbe):
+    """The 65^3 lpbe workload (500-atom globule, 0.5 A, spl2/spl2, 0.15 M) and
+    its npbe form, solved to etol 1e-12 on both sides: potential and energy
+    within 1e-9 relative of the oracle's reference-schedule solve."""
+    atoms = synth.make("lpbe65")
+    pbe = dict(synth.CONFIGS["lpbe65"]["pbe"], etol=1e-12, npbe=npbe)
+    eg, g, eo, o = _pair(atoms, 65, 0.5, pbe)
+    l2, linf = pot_err(g.array("u"), o.array("u"))
+    print(f"npbe={npbe}: energy gpu {eg:.12f} oracle {eo:.12f} rel {rel(eg, eo):.2e}; "
+          f"potential rel L2 {l2:.2e} Linf {linf:.2e}; stats {g.stats()}")
+    assert rel(eg, eo) < 1e-9
+    assert l2 < 1e-9 and linf < 1e-9
+
+
+@pytest.mark.parametrize("npbe", [False, True])
+def test_same_schedule_parity(npbe):
+    """Default etol; oracle with the GPU's schedule (8-colour coarse sweeps,
+    deep coarsening, exact coarsest solve): same iteration counts, energies
+    to 1e-11."""
+    atoms = small_mol(300, 9.0, 5, net=-3)
+    pbe = pbe_of("smol", "spl2", npbe=npbe)
+    eg, g, eo, o = _pair(atoms, 49, 0.45, pbe, dict(coarse_smoother=1, deep_coarse=1))
+    sg, so = g.stats(), o.stats()
+    print(f"npbe={npbe}: gpu {eg:.12f} ({sg}) oracle {eo:.12f} ({so})")
+    assert sg["iters"] == so["iters"] and sg["inner"] == so["inner"]
+    assert sg["nlev"] == so["nlev"]
+    assert rel(eg, eo) < 1e-11
+    l2, linf = pot_err(g.array("u"), o.array("u"))
+    assert linf < 1e-10
+
+
+# ----------------------------------------------------------------- forces
+
+def test_forces_against_oracle():
+    """Vpmg_qfForce / ibForce / dbForce (srfm spl2, chgm spl2) on a tightly
+    converged solve, per atom, against the oracle."""
+    atoms = small_mol(40, 5.0, 17, net=2)
+    pbe = pbe_of("spl2", "spl2", etol=1e-12)
+    eg, g, eo, o = _pair(atoms, 41, 0.4, pbe)
+    fmax = 0.0
+    diffs = []
+    for i in range(len(atoms[1])):
+        rg, *fg = g.force(i)
+        ro, *fo = o.force(i)
+        assert rg and ro
+        for a, b in zip(fg, fo):
+            fmax = max(fmax, np.abs(b).max())
+            diffs.append(np.abs(a - b).max())
+    print(f"force max |F| {fmax:.4e}, max diff {max(diffs):.2e}")
+    assert max(diffs) <= 1e-8 * fmax
+
+
+# ---------------------------------------------------------- mixed precision
+
+@pytest.mark.parametrize("npbe", [False, True])
+def test_mixed_precision_energy(npbe):
+    """FP32 V-cycles inside FP64 defect correction: energy within 1e-6 of the
+    FP64 GPU solve, potential errors stated."""
+    atoms = synth.make("lpbe65")
+    pbe = dict(synth.CONFIGS["lpbe65"]["pbe"], npbe=npbe)
+    c = od.mol_center(atoms[0])
+    mol = driver.Molecule(*atoms)
+    ref_pbe = dict(pbe, etol=1e-12)
+    e64, g64 = driver.run_manual(mol, [65] * 3, c, ref_pbe, grid=[0.5] * 3)
+    emx, gmx = driver.run_manual(mol, [65] * 3, c, pbe, grid=[0.5] * 3, precision=1,
+                                 mixed_tol=1e-10)
+    l2, linf = pot_err(gmx.array("u"), g64.array("u"))
+    print(f"npbe={npbe}: fp64 {e64:.10f} mixed {emx:.10f} rel {rel(emx, e64):.2e} "
+          f"L2 {l2:.2e} Linf {linf:.2e} stats {gmx.stats()}")
+    assert rel(emx, e64) < 1e-6
+    assert linf < 1e-6
