@@ -114,6 +114,36 @@ class Vpmg(ctypes.Structure):
     ]
 
 
+MAXION = 10
+
+
+class PmgRequest(ctypes.Structure):
+    """include/apbs_pmg_server.h PmgRequest"""
+    _fields_ = [
+        ("natoms", ctypes.c_int), ("xyz", _dp), ("radius", _dp), ("charge", _dp),
+        ("dime", ctypes.c_int * 3), ("grid", ctypes.c_double * 3),
+        ("center", ctypes.c_double * 3), ("npbe", ctypes.c_int), ("nion", ctypes.c_int),
+        ("ionq", ctypes.c_double * MAXION), ("ionc", ctypes.c_double * MAXION),
+        ("ionr", ctypes.c_double * MAXION),
+        ("pdie", ctypes.c_double), ("sdie", ctypes.c_double), ("srad", ctypes.c_double),
+        ("swin", ctypes.c_double), ("sdens", ctypes.c_double), ("temp", ctypes.c_double),
+        ("srfm", ctypes.c_int), ("chgm", ctypes.c_int), ("bcfl", ctypes.c_int),
+        ("etol", ctypes.c_double), ("precision", ctypes.c_int), ("mixed_tol", ctypes.c_double),
+        ("potential", _dp), ("tag", ctypes.c_long),
+    ]
+
+
+class PmgResult(ctypes.Structure):
+    """include/apbs_pmg_server.h PmgResult"""
+    _fields_ = [
+        ("id", ctypes.c_long), ("tag", ctypes.c_long), ("status", ctypes.c_int),
+        ("energy_kT", ctypes.c_double), ("iters", ctypes.c_int), ("inner", ctypes.c_int),
+        ("relres", ctypes.c_double), ("t_submit", ctypes.c_double),
+        ("t_start", ctypes.c_double), ("t_done", ctypes.c_double),
+        ("footprint", ctypes.c_size_t),
+    ]
+
+
 # (name, restype, argtypes) of every exported entry point
 _SIGS = [
     ("Valist_ctor", ctypes.POINTER(Valist), []),
@@ -186,6 +216,16 @@ _SIGS = [
     ("Vpmg_setKernelTiming", ctypes.c_int, [ctypes.POINTER(Vpmg), ctypes.c_int]),
     ("Vpmg_getKernelTiming", ctypes.c_int, [ctypes.POINTER(Vpmg), _dp, ctypes.c_int]),
     ("apbs_pmg_synchronize", ctypes.c_int, []),
+    ("PmgServer_ctor", ctypes.c_void_p, [ctypes.c_int, ctypes.c_size_t]),
+    ("PmgServer_submit", ctypes.c_long, [ctypes.c_void_p, ctypes.POINTER(PmgRequest)]),
+    ("PmgServer_poll", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PmgResult), ctypes.c_int]),
+    ("PmgServer_drain", None, [ctypes.c_void_p]),
+    ("PmgServer_stats", None, [ctypes.c_void_p, _dp]),
+    ("PmgServer_dtor", None, [ctypes.POINTER(ctypes.c_void_p)]),
+    ("PmgServer_footprint", ctypes.c_size_t, [ctypes.POINTER(PmgRequest)]),
+    ("apbs_pmg_now", ctypes.c_double, []),
+    ("apbs_pmg_poolStats", None, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
+                                  ctypes.POINTER(ctypes.c_size_t), ctypes.c_int]),
     ("apbs_pmg_version", ctypes.c_char_p, []),
 ]
 

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libapbs_pmg.so")
-SOURCES = ["pmg_device.hip", "fill_device.hip", "vpmg.cpp", "testhooks.cpp"]
+SOURCES = ["pmg_device.hip", "fill_device.hip", "pool.cpp", "vpmg.cpp", "server.cpp", "testhooks.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "--offload-arch=gfx950", "-ffp-contract=off", "-fPIC", "-std=c++17",

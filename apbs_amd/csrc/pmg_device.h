@@ -36,6 +36,14 @@ void gsrb7(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, c
            T* x, int nu, int iadjoint);
 // kernel launches one gsrb7 call of nu sweeps makes
 int gsrb7_launches(int nu);
+// The same nu sweeps with one fused red-black launch per sweep, ping-ponging
+// between x and tmp (fine-size scratch).  Only interior points are written,
+// so x and tmp must hold the same boundary values (zero in the V-cycle).
+// The result is bitwise that of gsrb7 and ends in x.
+template <class T>
+void gsrb7_fused(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D,
+                 const T* fc, T* x, T* tmp, int nu, int iadjoint);
+int gsrb7_fused_launches(int nu);
 template <class T>
 void gsrb27(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
             T* x, int nu, int iadjoint);

@@ -61,6 +61,81 @@ __global__ void __launch_bounds__(kBlock)
   x[p] = acc7(fc[p], oE, oN, uC, x, p, nx, nxy) / D[p];
 }
 
+// One complete red-black sweep, out of place: xo = sweep(xi).  In the 7-point
+// stencil every neighbour of a point has the other colour, so the first colour
+// F reads only old values of the second colour S, and S reads only new F
+// values.  A workgroup owns a TX x TY tile of (i, j) and marches over a chunk
+// of k planes: the new F values of plane k+1 (tile plus a one-point halo) go
+// to a three-plane LDS ring, then the S points of plane k are updated from the
+// ring and the whole tile row of plane k is written.  Each value is the
+// in-place sweep's value bit for bit (same terms, same order, no FMA).
+// Points outside the interior are not written.
+constexpr int kTX = 64, kTY = 4, kKC = 8;
+
+template <class T>
+__global__ void __launch_bounds__(kTX * kTY)
+    k_gsrb7_fused(int nx, int ny, int nz, const T* __restrict__ oE, const T* __restrict__ oN,
+                  const T* __restrict__ uC, const T* __restrict__ D, const T* __restrict__ fc,
+                  const T* __restrict__ xi, T* __restrict__ xo, int first) {
+  constexpr int RX = kTX + 2, RY = kTY + 2, RN = RX * RY;
+  __shared__ T ring[3][RN];
+  const int tid = threadIdx.x;
+  const int i0 = 1 + blockIdx.x * kTX, j0 = 1 + blockIdx.y * kTY;
+  const int k0 = 1 + blockIdx.z * kKC;
+  const int k1 = min(k0 + kKC, nz - 1);  // planes k0 .. k1-1
+  const size_t nxy = (size_t)nx * ny;
+  const int F = first;                   // parity of the first colour
+
+  // new F values of plane kk over the tile and its halo
+  auto stageF = [&](int kk) {
+    T* r = ring[kk % 3];
+    for (int e = tid; e < RN; e += kTX * kTY) {
+      const int i = i0 - 1 + e % RX, j = j0 - 1 + e / RX;
+      if (i > nx - 1 || j > ny - 1) continue;
+      if (((i + j + kk) & 1) != F) continue;
+      const size_t p = ((size_t)kk * ny + j) * nx + i;
+      T v;
+      if (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && kk >= 1 && kk <= nz - 2)
+        v = acc7(fc[p], oE, oN, uC, xi, p, nx, nxy) / D[p];
+      else
+        v = xi[p];
+      r[e] = v;
+    }
+  };
+
+  stageF(k0 - 1);
+  stageF(k0);
+  const int ti = tid % kTX, tj = tid / kTX;
+  const int i = i0 + ti, j = j0 + tj;
+  const int e = (tj + 1) * RX + (ti + 1);
+  const bool live = (i <= nx - 2) && (j <= ny - 2);
+  for (int k = k0; k < k1; k++) {
+    stageF(k + 1);
+    __syncthreads();
+    if (live) {
+      const size_t p = ((size_t)k * ny + j) * nx + i;
+      const T* rm = ring[(k - 1) % 3];
+      const T* r0 = ring[k % 3];
+      const T* rp = ring[(k + 1) % 3];
+      T v;
+      if (((i + j + k) & 1) == F) {
+        v = r0[e];
+      } else {
+        T acc = fc[p];
+        acc += oN[p] * r0[e + RX];
+        acc += oN[p - nx] * r0[e - RX];
+        acc += oE[p] * r0[e + 1];
+        acc += oE[p - 1] * r0[e - 1];
+        acc += uC[p - nxy] * rm[e];
+        acc += uC[p] * rp[e];
+        v = acc / D[p];
+      }
+      xo[p] = v;
+    }
+    __syncthreads();
+  }
+}
+
 // ----------------------------------------------------------------- 27-point
 template <class T>
 struct P27 {
@@ -721,6 +796,26 @@ Ops27<T> to27(const Ops<T>& A) {
 // =================================================================== wrappers
 
 template <class T>
+void gsrb7_fused(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
+                 T* x, T* tmp, int nu, int iadjoint) {
+  dim3 g((unsigned)((nx - 2 + kTX - 1) / kTX), (unsigned)((ny - 2 + kTY - 1) / kTY),
+         (unsigned)((nz - 2 + kKC - 1) / kKC));
+  const int first = iadjoint ? 0 : 1;
+  T* in = x;
+  T* out = tmp;
+  for (int it = 0; it < nu; it++) {
+    k_gsrb7_fused<T><<<g, kTX * kTY, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, in, out,
+                                             first);
+    T* t = in;
+    in = out;
+    out = t;
+  }
+  if (in != x) copy_interior<T>(s, nx, ny, nz, in, x);
+}
+
+int gsrb7_fused_launches(int nu) { return nu + (nu & 1); }
+
+template <class T>
 void gsrb7(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc, T* x,
            int nu, int iadjoint) {
   const int npair = (nx - 1) / 2;
@@ -895,7 +990,9 @@ void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int 
   template void axpy_interior<T>(hipStream_t, int, int, int, double, const T*, T*);             \
   template void copy_interior<T>(hipStream_t, int, int, int, const T*, T*);                     \
   template void add_diag<T>(hipStream_t, size_t, const T*, const T*, T*);                       \
-  template void dense_solve<T>(hipStream_t, int, int, int, const double*, int, const T*, T*);
+  template void dense_solve<T>(hipStream_t, int, int, int, const double*, int, const T*, T*);  \
+  template void gsrb7_fused<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*, T*, \
+                               T*, int, int);
 INST(double)
 INST(float)
 #undef INST

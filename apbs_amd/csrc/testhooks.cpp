@@ -54,9 +54,14 @@ int pmgtest_gsrb(int nx, int ny, int nz, int st, const double* a, const double* 
   OpsHolder A(a, n, st);
   double *dcc = up(cc, n), *dfc = up(fc, n), *dx = up(x, n), *dD = up(nullptr, n);
   pmg::add_diag<double>(0, n, A.ops.a[0], dcc, dD);
-  if (st == 7)
-    pmg::gsrb7<double>(0, nx, ny, nz, A.ops, dD, dfc, dx, nu, iadjoint);
-  else
+  if (st == 7) {
+    // the fused sweep kernel of the V-cycle; its partner buffer carries the
+    // boundary of x
+    double* dt = up(x, n);
+    pmg::gsrb7_fused<double>(0, nx, ny, nz, A.ops, dD, dfc, dx, dt, nu, iadjoint);
+    TCHECK(hipDeviceSynchronize());
+    TCHECK(hipFree(dt));
+  } else
     pmg::gsrb27<double>(0, nx, ny, nz, A.ops, dD, dfc, dx, nu, iadjoint);
   TCHECK(hipDeviceSynchronize());
   down(x, dx, n);

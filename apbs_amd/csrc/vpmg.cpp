@@ -24,6 +24,8 @@
 #include "../../include/apbs_pmg.h"
 #include "fill_device.h"
 #include "pmg_device.h"
+#include "pool.h"
+#include "vpmg_internal.h"
 
 #define HIPCHECK(x)                                                                      \
   do {                                                                                   \
@@ -53,17 +55,17 @@ double now() {
       .count();
 }
 
+// Device memory comes from the caching pool; every dfree follows a
+// synchronisation of the stream that last used the block.
 template <class T>
 T* dalloc(size_t n) {
-  T* p = nullptr;
   if (n == 0) n = 1;
-  HIPCHECK(hipMalloc(&p, n * sizeof(T)));
-  return p;
+  return (T*)pmgpool::dev_alloc(n * sizeof(T));
 }
 
 template <class T>
 void dfree(T*& p) {
-  if (p) HIPCHECK(hipFree(p));
+  if (p) pmgpool::dev_free((void*)p);
   p = nullptr;
 }
 
@@ -565,6 +567,8 @@ struct PmgDev {
   double *w1 = nullptr, *F = nullptr, *wn1 = nullptr, *wn2 = nullptr, *wn3 = nullptr,
          *nlw = nullptr;
   float* w1f = nullptr;
+  double* xs64 = nullptr;  // fine-level ping-pong partners of the smoother (zero boundary)
+  float* xs32 = nullptr;
   double* ecorr = nullptr;  // FP64 image of the FP32 correction (mixed solves)
   double *part = nullptr, *scal = nullptr;
   double* hscal = nullptr;  // pinned
@@ -592,7 +596,7 @@ struct PmgDev {
   };
   std::vector<Mark> marks;
   int ev_next = 0;    // next free event of the pool
-  double kt[6] = {};  // ms0, launches0, ms1, count1, word bytes, fine points
+  double kt[7] = {};  // ms0, launches0, ms1, count1, word bytes, fine points, words/pt/launch
 };
 
 PmgDev* DEV(Vpmg* p) { return (PmgDev*)p->dev; }
@@ -742,6 +746,8 @@ void free_hierarchy(PmgDev* d) {
   }
   dfree(d->w1);
   dfree(d->w1f);
+  dfree(d->xs64);
+  dfree(d->xs32);
   dfree(d->ecorr);
   dfree(d->F);
   dfree(d->wn1);
@@ -783,6 +789,45 @@ int plan_levels(int nx, int ny, int nz, int nlev, int dims[][3]) {
   }
   return nt;
 }
+
+}  // namespace
+
+size_t pmgint::footprint_bytes(int nx, int ny, int nz, int nlev, bool newton, bool mixed,
+                               int natoms, int nref) {
+  auto blk = [](double bytes) {  // pool rounding and reuse slack
+    return bytes < (1 << 20) ? bytes + 512.0 : bytes * 1.25 + (1 << 20);
+  };
+  const double n = (double)nx * ny * nz;
+  double b = 0;
+  // Vpmg_ctor: 8 fine maps, faces, axes, reduction scratch, atoms, forces
+  b += 8 * blk(8 * n);
+  b += blk(16.0 * ny * nz) + blk(16.0 * nx * nz) + blk(16.0 * nx * ny);
+  b += 3 * blk(8.0 * std::max(nx, std::max(ny, nz))) + blk(16.0 * pmg::kParts) + blk(256);
+  b += 5 * blk(8.0 * std::max(natoms, 1) * 3) + 4 * blk(24.0 * std::max(natoms, 1));
+  // Vpmg_fillco temporaries (released before the solve): SAS flags and
+  // points, or the three smoothing copies
+  const double sas = (double)natoms * nref;
+  const double fill = std::max(blk(sas) + blk(24 * sas), 3 * blk(8 * n)) + (64 << 20);
+  // hierarchy
+  int dims[24][3];
+  const int nt = plan_levels(nx, ny, nz, nlev, dims);
+  double h = 0;
+  for (int l = 0; l < nt; l++) {
+    const double m = (double)dims[l][0] * dims[l][1] * dims[l][2];
+    const int na = l == 0 ? 4 : 14;
+    h += (na + 4 + (l > 0 ? 1 : 0)) * blk(8 * m);     // ops, cc, fc, D, w0 (+x)
+    if (newton) h += 2 * blk(8 * m);                 // cprime, xtmp
+    if (mixed) h += (na - 1 + 3) * blk(4 * m);       // af, Df, xfl, w0f
+  }
+  h += 3 * blk(8 * n);                               // w1, F, xs64
+  if (newton) h += 4 * blk(8 * n);                   // wn1..3, nlw
+  if (mixed) h += 2 * blk(4 * n) + blk(8 * n);       // w1f, xs32, ecorr
+  const double dn = (double)(dims[nt - 1][0] - 2) * (dims[nt - 1][1] - 2) * (dims[nt - 1][2] - 2);
+  h += blk(8 * dn * dn) + blk(8 * dn);
+  return (size_t)(b + std::max(fill, h));
+}
+
+namespace {
 
 void alloc_hierarchy(PmgDev* d, int nlev, bool newton, bool mixed) {
   // A hierarchy of the same shape is reused: every array in it is written
@@ -826,9 +871,13 @@ void alloc_hierarchy(PmgDev* d, int nlev, bool newton, bool mixed) {
   }
   d->w1 = dalloc<double>(d->n);
   pmg::fill<double>(d->s, d->w1, d->n, 0.0);
+  d->xs64 = dalloc<double>(d->n);
+  pmg::fill<double>(d->s, d->xs64, d->n, 0.0);
   if (mixed) {
     d->w1f = dalloc<float>(d->n);
     pmg::fill<float>(d->s, d->w1f, d->n, 0.0f);
+    d->xs32 = dalloc<float>(d->n);
+    pmg::fill<float>(d->s, d->xs32, d->n, 0.0f);
     d->ecorr = dalloc<double>(d->n);
     pmg::fill<double>(d->s, d->ecorr, d->n, 0.0);
   }
@@ -901,6 +950,7 @@ struct CycleArgs {
   T* w0[24];
   const T* f;
   T* w1;
+  T* xs;  // fine-size smoother partner buffer
 };
 
 template <class T>
@@ -910,10 +960,11 @@ void vcycle(PmgDev* d, CycleArgs<T>& c) {
   DLevel* L = d->L;
   d->kt[4] = sizeof(T);
   d->kt[5] = (double)L[0].n;
+  d->kt[6] = 7.0;  // algorithmic words per fine point per fused sweep launch
   const int ecyc = tbegin(d);
   int e0 = tbegin(d);
-  pmg::gsrb7<T>(s, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], 2, 0);
-  tend(d, e0, 0, pmg::gsrb7_launches(2));
+  pmg::gsrb7_fused<T>(s, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], c.xs, 2, 0);
+  tend(d, e0, 0, pmg::gsrb7_fused_launches(2));
   pmg::resid<T>(s, 7, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], c.w1);
   for (int l = 1; l < nt; l++) {
     pmg::restrict3<T>(s, L[l - 1].nx, L[l - 1].ny, L[l - 1].nz, L[l].nx, L[l].ny, L[l].nz, c.w1,
@@ -934,8 +985,9 @@ void vcycle(PmgDev* d, CycleArgs<T>& c) {
                         d->scal + 4);
     if (l == 0) {
       e0 = tbegin(d);
-      pmg::gsrb7<T>(s, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], 2, 1);
-      tend(d, e0, 0, pmg::gsrb7_launches(2));
+      pmg::gsrb7_fused<T>(s, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], c.xs, 2,
+                          1);
+      tend(d, e0, 0, pmg::gsrb7_fused_launches(2));
     } else
       pmg::gsrb27<T>(s, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l], 2, 1);
   }
@@ -952,6 +1004,7 @@ CycleArgs<double> args64(PmgDev* d, double* const* X, const double* f) {
   }
   c.f = f;
   c.w1 = d->w1;
+  c.xs = d->xs64;
   return c;
 }
 
@@ -998,6 +1051,7 @@ int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int i
     c.w0[l] = d->L[l].w0f;
   }
   c.w1 = d->w1f;
+  c.xs = d->xs32;
   // r (FP64) -> FP32 rhs stored in L[0].w0f
   c.f = d->L[0].w0f;
   const pmg::Ops<double> A0 = F.ops();
@@ -1159,7 +1213,7 @@ extern "C" int Vpmg_ctor2(Vpmg* thee, Vpmgp* pmgp, Vpbe* pbe, int focusFlag, Vpm
     pmgp->nrwk += 2 * (size_t)pmgp->nf;
   PmgDev* d = new PmgDev();
   thee->dev = d;
-  HIPCHECK(hipStreamCreateWithFlags(&d->s, hipStreamNonBlocking));
+  d->s = pmgpool::stream_acquire();
   d->nx = pmgp->nx;
   d->ny = pmgp->ny;
   d->nz = pmgp->nz;
@@ -1182,7 +1236,7 @@ extern "C" int Vpmg_ctor2(Vpmg* thee, Vpmgp* pmgp, Vpbe* pbe, int focusFlag, Vpm
   d->zf = dalloc<double>(d->nz);
   d->part = dalloc<double>(2 * pmg::kParts);
   d->scal = dalloc<double>(32);
-  HIPCHECK(hipHostMalloc(&d->hscal, 32 * sizeof(double), hipHostMallocDefault));
+  d->hscal = pmgpool::pinned_scalars();
   Valist* al = pbe->alist;
   d->natoms = al->number;
   std::vector<double> pos(3 * (size_t)d->natoms), rad(d->natoms), chg(d->natoms);
@@ -1264,9 +1318,9 @@ extern "C" void Vpmg_dtor2(Vpmg* thee) {
     dfree(d->fib);
     dfree(d->fdb);
     dfree(d->frc);
-    if (d->hscal) HIPCHECK(hipHostFree(d->hscal));
+    pmgpool::pinned_release(d->hscal);
     for (hipEvent_t h : d->ev) HIPCHECK(hipEventDestroy(h));
-    HIPCHECK(hipStreamDestroy(d->s));
+    pmgpool::stream_release(d->s);
     delete d;
     thee->dev = nullptr;
   }
@@ -1737,7 +1791,8 @@ extern "C" double Vpmg_qmEnergy(Vpmg* thee, int extFlag) {
                  d->part, d->scal, nchop);
   double e = read_scalar(d, d->scal);
   int hchop = 0;
-  HIPCHECK(hipMemcpy(&hchop, nchop, sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpyAsync(&hchop, nchop, sizeof(int), hipMemcpyDeviceToHost, d->s));
+  HIPCHECK(hipStreamSynchronize(d->s));
   dfree(nchop);
   if (p->nonlin && hchop > 0) {
     // the reference aborts here (vpmg.c:1468-1472)
@@ -1969,7 +2024,7 @@ extern "C" int Vpmg_setKernelTiming(Vpmg* thee, int on) {
 extern "C" int Vpmg_getKernelTiming(Vpmg* thee, double* out, int n) {
   if (!thee) return 0;
   PmgDev* d = DEV(thee);
-  int m = std::min(n, 6);
+  int m = std::min(n, 7);
   for (int i = 0; i < m; i++) out[i] = d->kt[i];
   return m;
 }

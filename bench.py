@@ -91,16 +91,16 @@ def run_mode(mol, cfg, precision, steps, warmup):
 
 def smoother_roofline(kt):
     """Fine-level red-black smoother: algorithmic bytes 7 words per point per
-    full sweep (D = oC + cc, oE, oN, uC, f, x in, x out); one colour launch is
-    half a sweep."""
+    full sweep (D = oC + cc, oE, oN, uC, f, x in, x out); the library reports
+    the words per point one launch moves (7 for the fused sweep kernel)."""
     if kt["smoother_launches"] == 0:
         return None
     per_launch_ms = kt["smoother_ms"] / kt["smoother_launches"]
-    bytes_per_launch = 3.5 * kt["word"] * kt["points"]
+    bytes_per_launch = kt["words_per_point"] * kt["word"] * kt["points"]
     achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_gsrb7<%s>" % ("float" if kt["word"] == 4 else "double"),
+            "kernel": "k_gsrb7_fused<%s>" % ("float" if kt["word"] == 4 else "double"),
             "bytes_per_launch": bytes_per_launch, "avg_launch_us": per_launch_ms * 1e3}
 
 

@@ -26,7 +26,8 @@ def declared_symbols():
             # declarations: identifier followed by '(' at top level of a prototype
             names.add(name)
     # keep only names that look like API functions
-    return sorted(n for n in names if re.match(r"^(Vpmg|Vpmgp|Vpbe|Valist|MGparm|apbs_pmg|pmgtest)_", n))
+    return sorted(n for n in names
+                  if re.match(r"^(Vpmg|Vpmgp|Vpbe|Valist|MGparm|apbs_pmg|pmgtest|PmgServer)_", n))
 
 
 @pytest.fixture(scope="module")

@@ -1,6 +1,6 @@
 """Kernel-by-kernel parity of the HIP path against the oracle.
 
-Fine-level kernels (7-point red-black sweep, residuals, restriction,
+Fine-level kernels (the fused 7-point red-black sweep, residuals, restriction,
 prolongation, VbuildA_fv, Galerkin) evaluate the reference's expressions in
 the reference's order without FMA contraction, so the bar is bitwise
 equality.  The 8-colour 27-point sweep is compared with the oracle's
@@ -51,7 +51,8 @@ def interior(x, n3):
     return x.reshape(nz, ny, nx)[1:-1, 1:-1, 1:-1]
 
 
-@pytest.mark.parametrize("st,n3", [(7, (17, 19, 21)), (7, (33, 33, 33)), (27, (17, 17, 17)),
+@pytest.mark.parametrize("st,n3", [(7, (17, 19, 21)), (7, (33, 33, 33)), (7, (67, 13, 20)),
+                                   (7, (130, 6, 11)), (7, (5, 70, 3)), (27, (17, 17, 17)),
                                    (27, (9, 11, 13))])
 @pytest.mark.parametrize("adj", [0, 1])
 def test_gsrb_bitwise(L, O, st, n3, adj):
