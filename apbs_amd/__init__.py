@@ -29,21 +29,28 @@ _dp = ctypes.POINTER(ctypes.c_double)
 
 
 class Vatom(ctypes.Structure):
+    """struct sVatom (src/generic/vatom.h:84-100)"""
     _fields_ = [("position", ctypes.c_double * 3), ("radius", ctypes.c_double),
-                ("charge", ctypes.c_double), ("partID", ctypes.c_double), ("id", ctypes.c_int)]
+                ("charge", ctypes.c_double), ("partID", ctypes.c_double),
+                ("epsilon", ctypes.c_double), ("id", ctypes.c_int),
+                ("resName", ctypes.c_char * 64), ("atomName", ctypes.c_char * 64)]
 
 
 class Valist(ctypes.Structure):
+    """struct sValist (src/generic/valist.h:78-89)"""
     _fields_ = [("number", ctypes.c_int), ("center", ctypes.c_double * 3),
                 ("mincrd", ctypes.c_double * 3), ("maxcrd", ctypes.c_double * 3),
                 ("maxrad", ctypes.c_double), ("charge", ctypes.c_double),
-                ("atoms", ctypes.POINTER(Vatom))]
+                ("atoms", ctypes.POINTER(Vatom)), ("vmem", ctypes.c_void_p)]
 
 
 class MGparm(ctypes.Structure):
+    """struct sMGparm (src/generic/mgparm.h:112-183)"""
     _fields_ = [
         ("type", ctypes.c_int), ("parsed", ctypes.c_int),
         ("dime", ctypes.c_int * 3), ("setdime", ctypes.c_int),
+        ("chgm", ctypes.c_int), ("setchgm", ctypes.c_int),
+        ("chgs", ctypes.c_int),
         ("nlev", ctypes.c_int), ("setnlev", ctypes.c_int),
         ("etol", ctypes.c_double), ("setetol", ctypes.c_int),
         ("grid", ctypes.c_double * 3), ("setgrid", ctypes.c_int),
@@ -58,11 +65,14 @@ class MGparm(ctypes.Structure):
         ("setfgcent", ctypes.c_int),
         ("partDisjCenter", ctypes.c_double * 3), ("partDisjLength", ctypes.c_double * 3),
         ("partDisjOwnSide", ctypes.c_int * 6),
-        ("nonlintype", ctypes.c_int), ("setnonlin", ctypes.c_int),
+        ("pdime", ctypes.c_int * 3), ("setpdime", ctypes.c_int),
+        ("proc_rank", ctypes.c_int), ("setrank", ctypes.c_int),
+        ("proc_size", ctypes.c_int), ("setsize", ctypes.c_int),
+        ("ofrac", ctypes.c_double), ("setofrac", ctypes.c_int),
+        ("async", ctypes.c_int), ("setasync", ctypes.c_int),
+        ("nonlintype", ctypes.c_int), ("setnonlintype", ctypes.c_int),
         ("method", ctypes.c_int), ("setmethod", ctypes.c_int),
         ("useAqua", ctypes.c_int), ("setUseAqua", ctypes.c_int),
-        ("chgm", ctypes.c_int), ("setchgm", ctypes.c_int),
-        ("chgs", ctypes.c_int),
     ]
 
 

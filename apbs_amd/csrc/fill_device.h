@@ -66,8 +66,11 @@ void mol_smooth(hipStream_t s, const Geo& g, const double* a1, const double* a2,
 
 // ------------------------------------------------------------- boundaries
 // bcflnew (vpmg.c:3289-3367): Debye-Hueckel sum over all atoms on the faces.
+// Doubles of scratch bc_mdh needs for natoms atoms.
+size_t bc_mdh_scratch(const Geo& g, int natoms);
 void bc_mdh(hipStream_t s, const Geo& g, const double* xf, const double* yf, const double* zf,
-            const AtomsDev& A, double pre1, double xkappa, double* gx, double* gy, double* gz);
+            const AtomsDev& A, double pre1, double xkappa, double* part, double* gx, double* gy,
+            double* gz);
 // focusFillBound (vpmg.c:1914-2414): trilinear values of the old solution.
 struct OldGrid {
   int nx, ny, nz;

@@ -341,10 +341,17 @@ __global__ void k_sas_compact(AtomsDev A, const double* ref, int nref, double pr
 // mode 0: atom spheres r_a + add with the strict on-mesh test (dielectric),
 // mode 1: atom spheres r_a + add with the inflated on-mesh test (ions),
 // mode 2: SAS points with radius add (no mesh test).
+// One 64-lane wavefront per sphere; the lanes stride over the sphere's
+// bounding box.  The point set is the reference loop's: its row skip
+// ((dx2 + dy2) > rtot2) only drops points that also fail the final test,
+// because rounding is monotone.
+constexpr int kMarkWaves = kB / 64;
+
 __global__ void __launch_bounds__(kB)
     k_mark(Geo g, int npts, const double* pos, const double* rad, double add, int mode, double xs,
            double ys, double zs, double val, double* arr) {
-  const int a = blockIdx.x * kB + threadIdx.x;
+  const int a = blockIdx.x * kMarkWaves + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
   if (a >= npts) return;
   const double* tp = pos + 3 * a;
   double rtot = add;
@@ -373,19 +380,16 @@ __global__ void __launch_bounds__(kB)
   const int imax = imin2(g.nx - 1, (int)floor((posx + xr) / g.hx));
   const int jmax = imin2(g.ny - 1, (int)floor((posy + yr) / g.hy));
   const int kmax = imin2(g.nz - 1, (int)floor((posz + zr) / g.hz));
-  double fi = imin;
-  for (int i = imin; i <= imax; i++, fi += 1.) {
+  const int ni = imax - imin + 1, nj = jmax - jmin + 1, nk = kmax - kmin + 1;
+  if (ni <= 0 || nj <= 0 || nk <= 0) return;
+  const int nbox = ni * nj * nk;
+  for (int e = lane; e < nbox; e += 64) {
+    const int i = imin + e % ni, j = jmin + (e / ni) % nj, k = kmin + e / (ni * nj);
+    const double fi = i, fj = j, fk = k;
     const double dx2 = (posx - g.hx * fi) * (posx - g.hx * fi);
-    double fj = jmin;
-    for (int j = jmin; j <= jmax; j++, fj += 1.) {
-      const double dy2 = (posy - g.hy * fj) * (posy - g.hy * fj);
-      if ((dx2 + dy2) > rtot2) continue;
-      double fk = kmin;
-      for (int k = kmin; k <= kmax; k++, fk += 1.) {
-        const double dz2 = (posz - g.hz * fk) * (posz - g.hz * fk);
-        if ((dz2 + dy2 + dx2) <= rtot2) arr[IX(i, j, k, g.nx, g.ny)] = val;
-      }
-    }
+    const double dy2 = (posy - g.hy * fj) * (posy - g.hy * fj);
+    const double dz2 = (posz - g.hz * fk) * (posz - g.hz * fk);
+    if ((dz2 + dy2 + dx2) <= rtot2) arr[IX(i, j, k, g.nx, g.ny)] = val;
   }
 }
 
@@ -474,9 +478,14 @@ __device__ inline int face_count(const Geo& g, int f) {
   return f < 2 ? g.ny * g.nz : (f < 4 ? g.nx * g.nz : g.nx * g.ny);
 }
 
+// bcflnew / MDH (vpmg.c:3289-3367): the potential of every atom at every
+// boundary point.  Atoms are split into nch contiguous chunks; blockIdx.z is
+// the chunk, whose sequential partial sum goes to part[chunk][face][t], and
+// k_bc_mdh_sum adds the partials in chunk order.  With one chunk the sum is
+// the reference's atom-order sum.
 __global__ void __launch_bounds__(kB)
     k_bc_mdh(Geo g, const double* xf, const double* yf, const double* zf, AtomsDev A, double pre1,
-             double xkappa, double* gx, double* gy, double* gz) {
+             double xkappa, int nch, int mx, double* part) {
   const int f = blockIdx.y;
   const int t = blockIdx.x * kB + threadIdx.x;
   if (t >= face_count(g, f)) return;
@@ -484,8 +493,11 @@ __global__ void __launch_bounds__(kB)
   face_point(g, f, t, i, j, k);
   const double px = xf[i], py = yf[j], pz = zf[k];
   const double ec = 1.6021773e-19;  // Vunit_ec, src/generic/vunit.h:80
+  const int c = blockIdx.z;
+  const int per = (A.n + nch - 1) / nch;
+  const int a0 = c * per, a1 = min(A.n, a0 + per);
   double val = 0.0;
-  for (int a = 0; a < A.n; a++) {
+  for (int a = a0; a < a1; a++) {
     const double* ap = A.pos + 3 * a;
     const double q = ec * A.chg[a];
     const double dist = sqrt((px - ap[0]) * (px - ap[0]) + (py - ap[1]) * (py - ap[1]) +
@@ -495,6 +507,18 @@ __global__ void __launch_bounds__(kB)
     else
       val += pre1 * (q / dist);
   }
+  part[((size_t)c * 6 + f) * mx + t] = val;
+}
+
+__global__ void __launch_bounds__(kB)
+    k_bc_mdh_sum(Geo g, int nch, int mx, const double* part, double* gx, double* gy, double* gz) {
+  const int f = blockIdx.y;
+  const int t = blockIdx.x * kB + threadIdx.x;
+  if (t >= face_count(g, f)) return;
+  int i, j, k;
+  face_point(g, f, t, i, j, k);
+  double val = part[(size_t)f * mx + t];
+  for (int c = 1; c < nch; c++) val += part[((size_t)c * 6 + f) * mx + t];
   *face_slot(g, f, i, j, k, gx, gy, gz) = val;
 }
 
@@ -972,7 +996,8 @@ void mark_spheres(hipStream_t s, const Geo& g, int npts, const double* pos, cons
   const double xs = g.xmin + (which == 0 ? 0.5 * g.hx : 0.0);
   const double ys = g.ymin + (which == 1 ? 0.5 * g.hy : 0.0);
   const double zs = g.zmin + (which == 2 ? 0.5 * g.hz : 0.0);
-  k_mark<<<(npts + kB - 1) / kB, kB, 0, s>>>(g, npts, pos, rad, add, mode, xs, ys, zs, val, arr);
+  k_mark<<<(npts + kMarkWaves - 1) / kMarkWaves, kB, 0, s>>>(g, npts, pos, rad, add, mode, xs, ys,
+                                                              zs, val, arr);
 }
 
 void mol_smooth(hipStream_t s, const Geo& g, const double* a1, const double* a2, const double* a3,
@@ -980,12 +1005,25 @@ void mol_smooth(hipStream_t s, const Geo& g, const double* a1, const double* a2,
   k_mol_smooth<<<grid_all(g), kB, 0, s>>>(g, a1, a2, a3, epsx, epsy, epsz);
 }
 
+int bc_mdh_chunks(int natoms) { return natoms >= 4096 ? 16 : (natoms >= 512 ? natoms / 256 : 1); }
+
+size_t bc_mdh_scratch(const Geo& g, int natoms) {
+  size_t mx = (size_t)g.ny * g.nz;
+  if ((size_t)g.nx * g.nz > mx) mx = (size_t)g.nx * g.nz;
+  if ((size_t)g.nx * g.ny > mx) mx = (size_t)g.nx * g.ny;
+  return (size_t)bc_mdh_chunks(natoms) * 6 * mx;
+}
+
 void bc_mdh(hipStream_t s, const Geo& g, const double* xf, const double* yf, const double* zf,
-            const AtomsDev& A, double pre1, double xkappa, double* gx, double* gy, double* gz) {
+            const AtomsDev& A, double pre1, double xkappa, double* part, double* gx, double* gy,
+            double* gz) {
   int mx = g.ny * g.nz;
   if (g.nx * g.nz > mx) mx = g.nx * g.nz;
   if (g.nx * g.ny > mx) mx = g.nx * g.ny;
-  k_bc_mdh<<<dim3((mx + kB - 1) / kB, 6), kB, 0, s>>>(g, xf, yf, zf, A, pre1, xkappa, gx, gy, gz);
+  const int nch = bc_mdh_chunks(A.n);
+  k_bc_mdh<<<dim3((mx + kB - 1) / kB, 6, nch), kB, 0, s>>>(g, xf, yf, zf, A, pre1, xkappa, nch, mx,
+                                                           part);
+  k_bc_mdh_sum<<<dim3((mx + kB - 1) / kB, 6), kB, 0, s>>>(g, nch, mx, part, gx, gy, gz);
 }
 
 void focus_bound(hipStream_t s, const Geo& g, double xminN, double yminN, double zminN,

@@ -70,14 +70,15 @@ __global__ void __launch_bounds__(kBlock)
 // ring and the whole tile row of plane k is written.  Each value is the
 // in-place sweep's value bit for bit (same terms, same order, no FMA).
 // Points outside the interior are not written.
-constexpr int kTX = 64, kTY = 4, kKC = 8;
+constexpr int kTX = 64, kTY = 8, kKC = 16, kFT = 256;
 
 template <class T>
-__global__ void __launch_bounds__(kTX * kTY)
+__global__ void __launch_bounds__(kFT)
     k_gsrb7_fused(int nx, int ny, int nz, const T* __restrict__ oE, const T* __restrict__ oN,
                   const T* __restrict__ uC, const T* __restrict__ D, const T* __restrict__ fc,
                   const T* __restrict__ xi, T* __restrict__ xo, int first) {
   constexpr int RX = kTX + 2, RY = kTY + 2, RN = RX * RY;
+  constexpr int FPR = RX / 2, FN = FPR * RY;  // F points per region row / per plane
   __shared__ T ring[3][RN];
   const int tid = threadIdx.x;
   const int i0 = 1 + blockIdx.x * kTX, j0 = 1 + blockIdx.y * kTY;
@@ -86,37 +87,42 @@ __global__ void __launch_bounds__(kTX * kTY)
   const size_t nxy = (size_t)nx * ny;
   const int F = first;                   // parity of the first colour
 
-  // new F values of plane kk over the tile and its halo
+  // new F values of plane kk over the tile and its halo: one thread per F point
   auto stageF = [&](int kk) {
     T* r = ring[kk % 3];
-    for (int e = tid; e < RN; e += kTX * kTY) {
-      const int i = i0 - 1 + e % RX, j = j0 - 1 + e / RX;
+    for (int f = tid; f < FN; f += kFT) {
+      const int row = f / FPR, c = f % FPR;
+      const int j = j0 - 1 + row;
+      const int ib = i0 - 1;
+      const int i = ib + ((F - ib - j - kk) & 1) + 2 * c;
       if (i > nx - 1 || j > ny - 1) continue;
-      if (((i + j + kk) & 1) != F) continue;
       const size_t p = ((size_t)kk * ny + j) * nx + i;
       T v;
       if (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && kk >= 1 && kk <= nz - 2)
         v = acc7(fc[p], oE, oN, uC, xi, p, nx, nxy) / D[p];
       else
         v = xi[p];
-      r[e] = v;
+      r[row * RX + (i - ib)] = v;
     }
   };
 
   stageF(k0 - 1);
   stageF(k0);
-  const int ti = tid % kTX, tj = tid / kTX;
-  const int i = i0 + ti, j = j0 + tj;
-  const int e = (tj + 1) * RX + (ti + 1);
-  const bool live = (i <= nx - 2) && (j <= ny - 2);
+  const int ti = tid % kTX, tj = tid / kTX;  // rows tj and tj + kTY/2
+  const int i = i0 + ti;
   for (int k = k0; k < k1; k++) {
     stageF(k + 1);
     __syncthreads();
-    if (live) {
+    const T* rm = ring[(k - 1) % 3];
+    const T* r0 = ring[k % 3];
+    const T* rp = ring[(k + 1) % 3];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int jj = tj + h * (kTY / 2);
+      const int j = j0 + jj;
+      if (i > nx - 2 || j > ny - 2) continue;
+      const int e = (jj + 1) * RX + (ti + 1);
       const size_t p = ((size_t)k * ny + j) * nx + i;
-      const T* rm = ring[(k - 1) % 3];
-      const T* r0 = ring[k % 3];
-      const T* rp = ring[(k + 1) % 3];
       T v;
       if (((i + j + k) & 1) == F) {
         v = r0[e];
@@ -648,12 +654,16 @@ struct OutPtrs {
 __global__ void __launch_bounds__(kBlock)
     k_galerkin(int st, int nxf, int nyf, int nzf, Ops27<double> Af, int nxc, int nyc, int nzc,
                OutPtrs out) {
+  // 64 consecutive coarse points per wavefront, one stored direction s per
+  // wavefront: the pruned loop bounds below depend on s only, so the lanes
+  // of a wavefront never diverge.
   const long ni = nxc - 2, nj = nyc - 2, nk = nzc - 2;
-  const long tot = ni * nj * nk * 14;
+  const long nq = ni * nj * nk;
   const long t = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= tot) return;
-  const int s = (int)(t % 14);
-  const long q = t / 14;
+  const long w = t / 64;
+  const int s = (int)(w % 14);
+  const long q = (w / 14) * 64 + (t % 64);
+  if (q >= nq) return;
   const int I = 1 + (int)(q % ni), J = 1 + (int)((q / ni) % nj), K = 1 + (int)(q / (ni * nj));
   const int ex = c_fwd[s][0], ey = c_fwd[s][1], ez = c_fwd[s][2];
   const int I2 = I + ex, J2 = J + ey, K2 = K + ez;
@@ -804,8 +814,7 @@ void gsrb7_fused(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T
   T* in = x;
   T* out = tmp;
   for (int it = 0; it < nu; it++) {
-    k_gsrb7_fused<T><<<g, kTX * kTY, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, in, out,
-                                             first);
+    k_gsrb7_fused<T><<<g, kFT, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, in, out, first);
     T* t = in;
     in = out;
     out = t;
@@ -948,7 +957,8 @@ void galerkin(hipStream_t s, int st, int nxf, int nyf, int nzf, const Ops<double
               int nyc, int nzc, double* const ac[14]) {
   OutPtrs o;
   for (int k = 0; k < 14; k++) o.a[k] = ac[k];
-  const long tot = (long)(nxc - 2) * (nyc - 2) * (nzc - 2) * 14;
+  const long nq = (long)(nxc - 2) * (nyc - 2) * (nzc - 2);
+  const long tot = ((nq + 63) / 64) * 64 * 14;
   k_galerkin<<<(unsigned)((tot + kBlock - 1) / kBlock), kBlock, 0, s>>>(st, nxf, nyf, nzf, to27(Af),
                                                                         nxc, nyc, nzc, o);
 }

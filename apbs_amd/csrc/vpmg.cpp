@@ -155,16 +155,9 @@ extern "C" Vatom* Valist_getAtom(Valist* thee, int i) {
 
 // ===================================================================== Vpbe
 
-struct sVpbe {
-  Valist* alist;
-  double soluteCenter[3], soluteRadius, soluteXlen, soluteYlen, soluteZlen, soluteCharge;
-  int numIon;
-  double ionConc[APBS_PMG_MAXION], ionRadii[APBS_PMG_MAXION], ionQ[APBS_PMG_MAXION];
-  double bulkIonicStrength, maxIonRadius;
-  double T, soluteDiel, solventDiel, solventRadius, sdens;
-  double xkappa, deblen, zkappa2, zmagic;
-  double clistMaxRadius;  // Vclist max radius (vpbe.c:427-431)
-};
+namespace {
+std::vector<double> ref_sphere(int npts);
+}
 
 // Vpbe_ctor2 (vpbe.c:264-465)
 extern "C" Vpbe* Vpbe_ctor(Valist* alist, int ionNum, double* ionConc, double* ionRadii,
@@ -175,7 +168,7 @@ extern "C" Vpbe* Vpbe_ctor(Valist* alist, int ionNum, double* ionConc, double* i
   if (!alist || alist->number <= 0) return nullptr;
   const double N_A = 6.022045000e+23, e_c = 4.803242384e-10, k_B = 1.380662000e-16;
   const double pi = 4. * atan(1.);
-  sVpbe* t = (sVpbe*)calloc(1, sizeof(sVpbe));
+  Vpbe* t = (Vpbe*)calloc(1, sizeof(Vpbe));
   t->alist = alist;
   for (int d = 0; d < 3; d++) t->soluteCenter[d] = alist->center[d];
   double radius = 0;
@@ -228,7 +221,6 @@ extern "C" Vpbe* Vpbe_ctor(Valist* alist, int ionNum, double* ionConc, double* i
   t->soluteDiel = soluteDiel;
   t->solventDiel = solventDiel;
   t->solventRadius = solventRadius;
-  t->sdens = sdens;
   if (t->bulkIonicStrength == 0.) {
     t->xkappa = t->deblen = t->zkappa2 = 0.;
   } else {
@@ -238,14 +230,66 @@ extern "C" Vpbe* Vpbe_ctor(Valist* alist, int ionNum, double* ionConc, double* i
     t->zkappa2 = t->solventDiel * t->xkappa * t->xkappa;
   }
   t->zmagic = ((4.0 * pi * e_c * e_c) / (k_B * t->T)) * 1.0e+8;
-  t->clistMaxRadius =
+  t->ipkey = IPKEY_LPBE;
+  t->paramFlag = 1;
+  // cell-list radius and hash dimensions (vpbe.c:415-434, MAX_SPLINE_WINDOW
+  // 0.5, MAX_HASH_DIM 75); the cell list itself is built on the device
+  Vclist* cl = (Vclist*)calloc(1, sizeof(Vclist));
+  cl->alist = alist;
+  cl->max_radius =
       (t->maxIonRadius > t->solventRadius ? t->maxIonRadius : t->solventRadius) + 0.5;
+  const double len[3] = {t->soluteXlen, t->soluteYlen, t->soluteZlen};
+  cl->n = 1;
+  for (int d = 0; d < 3; d++) {
+    int nh = (int)(len[d] / 0.5);
+    cl->npts[d] = nh < 3 ? 3 : (nh > 75 ? 75 : nh);
+    cl->n *= cl->npts[d];
+  }
+  t->clist = cl;
+  // Vacc_storeParms (vacc.c:146-189): reference sphere of
+  // ceil(4 pi (max atom radius + clist radius)^2 sdens) points
+  Vacc* acc = (Vacc*)calloc(1, sizeof(Vacc));
+  acc->alist = alist;
+  acc->clist = cl;
+  double maxrad = 0.0;
+  for (int i = 0; i < alist->number; i++)
+    if (alist->atoms[i].radius > maxrad) maxrad = alist->atoms[i].radius;
+  maxrad = maxrad + cl->max_radius;
+  const int nsphere = (int)ceil(4.0 * VPI * maxrad * maxrad * sdens);
+  std::vector<double> ref = ref_sphere(nsphere);
+  VaccSurf* rs = (VaccSurf*)calloc(1, sizeof(VaccSurf));
+  rs->npts = (int)(ref.size() / 3);
+  rs->xpts = (double*)calloc(rs->npts + 1, sizeof(double));
+  rs->ypts = (double*)calloc(rs->npts + 1, sizeof(double));
+  rs->zpts = (double*)calloc(rs->npts + 1, sizeof(double));
+  rs->bpts = (char*)calloc(rs->npts + 1, 1);
+  for (int q = 0; q < rs->npts; q++) {
+    rs->xpts[q] = ref[3 * q];
+    rs->ypts[q] = ref[3 * q + 1];
+    rs->zpts[q] = ref[3 * q + 2];
+    rs->bpts[q] = 1;
+  }
+  rs->area = 4.0 * VPI;
+  acc->refSphere = rs;
+  t->acc = acc;
   return t;
 }
 
 extern "C" void Vpbe_dtor(Vpbe** thee) {
   if (thee && *thee) {
-    free(*thee);
+    Vpbe* t = *thee;
+    if (t->acc) {
+      if (VaccSurf* rs = t->acc->refSphere) {
+        free(rs->xpts);
+        free(rs->ypts);
+        free(rs->zpts);
+        free(rs->bpts);
+        free(rs);
+      }
+      free(t->acc);
+    }
+    free(t->clist);
+    free(t);
     *thee = nullptr;
   }
 }
@@ -584,6 +628,9 @@ struct PmgDev {
   // options / stats
   int precision = 0;
   double mixed_tol = -1.0;
+  // fine-level smoother: 0 fused out-of-place sweep, 1 one launch per colour
+  // (environment APBS_PMG_SMOOTHER=colour selects 1)
+  int smoother = 0;
   int mirror = 1;
   double stats[8] = {};
   pmg::Geo geo{};
@@ -830,8 +877,11 @@ size_t pmgint::footprint_bytes(int nx, int ny, int nz, int nlev, bool newton, bo
 namespace {
 
 void alloc_hierarchy(PmgDev* d, int nlev, bool newton, bool mixed) {
-  // A hierarchy of the same shape is reused: every array in it is written
-  // before it is read within a solve.
+  // Every array starts at zero, like the reference's zeroed workspace: the
+  // operator builders write interior entries only, and the 27-point stencil
+  // reads backward coefficients of the boundary layer (multiplied by the zero
+  // boundary of x).  A hierarchy of the same shape is reused by later solves;
+  // its interiors are rewritten before they are read.
   if (d->ntot > 0 && d->nlev_ref == nlev && d->h_newton == newton && d->h_mixed == mixed) return;
   free_hierarchy(d);
   d->h_newton = newton;
@@ -847,22 +897,33 @@ void alloc_hierarchy(PmgDev* d, int nlev, bool newton, bool mixed) {
     L.n = (size_t)L.nx * L.ny * L.nz;
     L.st = l == 0 ? 7 : 27;
     const int na = l == 0 ? 4 : 14;
-    for (int s = 0; s < na; s++) L.a[s] = dalloc<double>(L.n);
+    for (int s = 0; s < na; s++) {
+      L.a[s] = dalloc<double>(L.n);
+      pmg::fill<double>(d->s, L.a[s], L.n, 0.0);
+    }
     L.cc = dalloc<double>(L.n);
     L.fc = dalloc<double>(L.n);
     L.D = dalloc<double>(L.n);
+    pmg::fill<double>(d->s, L.cc, L.n, 0.0);
+    pmg::fill<double>(d->s, L.fc, L.n, 0.0);
+    pmg::fill<double>(d->s, L.D, L.n, 0.0);
     L.x = l == 0 ? d->u : dalloc<double>(L.n);
     L.w0 = dalloc<double>(L.n);
     pmg::fill<double>(d->s, L.w0, L.n, 0.0);
     if (l > 0) pmg::fill<double>(d->s, L.x, L.n, 0.0);
     if (newton) {
       L.cprime = dalloc<double>(L.n);
+      pmg::fill<double>(d->s, L.cprime, L.n, 0.0);
       L.xtmp = dalloc<double>(L.n);
       pmg::fill<double>(d->s, L.xtmp, L.n, 0.0);
     }
     if (mixed) {
-      for (int s = 1; s < na; s++) L.af[s] = dalloc<float>(L.n);  // oC is folded into Df
+      for (int s = 1; s < na; s++) {  // oC is folded into Df
+        L.af[s] = dalloc<float>(L.n);
+        pmg::fill<float>(d->s, L.af[s], L.n, 0.0f);
+      }
       L.Df = dalloc<float>(L.n);
+      pmg::fill<float>(d->s, L.Df, L.n, 0.0f);
       L.xfl = dalloc<float>(L.n);
       L.w0f = dalloc<float>(L.n);
       pmg::fill<float>(d->s, L.xfl, L.n, 0.0f);
@@ -888,6 +949,7 @@ void alloc_hierarchy(PmgDev* d, int nlev, bool newton, bool mixed) {
     d->wn2 = dalloc<double>(d->n);
     d->wn3 = dalloc<double>(d->n);
     d->nlw = dalloc<double>(d->n);
+    pmg::fill<double>(d->s, d->nlw, d->n, 0.0);
     pmg::fill<double>(d->s, d->wn1, d->n, 0.0);
     pmg::fill<double>(d->s, d->wn2, d->n, 0.0);
     pmg::fill<double>(d->s, d->wn3, d->n, 0.0);
@@ -960,11 +1022,21 @@ void vcycle(PmgDev* d, CycleArgs<T>& c) {
   DLevel* L = d->L;
   d->kt[4] = sizeof(T);
   d->kt[5] = (double)L[0].n;
-  d->kt[6] = 7.0;  // algorithmic words per fine point per fused sweep launch
+  // algorithmic words per fine point per smoother launch: 7 for a full sweep
+  d->kt[6] = d->smoother == 0 ? 7.0 : 3.5;
+  auto fine_smooth = [&](int adj) {
+    const int e0 = tbegin(d);
+    if (d->smoother == 0) {
+      pmg::gsrb7_fused<T>(s, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], c.xs, 2,
+                          adj);
+      tend(d, e0, 0, pmg::gsrb7_fused_launches(2));
+    } else {
+      pmg::gsrb7<T>(s, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], 2, adj);
+      tend(d, e0, 0, pmg::gsrb7_launches(2));
+    }
+  };
   const int ecyc = tbegin(d);
-  int e0 = tbegin(d);
-  pmg::gsrb7_fused<T>(s, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], c.xs, 2, 0);
-  tend(d, e0, 0, pmg::gsrb7_fused_launches(2));
+  fine_smooth(0);
   pmg::resid<T>(s, 7, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], c.w1);
   for (int l = 1; l < nt; l++) {
     pmg::restrict3<T>(s, L[l - 1].nx, L[l - 1].ny, L[l - 1].nz, L[l].nx, L[l].ny, L[l].nz, c.w1,
@@ -983,12 +1055,9 @@ void vcycle(PmgDev* d, CycleArgs<T>& c) {
                         d->part, d->scal + 4);
     pmg::prolong_add<T>(s, Cl.nx, Cl.ny, Cl.nz, L[l].nx, L[l].ny, L[l].nz, c.X[l + 1], c.X[l],
                         d->scal + 4);
-    if (l == 0) {
-      e0 = tbegin(d);
-      pmg::gsrb7_fused<T>(s, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], c.xs, 2,
-                          1);
-      tend(d, e0, 0, pmg::gsrb7_fused_launches(2));
-    } else
+    if (l == 0)
+      fine_smooth(1);
+    else
       pmg::gsrb27<T>(s, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l], 2, 1);
   }
   tend(d, ecyc, 1, 1);
@@ -1214,6 +1283,7 @@ extern "C" int Vpmg_ctor2(Vpmg* thee, Vpmgp* pmgp, Vpbe* pbe, int focusFlag, Vpm
   PmgDev* d = new PmgDev();
   thee->dev = d;
   d->s = pmgpool::stream_acquire();
+  if (const char* sm = getenv("APBS_PMG_SMOOTHER")) d->smoother = strcmp(sm, "colour") == 0 ? 1 : 0;
   d->nx = pmgp->nx;
   d->ny = pmgp->ny;
   d->nz = pmgp->nz;
@@ -1227,7 +1297,6 @@ extern "C" int Vpmg_ctor2(Vpmg* thee, Vpmgp* pmgp, Vpbe* pbe, int focusFlag, Vpm
   d->u = dalloc<double>(n);
   d->pvec = dalloc<double>(n);
   d->ccf = dalloc<double>(n);
-  pmg::fill<double>(d->s, d->u, n, 0.0);
   d->gx = dalloc<double>(2 * (size_t)d->ny * d->nz);
   d->gy = dalloc<double>(2 * (size_t)d->nx * d->nz);
   d->gz = dalloc<double>(2 * (size_t)d->nx * d->ny);
@@ -1236,6 +1305,16 @@ extern "C" int Vpmg_ctor2(Vpmg* thee, Vpmgp* pmgp, Vpbe* pbe, int focusFlag, Vpm
   d->zf = dalloc<double>(d->nz);
   d->part = dalloc<double>(2 * pmg::kParts);
   d->scal = dalloc<double>(32);
+  // device memory is recycled between objects: start every array at zero,
+  // as the reference's zeroed allocations do (the boundary values of
+  // bcfl zero, for one, are never written)
+  for (double* a : {d->epsx, d->epsy, d->epsz, d->kappa, d->charge, d->u, d->pvec, d->ccf})
+    pmg::fill<double>(d->s, a, n, 0.0);
+  pmg::fill<double>(d->s, d->gx, 2 * (size_t)d->ny * d->nz, 0.0);
+  pmg::fill<double>(d->s, d->gy, 2 * (size_t)d->nx * d->nz, 0.0);
+  pmg::fill<double>(d->s, d->gz, 2 * (size_t)d->nx * d->ny, 0.0);
+  pmg::fill<double>(d->s, d->part, 2 * pmg::kParts, 0.0);
+  pmg::fill<double>(d->s, d->scal, 32, 0.0);
   d->hscal = pmgpool::pinned_scalars();
   Valist* al = pbe->alist;
   d->natoms = al->number;
@@ -1250,6 +1329,8 @@ extern "C" int Vpmg_ctor2(Vpmg* thee, Vpmgp* pmgp, Vpbe* pbe, int focusFlag, Vpm
   d->achg = dalloc<double>(d->natoms);
   d->apart = dalloc<double>(d->natoms);
   d->atmp = dalloc<double>(d->natoms);
+  pmg::fill<double>(d->s, d->apart, d->natoms, 0.0);
+  pmg::fill<double>(d->s, d->atmp, d->natoms, 0.0);
   HIPCHECK(hipMemcpyAsync(d->apos, pos.data(), pos.size() * sizeof(double), hipMemcpyHostToDevice, d->s));
   HIPCHECK(hipMemcpyAsync(d->arad, rad.data(), rad.size() * sizeof(double), hipMemcpyHostToDevice, d->s));
   HIPCHECK(hipMemcpyAsync(d->achg, chg.data(), chg.size() * sizeof(double), hipMemcpyHostToDevice, d->s));
@@ -1436,10 +1517,16 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
         pmg::mark_spheres(d->s, g, d->natoms, d->apos, d->arad, srad, 0, w, epsp, maps[w]);
       }
       if (srad > VSMALL) {
-        double maxrad = rmax + pbe->clistMaxRadius;
-        int nsphere = (int)ceil(4.0 * VPI * maxrad * maxrad * pbe->sdens);
-        std::vector<double> ref = ref_sphere(nsphere);
-        const int nref = (int)(ref.size() / 3);
+        // the accessibility object's reference sphere (vacc.c:146-189), read
+        // from the Vpbe so a Vpbe built by the reference is honoured
+        const VaccSurf* rs = pbe->acc->refSphere;
+        const int nref = rs->npts;
+        std::vector<double> ref(3 * (size_t)nref);
+        for (int q = 0; q < nref; q++) {
+          ref[3 * q] = rs->xpts[q];
+          ref[3 * q + 1] = rs->ypts[q];
+          ref[3 * q + 2] = rs->zpts[q];
+        }
         double* dref = dalloc<double>(ref.size());
         HIPCHECK(hipMemcpyAsync(dref, ref.data(), ref.size() * sizeof(double),
                                 hipMemcpyHostToDevice, d->s));
@@ -1489,7 +1576,10 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
     if (p->bcfl == BCFL_MDH) {
       const double pre1 =
           ((Vunit_ec) / (4 * VPI * Vunit_eps0 * epsw * Vunit_kb * pbe->T)) * (1.0e10);
-      pmg::bc_mdh(d->s, g, d->xf, d->yf, d->zf, A, pre1, pbe->xkappa, d->gx, d->gy, d->gz);
+      double* part = dalloc<double>(pmg::bc_mdh_scratch(g, d->natoms));
+      pmg::bc_mdh(d->s, g, d->xf, d->yf, d->zf, A, pre1, pbe->xkappa, part, d->gx, d->gy, d->gz);
+      HIPCHECK(hipStreamSynchronize(d->s));
+      dfree(part);
     } else if (p->bcfl != BCFL_ZERO) {
       fprintf(stderr, "Vpmg_fillco: bcfl %d is not supported (zero, mdh, focus are)\n",
               (int)p->bcfl);

@@ -11,12 +11,17 @@ multigrid to etol 1e-6) + Vpmg_energy.  The potential stays in HBM (no host
 mirror).  The atom list (120 KB) is uploaded inside fillco; that copy is part
 of the step.
 
-The JSON line reports the mixed-precision path as `value` (FP32 V-cycles in
-an FP64 defect-correction loop), the FP64 path beside it, the energy and
-potential error of mixed against FP64, the roofline of the dominant kernel
-(the fine-level red-black smoother) from HIP events recorded on the solver's
-stream during the timed steps, and the oracle (the FP64 CPU restatement of the
-reference) timed on a bounded sample on the host.
+Two paths are timed for each precision (FP64, and mixed: FP32 V-cycles in an
+FP64 defect-correction loop):
+  * single: one request at a time on one Vpmg (steps timed after warmup);
+  * served: --served requests submitted at once to the continuous-batching
+    service with --slots calculations in flight (closed loop).
+`value` is the served mixed-precision solves/s (p50/p99 are its request
+latencies); the single-request figures, the energy and potential error of
+mixed against FP64, the roofline of the dominant kernel (the fine-level
+red-black smoother, from HIP events recorded on the solver's stream during
+the single-request mixed run) and the oracle (the FP64 CPU restatement of the
+reference) timed on a bounded sample on the host are reported beside it.
 """
 import argparse
 import json
@@ -41,6 +46,8 @@ def parse():
     ap.add_argument("--workload", default=WORKLOAD)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--slots", type=int, default=8, help="concurrent requests of the served path")
+    ap.add_argument("--served", type=int, default=48, help="requests in the served run (0: none)")
     return ap.parse_args()
 
 
@@ -100,7 +107,8 @@ def smoother_roofline(kt):
     achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_gsrb7_fused<%s>" % ("float" if kt["word"] == 4 else "double"),
+            "kernel": "%s<%s>" % ("k_gsrb7_fused" if kt["words_per_point"] == 7 else "k_gsrb7",
+                                  "float" if kt["word"] == 4 else "double"),
             "bytes_per_launch": bytes_per_launch, "avg_launch_us": per_launch_ms * 1e3}
 
 
@@ -152,6 +160,30 @@ def cpu_baseline(cfg, newton_steps, sample_steps=3):
                       f"steps; one host thread"}
 
 
+def run_served(xyz, q, r, cfg, precision, nreq, slots, warmup):
+    """The served path: nreq requests of the workload submitted at once to a
+    PmgServer with `slots` concurrent calculations (closed loop); solves/s is
+    nreq over the time from the first submission to the last completion,
+    latency is completion minus submission per request."""
+    from apbs_amd import serve
+    pbe = dict(cfg["pbe"])
+    from apbs_amd import driver
+    center = driver.Molecule(xyz, q, r).center
+    spec = dict(xyz=xyz, q=q, r=r, dime=(cfg["dime"],) * 3, grid=[cfg["h"]] * 3, center=center,
+                pbe=pbe)
+    srv = serve.Server(slots=slots)
+    serve.run_stream(srv, [spec] * max(warmup, slots), rate=0, precision=precision)
+    res, wall = serve.run_stream(srv, [spec] * nreq, rate=0, precision=precision)
+    st = srv.stats()
+    srv.close()
+    summ = serve.latency_summary(res, wall)
+    summ["energy_kT"] = res[0]["energy_kT"]
+    summ["peak_admitted_GB"] = st["peak_admitted_bytes"] / 1e9
+    summ["peak_pool_GB"] = st["peak_pool_bytes"] / 1e9
+    summ["slots"] = slots
+    return summ
+
+
 def main():
     a = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -173,6 +205,12 @@ def main():
         print(f"# {mode}: {res['solves_per_s']:.2f} solves/s, p50 {res['p50_ms']:.2f} ms, "
               f"p99 {res['p99_ms']:.2f} ms, newton {res['newton_steps']}, "
               f"inner {res['inner_cycles']}, kernel {res['kernel']}", file=sys.stderr, flush=True)
+    served = {}
+    if a.served > 0:
+        for mode in modes:
+            served[mode] = run_served(xyz, q, r, cfg, 1 if mode == "mixed" else 0, a.served,
+                                      a.slots, a.warmup)
+            print(f"# served {mode}: {served[mode]}", file=sys.stderr, flush=True)
     head = "mixed" if "mixed" in out else modes[0]
     err = None
     if "mixed" in out and "fp64" in out:
@@ -184,23 +222,35 @@ def main():
                "potential_rel_L2": float(np.linalg.norm(d) / np.linalg.norm(u64)),
                "potential_rel_Linf": float(np.abs(d).max() / np.abs(u64).max())}
     kt = out[head]["kernel"]
+    top = served.get(head)
+    if top is not None:
+        value, ms, p50, p99 = (top["solves_per_s"], 1e3 / top["solves_per_s"], top["p50_ms"],
+                               top["p99_ms"])
+    else:
+        value, ms, p50, p99 = (out[head]["solves_per_s"], out[head]["ms_per_solve"],
+                               out[head]["p50_ms"], out[head]["p99_ms"])
     line = {
         "metric": "solves/s + p50/p99 latency, mixed vs FP64 PMG at 161^3 npbe; HBM GB/s",
-        "value": round(out[head]["solves_per_s"], 3),
+        "value": round(value, 3),
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": a.steps,
+        "steps": a.served if top is not None else a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(out[head]["ms_per_solve"], 3),
-        "p50_ms": round(out[head]["p50_ms"], 3),
-        "p99_ms": round(out[head]["p99_ms"], 3),
+        "ms_per_step": round(ms, 3),
+        "p50_ms": round(p50, 3),
+        "p99_ms": round(p99, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64" if head == "mixed" else "f64",
         "data": "synthetic (seeded globule)",
         "config": {"workload": f"{a.workload}: npbe 161^3 at 0.4 A, 5000 atoms, smol, spl2, mdh",
-                   "path": head, "step": "fillco + solve + energy"},
+                   "path": f"{head}, served ({a.slots} slots)" if top is not None else head,
+                   "step": "one request: fillco + solve + energy"},
+        "served": served,
+        "speedup_served_mixed_over_fp64": (served["mixed"]["solves_per_s"]
+                                           / out["fp64"]["solves_per_s"]
+                                           if "mixed" in served and "fp64" in out else None),
         "modes": {k: {kk: v[kk] for kk in ("solves_per_s", "ms_per_solve", "p50_ms", "p99_ms",
                                            "newton_steps", "inner_cycles", "levels", "relres",
                                            "energy_kT")} for k, v in out.items()},

@@ -10,10 +10,10 @@
  * replaces.  Plain pointers and sizes only; all device memory is owned by the
  * library.  Return codes: 1 = success, 0 = failure, as in the reference.
  *
- * The physics objects the Vpmg constructor borrows (Valist, Vpbe, MGparm)
- * are this library's own types with the fields the PMG path reads; their
- * constructors keep the reference signatures where the reference's own
- * arguments are plain data.
+ * The objects the Vpmg constructor borrows (Valist, Vpbe, MGparm) have the
+ * reference's memory layout, so objects built by APBS's own src/generic code
+ * can be passed in.  The library also supplies constructors for them with
+ * the reference signatures, for callers without the rest of APBS.
  */
 #ifndef APBS_PMG_H
 #define APBS_PMG_H
@@ -57,16 +57,21 @@ typedef struct sVmem Vmem;   /* opaque; unused by this library */
 typedef struct sVgrid Vgrid; /* opaque; external maps are not supported */
 
 /* ------------------------------------------------------------------ atoms
- * struct sVatom (src/generic/vatom.h:74-90) without the WCA/name fields. */
+ * struct sVatom (src/generic/vatom.h:84-100) and struct sValist
+ * (src/generic/valist.h:78-89), byte for byte: a Valist built by APBS's own
+ * src/generic code can be handed to this library. */
+#define APBS_PMG_RECLEN 64   /* src/generic/vatom.h:77 VMAX_RECLEN */
 typedef struct sVatom {
     double position[3];
     double radius;
     double charge;
     double partID;
+    double epsilon;
     int id;
+    char resName[APBS_PMG_RECLEN];
+    char atomName[APBS_PMG_RECLEN];
 } Vatom;
 
-/* struct sValist (src/generic/valist.h:78-89) */
 typedef struct sValist {
     int number;
     double center[3];
@@ -75,6 +80,7 @@ typedef struct sValist {
     double maxrad;
     double charge;
     Vatom *atoms;
+    Vmem *vmem;
 } Valist;
 
 /* Valist_ctor / Valist_dtor (src/generic/valist.c:122-186) */
@@ -89,43 +95,114 @@ int Valist_setAtoms(Valist *thee, int n, const double *xyz, const double *radius
                     const double *charge);
 /* Valist_getStatistics (src/generic/valist.c:869-921) */
 int Valist_getStatistics(Valist *thee);
-int Valist_getNumberAtoms(Valist *thee);        /* valist.h:173 */
-Vatom *Valist_getAtom(Valist *thee, int i);      /* valist.h:181 */
+int Valist_getNumberAtoms(Valist *thee);        /* valist.h:141 */
+Vatom *Valist_getAtom(Valist *thee, int i);      /* valist.h:150 */
 
-/* ------------------------------------------------------------------- Vpbe */
-typedef struct sVpbe Vpbe;
-/* Vpbe_ctor (src/generic/vpbe.c:246-262, body :264-465).  The cell list and
- * accessibility object of the reference are built on the device when needed. */
+/* ------------------------------------------------------------------- Vpbe
+ * struct sVpbe (src/generic/vpbe.h:84-137) byte for byte, and the leading
+ * fields of struct sVclist (src/generic/vclist.h:118-130), struct sVacc
+ * (src/generic/vacc.h:109-125) and struct sVaccSurf (vacc.h:84-95) up to the
+ * last field this library reads (Vclist max_radius, the Vacc reference
+ * sphere).  A Vpbe built by APBS's own Vpbe_ctor can therefore be passed to
+ * Vpmg_ctor; the library never reads past these fields. */
+typedef struct sVaccSurf {
+    Vmem *mem;
+    double *xpts, *ypts, *zpts;
+    char *bpts;
+    double area;
+    int npts;
+    double probe_radius;
+} VaccSurf;
+
+typedef struct sVclist {
+    Vmem *vmem;
+    Valist *alist;
+    int mode;               /* Vclist_DomainMode */
+    int npts[3];
+    int n;
+    double max_radius;
+    /* ... further fields of the reference are not read */
+} Vclist;
+
+typedef struct sVacc {
+    Vmem *mem;
+    Valist *alist;
+    Vclist *clist;
+    int *atomFlags;
+    VaccSurf *refSphere;
+    /* ... further fields of the reference are not read */
+} Vacc;
+
+typedef struct sVpbe {
+    Vmem *vmem;
+    Valist *alist;
+    Vclist *clist;
+    Vacc *acc;
+    double T;
+    double soluteDiel;
+    double solventDiel;
+    double solventRadius;
+    double bulkIonicStrength;
+    double maxIonRadius;
+    int numIon;
+    double ionConc[APBS_PMG_MAXION];
+    double ionRadii[APBS_PMG_MAXION];
+    double ionQ[APBS_PMG_MAXION];
+    double xkappa;
+    double deblen;
+    double zkappa2;
+    double zmagic;
+    double soluteCenter[3];
+    double soluteRadius;
+    double soluteXlen;
+    double soluteYlen;
+    double soluteZlen;
+    double soluteCharge;
+    double smvolume;
+    double smsize;
+    int ipkey;
+    int paramFlag;
+    double z_mem;
+    double L;
+    double membraneDiel;
+    double V;
+    int param2Flag;
+} Vpbe;
+
+/* Vpbe_ctor (src/generic/vpbe.c:246-262, body :264-465): also builds the
+ * cell-list radius and the reference sphere of vpbe.c:415-439 and
+ * vacc.c:146-189. */
 Vpbe *Vpbe_ctor(Valist *alist, int ionNum, double *ionConc, double *ionRadii, double *ionQ,
                 double T, double soluteDiel, double solventDiel, double solventRadius,
                 int focusFlag, double sdens, double z_mem, double L, double membraneDiel,
                 double V);
 void Vpbe_dtor(Vpbe **thee);
-double Vpbe_getBulkIonicStrength(Vpbe *thee);   /* vpbe.h:268 */
-double Vpbe_getMaxIonRadius(Vpbe *thee);        /* vpbe.h:277 */
-double Vpbe_getTemperature(Vpbe *thee);         /* vpbe.h:284 */
-double Vpbe_getSoluteDiel(Vpbe *thee);          /* vpbe.h:291 */
-double Vpbe_getSolventDiel(Vpbe *thee);         /* vpbe.h:316 */
-double Vpbe_getSolventRadius(Vpbe *thee);       /* vpbe.h:323 */
-double Vpbe_getXkappa(Vpbe *thee);              /* vpbe.h:330 */
-double Vpbe_getZkappa2(Vpbe *thee);             /* vpbe.h:344 */
-double Vpbe_getZmagic(Vpbe *thee);              /* vpbe.h:351 */
-double Vpbe_getSoluteRadius(Vpbe *thee);        /* vpbe.h:358 */
-double Vpbe_getSoluteCharge(Vpbe *thee);        /* vpbe.h:393 */
-double *Vpbe_getSoluteCenter(Vpbe *thee);       /* vpbe.h:300 */
-Valist *Vpbe_getValist(Vpbe *thee);             /* vpbe.h:251 */
+double Vpbe_getBulkIonicStrength(Vpbe *thee);   /* vpbe.h:174 */
+double Vpbe_getMaxIonRadius(Vpbe *thee);        /* vpbe.h:182 */
+double Vpbe_getTemperature(Vpbe *thee);         /* vpbe.h:190 */
+double Vpbe_getSoluteDiel(Vpbe *thee);          /* vpbe.h:198 */
+double Vpbe_getSolventDiel(Vpbe *thee);         /* vpbe.h:262 */
+double Vpbe_getSolventRadius(Vpbe *thee);       /* vpbe.h:270 */
+double Vpbe_getXkappa(Vpbe *thee);              /* vpbe.h:278 */
+double Vpbe_getZkappa2(Vpbe *thee);             /* vpbe.h:294 */
+double Vpbe_getZmagic(Vpbe *thee);              /* vpbe.h:302 */
+double Vpbe_getSoluteRadius(Vpbe *thee);        /* vpbe.h:214 */
+double Vpbe_getSoluteCharge(Vpbe *thee);        /* vpbe.h:254 */
+double *Vpbe_getSoluteCenter(Vpbe *thee);       /* vpbe.h:246 */
+Valist *Vpbe_getValist(Vpbe *thee);             /* vpbe.h:158 */
 int Vpbe_getIons(Vpbe *thee, int *nion, double ionConc[APBS_PMG_MAXION],
                  double ionRadii[APBS_PMG_MAXION], double ionQ[APBS_PMG_MAXION]);
 
 /* ----------------------------------------------------------------- MGparm
- * The fields of struct sMGparm (src/generic/mgparm.h:112-183) that the PMG
- * path and the mg-manual / mg-auto set-up read. */
+ * struct sMGparm (src/generic/mgparm.h:112-183) byte for byte. */
 typedef struct sMGparm {
     MGparm_CalcType type;
     int parsed;
-    int dime[3];  int setdime;
-    int nlev;     int setnlev;
-    double etol;  int setetol;
+    int dime[3];      int setdime;
+    Vchrg_Meth chgm;  int setchgm;
+    Vchrg_Src chgs;
+    int nlev;         int setnlev;
+    double etol;      int setetol;
     double grid[3];   int setgrid;
     double glen[3];   int setglen;
     MGparm_CentMeth cmeth;   double center[3];  int centmol;  int setgcent;
@@ -136,15 +213,18 @@ typedef struct sMGparm {
     double partDisjCenter[3];
     double partDisjLength[3];
     int partDisjOwnSide[6];
-    Vhal_NonlinType nonlintype;  int setnonlin;
-    Vsol_Meth method;            int setmethod;
-    int useAqua;                 int setUseAqua;
-    Vchrg_Meth chgm;  int setchgm;
-    Vchrg_Src chgs;
+    int pdime[3];     int setpdime;
+    int proc_rank;    int setrank;
+    int proc_size;    int setsize;
+    double ofrac;     int setofrac;
+    int async;        int setasync;
+    int nonlintype;   int setnonlintype;
+    int method;       int setmethod;
+    int useAqua;      int setUseAqua;
 } MGparm;
 
-MGparm *MGparm_ctor(MGparm_CalcType type);      /* src/generic/mgparm.c:100 */
-void MGparm_dtor(MGparm **thee);                /* src/generic/mgparm.c:163 */
+MGparm *MGparm_ctor(MGparm_CalcType type);      /* src/generic/mgparm.c:114 */
+void MGparm_dtor(MGparm **thee);                /* src/generic/mgparm.c:172 */
 /* MGparm_check (src/generic/mgparm.c:182-330): validates and derives nlev
  * and dime; returns 1 (success) / 0. */
 int MGparm_check(MGparm *thee);
@@ -173,11 +253,11 @@ typedef struct sVpmgp {
     double xlen, ylen, zlen, xmin, ymin, zmin, xmax, ymax, zmax;
 } Vpmgp;
 
-Vpmgp *Vpmgp_ctor(MGparm *mgparm);              /* src/mg/vpmgp.c:73-93 */
+Vpmgp *Vpmgp_ctor(MGparm *mgparm);              /* src/mg/vpmgp.c:76-170 */
 void Vpmgp_dtor(Vpmgp **thee);                  /* src/mg/vpmgp.c:178 */
 void Vpmgp_size(Vpmgp *thee);                   /* src/mg/vpmgp.c:196-291 */
 void Vpmgp_makeCoarse(int numLevel, int nxOld, int nyOld, int nzOld, int *nxNew,
-                      int *nyNew, int *nzNew);  /* src/mg/vpmgp.c:313-334 */
+                      int *nyNew, int *nzNew);  /* src/mg/vpmgp.c:312-334 */
 
 /* ------------------------------------------------------------------- Vpmg
  * struct sVpmg: the reference field order up to chargeMap
@@ -224,7 +304,7 @@ Vpmg *Vpmg_ctor(Vpmgp *parms, Vpbe *pbe, int focusFlag, Vpmg *pmgOLD, MGparm *mg
 int Vpmg_ctor2(Vpmg *thee, Vpmgp *parms, Vpbe *pbe, int focusFlag, Vpmg *pmgOLD,
                MGparm *mgparm, PBEparm_calcEnergy energyFlag);
 void Vpmg_dtor(Vpmg **thee);                    /* src/mg/vpmg.h:260 */
-void Vpmg_dtor2(Vpmg *thee);                    /* src/mg/vpmg.h:270 */
+void Vpmg_dtor2(Vpmg *thee);                    /* src/mg/vpmg.h:269 */
 /* Vpmg_fillco (src/mg/vpmg.h:281-298, vpmg.c:5250-5415).  External maps
  * (use*Map != 0) are not supported and return 0. */
 int Vpmg_fillco(Vpmg *thee, Vsurf_Meth surfMeth, double splineWin, Vchrg_Meth chargeMeth,

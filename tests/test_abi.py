@@ -86,3 +86,39 @@ def test_mgauto_chain_matches_restatement():
         assert np.allclose(m.contents.grid[:], grid, rtol=0, atol=0)
         assert np.allclose(m.contents.glen[:], glen, rtol=0, atol=0)
         assert np.allclose(m.contents.center[:], center, rtol=0, atol=0)
+
+
+def build_c_caller(tmpdir):
+    """tests/c_caller.c built with gcc against include/apbs_pmg.h and the
+    library (a C binding needs nothing else)."""
+    exe = os.path.join(str(tmpdir), "c_caller")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    "-o", exe, os.path.join(ROOT, "tests", "c_caller.c"),
+                    "-L", os.path.join(ROOT, "apbs_amd"), "-lapbs_pmg",
+                    "-Wl,-rpath," + os.path.join(ROOT, "apbs_amd")], check=True)
+    return exe
+
+
+def test_c_header_layout_matches_binding(lib, tmp_path):
+    """The struct layouts a C caller compiles against equal the ones the
+    Python binding uses (same sizes and field offsets)."""
+    import apbs_amd
+    exe = build_c_caller(tmp_path)
+    out = subprocess.run([exe, "layout"], capture_output=True, text=True, check=True).stdout
+    vals = dict(l.split() for l in out.splitlines())
+    types = {"Vatom": apbs_amd.Vatom, "Valist": apbs_amd.Valist, "MGparm": apbs_amd.MGparm,
+             "Vpmgp": apbs_amd.Vpmgp, "Vpmg": apbs_amd.Vpmg}
+    checked = 0
+    for key, v in vals.items():
+        kind, field = key.split(".")
+        if kind == "sizeof" and field in types:
+            assert ctypes.sizeof(types[field]) == int(v), key
+            checked += 1
+        elif kind in types:
+            assert getattr(types[kind], field).offset == int(v), key
+            checked += 1
+    assert checked >= 25
+    # reference layout facts: sVatom carries two 64-byte name fields after
+    # id (vatom.h:84-100); sMGparm puts chgm right after setdime
+    # (mgparm.h:112-120)
+    assert int(vals["sizeof.Vatom"]) == 192 and int(vals["MGparm.chgm"]) == 24

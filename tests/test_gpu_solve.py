@@ -199,3 +199,24 @@ def test_mixed_precision_energy(npbe):
           f"L2 {l2:.2e} Linf {linf:.2e} stats {gmx.stats()}")
     assert rel(emx, e64) < 1e-6
     assert linf < 1e-6
+
+
+def test_c_caller_matches_python(tmp_path):
+    """tests/c_caller.c runs the routines.c sequence through the C header
+    alone; its energy and force equal the Python driver's bitwise."""
+    import subprocess
+    from test_abi import build_c_caller
+    exe = build_c_caller(tmp_path)
+    pqr = os.path.join(HERE, "golden", "born_ion.pqr")
+    out = subprocess.run([exe, "solve", pqr, "33", "0.5"], capture_output=True, text=True,
+                         check=True, timeout=120).stdout
+    vals = {l.split()[0]: [float(x) for x in l.split()[1:]] for l in out.splitlines()}
+    atoms = od.read_pqr(pqr)
+    mol = driver.Molecule(*atoms)
+    pbe = dict(pdie=1.0, sdie=78.54, srfm="spl2", chgm="spl2", srad=0.3, swin=0.3, sdens=10.0,
+               temp=298.15, bcfl="mdh", ions=SALT)
+    e, calc = driver.run_manual(mol, [33] * 3, mol.center, pbe, grid=[0.5] * 3)
+    rc, qf, ib, db = calc.force(0)
+    assert vals["energy"][0] == e
+    assert np.array_equal(np.array(vals["force"]), qf + db + ib)  # vpmg.c:5434-5436 order
+    assert vals["umax"][0] == calc.array("u").max()

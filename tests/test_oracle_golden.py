@@ -82,3 +82,39 @@ def test_ion_protein_npbe():
     for got, want in zip(out, G["ionprot_smol_pdiel2"]["energies"]):
         assert rel(got, want) < TOL
     assert rel(out[2] - out[0] - out[1], G["ionprot_smol_pdiel2"]["difference"]) < 1e-4
+
+
+@pytest.mark.parametrize("case,pdie,salt", [("dielectric+salt", 2.0, 0.15),
+                                            ("dielectric", 2.0, 0.0),
+                                            ("salt", 78.54, 0.15)])
+def test_forces_match_energy_gradient(case, pdie, salt):
+    """Vpmg_qfForce + ibForce + dbForce (vpmg.c:5440-6154; srfm and chgm
+    spl2) against the central difference of the total energy, for an ion
+    pair on the x axis at grid points (the self-force vanishes there by
+    symmetry).  The three cases give each term weight: without salt ib is
+    zero, with pdie = sdie db is zero; in the salt-only case ib is 1.7 % of
+    the force, so a dropped or sign-flipped term fails the 1 % bar."""
+    kT = od.kT_to_kJmol(1.0, 298.15)
+    q, r = np.array([1.0, -1.0]), np.array([2.0, 1.8])
+    pbe = dict(pdie=pdie, sdie=78.54, srfm="spl2", chgm="spl2", srad=1.4, swin=0.3, sdens=10.0,
+               temp=298.15, bcfl="mdh", ions=[(1, salt, 2.0), (-1, salt, 2.0)], etol=1e-12)
+
+    def run(xyz):
+        return od.run_manual((xyz, q, r), [65] * 3, [0.0, 0.0, 0.0], pbe, grid=[0.3] * 3)
+
+    x0 = np.array([[-3.0, 0.0, 0.0], [3.0, 0.0, 0.0]])
+    _, c = run(x0)
+    rc, qf, ib, db = c.force(0)
+    assert rc == 1
+    f = qf + ib + db
+    d = 0.02
+    xp, xm = x0.copy(), x0.copy()
+    xp[0, 0] += d
+    xm[0, 0] -= d
+    fd = -(run(xp)[0] - run(xm)[0]) / (2 * d) / kT
+    assert abs(f[0] - fd) < 0.01 * abs(fd), (case, f[0], fd)
+    assert abs(f[1]) < 1e-8 and abs(f[2]) < 1e-8
+    if salt == 0.0:
+        assert np.all(ib == 0.0)
+    if pdie == 78.54:
+        assert np.all(db == 0.0)
