@@ -259,11 +259,36 @@ template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_nrm1_part(int nx, int ny, int nz, const T* __restrict__ x, double* part) {
   __shared__ double sh[kBlock];
-  const long ni = nx - 2, nij = ni * (ny - 2), n = nij * (nz - 2);
+  const int ni = nx - 2, nij = ni * (ny - 2), n = nij * (nz - 2);
   double acc = 0.0;
-  for (long t = (long)blockIdx.x * kBlock + threadIdx.x; t < n; t += (long)kParts * kBlock) {
-    const int i = 1 + (int)(t % ni), j = 1 + (int)((t / ni) % (ny - 2)), k = 1 + (int)(t / nij);
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += kParts * kBlock) {
+    const int k = 1 + t / nij, rem = t - (k - 1) * nij;
+    const int j = 1 + rem / ni, i = 1 + rem - (j - 1) * ni;
     acc += fabs((double)x[IX(i, j, k, nx, ny)]);
+  }
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// The 7-point residual and the partial sums of its L1 norm in one pass: the
+// points are visited in k_nrm1_part's order, so the norm is bitwise that of
+// storing r and calling k_nrm1_part.  r is stored when non-null.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_resid7_nrm(int nx, int ny, int nz, const T* __restrict__ oE, const T* __restrict__ oN,
+                 const T* __restrict__ uC, const T* __restrict__ D, const T* __restrict__ fc,
+                 const T* __restrict__ x, T* __restrict__ r, double* part) {
+  __shared__ double sh[kBlock];
+  const int ni = nx - 2, nij = ni * (ny - 2), n = nij * (nz - 2);
+  const size_t nxy = (size_t)nx * ny;
+  double acc = 0.0;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += kParts * kBlock) {
+    const int k = 1 + t / nij, rem = t - (k - 1) * nij;
+    const int j = 1 + rem / ni, i = 1 + rem - (j - 1) * ni;
+    const size_t p = IX(i, j, k, nx, ny);
+    const T v = acc7(fc[p], oE, oN, uC, x, p, nx, nxy) - D[p] * x[p];
+    if (r) r[p] = v;
+    acc += fabs((double)v);
   }
   double s = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -539,6 +564,30 @@ __global__ void __launch_bounds__(kBlock)
   r[p] = acc7(fc[p], oE, oN, uC, x, p, nx, nxy) - oC[p] * x[p] - w;
 }
 
+// k_nresid7 with the L1-norm partials of the residual (order of k_nrm1_part).
+__global__ void __launch_bounds__(kBlock)
+    k_nresid7_nrm(int nx, int ny, int nz, const double* __restrict__ oC,
+                  const double* __restrict__ oE, const double* __restrict__ oN,
+                  const double* __restrict__ uC, const double* __restrict__ cc,
+                  const double* __restrict__ fc, const double* __restrict__ x,
+                  double* __restrict__ r, Ions ions, double* part) {
+  __shared__ double sh[kBlock];
+  const int ni = nx - 2, nij = ni * (ny - 2), n = nij * (nz - 2);
+  const size_t nxy = (size_t)nx * ny;
+  double acc = 0.0;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += kParts * kBlock) {
+    const int k = 1 + t / nij, rem = t - (k - 1) * nij;
+    const int j = 1 + rem / ni, i = 1 + rem - (j - 1) * ni;
+    const size_t p = IX(i, j, k, nx, ny);
+    const double w = ncvec(cc[p], x[p], ions);
+    const double v = acc7(fc[p], oE, oN, uC, x, p, nx, nxy) - oC[p] * x[p] - w;
+    r[p] = v;
+    acc += fabs(v);
+  }
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
 __global__ void k_dcvec(size_t n, const double* __restrict__ cc, const double* __restrict__ x,
                         double* __restrict__ out, Ions I) {
   for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (size_t)gridDim.x * kBlock) {
@@ -721,30 +770,50 @@ __global__ void k_dense_assemble(int nx, int ny, int nz, Ops27<double> A, const 
   }
 }
 
-// In-place Gauss-Jordan inverse of an SPD matrix, one workgroup.
-__global__ void __launch_bounds__(1024) k_dense_invert(double* M, double* col, int N) {
-  for (int k = 0; k < N; k++) {
-    // save column k, then scale pivot row
-    for (int i = threadIdx.x; i < N; i += blockDim.x) col[i] = M[(size_t)i * N + k];
-    __syncthreads();
-    const double piv = col[k];
-    for (int j = threadIdx.x; j < N; j += blockDim.x) {
-      const double v = (j == k) ? 1.0 : M[(size_t)k * N + j];
-      M[(size_t)k * N + j] = v / piv;
-    }
-    __syncthreads();
-    for (long e = threadIdx.x; e < (long)N * N; e += blockDim.x) {
-      const int i = (int)(e / N), j = (int)(e % N);
-      if (i == k) continue;
-      const double f = col[i];
-      const double base = (j == k) ? 0.0 : M[(size_t)i * N + j];
-      M[(size_t)i * N + j] = base - f * M[(size_t)k * N + j];
-    }
-    __syncthreads();
-  }
+// In-place Gauss-Jordan inverse of the (SPD) coarsest operator.  For large N
+// the elimination is spread over the whole GPU, one pivot at a time: save
+// column k, scale the pivot row, eliminate the other rows.
+__global__ void __launch_bounds__(kBlock) k_gj_col(const double* M, double* col, int N, int k) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < N) col[i] = M[(size_t)i * N + k];
 }
 
-// The same elimination with the matrix held in LDS (N*N + N doubles).
+__global__ void __launch_bounds__(kBlock) k_gj_row(double* M, const double* col, int N, int k) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= N) return;
+  const double v = (j == k) ? 1.0 : M[(size_t)k * N + j];
+  M[(size_t)k * N + j] = v / col[k];
+}
+
+__global__ void __launch_bounds__(kBlock) k_gj_elim(double* M, const double* col, int N, int k) {
+  const size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= (size_t)N * N) return;
+  const int i = (int)(e / N), j = (int)(e % N);
+  if (i == k) return;
+  const double f = col[i];
+  const double base = (j == k) ? 0.0 : M[(size_t)i * N + j];
+  M[(size_t)i * N + j] = base - f * M[(size_t)k * N + j];
+}
+
+// x = Minv f reading Minv by columns (coalesced across rows); Minv is the
+// inverse of a symmetric operator.
+template <class T>
+__global__ void k_dense_solve_cols(int nx, int ny, int nz, const double* Minv, int N, const T* f,
+                                   T* x) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= N) return;
+  const int ni = nx - 2, nj = ny - 2;
+  double s = 0.0;
+  for (int c = 0; c < N; c++) {
+    const int i = c % ni + 1, j = (c / ni) % nj + 1, k = c / (ni * nj) + 1;
+    s += Minv[(size_t)c * N + r] * (double)f[IX(i, j, k, nx, ny)];
+  }
+  const int i = r % ni + 1, j = (r / ni) % nj + 1, k = r / (ni * nj) + 1;
+  x[IX(i, j, k, nx, ny)] = (T)s;
+}
+
+// The same elimination in one workgroup with the matrix held in LDS
+// (N*N + N doubles, N <= 89).
 __global__ void __launch_bounds__(1024) k_dense_invert_lds(double* Mg, int N) {
   extern __shared__ double sm[];
   double* M = sm;
@@ -897,6 +966,21 @@ void nrm1(hipStream_t s, int nx, int ny, int nz, const T* x, double* part, doubl
 }
 
 template <class T>
+void resid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
+                 const T* x, T* r, double* part, double* out) {
+  k_resid7_nrm<T><<<kParts, kBlock, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, x, r, part);
+  k_finish<<<1, kBlock, 0, s>>>(part, out, 0);
+}
+
+void nresid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* cc,
+                  const double* fc, const double* x, double* r, const Ions& ions, double* part,
+                  double* out) {
+  k_nresid7_nrm<<<kParts, kBlock, 0, s>>>(nx, ny, nz, A.a[0], A.a[1], A.a[2], A.a[3], cc, fc, x, r,
+                                          ions, part);
+  k_finish<<<1, kBlock, 0, s>>>(part, out, 0);
+}
+
+template <class T>
 void damp_factor(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* x,
                  const T* w0, double* part, double* out) {
   k_damp_part<T><<<kParts, kBlock, 0, s>>>(nx, ny, nz, to27(A), D, x, w0, part);
@@ -970,16 +1054,26 @@ void dense_assemble(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A,
 
 void dense_invert(hipStream_t s, double* M, double* work, int N) {
   const size_t lds = ((size_t)N * N + N) * sizeof(double);
-  if (lds <= 64 * 1024)
+  if (lds <= 64 * 1024) {
     k_dense_invert_lds<<<1, 1024, lds, s>>>(M, N);
-  else
-    k_dense_invert<<<1, 1024, 0, s>>>(M, work, N);
+    return;
+  }
+  const unsigned gn = (unsigned)((N + kBlock - 1) / kBlock);
+  const unsigned gnn = (unsigned)(((size_t)N * N + kBlock - 1) / kBlock);
+  for (int k = 0; k < N; k++) {
+    k_gj_col<<<gn, kBlock, 0, s>>>(M, work, N, k);
+    k_gj_row<<<gn, kBlock, 0, s>>>(M, work, N, k);
+    k_gj_elim<<<gnn, kBlock, 0, s>>>(M, work, N, k);
+  }
 }
 
 template <class T>
 void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int N, const T* f,
                  T* x) {
-  k_dense_solve<T><<<(N + kBlock - 1) / kBlock, kBlock, 0, s>>>(nx, ny, nz, Minv, N, f, x);
+  if (((size_t)N * N + N) * sizeof(double) <= 64 * 1024)
+    k_dense_solve<T><<<(N + kBlock - 1) / kBlock, kBlock, 0, s>>>(nx, ny, nz, Minv, N, f, x);
+  else
+    k_dense_solve_cols<T><<<(N + kBlock - 1) / kBlock, kBlock, 0, s>>>(nx, ny, nz, Minv, N, f, x);
 }
 
 // ------------------------------------------------------ explicit instances
@@ -994,6 +1088,8 @@ void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int 
   template void prolong_add<T>(hipStream_t, int, int, int, int, int, int, const T*, T*,         \
                                const double*);                                                  \
   template void nrm1<T>(hipStream_t, int, int, int, const T*, double*, double*);                \
+  template void resid7_nrm1<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*,   \
+                               const T*, T*, double*, double*);                                 \
   template void damp_factor<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*,   \
                                const T*, double*, double*);                                     \
   template void fill<T>(hipStream_t, T*, size_t, T);                                            \

@@ -45,7 +45,26 @@ struct OpsHolder {
 };
 }  // namespace
 
+__global__ void k_stream_copy(const double* __restrict__ a, double* __restrict__ b, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
 extern "C" {
+
+// Calibration for hardware byte counters: reps copies of n doubles (n*8
+// bytes read and n*8 bytes written per launch) between two device buffers.
+int pmgtest_stream_copy(size_t n, int reps) {
+  double *a = up(nullptr, n), *b = up(nullptr, n);
+  TCHECK(hipMemset(a, 0, n * sizeof(double)));
+  TCHECK(hipMemset(b, 0, n * sizeof(double)));
+  for (int r = 0; r < reps; r++) k_stream_copy<<<4096, 256>>>(a, b, n);
+  TCHECK(hipDeviceSynchronize());
+  TCHECK(hipFree(a));
+  TCHECK(hipFree(b));
+  return 1;
+}
 
 // x <- nu sweeps (7-point red-black or 27-point 8-colour), D = oC + cc
 int pmgtest_gsrb(int nx, int ny, int nz, int st, const double* a, const double* cc,

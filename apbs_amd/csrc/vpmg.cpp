@@ -1093,8 +1093,8 @@ int mvcs64(PmgDev* d, double* const* X, const double* f, int istop, int itmax, d
   do {
     vcycle<double>(d, c);
     iters++;
-    pmg::resid<double>(d->s, 7, F.nx, F.ny, F.nz, c.A[0], c.Dl[0], f, X[0], d->w1);
-    pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, d->w1, d->part, d->scal);
+    pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, c.A[0], c.Dl[0], f, X[0], nullptr, d->part,
+                             d->scal);
     rsnrm = read_scalar(d, d->scal);
   } while (iters < itmax && (rsnrm / rsden) > errtol);
   if (relres) *relres = rsnrm / rsden;
@@ -1126,8 +1126,7 @@ int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int i
   const pmg::Ops<double> A0 = F.ops();
   int it = 0;
   double rsnrm = rsden;
-  pmg::resid<double>(d->s, 7, F.nx, F.ny, F.nz, A0, F.D, f, X64[0], d->w1);
-  pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, d->w1, d->part, d->scal);
+  pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, A0, F.D, f, X64[0], d->w1, d->part, d->scal);
   rsnrm = read_scalar(d, d->scal);
   while (it < itmax && (rsnrm / rsden) > errtol) {
     pmg::convert<double, float>(d->s, F.n, d->w1, d->L[0].w0f);
@@ -1137,8 +1136,8 @@ int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int i
     pmg::convert<float, double>(d->s, F.n, d->L[0].xfl, d->ecorr);
     pmg::axpy_interior<double>(d->s, F.nx, F.ny, F.nz, 1.0, d->ecorr, X64[0]);
     it++;
-    pmg::resid<double>(d->s, 7, F.nx, F.ny, F.nz, A0, F.D, f, X64[0], d->w1);
-    pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, d->w1, d->part, d->scal);
+    pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, A0, F.D, f, X64[0], d->w1, d->part,
+                             d->scal);
     rsnrm = read_scalar(d, d->scal);
   }
   if (relres) *relres = rsnrm / rsden;
@@ -1168,12 +1167,10 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
   hipStream_t s = d->s;
   // rsden = ||F(0)||1
   pmg::fill<double>(s, d->wn1, d->n, 0.0);
-  pmg::nresid7(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->wn2, I);
-  pmg::nrm1<double>(s, nx, ny, nz, d->wn2, d->part, d->scal);
+  pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->wn2, I, d->part, d->scal);
   double rsden = read_scalar(d, d->scal);
   if (rsden == 0.0) rsden = 1.0;
-  pmg::nresid7(s, nx, ny, nz, A0, F.cc, F.fc, d->u, d->F, I);
-  pmg::nrm1<double>(s, nx, ny, nz, d->F, d->part, d->scal);
+  pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->u, d->F, I, d->part, d->scal);
   double xnorm_old = read_scalar(d, d->scal);
   double xnorm_new = xnorm_old, xnorm_med = 0.0, rsnrm = rsden;
   int idamp = 1, iters = 0, inner = 0;
@@ -1207,8 +1204,7 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
     double* xtmp = L[0].xtmp;
     if (idamp == 1) {
       pmg::xpay_interior(s, nx, ny, nz, d->u, 1.0, xtmp, d->wn1);
-      pmg::nresid7(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->F, I);
-      pmg::nrm1<double>(s, nx, ny, nz, d->F, d->part, d->scal);
+      pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->F, I, d->part, d->scal);
       xnorm_new = read_scalar(d, d->scal);
       double damp = 1.0;
       int iter_d = 0, mode = 0;
@@ -1223,8 +1219,7 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
         xnorm_med = xnorm_new;
         damp = damp / 2.0;
         pmg::xpay_interior(s, nx, ny, nz, d->u, damp, xtmp, d->wn1);
-        pmg::nresid7(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->F, I);
-        pmg::nrm1<double>(s, nx, ny, nz, d->F, d->part, d->scal);
+        pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->F, I, d->part, d->scal);
         xnorm_new = read_scalar(d, d->scal);
         iter_d++;
       }
@@ -1235,8 +1230,7 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
       if ((iter_d - 1) == 0) idamp = 0;
     } else {
       pmg::axpy_interior<double>(s, nx, ny, nz, 1.0, xtmp, d->u);
-      pmg::nresid7(s, nx, ny, nz, A0, F.cc, F.fc, d->u, d->F, I);
-      pmg::nrm1<double>(s, nx, ny, nz, d->F, d->part, d->scal);
+      pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->u, d->F, I, d->part, d->scal);
       xnorm_new = read_scalar(d, d->scal);
       xnorm_old = xnorm_new;
     }

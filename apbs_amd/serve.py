@@ -198,3 +198,36 @@ def latency_summary(results, wall):
     return {"requests": len(results), "solves_per_s": len(results) / wall,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()), "failed": int(sum(r["status"] != 1 for r in results))}
+
+
+def main(argv=None):
+    """The served stream of binding triplets at a sweep of Poisson arrival
+    rates: one line per rate with p50/p99 latency, solves/s and the peak
+    device footprint."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--requests", type=int, default=512)
+    ap.add_argument("--rates", default="10,20,40,0", help="requests/s; 0 = all at once")
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--precision", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=3)
+    a = ap.parse_args(argv)
+    specs = binding_stream(a.requests, seed=a.seed)
+    for rate in [float(x) for x in a.rates.split(",")]:
+        srv = Server(slots=a.slots, hbm_budget=int(a.budget_gb * 2 ** 30))
+        res, wall = run_stream(srv, specs, rate, seed=a.seed, precision=a.precision)
+        st = srv.stats()
+        srv.close()
+        out = latency_summary(res, wall)
+        out.update(rate=rate, slots=a.slots, precision=a.precision,
+                   peak_admitted_GB=st["peak_admitted_bytes"] / 1e9,
+                   peak_pool_GB=st["peak_pool_bytes"] / 1e9,
+                   dims=sorted(set(s["dime"][0] for s in specs)),
+                   npbe_fraction=float(np.mean([bool(s["pbe"].get("npbe")) for s in specs])))
+        print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -86,6 +86,12 @@ CONFIGS = {
                     pbe=dict(pdie=2.0, sdie=78.54, srfm="spl2", chgm="spl2", swin=0.3, srad=1.4,
                              sdens=10.0, temp=298.15, bcfl="mdh",
                              ions=[(1, 0.15, 2.0), (-1, 0.15, 2.0)])),
+    # focusing: coarse 97^3 at 1.2 A gives the boundary of a 353^3 grid at 0.3 A
+    "focus353": dict(dime=353, h=0.3, coarse_dime=97, coarse_h=1.2, natoms=60000,
+                     axes=(50, 45, 40), net=-40, seed=4,
+                     pbe=dict(npbe=True, pdie=4.0, sdie=78.54, srfm="smol", chgm="spl2", swin=0.3,
+                              srad=1.4, sdens=10.0, temp=298.15, bcfl="mdh",
+                              ions=[(1, 0.15, 2.0), (-1, 0.15, 2.0)])),
 }
 
 

@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--slots", type=int, default=8, help="concurrent requests of the served path")
     ap.add_argument("--served", type=int, default=48, help="requests in the served run (0: none)")
+    ap.add_argument("--pmc", action="store_true",
+                    help="counter-collection workload: a few single-request calculations of each "
+                         "precision and a known-size copy kernel (run under rocprofv3 --pmc)")
     return ap.parse_args()
 
 
@@ -96,6 +99,35 @@ def run_mode(mol, cfg, precision, steps, warmup):
     return res, calc
 
 
+COPY_DOUBLES = 32 * 2 ** 20   # calibration copy: 256 MiB read + 256 MiB written per launch
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_smoother_npbe161.md")
+
+
+def measured_traffic(word):
+    """HBM bytes per fused-sweep launch from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), or
+    None when it has not been measured."""
+    key = "traffic_bytes_per_launch_" + ("fp32" if word == 4 else "fp64")
+    try:
+        for line in open(PMC_SUMMARY):
+            if line.startswith(key + ":"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def pmc_workload(mol, cfg):
+    from apbs_amd import lib
+    import ctypes
+    for prec in (0, 1):
+        run_mode(mol, cfg, prec, 2, 1)
+    L = lib()
+    L.pmgtest_stream_copy.argtypes = [ctypes.c_size_t, ctypes.c_int]
+    L.pmgtest_stream_copy(COPY_DOUBLES, 3)
+    print(f"# pmc workload done: copy kernel {COPY_DOUBLES * 8} bytes each way x 3", flush=True)
+
+
 def smoother_roofline(kt):
     """Fine-level red-black smoother: algorithmic bytes 7 words per point per
     full sweep (D = oC + cc, oE, oN, uC, f, x in, x out); the library reports
@@ -106,7 +138,7 @@ def smoother_roofline(kt):
     bytes_per_launch = kt["words_per_point"] * kt["word"] * kt["points"]
     achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kt["word"]),
             "kernel": "%s<%s>" % ("k_gsrb7_fused" if kt["words_per_point"] == 7 else "k_gsrb7",
                                   "float" if kt["word"] == 4 else "double"),
             "bytes_per_launch": bytes_per_launch, "avg_launch_us": per_launch_ms * 1e3}
@@ -195,6 +227,9 @@ def main():
     cfg = synth.CONFIGS[a.workload]
     xyz, q, r = synth.make(a.workload)
     mol = driver.Molecule(xyz, q, r)
+    if a.pmc:
+        pmc_workload(mol, cfg)
+        return
     modes = a.modes.split(",")
     out = {}
     calcs = {}

@@ -6,6 +6,8 @@
  */
 #ifndef APBS_PMG_TESTHOOKS_H
 #define APBS_PMG_TESTHOOKS_H
+
+#include <stddef.h>
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -35,6 +37,9 @@ int pmgtest_buildA(int nx, int ny, int nz, const double *xf, const double *yf, c
 int pmgtest_nonlinear(int nx, int ny, int nz, const double *a, const double *cc,
                       const double *fc, const double *x, int nion, const double *q,
                       const double *sconc, double *out_r, double *out_d);
+/* No reference counterpart: reps device copies of n doubles, the known-size
+ * calibration for hardware byte counters (FETCH_SIZE / WRITE_SIZE). */
+int pmgtest_stream_copy(size_t n, int reps);
 #ifdef __cplusplus
 }
 #endif

@@ -122,3 +122,23 @@ def test_c_header_layout_matches_binding(lib, tmp_path):
     # id (vatom.h:84-100); sMGparm puts chgm right after setdime
     # (mgparm.h:112-120)
     assert int(vals["sizeof.Vatom"]) == 192 and int(vals["MGparm.chgm"]) == 24
+
+
+def test_footprint_estimate_host_only(lib):
+    """PmgServer_footprint needs no GPU: it grows with the grid, with Newton
+    and with the FP32 copies, and 161^3 npbe mixed sits between the bytes of
+    its arrays (~30 fine FP64-equivalent arrays) and twice that."""
+    import numpy as np
+    from apbs_amd import serve
+    xyz = np.zeros((1, 3))
+    one = (xyz, np.array([1.0]), np.array([2.0]))
+
+    def fp(dime, pbe, prec):
+        return serve.Request(*one, (dime,) * 3, [0.4] * 3, [0, 0, 0], pbe, precision=prec).footprint()
+
+    big = fp(161, serve.NPBE, 1)
+    assert fp(161, serve.NPBE, 0) < big
+    assert fp(161, serve.LPBE, 0) < fp(161, serve.NPBE, 0)
+    assert fp(97, serve.NPBE, 1) < big
+    words = 161 ** 3 * 8
+    assert 25 * words < big < 60 * words
