@@ -643,6 +643,16 @@ struct PmgDev {
   };
   std::vector<Mark> marks;
   int ev_next = 0;    // next free event of the pool
+  // V-cycles captured as HIP graphs, keyed by precision, solution and
+  // right-hand side (environment APBS_PMG_GRAPHS=0 launches eagerly)
+  int graphs = 1;
+  struct CycleGraph {
+    int word;
+    const void *f, *x;
+    hipGraph_t g;
+    hipGraphExec_t e;
+  };
+  std::vector<CycleGraph> cycle_graphs;
   double kt[7] = {};  // ms0, launches0, ms1, count1, word bytes, fine points, words/pt/launch
 };
 
@@ -772,7 +782,16 @@ void upload_part(PmgDev* d) {
 
 // --------------------------------------------------------------- hierarchy
 
+void free_graphs(PmgDev* d) {
+  for (auto& cg : d->cycle_graphs) {
+    HIPCHECK(hipGraphExecDestroy(cg.e));
+    HIPCHECK(hipGraphDestroy(cg.g));
+  }
+  d->cycle_graphs.clear();
+}
+
 void free_hierarchy(PmgDev* d) {
+  free_graphs(d);
   for (int l = 0; l < d->ntot; l++) {
     DLevel& L = d->L[l];
     for (int s = 0; s < 14; s++) {
@@ -1063,6 +1082,34 @@ void vcycle(PmgDev* d, CycleArgs<T>& c) {
   tend(d, ecyc, 1, 1);
 }
 
+// One V-cycle: replayed from a captured graph, or launched eagerly while
+// kernel timing is on (its events are host-side bookkeeping).  The cycle's
+// kernels depend only on device data, so the graph of a given (precision,
+// solution, right-hand side) is valid for every later cycle on this
+// hierarchy.
+template <class T>
+void run_vcycle(PmgDev* d, CycleArgs<T>& c) {
+  if (!d->graphs || d->timing) {
+    vcycle<T>(d, c);
+    return;
+  }
+  for (auto& cg : d->cycle_graphs)
+    if (cg.word == (int)sizeof(T) && cg.f == (const void*)c.f && cg.x == (const void*)c.X[0]) {
+      HIPCHECK(hipGraphLaunch(cg.e, d->s));
+      return;
+    }
+  PmgDev::CycleGraph cg;
+  cg.word = (int)sizeof(T);
+  cg.f = c.f;
+  cg.x = c.X[0];
+  HIPCHECK(hipStreamBeginCapture(d->s, hipStreamCaptureModeThreadLocal));
+  vcycle<T>(d, c);
+  HIPCHECK(hipStreamEndCapture(d->s, &cg.g));
+  HIPCHECK(hipGraphInstantiate(&cg.e, cg.g, nullptr, nullptr, 0));
+  d->cycle_graphs.push_back(cg);
+  HIPCHECK(hipGraphLaunch(cg.e, d->s));
+}
+
 CycleArgs<double> args64(PmgDev* d, double* const* X, const double* f) {
   CycleArgs<double> c;
   for (int l = 0; l < d->ntot; l++) {
@@ -1091,7 +1138,7 @@ int mvcs64(PmgDev* d, double* const* X, const double* f, int istop, int itmax, d
   int iters = 0;
   double rsnrm = rsden;
   do {
-    vcycle<double>(d, c);
+    run_vcycle<double>(d, c);
     iters++;
     pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, c.A[0], c.Dl[0], f, X[0], nullptr, d->part,
                              d->scal);
@@ -1131,7 +1178,7 @@ int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int i
   while (it < itmax && (rsnrm / rsden) > errtol) {
     pmg::convert<double, float>(d->s, F.n, d->w1, d->L[0].w0f);
     pmg::fill<float>(d->s, d->L[0].xfl, F.n, 0.0f);
-    vcycle<float>(d, c);
+    run_vcycle<float>(d, c);
     // X64 += (double) e
     pmg::convert<float, double>(d->s, F.n, d->L[0].xfl, d->ecorr);
     pmg::axpy_interior<double>(d->s, F.nx, F.ny, F.nz, 1.0, d->ecorr, X64[0]);
@@ -1278,6 +1325,7 @@ extern "C" int Vpmg_ctor2(Vpmg* thee, Vpmgp* pmgp, Vpbe* pbe, int focusFlag, Vpm
   thee->dev = d;
   d->s = pmgpool::stream_acquire();
   if (const char* sm = getenv("APBS_PMG_SMOOTHER")) d->smoother = strcmp(sm, "colour") == 0 ? 1 : 0;
+  if (const char* gr = getenv("APBS_PMG_GRAPHS")) d->graphs = strcmp(gr, "0") == 0 ? 0 : 1;
   d->nx = pmgp->nx;
   d->ny = pmgp->ny;
   d->nz = pmgp->nz;

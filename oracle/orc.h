@@ -12,16 +12,19 @@
  * layout as the reference's 1-based VAT3 (src/generic/vmatrix.h:76-87) and
  * IJK (src/mg/vpmg.h:1383).
  *
- * Parity pinning (see tests/test_oracle_*.py):
- *  - golden energies of tests/test_cases.cfg (born, solv, ion-protein) and
- *    examples/born/polarforces, at the reference harness tolerance;
- *  - closed forms (Born ion, Coulomb/Debye-Hueckel boundary, Laplacian with
- *    uniform dielectric), invariants (Galerkin RAP against a dense brute-force
- *    P^T A P, symmetry of all stencils, restriction = P^T, colour-order
- *    independence of the 7-point red-black sweep).
+ * Parity pinning (tests/test_oracle_golden.py, tests/test_oracle_invariants.py):
+ *  - golden energies of tests/test_cases.cfg (born mg-auto smol/mol at every
+ *    focusing level, solv smol/mol, ion-protein npbe) to 1e-9 relative;
+ *  - closed forms: Born's solvation energy, the Coulomb potential of a charge
+ *    in a uniform dielectric, the Debye-Hueckel sphere with salt;
+ *  - invariants: Galerkin = P^T A P (dense brute force, 7- and 27-point),
+ *    restriction = P^T, interpolation = P, residual = f - (A + cc) x, one
+ *    red-black sweep = the colour-by-colour update;
+ *  - forces (qf + ib + db) = minus the central difference of the energy.
  *  - "parity unpinned": the exact iterate sequence of the lexicographic
  *    27-point Gauss-Seidel sweep and the banded LINPACK coarse solve beyond
- *    what the golden energies constrain.
+ *    what the golden energies constrain; examples/born/polarforces is not a
+ *    usable pin (the reference's own harness does not check it).
  */
 #ifndef ORC_H
 #define ORC_H

@@ -220,3 +220,30 @@ def test_c_caller_matches_python(tmp_path):
     assert vals["energy"][0] == e
     assert np.array_equal(np.array(vals["force"]), qf + db + ib)  # vpmg.c:5434-5436 order
     assert vals["umax"][0] == calc.array("u").max()
+
+
+@pytest.mark.parametrize("npbe,precision", [(False, 0), (True, 0), (True, 1)])
+def test_graph_replay_equals_eager(npbe, precision, monkeypatch):
+    """V-cycles replayed from captured HIP graphs give bitwise the results of
+    eager launches, over three solves on one object (each solve replays the
+    captured cycles many times, with eager Newton work in between)."""
+    atoms = small_mol(300, 9.0, 21, net=-2)
+    pbe = pbe_of("smol", "spl2", npbe=npbe)
+    mol = driver.Molecule(*atoms)
+    c = od.mol_center(atoms[0])
+    res = {}
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("APBS_PMG_GRAPHS", graphs)
+        calc = driver.Calc(mol, driver.manual_mgparm([49] * 3, c, grid=[0.45] * 3), pbe,
+                           precision=precision)
+        runs = []
+        for _ in range(3):
+            calc.fillco()
+            calc.solve()
+            calc.set_part_domain()
+            runs.append((calc.energy(1), calc.array("u").copy()))
+        res[graphs] = runs
+    for (eg, ug), (ee, ue) in zip(res["1"], res["0"]):
+        assert eg == ee
+        assert np.array_equal(ug, ue)
+    assert res["1"][0][0] == res["1"][2][0]
