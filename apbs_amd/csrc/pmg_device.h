@@ -40,9 +40,11 @@ int gsrb7_launches(int nu);
 // between x and tmp (fine-size scratch).  Only interior points are written,
 // so x and tmp must hold the same boundary values (zero in the V-cycle).
 // The result is bitwise that of gsrb7 and ends in x.
+// tile_shape: rows*100 + planes of a workgroup's block (416, 48, 816, 88,
+// 84, 1616, 832); <= 0 takes APBS_PMG_FUSED_TILE or the default 816.
 template <class T>
 void gsrb7_fused(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D,
-                 const T* fc, T* x, T* tmp, int nu, int iadjoint);
+                 const T* fc, T* x, T* tmp, int nu, int iadjoint, int tile_shape = 0);
 int gsrb7_fused_launches(int nu);
 template <class T>
 void gsrb27(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,

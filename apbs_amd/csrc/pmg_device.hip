@@ -9,6 +9,7 @@
 #include "pmg_device.h"
 
 #include <cmath>
+#include <cstdlib>
 
 namespace pmg {
 
@@ -70,9 +71,11 @@ __global__ void __launch_bounds__(kBlock)
 // ring and the whole tile row of plane k is written.  Each value is the
 // in-place sweep's value bit for bit (same terms, same order, no FMA).
 // Points outside the interior are not written.
-constexpr int kTX = 64, kTY = 8, kKC = 16, kFT = 256;
+constexpr int kTX = 64, kFT = 256;
 
-template <class T>
+// kTY rows (a multiple of 4: each thread owns kTY/4 rows of one column) and
+// kKC planes per workgroup.
+template <class T, int kTY, int kKC>
 __global__ void __launch_bounds__(kFT)
     k_gsrb7_fused(int nx, int ny, int nz, const T* __restrict__ oE, const T* __restrict__ oN,
                   const T* __restrict__ uC, const T* __restrict__ D, const T* __restrict__ fc,
@@ -108,7 +111,7 @@ __global__ void __launch_bounds__(kFT)
 
   stageF(k0 - 1);
   stageF(k0);
-  const int ti = tid % kTX, tj = tid / kTX;  // rows tj and tj + kTY/2
+  const int ti = tid % kTX, tj = tid / kTX;  // rows tj, tj + 4, ...
   const int i = i0 + ti;
   for (int k = k0; k < k1; k++) {
     stageF(k + 1);
@@ -117,8 +120,8 @@ __global__ void __launch_bounds__(kFT)
     const T* r0 = ring[k % 3];
     const T* rp = ring[(k + 1) % 3];
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int jj = tj + h * (kTY / 2);
+    for (int h = 0; h < kTY / 4; h++) {
+      const int jj = tj + h * 4;
       const int j = j0 + jj;
       if (i > nx - 2 || j > ny - 2) continue;
       const int e = (jj + 1) * RX + (ti + 1);
@@ -813,32 +816,44 @@ __global__ void k_dense_solve_cols(int nx, int ny, int nz, const double* Minv, i
 }
 
 // The same elimination in one workgroup with the matrix held in LDS
-// (N*N + N doubles, N <= 89).
+// (N*N + 2N doubles, N <= 89).  Per pivot: stage column k and the scaled
+// pivot row (v / piv, as k_gj_row computes it), then update every element
+// with k_gj_elim's expression; each thread's (i, j) pairs are decoded once.
+constexpr int kInvPer = 8;  // elements per thread: 89 * 89 <= 8 * 1024
+
 __global__ void __launch_bounds__(1024) k_dense_invert_lds(double* Mg, int N) {
   extern __shared__ double sm[];
   double* M = sm;
   double* col = sm + (size_t)N * N;
-  for (int e = threadIdx.x; e < N * N; e += blockDim.x) M[e] = Mg[e];
+  double* prow = col + N;
+  const int tid = threadIdx.x;
+  int ei[kInvPer], ej[kInvPer];
+  int ne = 0;
+  for (int e = tid; e < N * N && ne < kInvPer; e += 1024, ne++) {
+    ei[ne] = e / N;
+    ej[ne] = e - ei[ne] * N;
+  }
+  for (int e = tid; e < N * N; e += 1024) M[e] = Mg[e];
   __syncthreads();
   for (int k = 0; k < N; k++) {
-    for (int i = threadIdx.x; i < N; i += blockDim.x) col[i] = M[i * N + k];
-    __syncthreads();
-    const double piv = col[k];
-    for (int j = threadIdx.x; j < N; j += blockDim.x) {
-      const double v = (j == k) ? 1.0 : M[k * N + j];
-      M[k * N + j] = v / piv;
+    if (tid < N) {
+      col[tid] = M[tid * N + k];
+      const double v = (tid == k) ? 1.0 : M[k * N + tid];
+      prow[tid] = v / M[k * N + k];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
-      const int i = e / N, j = e % N;
-      if (i == k) continue;
-      const double f = col[i];
-      const double base = (j == k) ? 0.0 : M[i * N + j];
-      M[i * N + j] = base - f * M[k * N + j];
+    for (int q = 0; q < ne; q++) {
+      const int i = ei[q], j = ej[q];
+      if (i == k) {
+        M[i * N + j] = prow[j];
+      } else {
+        const double base = (j == k) ? 0.0 : M[i * N + j];
+        M[i * N + j] = base - col[i] * prow[j];
+      }
     }
     __syncthreads();
   }
-  for (int e = threadIdx.x; e < N * N; e += blockDim.x) Mg[e] = M[e];
+  for (int e = tid; e < N * N; e += 1024) Mg[e] = M[e];
 }
 
 template <class T>
@@ -874,16 +889,44 @@ Ops27<T> to27(const Ops<T>& A) {
 
 // =================================================================== wrappers
 
+template <class T, int TY, int KC>
+void fused_sweep(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
+                 const T* in, T* out, int first) {
+  dim3 g((unsigned)((nx - 2 + kTX - 1) / kTX), (unsigned)((ny - 2 + TY - 1) / TY),
+         (unsigned)((nz - 2 + KC - 1) / KC));
+  k_gsrb7_fused<T, TY, KC><<<g, kFT, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, in, out,
+                                             first);
+}
+
+// tile shape of the fused sweep: rows x planes per workgroup
+int fused_tile() {
+  static int t = [] {
+    const char* e = getenv("APBS_PMG_FUSED_TILE");
+    if (!e) return 816;
+    const int v = atoi(e);
+    return (v == 416 || v == 48 || v == 816 || v == 88 || v == 84 || v == 1616 || v == 832) ? v
+                                                                                          : 816;
+  }();
+  return t;
+}
+
 template <class T>
 void gsrb7_fused(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
-                 T* x, T* tmp, int nu, int iadjoint) {
-  dim3 g((unsigned)((nx - 2 + kTX - 1) / kTX), (unsigned)((ny - 2 + kTY - 1) / kTY),
-         (unsigned)((nz - 2 + kKC - 1) / kKC));
+                 T* x, T* tmp, int nu, int iadjoint, int tile_shape) {
   const int first = iadjoint ? 0 : 1;
   T* in = x;
   T* out = tmp;
+  const int tile = tile_shape > 0 ? tile_shape : fused_tile();
   for (int it = 0; it < nu; it++) {
-    k_gsrb7_fused<T><<<g, kFT, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, in, out, first);
+    switch (tile) {
+      case 416: fused_sweep<T, 4, 16>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+      case 48: fused_sweep<T, 4, 8>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+      case 88: fused_sweep<T, 8, 8>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+      case 84: fused_sweep<T, 8, 4>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+      case 1616: fused_sweep<T, 16, 16>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+      case 832: fused_sweep<T, 8, 32>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+      default: fused_sweep<T, 8, 16>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+    }
     T* t = in;
     in = out;
     out = t;
@@ -1053,8 +1096,8 @@ void dense_assemble(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A,
 }
 
 void dense_invert(hipStream_t s, double* M, double* work, int N) {
-  const size_t lds = ((size_t)N * N + N) * sizeof(double);
-  if (lds <= 64 * 1024) {
+  const size_t lds = ((size_t)N * N + 2 * N) * sizeof(double);
+  if (lds <= 64 * 1024 && (size_t)N * N <= (size_t)kInvPer * 1024) {
     k_dense_invert_lds<<<1, 1024, lds, s>>>(M, N);
     return;
   }
@@ -1098,7 +1141,7 @@ void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int 
   template void add_diag<T>(hipStream_t, size_t, const T*, const T*, T*);                       \
   template void dense_solve<T>(hipStream_t, int, int, int, const double*, int, const T*, T*);  \
   template void gsrb7_fused<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*, T*, \
-                               T*, int, int);
+                               T*, int, int, int);
 INST(double)
 INST(float)
 #undef INST

@@ -66,9 +66,25 @@ int pmgtest_stream_copy(size_t n, int reps) {
   return 1;
 }
 
+int gsrb_impl(int nx, int ny, int nz, int st, const double* a, const double* cc, const double* fc,
+              double* x, int nu, int iadjoint, int tile);
+
 // x <- nu sweeps (7-point red-black or 27-point 8-colour), D = oC + cc
 int pmgtest_gsrb(int nx, int ny, int nz, int st, const double* a, const double* cc,
                  const double* fc, double* x, int nu, int iadjoint) {
+  return gsrb_impl(nx, ny, nz, st, a, cc, fc, x, nu, iadjoint, 0);
+}
+
+// the same with an explicit tile shape of the fused 7-point sweep
+int pmgtest_gsrb_tile(int nx, int ny, int nz, const double* a, const double* cc,
+                      const double* fc, double* x, int nu, int iadjoint, int tile) {
+  return gsrb_impl(nx, ny, nz, 7, a, cc, fc, x, nu, iadjoint, tile);
+}
+
+}  // extern "C"
+
+int gsrb_impl(int nx, int ny, int nz, int st, const double* a, const double* cc, const double* fc,
+              double* x, int nu, int iadjoint, int tile) {
   const size_t n = (size_t)nx * ny * nz;
   OpsHolder A(a, n, st);
   double *dcc = up(cc, n), *dfc = up(fc, n), *dx = up(x, n), *dD = up(nullptr, n);
@@ -77,7 +93,7 @@ int pmgtest_gsrb(int nx, int ny, int nz, int st, const double* a, const double* 
     // the fused sweep kernel of the V-cycle; its partner buffer carries the
     // boundary of x
     double* dt = up(x, n);
-    pmg::gsrb7_fused<double>(0, nx, ny, nz, A.ops, dD, dfc, dx, dt, nu, iadjoint);
+    pmg::gsrb7_fused<double>(0, nx, ny, nz, A.ops, dD, dfc, dx, dt, nu, iadjoint, tile);
     TCHECK(hipDeviceSynchronize());
     TCHECK(hipFree(dt));
   } else
@@ -90,6 +106,8 @@ int pmgtest_gsrb(int nx, int ny, int nz, int st, const double* a, const double* 
   TCHECK(hipFree(dD));
   return 1;
 }
+
+extern "C" {
 
 int pmgtest_resid(int nx, int ny, int nz, int st, const double* a, const double* cc,
                   const double* fc, const double* x, double* r) {

@@ -37,6 +37,10 @@ int pmgtest_buildA(int nx, int ny, int nz, const double *xf, const double *yf, c
 int pmgtest_nonlinear(int nx, int ny, int nz, const double *a, const double *cc,
                       const double *fc, const double *x, int nion, const double *q,
                       const double *sconc, double *out_r, double *out_d);
+/* pmgtest_gsrb (7-point) with an explicit tile shape of the fused sweep
+ * (rows*100 + planes: 416, 48, 816, 88, 84, 1616, 832). */
+int pmgtest_gsrb_tile(int nx, int ny, int nz, const double *a, const double *cc,
+                      const double *fc, double *x, int nu, int iadjoint, int tile);
 /* No reference counterpart: reps device copies of n doubles, the known-size
  * calibration for hardware byte counters (FETCH_SIZE / WRITE_SIZE). */
 int pmgtest_stream_copy(size_t n, int reps);

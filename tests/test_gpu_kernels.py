@@ -71,6 +71,25 @@ def test_gsrb_bitwise(L, O, st, n3, adj):
     assert np.array_equal(xg, xo)
 
 
+@pytest.mark.parametrize("tile", [416, 48, 816, 88, 84, 1616, 832])
+def test_fused_sweep_tiles_bitwise(L, O, tile):
+    """Every tile shape of the fused red-black sweep gives the oracle's
+    sweep bit for bit (partial tiles in every direction)."""
+    rng = np.random.default_rng(tile)
+    n3 = (133, 21, 38)
+    nx, ny, nz = n3
+    n = nx * ny * nz
+    a = rand_op(rng, n3, 7)
+    cc, fc = rng.uniform(0, 1, n), rng.normal(size=n)
+    x0 = np.zeros(n)
+    x0.reshape(nz, ny, nx)[1:-1, 1:-1, 1:-1] = rng.normal(size=(nz - 2, ny - 2, nx - 2))
+    for adj in (0, 1):
+        xg, xo = x0.copy(), x0.copy()
+        L.pmgtest_gsrb_tile(nx, ny, nz, P(a), P(cc), P(fc), P(xg), 2, adj, tile)
+        O.orc_gsrb(nx, ny, nz, 7, P(a), P(cc), P(fc), P(xo), 2, adj, 1)
+        assert np.array_equal(xg, xo), (tile, adj)
+
+
 @pytest.mark.parametrize("st", [7, 27])
 def test_resid_bitwise(L, O, st):
     rng = np.random.default_rng(11)
