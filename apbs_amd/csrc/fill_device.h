@@ -66,6 +66,15 @@ void mol_smooth(hipStream_t s, const Geo& g, const double* a1, const double* a2,
 
 // ------------------------------------------------------------- boundaries
 // bcflnew (vpmg.c:3289-3367): Debye-Hueckel sum over all atoms on the faces.
+// The single Debye-Hueckel sphere of bcfl sdh: solute centre and radius, the
+// molecule's charge, dipole and traceless quadrupole (already divided by 3),
+// pre = e_c^2 / (4 pi eps0 kB T) * 1e10.
+struct SdhMoments {
+  double pos[3], size, q, u[3], qxx, qxy, qxz, qyy, qyz, qzz;
+  double pre, xkappa, eps_p, eps_w;
+};
+void bc_sdh(hipStream_t s, const Geo& g, const double* xf, const double* yf, const double* zf,
+            const SdhMoments& M, double* gx, double* gy, double* gz);
 // Doubles of scratch bc_mdh needs for natoms atoms.
 size_t bc_mdh_scratch(const Geo& g, int natoms);
 void bc_mdh(hipStream_t s, const Geo& g, const double* xf, const double* yf, const double* zf,

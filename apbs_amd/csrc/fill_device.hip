@@ -522,6 +522,56 @@ __global__ void __launch_bounds__(kB)
   *face_slot(g, f, i, j, k, gx, gy, gz) = val;
 }
 
+// multipolebc (vpmg.c:3369-3477): radial factors of the monopole, dipole and
+// traceless-quadrupole potentials outside a dielectric sphere.
+__device__ inline void multipole_tensor(double r, double kappa, double eps_p, double eps_w,
+                                        double rad, double tsr[3]) {
+  const double eps_r = eps_w / eps_p;
+  const double r2 = r * r, r3 = r2 * r, r5 = r3 * r2;
+  tsr[0] = (1.0 / eps_w) / r;
+  tsr[1] = (1.0 / eps_w) * (-1.0) / r3;
+  tsr[2] = (1.0 / eps_w) * (3.0) / r5;
+  if (kappa < VSMALL) {
+    tsr[1] = (3.0 * eps_r) / (1.0 + 2.0 * eps_r) * tsr[1];
+    tsr[2] = (5.0 * eps_r) / (2.0 + 3.0 * eps_r) * tsr[2];
+  } else {
+    const double ka = kappa * rad, ka2 = ka * ka, ka3 = ka2 * ka;
+    const double kr = kappa * r, kr2 = kr * kr;
+    tsr[0] = exp(ka - kr) / (1.0 + ka) * tsr[0];
+    tsr[1] = 3.0 * eps_r * exp(ka - kr) * (1.0 + kr) * tsr[1];
+    tsr[1] = tsr[1] / (1.0 + ka + eps_r * (2.0 + 2.0 * ka + ka2));
+    tsr[2] = 5.0 * eps_r * exp(ka - kr) * (3.0 + 3.0 * kr + kr2) * tsr[2];
+    tsr[2] = tsr[2] / (6.0 + 6.0 * ka + 2.0 * ka2 + eps_r * (9.0 + 9.0 * ka + 4.0 * ka2 + ka3));
+  }
+}
+
+// bcfl_sdh (vpmg.c:3479-3645) on the boundary faces.
+__global__ void __launch_bounds__(kB)
+    k_bc_sdh(Geo g, const double* xf, const double* yf, const double* zf, SdhMoments M,
+             double* gx, double* gy, double* gz) {
+  const int f = blockIdx.y;
+  const int t = blockIdx.x * kB + threadIdx.x;
+  if (t >= face_count(g, f)) return;
+  int i, j, k;
+  face_point(g, f, t, i, j, k);
+  const double xr = xf[i] - M.pos[0], yr = yf[j] - M.pos[1], zr = zf[k] - M.pos[2];
+  const double dist = sqrt(xr * xr + yr * yr + zr * zr);
+  double tsr[3];
+  multipole_tensor(dist, M.xkappa, M.eps_p, M.eps_w, M.size, tsr);
+  const double pre = M.pre;
+  double val = pre * M.q * tsr[0];
+  val -= pre * M.u[0] * xr * tsr[1];
+  val -= pre * M.u[1] * yr * tsr[1];
+  val -= pre * M.u[2] * zr * tsr[1];
+  val += pre * M.qxx * xr * xr * tsr[2];
+  val += pre * M.qyy * yr * yr * tsr[2];
+  val += pre * M.qzz * zr * zr * tsr[2];
+  val += pre * 2.0 * M.qxy * xr * yr * tsr[2];
+  val += pre * 2.0 * M.qxz * xr * zr * tsr[2];
+  val += pre * 2.0 * M.qyz * yr * zr * tsr[2];
+  *face_slot(g, f, i, j, k, gx, gy, gz) = val;
+}
+
 __device__ inline double trilin(const OldGrid& o, double x, double y, double z, int* bad) {
   const double VS = VSMALL;
   const bool ok = (x >= (o.xmin - VS)) && (y >= (o.ymin - VS)) && (z >= (o.zmin - VS)) &&
@@ -1024,6 +1074,14 @@ void bc_mdh(hipStream_t s, const Geo& g, const double* xf, const double* yf, con
   k_bc_mdh<<<dim3((mx + kB - 1) / kB, 6, nch), kB, 0, s>>>(g, xf, yf, zf, A, pre1, xkappa, nch, mx,
                                                            part);
   k_bc_mdh_sum<<<dim3((mx + kB - 1) / kB, 6), kB, 0, s>>>(g, nch, mx, part, gx, gy, gz);
+}
+
+void bc_sdh(hipStream_t s, const Geo& g, const double* xf, const double* yf, const double* zf,
+            const SdhMoments& M, double* gx, double* gy, double* gz) {
+  int mx = g.ny * g.nz;
+  if (g.nx * g.nz > mx) mx = g.nx * g.nz;
+  if (g.nx * g.ny > mx) mx = g.nx * g.ny;
+  k_bc_sdh<<<dim3((mx + kB - 1) / kB, 6), kB, 0, s>>>(g, xf, yf, zf, M, gx, gy, gz);
 }
 
 void focus_bound(hipStream_t s, const Geo& g, double xminN, double yminN, double zminN,

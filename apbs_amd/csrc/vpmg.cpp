@@ -1622,8 +1622,41 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
       pmg::bc_mdh(d->s, g, d->xf, d->yf, d->zf, A, pre1, pbe->xkappa, part, d->gx, d->gy, d->gz);
       HIPCHECK(hipStreamSynchronize(d->s));
       dfree(part);
+    } else if (p->bcfl == BCFL_SDH) {
+      // bcfl_sdh (vpmg.c:3479-3645): moments summed on the host in atom order
+      pmg::SdhMoments M{};
+      const double* c = pbe->soluteCenter;
+      double quad[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, tr[9];
+      for (int d = 0; d < 3; d++) M.pos[d] = c[d];
+      M.size = pbe->soluteRadius;
+      for (int a = 0; a < d->natoms; a++) {
+        const Vatom* at = &al->atoms[a];
+        const double xr = at->position[0] - c[0], yr = at->position[1] - c[1],
+                     zr = at->position[2] - c[2];
+        const double ch = at->charge;
+        M.q += ch;
+        M.u[0] += xr * ch;
+        M.u[1] += yr * ch;
+        M.u[2] += zr * ch;
+        tr[0] = xr * xr * ch; tr[1] = xr * yr * ch; tr[2] = xr * zr * ch;
+        tr[3] = yr * xr * ch; tr[4] = yr * yr * ch; tr[5] = yr * zr * ch;
+        tr[6] = zr * xr * ch; tr[7] = zr * yr * ch; tr[8] = zr * zr * ch;
+        const double qave = (tr[0] + tr[4] + tr[8]) / 3.0;
+        for (int e = 0; e < 9; e++) quad[e] += 1.5 * ((e % 4 == 0) ? (tr[e] - qave) : tr[e]);
+      }
+      M.qxx = quad[0] / 3.0;
+      M.qxy = quad[1] / 3.0;
+      M.qxz = quad[2] / 3.0;
+      M.qyy = quad[4] / 3.0;
+      M.qyz = quad[5] / 3.0;
+      M.qzz = quad[8] / 3.0;
+      M.pre = ((Vunit_ec * Vunit_ec) / (4 * VPI * Vunit_eps0 * Vunit_kb * pbe->T)) * (1.0e10);
+      M.xkappa = pbe->xkappa;
+      M.eps_p = pbe->soluteDiel;
+      M.eps_w = pbe->solventDiel;
+      pmg::bc_sdh(d->s, g, d->xf, d->yf, d->zf, M, d->gx, d->gy, d->gz);
     } else if (p->bcfl != BCFL_ZERO) {
-      fprintf(stderr, "Vpmg_fillco: bcfl %d is not supported (zero, mdh, focus are)\n",
+      fprintf(stderr, "Vpmg_fillco: bcfl %d is not supported (zero, sdh, mdh, focus are)\n",
               (int)p->bcfl);
       return 0;
     }

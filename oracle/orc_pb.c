@@ -551,6 +551,95 @@ static void bc_mdh(orc_pmg *m)
             }
 }
 
+/* multipolebc (vpmg.c:3369-3477): the radial factors of the monopole,
+ * dipole and traceless-quadrupole potentials outside a sphere of radius rad
+ * and dielectric eps_p in a solvent eps_w with Debye-Hueckel kappa. */
+static void multipole_tensor(double r, double kappa, double eps_p, double eps_w, double rad,
+                             double tsr[3])
+{
+    double eps_r = eps_w / eps_p;
+    double r2 = r * r, r3 = r2 * r, r5 = r3 * r2;
+    tsr[0] = (1.0 / eps_w) / r;
+    tsr[1] = (1.0 / eps_w) * (-1.0) / r3;
+    tsr[2] = (1.0 / eps_w) * (3.0) / r5;
+    if (kappa < O_VSMALL) {
+        tsr[1] = (3.0 * eps_r) / (1.0 + 2.0 * eps_r) * tsr[1];
+        tsr[2] = (5.0 * eps_r) / (2.0 + 3.0 * eps_r) * tsr[2];
+    } else {
+        double ka = kappa * rad, ka2 = ka * ka, ka3 = ka2 * ka;
+        double kr = kappa * r, kr2 = kr * kr;
+        tsr[0] = exp(ka - kr) / (1.0 + ka) * tsr[0];
+        tsr[1] = 3.0 * eps_r * exp(ka - kr) * (1.0 + kr) * tsr[1];
+        tsr[1] = tsr[1] / (1.0 + ka + eps_r * (2.0 + 2.0 * ka + ka2));
+        tsr[2] = 5.0 * eps_r * exp(ka - kr) * (3.0 + 3.0 * kr + kr2) * tsr[2];
+        tsr[2] = tsr[2] / (6.0 + 6.0 * ka + 2.0 * ka2 + eps_r * (9.0 + 9.0 * ka + 4.0 * ka2 + ka3));
+    }
+}
+
+/* bcfl_sdh (vpmg.c:3479-3645): one sphere at the solute centre with the
+ * solute radius, carrying the molecule's charge, dipole and traceless
+ * quadrupole (atom order sums). */
+static void bc_sdh(orc_pmg *m)
+{
+    int nx = m->nx, ny = m->ny, nz = m->nz;
+    double pre = (O_Vunit_ec * O_Vunit_ec) / (4 * O_VPI * O_Vunit_eps0 * O_Vunit_kb * m->p.temp);
+    pre = pre * (1.0e10);
+    double xk = m->xkappa, size = m->soluteRadius;
+    const double *pos = m->center;
+    double q = 0.0, dip[3] = {0, 0, 0}, quad[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, tr[9];
+    for (int a = 0; a < m->natoms; a++) {
+        double xr = m->apos[3 * a] - pos[0], yr = m->apos[3 * a + 1] - pos[1],
+               zr = m->apos[3 * a + 2] - pos[2];
+        double c = m->achg[a];
+        q += c;
+        dip[0] += xr * c;
+        dip[1] += yr * c;
+        dip[2] += zr * c;
+        tr[0] = xr * xr * c; tr[1] = xr * yr * c; tr[2] = xr * zr * c;
+        tr[3] = yr * xr * c; tr[4] = yr * yr * c; tr[5] = yr * zr * c;
+        tr[6] = zr * xr * c; tr[7] = zr * yr * c; tr[8] = zr * zr * c;
+        double qave = (tr[0] + tr[4] + tr[8]) / 3.0;
+        quad[0] += 1.5 * (tr[0] - qave);
+        quad[1] += 1.5 * (tr[1]);
+        quad[2] += 1.5 * (tr[2]);
+        quad[3] += 1.5 * (tr[3]);
+        quad[4] += 1.5 * (tr[4] - qave);
+        quad[5] += 1.5 * (tr[5]);
+        quad[6] += 1.5 * (tr[6]);
+        quad[7] += 1.5 * (tr[7]);
+        quad[8] += 1.5 * (tr[8] - qave);
+    }
+    double ux = dip[0], uy = dip[1], uz = dip[2];
+    double qxx = quad[0] / 3.0, qxy = quad[1] / 3.0, qxz = quad[2] / 3.0;
+    double qyy = quad[4] / 3.0, qyz = quad[5] / 3.0, qzz = quad[8] / 3.0;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                if (!((k == 0) || (k == nz - 1) || (j == 0) || (j == ny - 1) || (i == 0) || (i == nx - 1)))
+                    continue;
+                double xr = m->xf[i] - pos[0], yr = m->yf[j] - pos[1], zr = m->zf[k] - pos[2];
+                double dist = sqrt(OSQR(xr) + OSQR(yr) + OSQR(zr));
+                double t[3];
+                multipole_tensor(dist, xk, m->p.pdie, m->p.sdie, size, t);
+                double val = pre * q * t[0];
+                val -= pre * ux * xr * t[1];
+                val -= pre * uy * yr * t[1];
+                val -= pre * uz * zr * t[1];
+                val += pre * qxx * xr * xr * t[2];
+                val += pre * qyy * yr * yr * t[2];
+                val += pre * qzz * zr * zr * t[2];
+                val += pre * 2.0 * qxy * xr * yr * t[2];
+                val += pre * 2.0 * qxz * xr * zr * t[2];
+                val += pre * 2.0 * qyz * yr * zr * t[2];
+                if (i == 0) m->gxcf[j + ny * k] = val;
+                if (i == nx - 1) m->gxcf[j + ny * k + (size_t)ny * nz] = val;
+                if (j == 0) m->gycf[i + nx * k] = val;
+                if (j == ny - 1) m->gycf[i + nx * k + (size_t)nx * nz] = val;
+                if (k == 0) m->gzcf[i + nx * j] = val;
+                if (k == nz - 1) m->gzcf[i + nx * j + (size_t)nx * ny] = val;
+            }
+}
+
 /* ---------------------------------------------------------------- focusing */
 
 static double trilin_old(const orc_pmg *o, double x, double y, double z, int *ok)
@@ -811,7 +900,8 @@ int orc_fillco(orc_pmg *m)
         memset(m->gycf, 0, (size_t)2 * m->nx * m->nz * sizeof(double));
         memset(m->gzcf, 0, (size_t)2 * m->nx * m->ny * sizeof(double));
         if (m->p.bcfl == ORC_BCFL_MDH) bc_mdh(m);
-        else if (m->p.bcfl != ORC_BCFL_ZERO) return 0;   /* sdh: not restated */
+        else if (m->p.bcfl == ORC_BCFL_SDH) bc_sdh(m);
+        else if (m->p.bcfl != ORC_BCFL_ZERO) return 0;
     }
     m->filled = 1;
     return 1;

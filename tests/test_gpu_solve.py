@@ -247,3 +247,43 @@ def test_graph_replay_equals_eager(npbe, precision, monkeypatch):
         assert eg == ee
         assert np.array_equal(ug, ue)
     assert res["1"][0][0] == res["1"][2][0]
+
+
+@pytest.mark.parametrize("srfm,key", [("mol", "actin_mol_auto"), ("smol", "actin_smol_auto")])
+def test_actin_dimer_golden(srfm, key):
+    """examples/actin-dimer/apbs-{s}mol-auto.in on the GPU: mg-auto at 161^3,
+    npbe, bcfl sdh on the coarse level, chgm spl0, 11,754-atom complex;
+    every focusing level's energy against tests/test_cases.cfg.  The binding
+    energy is a difference of numbers 5000x larger, so the reference's own
+    convergence error (etol 1e-6) bounds how closely it can be met."""
+    c = od.mol_center(od.read_pqr(os.path.join(HERE, "golden", "actin_complex.pqr"))[0])
+    pbe = dict(npbe=True, pdie=2.0, sdie=78.4, srfm=srfm, chgm="spl0", srad=1.4, swin=0.3,
+               sdens=10.0, temp=298.15, bcfl="sdh", ions=[(1, 0.050, 2.0), (-1, 0.050, 2.0)])
+    fine = {}
+    for name in ("mol1", "mol2", "complex"):
+        atoms = od.read_pqr(os.path.join(HERE, "golden", f"actin_{name}.pqr"))
+        mol = driver.Molecule(*atoms)
+        e, _ = driver.run_auto(mol, [161] * 3, [156, 121, 162], [112, 91, 116], c, c, pbe)
+        want = G[key]["levels"][name]
+        assert len(e) == len(want)
+        for got, w in zip(e, want):
+            assert rel(got, w) < 1e-6, (name, got, w)
+        fine[name] = e[-1]
+    dg = fine["complex"] - fine["mol1"] - fine["mol2"]
+    print(f"{key}: binding {dg:.6f} kJ/mol, golden {G[key]['difference']:.6f}")
+    assert rel(dg, G[key]["difference"]) < 1e-3
+
+
+@pytest.mark.parametrize("salt", [0.0, 0.15])
+def test_sdh_boundary_against_oracle(salt):
+    """bcfl sdh boundary values (vpmg.c:3479-3645) against the oracle."""
+    atoms = small_mol(60, 6.0, 23, net=3)
+    c = od.mol_center(atoms[0])
+    pbe = pbe_of("mol", "spl0", bcfl="sdh", ions=[(1, salt, 2.0), (-1, salt, 2.0)])
+    mol = driver.Molecule(*atoms)
+    g = driver.Calc(mol, driver.manual_mgparm([41] * 3, c, grid=[0.5] * 3, chgm="spl0"), pbe)
+    g.fillco()
+    o = od.OrcCalc(od.make_params([41] * 3, [0.5] * 3, c, 4, pbe), atoms)
+    o.fillco()
+    for a, b in zip(g.boundary(), o.boundary()):
+        assert np.abs(a - b).max() <= 8 * np.finfo(float).eps * np.abs(b).max()

