@@ -51,6 +51,9 @@ void sas_flags(hipStream_t s, const AtomsDev& A, const Cells& C, const double* r
                double probe, unsigned char* flags);
 // compact flagged points to xyz (order of (atom, ref) index); returns count
 // on the host (synchronises).
+// number of accessible points flagged by sas_flags
+int sas_count(hipStream_t s, const AtomsDev& A, int nref, const unsigned char* flags,
+              int* counter);
 int sas_compact(hipStream_t s, const AtomsDev& A, const double* ref, int nref, double probe,
                 const unsigned char* flags, double* xyz, int* counter, int cap);
 

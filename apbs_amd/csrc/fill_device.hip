@@ -320,24 +320,42 @@ __global__ void __launch_bounds__(kB)
   }
 }
 
-__global__ void k_sas_compact(AtomsDev A, const double* ref, int nref, double probe,
-                              const unsigned char* flags, double* xyz, int* counter, int cap) {
+// Accessible SAS points: counted, then written to a buffer of that size.
+// Each wavefront takes one atomic slot range for its accepted lanes (ballot
+// and lane prefix); the marking that consumes the points is order-free.
+__global__ void __launch_bounds__(kB)
+    k_sas_count(const unsigned char* flags, long n, int* counter) {
   const long t = (long)blockIdx.x * kB + threadIdx.x;
-  if (t >= (long)A.n * nref) return;
-  if (!flags[t]) return;
+  const bool ok = t < n && flags[t];
+  const unsigned long long m = __ballot(ok);
+  if ((threadIdx.x % 64) == 0 && m) atomicAdd(counter, (int)__popcll(m));
+}
+
+__global__ void __launch_bounds__(kB)
+    k_sas_compact(AtomsDev A, const double* ref, int nref, double probe,
+                  const unsigned char* flags, double* xyz, int* counter, int cap) {
+  const long t = (long)blockIdx.x * kB + threadIdx.x;
+  const long n = (long)A.n * nref;
+  const bool ok = t < n && flags[t];
+  const unsigned long long m = __ballot(ok);
+  const int lane = threadIdx.x % 64;
+  int base = 0;
+  if (lane == 0 && m) base = atomicAdd(counter, (int)__popcll(m));
+  base = __shfl(base, 0, 64);
+  if (!ok) return;
+  const int slot = base + (int)__popcll(m & ((1ull << lane) - 1ull));
+  if (slot >= cap) return;
   const int a = (int)(t / nref), q = (int)(t % nref);
   const double rad = A.rad[a] + probe;
   const double* ap = A.pos + 3 * a;
-  const int slot = atomicAdd(counter, 1);
-  if (slot >= cap) return;
   xyz[3 * slot] = rad * (ref[3 * q]) + ap[0];
   xyz[3 * slot + 1] = rad * (ref[3 * q + 1]) + ap[1];
   xyz[3 * slot + 2] = rad * (ref[3 * q + 2]) + ap[2];
 }
 
-// markSphere (vpmg.c:6444-6491) as a scatter: one thread per sphere writes
-// val at every grid point of the sphere.  Every writer of a point writes the
-// same constant, so the result does not depend on thread order.
+// markSphere (vpmg.c:6444-6491) as a scatter: every grid point of a sphere
+// gets val.  Every writer of a point writes the same constant, so the result
+// does not depend on the order of the writes.
 // mode 0: atom spheres r_a + add with the strict on-mesh test (dielectric),
 // mode 1: atom spheres r_a + add with the inflated on-mesh test (ions),
 // mode 2: SAS points with radius add (no mesh test).
@@ -1026,6 +1044,17 @@ void sas_flags(hipStream_t s, const AtomsDev& A, const Cells& C, const double* r
                double probe, unsigned char* flags) {
   if (A.n <= 0) return;
   k_sas_flags<<<(unsigned)A.n, kB, 0, s>>>(A, C, ref, nref, probe, flags);
+}
+
+int sas_count(hipStream_t s, const AtomsDev& A, int nref, const unsigned char* flags,
+              int* counter) {
+  const long n = (long)A.n * nref;
+  hipMemsetAsync(counter, 0, sizeof(int), s);
+  k_sas_count<<<(unsigned)((n + kB - 1) / kB), kB, 0, s>>>(flags, n, counter);
+  int h = 0;
+  hipMemcpyAsync(&h, counter, sizeof(int), hipMemcpyDeviceToHost, s);
+  hipStreamSynchronize(s);
+  return h;
 }
 
 int sas_compact(hipStream_t s, const AtomsDev& A, const double* ref, int nref, double probe,
