@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(kBlock)
 // ring and the whole tile row of plane k is written.  Each value is the
 // in-place sweep's value bit for bit (same terms, same order, no FMA).
 // Points outside the interior are not written.
-constexpr int kTX = 64, kFT = 256;
+constexpr int kTX = 64, kFT = 256, kXCD = 8;
 
 // kTY rows (a multiple of 4: each thread owns kTY/4 rows of one column) and
 // kKC planes per workgroup.
@@ -84,8 +84,16 @@ __global__ void __launch_bounds__(kFT)
   constexpr int FPR = RX / 2, FN = FPR * RY;  // F points per region row / per plane
   __shared__ T ring[3][RN];
   const int tid = threadIdx.x;
-  const int i0 = 1 + blockIdx.x * kTX, j0 = 1 + blockIdx.y * kTY;
-  const int k0 = 1 + blockIdx.z * kKC;
+  // workgroups are dealt round-robin to the 8 XCDs: give XCD x the tiles
+  // [x*per, (x+1)*per) in z-major order, so tiles that share halo rows and
+  // planes share an L2
+  const int nbx = (nx - 2 + kTX - 1) / kTX, nby = (ny - 2 + kTY - 1) / kTY;
+  const int nb = nbx * nby * ((nz - 2 + kKC - 1) / kKC);
+  const int per = (nb + kXCD - 1) / kXCD;
+  const int tile = (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
+  if (tile >= nb) return;
+  const int i0 = 1 + (tile % nbx) * kTX, j0 = 1 + ((tile / nbx) % nby) * kTY;
+  const int k0 = 1 + (tile / (nbx * nby)) * kKC;
   const int k1 = min(k0 + kKC, nz - 1);  // planes k0 .. k1-1
   const size_t nxy = (size_t)nx * ny;
   const int F = first;                   // parity of the first colour
@@ -258,13 +266,26 @@ __device__ inline double block_sum(double v, double* sh) {
   return r;
 }
 
+// The interior points [begin, end) (x fastest) a reduction block owns: kParts
+// contiguous chunks, dealt so that XCD x (blocks x, x + 8, ...) owns the
+// chunks [x * kParts/8, (x + 1) * kParts/8), a contiguous slab of the grid.
+__device__ inline void part_range(int n, int& begin, int& end) {
+  const int b = blockIdx.x;
+  const int c = (b % 8) * (kParts / 8) + b / 8;
+  const int chunk = (n + kParts - 1) / kParts;
+  begin = c * chunk;
+  end = begin + chunk < n ? begin + chunk : n;
+}
+
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_nrm1_part(int nx, int ny, int nz, const T* __restrict__ x, double* part) {
   __shared__ double sh[kBlock];
   const int ni = nx - 2, nij = ni * (ny - 2), n = nij * (nz - 2);
+  int t0, t1;
+  part_range(n, t0, t1);
   double acc = 0.0;
-  for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += kParts * kBlock) {
+  for (int t = t0 + threadIdx.x; t < t1; t += kBlock) {
     const int k = 1 + t / nij, rem = t - (k - 1) * nij;
     const int j = 1 + rem / ni, i = 1 + rem - (j - 1) * ni;
     acc += fabs((double)x[IX(i, j, k, nx, ny)]);
@@ -274,8 +295,9 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // The 7-point residual and the partial sums of its L1 norm in one pass: the
-// points are visited in k_nrm1_part's order, so the norm is bitwise that of
-// storing r and calling k_nrm1_part.  r is stored when non-null.
+// points are visited in k_nrm1_part's order (part_range), so the norm is
+// bitwise that of storing r and calling k_nrm1_part.  r is stored when
+// non-null.
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_resid7_nrm(int nx, int ny, int nz, const T* __restrict__ oE, const T* __restrict__ oN,
@@ -284,8 +306,10 @@ __global__ void __launch_bounds__(kBlock)
   __shared__ double sh[kBlock];
   const int ni = nx - 2, nij = ni * (ny - 2), n = nij * (nz - 2);
   const size_t nxy = (size_t)nx * ny;
+  int t0, t1;
+  part_range(n, t0, t1);
   double acc = 0.0;
-  for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += kParts * kBlock) {
+  for (int t = t0 + threadIdx.x; t < t1; t += kBlock) {
     const int k = 1 + t / nij, rem = t - (k - 1) * nij;
     const int j = 1 + rem / ni, i = 1 + rem - (j - 1) * ni;
     const size_t p = IX(i, j, k, nx, ny);
@@ -577,8 +601,10 @@ __global__ void __launch_bounds__(kBlock)
   __shared__ double sh[kBlock];
   const int ni = nx - 2, nij = ni * (ny - 2), n = nij * (nz - 2);
   const size_t nxy = (size_t)nx * ny;
+  int t0, t1;
+  part_range(n, t0, t1);
   double acc = 0.0;
-  for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += kParts * kBlock) {
+  for (int t = t0 + threadIdx.x; t < t1; t += kBlock) {
     const int k = 1 + t / nij, rem = t - (k - 1) * nij;
     const int j = 1 + rem / ni, i = 1 + rem - (j - 1) * ni;
     const size_t p = IX(i, j, k, nx, ny);
@@ -892,8 +918,8 @@ Ops27<T> to27(const Ops<T>& A) {
 template <class T, int TY, int KC>
 void fused_sweep(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
                  const T* in, T* out, int first) {
-  dim3 g((unsigned)((nx - 2 + kTX - 1) / kTX), (unsigned)((ny - 2 + TY - 1) / TY),
-         (unsigned)((nz - 2 + KC - 1) / KC));
+  const int nb = ((nx - 2 + kTX - 1) / kTX) * ((ny - 2 + TY - 1) / TY) * ((nz - 2 + KC - 1) / KC);
+  const unsigned g = (unsigned)(((nb + kXCD - 1) / kXCD) * kXCD);
   k_gsrb7_fused<T, TY, KC><<<g, kFT, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, in, out,
                                              first);
 }

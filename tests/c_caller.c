@@ -71,8 +71,8 @@ static int solve(const char *pqr, int dime, double h) {
         hi[j] = mg->center[j] + 0.5 * mg->glen[j];
     }
     Vpmg_setPart(pmg, lo, hi, mg->partDisjOwnSide);
-    /* energyMG (routines.c:1756-1759): kT -> kJ/mol */
-    double e = Vpmg_energy(pmg, 1) * 1.3806581e-23 * 298.15 * 1e-3 * 6.0221367e+23;
+    /* energyMG (routines.c:1756-1759): kT -> kJ/mol, in that order */
+    double e = 1.3806581e-23 * 298.15 * 1e-3 * 6.0221367e+23 * Vpmg_energy(pmg, 1);
     /* forceMG: total = qf + ib + db (Vpmg_force) */
     double f[3];
     if (!Vpmg_force(pmg, f, 0, VSM_SPLINE, VCM_BSPL2)) return 6;
@@ -80,7 +80,7 @@ static int solve(const char *pqr, int dime, double h) {
     double umax = 0;
     for (int i = 0; i < pmgp->nx * pmgp->ny * pmgp->nz; i++)
         if (pmg->u[i] > umax) umax = pmg->u[i];
-    printf("energy %.12f\nforce %.12e %.12e %.12e\numax %.12f\n", e, f[0], f[1], f[2], umax);
+    printf("energy %.17g\nforce %.17g %.17g %.17g\numax %.17g\n", e, f[0], f[1], f[2], umax);
     Vpmg_dtor(&pmg);
     Vpmgp_dtor(&pmgp);
     MGparm_dtor(&mg);
