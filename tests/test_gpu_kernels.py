@@ -53,7 +53,7 @@ def interior(x, n3):
 
 @pytest.mark.parametrize("st,n3", [(7, (17, 19, 21)), (7, (33, 33, 33)), (7, (67, 13, 20)),
                                    (7, (130, 6, 11)), (7, (5, 70, 3)), (27, (17, 17, 17)),
-                                   (27, (9, 11, 13))])
+                                   (27, (9, 11, 13)), (27, (45, 43, 41))])
 @pytest.mark.parametrize("adj", [0, 1])
 def test_gsrb_bitwise(L, O, st, n3, adj):
     rng = np.random.default_rng(7 + st + adj)
