@@ -225,35 +225,6 @@ __global__ void __launch_bounds__(kBlock)
   x[p] = (fc[p] + (s.o + s.u + s.d)) / D[p];
 }
 
-// All nu sweeps of the 8-colour order in one workgroup, a barrier between
-// colours: for small coarse levels, where one launch per colour costs more
-// than the colour's work.  Per point the same expression as k_gsrb27.
-constexpr int kWG = 1024;
-constexpr int kSmallLevel = 16384;  // interior points handled by one workgroup
-
-template <class T>
-__global__ void __launch_bounds__(kWG)
-    k_gsrb27_wg(int nx, int ny, int nz, Ops27<T> A, const T* __restrict__ D,
-                const T* __restrict__ fc, T* x, int nu, int iadjoint) {
-  const size_t nxy = (size_t)nx * ny;
-  for (int it = 0; it < nu; it++)
-    for (int cc = 0; cc < 8; cc++) {
-      const int c = iadjoint ? 7 - cc : cc;
-      const int i0 = (c & 1) ? 1 : 2, j0 = ((c >> 1) & 1) ? 1 : 2, k0 = ((c >> 2) & 1) ? 1 : 2;
-      const int ni = (nx - 2 - i0) / 2 + 1, nj = (ny - 2 - j0) / 2 + 1, nk = (nz - 2 - k0) / 2 + 1;
-      if (ni > 0 && nj > 0 && nk > 0) {
-        const int tot = ni * nj * nk;
-        for (int t = threadIdx.x; t < tot; t += kWG) {
-          const int ti = t % ni, tj = (t / ni) % nj, tk = t / (ni * nj);
-          const size_t p = IX(i0 + 2 * ti, j0 + 2 * tj, k0 + 2 * tk, nx, ny);
-          P27<T> sm = part27<T>(A.a, x, p, nx, nxy);
-          x[p] = (fc[p] + (sm.o + sm.u + sm.d)) / D[p];
-        }
-      }
-      __syncthreads();
-    }
-}
-
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_resid7(int nx, int ny, int nz, const T* __restrict__ oE, const T* __restrict__ oN,
@@ -911,39 +882,6 @@ __global__ void __launch_bounds__(1024) k_dense_invert_lds(double* Mg, int N) {
   for (int e = tid; e < N * N; e += 1024) Mg[e] = M[e];
 }
 
-// N <= 64: one wavefront, lane j owns column j of the LDS matrix; the same
-// per-element expressions as above.
-__global__ void __launch_bounds__(64) k_dense_invert_wave(double* Mg, int N) {
-  extern __shared__ double sm[];
-  double* M = sm;
-  double* col = sm + (size_t)N * N;
-  double* prow = col + N;
-  const int j = threadIdx.x;
-  for (int e = j; e < N * N; e += 64) M[e] = Mg[e];
-  __syncthreads();
-  for (int k = 0; k < N; k++) {
-    if (j < N) {
-      col[j] = M[j * N + k];
-      const double v = (j == k) ? 1.0 : M[k * N + j];
-      prow[j] = v / M[k * N + k];
-    }
-    __syncthreads();
-    if (j < N) {
-      const double pj = prow[j];
-      for (int i = 0; i < N; i++) {
-        if (i == k) {
-          M[i * N + j] = pj;
-        } else {
-          const double base = (j == k) ? 0.0 : M[i * N + j];
-          M[i * N + j] = base - col[i] * pj;
-        }
-      }
-    }
-    __syncthreads();
-  }
-  for (int e = j; e < N * N; e += 64) Mg[e] = M[e];
-}
-
 template <class T>
 __global__ void k_dense_solve(int nx, int ny, int nz, const double* Minv, int N, const T* f,
                               T* x) {
@@ -1044,10 +982,6 @@ template <class T>
 void gsrb27(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc, T* x,
             int nu, int iadjoint) {
   const Ops27<T> o = to27(A);
-  if ((long)(nx - 2) * (ny - 2) * (nz - 2) <= kSmallLevel) {
-    k_gsrb27_wg<T><<<1, kWG, 0, s>>>(nx, ny, nz, o, D, fc, x, nu, iadjoint);
-    return;
-  }
   for (int it = 0; it < nu; it++)
     for (int cc = 0; cc < 8; cc++) {
       const int c = iadjoint ? 7 - cc : cc;
@@ -1189,10 +1123,6 @@ void dense_assemble(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A,
 
 void dense_invert(hipStream_t s, double* M, double* work, int N) {
   const size_t lds = ((size_t)N * N + 2 * N) * sizeof(double);
-  if (N <= 64) {
-    k_dense_invert_wave<<<1, 64, lds, s>>>(M, N);
-    return;
-  }
   if (lds <= 64 * 1024 && (size_t)N * N <= (size_t)kInvPer * 1024) {
     k_dense_invert_lds<<<1, 1024, lds, s>>>(M, N);
     return;
