@@ -51,11 +51,12 @@ void sas_flags(hipStream_t s, const AtomsDev& A, const Cells& C, const double* r
                double probe, unsigned char* flags);
 // compact flagged points to xyz (order of (atom, ref) index); returns count
 // on the host (synchronises).
-// number of accessible points flagged by sas_flags
-int sas_count(hipStream_t s, const AtomsDev& A, int nref, const unsigned char* flags,
-              int* counter);
+// Writes the accessible SAS points (flag order) to a buffer obtained from
+// alloc(doubles), returned in *xyz_out; returns their number.  scratch holds
+// 1024 ints.
 int sas_compact(hipStream_t s, const AtomsDev& A, const double* ref, int nref, double probe,
-                const unsigned char* flags, double* xyz, int* counter, int cap);
+                const unsigned char* flags, int* scratch, double** xyz_out,
+                double* (*alloc)(size_t));
 
 // markSphere (vpmg.c:6444-6491) for a set of spheres: mode 0 atoms (radius
 // r_a + add, strict on-mesh test, dielectric), mode 1 atoms (r_a + add,

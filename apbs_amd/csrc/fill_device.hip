@@ -8,6 +8,7 @@
 #include "fill_device.h"
 
 #include <cmath>
+#include <vector>
 
 namespace pmg {
 namespace {
@@ -320,37 +321,67 @@ __global__ void __launch_bounds__(kB)
   }
 }
 
-// Accessible SAS points: counted, then written to a buffer of that size.
-// Each wavefront takes one atomic slot range for its accepted lanes (ballot
-// and lane prefix); the marking that consumes the points is order-free.
+// Accessible SAS points, written in flag order without atomics: kSasParts
+// blocks own contiguous ranges of the flags; pass 1 counts each range, the
+// host turns the counts into offsets, pass 2 writes each range's points at
+// its offset (wavefront ballot prefix within the block).
+constexpr int kSasParts = 1024;
+
+__device__ inline void sas_range(long n, long& b0, long& b1) {
+  const long chunk = (n + kSasParts - 1) / kSasParts;
+  b0 = (long)blockIdx.x * chunk;
+  b1 = b0 + chunk < n ? b0 + chunk : n;
+}
+
 __global__ void __launch_bounds__(kB)
-    k_sas_count(const unsigned char* flags, long n, int* counter) {
-  const long t = (long)blockIdx.x * kB + threadIdx.x;
-  const bool ok = t < n && flags[t];
-  const unsigned long long m = __ballot(ok);
-  if ((threadIdx.x % 64) == 0 && m) atomicAdd(counter, (int)__popcll(m));
+    k_sas_count(const unsigned char* flags, long n, int* counts) {
+  __shared__ int sh[kB / 64];
+  long b0, b1;
+  sas_range(n, b0, b1);
+  int c = 0;
+  for (long t = b0 + threadIdx.x; t < b1; t += kB) c += flags[t] ? 1 : 0;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+  if ((threadIdx.x % 64) == 0) sh[threadIdx.x / 64] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int w = 0; w < kB / 64; w++) tot += sh[w];
+    counts[blockIdx.x] = tot;
+  }
 }
 
 __global__ void __launch_bounds__(kB)
     k_sas_compact(AtomsDev A, const double* ref, int nref, double probe,
-                  const unsigned char* flags, double* xyz, int* counter, int cap) {
-  const long t = (long)blockIdx.x * kB + threadIdx.x;
+                  const unsigned char* flags, const int* offsets, double* xyz) {
+  __shared__ int wsum[kB / 64];
   const long n = (long)A.n * nref;
-  const bool ok = t < n && flags[t];
-  const unsigned long long m = __ballot(ok);
-  const int lane = threadIdx.x % 64;
-  int base = 0;
-  if (lane == 0 && m) base = atomicAdd(counter, (int)__popcll(m));
-  base = __shfl(base, 0, 64);
-  if (!ok) return;
-  const int slot = base + (int)__popcll(m & ((1ull << lane) - 1ull));
-  if (slot >= cap) return;
-  const int a = (int)(t / nref), q = (int)(t % nref);
-  const double rad = A.rad[a] + probe;
-  const double* ap = A.pos + 3 * a;
-  xyz[3 * slot] = rad * (ref[3 * q]) + ap[0];
-  xyz[3 * slot + 1] = rad * (ref[3 * q + 1]) + ap[1];
-  xyz[3 * slot + 2] = rad * (ref[3 * q + 2]) + ap[2];
+  long b0, b1;
+  sas_range(n, b0, b1);
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  int base = offsets[blockIdx.x];
+  for (long s0 = b0; s0 < b1; s0 += kB) {
+    const long t = s0 + threadIdx.x;
+    const bool ok = t < b1 && flags[t];
+    const unsigned long long m = __ballot(ok);
+    if (lane == 0) wsum[w] = (int)__popcll(m);
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int v = 0; v < kB / 64; v++) {
+      if (v < w) before += wsum[v];
+      all += wsum[v];
+    }
+    if (ok) {
+      const int slot = base + before + (int)__popcll(m & ((1ull << lane) - 1ull));
+      const int a = (int)(t / nref), q = (int)(t % nref);
+      const double rad = A.rad[a] + probe;
+      const double* ap = A.pos + 3 * a;
+      xyz[3 * (size_t)slot] = rad * (ref[3 * q]) + ap[0];
+      xyz[3 * (size_t)slot + 1] = rad * (ref[3 * q + 1]) + ap[1];
+      xyz[3 * (size_t)slot + 2] = rad * (ref[3 * q + 2]) + ap[2];
+    }
+    base += all;
+    __syncthreads();
+  }
 }
 
 // markSphere (vpmg.c:6444-6491) as a scatter: every grid point of a sphere
@@ -1046,27 +1077,25 @@ void sas_flags(hipStream_t s, const AtomsDev& A, const Cells& C, const double* r
   k_sas_flags<<<(unsigned)A.n, kB, 0, s>>>(A, C, ref, nref, probe, flags);
 }
 
-int sas_count(hipStream_t s, const AtomsDev& A, int nref, const unsigned char* flags,
-              int* counter) {
-  const long n = (long)A.n * nref;
-  hipMemsetAsync(counter, 0, sizeof(int), s);
-  k_sas_count<<<(unsigned)((n + kB - 1) / kB), kB, 0, s>>>(flags, n, counter);
-  int h = 0;
-  hipMemcpyAsync(&h, counter, sizeof(int), hipMemcpyDeviceToHost, s);
-  hipStreamSynchronize(s);
-  return h;
-}
-
 int sas_compact(hipStream_t s, const AtomsDev& A, const double* ref, int nref, double probe,
-                const unsigned char* flags, double* xyz, int* counter, int cap) {
+                const unsigned char* flags, int* scratch, double** xyz_out,
+                double* (*alloc)(size_t)) {
   const long n = (long)A.n * nref;
-  hipMemsetAsync(counter, 0, sizeof(int), s);
-  k_sas_compact<<<(unsigned)((n + kB - 1) / kB), kB, 0, s>>>(A, ref, nref, probe, flags, xyz,
-                                                              counter, cap);
-  int h = 0;
-  hipMemcpyAsync(&h, counter, sizeof(int), hipMemcpyDeviceToHost, s);
+  k_sas_count<<<kSasParts, kB, 0, s>>>(flags, n, scratch);
+  std::vector<int> cnt(kSasParts), off(kSasParts);
+  hipMemcpyAsync(cnt.data(), scratch, kSasParts * sizeof(int), hipMemcpyDeviceToHost, s);
   hipStreamSynchronize(s);
-  return h;
+  int tot = 0;
+  for (int b = 0; b < kSasParts; b++) {
+    off[b] = tot;
+    tot += cnt[b];
+  }
+  double* xyz = alloc(3 * (size_t)(tot > 0 ? tot : 1));
+  hipMemcpyAsync(scratch, off.data(), kSasParts * sizeof(int), hipMemcpyHostToDevice, s);
+  k_sas_compact<<<kSasParts, kB, 0, s>>>(A, ref, nref, probe, flags, scratch, xyz);
+  hipStreamSynchronize(s);
+  *xyz_out = xyz;
+  return tot;
 }
 
 void mark_spheres(hipStream_t s, const Geo& g, int npts, const double* pos, const double* rad,

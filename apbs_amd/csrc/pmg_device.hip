@@ -728,52 +728,62 @@ struct OutPtrs {
   double* a[14];
 };
 
-// One thread per (coarse interior point, stored coefficient).
+// One 32-lane group per (coarse interior point, stored coefficient): lane d
+// (d = 9(dz+1) + 3(dy+1) + (dx+1), the reference's loop order) computes the
+// inner sum tt_d for the fine neighbour f0 + (dx, dy, dz); lane 0 then adds
+// p1_d * tt_d for d = 0..26 in order.  The inner loops visit only the terms
+// with a non-zero second weight pweight(f + s - g0), i.e. |f + s - g0| <= 1
+// per axis, in the full scan's order, so every sum is the full scan's.
+constexpr int kGalGroups = kBlock / 32;
+
 __global__ void __launch_bounds__(kBlock)
     k_galerkin(int st, int nxf, int nyf, int nzf, Ops27<double> Af, int nxc, int nyc, int nzc,
                OutPtrs out) {
-  // 64 consecutive coarse points per wavefront, one stored direction s per
-  // wavefront: the pruned loop bounds below depend on s only, so the lanes
-  // of a wavefront never diverge.
+  __shared__ double tt_sh[kGalGroups][27];
   const long ni = nxc - 2, nj = nyc - 2, nk = nzc - 2;
   const long nq = ni * nj * nk;
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
-  const long w = t / 64;
-  const int s = (int)(w % 14);
-  const long q = (w / 14) * 64 + (t % 64);
-  if (q >= nq) return;
-  const int I = 1 + (int)(q % ni), J = 1 + (int)((q / ni) % nj), K = 1 + (int)(q / (ni * nj));
-  const int ex = c_fwd[s][0], ey = c_fwd[s][1], ez = c_fwd[s][2];
-  const int I2 = I + ex, J2 = J + ey, K2 = K + ez;
+  const int grp = threadIdx.x / 32, d = threadIdx.x % 32;
+  const long g = (long)blockIdx.x * kGalGroups + grp;
+  const int s = (int)(g % 14);
+  const long q = g / 14;
+  bool live = q < nq;
+  int I = 0, J = 0, K = 0, I2 = 0, J2 = 0, K2 = 0;
+  if (live) {
+    I = 1 + (int)(q % ni);
+    J = 1 + (int)((q / ni) % nj);
+    K = 1 + (int)(q / (ni * nj));
+    I2 = I + c_fwd[s][0];
+    J2 = J + c_fwd[s][1];
+    K2 = K + c_fwd[s][2];
+  }
+  const bool inside = live && !(I2 < 1 || I2 > nxc - 2 || J2 < 1 || J2 > nyc - 2 || K2 < 1 ||
+                                K2 > nzc - 2);
+  if (inside && d < 27) {
+    const int dz = d / 9 - 1, dy = (d / 3) % 3 - 1, dx = d % 3 - 1;
+    const int fx = 2 * I + dx, fy = 2 * J + dy, fz = 2 * K + dz;
+    const int g0x = 2 * I2, g0y = 2 * J2, g0z = 2 * K2;
+    const int sx0 = max(-1, g0x - 1 - fx), sx1 = min(1, g0x + 1 - fx);
+    const int sy0 = max(-1, g0y - 1 - fy), sy1 = min(1, g0y + 1 - fy);
+    const int sz0 = max(-1, g0z - 1 - fz), sz1 = min(1, g0z + 1 - fz);
+    double tt = 0.0;
+    for (int sz = sz0; sz <= sz1; sz++)
+      for (int sy = sy0; sy <= sy1; sy++)
+        for (int sx = sx0; sx <= sx1; sx++) {
+          if (st == 7 && abs(sx) + abs(sy) + abs(sz) > 1) continue;
+          const double p2 = pweight(fx + sx - g0x, fy + sy - g0y, fz + sz - g0z);
+          tt += fine_entry(st, Af, nxf, nyf, fx, fy, fz, sx, sy, sz) * p2;
+        }
+    tt_sh[grp][d] = tt;
+  }
+  __syncthreads();
+  if (!live || d != 0) return;
   const size_t pc = IX(I, J, K, nxc, nyc);
-  if (I2 < 1 || I2 > nxc - 2 || J2 < 1 || J2 > nyc - 2 || K2 < 1 || K2 > nzc - 2) {
+  if (!inside) {
     out.a[s][pc] = 0.0;
     return;
   }
-  const int f0x = 2 * I, f0y = 2 * J, f0z = 2 * K;
-  const int g0x = 2 * I2, g0y = 2 * J2, g0z = 2 * K2;
-  // The terms with a non-zero second weight pweight(f + s - g0) are those
-  // with |f + s - g0| <= 1 per axis; visiting only that range (in the same
-  // order) adds exactly the terms the full 27-point scan adds.
   double sum = 0.0;
-  for (int dz = -1; dz <= 1; dz++)
-    for (int dy = -1; dy <= 1; dy++)
-      for (int dx = -1; dx <= 1; dx++) {
-        const int fx = f0x + dx, fy = f0y + dy, fz = f0z + dz;
-        const double p1 = pweight(dx, dy, dz);
-        const int sx0 = max(-1, g0x - 1 - fx), sx1 = min(1, g0x + 1 - fx);
-        const int sy0 = max(-1, g0y - 1 - fy), sy1 = min(1, g0y + 1 - fy);
-        const int sz0 = max(-1, g0z - 1 - fz), sz1 = min(1, g0z + 1 - fz);
-        double tt = 0.0;
-        for (int sz = sz0; sz <= sz1; sz++)
-          for (int sy = sy0; sy <= sy1; sy++)
-            for (int sx = sx0; sx <= sx1; sx++) {
-              if (st == 7 && abs(sx) + abs(sy) + abs(sz) > 1) continue;
-              const double p2 = pweight(fx + sx - g0x, fy + sy - g0y, fz + sz - g0z);
-              tt += fine_entry(st, Af, nxf, nyf, fx, fy, fz, sx, sy, sz) * p2;
-            }
-        sum += p1 * tt;
-      }
+  for (int e = 0; e < 27; e++) sum += pweight(e % 3 - 1, (e / 3) % 3 - 1, e / 9 - 1) * tt_sh[grp][e];
   out.a[s][pc] = (s == 0) ? sum : -sum;
 }
 
@@ -1111,8 +1121,8 @@ void galerkin(hipStream_t s, int st, int nxf, int nyf, int nzf, const Ops<double
   OutPtrs o;
   for (int k = 0; k < 14; k++) o.a[k] = ac[k];
   const long nq = (long)(nxc - 2) * (nyc - 2) * (nzc - 2);
-  const long tot = ((nq + 63) / 64) * 64 * 14;
-  k_galerkin<<<(unsigned)((tot + kBlock - 1) / kBlock), kBlock, 0, s>>>(st, nxf, nyf, nzf, to27(Af),
+  const long groups = nq * 14;
+  k_galerkin<<<(unsigned)((groups + kGalGroups - 1) / kGalGroups), kBlock, 0, s>>>(st, nxf, nyf, nzf, to27(Af),
                                                                         nxc, nyc, nzc, o);
 }
 

@@ -1576,10 +1576,10 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
         const size_t tot = (size_t)d->natoms * nref;
         unsigned char* flags = dalloc<unsigned char>(tot);
         pmg::sas_flags(d->s, A, cells.c, dref, nref, srad, flags);
-        int* counter = dalloc<int>(1);
-        const int cap = pmg::sas_count(d->s, A, nref, flags, counter);
-        double* sas = dalloc<double>(3 * (size_t)std::max(cap, 1));
-        int nsas = pmg::sas_compact(d->s, A, dref, nref, srad, flags, sas, counter, cap);
+        int* counter = dalloc<int>(1024);
+        double* sas = nullptr;
+        int nsas = pmg::sas_compact(d->s, A, dref, nref, srad, flags, counter, &sas,
+                                    [](size_t n) { return dalloc<double>(n); });
         for (int w = 0; w < 3; w++)
           pmg::mark_spheres(d->s, g, nsas, sas, nullptr, srad, 2, w, epsw, maps[w]);
         HIPCHECK(hipStreamSynchronize(d->s));
