@@ -114,8 +114,10 @@ void buildA_fv(hipStream_t s, int nx, int ny, int nz, const double* xf, const do
                const double* fcf, double* oC, double* cc, double* fc, double* oE, double* oN,
                double* uC);
 // Galerkin A_c = P^T A P (without the Helmholtz term); ac holds 14 arrays.
+// form 0 picks by level size; 1 = one thread per coefficient, 2 = one 32-lane
+// group per coefficient.  Both give the same bits.
 void galerkin(hipStream_t s, int st, int nxf, int nyf, int nzf, const Ops<double>& Af, int nxc,
-              int nyc, int nzc, double* const ac[14]);
+              int nyc, int nzc, double* const ac[14], int form = 0);
 
 // ----------------------------------------------------- coarsest level (dense)
 // Assemble the dense (N x N) operator of the interior of a 27-point level with

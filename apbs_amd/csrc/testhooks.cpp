@@ -146,8 +146,9 @@ int pmgtest_prolong(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf, const 
   return 1;
 }
 
-int pmgtest_galerkin(int nxf, int nyf, int nzf, int st, const double* af, int nxc, int nyc,
-                     int nzc, double* ac) {
+int pmgtest_galerkin_form(int nxf, int nyf, int nzf, int st, const double* af, int nxc, int nyc,
+                          int nzc, double* ac, int form) {
+  if (form < 0 || form > 2) return 0;
   const size_t nf = (size_t)nxf * nyf * nzf, nc = (size_t)nxc * nyc * nzc;
   OpsHolder A(af, nf, st);
   double* out[14];
@@ -155,13 +156,18 @@ int pmgtest_galerkin(int nxf, int nyf, int nzf, int st, const double* af, int nx
     out[s] = up(nullptr, nc);
     TCHECK(hipMemset(out[s], 0, nc * sizeof(double)));
   }
-  pmg::galerkin(0, st, nxf, nyf, nzf, A.ops, nxc, nyc, nzc, out);
+  pmg::galerkin(0, st, nxf, nyf, nzf, A.ops, nxc, nyc, nzc, out, form);
   TCHECK(hipDeviceSynchronize());
   for (int s = 0; s < 14; s++) {
     down(ac + (size_t)s * nc, out[s], nc);
     TCHECK(hipFree(out[s]));
   }
   return 1;
+}
+
+int pmgtest_galerkin(int nxf, int nyf, int nzf, int st, const double* af, int nxc, int nyc,
+                     int nzc, double* ac) {
+  return pmgtest_galerkin_form(nxf, nyf, nzf, st, af, nxc, nyc, nzc, ac, 0);
 }
 
 int pmgtest_buildA(int nx, int ny, int nz, const double* xf, const double* yf, const double* zf,

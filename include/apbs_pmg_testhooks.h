@@ -27,6 +27,10 @@ int pmgtest_prolong(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf, const 
 /* VbuildG_7 / VbuildG_27 (src/pmgc/buildGd.c:450-3115) */
 int pmgtest_galerkin(int nxf, int nyf, int nzf, int stencil, const double *af, int nxc, int nyc,
                      int nzc, double *ac);
+/* The same with the kernel form forced: 0 by level size, 1 one thread per
+ * coefficient, 2 one 32-lane group per coefficient (both give the same bits) */
+int pmgtest_galerkin_form(int nxf, int nyf, int nzf, int stencil, const double *af, int nxc,
+                          int nyc, int nzc, double *ac, int form);
 /* VbuildA_fv (src/pmgc/buildAd.c:105-281) */
 int pmgtest_buildA(int nx, int ny, int nz, const double *xf, const double *yf, const double *zf,
                    const double *gx, const double *gy, const double *gz, const double *a1,

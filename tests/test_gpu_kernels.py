@@ -124,14 +124,19 @@ def test_transfers_bitwise(L, O):
     assert np.array_equal(interior(fg, f3), interior(fo, f3))
 
 
-@pytest.mark.parametrize("st", [7, 27])
-def test_galerkin_bitwise(L, O, st):
+@pytest.mark.parametrize("form", [0, 1, 2])
+@pytest.mark.parametrize("st,f3,c3", [(7, (17, 17, 17), (9, 9, 9)),
+                                      (27, (17, 17, 17), (9, 9, 9)),
+                                      (27, (65, 57, 61), (33, 29, 31))])
+def test_galerkin_bitwise(L, O, st, f3, c3, form):
+    """Both kernel forms (one thread, one 32-lane group per coefficient) and
+    the size-based choice; the 33x29x31 level has more coarse points than the
+    group form's cut-off, so form 0 takes the per-thread kernel there."""
     rng = np.random.default_rng(5 + st)
-    f3, c3 = (17, 17, 17), (9, 9, 9)
     nf, nc = np.prod(f3), np.prod(c3)
     af = rand_op(rng, f3, st)
     ag, ao = np.zeros(14 * nc), np.zeros(14 * nc)
-    L.pmgtest_galerkin(*f3, st, P(af), *c3, P(ag))
+    assert L.pmgtest_galerkin_form(*f3, st, P(af), *c3, P(ag), form) == 1
     O.orc_galerkin(*f3, st, P(af), *c3, P(ao))
     for s in range(14):
         assert np.array_equal(interior(ag[s * nc:(s + 1) * nc], c3),
