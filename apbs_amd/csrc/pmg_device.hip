@@ -73,8 +73,7 @@ __global__ void __launch_bounds__(kBlock)
 // Points outside the interior are not written.
 constexpr int kTX = 64, kFT = 256, kXCD = 8;
 
-// kTY rows (a multiple of 4: each thread owns kTY/4 rows of one column) and
-// kKC planes per workgroup.
+// kTY rows and kKC planes per workgroup.
 template <class T, int kTY, int kKC>
 __global__ void __launch_bounds__(kFT)
     k_gsrb7_fused(int nx, int ny, int nz, const T* __restrict__ oE, const T* __restrict__ oN,
@@ -98,58 +97,142 @@ __global__ void __launch_bounds__(kFT)
   const size_t nxy = (size_t)nx * ny;
   const int F = first;                   // parity of the first colour
 
-  // new F values of plane kk over the tile and its halo: one thread per F point
-  auto stageF = [&](int kk) {
-    T* r = ring[kk % 3];
-    for (int f = tid; f < FN; f += kFT) {
+  // New F values of plane kk over the tile and its halo, one thread per F
+  // point in kFR rounds.  loadF issues every round's loads at once into
+  // registers; storeF computes the values (acc7's terms and order) into the
+  // plane's LDS slot.
+  constexpr int kFR = (FN + kFT - 1) / kFT;
+  struct FRegs {
+    T v[kFR][14];
+    int slot[kFR];  // LDS index in the ring plane
+    int kind[kFR];  // 0 nothing, 1 boundary copy, 2 interior update
+  };
+  auto loadF = [&](int kk, FRegs& R) {
+#pragma unroll
+    for (int u = 0; u < kFR; u++) {
+      R.kind[u] = 0;
+      const int f = tid + u * kFT;
+      if (f >= FN) continue;
       const int row = f / FPR, c = f % FPR;
       const int j = j0 - 1 + row;
       const int ib = i0 - 1;
       const int i = ib + ((F - ib - j - kk) & 1) + 2 * c;
       if (i > nx - 1 || j > ny - 1) continue;
       const size_t p = ((size_t)kk * ny + j) * nx + i;
-      T v;
-      if (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && kk >= 1 && kk <= nz - 2)
-        v = acc7(fc[p], oE, oN, uC, xi, p, nx, nxy) / D[p];
-      else
-        v = xi[p];
-      r[row * RX + (i - ib)] = v;
+      R.slot[u] = row * RX + (i - ib);
+      if (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && kk >= 1 && kk <= nz - 2) {
+        R.kind[u] = 2;
+        R.v[u][0] = fc[p];
+        R.v[u][1] = oN[p];
+        R.v[u][2] = xi[p + nx];
+        R.v[u][3] = oN[p - nx];
+        R.v[u][4] = xi[p - nx];
+        R.v[u][5] = oE[p];
+        R.v[u][6] = xi[p + 1];
+        R.v[u][7] = oE[p - 1];
+        R.v[u][8] = xi[p - 1];
+        R.v[u][9] = uC[p - nxy];
+        R.v[u][10] = xi[p - nxy];
+        R.v[u][11] = uC[p];
+        R.v[u][12] = xi[p + nxy];
+        R.v[u][13] = D[p];
+      } else {
+        R.kind[u] = 1;
+        R.v[u][0] = xi[p];
+      }
+    }
+  };
+  auto storeF = [&](int kk, const FRegs& R) {
+    T* r = ring[kk % 3];
+#pragma unroll
+    for (int u = 0; u < kFR; u++) {
+      if (R.kind[u] == 2) {
+        T acc = R.v[u][0];
+        acc += R.v[u][1] * R.v[u][2];
+        acc += R.v[u][3] * R.v[u][4];
+        acc += R.v[u][5] * R.v[u][6];
+        acc += R.v[u][7] * R.v[u][8];
+        acc += R.v[u][9] * R.v[u][10];
+        acc += R.v[u][11] * R.v[u][12];
+        r[R.slot[u]] = acc / R.v[u][13];
+      } else if (R.kind[u] == 1) {
+        r[R.slot[u]] = R.v[u][0];
+      }
     }
   };
 
-  stageF(k0 - 1);
-  stageF(k0);
-  const int ti = tid % kTX, tj = tid / kTX;  // rows tj, tj + 4, ...
-  const int i = i0 + ti;
+  // The S step: thread u owns the (i, i+1) pair u % 32 of tile row u / 32 (one
+  // F and one S point).  The eight operands of the S point of plane kk are
+  // loaded one plane ahead, together with stageF(kk), so each coefficient
+  // line is fetched once for both colours while it is still cached.
+  constexpr int kSP = (kTX / 2 * kTY + kFT - 1) / kFT;
+  T cur[kSP][8], nxt[kSP][8];
+  auto loadS = [&](int kk, T (&r)[kSP][8]) {
+#pragma unroll
+    for (int u = 0; u < kSP; u++) {
+      const int sidx = tid + u * kFT;
+      if (sidx >= kTX / 2 * kTY) continue;
+      const int j = j0 + sidx / (kTX / 2);
+      const int is = i0 + 2 * (sidx % (kTX / 2)) + ((1 - F - i0 - j - kk) & 1);
+      if (is > nx - 2 || j > ny - 2) continue;
+      const size_t p = ((size_t)kk * ny + j) * nx + is;
+      r[u][0] = fc[p];
+      r[u][1] = oN[p];
+      r[u][2] = oN[p - nx];
+      r[u][3] = oE[p];
+      r[u][4] = oE[p - 1];
+      r[u][5] = uC[p - nxy];
+      r[u][6] = uC[p];
+      r[u][7] = D[p];
+    }
+  };
+
+  FRegs fr;
+  loadS(k0, cur);
+  loadF(k0 - 1, fr);
+  storeF(k0 - 1, fr);
+  loadF(k0, fr);
+  storeF(k0, fr);
+  loadF(k0 + 1, fr);
   for (int k = k0; k < k1; k++) {
-    stageF(k + 1);
+    // F(k+1) from registers loaded one step earlier; the loads of F(k+2) and
+    // of S(k+1)'s operands are in flight while S(k) is computed
+    if (k + 1 < k1) loadS(k + 1, nxt);
+    storeF(k + 1, fr);
     __syncthreads();
+    if (k + 1 < k1) loadF(k + 2, fr);
     const T* rm = ring[(k - 1) % 3];
     const T* r0 = ring[k % 3];
     const T* rp = ring[(k + 1) % 3];
 #pragma unroll
-    for (int h = 0; h < kTY / 4; h++) {
-      const int jj = tj + h * 4;
+    for (int u = 0; u < kSP; u++) {
+      const int sidx = tid + u * kFT;
+      if (sidx >= kTX / 2 * kTY) continue;
+      const int jj = sidx / (kTX / 2);
       const int j = j0 + jj;
-      if (i > nx - 2 || j > ny - 2) continue;
-      const int e = (jj + 1) * RX + (ti + 1);
-      const size_t p = ((size_t)k * ny + j) * nx + i;
-      T v;
-      if (((i + j + k) & 1) == F) {
-        v = r0[e];
-      } else {
-        T acc = fc[p];
-        acc += oN[p] * r0[e + RX];
-        acc += oN[p - nx] * r0[e - RX];
-        acc += oE[p] * r0[e + 1];
-        acc += oE[p - 1] * r0[e - 1];
-        acc += uC[p - nxy] * rm[e];
-        acc += uC[p] * rp[e];
-        v = acc / D[p];
+      if (j > ny - 2) continue;
+      const int ia = i0 + 2 * (sidx % (kTX / 2));
+      const int is = ia + ((1 - F - i0 - j - k) & 1);
+      const int iF = ia + ((F - i0 - j - k) & 1);
+      const size_t row = ((size_t)k * ny + j) * nx;
+      if (iF <= nx - 2) xo[row + iF] = r0[(jj + 1) * RX + (iF - i0 + 1)];
+      if (is <= nx - 2) {
+        const int e = (jj + 1) * RX + (is - i0 + 1);
+        T acc = cur[u][0];
+        acc += cur[u][1] * r0[e + RX];
+        acc += cur[u][2] * r0[e - RX];
+        acc += cur[u][3] * r0[e + 1];
+        acc += cur[u][4] * r0[e - 1];
+        acc += cur[u][5] * rm[e];
+        acc += cur[u][6] * rp[e];
+        xo[row + is] = acc / cur[u][7];
       }
-      xo[p] = v;
     }
     __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kSP; u++)
+#pragma unroll
+      for (int w = 0; w < 8; w++) cur[u][w] = nxt[u][w];
   }
 }
 
@@ -205,6 +288,8 @@ struct Ops27 {
 };
 
 // One colour of the 8-colour sweep: 0-based parities (c&1, (c>>1)&1, (c>>2)&1).
+// Latency-bound on the small coarse levels: up to 128 VGPRs, so that all 54
+// operands of a point are in flight at once.
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_gsrb27(int nx, int ny, int nz, Ops27<T> A, const T* __restrict__ D,
@@ -975,10 +1060,12 @@ void fused_sweep(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T
 int fused_tile() {
   static int t = [] {
     const char* e = getenv("APBS_PMG_FUSED_TILE");
-    if (!e) return 816;
+    // 4 rows x 8 planes: fastest or within 6% of the fastest shape in both
+    // precisions at 161^3 and 193^3 (73 / 54 us and 152 / 77 us per sweep)
+    if (!e) return 48;
     const int v = atoi(e);
     return (v == 416 || v == 48 || v == 816 || v == 88 || v == 84 || v == 1616 || v == 832) ? v
-                                                                                          : 816;
+                                                                                          : 48;
   }();
   return t;
 }

@@ -51,6 +51,38 @@ __global__ void k_stream_copy(const double* __restrict__ a, double* __restrict__
     b[i] = a[i];
 }
 
+// The fused red-black sweep alone on an n^3 7-point operator (diagonally
+// dominant, values irrelevant to its memory traffic): one untimed sweep, then
+// `sweeps` (even) timed launches on the null stream.
+template <class T>
+double fused_sweeps_us(int n, int sweeps, int tile) {
+  const size_t N = (size_t)n * n * n;
+  std::vector<T> h(N);
+  T* d[8];
+  const T val[8] = {T(6.5), T(1), T(1), T(1), T(6.5), T(1), T(0), T(0)};  // oC oE oN uC D fc x t
+  for (int a = 0; a < 8; a++) {
+    TCHECK(hipMalloc(&d[a], N * sizeof(T)));
+    for (size_t i = 0; i < N; i++) h[i] = val[a];
+    TCHECK(hipMemcpy(d[a], h.data(), N * sizeof(T), hipMemcpyHostToDevice));
+  }
+  pmg::Ops<T> ops{};
+  for (int a = 0; a < 4; a++) ops.a[a] = d[a];
+  pmg::gsrb7_fused<T>(0, n, n, n, ops, d[4], d[5], d[6], d[7], 2, 0, tile);
+  hipEvent_t e0, e1;
+  TCHECK(hipEventCreate(&e0));
+  TCHECK(hipEventCreate(&e1));
+  TCHECK(hipEventRecord(e0, 0));
+  pmg::gsrb7_fused<T>(0, n, n, n, ops, d[4], d[5], d[6], d[7], sweeps, 0, tile);
+  TCHECK(hipEventRecord(e1, 0));
+  TCHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  TCHECK(hipEventElapsedTime(&ms, e0, e1));
+  TCHECK(hipEventDestroy(e0));
+  TCHECK(hipEventDestroy(e1));
+  for (int a = 0; a < 8; a++) TCHECK(hipFree(d[a]));
+  return 1e3 * ms / sweeps;
+}
+
 extern "C" {
 
 // Calibration for hardware byte counters: reps copies of n doubles (n*8
@@ -63,6 +95,15 @@ int pmgtest_stream_copy(size_t n, int reps) {
   TCHECK(hipDeviceSynchronize());
   TCHECK(hipFree(a));
   TCHECK(hipFree(b));
+  return 1;
+}
+
+// Average microseconds per fused sweep launch on an n^3 grid; word 8 (FP64)
+// or 4 (FP32); sweeps even and positive; tile 0 = the library's default.
+int pmgtest_fused_sweeps(int n, int word, int sweeps, int tile, double* us_per_sweep) {
+  if (n < 3 || sweeps <= 0 || (sweeps & 1) || (word != 4 && word != 8)) return 0;
+  *us_per_sweep = word == 8 ? fused_sweeps_us<double>(n, sweeps, tile)
+                            : fused_sweeps_us<float>(n, sweeps, tile);
   return 1;
 }
 

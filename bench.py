@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--slots", type=int, default=8, help="concurrent requests of the served path")
     ap.add_argument("--served", type=int, default=48, help="requests in the served run (0: none)")
     ap.add_argument("--pmc", action="store_true",
-                    help="counter-collection workload: a few single-request calculations of each "
+                    help="counter-collection workload: the fine level's fused sweeps in each "
                          "precision and a known-size copy kernel (run under rocprofv3 --pmc)")
     return ap.parse_args()
 
@@ -121,12 +121,24 @@ def measured_traffic(word):
     return None
 
 
-def pmc_workload(mol, cfg):
+PMC_SWEEPS = 20
+
+
+def pmc_workload(cfg):
+    """The fine level's fused sweep launches alone (FP64 and FP32, on a
+    synthetic operator of the workload's grid: the kernel's memory traffic
+    depends on the grid shape only), then the known-size copy kernel."""
     from apbs_amd import lib
     import ctypes
-    for prec in (0, 1):
-        run_mode(mol, cfg, prec, 2, 1)
     L = lib()
+    L.pmgtest_fused_sweeps.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_double)]
+    us = ctypes.c_double()
+    for word in (8, 4):
+        if L.pmgtest_fused_sweeps(cfg["dime"], word, PMC_SWEEPS, 0, ctypes.byref(us)) != 1:
+            raise RuntimeError("pmgtest_fused_sweeps failed")
+        print(f"# pmc workload: {PMC_SWEEPS} fused sweeps, {cfg['dime']}^3, word {word}: "
+              f"{us.value:.2f} us per launch", flush=True)
     L.pmgtest_stream_copy.argtypes = [ctypes.c_size_t, ctypes.c_int]
     L.pmgtest_stream_copy(COPY_DOUBLES, 3)
     print(f"# pmc workload done: copy kernel {COPY_DOUBLES * 8} bytes each way x 3", flush=True)
@@ -232,7 +244,7 @@ def main():
     xyz, q, r = synth.make(a.workload)
     mol = driver.Molecule(xyz, q, r)
     if a.pmc:
-        pmc_workload(mol, cfg)
+        pmc_workload(cfg)
         return
     modes = a.modes.split(",")
     out = {}

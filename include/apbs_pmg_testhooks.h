@@ -48,6 +48,11 @@ int pmgtest_gsrb_tile(int nx, int ny, int nz, const double *a, const double *cc,
 /* No reference counterpart: reps device copies of n doubles, the known-size
  * calibration for hardware byte counters (FETCH_SIZE / WRITE_SIZE). */
 int pmgtest_stream_copy(size_t n, int reps);
+/* No reference counterpart: the fused red-black sweep alone on a synthetic
+ * n^3 7-point operator, word 8 (FP64) or 4 (FP32), `sweeps` (even) timed
+ * launches; average microseconds per launch in *us_per_sweep.  The same
+ * launches as the solver's fine level, for counter and timing runs. */
+int pmgtest_fused_sweeps(int n, int word, int sweeps, int tile, double *us_per_sweep);
 #ifdef __cplusplus
 }
 #endif
