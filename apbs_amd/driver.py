@@ -43,6 +43,27 @@ class Molecule:
         if not L.Valist_setAtoms(self.ptr, len(self.q), _dp(self.xyz), _dp(self.r), _dp(self.q)):
             raise ValueError("Valist_setAtoms failed")
 
+    @classmethod
+    def from_pqr(cls, path):
+        """A molecule read by the library's Valist_readPQRFile
+        (valist.c:606-723 in the reference)."""
+        L = lib()
+        ptr = L.Valist_ctor()
+        if not L.Valist_readPQRFile(ptr, str(path).encode()):
+            L.Valist_dtor(ctypes.byref(ptr))
+            raise ValueError(f"cannot read PQR file {path}")
+        n = L.Valist_getNumberAtoms(ptr)
+        atoms = [L.Valist_getAtom(ptr, i).contents for i in range(n)]
+        self = cls.__new__(cls)
+        self.xyz = np.array([a.position[:] for a in atoms], dtype=np.float64).reshape(-1, 3)
+        self.q = np.array([a.charge for a in atoms], dtype=np.float64)
+        self.r = np.array([a.radius for a in atoms], dtype=np.float64)
+        self.ptr = ptr
+        return self
+
+    def __len__(self):
+        return len(self.q)
+
     @property
     def center(self):
         return np.array(self.ptr.contents.center[:])

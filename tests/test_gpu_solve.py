@@ -83,7 +83,7 @@ def test_fillco_maps(srfm, chgm):
 def test_born_focusing_golden(srfm, key):
     """examples/born/apbs-{s}mol-auto.in on the GPU: every focusing level's
     energy against tests/test_cases.cfg."""
-    atoms = od.read_pqr(os.path.join(HERE, "golden", "born_ion.pqr"))
+    atoms = od.read_pqr(os.path.join(HERE, "golden", "examples", "born", "ion.pqr"))
     c = od.mol_center(atoms[0])
     mol = driver.Molecule(*atoms)
     last = {}
@@ -103,7 +103,7 @@ def test_solv_manual_golden(srfm):
     """examples/solv/apbs-{s}mol.in on the GPU (chgm spl0, no salt)."""
     out = []
     for name in ("methanol", "methoxide"):
-        atoms = od.read_pqr(os.path.join(HERE, "golden", f"{name}.pqr"))
+        atoms = od.read_pqr(os.path.join(HERE, "golden", "examples", "solv", f"{name}.pqr"))
         c = od.mol_center(atoms[0])
         mol = driver.Molecule(*atoms)
         for sdie in (78.0, 1.0):
@@ -207,7 +207,7 @@ def test_c_caller_matches_python(tmp_path):
     import subprocess
     from test_abi import build_c_caller
     exe = build_c_caller(tmp_path)
-    pqr = os.path.join(HERE, "golden", "born_ion.pqr")
+    pqr = os.path.join(HERE, "golden", "examples", "born", "ion.pqr")
     out = subprocess.run([exe, "solve", pqr, "33", "0.5"], capture_output=True, text=True,
                          check=True, timeout=120).stdout
     vals = {l.split()[0]: [float(x) for x in l.split()[1:]] for l in out.splitlines()}
@@ -256,12 +256,12 @@ def test_actin_dimer_golden(srfm, key):
     every focusing level's energy against tests/test_cases.cfg.  The binding
     energy is a difference of numbers 5000x larger, so the reference's own
     convergence error (etol 1e-6) bounds how closely it can be met."""
-    c = od.mol_center(od.read_pqr(os.path.join(HERE, "golden", "actin_complex.pqr"))[0])
+    c = od.mol_center(od.read_pqr(os.path.join(HERE, "golden", "examples", "actin-dimer", "complex.pqr"))[0])
     pbe = dict(npbe=True, pdie=2.0, sdie=78.4, srfm=srfm, chgm="spl0", srad=1.4, swin=0.3,
                sdens=10.0, temp=298.15, bcfl="sdh", ions=[(1, 0.050, 2.0), (-1, 0.050, 2.0)])
     fine = {}
     for name in ("mol1", "mol2", "complex"):
-        atoms = od.read_pqr(os.path.join(HERE, "golden", f"actin_{name}.pqr"))
+        atoms = od.read_pqr(os.path.join(HERE, "golden", "examples", "actin-dimer", f"{name}.pqr"))
         mol = driver.Molecule(*atoms)
         e, _ = driver.run_auto(mol, [161] * 3, [156, 121, 162], [112, 91, 116], c, c, pbe)
         want = G[key]["levels"][name]

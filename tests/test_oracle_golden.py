@@ -25,7 +25,7 @@ def rel(a, b):
 def test_born_focusing_chain(srfm, key):
     """examples/born/apbs-{s}mol-auto.in: 3-level mg-auto focusing, lpbe,
     chgm spl2, bcfl mdh, solvated and reference states."""
-    atoms = od.read_pqr(os.path.join(HERE, "golden", "born_ion.pqr"))
+    atoms = od.read_pqr(os.path.join(HERE, "golden", "examples", "born", "ion.pqr"))
     c = od.mol_center(atoms[0])
     levels = {}
     for state, sdie in (("solvated_levels", 78.54), ("reference_levels", 1.0)):
@@ -50,7 +50,7 @@ def test_solv_manual(srfm):
     zero salt, srad 0 (no SAS re-marking)."""
     out = []
     for mol in ("methanol", "methoxide"):
-        atoms = od.read_pqr(os.path.join(HERE, "golden", f"{mol}.pqr"))
+        atoms = od.read_pqr(os.path.join(HERE, "golden", "examples", "solv", f"{mol}.pqr"))
         c = od.mol_center(atoms[0])
         for sdie in (78.0, 1.0):
             pbe = dict(pdie=2.0, sdie=sdie, srfm=srfm, chgm="spl0", srad=0.0, swin=0.3,
@@ -64,15 +64,18 @@ def test_solv_manual(srfm):
     assert rel(ddg, G["solv_" + srfm]["differences"][2]) < 1e-4
 
 
+IONPROT = {"protein": "small491.pqr", "ion": "491.pqr", "complex": "complex.pqr"}
+
+
 @pytest.mark.slow
 def test_ion_protein_npbe():
     """examples/ion-protein/apbs-smol-pdiel2.in: npbe (Newton), 129^3,
     chgm spl0, srfm smol, two 0.05 M monovalent species."""
-    cpx = od.read_pqr(os.path.join(HERE, "golden", "ionprot_complex.pqr"))
+    cpx = od.read_pqr(os.path.join(HERE, "golden", "examples", "ion-protein", "complex.pqr"))
     c = od.mol_center(cpx[0])
     out = []
     for mol in ("protein", "ion", "complex"):
-        atoms = od.read_pqr(os.path.join(HERE, "golden", f"ionprot_{mol}.pqr"))
+        atoms = od.read_pqr(os.path.join(HERE, "golden", "examples", "ion-protein", IONPROT[mol]))
         pbe = dict(npbe=True, pdie=2.0, sdie=78.4, srfm="smol", chgm="spl0", srad=1.4,
                    swin=0.3, sdens=10.0, temp=298.15, bcfl="mdh",
                    ions=[(1, 0.05, 2.0), (-1, 0.05, 2.0)])
