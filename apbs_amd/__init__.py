@@ -21,7 +21,7 @@ VSM_MOL, VSM_MOLSMOOTH, VSM_SPLINE = 0, 1, 2
 VCM_TRIL, VCM_BSPL2 = 0, 1
 BCFL_ZERO, BCFL_SDH, BCFL_MDH, BCFL_FOCUS = 0, 1, 2, 4
 NONLIN_LPBE, NONLIN_NPBE = 0, 1
-MCT_MANUAL, MCT_AUTO = 0, 1
+MCT_MANUAL, MCT_AUTO, MCT_PARALLEL = 0, 1, 2
 PCE_NO, PCE_TOTAL = 0, 1
 VDT_CHARGE, VDT_POT, VDT_DIELX, VDT_DIELY, VDT_DIELZ, VDT_KAPPA = 0, 1, 11, 12, 13, 14
 
@@ -187,6 +187,8 @@ _SIGS = [
     ("MGparm_check", ctypes.c_int, [ctypes.POINTER(MGparm)]),
     ("MGparm_setupAuto", ctypes.c_int,
      [ctypes.POINTER(MGparm), ctypes.POINTER(ctypes.POINTER(MGparm)), ctypes.c_int]),
+    ("MGparm_setupPara", ctypes.c_int,
+     [ctypes.POINTER(MGparm), ctypes.c_int, ctypes.POINTER(ctypes.POINTER(MGparm)), ctypes.c_int]),
     ("Vpmgp_ctor", ctypes.POINTER(Vpmgp), [ctypes.POINTER(MGparm)]),
     ("Vpmgp_dtor", None, [ctypes.POINTER(ctypes.POINTER(Vpmgp))]),
     ("Vpmgp_size", None, [ctypes.POINTER(Vpmgp)]),

@@ -346,6 +346,7 @@ extern "C" int MGparm_check(MGparm* thee) {
   if (thee->type == MCT_AUTO || thee->type == MCT_PARALLEL) {
     if (!thee->setcglen || !thee->setfglen || !thee->setcgcent || !thee->setfgcent) return 0;
   }
+  if (thee->type == MCT_PARALLEL && (!thee->setpdime || !thee->setofrac)) return 0;
   int tdime[3], tnlev[3];
   if (thee->type != MCT_DUMMY) {
     for (int i = 0; i < 3; i++) {
@@ -393,6 +394,10 @@ extern "C" int MGparm_check(MGparm* thee) {
 // NOsh_setupCalcMGAUTO (nosh.c:1828-2101)
 extern "C" int MGparm_setupAuto(MGparm* elec, MGparm** out, int maxcalc) {
   if (!elec || !out) return 0;
+  // Every level of an mg-para chain stays MCT_PARALLEL and keeps the
+  // processor's disjoint partition (nosh.c:2085-2092 resets it for the
+  // others), so setPartMG and the focusing energy use the partition.
+  const bool para = elec->type == MCT_PARALLEL;
   double cgrid[3], fgrid[3], redrat[3];
   int tn[3];
   for (int j = 0; j < 3; j++) {
@@ -415,7 +420,7 @@ extern "C" int MGparm_setupAuto(MGparm* elec, MGparm** out, int maxcalc) {
     MGparm* c = out[f];
     MGparm* p = f > 0 ? out[f - 1] : nullptr;
     *c = *elec;
-    c->type = MCT_MANUAL;
+    c->type = para ? MCT_PARALLEL : MCT_MANUAL;
     for (int j = 0; j < 3; j++) {
       if (f == 0) {
         c->grid[j] = cgrid[j];
@@ -456,14 +461,78 @@ extern "C" int MGparm_setupAuto(MGparm* elec, MGparm** out, int maxcalc) {
         }
       }
     }
-    for (int j = 0; j < 6; j++) c->partDisjOwnSide[j] = 0;
-    for (int j = 0; j < 3; j++) {
-      c->partDisjCenter[j] = 0;
-      c->partDisjLength[j] = c->glen[j];
+    if (!para) {
+      for (int j = 0; j < 6; j++) c->partDisjOwnSide[j] = 0;
+      for (int j = 0; j < 3; j++) {
+        c->partDisjCenter[j] = 0;
+        c->partDisjLength[j] = c->glen[j];
+      }
     }
     c->setgcent = 1;
   }
   return nfocus;
+}
+
+// NOsh_setupCalcMGPARA (nosh.c:2103-2394) for the processor `rank` of the
+// pdime processor grid (the reference's `async` rank): each axis of the fine
+// domain fglen is cut into disjoint slabs of dime/(1 + 2 ofrac) points on a
+// global grid of npx*disj points; the processor's fine grid keeps dime points
+// and overlaps its neighbours (flush with the domain at the outer
+// processors), its disjoint slab becomes partDisjCenter/Length, and
+// partDisjOwnSide flags the sides that face another processor.  The result
+// is the focusing chain of MGparm_setupAuto over that fine grid.
+extern "C" int MGparm_setupPara(MGparm* elec, int rank, MGparm** out, int maxcalc) {
+  if (!elec || !out || elec->type != MCT_PARALLEL || !elec->setpdime || !elec->setofrac)
+    return 0;
+  const int np[3] = {elec->pdime[0], elec->pdime[1], elec->pdime[2]};
+  if (np[0] < 1 || np[1] < 1 || np[2] < 1) return 0;
+  if (rank < 0 || rank >= np[0] * np[1] * np[2]) return 0;
+  const int kp = rank / (np[0] * np[1]);
+  const int jp = (rank - kp * np[0] * np[1]) / np[0];
+  const int ip = rank - kp * np[0] * np[1] - jp * np[0];
+  const int pc[3] = {ip, jp, kp};
+  MGparm e = *elec;
+  double olapLen[3], olapCen[3], disjLen[3], disjCen[3];
+  for (int d = 0; d < 3; d++) {
+    const double ofr = np[d] == 1 ? 0.0 : e.ofrac;
+    const int disj = (int)floor(e.dime[d] / (1 + 2 * ofr) + 0.5);
+    const int effGlob = np[d] * disj;
+    const double h = e.fglen[d] / (double)(effGlob - 1);
+    const int igmin = pc[d] * disj, igmax = igmin + disj - 1;
+    const double mn = std::max(h * (igmin - 0.5), 0.0);
+    const double mx = std::min(h * (igmax + 0.5), e.fglen[d]);
+    disjLen[d] = mx - mn;
+    const double cen = 0.5 * e.fglen[d];
+    disjCen[d] = mn + 0.5 * disjLen[d] - cen;
+    if (fabs(disjCen[d]) < VSMALL) disjCen[d] = 0.0;
+    const int olapReg = np[d] != 1 ? (int)floor(ofr * e.fglen[d] / np[d] / h + 0.5) + 1 : 0;
+    int igminO;
+    if (pc[d] == 0)
+      igminO = 0;
+    else if (pc[d] == np[d] - 1)
+      igminO = effGlob - e.dime[d];
+    else
+      igminO = igmin - olapReg;
+    const int igmaxO = igminO + e.dime[d] - 1;
+    const double mnO = h * igminO, mxO = h * igmaxO;
+    olapLen[d] = mxO - mnO;
+    olapCen[d] = (mnO + 0.5 * olapLen[d]) - cen;
+    if (fabs(olapCen[d]) < VSMALL) olapCen[d] = 0.0;
+  }
+  // sides facing another processor (VAPBS_RIGHT 0, FRONT 1, UP 2, LEFT 3, BACK 4, DOWN 5)
+  e.partDisjOwnSide[3] = ip == 0 ? 0 : 1;
+  e.partDisjOwnSide[0] = ip == np[0] - 1 ? 0 : 1;
+  e.partDisjOwnSide[4] = jp == 0 ? 0 : 1;
+  e.partDisjOwnSide[1] = jp == np[1] - 1 ? 0 : 1;
+  e.partDisjOwnSide[5] = kp == 0 ? 0 : 1;
+  e.partDisjOwnSide[2] = kp == np[2] - 1 ? 0 : 1;
+  for (int d = 0; d < 3; d++) {
+    e.fglen[d] = olapLen[d];
+    e.partDisjLength[d] = disjLen[d];
+    e.partDisjCenter[d] = e.fcenter[d] + disjCen[d];
+    e.fcenter[d] += olapCen[d];
+  }
+  return MGparm_setupAuto(&e, out, maxcalc);
 }
 
 // ==================================================================== Vpmgp
@@ -1808,8 +1877,28 @@ static double atom_axis_h(double x, double lo, double hi, int bl, int bh) {
 }
 
 // bflags order of src/generic/vhal.h VAPBS_{RIGHT,FRONT,UP,LEFT,BACK,DOWN} = 0..5
+// invert = 1 is extEnergy's use (vpmg.c:2416-2545): after the partition of
+// [lo, hi] is set, every grid point and atom is flipped to 1 - [weight > 0]
+// and then zeroed where it lies beyond the processor partition
+// [partMin, partMax] on a side owned by another processor (bflags), so the
+// old grid's energy is counted outside the new grid and inside this
+// processor's share only.
+static inline double beyond_mask(const double x[3], const double* partMin, const double* partMax,
+                                 const int bflags[6]) {
+  if (!partMin) return 1.0;
+  double xv = 1, yv = 1, zv = 1;
+  if (x[0] < partMin[0] && bflags[3] == 1) xv = 0;
+  else if (x[0] > partMax[0] && bflags[0] == 1) xv = 0;
+  if (x[1] < partMin[1] && bflags[4] == 1) yv = 0;
+  else if (x[1] > partMax[1] && bflags[1] == 1) yv = 0;
+  if (x[2] < partMin[2] && bflags[5] == 1) zv = 0;
+  else if (x[2] > partMax[2] && bflags[2] == 1) zv = 0;
+  return xv * yv * zv;
+}
+
 static void set_part_impl(Vpmg* thee, const double lo[3], const double hi[3], const int bflags[6],
-                          int invert) {
+                          int invert, const double* partMin = nullptr,
+                          const double* partMax = nullptr) {
   PmgDev* d = DEV(thee);
   Vpmgp* p = thee->pmgp;
   const int bR = bflags[0], bF = bflags[1], bU = bflags[2], bL = bflags[3], bB = bflags[4],
@@ -1844,7 +1933,12 @@ static void set_part_impl(Vpmg* thee, const double lo[3], const double hi[3], co
         for (int k = 0; k < p->nz; k++) {
           double zok = part_axis_h(k * p->hzed + zm, lo[2], hi[2], p->hzed, bD, bU);
           double v = xok * yok * zok;
-          pv[((size_t)k * p->ny + j) * p->nx + i] = (fabs(v) < VPMGSMALL) ? 0.0 : v;
+          v = (fabs(v) < VPMGSMALL) ? 0.0 : v;
+          if (invert) {
+            const double pos[3] = {i * p->hx + xm, j * p->hy + ym, k * p->hzed + zm};
+            v = (1 - (v > VSMALL ? 1.0 : v)) * beyond_mask(pos, partMin, partMax, bflags);
+          }
+          pv[((size_t)k * p->ny + j) * p->nx + i] = v;
         }
       }
     }
@@ -1854,7 +1948,7 @@ static void set_part_impl(Vpmg* thee, const double lo[3], const double hi[3], co
     for (int a = 0; a < d->natoms; a++) {
       double v = d->hpart[a];
       if (v > VSMALL) v = 1.0;
-      v = (1 - v) * 1.0;
+      v = (1 - v) * beyond_mask(al->atoms[a].position, partMin, partMax, bflags);
       d->hpart[a] = v;
       al->atoms[a].partID = v;
     }
@@ -1896,13 +1990,7 @@ static void ext_energy(Vpmg* thee, Vpmg* old, PBEparm_calcEnergy, double* partMi
   hi[1] = p->ycent + ((double)(p->ny - 1) * p->hy) / 2.0;
   lo[2] = p->zcent - ((double)(p->nz - 1) * p->hzed) / 2.0;
   hi[2] = p->zcent + ((double)(p->nz - 1) * p->hzed) / 2.0;
-  const bool shared = bflags[0] || bflags[1] || bflags[2] || bflags[3] || bflags[4] || bflags[5];
-  if (shared) {
-    fprintf(stderr, "extEnergy: parallel-focusing partitions are not supported\n");
-  }
-  (void)partMin;
-  (void)partMax;
-  set_part_impl(old, lo, hi, bflags, 1);
+  set_part_impl(old, lo, hi, bflags, 1, partMin, partMax);
   thee->extQmEnergy = Vpmg_qmEnergy(old, 1);
   thee->extQfEnergy = Vpmg_qfEnergy(old, 1);
   thee->extDiEnergy = Vpmg_dielEnergy(old, 1);

@@ -10,7 +10,8 @@ import ctypes
 
 import numpy as np
 
-from . import (BCFL_FOCUS, BCFL_MDH, BCFL_SDH, BCFL_ZERO, MCT_AUTO, MCT_MANUAL, NONLIN_LPBE, NONLIN_NPBE,
+from . import (BCFL_FOCUS, BCFL_MDH, BCFL_SDH, BCFL_ZERO, MCT_AUTO, MCT_MANUAL, MCT_PARALLEL,
+               NONLIN_LPBE, NONLIN_NPBE,
                PCE_NO, PCE_TOTAL, VCM_BSPL2, VCM_TRIL, VSM_MOL, VSM_MOLSMOOTH, VSM_SPLINE,
                VSOL_MG, VSOL_Newton, MGparm, lib)
 
@@ -133,10 +134,15 @@ class Calc:
             raise RuntimeError("Vpmg_solve failed")
 
     def set_part_domain(self):
-        """setPartMG for a sequential calculation (routines.c:1676-1724)."""
+        """setPartMG (routines.c:1676-1724): the whole grid, or for an
+        mg-para level the processor's disjoint partition."""
         c = self.mgparm.contents
-        lo = np.array([c.center[j] - 0.5 * c.glen[j] for j in range(3)])
-        hi = np.array([c.center[j] + 0.5 * c.glen[j] for j in range(3)])
+        if c.type == MCT_PARALLEL:
+            lo = np.array([c.partDisjCenter[j] - 0.5 * c.partDisjLength[j] for j in range(3)])
+            hi = np.array([c.partDisjCenter[j] + 0.5 * c.partDisjLength[j] for j in range(3)])
+        else:
+            lo = np.array([c.center[j] - 0.5 * c.glen[j] for j in range(3)])
+            hi = np.array([c.center[j] + 0.5 * c.glen[j] for j in range(3)])
         bf = (ctypes.c_int * 6)(*c.partDisjOwnSide)
         lib().Vpmg_setPart(self.pmg, _dp(lo), _dp(hi), bf)
 
@@ -270,6 +276,37 @@ def auto_chain(dime, cglen, fglen, ccenter, fcenter, chgm="spl2"):
     nf = L.MGparm_setupAuto(e, arr, 20)
     if nf <= 0:
         raise ValueError("MGparm_setupAuto failed")
+    for k in range(nf, 20):
+        L.MGparm_dtor(ctypes.byref(outs[k]))
+    L.MGparm_dtor(ctypes.byref(e))
+    return outs[:nf]
+
+
+def para_chain(dime, cglen, fglen, ccenter, fcenter, pdime, ofrac, rank, chgm="spl2"):
+    """The focusing chain of processor `rank` of one mg-para block
+    (NOsh_setupCalcMGPARA, nosh.c:2103-2394) as checked MGparm objects."""
+    L = lib()
+    e = _new_mgparm(MCT_PARALLEL)
+    c = e.contents
+    for j in range(3):
+        c.dime[j] = int(dime[j])
+        c.cglen[j], c.fglen[j] = float(cglen[j]), float(fglen[j])
+        c.ccenter[j], c.fcenter[j] = float(ccenter[j]), float(fcenter[j])
+        c.pdime[j] = int(pdime[j])
+    c.ofrac = float(ofrac)
+    c.setdime = c.setcglen = c.setfglen = c.setcgcent = c.setfgcent = 1
+    c.setpdime = c.setofrac = 1
+    setattr(c, "async", int(rank))     # a Python keyword, so not c.async
+    c.setasync = 1
+    c.chgm = CHGM[chgm]
+    c.setchgm = 1
+    if not L.MGparm_check(e):
+        raise ValueError("MGparm_check failed")
+    outs = [_new_mgparm(MCT_MANUAL) for _ in range(20)]
+    arr = (ctypes.POINTER(MGparm) * 20)(*outs)
+    nf = L.MGparm_setupPara(e, int(rank), arr, 20)
+    if nf <= 0:
+        raise ValueError("MGparm_setupPara failed")
     for k in range(nf, 20):
         L.MGparm_dtor(ctypes.byref(outs[k]))
     L.MGparm_dtor(ctypes.byref(e))

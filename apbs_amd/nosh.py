@@ -434,7 +434,7 @@ def check_runnable(inp):
         why.append(f"READ {kind} {fmt}: external maps are not used by this build")
     for i, e in enumerate(inp.elecs):
         tag = e.name or str(i + 1)
-        if e.method not in ("mg-manual", "mg-auto"):
+        if e.method not in ("mg-manual", "mg-auto", "mg-para"):
             why.append(f"ELEC {tag}: method {e.method} is not run by this build")
             continue
         if e.unsupported:
@@ -466,12 +466,27 @@ def check_runnable(inp):
             missing.append("grid or glen")
         if e.method == "mg-manual" and e.gcent is None:
             missing.append("gcent")
-        if e.method == "mg-auto":
+        if e.method in ("mg-auto", "mg-para"):
             missing += [k for k in ("cglen", "fglen", "cgcent", "fgcent") if getattr(e, k) is None]
+        if e.method == "mg-para":
+            missing += [k for k in ("pdime", "ofrac") if getattr(e, k) is None]
         if missing:
             why.append(f"ELEC {tag}: not set: {', '.join(missing)}")
         if e.bcfl == "focus" and i == 0:
             why.append(f"ELEC {tag}: can't focus the first calculation")
+    para = [e for e in inp.elecs if e.method == "mg-para" and e.pdime]
+    nprocs = {e.pdime[0] * e.pdime[1] * e.pdime[2] for e in para}
+    if len(nprocs) > 1:
+        why.append("mg-para sections with different processor counts (pdime)")
+    asyncs = {e.async_rank for e in para}
+    if len(asyncs) > 1:
+        why.append("mg-para sections with different async ranks")
+    elif para and None not in asyncs and len(nprocs) == 1:
+        rank = asyncs.pop()
+        if not 0 <= rank < nprocs.pop():
+            why.append(f"async {rank} is outside the pdime processor grid")
+    if para and len(inp.elecs) != len(para):
+        why.append("mg-para sections mixed with other ELEC methods")
     for p in inp.prints:
         if p.what.startswith("elec"):
             for ident in p.ids:

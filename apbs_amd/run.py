@@ -5,7 +5,10 @@
 Restates APBS's main loop (src/main.c:528-600 and the PRINT loop :931-960)
 for the sections `apbs_amd.nosh` accepts: every ELEC section expands into its
 calculations (one for mg-manual; the focusing chain of MGparm_setupAuto for
-mg-auto), each run as initMG / solveMG / setPartMG / energyMG / forceMG /
+mg-auto; for mg-para the chain of one processor's partition from
+MGparm_setupPara, with the `async` rank or, without one, every processor in
+turn on this GPU, whose PRINT values are summed as an MPI run reduces them),
+each run as initMG / solveMG / setPartMG / energyMG / forceMG /
 writedataMG (src/routines.c:1314-1941, 2620-2990) through the library's Vpmg
 C API.  A `bcfl focus` calculation takes over the previous calculation's
 Vpmg (routines.c:1459-1477); any other calculation frees it.
@@ -55,14 +58,32 @@ def _pbe_dict(e):
             "bcfl": e.bcfl, "ions": list(e.ions), "etol": e.etol}
 
 
-def _mgparm_chain(e, centers):
-    """The MGparm objects of one ELEC section's calculations."""
+def _mgparm_chain(e, centers, rank):
+    """The MGparm objects of one ELEC section's calculations (for mg-para,
+    those of processor `rank`)."""
     if e.method == "mg-manual":
         c = e.gcent.resolve(centers)
         return [driver.manual_mgparm(e.dime, c, grid=e.grid, glen=e.glen if e.grid is None else None,
                                      chgm=e.chgm, nlev=e.nlev or 4)]
+    if e.method == "mg-para":
+        return driver.para_chain(e.dime, e.cglen, e.fglen, e.cgcent.resolve(centers),
+                                 e.fgcent.resolve(centers), e.pdime, e.ofrac, rank, e.chgm)
     return driver.auto_chain(e.dime, e.cglen, e.fglen, e.cgcent.resolve(centers),
                              e.fgcent.resolve(centers), e.chgm)
+
+
+def processor_ranks(inp):
+    """(ranks to run, processor count) for the mg-para sections: the `async`
+    rank when the sections name one (the reference's asynchronous mode, one
+    processor per run), else every rank of the pdime grid in turn."""
+    para = [e for e in inp.elecs if e.method == "mg-para"]
+    if not para:
+        return [0], 1
+    nproc = para[0].pdime[0] * para[0].pdime[1] * para[0].pdime[2]
+    asyncs = {e.async_rank for e in para}
+    if asyncs == {None}:
+        return list(range(nproc)), nproc
+    return [asyncs.pop()], nproc
 
 
 def _kj(temp, e):
@@ -77,8 +98,10 @@ class Runner:
         self.outdir = outdir      # where `write` files go (default: the input's directory)
         self.precision = precision
         self.mixed_tol = mixed_tol
-        self.results = []        # per calculation
+        self.results = []        # per calculation (of the processor being run)
         self.elec2calc = []      # ELEC section -> index of its last calculation
+        self.ranks = [0]
+        self.per_rank = []
         self.mols = []
 
     def p(self, s=""):
@@ -92,15 +115,38 @@ class Runner:
             mol = driver.Molecule.from_pqr(self.inp.path(m.path))
             self.p(f"Got {len(mol)} atoms from {m.path}")
             self.mols.append(mol)
+        ranks, nproc = processor_ranks(self.inp)
+        self.ranks = ranks
+        self.per_rank = []     # (results, elec2calc) per processor run
+        for r in ranks:
+            if nproc > 1:
+                self.p("========================================")
+                self.p(f"PROCESSOR {r} OF {nproc}")
+            self.results, self.elec2calc = [], []
+            self._calculations(r)
+            self.per_rank.append((self.results, self.elec2calc))
+        self.p("----------------------------------------")
+        self.p("PRINT STATEMENTS")
+        for pr in self.inp.prints:
+            if pr.what == "elecenergy":
+                self._print_energy(pr)
+            elif pr.what == "elecforce":
+                self._print_force(pr)
+        self.p("----------------------------------------")
+        return self.per_rank
+
+    def _calculations(self, rank):
+        """main.c's calculation loop for one processor."""
         centers = [m.center for m in self.mols]
         prev = None
         icalc = 0
-        for ie, e in enumerate(self.inp.elecs):
-            chain = _mgparm_chain(e, centers)
+        for e in self.inp.elecs:
+            chain = _mgparm_chain(e, centers, rank)
             pbe = _pbe_dict(e)
             mol = self.mols[e.mol - 1]
             for level, mgp in enumerate(chain):
                 focus = level > 0 or e.bcfl == "focus"
+                finest = level == len(chain) - 1
                 self.p("----------------------------------------")
                 label = f" ({e.name})" if e.name else ""
                 self.p(f"CALCULATION #{icalc + 1}{label}: MULTIGRID")
@@ -117,22 +163,14 @@ class Runner:
                 res = CalcResult(e.temp, e.calcenergy, e.calcforce)
                 self._energy(calc, e, res)
                 self._forces(calc, e, res, mol)
-                self._write(calc, e)
+                if finest:    # only the finest level of a chain writes (nosh.c:2082-2083)
+                    self._write(calc, e)
                 self.results.append(res)
                 prev = calc
                 icalc += 1
             self.elec2calc.append(icalc - 1)
         if prev is not None:
             prev.release()
-        self.p("----------------------------------------")
-        self.p("PRINT STATEMENTS")
-        for pr in self.inp.prints:
-            if pr.what == "elecenergy":
-                self._print_energy(pr)
-            elif pr.what == "elecforce":
-                self._print_force(pr)
-        self.p("----------------------------------------")
-        return self.results
 
     # ------------------------------------------------------------ per calc
     def _energy(self, calc, e, res):
@@ -232,35 +270,43 @@ class Runner:
         return ids
 
     def _print_energy(self, pr):
-        """printElecEnergy (routines.c:3117-3195)."""
+        """printElecEnergy (routines.c:3117-3195): the signed sum of the
+        sections' energies on each processor, then their sum over the
+        processors (Vcom_reduce)."""
         ids = self._header(pr, "energy")
-        res0 = self.results[self.elec2calc[ids[0]]]
-        if res0.calcenergy == "no":
-            self.p(f"  Didn't calculate energy in Calculation #{self.elec2calc[ids[0]] + 1}")
-            return
-        e = VUNIT_KB * 1e-3 * VUNIT_NA * res0.temp * res0.energy
-        for op, k in zip(pr.ops, ids[1:]):
-            r = self.results[self.elec2calc[k]]
-            scalar = 1.0 if op == "+" else -1.0
-            e += scalar * VUNIT_KB * 1e-3 * VUNIT_NA * r.temp * r.energy
-        self.p("  Local net energy (PE 0) = %1.12E kJ/mol" % e)
-        self.p("  Global net ELEC energy = %1.12E kJ/mol" % e)
+        total = 0.0
+        for r, (results, elec2calc) in zip(self.ranks, self.per_rank):
+            res0 = results[elec2calc[ids[0]]]
+            if res0.calcenergy == "no":
+                self.p(f"  Didn't calculate energy in Calculation #{elec2calc[ids[0]] + 1}")
+                return
+            e = VUNIT_KB * 1e-3 * VUNIT_NA * res0.temp * res0.energy
+            for op, k in zip(pr.ops, ids[1:]):
+                res = results[elec2calc[k]]
+                scalar = 1.0 if op == "+" else -1.0
+                e += scalar * VUNIT_KB * 1e-3 * VUNIT_NA * res.temp * res.energy
+            self.p("  Local net energy (PE %d) = %1.12E kJ/mol" % (r, e))
+            total = total + e
+        self.p("  Global net ELEC energy = %1.12E kJ/mol" % total)
 
     def _print_force(self, pr):
         """printElecForce (routines.c:3549-3821)."""
         ids = self._header(pr, "force")
-        res0 = self.results[self.elec2calc[ids[0]]]
-        if res0.calcforce == "no":
-            self.p(f"  Didn't calculate force in calculation #{self.elec2calc[ids[0]] + 1}")
-            return
-        f = VUNIT_KB * 1e-3 * VUNIT_NA * res0.temp * res0.forces
-        for op, k in zip(pr.ops, ids[1:]):
-            r = self.results[self.elec2calc[k]]
-            if r.calcforce != res0.calcforce or r.forces.shape != res0.forces.shape:
-                self.p("  Inconsistent calcforce declarations or force counts")
+        f = None
+        for results, elec2calc in self.per_rank:
+            res0 = results[elec2calc[ids[0]]]
+            if res0.calcforce == "no":
+                self.p(f"  Didn't calculate force in calculation #{elec2calc[ids[0]] + 1}")
                 return
-            scalar = 1.0 if op == "+" else -1.0
-            f = f + scalar * VUNIT_KB * 1e-3 * VUNIT_NA * r.temp * r.forces
+            lf = VUNIT_KB * 1e-3 * VUNIT_NA * res0.temp * res0.forces
+            for op, k in zip(pr.ops, ids[1:]):
+                r = results[elec2calc[k]]
+                if r.calcforce != res0.calcforce or r.forces.shape != res0.forces.shape:
+                    self.p("  Inconsistent calcforce declarations or force counts")
+                    return
+                scalar = 1.0 if op == "+" else -1.0
+                lf = lf + scalar * VUNIT_KB * 1e-3 * VUNIT_NA * r.temp * r.forces
+            f = lf if f is None else f + lf
         if res0.calcforce == "total":
             self.p("  Printing net forces (kJ/mol/A).")
             self.p("  Legend:")

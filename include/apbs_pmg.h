@@ -229,11 +229,18 @@ void MGparm_dtor(MGparm **thee);                /* src/generic/mgparm.c:172 */
  * and dime; returns 1 (success) / 0. */
 int MGparm_check(MGparm *thee);
 /* NOsh_setupCalcMGAUTO (src/generic/nosh.c:1828-2101): expands an mg-auto
- * ELEC block into its focusing chain; writes up to maxcalc mg-manual MGparm
+ * ELEC block into its focusing chain; writes up to maxcalc MGparm
  * copies to out (the caller allocates them with MGparm_ctor) and returns
  * the number of levels, 0 on failure.  No reference counterpart as a
  * separate call: the reference appends to the NOsh calculation list. */
 int MGparm_setupAuto(MGparm *elec, MGparm **out, int maxcalc);
+/* NOsh_setupCalcMGPARA (src/generic/nosh.c:2103-2394) for processor `rank`
+ * (the `async` rank) of an mg-para block (type MCT_PARALLEL, pdime and ofrac
+ * set): the processor's overlapping fine grid and disjoint partition, then
+ * that grid's focusing chain as MGparm_setupAuto writes it, every level
+ * MCT_PARALLEL with the partition kept.  Returns the number of levels, 0 on
+ * failure.  No reference counterpart as a separate call (see above). */
+int MGparm_setupPara(MGparm *elec, int rank, MGparm **out, int maxcalc);
 
 /* ------------------------------------------------------------------ Vpmgp
  * struct sVpmgp, field for field (src/mg/vpmgp.h:80-198). */

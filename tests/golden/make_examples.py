@@ -18,7 +18,8 @@ import sys
 
 # example directory -> input stems run by tests/test_gpu_examples.py
 EXAMPLES = {
-    "born": ["apbs-mol-auto", "apbs-smol-auto", "apbs-forces"],
+    "born": ["apbs-mol-auto", "apbs-smol-auto", "apbs-forces", "apbs-mol-parallel",
+             "apbs-smol-parallel"],
     "solv": ["apbs-mol", "apbs-smol"],
     "ion-protein": ["apbs-mol-pdiel2", "apbs-smol-pdiel2", "apbs-mol-pdiel12",
                     "apbs-smol-pdiel12"],
@@ -27,12 +28,14 @@ EXAMPLES = {
     "FKBP": ["1d7h-dmso-mol", "1d7h-dmso-smol", "1d7i-dss-mol", "1d7i-dss-smol"],
     "ionize": ["apbs-mol", "apbs-smol"],
     "point-pmf": ["complex-0_1", "complex-0_2", "complex-0_3", "complex-0_4"],
-    "actin-dimer": ["apbs-mol-auto", "apbs-smol-auto"],
+    "actin-dimer": ["apbs-mol-auto", "apbs-smol-auto", "apbs-mol-parallel", "apbs-smol-parallel"],
 }
 # test_cases.cfg section -> example directory
 SECTIONS = {"born": "born", "solv": "solv", "ion-protein": "ion-protein", "hca-bind": "hca-bind",
             "pka-lig": "pka-lig", "FKBP": "FKBP", "ionize": "ionize", "point-pmf": "point-pmf",
-            "actin-dimer-auto": "actin-dimer"}
+            "actin-dimer-auto": "actin-dimer", "actin-dimer-parallel": "actin-dimer"}
+# sections test_cases.cfg keeps commented out as slow; their values are still read
+COMMENTED = {"actin-dimer-parallel"}
 EXTRA = {"born": ["polarforces"]}
 
 
@@ -49,11 +52,15 @@ def main(ref):
     with open(os.path.join(ref, "tests", "test_cases.cfg")) as fh:
         for line in fh:
             s = line.strip()
-            if not s or s.startswith("#"):
-                continue
-            m = re.match(r"\[(.+)\]", s)
+            m = re.match(r"#?\s*\[(.+)\]", s)
             if m:
                 section = m.group(1)
+                if s.startswith("#") and section not in COMMENTED:
+                    section = None
+                continue
+            if s.startswith("#") and section in COMMENTED:
+                s = s.lstrip("#").strip()
+            if not s or s.startswith("#"):
                 continue
             key, _, val = s.partition(":")
             key = key.strip()

@@ -97,7 +97,13 @@ def test_print_ids_digits_before_names():
 
 @pytest.mark.parametrize("snippet,reason", [
     ("read mol pdb a.pdb end elec mg-manual end quit", "only PQR"),
-    ("read mol pqr a.pqr end elec mg-para end quit", "mg-para"),
+    ("read mol pqr a.pqr end elec mg-dummy end quit", "mg-dummy"),
+    ("read mol pqr a.pqr end elec mg-para dime 9 9 9 cglen 9 9 9 fglen 5 5 5 cgcent mol 1 "
+     "fgcent mol 1 mol 1 lpbe bcfl mdh pdie 1 sdie 1 chgm spl2 srfm spl2 swin 0.3 temp 300 "
+     "end quit", "pdime"),
+    ("read mol pqr a.pqr end elec mg-para pdime 2 1 1 ofrac 0.1 async 2 dime 9 9 9 cglen 9 9 9 "
+     "fglen 5 5 5 cgcent mol 1 fgcent mol 1 mol 1 lpbe bcfl mdh pdie 1 sdie 1 chgm spl2 "
+     "srfm spl2 swin 0.3 temp 300 end quit", "outside the pdime"),
     ("read mol pqr a.pqr end elec fe-manual end quit", "fe-manual"),
     ("read mol pqr a.pqr end elec mg-manual dime 9 9 9 glen 5 5 5 gcent mol 1 mol 1 lpbe "
      "bcfl map pdie 1 sdie 1 chgm spl2 srfm spl2 swin 0.3 temp 300 end quit", "bcfl map"),
@@ -168,8 +174,8 @@ def _runner_with(results, inp):
     from apbs_amd import run
     out = io.StringIO()
     r = run.Runner(inp, out=out)
-    r.results = results
-    r.elec2calc = list(range(len(results)))
+    r.per_rank = [(results, list(range(len(results))))]
+    r.ranks = [0]
     return r, out
 
 
