@@ -17,8 +17,8 @@ LIB_PATH = os.path.join(HERE, "libapbs_pmg.so")
 
 # enums of include/apbs_pmg.h (values of src/generic/vhal.h)
 VSOL_Newton, VSOL_MG = 1, 2
-VSM_MOL, VSM_MOLSMOOTH, VSM_SPLINE = 0, 1, 2
-VCM_TRIL, VCM_BSPL2 = 0, 1
+VSM_MOL, VSM_MOLSMOOTH, VSM_SPLINE, VSM_SPLINE3, VSM_SPLINE4 = 0, 1, 2, 3, 4
+VCM_TRIL, VCM_BSPL2, VCM_BSPL4 = 0, 1, 2
 BCFL_ZERO, BCFL_SDH, BCFL_MDH, BCFL_FOCUS = 0, 1, 2, 4
 NONLIN_LPBE, NONLIN_NPBE = 0, 1
 MCT_MANUAL, MCT_AUTO, MCT_PARALLEL = 0, 1, 2

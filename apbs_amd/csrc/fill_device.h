@@ -35,15 +35,30 @@ struct AtomsDev {
 // ------------------------------------------------------------------ charges
 // chgm spl2 (vpmg.c:5123-5248) / spl0 (vpmg.c:4986-5089), charge already
 // scaled by zmagic/(hx hy hz) per atom order of the reference.
-void charge_map(hipStream_t s, const Geo& g, const AtomsDev& A, const Cells& C, int spl2,
+// mode: 0 trilinear (fillcoChargeSpline1), 1 cubic B-spline (fillcoChargeSpline2),
+// 2 5th-order B-spline (chgm spl4, fillcoPermanentMultipole with charges only).
+void charge_map(hipStream_t s, const Geo& g, const AtomsDev& A, const Cells& C, int mode,
                 double zmagic, int bcfl_lenient, double* charge);
 
 // ------------------------------------------------------------- coefficients
 // srfm spl2 (vpmg.c:4631-4854): kappa, epsx/y/z in [0,1] windows, then the
 // linear map to epsp..epsw and the ion mask.
+// order 2: srfm spl2 (cubic window); 3 / 4: srfm spl3 / spl4 (polynomial
+// windows), coef = kSplCoefHost doubles per atom from spline_coefficients.
 void coef_spline(hipStream_t s, const Geo& g, const AtomsDev& A, const Cells& C, double win,
                  double irad, double ionmask, double epsw, double epsp, double* kappa,
-                 double* epsx, double* epsy, double* epsz);
+                 double* epsx, double* epsy, double* epsz, int order = 2,
+                 const double* coef = nullptr);
+
+// Per-atom window coefficients of srfm spl3 / spl4, kSplCoefHost doubles per
+// atom: [0, 8) the dielectric window (atom radius +- win; also the gradient
+// of the dielectric-boundary force), [8, 16) the ion window of the kappa map,
+// [16, 24) the ion window of the ion-boundary force gradient.  spl3 keeps the
+// reference's kappa coefficients, which are the dielectric ones divided once
+// more by the ion denominator (vpmg.c:7632-7643).
+constexpr int kSplCoefHost = 24;
+void spline_coefficients(int order, int natoms, const double* radius, double win, double irad,
+                         double* out);
 
 // SAS points of every atom (Vacc_atomSurf, vacc.c:866-922): flags[a*nref+q]
 // = 1 when the point rad*ref+apos is outside every other inflated atom.
@@ -122,13 +137,17 @@ void energy_qf_point(hipStream_t s, const Geo& g, const AtomsDev& A, const doubl
 // ------------------------------------------------------------------ forces
 // Per-atom force components (kT/A), out[3*a..3*a+2]; rc[a] = 1/0 like the
 // reference's return codes.
-void force_qf(hipStream_t s, const Geo& g, const AtomsDev& A, int spl2, const double* u,
+void force_qf(hipStream_t s, const Geo& g, const AtomsDev& A, int mode, const double* u,
               double* out);
+// order / coef: the surface method's window (2 = spl2; 3, 4 = spl3, spl4 with
+// the per-atom coefficients of spline_coefficients).
 void force_ib(hipStream_t s, const Geo& g, const AtomsDev& A, double win, double irad,
               double zkappa2, double izmagic, int nonlin, double ionstr, const IonBulk& ions,
-              const double* kappa, const double* u, double* out);
+              const double* kappa, const double* u, double* out, int order = 2,
+              const double* coef = nullptr);
 void force_db(hipStream_t s, const Geo& g, const AtomsDev& A, double win, double srad,
               double epsp, double epsw, double izmagic, const double* epsx, const double* epsy,
-              const double* epsz, const double* u, double* out, int* rc);
+              const double* epsz, const double* u, double* out, int* rc, int order = 2,
+              const double* coef = nullptr);
 
 }  // namespace pmg

@@ -87,12 +87,94 @@ __device__ inline double dbspline2(double x) {  // vpmg.c:5106-5121
 
 __device__ inline double vfchi(int iint, double iflt) { return 1.5 + ((double)(iint) - (iflt)); }
 
+// The 5th-order B-spline of chgm spl4 and its first two derivatives
+// (bspline4 / dbspline4 / d2bspline4, vpmg.c:6731-6822), argument
+// VFCHI4(i, f) = 2.5 + (i - f) (vpmg.h:939).
+__device__ inline double vfchi4(int iint, double iflt) { return 2.5 + ((double)(iint) - (iflt)); }
+
+__device__ inline double bspline4(double x) {
+  const double one6 = 1.0 / 6.0, one8 = 1.0 / 8.0, one24 = 1.0 / 24.0;
+  const double thirteen24 = 13.0 / 24.0, fourtyseven24 = 47.0 / 24.0, seventeen24 = 17.0 / 24.0;
+  double m, m2;
+  if ((x > 0.0) && (x <= 1.0)) {
+    m = x * x;
+    return one24 * m * m;
+  } else if ((x > 1.0) && (x <= 2.0)) {
+    m = x - 1.0;
+    m2 = m * m;
+    return -one8 + one6 * x + m2 * (0.25 + one6 * m - one6 * m2);
+  } else if ((x > 2.0) && (x <= 3.0)) {
+    m = x - 2.0;
+    m2 = m * m;
+    return -thirteen24 + 0.5 * x + m2 * (-0.25 - 0.5 * m + 0.25 * m2);
+  } else if ((x > 3.0) && (x <= 4.0)) {
+    m = x - 3.0;
+    m2 = m * m;
+    return fourtyseven24 - 0.5 * x + m2 * (-0.25 + 0.5 * m - one6 * m2);
+  } else if ((x > 4.0) && (x <= 5.0)) {
+    m = x - 4.0;
+    m2 = m * m;
+    return seventeen24 - one6 * x + m2 * (0.25 - one6 * m + one24 * m2);
+  }
+  return 0.0;
+}
+
+__device__ inline double dbspline4(double x) {
+  const double one6 = 1.0 / 6.0, one3 = 1.0 / 3.0, two3 = 2.0 / 3.0, thirteen6 = 13.0 / 6.0;
+  double m, m2;
+  if ((x > 0.0) && (x <= 1.0)) {
+    m2 = x * x;
+    return one6 * x * m2;
+  } else if ((x > 1.0) && (x <= 2.0)) {
+    m = x - 1.0;
+    m2 = m * m;
+    return -one3 + 0.5 * x + m2 * (0.5 - two3 * m);
+  } else if ((x > 2.0) && (x <= 3.0)) {
+    m = x - 2.0;
+    m2 = m * m;
+    return 1.5 - 0.5 * x + m2 * (-1.5 + m);
+  } else if ((x > 3.0) && (x <= 4.0)) {
+    m = x - 3.0;
+    m2 = m * m;
+    return 1.0 - 0.5 * x + m2 * (1.5 - two3 * m);
+  } else if ((x > 4.0) && (x <= 5.0)) {
+    m = x - 4.0;
+    m2 = m * m;
+    return -thirteen6 + 0.5 * x + m2 * (-0.5 + one6 * m);
+  }
+  return 0.0;
+}
+
+__device__ inline double d2bspline4(double x) {
+  double m, m2;
+  if ((x > 0.0) && (x <= 1.0)) {
+    return 0.5 * x * x;
+  } else if ((x > 1.0) && (x <= 2.0)) {
+    m = x - 1.0;
+    m2 = m * m;
+    return -0.5 + x - 2.0 * m2;
+  } else if ((x > 2.0) && (x <= 3.0)) {
+    m = x - 2.0;
+    m2 = m * m;
+    return 5.5 - 3.0 * x + 3.0 * m2;
+  } else if ((x > 3.0) && (x <= 4.0)) {
+    m = x - 3.0;
+    m2 = m * m;
+    return -9.5 + 3.0 * x - 2.0 * m2;
+  } else if ((x > 4.0) && (x <= 5.0)) {
+    m = x - 4.0;
+    m2 = m * m;
+    return 4.5 - x + 0.5 * m2;
+  }
+  return 0.0;
+}
+
 __device__ inline int imin2(int a, int b) { return a <= b ? a : b; }
 __device__ inline int imax2(int a, int b) { return a >= b ? a : b; }
 
 // ----------------------------------------------------------------- charges
 __global__ void __launch_bounds__(kB)
-    k_charge(Geo g, AtomsDev A, Cells C, int spl2, double zmagic, double* charge) {
+    k_charge(Geo g, AtomsDev A, Cells C, int mode, double zmagic, double* charge) {
   const int t = blockIdx.x * kB + threadIdx.x;
   const int k = blockIdx.y;
   const int i = t % g.nx, j = t / g.nx;
@@ -106,7 +188,31 @@ __global__ void __launch_bounds__(kB)
     const double fi = (ap[0] - g.xmin) / g.hx, fj = (ap[1] - g.ymin) / g.hy,
                  fk = (ap[2] - g.zmin) / g.hz;
     const double c = A.chg[a] * zmagic / (g.hx * g.hy * g.hz);
-    if (spl2) {
+    if (mode == 2) {
+      // chgm spl4: fillcoPermanentMultipole (vpmg.c:6835-7050) with the atom's
+      // charge only; its dipole and quadrupole terms are exact zeros here and
+      // are kept so each contribution rounds as the reference's expression
+      if ((ap[0] <= (g.xmin - 2 * g.hx)) || (ap[0] >= (g.xmax + 2 * g.hx)) ||
+          (ap[1] <= (g.ymin - 2 * g.hy)) || (ap[1] >= (g.ymax + 2 * g.hy)) ||
+          (ap[2] <= (g.zmin - 2 * g.hz)) || (ap[2] >= (g.zmax + 2 * g.hz)))
+        continue;
+      const int ip2 = imin2((int)ceil(fi) + 2, g.nx - 1), im2 = imax2((int)floor(fi) - 2, 0);
+      const int jp2 = imin2((int)ceil(fj) + 2, g.ny - 1), jm2 = imax2((int)floor(fj) - 2, 0);
+      const int kp2 = imin2((int)ceil(fk) + 2, g.nz - 1), km2 = imax2((int)floor(fk) - 2, 0);
+      if (i < im2 || i > ip2 || j < jm2 || j > jp2 || k < km2 || k > kp2) continue;
+      const double f = zmagic / (g.hx * g.hy * g.hz);
+      const double c4 = A.chg[a] * f;
+      const double zero = 0.0;
+      const double mi = vfchi4(i, fi), mj = vfchi4(j, fj), mk = vfchi4(k, fk);
+      const double mx = bspline4(mi), dmx = dbspline4(mi), d2mx = d2bspline4(mi);
+      const double my = bspline4(mj), dmy = dbspline4(mj), d2my = d2bspline4(mj);
+      const double mz = bspline4(mk), dmz = dbspline4(mk), d2mz = d2bspline4(mk);
+      const double v = mx * my * mz * c4 - dmx * my * mz * zero - mx * dmy * mz * zero -
+                       mx * my * dmz * zero + d2mx * my * mz * zero + dmx * dmy * mz * zero +
+                       mx * d2my * mz * zero + dmx * my * dmz * zero + mx * dmy * dmz * zero +
+                       mx * my * d2mz * zero;
+      sum += v;
+    } else if (mode == 1) {
       if ((ap[0] <= (g.xmin - g.hx)) || (ap[0] >= (g.xmax + g.hx)) || (ap[1] <= (g.ymin - g.hy)) ||
           (ap[1] >= (g.ymax + g.hy)) || (ap[2] <= (g.zmin - g.hz)) || (ap[2] >= (g.zmax + g.hz)))
         continue;
@@ -139,9 +245,38 @@ __global__ void __launch_bounds__(kB)
 }
 
 // ------------------------------------------------------------ spline coefs
+// Polynomial windows of srfm spl3 / spl4 (fillcoCoefSpline3 vpmg.c:7525,
+// fillcoCoefSpline4 vpmg.c:7052): chi(s) = sum_n c_n s^n of the distance s,
+// clamped to [0, 1], with s^2 taken as given (the reference squares s in the
+// x and y branches and reuses the squared distance in the kappa and z ones).
+// The per-atom coefficients come from the host (kSplCoef doubles per atom:
+// dielectric, ion for the maps, ion for the ion-boundary force).
+constexpr int kSplCoef = 24;
+
+template <int kOrd>
+__device__ inline double poly_chi(const double* c, double sm, double sm2) {
+  const double sm3 = sm2 * sm, sm4 = sm3 * sm, sm5 = sm4 * sm;
+  double v;
+  if (kOrd == 3) {
+    v = c[0] + c[1] * sm + c[2] * sm2 + c[3] * sm3 + c[4] * sm4 + c[5] * sm5;
+  } else {
+    const double sm6 = sm5 * sm, sm7 = sm6 * sm;
+    v = c[0] + c[1] * sm + c[2] * sm2 + c[3] * sm3 + c[4] * sm4 + c[5] * sm5 + c[6] * sm6 +
+        c[7] * sm7;
+  }
+  if (v > 1.0) {
+    v = 1.0;
+  } else if (v < 0.0) {
+    v = 0.0;
+  }
+  return v;
+}
+
+template <int kOrd>
 __global__ void __launch_bounds__(kB)
     k_coef_spline(Geo g, AtomsDev A, Cells C, double win, double irad, double ionmask, double epsw,
-                  double epsp, double* kappa, double* epsx, double* epsy, double* epsz) {
+                  double epsp, const double* pc, double* kappa, double* epsx, double* epsy,
+                  double* epsz) {
   const int t = blockIdx.x * kB + threadIdx.x;
   const int k = blockIdx.y;
   const int i = t % g.nx, j = t / g.nx;
@@ -176,8 +311,13 @@ __global__ void __launch_bounds__(kB)
       const double d2 = dz2 + dy2 + dx2;
       if (d2 <= ictot2) ka = 0.0;
       if ((d2 < itot2) && (d2 > ictot2)) {
-        const double dist = sqrt(d2), sm = dist - (arad + irad) + win, sm2 = sm * sm;
-        ka *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+        if (kOrd == 2) {
+          const double dist = sqrt(d2), sm = dist - (arad + irad) + win, sm2 = sm * sm;
+          ka *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+        } else {
+          const double dist = sqrt(d2);
+          ka *= poly_chi<kOrd>(pc + kSplCoef * a + 8, dist, d2);
+        }
       }
     }
     if (ex > VPMGSMALL) {
@@ -186,8 +326,13 @@ __global__ void __launch_bounds__(kB)
       if (d2 >= stot2) ex *= 1.0;
       if (d2 <= sctot2) ex = 0.0;
       if ((d2 > sctot2) && (d2 < stot2)) {
-        const double dist = sqrt(d2), sm = dist - arad + win, sm2 = sm * sm;
-        ex *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+        if (kOrd == 2) {
+          const double dist = sqrt(d2), sm = dist - arad + win, sm2 = sm * sm;
+          ex *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+        } else {
+          const double sm = sqrt(d2);
+          ex *= poly_chi<kOrd>(pc + kSplCoef * a, sm, sm * sm);
+        }
       }
     }
     if (ey > VPMGSMALL) {
@@ -196,8 +341,13 @@ __global__ void __launch_bounds__(kB)
       if (d2 >= stot2) ey *= 1.0;
       if (d2 <= sctot2) ey = 0.0;
       if ((d2 > sctot2) && (d2 < stot2)) {
-        const double dist = sqrt(d2), sm = dist - arad + win, sm2 = sm * sm;
-        ey *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+        if (kOrd == 2) {
+          const double dist = sqrt(d2), sm = dist - arad + win, sm2 = sm * sm;
+          ey *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+        } else {
+          const double sm = sqrt(d2);
+          ey *= poly_chi<kOrd>(pc + kSplCoef * a, sm, sm * sm);
+        }
       }
     }
     if (ez > VPMGSMALL) {
@@ -206,8 +356,13 @@ __global__ void __launch_bounds__(kB)
       if (d2 >= stot2) ez *= 1.0;
       if (d2 <= sctot2) ez = 0.0;
       if ((d2 > sctot2) && (d2 < stot2)) {
-        const double dist = sqrt(d2), sm = dist - arad + win, sm2 = sm * sm;
-        ez *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+        if (kOrd == 2) {
+          const double dist = sqrt(d2), sm = dist - arad + win, sm2 = sm * sm;
+          ez *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+        } else {
+          const double sm = sqrt(d2);
+          ez *= poly_chi<kOrd>(pc + kSplCoef * a, sm, d2);
+        }
       }
     }
   }
@@ -831,7 +986,7 @@ __global__ void k_seq_sum(const double* v, int n, double* out) {
 }
 
 // ------------------------------------------------------------------ forces
-__global__ void k_force_qf(Geo g, AtomsDev A, int spl2, const double* u, double* out) {
+__global__ void k_force_qf(Geo g, AtomsDev A, int mode, const double* u, double* out) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= A.n) return;
   const double* ap = A.pos + 3 * a;
@@ -839,7 +994,34 @@ __global__ void k_force_qf(Geo g, AtomsDev A, int spl2, const double* u, double*
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
   const double charge = A.chg[a];
   if (A.partID[a] != 0) {
-    if (spl2) {  // qfForceSpline2 (vpmg.c:6043-6154)
+    if (mode == 2) {  // qfForceSpline4 (vpmg.c:6156-6275)
+      if (!((ap[0] <= (g.xmin + 2 * g.hx)) || (ap[0] >= (g.xmax - 2 * g.hx)) ||
+            (ap[1] <= (g.ymin + 2 * g.hy)) || (ap[1] >= (g.ymax - 2 * g.hy)) ||
+            (ap[2] <= (g.zmin + 2 * g.hz)) || (ap[2] >= (g.zmax - 2 * g.hz)))) {
+        const double fi = (ap[0] - g.xmin) / g.hx, fj = (ap[1] - g.ymin) / g.hy,
+                     fk = (ap[2] - g.zmin) / g.hz;
+        const int ip2 = imin2((int)ceil(fi) + 2, g.nx - 1), im2 = imax2((int)floor(fi) - 2, 0);
+        const int jp2 = imin2((int)ceil(fj) + 2, g.ny - 1), jm2 = imax2((int)floor(fj) - 2, 0);
+        const int kp2 = imin2((int)ceil(fk) + 2, g.nz - 1), km2 = imax2((int)floor(fk) - 2, 0);
+        double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+        for (int ii = im2; ii <= ip2; ii++) {
+          const double mi = vfchi4(ii, fi), mx = bspline4(mi), dmx = dbspline4(mi);
+          for (int jj = jm2; jj <= jp2; jj++) {
+            const double mj = vfchi4(jj, fj), my = bspline4(mj), dmy = dbspline4(mj);
+            for (int kk = km2; kk <= kp2; kk++) {
+              const double mk = vfchi4(kk, fk), mz = bspline4(mk), dmz = dbspline4(mk);
+              const double f = u[IX(ii, jj, kk, nx, ny)];
+              e0 += f * dmx * my * mz / g.hx;
+              e1 += f * mx * dmy * mz / g.hy;
+              e2 += f * mx * my * dmz / g.hz;
+            }
+          }
+        }
+        f0 = e0 * charge;
+        f1 = e1 * charge;
+        f2 = e2 * charge;
+      }
+    } else if (mode == 1) {  // qfForceSpline2 (vpmg.c:6043-6154)
       if (!((ap[0] <= (g.xmin + g.hx)) || (ap[0] >= (g.xmax - g.hx)) || (ap[1] <= (g.ymin + g.hy)) ||
             (ap[1] >= (g.ymax - g.hy)) || (ap[2] <= (g.zmin + g.hz)) || (ap[2] >= (g.zmax - g.hz)))) {
         const double fi = (ap[0] - g.xmin) / g.hx, fj = (ap[1] - g.ymin) / g.hy,
@@ -911,12 +1093,57 @@ __device__ inline void spline_grad(const double* c, double win, double infrad, c
   }
 }
 
+// Vacc_splineAccGradAtomNorm3 / Norm4 (vacc.c:1097, 1004) with the window
+// coefficients of (radius + infrad) +- win precomputed on the host.
+template <int kOrd>
+__device__ inline void poly_grad(const double* cf, double win, double infrad, const double* c,
+                                 const double* ap, double aradius, double* grad) {
+  grad[0] = grad[1] = grad[2] = 0.0;
+  if (aradius > 0.0) {
+    const double arad = aradius + infrad;
+    const double dist = sqrt((ap[0] - c[0]) * (ap[0] - c[0]) + (ap[1] - c[1]) * (ap[1] - c[1]) +
+                             (ap[2] - c[2]) * (ap[2] - c[2]));
+    if (dist < (arad - win)) return;
+    if (dist > (arad + win)) return;
+    if ((fabs(dist - (arad - win)) < VSMALL) || (fabs(dist - (arad + win)) < VSMALL)) return;
+    const double sm = dist, sm2 = sm * sm, sm3 = sm2 * sm, sm4 = sm3 * sm;
+    double mychi, mygrad;
+    if (kOrd == 3) {
+      const double sm5 = sm4 * sm;
+      mychi = cf[0] + cf[1] * sm + cf[2] * sm2 + cf[3] * sm3 + cf[4] * sm4 + cf[5] * sm5;
+      mygrad = cf[1] + 2.0 * cf[2] * sm + 3.0 * cf[3] * sm2 + 4.0 * cf[4] * sm3 +
+               5.0 * cf[5] * sm4;
+    } else {
+      const double sm5 = sm4 * sm, sm6 = sm5 * sm, sm7 = sm6 * sm;
+      mychi = cf[0] + cf[1] * sm + cf[2] * sm2 + cf[3] * sm3 + cf[4] * sm4 + cf[5] * sm5 +
+              cf[6] * sm6 + cf[7] * sm7;
+      mygrad = cf[1] + 2.0 * cf[2] * sm + 3.0 * cf[3] * sm2 + 4.0 * cf[4] * sm3 +
+               5.0 * cf[5] * sm4 + 6.0 * cf[6] * sm5 + 7.0 * cf[7] * sm6;
+    }
+    if (mychi <= 0.0) return;
+    if (mychi > 1.0) mychi = 1.0;
+    for (int i = 0; i < 3; i++) grad[i] = -(mygrad / mychi) * ((c[i] - ap[i]) / dist);
+  }
+}
+
+// The gradient of the surface method in use (Vpmg_splineSelect, vpmg.c:1890-1910).
+__device__ inline void win_grad(int ord, const double* cf, const double* c, double win,
+                                double infrad, const double* ap, double aradius, double* grad) {
+  if (ord == 3)
+    poly_grad<3>(cf, win, infrad, c, ap, aradius, grad);
+  else if (ord == 4)
+    poly_grad<4>(cf, win, infrad, c, ap, aradius, grad);
+  else
+    spline_grad(c, win, infrad, ap, aradius, grad);
+}
+
 // Ion-boundary force (vpmg.c:5440-5603): one block per atom.  The box is
 // walked by a single thread per atom in the reference's i, j, k order so
 // the sum rounds the same way.
 __global__ void k_force_ib(Geo g, AtomsDev A, double win, double irad, double zkappa2,
                            double izmagic, int nonlin, double ionstr, IonBulk I,
-                           const double* kappa, const double* u, double* out) {
+                           const double* kappa, const double* u, double* out, int ord,
+                           const double* pc) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= A.n) return;
   const double* ap = A.pos + 3 * a;
@@ -946,7 +1173,7 @@ __global__ void k_force_ib(Geo g, AtomsDev A, double win, double irad, double zk
           if ((dz2 + dy2 + dx2) <= rtot2) {
             const double gp[3] = {i * hx + g.xmin, j * hy + g.ymin, k * hzed + g.zmin};
             double tg[3];
-            spline_grad(gp, win, irad, ap, arad, tg);
+            win_grad(ord, pc ? pc + kSplCoef * a + 16 : nullptr, gp, win, irad, ap, arad, tg);
             const size_t p = IX(i, j, k, g.nx, g.ny);
             double fmag;
             if (nonlin) {
@@ -976,7 +1203,8 @@ __global__ void k_force_ib(Geo g, AtomsDev A, double win, double irad, double zk
 // the reference's loop order.
 __global__ void k_force_db(Geo g, AtomsDev A, double win, double srad, double epsp, double epsw,
                            double izmagic, const double* epsx, const double* epsy,
-                           const double* epsz, const double* u, double* out, int* rcs) {
+                           const double* epsz, const double* u, double* out, int* rcs, int ord,
+                           const double* pc) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= A.n) return;
   const double* ap = A.pos + 3 * a;
@@ -1004,6 +1232,7 @@ __global__ void k_force_db(Geo g, AtomsDev A, double win, double srad, double ep
     }
   }
   if (go) {
+    const double* cf = pc ? pc + kSplCoef * a : nullptr;
     const double deps = (epsw - epsp), depsi = 1.0 / deps;
     for (int i = imin; i <= imax; i++)
       for (int j = jmin; j <= jmax; j++)
@@ -1011,27 +1240,27 @@ __global__ void k_force_db(Geo g, AtomsDev A, double win, double srad, double ep
           double gp[3], dHx[3], dHy[3], dHz[3], dHxm[3], dHym[3], dHzm[3];
           gp[0] = (i + 0.5) * hx + g.xmin; gp[1] = j * hy + g.ymin; gp[2] = k * hzed + g.zmin;
           const double Hx = (epsx[IX(i, j, k, nx, ny)] - epsp) * depsi;
-          spline_grad(gp, win, 0., ap, arad, dHx);
+          win_grad(ord, cf, gp, win, 0., ap, arad, dHx);
           for (int l = 0; l < 3; l++) dHx[l] *= Hx;
           gp[0] = i * hx + g.xmin; gp[1] = (j + 0.5) * hy + g.ymin; gp[2] = k * hzed + g.zmin;
           const double Hy = (epsy[IX(i, j, k, nx, ny)] - epsp) * depsi;
-          spline_grad(gp, win, 0., ap, arad, dHy);
+          win_grad(ord, cf, gp, win, 0., ap, arad, dHy);
           for (int l = 0; l < 3; l++) dHy[l] *= Hy;
           gp[0] = i * hx + g.xmin; gp[1] = j * hy + g.ymin; gp[2] = (k + 0.5) * hzed + g.zmin;
           const double Hz = (epsz[IX(i, j, k, nx, ny)] - epsp) * depsi;
-          spline_grad(gp, win, 0., ap, arad, dHz);
+          win_grad(ord, cf, gp, win, 0., ap, arad, dHz);
           for (int l = 0; l < 3; l++) dHz[l] *= Hz;
           gp[0] = (i - 0.5) * hx + g.xmin; gp[1] = j * hy + g.ymin; gp[2] = k * hzed + g.zmin;
           const double Hxm = (epsx[IX(i - 1, j, k, nx, ny)] - epsp) * depsi;
-          spline_grad(gp, win, 0., ap, arad, dHxm);
+          win_grad(ord, cf, gp, win, 0., ap, arad, dHxm);
           for (int l = 0; l < 3; l++) dHxm[l] *= Hxm;
           gp[0] = i * hx + g.xmin; gp[1] = (j - 0.5) * hy + g.ymin; gp[2] = k * hzed + g.zmin;
           const double Hym = (epsy[IX(i, j - 1, k, nx, ny)] - epsp) * depsi;
-          spline_grad(gp, win, 0., ap, arad, dHym);
+          win_grad(ord, cf, gp, win, 0., ap, arad, dHym);
           for (int l = 0; l < 3; l++) dHym[l] *= Hym;
           gp[0] = i * hx + g.xmin; gp[1] = j * hy + g.ymin; gp[2] = (k - 0.5) * hzed + g.zmin;
           const double Hzm = (epsz[IX(i, j, k - 1, nx, ny)] - epsp) * depsi;
-          spline_grad(gp, win, 0., ap, arad, dHzm);
+          win_grad(ord, cf, gp, win, 0., ap, arad, dHzm);
           for (int l = 0; l < 3; l++) dHzm[l] *= Hzm;
           const size_t p = IX(i, j, k, nx, ny);
           const double fmag = u[p];
@@ -1059,16 +1288,23 @@ static dim3 grid_all(const Geo& g) {
   return dim3((unsigned)(((long)g.nx * g.ny + kB - 1) / kB), (unsigned)g.nz);
 }
 
-void charge_map(hipStream_t s, const Geo& g, const AtomsDev& A, const Cells& C, int spl2,
+void charge_map(hipStream_t s, const Geo& g, const AtomsDev& A, const Cells& C, int mode,
                 double zmagic, int, double* charge) {
-  k_charge<<<grid_all(g), kB, 0, s>>>(g, A, C, spl2, zmagic, charge);
+  k_charge<<<grid_all(g), kB, 0, s>>>(g, A, C, mode, zmagic, charge);
 }
 
 void coef_spline(hipStream_t s, const Geo& g, const AtomsDev& A, const Cells& C, double win,
                  double irad, double ionmask, double epsw, double epsp, double* kappa,
-                 double* epsx, double* epsy, double* epsz) {
-  k_coef_spline<<<grid_all(g), kB, 0, s>>>(g, A, C, win, irad, ionmask, epsw, epsp, kappa, epsx,
-                                           epsy, epsz);
+                 double* epsx, double* epsy, double* epsz, int order, const double* coef) {
+  if (order == 3)
+    k_coef_spline<3><<<grid_all(g), kB, 0, s>>>(g, A, C, win, irad, ionmask, epsw, epsp, coef,
+                                                kappa, epsx, epsy, epsz);
+  else if (order == 4)
+    k_coef_spline<4><<<grid_all(g), kB, 0, s>>>(g, A, C, win, irad, ionmask, epsw, epsp, coef,
+                                                kappa, epsx, epsy, epsz);
+  else
+    k_coef_spline<2><<<grid_all(g), kB, 0, s>>>(g, A, C, win, irad, ionmask, epsw, epsp, nullptr,
+                                                kappa, epsx, epsy, epsz);
 }
 
 void sas_flags(hipStream_t s, const AtomsDev& A, const Cells& C, const double* ref, int nref,
@@ -1184,23 +1420,77 @@ void energy_qf_point(hipStream_t s, const Geo& g, const AtomsDev& A, const doubl
   k_seq_sum<<<1, 64, 0, s>>>(part, A.n, out);
 }
 
-void force_qf(hipStream_t s, const Geo& g, const AtomsDev& A, int spl2, const double* u,
+void force_qf(hipStream_t s, const Geo& g, const AtomsDev& A, int mode, const double* u,
               double* out) {
-  k_force_qf<<<(A.n + 63) / 64, 64, 0, s>>>(g, A, spl2, u, out);
+  k_force_qf<<<(A.n + 63) / 64, 64, 0, s>>>(g, A, mode, u, out);
 }
 
 void force_ib(hipStream_t s, const Geo& g, const AtomsDev& A, double win, double irad,
               double zkappa2, double izmagic, int nonlin, double ionstr, const IonBulk& ions,
-              const double* kappa, const double* u, double* out) {
+              const double* kappa, const double* u, double* out, int order, const double* coef) {
   k_force_ib<<<(A.n + 63) / 64, 64, 0, s>>>(g, A, win, irad, zkappa2, izmagic, nonlin, ionstr, ions,
-                                            kappa, u, out);
+                                            kappa, u, out, order, coef);
 }
 
 void force_db(hipStream_t s, const Geo& g, const AtomsDev& A, double win, double srad, double epsp,
               double epsw, double izmagic, const double* epsx, const double* epsy,
-              const double* epsz, const double* u, double* out, int* rc) {
+              const double* epsz, const double* u, double* out, int* rc, int order,
+              const double* coef) {
   k_force_db<<<(A.n + 63) / 64, 64, 0, s>>>(g, A, win, srad, epsp, epsw, izmagic, epsx, epsy, epsz,
-                                            u, out, rc);
+                                            u, out, rc, order, coef);
+}
+
+// Window coefficients of one (b, e) = (r - win, r + win) pair: fillcoCoefSpline3
+// (vpmg.c:7597-7619) for order 3, fillcoCoefSpline4 (vpmg.c:7124-7147) for 4.
+static void window_coef(int order, double b, double e, double* c) {
+  for (int n = 0; n < 8; n++) c[n] = 0.0;
+  if (order == 3) {
+    const double e2 = e * e, e3 = e2 * e, e4 = e3 * e, e5 = e4 * e;
+    const double b2 = b * b, b3 = b2 * b, b4 = b3 * b, b5 = b4 * b;
+    (void)e4;
+    (void)e5;
+    const double denom = pow((e - b), 5.0);
+    c[0] = -10.0 * e2 * b3 + 5.0 * e * b4 - b5;
+    c[1] = 30.0 * e2 * b2;
+    c[2] = -30.0 * (e2 * b + e * b2);
+    c[3] = 10.0 * (e2 + 4.0 * e * b + b2);
+    c[4] = -15.0 * (e + b);
+    c[5] = 6;
+    for (int n = 0; n < 6; n++) c[n] = c[n] / denom;
+  } else {
+    const double e2 = e * e, e3 = e2 * e, e4 = e3 * e, e5 = e4 * e, e6 = e5 * e, e7 = e6 * e;
+    const double b2 = b * b, b3 = b2 * b, b4 = b3 * b, b5 = b4 * b, b6 = b5 * b, b7 = b6 * b;
+    const double denom = e7 - 7.0 * b * e6 + 21.0 * b2 * e5 - 35.0 * e4 * b3 + 35.0 * e3 * b4 -
+                         21.0 * b5 * e2 + 7.0 * e * b6 - b7;
+    c[0] = b4 * (35.0 * e3 - 21.0 * b * e2 + 7 * e * b2 - b3) / denom;
+    c[1] = -140.0 * b3 * e3 / denom;
+    c[2] = 210.0 * e2 * b2 * (e + b) / denom;
+    c[3] = -140.0 * e * b * (e2 + 3.0 * b * e + b2) / denom;
+    c[4] = 35.0 * (e3 + 9.0 * b * e2 + +9.0 * e * b2 + b3) / denom;
+    c[5] = -84.0 * (e2 + 3.0 * b * e + b2) / denom;
+    c[6] = 70.0 * (e + b) / denom;
+    c[7] = -20.0 / denom;
+  }
+}
+
+void spline_coefficients(int order, int natoms, const double* radius, double win, double irad,
+                         double* out) {
+  for (int a = 0; a < natoms; a++) {
+    const double arad = radius[a];
+    double* o = out + (size_t)kSplCoefHost * a;
+    window_coef(order, arad - win, arad + win, o);
+    // the ion window of the kappa map: b, e = irad + arad -+ win
+    if (order == 3) {
+      const double b = irad + arad - win, e = irad + arad + win;
+      const double denom = pow((e - b), 5.0);
+      for (int n = 0; n < 8; n++) o[8 + n] = n < 6 ? o[n] / denom : 0.0;
+    } else {
+      window_coef(order, irad + arad - win, irad + arad + win, o + 8);
+    }
+    // the ion-boundary force gradient: (arad + irad) -+ win
+    const double ar = arad + irad;
+    window_coef(order, ar - win, ar + win, o + 16);
+  }
 }
 
 }  // namespace pmg

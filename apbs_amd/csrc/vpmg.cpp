@@ -689,6 +689,10 @@ struct PmgDev {
   int dn = 0;
   // forces cache
   double *fqf = nullptr, *fib = nullptr, *fdb = nullptr;
+  int fchgm = -1, fsrfm = -1;      // the methods the cached forces were computed with
+  double* splcoef = nullptr;       // srfm spl3 / spl4 window coefficients per atom
+  int splorder = 0;
+  double splwin = 0.0;
   int* frc = nullptr;
   double* atmp = nullptr;  // per-atom scratch
   bool forces_valid = false;
@@ -1507,6 +1511,7 @@ extern "C" void Vpmg_dtor2(Vpmg* thee) {
     dfree(d->part);
     dfree(d->scal);
     dfree(d->fqf);
+    dfree(d->splcoef);
     dfree(d->fib);
     dfree(d->fdb);
     dfree(d->frc);
@@ -1554,6 +1559,29 @@ static pmg::Geo geo_of(Vpmg* thee) {
   return g;
 }
 
+static int srfm_order(Vsurf_Meth s) {
+  return s == VSM_SPLINE3 ? 3 : (s == VSM_SPLINE4 ? 4 : 2);
+}
+
+// Per-atom window coefficients of srfm spl3 / spl4 on the device (computed
+// on the host, so the reference's divisions and pow round the same way).
+static void ensure_spline_coef(Vpmg* thee, int order) {
+  PmgDev* d = DEV(thee);
+  if (order == 2 || d->natoms == 0) return;
+  if (d->splcoef && d->splorder == order && d->splwin == thee->splineWin) return;
+  const Valist* al = thee->pbe->alist;
+  std::vector<double> hr(d->natoms), hc((size_t)pmg::kSplCoefHost * d->natoms);
+  for (int a = 0; a < d->natoms; a++) hr[a] = al->atoms[a].radius;
+  pmg::spline_coefficients(order, d->natoms, hr.data(), thee->splineWin,
+                           thee->pbe->maxIonRadius, hc.data());
+  if (!d->splcoef) d->splcoef = dalloc<double>(hc.size());
+  HIPCHECK(hipMemcpyAsync(d->splcoef, hc.data(), hc.size() * sizeof(double),
+                          hipMemcpyHostToDevice, d->s));
+  HIPCHECK(hipStreamSynchronize(d->s));
+  d->splorder = order;
+  d->splwin = thee->splineWin;
+}
+
 // Vpmg_fillco (vpmg.c:5250-5415)
 extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
                            Vchrg_Meth chargeMeth, int useDielXMap, Vgrid*, int useDielYMap,
@@ -1562,10 +1590,6 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
   if (!thee) return 0;
   if (useDielXMap || useDielYMap || useDielZMap || useKappaMap || usePotMap || useChargeMap) {
     fprintf(stderr, "Vpmg_fillco: external maps are not supported\n");
-    return 0;
-  }
-  if (chargeMeth == VCM_BSPL4 || surfMeth == VSM_SPLINE3 || surfMeth == VSM_SPLINE4) {
-    fprintf(stderr, "Vpmg_fillco: chgm spl4 / srfm spl4 are not supported\n");
     return 0;
   }
   const double t0 = now();
@@ -1602,21 +1626,25 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
     rmax = std::max(rmax, hrad[a]);
   }
   const double hmax = std::max(std::max(p->hx, p->hy), p->hzed);
-  // charges: every atom reaches at most 2 grid spacings per axis
+  // charges: an atom reaches at most 2 grid spacings per axis (3 for spl4)
   {
-    DevCells cells = build_cells(d->s, hpos.data(), d->natoms, 2.0 * hmax + 1e-9);
-    pmg::charge_map(d->s, g, A, cells.c, chargeMeth == VCM_BSPL2, pbe->zmagic, 0, d->charge);
+    const double reach = (chargeMeth == VCM_BSPL4 ? 3.0 : 2.0) * hmax + 1e-9;
+    DevCells cells = build_cells(d->s, hpos.data(), d->natoms, reach);
+    pmg::charge_map(d->s, g, A, cells.c, (int)chargeMeth, pbe->zmagic, 0, d->charge);
     HIPCHECK(hipStreamSynchronize(d->s));
     cells.release();
   }
   if (!islap) {
     const double irad = pbe->maxIonRadius;
     const double ionmask = (ionstr > VPMGSMALL) ? 1.0 : 0.0;
-    if (surfMeth == VSM_SPLINE) {
+    if (surfMeth == VSM_SPLINE || surfMeth == VSM_SPLINE3 || surfMeth == VSM_SPLINE4) {
+      // fillcoCoefSpline (vpmg.c:4631), fillcoCoefSpline3 (:7525), fillcoCoefSpline4 (:7052)
+      const int order = srfm_order(surfMeth);
+      ensure_spline_coef(thee, order);
       const double reach = std::max(irad + rmax + splineWin, rmax + splineWin) + 1.5 * hmax;
       DevCells cells = build_cells(d->s, hpos.data(), d->natoms, reach);
       pmg::coef_spline(d->s, g, A, cells.c, splineWin, irad, ionmask, epsw, epsp, d->kappa,
-                       d->epsx, d->epsy, d->epsz);
+                       d->epsx, d->epsy, d->epsz, order, order == 2 ? nullptr : d->splcoef);
       HIPCHECK(hipStreamSynchronize(d->s));
       cells.release();
     } else {
@@ -2100,9 +2128,16 @@ extern "C" double Vpmg_qfAtomEnergy(Vpmg* thee, Vatom* atom) {
 // ================================================================== forces
 
 namespace {
-void ensure_forces(Vpmg* thee) {
+// The three force kinds for every atom, with the charge method chgm and the
+// surface method srfm the caller asks for (Vpmg_qfForce / Vpmg_ibForce /
+// Vpmg_dbForce take them as arguments); cached until the next solve or
+// partition change.
+void ensure_forces(Vpmg* thee, int chgm, int srfm) {
   PmgDev* d = DEV(thee);
-  if (d->forces_valid) return;
+  if (d->forces_valid && d->fchgm == chgm && d->fsrfm == srfm) return;
+  const int order = srfm_order((Vsurf_Meth)srfm);
+  ensure_spline_coef(thee, order);
+  const double* coef = order == 2 ? nullptr : d->splcoef;
   const int na = d->natoms;
   if (!d->fqf) {
     d->fqf = dalloc<double>(3 * (size_t)na);
@@ -2112,7 +2147,7 @@ void ensure_forces(Vpmg* thee) {
   }
   Vpbe* pbe = thee->pbe;
   const pmg::AtomsDev A = atoms_dev(d);
-  pmg::force_qf(d->s, d->geo, A, thee->chargeMeth == VCM_BSPL2, d->u, d->fqf);
+  pmg::force_qf(d->s, d->geo, A, chgm, d->u, d->fqf);
   pmg::IonBulk I{};
   I.nion = pbe->numIon;
   for (int i = 0; i < I.nion; i++) {
@@ -2121,10 +2156,10 @@ void ensure_forces(Vpmg* thee) {
   }
   pmg::force_ib(d->s, d->geo, A, thee->splineWin, pbe->maxIonRadius, pbe->zkappa2,
                 1.0 / pbe->zmagic, thee->pmgp->nonlin ? 1 : 0, pbe->bulkIonicStrength, I,
-                d->kappa, d->u, d->fib);
+                d->kappa, d->u, d->fib, order, coef);
   pmg::force_db(d->s, d->geo, A, thee->splineWin, pbe->solventRadius, pbe->soluteDiel,
                 pbe->solventDiel, 1.0 / pbe->zmagic, d->epsx, d->epsy, d->epsz, d->u, d->fdb,
-                d->frc);
+                d->frc, order, coef);
   d->hqf.resize(3 * (size_t)na);
   d->hib.resize(3 * (size_t)na);
   d->hdb.resize(3 * (size_t)na);
@@ -2135,15 +2170,17 @@ void ensure_forces(Vpmg* thee) {
   HIPCHECK(hipMemcpyAsync(d->hrc.data(), d->frc, na * sizeof(int), hipMemcpyDeviceToHost, d->s));
   HIPCHECK(hipStreamSynchronize(d->s));
   d->forces_valid = true;
+  d->fchgm = chgm;
+  d->fsrfm = srfm;
 }
 }  // namespace
 
 extern "C" int Vpmg_qfForce(Vpmg* thee, double* force, int atomID, Vchrg_Meth chgm) {
   force[0] = force[1] = force[2] = 0.0;
-  if (chgm == VCM_BSPL4) return 0;
+  if (chgm != VCM_TRIL && chgm != VCM_BSPL2 && chgm != VCM_BSPL4) return 0;
   PmgDev* d = DEV(thee);
   if (atomID < 0 || atomID >= d->natoms) return 0;
-  ensure_forces(thee);
+  ensure_forces(thee, chgm, thee->surfMeth);
   for (int k = 0; k < 3; k++) force[k] = d->hqf[3 * atomID + k];
   return 1;
 }
@@ -2157,7 +2194,7 @@ extern "C" int Vpmg_ibForce(Vpmg* thee, double* force, int atomID, Vsurf_Meth sr
   if (!spline_srfm(srfm)) return 0;  // vpmg.c:5471-5477
   PmgDev* d = DEV(thee);
   if (atomID < 0 || atomID >= d->natoms) return 0;
-  ensure_forces(thee);
+  ensure_forces(thee, thee->chargeMeth, srfm);
   for (int k = 0; k < 3; k++) force[k] = d->hib[3 * atomID + k];
   return 1;
 }
@@ -2168,7 +2205,7 @@ extern "C" int Vpmg_dbForce(Vpmg* thee, double* dbForce, int atomID, Vsurf_Meth 
   if (!spline_srfm(srfm)) return 0;  // vpmg.c:5638-5644
   PmgDev* d = DEV(thee);
   if (atomID < 0 || atomID >= d->natoms) return 0;
-  ensure_forces(thee);
+  ensure_forces(thee, thee->chargeMeth, srfm);
   for (int k = 0; k < 3; k++) dbForce[k] = d->hdb[3 * atomID + k];
   return d->hrc[atomID];
 }

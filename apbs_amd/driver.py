@@ -12,11 +12,12 @@ import numpy as np
 
 from . import (BCFL_FOCUS, BCFL_MDH, BCFL_SDH, BCFL_ZERO, MCT_AUTO, MCT_MANUAL, MCT_PARALLEL,
                NONLIN_LPBE, NONLIN_NPBE,
-               PCE_NO, PCE_TOTAL, VCM_BSPL2, VCM_TRIL, VSM_MOL, VSM_MOLSMOOTH, VSM_SPLINE,
-               VSOL_MG, VSOL_Newton, MGparm, lib)
+               PCE_NO, PCE_TOTAL, VCM_BSPL2, VCM_BSPL4, VCM_TRIL, VSM_MOL, VSM_MOLSMOOTH,
+               VSM_SPLINE, VSM_SPLINE3, VSM_SPLINE4, VSOL_MG, VSOL_Newton, MGparm, lib)
 
-SRFM = {"mol": VSM_MOL, "smol": VSM_MOLSMOOTH, "spl2": VSM_SPLINE}
-CHGM = {"spl0": VCM_TRIL, "spl2": VCM_BSPL2}
+SRFM = {"mol": VSM_MOL, "smol": VSM_MOLSMOOTH, "spl2": VSM_SPLINE, "spl3": VSM_SPLINE3,
+        "spl4": VSM_SPLINE4}
+CHGM = {"spl0": VCM_TRIL, "spl2": VCM_BSPL2, "spl4": VCM_BSPL4}
 BCFL = {"zero": BCFL_ZERO, "sdh": BCFL_SDH, "mdh": BCFL_MDH, "focus": BCFL_FOCUS}
 
 VUNIT_KB = 1.3806581e-23    # src/generic/vunit.h:75

@@ -443,10 +443,10 @@ def check_runnable(inp):
             why.append(f"ELEC {tag}: {e.pbe or 'no PBE type'} (lpbe and npbe are run)")
         if e.bcfl not in ("zero", "sdh", "mdh", "focus"):
             why.append(f"ELEC {tag}: bcfl {e.bcfl} (zero, sdh, mdh and focus are run)")
-        if e.chgm not in ("spl0", "spl2"):
-            why.append(f"ELEC {tag}: chgm {e.chgm} (spl0 and spl2 are run)")
-        if e.srfm not in ("mol", "smol", "spl2"):
-            why.append(f"ELEC {tag}: srfm {e.srfm} (mol, smol and spl2 are run)")
+        if e.chgm not in ("spl0", "spl2", "spl4"):
+            why.append(f"ELEC {tag}: chgm {e.chgm} (spl0, spl2 and spl4 are run)")
+        if e.srfm not in ("mol", "smol", "spl2", "spl3", "spl4"):
+            why.append(f"ELEC {tag}: srfm {e.srfm} (mol, smol, spl2, spl3 and spl4 are run)")
         if e.usemaps:
             why.append(f"ELEC {tag}: usemap is not supported")
         for what, fmt, _ in e.writes:
@@ -460,7 +460,7 @@ def check_runnable(inp):
             missing.append("srad")
         if e.srfm in ("mol", "smol") and e.sdens is None and (e.srad or 0) > 1e-9:
             missing.append("sdens")
-        if e.srfm == "spl2" and e.swin is None:
+        if e.srfm in ("spl2", "spl3", "spl4") and e.swin is None:
             missing.append("swin")
         if e.method == "mg-manual" and e.grid is None and e.glen is None:
             missing.append("grid or glen")

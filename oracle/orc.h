@@ -36,8 +36,8 @@ extern "C" {
 #define ORC_MAXION 10
 
 /* Enum values follow src/generic/vhal.h:103-123, 205-237. */
-enum { ORC_SRFM_MOL = 0, ORC_SRFM_SMOL = 1, ORC_SRFM_SPL2 = 2 };
-enum { ORC_CHGM_SPL0 = 0, ORC_CHGM_SPL2 = 1 };
+enum { ORC_SRFM_MOL = 0, ORC_SRFM_SMOL = 1, ORC_SRFM_SPL2 = 2, ORC_SRFM_SPL3 = 3, ORC_SRFM_SPL4 = 4 };
+enum { ORC_CHGM_SPL0 = 0, ORC_CHGM_SPL2 = 1, ORC_CHGM_SPL4 = 2 };
 enum { ORC_BCFL_ZERO = 0, ORC_BCFL_SDH = 1, ORC_BCFL_MDH = 2, ORC_BCFL_FOCUS = 4 };
 enum { ORC_LPBE = 0, ORC_NPBE = 1 };
 

@@ -353,8 +353,63 @@ static void fill_mol_smooth(orc_pmg *m)
     free(a1); free(a2); free(a3);
 }
 
-/* fillcoCoefSpline (vpmg.c:4631-4854): multiplicative cubic window per atom */
-static void fill_spline(orc_pmg *m)
+/* Coefficients of the polynomial windows of srfm spl3 and spl4 over
+ * [b, e] = [r - win, r + win]: fillcoCoefSpline3 (vpmg.c:7597-7619) and
+ * fillcoCoefSpline4 (vpmg.c:7124-7147); Vacc_splineAccGradAtomNorm3 / 4
+ * (vacc.c:1097, 1004) use the same formulas. */
+static void win_coef(int order, double b, double e, double *c)
+{
+    for (int n = 0; n < 8; n++) c[n] = 0.0;
+    double e2 = e * e, e3 = e2 * e, e4 = e3 * e, e5 = e4 * e;
+    double b2 = b * b, b3 = b2 * b, b4 = b3 * b, b5 = b4 * b;
+    if (order == 3) {
+        double denom = pow((e - b), 5.0);
+        c[0] = -10.0 * e2 * b3 + 5.0 * e * b4 - b5;
+        c[1] = 30.0 * e2 * b2;
+        c[2] = -30.0 * (e2 * b + e * b2);
+        c[3] = 10.0 * (e2 + 4.0 * e * b + b2);
+        c[4] = -15.0 * (e + b);
+        c[5] = 6;
+        for (int n = 0; n < 6; n++) c[n] = c[n] / denom;
+    } else {
+        double e6 = e5 * e, e7 = e6 * e, b6 = b5 * b, b7 = b6 * b;
+        double denom = e7 - 7.0 * b * e6 + 21.0 * b2 * e5 - 35.0 * e4 * b3 + 35.0 * e3 * b4 -
+                       21.0 * b5 * e2 + 7.0 * e * b6 - b7;
+        c[0] = b4 * (35.0 * e3 - 21.0 * b * e2 + 7 * e * b2 - b3) / denom;
+        c[1] = -140.0 * b3 * e3 / denom;
+        c[2] = 210.0 * e2 * b2 * (e + b) / denom;
+        c[3] = -140.0 * e * b * (e2 + 3.0 * b * e + b2) / denom;
+        c[4] = 35.0 * (e3 + 9.0 * b * e2 + 9.0 * e * b2 + b3) / denom;
+        c[5] = -84.0 * (e2 + 3.0 * b * e + b2) / denom;
+        c[6] = 70.0 * (e + b) / denom;
+        c[7] = -20.0 / denom;
+    }
+}
+
+/* the clamped window value of spl3 / spl4 at distance sm (with sm2 as the
+ * branch of the reference forms it) */
+static double win_chi(int order, const double *c, double sm, double sm2)
+{
+    double sm3 = sm2 * sm, sm4 = sm3 * sm, sm5 = sm4 * sm, v;
+    if (order == 3) {
+        v = c[0] + c[1] * sm + c[2] * sm2 + c[3] * sm3 + c[4] * sm4 + c[5] * sm5;
+    } else {
+        double sm6 = sm5 * sm, sm7 = sm6 * sm;
+        v = c[0] + c[1] * sm + c[2] * sm2 + c[3] * sm3 + c[4] * sm4 + c[5] * sm5 + c[6] * sm6 +
+            c[7] * sm7;
+    }
+    if (v > 1.0) v = 1.0;
+    else if (v < 0.0) v = 0.0;
+    return v;
+}
+
+/* fillcoCoefSpline (vpmg.c:4631-4854), order 2: multiplicative cubic window
+ * per atom; order 3 / 4: fillcoCoefSpline3 (vpmg.c:7525-7900) /
+ * fillcoCoefSpline4 (vpmg.c:7052-7372), the same loops with polynomial
+ * windows of the distance.  spl3's kappa window keeps the reference's
+ * coefficients: the dielectric ones divided again by the ion denominator
+ * (vpmg.c:7632-7643). */
+static void fill_spline(orc_pmg *m, int order)
 {
     int nx = m->nx, ny = m->ny, nz = m->nz;
     double win = m->p.swin;
@@ -368,6 +423,17 @@ static void fill_spline(orc_pmg *m)
         double arad = m->arad[a];
         if (off_mesh_strict(m, ap)) continue;
         if (!(arad > O_VPMGSMALL)) continue;
+        double cd[8], ci[8];
+        if (order > 2) {
+            win_coef(order, arad - win, arad + win, cd);
+            if (order == 3) {
+                double b = irad + arad - win, e = irad + arad + win;
+                double denom = pow((e - b), 5.0);
+                for (int n = 0; n < 8; n++) ci[n] = n < 6 ? cd[n] / denom : 0.0;
+            } else {
+                win_coef(order, irad + arad - win, irad + arad + win, ci);
+            }
+        }
         double pos[3] = {ap[0] - m->xmin, ap[1] - m->ymin, ap[2] - m->zmin};
         double itot = irad + arad + win, itot2 = OSQR(itot);
         double ictot = OMAX(0, (irad + arad - win)), ictot2 = OSQR(ictot);
@@ -392,8 +458,12 @@ static void fill_spline(orc_pmg *m)
                         double d2 = dz2 + dy2 + dx2;
                         if (d2 <= ictot2) m->kappa[p] = 0.0;
                         if ((d2 < itot2) && (d2 > ictot2)) {
-                            double dist = sqrt(d2), sm = dist - (arad + irad) + win, sm2 = OSQR(sm);
-                            m->kappa[p] *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+                            if (order == 2) {
+                                double dist = sqrt(d2), sm = dist - (arad + irad) + win, sm2 = OSQR(sm);
+                                m->kappa[p] *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+                            } else {
+                                m->kappa[p] *= win_chi(order, ci, sqrt(d2), d2);
+                            }
                         }
                     }
                     if (m->epsx[p] > O_VPMGSMALL) {
@@ -401,8 +471,13 @@ static void fill_spline(orc_pmg *m)
                         if (d2 >= stot2) m->epsx[p] *= 1.0;
                         if (d2 <= sctot2) m->epsx[p] = 0.0;
                         if ((d2 > sctot2) && (d2 < stot2)) {
-                            double dist = sqrt(d2), sm = dist - arad + win, sm2 = OSQR(sm);
-                            m->epsx[p] *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+                            if (order == 2) {
+                                double dist = sqrt(d2), sm = dist - arad + win, sm2 = OSQR(sm);
+                                m->epsx[p] *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+                            } else {
+                                double sm = sqrt(d2);
+                                m->epsx[p] *= win_chi(order, cd, sm, OSQR(sm));
+                            }
                         }
                     }
                     if (m->epsy[p] > O_VPMGSMALL) {
@@ -410,8 +485,13 @@ static void fill_spline(orc_pmg *m)
                         if (d2 >= stot2) m->epsy[p] *= 1.0;
                         if (d2 <= sctot2) m->epsy[p] = 0.0;
                         if ((d2 > sctot2) && (d2 < stot2)) {
-                            double dist = sqrt(d2), sm = dist - arad + win, sm2 = OSQR(sm);
-                            m->epsy[p] *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+                            if (order == 2) {
+                                double dist = sqrt(d2), sm = dist - arad + win, sm2 = OSQR(sm);
+                                m->epsy[p] *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+                            } else {
+                                double sm = sqrt(d2);
+                                m->epsy[p] *= win_chi(order, cd, sm, OSQR(sm));
+                            }
                         }
                     }
                     if (m->epsz[p] > O_VPMGSMALL) {
@@ -419,8 +499,13 @@ static void fill_spline(orc_pmg *m)
                         if (d2 >= stot2) m->epsz[p] *= 1.0;
                         if (d2 <= sctot2) m->epsz[p] = 0.0;
                         if ((d2 > sctot2) && (d2 < stot2)) {
-                            double dist = sqrt(d2), sm = dist - arad + win, sm2 = OSQR(sm);
-                            m->epsz[p] *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+                            if (order == 2) {
+                                double dist = sqrt(d2), sm = dist - arad + win, sm2 = OSQR(sm);
+                                m->epsz[p] *= 0.75 * sm2 * w2i - 0.25 * sm * sm2 * w3i;
+                            } else {
+                                double sm = sqrt(d2);
+                                m->epsz[p] *= win_chi(order, cd, sm, d2);
+                            }
                         }
                     }
                 }
@@ -485,6 +570,104 @@ static void fill_charge_spl2(orc_pmg *m)
                 for (int kk = km2; kk <= kp2; kk++) {
                     double mz = bspline2(VFCHI(kk, fk));
                     m->charge[IJK(ii, jj, kk)] += (charge * mx * my * mz);
+                }
+            }
+        }
+    }
+}
+
+/* bspline4 / dbspline4 / d2bspline4 (vpmg.c:6731-6822): the 5th-order
+ * B-spline of chgm spl4 and its derivatives, at VFCHI4(i, f) = 2.5 + (i - f)
+ * (vpmg.h:939). */
+static double bspline4(double x)
+{
+    double m, m2;
+    double one6 = 1.0 / 6.0, one8 = 1.0 / 8.0, one24 = 1.0 / 24.0;
+    double thirteen24 = 13.0 / 24.0, fourtyseven24 = 47.0 / 24.0, seventeen24 = 17.0 / 24.0;
+    if ((x > 0.0) && (x <= 1.0)) { m = x * x; return one24 * m * m; }
+    if ((x > 1.0) && (x <= 2.0)) {
+        m = x - 1.0; m2 = m * m;
+        return -one8 + one6 * x + m2 * (0.25 + one6 * m - one6 * m2);
+    }
+    if ((x > 2.0) && (x <= 3.0)) {
+        m = x - 2.0; m2 = m * m;
+        return -thirteen24 + 0.5 * x + m2 * (-0.25 - 0.5 * m + 0.25 * m2);
+    }
+    if ((x > 3.0) && (x <= 4.0)) {
+        m = x - 3.0; m2 = m * m;
+        return fourtyseven24 - 0.5 * x + m2 * (-0.25 + 0.5 * m - one6 * m2);
+    }
+    if ((x > 4.0) && (x <= 5.0)) {
+        m = x - 4.0; m2 = m * m;
+        return seventeen24 - one6 * x + m2 * (0.25 - one6 * m + one24 * m2);
+    }
+    return 0.0;
+}
+
+static double dbspline4(double x)
+{
+    double m, m2;
+    double one6 = 1.0 / 6.0, one3 = 1.0 / 3.0, two3 = 2.0 / 3.0, thirteen6 = 13.0 / 6.0;
+    if ((x > 0.0) && (x <= 1.0)) { m2 = x * x; return one6 * x * m2; }
+    if ((x > 1.0) && (x <= 2.0)) { m = x - 1.0; m2 = m * m; return -one3 + 0.5 * x + m2 * (0.5 - two3 * m); }
+    if ((x > 2.0) && (x <= 3.0)) { m = x - 2.0; m2 = m * m; return 1.5 - 0.5 * x + m2 * (-1.5 + m); }
+    if ((x > 3.0) && (x <= 4.0)) { m = x - 3.0; m2 = m * m; return 1.0 - 0.5 * x + m2 * (1.5 - two3 * m); }
+    if ((x > 4.0) && (x <= 5.0)) { m = x - 4.0; m2 = m * m; return -thirteen6 + 0.5 * x + m2 * (-0.5 + one6 * m); }
+    return 0.0;
+}
+
+static double d2bspline4(double x)
+{
+    double m, m2;
+    if ((x > 0.0) && (x <= 1.0)) return 0.5 * x * x;
+    if ((x > 1.0) && (x <= 2.0)) { m = x - 1.0; m2 = m * m; return -0.5 + x - 2.0 * m2; }
+    if ((x > 2.0) && (x <= 3.0)) { m = x - 2.0; m2 = m * m; return 5.5 - 3.0 * x + 3.0 * m2; }
+    if ((x > 3.0) && (x <= 4.0)) { m = x - 3.0; m2 = m * m; return -9.5 + 3.0 * x - 2.0 * m2; }
+    if ((x > 4.0) && (x <= 5.0)) { m = x - 4.0; m2 = m * m; return 4.5 - x + 0.5 * m2; }
+    return 0.0;
+}
+
+#define VFCHI4(iint, iflt) (2.5 + ((double)(iint) - (iflt)))
+
+/* chgm spl4: fillcoPermanentMultipole (vpmg.c:6835-7050) with the atom
+ * charge only; the dipole and quadrupole terms are zero and stay in the sum */
+static void fill_charge_spl4(orc_pmg *m)
+{
+    int nx = m->nx, ny = m->ny, nz = m->nz;
+    double hx = m->hx, hy = m->hy, hzed = m->hz;
+    double f = m->zmagic / (hx * hy * hzed);
+    for (size_t i = 0; i < m->n; i++) m->charge[i] = 0.0;
+    for (int a = 0; a < m->natoms; a++) {
+        const double *ap = m->apos + 3 * a;
+        double c = m->achg[a] * f;
+        double ux = 0.0, uy = 0.0, uz = 0.0, qxx = 0.0, qyx = 0.0, qyy = 0.0, qzx = 0.0,
+               qzy = 0.0, qzz = 0.0;
+        if ((ap[0] <= (m->xmin - 2 * hx)) || (ap[0] >= (m->xmax + 2 * hx)) ||
+            (ap[1] <= (m->ymin - 2 * hy)) || (ap[1] >= (m->ymax + 2 * hy)) ||
+            (ap[2] <= (m->zmin - 2 * hzed)) || (ap[2] >= (m->zmax + 2 * hzed)))
+            continue;
+        double pos[3] = {ap[0] - m->xmin, ap[1] - m->ymin, ap[2] - m->zmin};
+        double fi = pos[0] / hx, fj = pos[1] / hy, fk = pos[2] / hzed;
+        int ip1 = (int)ceil(fi), ip2 = ip1 + 2, im1 = (int)floor(fi), im2 = im1 - 2;
+        int jp1 = (int)ceil(fj), jp2 = jp1 + 2, jm1 = (int)floor(fj), jm2 = jm1 - 2;
+        int kp1 = (int)ceil(fk), kp2 = kp1 + 2, km1 = (int)floor(fk), km2 = km1 - 2;
+        ip2 = OMIN(ip2, nx - 1); im2 = OMAX(im2, 0);
+        jp2 = OMIN(jp2, ny - 1); jm2 = OMAX(jm2, 0);
+        kp2 = OMIN(kp2, nz - 1); km2 = OMAX(km2, 0);
+        for (int ii = im2; ii <= ip2; ii++) {
+            double mi = VFCHI4(ii, fi);
+            double mx = bspline4(mi), dmx = dbspline4(mi), d2mx = d2bspline4(mi);
+            for (int jj = jm2; jj <= jp2; jj++) {
+                double mj = VFCHI4(jj, fj);
+                double my = bspline4(mj), dmy = dbspline4(mj), d2my = d2bspline4(mj);
+                for (int kk = km2; kk <= kp2; kk++) {
+                    double mk = VFCHI4(kk, fk);
+                    double mz = bspline4(mk), dmz = dbspline4(mk), d2mz = d2bspline4(mk);
+                    double q = mx * my * mz * c - dmx * my * mz * ux - mx * dmy * mz * uy -
+                               mx * my * dmz * uz + d2mx * my * mz * qxx + dmx * dmy * mz * qyx +
+                               mx * d2my * mz * qyy + dmx * my * dmz * qzx + mx * dmy * dmz * qzy +
+                               mx * my * d2mz * qzz;
+                    m->charge[IJK(ii, jj, kk)] += q;
                 }
             }
         }
@@ -881,9 +1064,10 @@ int orc_fillco(orc_pmg *m)
     for (int i = 0; i < m->ny; i++) m->yf[i] = m->ymin + i * m->hy;
     for (int i = 0; i < m->nz; i++) m->zf[i] = m->zmin + i * m->hz;
     if (m->p.chgm == ORC_CHGM_SPL2) fill_charge_spl2(m);
+    else if (m->p.chgm == ORC_CHGM_SPL4) fill_charge_spl4(m);
     else fill_charge_spl0(m);
     if (!islap) {
-        if (m->p.srfm == ORC_SRFM_SPL2) fill_spline(m);
+        if (m->p.srfm >= ORC_SRFM_SPL2) fill_spline(m, m->p.srfm);
         else {
             fill_mol_diel(m);
             if (m->p.srfm == ORC_SRFM_SMOL) fill_mol_smooth(m);
@@ -1062,6 +1246,83 @@ static void spline_grad(const double *center, double win, double infrad,
     }
 }
 
+/* Vacc_splineAccGradAtomNorm3 / Norm4 (vacc.c:1097-1190, 1004-1095): the
+ * polynomial windows' normalized gradient, coefficients computed per call
+ * from (radius + infrad) +- win as the reference does */
+static void poly_grad(int order, const double *center, double win, double infrad,
+                      const double *apos, double aradius, double *grad)
+{
+    grad[0] = grad[1] = grad[2] = 0.0;
+    if (aradius > 0.0) {
+        double arad = aradius + infrad, c[8];
+        win_coef(order, arad - win, arad + win, c);
+        double dist = sqrt(OSQR(apos[0] - center[0]) + OSQR(apos[1] - center[1]) + OSQR(apos[2] - center[2]));
+        if (dist < (arad - win)) return;
+        else if (dist > (arad + win)) return;
+        else if ((fabs(dist - (arad - win)) < O_VSMALL) || (fabs(dist - (arad + win)) < O_VSMALL)) return;
+        double sm = dist, sm2 = sm * sm, sm3 = sm2 * sm, sm4 = sm3 * sm, sm5 = sm4 * sm;
+        double mychi, mygrad;
+        if (order == 3) {
+            mychi = c[0] + c[1] * sm + c[2] * sm2 + c[3] * sm3 + c[4] * sm4 + c[5] * sm5;
+            mygrad = c[1] + 2.0 * c[2] * sm + 3.0 * c[3] * sm2 + 4.0 * c[4] * sm3 + 5.0 * c[5] * sm4;
+        } else {
+            double sm6 = sm5 * sm, sm7 = sm6 * sm;
+            mychi = c[0] + c[1] * sm + c[2] * sm2 + c[3] * sm3 + c[4] * sm4 + c[5] * sm5 +
+                    c[6] * sm6 + c[7] * sm7;
+            mygrad = c[1] + 2.0 * c[2] * sm + 3.0 * c[3] * sm2 + 4.0 * c[4] * sm3 +
+                     5.0 * c[5] * sm4 + 6.0 * c[6] * sm5 + 7.0 * c[7] * sm6;
+        }
+        if (mychi <= 0.0) return;
+        else if (mychi > 1.0) mychi = 1.0;
+        for (int i = 0; i < 3; i++) grad[i] = -(mygrad / mychi) * ((center[i] - apos[i]) / dist);
+    }
+}
+
+/* Vpmg_splineSelect (vpmg.c:1890-1910) */
+static void win_grad(int srfm, const double *center, double win, double infrad,
+                     const double *apos, double aradius, double *grad)
+{
+    if (srfm == ORC_SRFM_SPL3 || srfm == ORC_SRFM_SPL4)
+        poly_grad(srfm, center, win, infrad, apos, aradius, grad);
+    else
+        spline_grad(center, win, infrad, apos, aradius, grad);
+}
+
+static void qf_force_spl4(orc_pmg *m, int a, double *force) /* vpmg.c:6156-6275 */
+{
+    int nx = m->nx, ny = m->ny, nz = m->nz;
+    double hx = m->hx, hy = m->hy, hzed = m->hz;
+    const double *ap = m->apos + 3 * a;
+    double e[3] = {0.0, 0.0, 0.0};
+    force[0] = force[1] = force[2] = 0.0;
+    if (m->partID[a] == 0) return;
+    double c = m->achg[a];
+    if ((ap[0] <= (m->xmin + 2 * hx)) || (ap[0] >= (m->xmax - 2 * hx)) ||
+        (ap[1] <= (m->ymin + 2 * hy)) || (ap[1] >= (m->ymax - 2 * hy)) ||
+        (ap[2] <= (m->zmin + 2 * hzed)) || (ap[2] >= (m->zmax - 2 * hzed)))
+        return;
+    double fi = (ap[0] - m->xmin) / hx, fj = (ap[1] - m->ymin) / hy, fk = (ap[2] - m->zmin) / hzed;
+    int ip2 = OMIN((int)ceil(fi) + 2, nx - 1), im2 = OMAX((int)floor(fi) - 2, 0);
+    int jp2 = OMIN((int)ceil(fj) + 2, ny - 1), jm2 = OMAX((int)floor(fj) - 2, 0);
+    int kp2 = OMIN((int)ceil(fk) + 2, nz - 1), km2 = OMAX((int)floor(fk) - 2, 0);
+    for (int ii = im2; ii <= ip2; ii++) {
+        double mi = VFCHI4(ii, fi), mx = bspline4(mi), dmx = dbspline4(mi);
+        for (int jj = jm2; jj <= jp2; jj++) {
+            double mj = VFCHI4(jj, fj), my = bspline4(mj), dmy = dbspline4(mj);
+            for (int kk = km2; kk <= kp2; kk++) {
+                double mk = VFCHI4(kk, fk), mz = bspline4(mk), dmz = dbspline4(mk);
+                double f = m->u[IJK(ii, jj, kk)];
+                e[0] += f * dmx * my * mz / hx;
+                e[1] += f * mx * dmy * mz / hy;
+                e[2] += f * mx * my * dmz / hzed;
+            }
+        }
+    }
+    force[0] = e[0] * c;
+    force[1] = e[1] * c;
+    force[2] = e[2] * c;
+}
+
 static void qf_force_spl2(orc_pmg *m, int a, double *f) /* vpmg.c:6043-6154 */
 {
     int nx = m->nx, ny = m->ny, nz = m->nz;
@@ -1136,7 +1397,7 @@ static int ib_force(orc_pmg *m, int a, double *force) /* vpmg.c:5440-5603 */
     const double *ap = m->apos + 3 * a;
     double arad = m->arad[a];
     force[0] = force[1] = force[2] = 0.0;
-    if (m->p.srfm != ORC_SRFM_SPL2) return 0;
+    if (m->p.srfm < ORC_SRFM_SPL2) return 0;   /* spline surfaces only (vpmg.c:5471, 5638) */
     if (m->partID[a] == 0) return 1;
     double irad = m->maxIonRadius, zkappa2 = m->zkappa2, izmagic = 1.0 / m->zmagic;
     if (zkappa2 < O_VPMGSMALL) return 1;
@@ -1161,7 +1422,7 @@ static int ib_force(orc_pmg *m, int a, double *force) /* vpmg.c:5440-5603 */
                 if ((dz2 + dy2 + dx2) <= rtot2) {
                     double gpos[3] = {i * hx + m->xmin, j * hy + m->ymin, k * hzed + m->zmin};
                     double tg[3];
-                    spline_grad(gpos, m->p.swin, irad, ap, arad, tg);
+                    win_grad(m->p.srfm, gpos, m->p.swin, irad, ap, arad, tg);
                     double fmag;
                     if (m->p.pbetype == ORC_NPBE) {
                         fmag = 0.0;
@@ -1191,7 +1452,7 @@ static int db_force(orc_pmg *m, int a, double *dbF) /* vpmg.c:5605-5860 */
     double arad = m->arad[a], srad = m->solventRadius;
     dbF[0] = dbF[1] = dbF[2] = 0.0;
     if (!m->filled) return 0;
-    if (m->p.srfm != ORC_SRFM_SPL2) return 0;
+    if (m->p.srfm < ORC_SRFM_SPL2) return 0;   /* spline surfaces only (vpmg.c:5471, 5638) */
     if (m->partID[a] == 0) return 1;
     double epsp = m->p.pdie, epsw = m->p.sdie, izmagic = 1.0 / m->zmagic;
     const double *u = m->u;
@@ -1215,27 +1476,27 @@ static int db_force(orc_pmg *m, int a, double *dbF) /* vpmg.c:5605-5860 */
                 double g[3], dHx[3], dHy[3], dHz[3], dHxm[3], dHym[3], dHzm[3];
                 g[0] = (i + 0.5) * hx + m->xmin; g[1] = j * hy + m->ymin; g[2] = k * hzed + m->zmin;
                 double Hx = (m->epsx[IJK(i, j, k)] - epsp) * depsi;
-                spline_grad(g, win, 0., ap, arad, dHx);
+                win_grad(m->p.srfm, g, win, 0., ap, arad, dHx);
                 for (int l = 0; l < 3; l++) dHx[l] *= Hx;
                 g[0] = i * hx + m->xmin; g[1] = (j + 0.5) * hy + m->ymin; g[2] = k * hzed + m->zmin;
                 double Hy = (m->epsy[IJK(i, j, k)] - epsp) * depsi;
-                spline_grad(g, win, 0., ap, arad, dHy);
+                win_grad(m->p.srfm, g, win, 0., ap, arad, dHy);
                 for (int l = 0; l < 3; l++) dHy[l] *= Hy;
                 g[0] = i * hx + m->xmin; g[1] = j * hy + m->ymin; g[2] = (k + 0.5) * hzed + m->zmin;
                 double Hz = (m->epsz[IJK(i, j, k)] - epsp) * depsi;
-                spline_grad(g, win, 0., ap, arad, dHz);
+                win_grad(m->p.srfm, g, win, 0., ap, arad, dHz);
                 for (int l = 0; l < 3; l++) dHz[l] *= Hz;
                 g[0] = (i - 0.5) * hx + m->xmin; g[1] = j * hy + m->ymin; g[2] = k * hzed + m->zmin;
                 double Hxm = (m->epsx[IJK(i - 1, j, k)] - epsp) * depsi;
-                spline_grad(g, win, 0., ap, arad, dHxm);
+                win_grad(m->p.srfm, g, win, 0., ap, arad, dHxm);
                 for (int l = 0; l < 3; l++) dHxm[l] *= Hxm;
                 g[0] = i * hx + m->xmin; g[1] = (j - 0.5) * hy + m->ymin; g[2] = k * hzed + m->zmin;
                 double Hym = (m->epsy[IJK(i, j - 1, k)] - epsp) * depsi;
-                spline_grad(g, win, 0., ap, arad, dHym);
+                win_grad(m->p.srfm, g, win, 0., ap, arad, dHym);
                 for (int l = 0; l < 3; l++) dHym[l] *= Hym;
                 g[0] = i * hx + m->xmin; g[1] = j * hy + m->ymin; g[2] = (k - 0.5) * hzed + m->zmin;
                 double Hzm = (m->epsz[IJK(i, j, k - 1)] - epsp) * depsi;
-                spline_grad(g, win, 0., ap, arad, dHzm);
+                win_grad(m->p.srfm, g, win, 0., ap, arad, dHzm);
                 for (int l = 0; l < 3; l++) dHzm[l] *= Hzm;
                 double fmag = u[IJK(i, j, k)];
                 double tg[3];
@@ -1257,6 +1518,7 @@ static int db_force(orc_pmg *m, int a, double *dbF) /* vpmg.c:5605-5860 */
 int orc_force(orc_pmg *m, int atom, double qf[3], double ib[3], double db[3])
 {
     if (m->p.chgm == ORC_CHGM_SPL2) qf_force_spl2(m, atom, qf);
+    else if (m->p.chgm == ORC_CHGM_SPL4) qf_force_spl4(m, atom, qf);
     else qf_force_spl0(m, atom, qf);
     int r1 = ib_force(m, atom, ib);
     int r2 = db_force(m, atom, db);

@@ -21,8 +21,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 ORC_SO = os.path.join(ROOT, "oracle", "liborc.so")
 
-SRFM = {"mol": 0, "smol": 1, "spl2": 2}
-CHGM = {"spl0": 0, "spl2": 1}
+SRFM = {"mol": 0, "smol": 1, "spl2": 2, "spl3": 3, "spl4": 4}
+CHGM = {"spl0": 0, "spl2": 1, "spl4": 2}
 BCFL = {"zero": 0, "sdh": 1, "mdh": 2, "focus": 4}
 
 # src/generic/vunit.h:75-80

@@ -108,7 +108,7 @@ def test_print_ids_digits_before_names():
     ("read mol pqr a.pqr end elec mg-manual dime 9 9 9 glen 5 5 5 gcent mol 1 mol 1 lpbe "
      "bcfl map pdie 1 sdie 1 chgm spl2 srfm spl2 swin 0.3 temp 300 end quit", "bcfl map"),
     ("read mol pqr a.pqr end elec mg-manual dime 9 9 9 glen 5 5 5 gcent mol 1 mol 1 lpbe "
-     "bcfl mdh pdie 1 sdie 1 chgm spl4 srfm spl2 swin 0.3 temp 300 end quit", "chgm spl4"),
+     "bcfl mdh pdie 1 sdie 1 chgm spl6 srfm spl2 swin 0.3 temp 300 end quit", "chgm spl6"),
     ("read mol pqr a.pqr end elec mg-manual dime 9 9 9 glen 5 5 5 gcent mol 1 mol 1 lpbe "
      "bcfl mdh pdie 1 sdie 1 chgm spl2 srfm spl2 swin 0.3 temp 300 usemap diel 1 end quit",
      "usemap"),
