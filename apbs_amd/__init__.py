@@ -19,9 +19,9 @@ LIB_PATH = os.path.join(HERE, "libapbs_pmg.so")
 VSOL_Newton, VSOL_MG = 1, 2
 VSM_MOL, VSM_MOLSMOOTH, VSM_SPLINE, VSM_SPLINE3, VSM_SPLINE4 = 0, 1, 2, 3, 4
 VCM_TRIL, VCM_BSPL2, VCM_BSPL4 = 0, 1, 2
-BCFL_ZERO, BCFL_SDH, BCFL_MDH, BCFL_FOCUS = 0, 1, 2, 4
+BCFL_ZERO, BCFL_SDH, BCFL_MDH, BCFL_FOCUS, BCFL_MAP = 0, 1, 2, 4, 6
 NONLIN_LPBE, NONLIN_NPBE = 0, 1
-MCT_MANUAL, MCT_AUTO, MCT_PARALLEL = 0, 1, 2
+MCT_MANUAL, MCT_AUTO, MCT_PARALLEL, MCT_DUMMY = 0, 1, 2, 3
 PCE_NO, PCE_TOTAL = 0, 1
 VDT_CHARGE, VDT_POT, VDT_DIELX, VDT_DIELY, VDT_DIELZ, VDT_KAPPA = 0, 1, 11, 12, 13, 14
 
@@ -102,6 +102,18 @@ class Vpmgp(ctypes.Structure):
     ]
 
 
+class Vgrid(ctypes.Structure):
+    """struct sVgrid (src/mg/vgrid.h:85-104); data x fastest."""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("hx", ctypes.c_double), ("hy", ctypes.c_double), ("hzed", ctypes.c_double),
+        ("xmin", ctypes.c_double), ("ymin", ctypes.c_double), ("zmin", ctypes.c_double),
+        ("xmax", ctypes.c_double), ("ymax", ctypes.c_double), ("zmax", ctypes.c_double),
+        ("data", _dp), ("readdata", ctypes.c_int), ("ctordata", ctypes.c_int),
+        ("mem", ctypes.c_void_p),
+    ]
+
+
 class Vpmg(ctypes.Structure):
     _fields_ = [
         ("vmem", ctypes.c_void_p), ("pmgp", ctypes.POINTER(Vpmgp)), ("pbe", ctypes.c_void_p),
@@ -114,12 +126,12 @@ class Vpmg(ctypes.Structure):
         ("extQfEnergy", ctypes.c_double), ("extNpEnergy", ctypes.c_double),
         ("surfMeth", ctypes.c_int), ("splineWin", ctypes.c_double),
         ("chargeMeth", ctypes.c_int), ("chargeSrc", ctypes.c_int), ("filled", ctypes.c_int),
-        ("useDielXMap", ctypes.c_int), ("dielXMap", ctypes.c_void_p),
-        ("useDielYMap", ctypes.c_int), ("dielYMap", ctypes.c_void_p),
-        ("useDielZMap", ctypes.c_int), ("dielZMap", ctypes.c_void_p),
-        ("useKappaMap", ctypes.c_int), ("kappaMap", ctypes.c_void_p),
-        ("usePotMap", ctypes.c_int), ("potMap", ctypes.c_void_p),
-        ("useChargeMap", ctypes.c_int), ("chargeMap", ctypes.c_void_p),
+        ("useDielXMap", ctypes.c_int), ("dielXMap", ctypes.POINTER(Vgrid)),
+        ("useDielYMap", ctypes.c_int), ("dielYMap", ctypes.POINTER(Vgrid)),
+        ("useDielZMap", ctypes.c_int), ("dielZMap", ctypes.POINTER(Vgrid)),
+        ("useKappaMap", ctypes.c_int), ("kappaMap", ctypes.POINTER(Vgrid)),
+        ("usePotMap", ctypes.c_int), ("potMap", ctypes.POINTER(Vgrid)),
+        ("useChargeMap", ctypes.c_int), ("chargeMap", ctypes.POINTER(Vgrid)),
         ("dev", ctypes.c_void_p),
     ]
 
@@ -189,6 +201,10 @@ _SIGS = [
      [ctypes.POINTER(MGparm), ctypes.POINTER(ctypes.POINTER(MGparm)), ctypes.c_int]),
     ("MGparm_setupPara", ctypes.c_int,
      [ctypes.POINTER(MGparm), ctypes.c_int, ctypes.POINTER(ctypes.POINTER(MGparm)), ctypes.c_int]),
+    ("Vgrid_ctor", ctypes.POINTER(Vgrid),
+     [ctypes.c_int] * 3 + [ctypes.c_double] * 6 + [_dp]),
+    ("Vgrid_dtor", None, [ctypes.POINTER(ctypes.POINTER(Vgrid))]),
+    ("Vgrid_value", ctypes.c_int, [ctypes.POINTER(Vgrid), _dp, _dp]),
     ("Vpmgp_ctor", ctypes.POINTER(Vpmgp), [ctypes.POINTER(MGparm)]),
     ("Vpmgp_dtor", None, [ctypes.POINTER(ctypes.POINTER(Vpmgp))]),
     ("Vpmgp_size", None, [ctypes.POINTER(Vpmgp)]),
@@ -205,7 +221,7 @@ _SIGS = [
     ("Vpmg_dtor2", None, [ctypes.POINTER(Vpmg)]),
     ("Vpmg_fillco", ctypes.c_int,
      [ctypes.POINTER(Vpmg), ctypes.c_int, ctypes.c_double, ctypes.c_int] +
-     [ctypes.c_int, ctypes.c_void_p] * 6),
+     [ctypes.c_int, ctypes.POINTER(Vgrid)] * 6),
     ("Vpmg_solve", ctypes.c_int, [ctypes.POINTER(Vpmg)]),
     ("Vpmg_energy", ctypes.c_double, [ctypes.POINTER(Vpmg), ctypes.c_int]),
     ("Vpmg_qfEnergy", ctypes.c_double, [ctypes.POINTER(Vpmg), ctypes.c_int]),

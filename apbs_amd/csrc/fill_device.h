@@ -99,7 +99,64 @@ size_t bc_mdh_scratch(const Geo& g, int natoms);
 void bc_mdh(hipStream_t s, const Geo& g, const double* xf, const double* yf, const double* zf,
             const AtomsDev& A, double pre1, double xkappa, double* part, double* gx, double* gy,
             double* gz);
-// focusFillBound (vpmg.c:1914-2414): trilinear values of the old solution.
+// --------------------------------------------------------------- maps
+// An external map on the device or the host (the fields of a Vgrid).
+struct MapDev {
+  int nx, ny, nz;
+  double hx, hy, hz, xmin, ymin, zmin, xmax, ymax, zmax;
+  const double* data;
+};
+// Vgrid_value (vgrid.c:179-260): trilinear value at (x, y, z), false off the
+// map.  Coordinates within 1e-4 of a face (Vcompare = 10^-(VGRID_DIGITS-2))
+// snap their index to it.  A point whose lower index falls below the map is
+// off it here; the reference reads outside its array there.
+__host__ __device__ inline bool map_value(const MapDev& m, double x, double y, double z,
+                                          double* v) {
+  const double kVcompare = 1.0e-4;
+  const double ifloat = (x - m.xmin) / m.hx, jfloat = (y - m.ymin) / m.hy,
+               kfloat = (z - m.zmin) / m.hz;
+  long ihi = (long)ceil(ifloat), jhi = (long)ceil(jfloat), khi = (long)ceil(kfloat);
+  long ilo = (long)floor(ifloat), jlo = (long)floor(jfloat), klo = (long)floor(kfloat);
+  if (fabs(x - m.xmin) < kVcompare) ilo = 0;
+  if (fabs(y - m.ymin) < kVcompare) jlo = 0;
+  if (fabs(z - m.zmin) < kVcompare) klo = 0;
+  if (fabs(x - m.xmax) < kVcompare) ihi = m.nx - 1;
+  if (fabs(y - m.ymax) < kVcompare) jhi = m.ny - 1;
+  if (fabs(z - m.zmax) < kVcompare) khi = m.nz - 1;
+  if (!(ilo >= 0 && jlo >= 0 && klo >= 0 && ihi >= 0 && jhi >= 0 && khi >= 0 && ilo < m.nx &&
+        jlo < m.ny && klo < m.nz && ihi < m.nx && jhi < m.ny && khi < m.nz))
+    return false;
+  const double dx = ifloat - (double)(ilo), dy = jfloat - (double)(jlo),
+               dz = kfloat - (double)(klo);
+  const size_t sx = 1, sy = (size_t)m.nx, sz = (size_t)m.nx * m.ny;
+  const double* d = m.data;
+  *v = dx * dy * dz * (d[ihi * sx + jhi * sy + khi * sz]) +
+       dx * (1.0 - dy) * dz * (d[ihi * sx + jlo * sy + khi * sz]) +
+       dx * dy * (1.0 - dz) * (d[ihi * sx + jhi * sy + klo * sz]) +
+       dx * (1.0 - dy) * (1.0 - dz) * (d[ihi * sx + jlo * sy + klo * sz]) +
+       (1.0 - dx) * dy * dz * (d[ilo * sx + jhi * sy + khi * sz]) +
+       (1.0 - dx) * (1.0 - dy) * dz * (d[ilo * sx + jlo * sy + khi * sz]) +
+       (1.0 - dx) * dy * (1.0 - dz) * (d[ilo * sx + jhi * sy + klo * sz]) +
+       (1.0 - dx) * (1.0 - dy) * (1.0 - dz) * (d[ilo * sx + jlo * sy + klo * sz]);
+  return true;
+}
+// Samples of a map (data on the device) at every grid point (xf + ox,
+// yf + oy, zf + oz):
+//   mode 0: the value (fillcoCoefMap's dielectric maps, vpmg.c:4180-4210)
+//   mode 1: the raw value, with one maximum per workgroup into part (the
+//           kappa map before its scaling, vpmg.c:4124-4160); bad[1] counts
+//           negative values.  part holds map_sample_parts(g) doubles.
+//   mode 2: value * scale (fillcoChargeMap, vpmg.c:4938-4985, scale = zmagic)
+// bad[0] counts points off the map; out == nullptr only counts.
+size_t map_sample_parts(const Geo& g);
+void map_sample(hipStream_t s, const Geo& g, const MapDev& m, const double* xf, const double* yf,
+                const double* zf, double ox, double oy, double oz, int mode, double scale,
+                double* out, double* part, int* bad);
+// kappa = (raw < VPMGSMALL ? 0 : raw) / kappamax (vpmg.c:4163-4178)
+void kappa_scale(hipStream_t s, size_t n, double kappamax, double* kappa);
+
+// focusFillBound (vpmg.c:1914-2414): trilinear values of the old solution
+// (or of the potential map for bcfl map, vpmg.c:1993-2006).
 struct OldGrid {
   int nx, ny, nz;
   double hx, hy, hz, xmin, ymin, zmin, xmax, ymax, zmax;

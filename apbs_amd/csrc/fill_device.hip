@@ -1280,6 +1280,47 @@ __global__ void k_force_db(Geo g, AtomsDev A, double win, double srad, double ep
   rcs[a] = rc;
 }
 
+// ------------------------------------------------------------------ maps
+__global__ void __launch_bounds__(kB)
+    k_map_sample(Geo g, MapDev m, const double* xf, const double* yf, const double* zf, double ox,
+                 double oy, double oz, int mode, double scale, double* out, double* part,
+                 int* bad) {
+  __shared__ double smax[kB];
+  const int t = blockIdx.x * kB + threadIdx.x;
+  const int k = blockIdx.y;
+  const int i = t % g.nx, j = t / g.nx;
+  double mx = -1.0;
+  if (j < g.ny) {
+    double v = 0.0;
+    if (!map_value(m, xf[i] + ox, yf[j] + oy, zf[k] + oz, &v)) {
+      atomicAdd(bad, 1);
+    } else {
+      if (mode == 1) {
+        if (v < 0.0) atomicAdd(bad + 1, 1);
+        mx = v;
+      }
+      if (out) out[IX(i, j, k, g.nx, g.ny)] = mode == 2 ? v * scale : v;
+    }
+  }
+  if (mode == 1) {   // block maximum, one partial per block
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = kB / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = smax[0];
+  }
+}
+
+__global__ void k_kappa_scale(size_t n, double kappamax, double* kappa) {
+  const size_t i = (size_t)blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  double tk = kappa[i];
+  if (tk < VPMGSMALL) tk = 0.0;
+  kappa[i] = (tk / kappamax);
+}
+
 }  // namespace
 
 // ================================================================ wrappers
@@ -1491,6 +1532,22 @@ void spline_coefficients(int order, int natoms, const double* radius, double win
     const double ar = arad + irad;
     window_coef(order, ar - win, ar + win, o + 16);
   }
+}
+
+size_t map_sample_parts(const Geo& g) {
+  const dim3 gr = grid_all(g);
+  return (size_t)gr.x * gr.y;
+}
+
+void map_sample(hipStream_t s, const Geo& g, const MapDev& m, const double* xf, const double* yf,
+                const double* zf, double ox, double oy, double oz, int mode, double scale,
+                double* out, double* part, int* bad) {
+  k_map_sample<<<grid_all(g), kB, 0, s>>>(g, m, xf, yf, zf, ox, oy, oz, mode, scale, out, part,
+                                          bad);
+}
+
+void kappa_scale(hipStream_t s, size_t n, double kappamax, double* kappa) {
+  k_kappa_scale<<<nb(n), kB, 0, s>>>(n, kappamax, kappa);
 }
 
 }  // namespace pmg

@@ -1582,15 +1582,178 @@ static void ensure_spline_coef(Vpmg* thee, int order) {
   d->splwin = thee->splineWin;
 }
 
+// ================================================================ Vgrid
+
+// Vgrid_ctor (vgrid.c:86-147): the map borrows data; xmax = xmin + (nx-1) hx.
+extern "C" Vgrid* Vgrid_ctor(int nx, int ny, int nz, double hx, double hy, double hzed,
+                             double xmin, double ymin, double zmin, double* data) {
+  if (nx < 1 || ny < 1 || nz < 1 || !(hx > 0) || !(hy > 0) || !(hzed > 0)) return nullptr;
+  Vgrid* g = (Vgrid*)calloc(1, sizeof(Vgrid));
+  if (!g) return nullptr;
+  g->nx = nx;
+  g->ny = ny;
+  g->nz = nz;
+  g->hx = hx;
+  g->hy = hy;
+  g->hzed = hzed;
+  g->xmin = xmin;
+  g->xmax = xmin + (nx - 1) * hx;
+  g->ymin = ymin;
+  g->ymax = ymin + (ny - 1) * hy;
+  g->zmin = zmin;
+  g->zmax = zmin + (nz - 1) * hzed;
+  g->data = data;
+  g->ctordata = data != nullptr;
+  g->readdata = 0;
+  return g;
+}
+
+// Vgrid_dtor (vgrid.c:152-175): the data stays the caller's.
+extern "C" void Vgrid_dtor(Vgrid** thee) {
+  if (!thee || !*thee) return;
+  free(*thee);
+  *thee = nullptr;
+}
+
+static pmg::MapDev map_fields(const Vgrid* v, const double* data) {
+  pmg::MapDev m;
+  m.nx = v->nx;
+  m.ny = v->ny;
+  m.nz = v->nz;
+  m.hx = v->hx;
+  m.hy = v->hy;
+  m.hz = v->hzed;
+  m.xmin = v->xmin;
+  m.ymin = v->ymin;
+  m.zmin = v->zmin;
+  m.xmax = v->xmax;
+  m.ymax = v->ymax;
+  m.zmax = v->zmax;
+  m.data = data;
+  return m;
+}
+
+// Vgrid_value (vgrid.c:179-260), the arithmetic the device maps use.
+extern "C" int Vgrid_value(Vgrid* thee, double pt[3], double* value) {
+  if (!thee || !thee->data || !(thee->ctordata || thee->readdata)) return 0;
+  double v = 0.0;
+  if (!pmg::map_value(map_fields(thee, thee->data), pt[0], pt[1], pt[2], &v)) return 0;
+  *value = v;
+  return 1;
+}
+
+// A map's data copied to the device for one Vpmg_fillco.
+struct DevMap {
+  pmg::MapDev m{};
+  double* data = nullptr;
+  DevMap(PmgDev* d, const Vgrid* v) {
+    const size_t n = (size_t)v->nx * v->ny * v->nz;
+    data = dalloc<double>(n);
+    HIPCHECK(hipMemcpyAsync(data, v->data, n * sizeof(double), hipMemcpyHostToDevice, d->s));
+    m = map_fields(v, data);
+  }
+  ~DevMap() { dfree(data); }
+  DevMap(const DevMap&) = delete;
+  DevMap& operator=(const DevMap&) = delete;
+};
+
+static bool map_ok(const Vgrid* v) {
+  return v && v->data && (v->ctordata || v->readdata) && v->nx >= 1 && v->ny >= 1 &&
+         v->nz >= 1 && v->hx > 0 && v->hy > 0 && v->hzed > 0;
+}
+
+// Samples one map at the grid points shifted by (ox, oy, oz); returns the
+// number of points off the map (and of negative values in *neg, mode 1).
+static int sample_map(PmgDev* d, const pmg::Geo& g, const Vgrid* v, double ox, double oy,
+                      double oz, int mode, double scale, double* out, double* vmax, int* neg) {
+  DevMap dm(d, v);
+  const size_t np = pmg::map_sample_parts(g);
+  double* part = dalloc<double>(np);
+  int* bad = dalloc<int>(2);
+  HIPCHECK(hipMemsetAsync(bad, 0, 2 * sizeof(int), d->s));
+  pmg::map_sample(d->s, g, dm.m, d->xf, d->yf, d->zf, ox, oy, oz, mode, scale, out, part, bad);
+  int hbad[2] = {0, 0};
+  std::vector<double> hp(mode == 1 ? np : 0);
+  HIPCHECK(hipMemcpyAsync(hbad, bad, 2 * sizeof(int), hipMemcpyDeviceToHost, d->s));
+  if (mode == 1)
+    HIPCHECK(hipMemcpyAsync(hp.data(), part, np * sizeof(double), hipMemcpyDeviceToHost, d->s));
+  HIPCHECK(hipStreamSynchronize(d->s));
+  dfree(part);
+  dfree(bad);
+  if (mode == 1) {
+    double mx = -1.0;
+    for (double x : hp) mx = std::max(mx, x);
+    *vmax = mx;
+    *neg = hbad[1];
+  }
+  return hbad[0];
+}
+
+// fillcoCoefMap (vpmg.c:4098-4220): every dielectric map, and the kappa map
+// when there is salt; kappa scaled into [0, 1] by the map's maximum.
+static int fill_coef_map(Vpmg* thee, const pmg::Geo& g, double ionstr) {
+  PmgDev* d = DEV(thee);
+  if (!thee->useDielXMap || !thee->useDielYMap || !thee->useDielZMap ||
+      (!thee->useKappaMap && ionstr > VPMGSMALL)) {
+    fprintf(stderr, "fillcoCoefMap: You need to use all coefficient maps!\n");
+    return 0;
+  }
+  if (ionstr > VPMGSMALL) {
+    double kmax = -1.0;
+    int neg = 0;
+    if (sample_map(d, g, thee->kappaMap, 0, 0, 0, 1, 1.0, d->kappa, &kmax, &neg)) {
+      fprintf(stderr, "Vpmg_fillco: Off kappaMap\n");
+      return 0;
+    }
+    if (neg) {
+      fprintf(stderr, "Vpmg_fillcoCoefMap: Kappa map less than 0\n");
+      return 0;
+    }
+    if (!(kmax > 1.0)) kmax = 1.0;
+    pmg::kappa_scale(d->s, d->n, kmax, d->kappa);
+  } else {
+    pmg::fill<double>(d->s, d->kappa, d->n, 0.0);
+  }
+  const Vgrid* maps[3] = {thee->dielXMap, thee->dielYMap, thee->dielZMap};
+  double* eps[3] = {d->epsx, d->epsy, d->epsz};
+  const char* names[3] = {"dielXMap", "dielYMap", "dielZMap"};
+  for (int w = 0; w < 3; w++) {
+    const double ox = w == 0 ? 0.5 * g.hx : 0.0, oy = w == 1 ? 0.5 * g.hy : 0.0,
+                 oz = w == 2 ? 0.5 * g.hz : 0.0;
+    if (sample_map(d, g, maps[w], ox, oy, oz, 0, 1.0, eps[w], nullptr, nullptr)) {
+      fprintf(stderr, "Vpmg_fillco: Off %s\n", names[w]);
+      return 0;
+    }
+  }
+  return 1;
+}
+
+static int focus_fill_bound_grid(Vpmg* thee, const pmg::OldGrid& og);
+
 // Vpmg_fillco (vpmg.c:5250-5415)
 extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
-                           Vchrg_Meth chargeMeth, int useDielXMap, Vgrid*, int useDielYMap,
-                           Vgrid*, int useDielZMap, Vgrid*, int useKappaMap, Vgrid*,
-                           int usePotMap, Vgrid*, int useChargeMap, Vgrid*) {
+                           Vchrg_Meth chargeMeth, int useDielXMap, Vgrid* dielXMap,
+                           int useDielYMap, Vgrid* dielYMap, int useDielZMap, Vgrid* dielZMap,
+                           int useKappaMap, Vgrid* kappaMap, int usePotMap, Vgrid* potMap,
+                           int useChargeMap, Vgrid* chargeMap) {
   if (!thee) return 0;
-  if (useDielXMap || useDielYMap || useDielZMap || useKappaMap || usePotMap || useChargeMap) {
-    fprintf(stderr, "Vpmg_fillco: external maps are not supported\n");
-    return 0;
+  // vpmg.c:5299-5310
+  const struct { int use; Vgrid* map; int* flag; Vgrid** slot; const char* name; } maps[6] = {
+      {useDielXMap, dielXMap, &thee->useDielXMap, &thee->dielXMap, "dielXMap"},
+      {useDielYMap, dielYMap, &thee->useDielYMap, &thee->dielYMap, "dielYMap"},
+      {useDielZMap, dielZMap, &thee->useDielZMap, &thee->dielZMap, "dielZMap"},
+      {useKappaMap, kappaMap, &thee->useKappaMap, &thee->kappaMap, "kappaMap"},
+      {usePotMap, potMap, &thee->usePotMap, &thee->potMap, "potMap"},
+      {useChargeMap, chargeMap, &thee->useChargeMap, &thee->chargeMap, "chargeMap"}};
+  for (const auto& m : maps) {
+    if (m.use && !map_ok(m.map)) {
+      fprintf(stderr, "Vpmg_fillco: %s is missing or has no data\n", m.name);
+      return 0;
+    }
+  }
+  for (const auto& m : maps) {
+    *m.flag = m.use;
+    *m.slot = m.use ? m.map : nullptr;
   }
   const double t0 = now();
   PmgDev* d = DEV(thee);
@@ -1606,7 +1769,11 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
   p->xmax = p->xcent + (p->xlen / 2.0);
   p->ymax = p->ycent + (p->ylen / 2.0);
   p->zmax = p->zcent + (p->zlen / 2.0);
-  const int islap = (ionstr < VPMGSMALL) && (fabs(epsp - epsw) < VPMGSMALL);
+  // the operator is not a plain Laplacian when maps are read (vpmg.c:5351-5362)
+  const bool coef_maps =
+      thee->useDielXMap || thee->useDielYMap || thee->useDielZMap || thee->useKappaMap;
+  const int islap = !(coef_maps || thee->usePotMap) && (ionstr < VPMGSMALL) &&
+                    (fabs(epsp - epsw) < VPMGSMALL);
   std::vector<double> xf(d->nx), yf(d->ny), zf(d->nz);
   for (int i = 0; i < d->nx; i++) xf[i] = p->xmin + i * p->hx;
   for (int i = 0; i < d->ny; i++) yf[i] = p->ymin + i * p->hy;
@@ -1627,14 +1794,22 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
   }
   const double hmax = std::max(std::max(p->hx, p->hy), p->hzed);
   // charges: an atom reaches at most 2 grid spacings per axis (3 for spl4)
-  {
+  if (thee->useChargeMap) {
+    // fillcoChargeMap (vpmg.c:4938-4985): the map in e/A^3 times zmagic
+    if (sample_map(d, g, thee->chargeMap, 0, 0, 0, 2, pbe->zmagic, d->charge, nullptr, nullptr)) {
+      fprintf(stderr, "fillcoChargeMap: Error -- fell off of charge map\n");
+      return 0;
+    }
+  } else {
     const double reach = (chargeMeth == VCM_BSPL4 ? 3.0 : 2.0) * hmax + 1e-9;
     DevCells cells = build_cells(d->s, hpos.data(), d->natoms, reach);
     pmg::charge_map(d->s, g, A, cells.c, (int)chargeMeth, pbe->zmagic, 0, d->charge);
     HIPCHECK(hipStreamSynchronize(d->s));
     cells.release();
   }
-  if (!islap) {
+  if (!islap && coef_maps) {
+    if (!fill_coef_map(thee, g, ionstr)) return 0;
+  } else if (!islap) {
     const double irad = pbe->maxIonRadius;
     const double ionmask = (ionstr > VPMGSMALL) ? 1.0 : 0.0;
     if (surfMeth == VSM_SPLINE || surfMeth == VSM_SPLINE3 || surfMeth == VSM_SPLINE4) {
@@ -1753,8 +1928,40 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
       M.eps_p = pbe->soluteDiel;
       M.eps_w = pbe->solventDiel;
       pmg::bc_sdh(d->s, g, d->xf, d->yf, d->zf, M, d->gx, d->gy, d->gz);
+    } else if (p->bcfl == BCFL_MAP) {
+      // bcfl_map (vpmg.c:3953-3990): every grid point must lie on the
+      // potential map; then focusFillBound with the map as the old solution
+      // (vpmg.c:1993-2006, 4084-4087)
+      if (!thee->usePotMap) {
+        fprintf(stderr, "Vpmg_fillco: bcfl map needs a potential map (usemap pot)\n");
+        return 0;
+      }
+      if (sample_map(d, g, thee->potMap, 0, 0, 0, 0, 1.0, nullptr, nullptr, nullptr)) {
+        fprintf(stderr, "bcfl_map: Error -- fell off of potential map\n");
+        return 0;
+      }
+      const Vgrid* pm = thee->potMap;
+      DevMap dm(d, pm);
+      pmg::OldGrid og;
+      og.nx = pm->nx;
+      og.ny = pm->ny;
+      og.nz = pm->nz;
+      og.hx = pm->hx;
+      og.hy = pm->hy;
+      og.hz = pm->hzed;
+      og.xmin = pm->xmin;
+      og.xmax = pm->xmax;
+      og.ymin = pm->ymin;
+      og.ymax = pm->ymax;
+      og.zmin = pm->zmin;
+      og.zmax = pm->zmax;
+      og.u = dm.data;
+      if (!focus_fill_bound_grid(thee, og)) {
+        fprintf(stderr, "Vpmg::focusFillBound -- new mesh not contained in the potential map\n");
+        return 0;
+      }
     } else if (p->bcfl != BCFL_ZERO) {
-      fprintf(stderr, "Vpmg_fillco: bcfl %d is not supported (zero, sdh, mdh, focus are)\n",
+      fprintf(stderr, "Vpmg_fillco: bcfl %d is not supported (zero, sdh, mdh, map, focus are)\n",
               (int)p->bcfl);
       return 0;
     }
@@ -1822,32 +2029,18 @@ extern "C" int Vpmg_solve(Vpmg* thee) {
 
 // ================================================================ focusing
 
-// focusFillBound (vpmg.c:1914-2414): Dirichlet faces from the old solution
-static int focus_fill_bound(Vpmg* thee, Vpmg* old) {
+// focusFillBound (vpmg.c:1914-2414): Dirichlet faces from an old solution on
+// the device (a coarser Vpmg's, or the potential map's); 0 when the new mesh
+// is not contained in the old one.
+static int focus_fill_bound_grid(Vpmg* thee, const pmg::OldGrid& og) {
   PmgDev* d = DEV(thee);
-  PmgDev* o = DEV(old);
   Vpmgp* p = thee->pmgp;
-  Vpmgp* q = old->pmgp;
   double xminN = p->xcent - ((double)(p->nx - 1) * p->hx) / 2.0;
   double xmaxN = p->xcent + ((double)(p->nx - 1) * p->hx) / 2.0;
   double yminN = p->ycent - ((double)(p->ny - 1) * p->hy) / 2.0;
   double ymaxN = p->ycent + ((double)(p->ny - 1) * p->hy) / 2.0;
   double zminN = p->zcent - ((double)(p->nz - 1) * p->hzed) / 2.0;
   double zmaxN = p->zcent + ((double)(p->nz - 1) * p->hzed) / 2.0;
-  pmg::OldGrid og;
-  og.nx = q->nx;
-  og.ny = q->ny;
-  og.nz = q->nz;
-  og.hx = q->hx;
-  og.hy = q->hy;
-  og.hz = q->hzed;
-  og.xmin = q->xcent - ((double)(q->nx - 1) * q->hx) / 2.0;
-  og.xmax = q->xcent + ((double)(q->nx - 1) * q->hx) / 2.0;
-  og.ymin = q->ycent - ((double)(q->ny - 1) * q->hy) / 2.0;
-  og.ymax = q->ycent + ((double)(q->ny - 1) * q->hy) / 2.0;
-  og.zmin = q->zcent - ((double)(q->nz - 1) * q->hzed) / 2.0;
-  og.zmax = q->zcent + ((double)(q->nz - 1) * q->hzed) / 2.0;
-  og.u = o->u;
   if (fabs(og.xmin - xminN) < VSMALL) xminN = og.xmin;
   if (fabs(og.xmax - xmaxN) < VSMALL) xmaxN = og.xmax;
   if (fabs(og.ymin - yminN) < VSMALL) yminN = og.ymin;
@@ -1857,7 +2050,6 @@ static int focus_fill_bound(Vpmg* thee, Vpmg* old) {
   if ((xmaxN > og.xmax) || (ymaxN > og.ymax) || (zmaxN > og.zmax) || (og.xmin > xminN) ||
       (og.ymin > yminN) || (og.zmin > zminN))
     return 0;
-  HIPCHECK(hipStreamSynchronize(o->s));
   pmg::Geo g{};
   g.nx = p->nx;
   g.ny = p->ny;
@@ -1873,6 +2065,27 @@ static int focus_fill_bound(Vpmg* thee, Vpmg* old) {
   HIPCHECK(hipStreamSynchronize(d->s));
   dfree(bad);
   return hbad ? 0 : 1;
+}
+
+static int focus_fill_bound(Vpmg* thee, Vpmg* old) {
+  PmgDev* o = DEV(old);
+  Vpmgp* q = old->pmgp;
+  pmg::OldGrid og;
+  og.nx = q->nx;
+  og.ny = q->ny;
+  og.nz = q->nz;
+  og.hx = q->hx;
+  og.hy = q->hy;
+  og.hz = q->hzed;
+  og.xmin = q->xcent - ((double)(q->nx - 1) * q->hx) / 2.0;
+  og.xmax = q->xcent + ((double)(q->nx - 1) * q->hx) / 2.0;
+  og.ymin = q->ycent - ((double)(q->ny - 1) * q->hy) / 2.0;
+  og.ymax = q->ycent + ((double)(q->ny - 1) * q->hy) / 2.0;
+  og.zmin = q->zcent - ((double)(q->nz - 1) * q->hzed) / 2.0;
+  og.zmax = q->zcent + ((double)(q->nz - 1) * q->hzed) / 2.0;
+  og.u = o->u;
+  HIPCHECK(hipStreamSynchronize(o->s));
+  return focus_fill_bound_grid(thee, og);
 }
 
 // Vpmg_setPart (vpmg.c:627-870), atom part on the host, grid part on the
@@ -2031,7 +2244,8 @@ extern "C" double Vpmg_qfEnergy(Vpmg* thee, int extFlag) {
   PmgDev* d = DEV(thee);
   Vpmgp* p = thee->pmgp;
   double e;
-  if (thee->chargeMeth == VCM_BSPL2) {  // Vpmg_qfEnergyVolume (vpmg.c:1861-1891)
+  // Vpmg_qfEnergy (vpmg.c:1687-1702): the volume sum for a charge map or spl2
+  if (thee->useChargeMap || thee->chargeMeth == VCM_BSPL2) {  // vpmg.c:1861-1891
     pmg::energy_qf_volume(d->s, d->n, d->pvec, d->u, d->charge, d->part, d->scal);
     e = read_scalar(d, d->scal);
     e = e * p->hx * p->hy * p->hzed / thee->pbe->zmagic;

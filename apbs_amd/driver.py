@@ -10,7 +10,8 @@ import ctypes
 
 import numpy as np
 
-from . import (BCFL_FOCUS, BCFL_MDH, BCFL_SDH, BCFL_ZERO, MCT_AUTO, MCT_MANUAL, MCT_PARALLEL,
+from . import (BCFL_FOCUS, BCFL_MAP, BCFL_MDH, BCFL_SDH, BCFL_ZERO, MCT_AUTO, MCT_DUMMY,
+               MCT_MANUAL, MCT_PARALLEL,
                NONLIN_LPBE, NONLIN_NPBE,
                PCE_NO, PCE_TOTAL, VCM_BSPL2, VCM_BSPL4, VCM_TRIL, VSM_MOL, VSM_MOLSMOOTH,
                VSM_SPLINE, VSM_SPLINE3, VSM_SPLINE4, VSOL_MG, VSOL_Newton, MGparm, lib)
@@ -18,7 +19,8 @@ from . import (BCFL_FOCUS, BCFL_MDH, BCFL_SDH, BCFL_ZERO, MCT_AUTO, MCT_MANUAL, 
 SRFM = {"mol": VSM_MOL, "smol": VSM_MOLSMOOTH, "spl2": VSM_SPLINE, "spl3": VSM_SPLINE3,
         "spl4": VSM_SPLINE4}
 CHGM = {"spl0": VCM_TRIL, "spl2": VCM_BSPL2, "spl4": VCM_BSPL4}
-BCFL = {"zero": BCFL_ZERO, "sdh": BCFL_SDH, "mdh": BCFL_MDH, "focus": BCFL_FOCUS}
+BCFL = {"zero": BCFL_ZERO, "sdh": BCFL_SDH, "mdh": BCFL_MDH, "focus": BCFL_FOCUS,
+        "map": BCFL_MAP}
 
 VUNIT_KB = 1.3806581e-23    # src/generic/vunit.h:75
 VUNIT_NA = 6.0221367e+23    # src/generic/vunit.h:77
@@ -76,6 +78,43 @@ class Molecule:
             self.ptr = None
 
 
+class Map:
+    """A Vgrid (Vgrid_ctor) over data (nz, ny, nx), x fastest in memory, with
+    its first point at origin and spacing h per axis.  The array is kept
+    alive with the map."""
+
+    def __init__(self, data, origin, spacing):
+        self.data = np.ascontiguousarray(data, dtype=np.float64)
+        if self.data.ndim != 3:
+            raise ValueError("a map is a 3-d array (nz, ny, nx)")
+        nz, ny, nx = self.data.shape
+        self.origin = tuple(float(v) for v in origin)
+        self.spacing = tuple(float(v) for v in spacing)
+        self.ptr = lib().Vgrid_ctor(nx, ny, nz, *self.spacing, *self.origin, _dp(self.data))
+        if not self.ptr:
+            raise ValueError("Vgrid_ctor failed")
+
+    @classmethod
+    def from_dx(cls, path):
+        from . import dx
+        data, origin, spacing = dx.read_dx(str(path))
+        return cls(data, origin, spacing)
+
+    def value(self, pt):
+        """Vgrid_value: the trilinear value at pt, or None off the map."""
+        p = np.ascontiguousarray(pt, dtype=np.float64)
+        v = np.zeros(1)
+        return float(v[0]) if lib().Vgrid_value(self.ptr, _dp(p), _dp(v)) else None
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().Vgrid_dtor(ctypes.byref(self.ptr))
+            self.ptr = None
+
+
+MAP_KEYS = ("dielx", "diely", "dielz", "kappa", "pot", "charge")
+
+
 class Calc:
     """One Vpmg with its Vpbe / Vpmgp / MGparm (one mg-manual calculation)."""
 
@@ -125,8 +164,18 @@ class Calc:
         self.chgm = CHGM[pbe["chgm"]]
 
     def fillco(self):
+        """Vpmg_fillco; pbe["maps"] may name Map objects under MAP_KEYS
+        (the usemap statements of routines.c:1482-1561)."""
+        maps = self.pbe_opts.get("maps") or {}
+        unknown = set(maps) - set(MAP_KEYS)
+        if unknown:
+            raise ValueError(f"unknown map kinds {sorted(unknown)}")
+        args = []
+        for k in MAP_KEYS:
+            m = maps.get(k)
+            args += [0, None] if m is None else [1, m.ptr]
         ok = lib().Vpmg_fillco(self.pmg, self.srfm, self.pbe_opts.get("swin", 0.3), self.chgm,
-                               0, None, 0, None, 0, None, 0, None, 0, None, 0, None)
+                               *args)
         if not ok:
             raise RuntimeError("Vpmg_fillco failed")
 
@@ -224,9 +273,11 @@ def _new_mgparm(kind):
     return lib().MGparm_ctor(kind)
 
 
-def manual_mgparm(dime, center, grid=None, glen=None, chgm="spl2", nlev=4):
+def manual_mgparm(dime, center, grid=None, glen=None, chgm="spl2", nlev=4, dummy=False):
+    """An mg-manual MGparm (mg-dummy with dummy=True: the same keywords, no
+    nlev/dime adjustment, mgparm.c:205-270)."""
     L = lib()
-    m = _new_mgparm(MCT_MANUAL)
+    m = _new_mgparm(MCT_DUMMY if dummy else MCT_MANUAL)
     c = m.contents
     for j in range(3):
         c.dime[j] = int(dime[j])

@@ -55,7 +55,9 @@ def write_dx(path, data, origin, spacing, title="", gz=False, source="apbs-pmg-m
 def read_dx(path):
     """Read a scalar OpenDX grid (Vgrid_readDX, vgrid.c:586-808): returns
     (data (nz, ny, nx), origin, spacing)."""
-    opener = gzip.open if path.endswith(".gz") else open
+    with open(path, "rb") as fh:       # gzip-compressed DX (VDF_GZ) or plain text
+        zipped = fh.read(2) == b"\x1f\x8b"
+    opener = gzip.open if zipped else open
     with opener(path, "rt") as fh:
         text = fh.read()
     counts = origin = None

@@ -430,25 +430,42 @@ def check_runnable(inp):
     for m in inp.molecules:
         if m.fmt != "pqr":
             why.append(f"READ mol {m.fmt}: only PQR molecules are read")
+    nmaps = {}
     for kind, fmt, _ in inp.maps:
-        why.append(f"READ {kind} {fmt}: external maps are not used by this build")
+        if kind not in MAP_KINDS:
+            why.append(f"READ {kind}: finite-element meshes are not used by the PMG path")
+        elif fmt not in ("dx", "gz"):
+            why.append(f"READ {kind} {fmt}: maps are read as OpenDX (dx, gz)")
+        nmaps[kind] = nmaps.get(kind, 0) + 1
     for i, e in enumerate(inp.elecs):
         tag = e.name or str(i + 1)
-        if e.method not in ("mg-manual", "mg-auto", "mg-para"):
+        if e.method not in ("mg-manual", "mg-auto", "mg-para", "mg-dummy"):
             why.append(f"ELEC {tag}: method {e.method} is not run by this build")
             continue
         if e.unsupported:
             why.append(f"ELEC {tag}: {', '.join(e.unsupported)} not supported")
         if e.pbe not in ("lpbe", "npbe"):
             why.append(f"ELEC {tag}: {e.pbe or 'no PBE type'} (lpbe and npbe are run)")
-        if e.bcfl not in ("zero", "sdh", "mdh", "focus"):
-            why.append(f"ELEC {tag}: bcfl {e.bcfl} (zero, sdh, mdh and focus are run)")
+        if e.bcfl not in ("zero", "sdh", "mdh", "focus", "map"):
+            why.append(f"ELEC {tag}: bcfl {e.bcfl} (zero, sdh, mdh, map and focus are run)")
         if e.chgm not in ("spl0", "spl2", "spl4"):
             why.append(f"ELEC {tag}: chgm {e.chgm} (spl0, spl2 and spl4 are run)")
         if e.srfm not in ("mol", "smol", "spl2", "spl3", "spl4"):
             why.append(f"ELEC {tag}: srfm {e.srfm} (mol, smol, spl2, spl3 and spl4 are run)")
-        if e.usemaps:
-            why.append(f"ELEC {tag}: usemap is not supported")
+        used = set()
+        for kind, ident in e.usemaps:     # routines.c:1482-1561
+            if kind not in MAP_KINDS:
+                why.append(f"ELEC {tag}: usemap {kind} (diel, kappa, charge and pot are read)")
+            elif not 1 <= ident <= nmaps.get(kind, 0):
+                why.append(f"ELEC {tag}: {ident} is not a valid {kind} map ID")
+            used.add(kind)
+        if e.bcfl == "map" and "pot" not in used:
+            why.append(f"ELEC {tag}: bcfl map needs a potential map (usemap pot)")
+        salt = any(c > 0 and q != 0 for (q, c, _) in e.ions)
+        if "kappa" in used and "diel" not in used:
+            why.append(f"ELEC {tag}: a kappa map is used only with dielectric maps (usemap diel)")
+        if "diel" in used and salt and "kappa" not in used:
+            why.append(f"ELEC {tag}: dielectric maps with salt need a kappa map (usemap kappa)")
         for what, fmt, _ in e.writes:
             if fmt not in ("dx", "gz"):
                 why.append(f"ELEC {tag}: write {what} {fmt} (dx and gz output are written)")
@@ -462,9 +479,9 @@ def check_runnable(inp):
             missing.append("sdens")
         if e.srfm in ("spl2", "spl3", "spl4") and e.swin is None:
             missing.append("swin")
-        if e.method == "mg-manual" and e.grid is None and e.glen is None:
+        if e.method in ("mg-manual", "mg-dummy") and e.grid is None and e.glen is None:
             missing.append("grid or glen")
-        if e.method == "mg-manual" and e.gcent is None:
+        if e.method in ("mg-manual", "mg-dummy") and e.gcent is None:
             missing.append("gcent")
         if e.method in ("mg-auto", "mg-para"):
             missing += [k for k in ("cglen", "fglen", "cgcent", "fgcent") if getattr(e, k) is None]
@@ -496,6 +513,9 @@ def check_runnable(inp):
                     why.append(str(err))
     return why
 
+
+# map kinds READ and usemap take (NOsh_parseREAD_*, PBEparm_parseUSEMAP)
+MAP_KINDS = ("diel", "kappa", "charge", "pot")
 
 # data arrays `write` can produce (Vpmg_copyArray names)
 WRITABLE = ("pot", "charge", "dielx", "diely", "dielz", "kappa")

@@ -61,10 +61,11 @@ def _pbe_dict(e):
 def _mgparm_chain(e, centers, rank):
     """The MGparm objects of one ELEC section's calculations (for mg-para,
     those of processor `rank`)."""
-    if e.method == "mg-manual":
+    if e.method in ("mg-manual", "mg-dummy"):
         c = e.gcent.resolve(centers)
         return [driver.manual_mgparm(e.dime, c, grid=e.grid, glen=e.glen if e.grid is None else None,
-                                     chgm=e.chgm, nlev=e.nlev or 4)]
+                                     chgm=e.chgm, nlev=e.nlev or 4,
+                                     dummy=e.method == "mg-dummy")]
     if e.method == "mg-para":
         return driver.para_chain(e.dime, e.cglen, e.fglen, e.cgcent.resolve(centers),
                                  e.fgcent.resolve(centers), e.pdime, e.ofrac, rank, e.chgm)
@@ -103,6 +104,7 @@ class Runner:
         self.ranks = [0]
         self.per_rank = []
         self.mols = []
+        self.maps = {k: [] for k in nosh.MAP_KINDS}
 
     def p(self, s=""):
         self.out.write(s + "\n")
@@ -115,6 +117,7 @@ class Runner:
             mol = driver.Molecule.from_pqr(self.inp.path(m.path))
             self.p(f"Got {len(mol)} atoms from {m.path}")
             self.mols.append(mol)
+        self.maps = self._load_maps()
         ranks, nproc = processor_ranks(self.inp)
         self.ranks = ranks
         self.per_rank = []     # (results, elec2calc) per processor run
@@ -143,6 +146,7 @@ class Runner:
         for e in self.inp.elecs:
             chain = _mgparm_chain(e, centers, rank)
             pbe = _pbe_dict(e)
+            pbe["maps"] = self._maps_of(e)
             mol = self.mols[e.mol - 1]
             for level, mgp in enumerate(chain):
                 focus = level > 0 or e.bcfl == "focus"
@@ -158,7 +162,12 @@ class Runner:
                                    mixed_tol=self.mixed_tol)
                 prev = None
                 calc.fillco()
-                calc.solve()
+                if e.method == "mg-dummy":
+                    # solveMG (routines.c:1653-1668): no solve, the solution
+                    # stays the zero array Vpmg_ctor starts it at
+                    self.p("  Skipping solve for mg-dummy run; zeroing solution array")
+                else:
+                    calc.solve()
                 calc.set_part_domain()
                 res = CalcResult(e.temp, e.calcenergy, e.calcforce)
                 self._energy(calc, e, res)
@@ -171,6 +180,33 @@ class Runner:
             self.elec2calc.append(icalc - 1)
         if prev is not None:
             prev.release()
+
+    def _load_maps(self):
+        """The READ maps in input order per kind (loadDielMaps, loadKappaMaps,
+        loadChargeMaps, loadPotMaps: routines.c:278-1040): a dielectric entry
+        is the (x, y, z)-shifted triple."""
+        maps = {k: [] for k in nosh.MAP_KINDS}
+        for kind, fmt, paths in self.inp.maps:
+            if kind not in maps:
+                continue
+            loaded = [driver.Map.from_dx(self.inp.path(pth)) for pth in paths]
+            for pth, m in zip(paths, loaded):
+                n = m.data.shape
+                self.p(f"Got {kind} map {pth}: {n[2]} x {n[1]} x {n[0]} points")
+            maps[kind].append(loaded if kind == "diel" else loaded[0])
+        return maps
+
+    def _maps_of(self, e):
+        """Vpmg_fillco's map arguments for one ELEC section (its usemap
+        statements, 1-based ids per kind)."""
+        out = {}
+        for kind, ident in e.usemaps:
+            m = self.maps[kind][ident - 1]
+            if kind == "diel":
+                out["dielx"], out["diely"], out["dielz"] = m
+            else:
+                out[kind] = m
+        return out
 
     # ------------------------------------------------------------ per calc
     def _energy(self, calc, e, res):
@@ -244,9 +280,14 @@ class Runner:
         nx, ny, nz = pp.nx, pp.ny, pp.nz
         h = (pp.hx, pp.hy, pp.hzed)
         cen = (pp.xcent, pp.ycent, pp.zcent)
-        origin = tuple(cen[d] - 0.5 * ((nx, ny, nz)[d] - 1) * h[d] for d in range(3))
         for what, fmt, stem in e.writes:
             vdt, title = _WRITE[what]
+            # the shifted dielectric maps sit half a spacing up their axis
+            # (routines.c:2812-2878)
+            shift = [0.5 * h[d] if what == ("dielx", "diely", "dielz")[d] else 0.0
+                     for d in range(3)]
+            origin = tuple(cen[d] + shift[d] - 0.5 * ((nx, ny, nz)[d] - 1) * h[d]
+                           for d in range(3))
             buf = np.empty(nx * ny * nz)
             if not L.Vpmg_fillArray(calc.pmg, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                     vdt, 0.0, 0, None):

@@ -54,7 +54,19 @@ typedef enum { PCE_NO = 0, PCE_TOTAL = 1, PCE_COMPS = 2 } PBEparm_calcEnergy;
 #define APBS_PMG_MAXION 10   /* src/generic/vhal.h:377 MAXION */
 
 typedef struct sVmem Vmem;   /* opaque; unused by this library */
-typedef struct sVgrid Vgrid; /* opaque; external maps are not supported */
+/* struct sVgrid, field for field (src/mg/vgrid.h:85-104): a scalar map on a
+ * regular grid, data x fastest.  Used by Vpmg_fillco for the coefficient,
+ * charge and potential maps (usemap, bcfl map). */
+typedef struct sVgrid {
+    int nx, ny, nz;
+    double hx, hy, hzed;
+    double xmin, ymin, zmin;
+    double xmax, ymax, zmax;
+    double *data;
+    int readdata;
+    int ctordata;
+    void *mem;
+} Vgrid;
 
 /* ------------------------------------------------------------------ atoms
  * struct sVatom (src/generic/vatom.h:84-100) and struct sValist
@@ -241,6 +253,16 @@ int MGparm_setupAuto(MGparm *elec, MGparm **out, int maxcalc);
  * MCT_PARALLEL with the partition kept.  Returns the number of levels, 0 on
  * failure.  No reference counterpart as a separate call (see above). */
 int MGparm_setupPara(MGparm *elec, int rank, MGparm **out, int maxcalc);
+
+/* ------------------------------------------------------------------ Vgrid */
+/* Vgrid_ctor (vgrid.c:86-110): a map over data (borrowed, not copied); xmax
+ * = xmin + (nx-1) hx and likewise.  Returns NULL on failure. */
+Vgrid *Vgrid_ctor(int nx, int ny, int nz, double hx, double hy, double hzed, double xmin,
+                  double ymin, double zmin, double *data);
+void Vgrid_dtor(Vgrid **thee);                                   /* vgrid.c:152 */
+/* Vgrid_value (vgrid.c:179-260): trilinear value at pt; 1 on the map, 0 off
+ * it.  Coordinates within 1e-4 of a face snap to it, as in the reference. */
+int Vgrid_value(Vgrid *thee, double pt[3], double *value);
 
 /* ------------------------------------------------------------------ Vpmgp
  * struct sVpmgp, field for field (src/mg/vpmgp.h:80-198). */

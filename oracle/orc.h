@@ -21,6 +21,9 @@
  *    restriction = P^T, interpolation = P, residual = f - (A + cc) x, one
  *    red-black sweep = the colour-by-colour update;
  *  - forces (qf + ib + db) = minus the central difference of the energy.
+ *  - external maps: trilinear interpolation reproduces linear fields exactly;
+ *    maps written from a solve's own coefficients, charges and potential
+ *    reproduce that solve (tests/test_oracle_maps.py).
  *  - "parity unpinned": the exact iterate sequence of the lexicographic
  *    27-point Gauss-Seidel sweep and the banded LINPACK coarse solve beyond
  *    what the golden energies constrain; examples/born/polarforces is not a
@@ -38,7 +41,10 @@ extern "C" {
 /* Enum values follow src/generic/vhal.h:103-123, 205-237. */
 enum { ORC_SRFM_MOL = 0, ORC_SRFM_SMOL = 1, ORC_SRFM_SPL2 = 2, ORC_SRFM_SPL3 = 3, ORC_SRFM_SPL4 = 4 };
 enum { ORC_CHGM_SPL0 = 0, ORC_CHGM_SPL2 = 1, ORC_CHGM_SPL4 = 2 };
-enum { ORC_BCFL_ZERO = 0, ORC_BCFL_SDH = 1, ORC_BCFL_MDH = 2, ORC_BCFL_FOCUS = 4 };
+enum { ORC_BCFL_ZERO = 0, ORC_BCFL_SDH = 1, ORC_BCFL_MDH = 2, ORC_BCFL_FOCUS = 4, ORC_BCFL_MAP = 6 };
+/* External maps (usemap): Vpmg_fillco's dielXMap..chargeMap arguments. */
+enum { ORC_MAP_DIELX = 0, ORC_MAP_DIELY = 1, ORC_MAP_DIELZ = 2, ORC_MAP_KAPPA = 3,
+       ORC_MAP_POT = 4, ORC_MAP_CHARGE = 5 };
 enum { ORC_LPBE = 0, ORC_NPBE = 1 };
 
 typedef struct orc_params {
@@ -80,6 +86,13 @@ double orc_diel_energy(orc_pmg *m, int extFlag);    /* vpmg.c:1279 */
 /* Vpmg_qfForce / ibForce / dbForce (vpmg.c:5440-6154); return 1/0. */
 int orc_force(orc_pmg *m, int atom, double qf[3], double ib[3], double db[3]);
 void orc_set_part_domain(orc_pmg *m);               /* setPartMG: routines.c:1676 */
+/* Attaches a map (data copied, x fastest) for the next orc_fillco: n points
+ * per axis, spacing h, first point origin.  Returns 1, or 0 on bad input. */
+int orc_set_map(orc_pmg *m, int which, const int n[3], const double h[3],
+                const double origin[3], const double *data);
+/* Vgrid_value (vgrid.c:179-260) on such a map: 1 on the map, 0 off it. */
+int orc_map_value(const int n[3], const double h[3], const double origin[3],
+                  const double *data, const double pt[3], double *value);
 
 /* Accessors: copies of the named n=nx*ny*nz array into out.
  * which: 0 u, 1 epsx, 2 epsy, 3 epsz, 4 kappa, 5 charge, 6 pvec. */

@@ -30,7 +30,18 @@ struct orc_pmg {
     double extQf, extQm, extDi;
     int filled, solved;
     omg g;
+    /* external maps (Vpmg_fillco's useXMap / XMap, vpmg.c:5299-5310) */
+    int usemap[6];
+    struct ogrid *map[6];
 };
+
+/* A scalar field on a regular grid: a Vgrid (vgrid.h:85-104), or a coarser
+ * solution seen by focusing. */
+typedef struct ogrid {
+    int nx, ny, nz;
+    double hx, hy, hz, xmin, ymin, zmin, xmax, ymax, zmax;
+    double *data;
+} ogrid;
 
 void orc_fbound_apply(int nx, int ny, int nz, double *u, const double *gx,
                       const double *gy, const double *gz);
@@ -825,20 +836,13 @@ static void bc_sdh(orc_pmg *m)
 
 /* ---------------------------------------------------------------- focusing */
 
-static double trilin_old(const orc_pmg *o, double x, double y, double z, int *ok)
+static double trilin_old(const ogrid *o, double x, double y, double z, int *ok)
 {
-    double hxO = o->hx, hyO = o->hy, hzO = o->hz;
-    double xminO = o->p.center[0] - ((double)(o->nx - 1) * hxO) / 2.0;
-    double yminO = o->p.center[1] - ((double)(o->ny - 1) * hyO) / 2.0;
-    double zminO = o->p.center[2] - ((double)(o->nz - 1) * hzO) / 2.0;
-    double xmaxO = o->p.center[0] + ((double)(o->nx - 1) * hxO) / 2.0;
-    double ymaxO = o->p.center[1] + ((double)(o->ny - 1) * hyO) / 2.0;
-    double zmaxO = o->p.center[2] + ((double)(o->nz - 1) * hzO) / 2.0;
-    *ok = (x >= (xminO - O_VSMALL)) && (y >= (yminO - O_VSMALL)) && (z >= (zminO - O_VSMALL)) &&
-          (x <= (xmaxO + O_VSMALL)) && (y <= (ymaxO + O_VSMALL)) && (z <= (zmaxO + O_VSMALL));
+    *ok = (x >= (o->xmin - O_VSMALL)) && (y >= (o->ymin - O_VSMALL)) && (z >= (o->zmin - O_VSMALL)) &&
+          (x <= (o->xmax + O_VSMALL)) && (y <= (o->ymax + O_VSMALL)) && (z <= (o->zmax + O_VSMALL));
     if (!*ok) return 0.0;
     int nx = o->nx, ny = o->ny;
-    double fi = (x - xminO) / hxO, fj = (y - yminO) / hyO, fk = (z - zminO) / hzO;
+    double fi = (x - o->xmin) / o->hx, fj = (y - o->ymin) / o->hy, fk = (z - o->zmin) / o->hz;
     int ihi = (int)ceil(fi); if (ihi > (o->nx - 1)) ihi = o->nx - 1;
     int ilo = (int)floor(fi); if (ilo < 0) ilo = 0;
     int jhi = (int)ceil(fj); if (jhi > (o->ny - 1)) jhi = o->ny - 1;
@@ -846,7 +850,7 @@ static double trilin_old(const orc_pmg *o, double x, double y, double z, int *ok
     int khi = (int)ceil(fk); if (khi > (o->nz - 1)) khi = o->nz - 1;
     int klo = (int)floor(fk); if (klo < 0) klo = 0;
     double dx = fi - (double)(ilo), dy = fj - (double)(jlo), dz = fk - (double)(klo);
-    const double *d = o->u;
+    const double *d = o->data;
     return dx * dy * dz * (d[IJK(ihi, jhi, khi)]) + dx * (1.0 - dy) * dz * (d[IJK(ihi, jlo, khi)]) +
            dx * dy * (1.0 - dz) * (d[IJK(ihi, jhi, klo)]) + dx * (1.0 - dy) * (1.0 - dz) * (d[IJK(ihi, jlo, klo)]) +
            (1.0 - dx) * dy * dz * (d[IJK(ilo, jhi, khi)]) + (1.0 - dx) * (1.0 - dy) * dz * (d[IJK(ilo, jlo, khi)]) +
@@ -854,8 +858,168 @@ static double trilin_old(const orc_pmg *o, double x, double y, double z, int *ok
            (1.0 - dx) * (1.0 - dy) * (1.0 - dz) * (d[IJK(ilo, jlo, klo)]);
 }
 
+/* the old Vpmg's mesh (vpmg.c:1982-1996) */
+static ogrid old_mesh(const orc_pmg *o)
+{
+    ogrid g;
+    g.nx = o->nx; g.ny = o->ny; g.nz = o->nz;
+    g.hx = o->hx; g.hy = o->hy; g.hz = o->hz;
+    g.xmin = o->p.center[0] - ((double)(o->nx - 1) * o->hx) / 2.0;
+    g.xmax = o->p.center[0] + ((double)(o->nx - 1) * o->hx) / 2.0;
+    g.ymin = o->p.center[1] - ((double)(o->ny - 1) * o->hy) / 2.0;
+    g.ymax = o->p.center[1] + ((double)(o->ny - 1) * o->hy) / 2.0;
+    g.zmin = o->p.center[2] - ((double)(o->nz - 1) * o->hz) / 2.0;
+    g.zmax = o->p.center[2] + ((double)(o->nz - 1) * o->hz) / 2.0;
+    g.data = o->u;
+    return g;
+}
+
+/* ------------------------------------------------------------- maps */
+
+/* Vgrid_value (vgrid.c:179-260).  Vcompare = 10^-(VGRID_DIGITS - 2) = 1e-4
+ * (vgrid.c:136, vgrid.h:70).  The reference keeps the indices in size_t, so a
+ * point less than one spacing below the map passes its only test (hi < n) with
+ * a wrapped lower index and reads outside the array; such a point is off the
+ * map here. */
+static int map_value(const ogrid *g, const double pt[3], double *value)
+{
+    const double Vcompare = 1e-4;
+    double f[3] = {(pt[0] - g->xmin) / g->hx, (pt[1] - g->ymin) / g->hy, (pt[2] - g->zmin) / g->hz};
+    const double lo[3] = {g->xmin, g->ymin, g->zmin}, hi[3] = {g->xmax, g->ymax, g->zmax};
+    const int n[3] = {g->nx, g->ny, g->nz};
+    long ih[3], il[3];
+    for (int d = 0; d < 3; d++) {
+        ih[d] = (long)ceil(f[d]);
+        il[d] = (long)floor(f[d]);
+    }
+    for (int d = 0; d < 3; d++)
+        if (fabs(pt[d] - lo[d]) < Vcompare) il[d] = 0;
+    for (int d = 0; d < 3; d++)
+        if (fabs(pt[d] - hi[d]) < Vcompare) ih[d] = n[d] - 1;
+    for (int d = 0; d < 3; d++)
+        if (ih[d] < 0 || ih[d] >= n[d] || il[d] < 0 || il[d] >= n[d]) return 0;
+    int nx = g->nx, ny = g->ny;
+    double dx = f[0] - (double)il[0], dy = f[1] - (double)il[1], dz = f[2] - (double)il[2];
+    const double *u = g->data;
+    long ihi = ih[0], jhi = ih[1], khi = ih[2], ilo = il[0], jlo = il[1], klo = il[2];
+    *value = dx * dy * dz * (u[IJK(ihi, jhi, khi)]) + dx * (1.0 - dy) * dz * (u[IJK(ihi, jlo, khi)]) +
+             dx * dy * (1.0 - dz) * (u[IJK(ihi, jhi, klo)]) +
+             dx * (1.0 - dy) * (1.0 - dz) * (u[IJK(ihi, jlo, klo)]) +
+             (1.0 - dx) * dy * dz * (u[IJK(ilo, jhi, khi)]) +
+             (1.0 - dx) * (1.0 - dy) * dz * (u[IJK(ilo, jlo, khi)]) +
+             (1.0 - dx) * dy * (1.0 - dz) * (u[IJK(ilo, jhi, klo)]) +
+             (1.0 - dx) * (1.0 - dy) * (1.0 - dz) * (u[IJK(ilo, jlo, klo)]);
+    return 1;
+}
+
+static void grid_of(ogrid *g, const int n[3], const double h[3], const double origin[3], double *data)
+{
+    g->nx = n[0]; g->ny = n[1]; g->nz = n[2];
+    g->hx = h[0]; g->hy = h[1]; g->hz = h[2];
+    g->xmin = origin[0]; g->ymin = origin[1]; g->zmin = origin[2];
+    /* Vgrid_ctor2 (vgrid.c:121-126) */
+    g->xmax = origin[0] + (n[0] - 1) * h[0];
+    g->ymax = origin[1] + (n[1] - 1) * h[1];
+    g->zmax = origin[2] + (n[2] - 1) * h[2];
+    g->data = data;
+}
+
+int orc_map_value(const int n[3], const double h[3], const double origin[3],
+                  const double *data, const double pt[3], double *value)
+{
+    ogrid g;
+    grid_of(&g, n, h, origin, (double *)data);
+    return map_value(&g, pt, value);
+}
+
+int orc_set_map(orc_pmg *m, int which, const int n[3], const double h[3],
+                const double origin[3], const double *data)
+{
+    if (which < 0 || which > 5 || n[0] < 1 || n[1] < 1 || n[2] < 1 || !(h[0] > 0) ||
+        !(h[1] > 0) || !(h[2] > 0) || !data)
+        return 0;
+    size_t npts = (size_t)n[0] * n[1] * n[2];
+    ogrid *g = m->map[which] ? m->map[which] : (ogrid *)calloc(1, sizeof(ogrid));
+    free(g->data);
+    double *copy = (double *)malloc(npts * sizeof(double));
+    memcpy(copy, data, npts * sizeof(double));
+    grid_of(g, n, h, origin, copy);
+    m->map[which] = g;
+    m->usemap[which] = 1;
+    return 1;
+}
+
+/* fillcoChargeMap (vpmg.c:4938-4985) */
+static int fill_charge_map(orc_pmg *m)
+{
+    int nx = m->nx, ny = m->ny, nz = m->nz;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                double pt[3] = {m->xf[i], m->yf[j], m->zf[k]}, q;
+                if (!map_value(m->map[ORC_MAP_CHARGE], pt, &q)) return 0;
+                m->charge[IJK(i, j, k)] = q * m->zmagic;
+            }
+    return 1;
+}
+
+/* fillcoCoefMap (vpmg.c:4098-4220) */
+static int fill_coef_map(orc_pmg *m)
+{
+    int nx = m->nx, ny = m->ny, nz = m->nz;
+    int salt = m->ionstr > O_VPMGSMALL;
+    if (!m->usemap[ORC_MAP_DIELX] || !m->usemap[ORC_MAP_DIELY] || !m->usemap[ORC_MAP_DIELZ] ||
+        (!m->usemap[ORC_MAP_KAPPA] && salt))
+        return 0;
+    double kappamax = -1.0, t;
+    if (salt) {
+        for (int k = 0; k < nz; k++)
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++) {
+                    double pt[3] = {m->xf[i], m->yf[j], m->zf[k]};
+                    if (!map_value(m->map[ORC_MAP_KAPPA], pt, &t)) return 0;
+                    if (t > kappamax) kappamax = t;
+                    if (t < 0.0) return 0;
+                }
+    }
+    if (!(kappamax > 1.0)) kappamax = 1.0;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                size_t c = IJK(i, j, k);
+                if (salt) {
+                    double pt[3] = {m->xf[i], m->yf[j], m->zf[k]};
+                    if (!map_value(m->map[ORC_MAP_KAPPA], pt, &t)) return 0;
+                    if (t < O_VPMGSMALL) t = 0.0;
+                    m->kappa[c] = t / kappamax;
+                } else {
+                    m->kappa[c] = 0.0;
+                }
+                double px[3] = {m->xf[i] + 0.5 * m->hx, m->yf[j], m->zf[k]};
+                double py[3] = {m->xf[i], m->yf[j] + 0.5 * m->hy, m->zf[k]};
+                double pz[3] = {m->xf[i], m->yf[j], m->zf[k] + 0.5 * m->hz};
+                if (!map_value(m->map[ORC_MAP_DIELX], px, &m->epsx[c])) return 0;
+                if (!map_value(m->map[ORC_MAP_DIELY], py, &m->epsy[c])) return 0;
+                if (!map_value(m->map[ORC_MAP_DIELZ], pz, &m->epsz[c])) return 0;
+            }
+    return 1;
+}
+
+/* bcfl_map (vpmg.c:3953-3990): every grid point on the potential map */
+static int bcfl_map_check(orc_pmg *m)
+{
+    double v;
+    for (int k = 0; k < m->nz; k++)
+        for (int j = 0; j < m->ny; j++)
+            for (int i = 0; i < m->nx; i++) {
+                double pt[3] = {m->xf[i], m->yf[j], m->zf[k]};
+                if (!map_value(m->map[ORC_MAP_POT], pt, &v)) return 0;
+            }
+    return 1;
+}
+
 /* focusFillBound (vpmg.c:1914-2414) */
-static int focus_fill_bound(orc_pmg *m, const orc_pmg *o)
+static int focus_fill_bound(orc_pmg *m, const ogrid *o)
 {
     int nx = m->nx, ny = m->ny, nz = m->nz;
     double xminN = m->p.center[0] - ((double)(nx - 1) * m->hx) / 2.0;
@@ -864,12 +1028,8 @@ static int focus_fill_bound(orc_pmg *m, const orc_pmg *o)
     double ymaxN = m->p.center[1] + ((double)(ny - 1) * m->hy) / 2.0;
     double zminN = m->p.center[2] - ((double)(nz - 1) * m->hz) / 2.0;
     double zmaxN = m->p.center[2] + ((double)(nz - 1) * m->hz) / 2.0;
-    double xminO = o->p.center[0] - ((double)(o->nx - 1) * o->hx) / 2.0;
-    double xmaxO = o->p.center[0] + ((double)(o->nx - 1) * o->hx) / 2.0;
-    double yminO = o->p.center[1] - ((double)(o->ny - 1) * o->hy) / 2.0;
-    double ymaxO = o->p.center[1] + ((double)(o->ny - 1) * o->hy) / 2.0;
-    double zminO = o->p.center[2] - ((double)(o->nz - 1) * o->hz) / 2.0;
-    double zmaxO = o->p.center[2] + ((double)(o->nz - 1) * o->hz) / 2.0;
+    double xminO = o->xmin, xmaxO = o->xmax, yminO = o->ymin, ymaxO = o->ymax;
+    double zminO = o->zmin, zmaxO = o->zmax;
     if (fabs(xminO - xminN) < O_VSMALL) xminN = xminO;
     if (fabs(xmaxO - xmaxN) < O_VSMALL) xmaxN = xmaxO;
     if (fabs(yminO - yminN) < O_VSMALL) yminN = yminO;
@@ -1032,7 +1192,8 @@ orc_pmg *orc_create(const orc_params *p, int natoms, const double *xyz,
     m->zf = (double *)calloc(m->nz, sizeof(double));
     if (old) {
         m->p.bcfl = ORC_BCFL_FOCUS;
-        if (!focus_fill_bound(m, old)) { orc_destroy(old); orc_destroy(m); return NULL; }
+        ogrid om = old_mesh(old);
+        if (!focus_fill_bound(m, &om)) { orc_destroy(old); orc_destroy(m); return NULL; }
         if (energy_flag) ext_energy(m, old);
         orc_destroy(old);
     }
@@ -1047,6 +1208,8 @@ void orc_destroy(orc_pmg *m)
     free(m->epsx); free(m->epsy); free(m->epsz); free(m->kappa); free(m->charge);
     free(m->u); free(m->pvec); free(m->gxcf); free(m->gycf); free(m->gzcf);
     free(m->xf); free(m->yf); free(m->zf);
+    for (int w = 0; w < 6; w++)
+        if (m->map[w]) { free(m->map[w]->data); free(m->map[w]); }
     omg_free(&m->g);
     free(m);
 }
@@ -1059,14 +1222,22 @@ int orc_fillco(orc_pmg *m)
     m->xmax = m->p.center[0] + (m->xlen / 2.0);
     m->ymax = m->p.center[1] + (m->ylen / 2.0);
     m->zmax = m->p.center[2] + (m->zlen / 2.0);
-    int islap = (m->ionstr < O_VPMGSMALL) && (fabs(m->p.pdie - m->p.sdie) < O_VPMGSMALL);
+    /* vpmg.c:5354-5362: not a Laplacian when coefficient or potential maps are used */
+    int coef_maps = m->usemap[ORC_MAP_DIELX] || m->usemap[ORC_MAP_DIELY] ||
+                    m->usemap[ORC_MAP_DIELZ] || m->usemap[ORC_MAP_KAPPA];
+    int islap = !(coef_maps || m->usemap[ORC_MAP_POT]) && (m->ionstr < O_VPMGSMALL) &&
+                (fabs(m->p.pdie - m->p.sdie) < O_VPMGSMALL);
     for (int i = 0; i < m->nx; i++) m->xf[i] = m->xmin + i * m->hx;
     for (int i = 0; i < m->ny; i++) m->yf[i] = m->ymin + i * m->hy;
     for (int i = 0; i < m->nz; i++) m->zf[i] = m->zmin + i * m->hz;
-    if (m->p.chgm == ORC_CHGM_SPL2) fill_charge_spl2(m);
+    if (m->usemap[ORC_MAP_CHARGE]) {
+        if (!fill_charge_map(m)) return 0;
+    } else if (m->p.chgm == ORC_CHGM_SPL2) fill_charge_spl2(m);
     else if (m->p.chgm == ORC_CHGM_SPL4) fill_charge_spl4(m);
     else fill_charge_spl0(m);
-    if (!islap) {
+    if (!islap && coef_maps) { /* fillcoCoef (vpmg.c:4856-4866) */
+        if (!fill_coef_map(m)) return 0;
+    } else if (!islap) {
         if (m->p.srfm >= ORC_SRFM_SPL2) fill_spline(m, m->p.srfm);
         else {
             fill_mol_diel(m);
@@ -1085,6 +1256,10 @@ int orc_fillco(orc_pmg *m)
         memset(m->gzcf, 0, (size_t)2 * m->nx * m->ny * sizeof(double));
         if (m->p.bcfl == ORC_BCFL_MDH) bc_mdh(m);
         else if (m->p.bcfl == ORC_BCFL_SDH) bc_sdh(m);
+        else if (m->p.bcfl == ORC_BCFL_MAP) { /* bcCalc (vpmg.c:4084-4087) */
+            if (!m->usemap[ORC_MAP_POT] || !bcfl_map_check(m)) return 0;
+            if (!focus_fill_bound(m, m->map[ORC_MAP_POT])) return 0;
+        }
         else if (m->p.bcfl != ORC_BCFL_ZERO) return 0;
     }
     m->filled = 1;
@@ -1183,7 +1358,8 @@ double orc_qm_energy(orc_pmg *m, int extFlag)
 double orc_qf_energy(orc_pmg *m, int extFlag)
 {
     double e = 0.0;
-    if (m->p.chgm == ORC_CHGM_SPL2) { /* Vpmg_qfEnergyVolume (vpmg.c:1861-1891) */
+    /* Vpmg_qfEnergy (vpmg.c:1687-1702) */
+    if (m->usemap[ORC_MAP_CHARGE] || m->p.chgm == ORC_CHGM_SPL2) { /* vpmg.c:1861-1891 */
         for (size_t i = 0; i < m->n; i++) e += (m->pvec[i] * m->u[i] * m->charge[i]);
         e = e * m->hx * m->hy * m->hz / m->zmagic;
         if (extFlag == 1) e += m->extQf;

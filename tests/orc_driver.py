@@ -23,7 +23,8 @@ ORC_SO = os.path.join(ROOT, "oracle", "liborc.so")
 
 SRFM = {"mol": 0, "smol": 1, "spl2": 2, "spl3": 3, "spl4": 4}
 CHGM = {"spl0": 0, "spl2": 1, "spl4": 2}
-BCFL = {"zero": 0, "sdh": 1, "mdh": 2, "focus": 4}
+BCFL = {"zero": 0, "sdh": 1, "mdh": 2, "focus": 4, "map": 6}
+MAPS = {"dielx": 0, "diely": 1, "dielz": 2, "kappa": 3, "pot": 4, "charge": 5}
 
 # src/generic/vunit.h:75-80
 VUNIT_KB = 1.3806581e-23
@@ -86,6 +87,11 @@ def lib():
         L.orc_get_array.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
         L.orc_get_boundary.argtypes = [ctypes.c_void_p, dp, dp, dp]
         L.orc_stats.argtypes = [ctypes.c_void_p, dp]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_set_map.argtypes = [ctypes.c_void_p, ctypes.c_int, ip, dp, dp, dp]
+        L.orc_set_map.restype = ctypes.c_int
+        L.orc_map_value.argtypes = [ip, dp, dp, dp, dp, dp]
+        L.orc_map_value.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -245,6 +251,17 @@ class OrcCalc:
     def fillco(self):
         assert lib().orc_fillco(self.handle) == 1
 
+    def try_fillco(self):
+        return lib().orc_fillco(self.handle)
+
+    def set_map(self, kind, data, origin, spacing):
+        """A usemap map: data (nz, ny, nx), first point origin, spacing h."""
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        n = (ctypes.c_int * 3)(data.shape[2], data.shape[1], data.shape[0])
+        h = np.asarray(spacing, dtype=np.float64)
+        o = np.asarray(origin, dtype=np.float64)
+        assert lib().orc_set_map(self.handle, MAPS[kind], n, dptr(h), dptr(o), dptr(data)) == 1
+
     def solve(self):
         assert lib().orc_solve(self.handle) == 1
 
@@ -289,6 +306,18 @@ class OrcCalc:
         if getattr(self, "handle", None):
             lib().orc_destroy(self.handle)
             self.handle = None
+
+
+def map_value(data, origin, spacing, pt):
+    """Vgrid_value of the oracle: the value at pt, or None off the map."""
+    data = np.ascontiguousarray(data, dtype=np.float64)
+    n = (ctypes.c_int * 3)(data.shape[2], data.shape[1], data.shape[0])
+    h = np.asarray(spacing, dtype=np.float64)
+    o = np.asarray(origin, dtype=np.float64)
+    p = np.asarray(pt, dtype=np.float64)
+    v = np.zeros(1)
+    ok = lib().orc_map_value(n, dptr(h), dptr(o), dptr(data), dptr(p), dptr(v))
+    return float(v[0]) if ok else None
 
 
 def run_manual(atoms, dime, center, pbe, grid=None, glen=None, nlev_user=4):

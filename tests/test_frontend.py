@@ -97,7 +97,7 @@ def test_print_ids_digits_before_names():
 
 @pytest.mark.parametrize("snippet,reason", [
     ("read mol pdb a.pdb end elec mg-manual end quit", "only PQR"),
-    ("read mol pqr a.pqr end elec mg-dummy end quit", "mg-dummy"),
+    ("read mol pqr a.pqr end elec mg-dummy end quit", "not set: dime"),
     ("read mol pqr a.pqr end elec mg-para dime 9 9 9 cglen 9 9 9 fglen 5 5 5 cgcent mol 1 "
      "fgcent mol 1 mol 1 lpbe bcfl mdh pdie 1 sdie 1 chgm spl2 srfm spl2 swin 0.3 temp 300 "
      "end quit", "pdime"),
@@ -111,7 +111,21 @@ def test_print_ids_digits_before_names():
      "bcfl mdh pdie 1 sdie 1 chgm spl6 srfm spl2 swin 0.3 temp 300 end quit", "chgm spl6"),
     ("read mol pqr a.pqr end elec mg-manual dime 9 9 9 glen 5 5 5 gcent mol 1 mol 1 lpbe "
      "bcfl mdh pdie 1 sdie 1 chgm spl2 srfm spl2 swin 0.3 temp 300 usemap diel 1 end quit",
-     "usemap"),
+     "1 is not a valid diel map ID"),
+    ("read mol pqr a.pqr diel dxbin x y z end elec mg-manual end quit", "read as OpenDX"),
+    ("read mol pqr a.pqr mesh mcsf m end elec mg-manual end quit", "finite-element"),
+    ("read mol pqr a.pqr kappa dx k.dx end elec mg-manual dime 9 9 9 glen 5 5 5 gcent mol 1 "
+     "mol 1 lpbe bcfl mdh ion 1 0.1 2 ion -1 0.1 2 pdie 1 sdie 1 chgm spl2 srfm spl2 swin 0.3 "
+     "temp 300 usemap kappa 1 end quit", "only with dielectric maps"),
+    ("read mol pqr a.pqr diel dx x y z end elec mg-manual dime 9 9 9 glen 5 5 5 gcent mol 1 "
+     "mol 1 lpbe bcfl mdh ion 1 0.1 2 ion -1 0.1 2 pdie 1 sdie 1 chgm spl2 srfm spl2 swin 0.3 "
+     "temp 300 usemap diel 1 end quit", "need a kappa map"),
+    ("read mol pqr a.pqr pot dx p.dx end elec mg-manual dime 9 9 9 glen 5 5 5 gcent mol 1 "
+     "mol 1 lpbe bcfl map pdie 1 sdie 1 chgm spl2 srfm spl2 swin 0.3 temp 300 usemap pot 2 "
+     "end quit", "2 is not a valid pot map ID"),
+    ("read mol pqr a.pqr end elec mg-manual dime 9 9 9 glen 5 5 5 gcent mol 1 mol 1 lpbe "
+     "bcfl mdh pdie 1 sdie 1 chgm spl2 srfm spl2 swin 0.3 temp 300 usemap phi 1 end quit",
+     "usemap phi"),
     ("read mol pqr a.pqr end elec mg-manual dime 9 9 9 glen 5 5 5 gcent mol 1 mol 1 lpbe "
      "bcfl focus pdie 1 sdie 1 chgm spl2 srfm spl2 swin 0.3 temp 300 end quit",
      "can't focus the first"),
@@ -121,6 +135,22 @@ def test_print_ids_digits_before_names():
 def test_refusals_name_the_keyword(snippet, reason):
     why = nosh.check_runnable(nosh.parse(snippet))
     assert any(reason in w for w in why), why
+
+
+def test_map_inputs_are_runnable():
+    """READ diel/kappa/charge/pot maps with usemap, bcfl map and mg-dummy."""
+    text = ("read mol pqr a.pqr diel dx x.dx y.dx z.dx kappa gz k.dx.gz charge dx q.dx "
+            "pot dx p.dx end "
+            "elec name d mg-dummy dime 9 9 9 glen 5 5 5 gcent mol 1 mol 1 lpbe bcfl mdh "
+            "pdie 1 sdie 1 chgm spl2 srfm spl2 swin 0.3 temp 300 write dielx dx dx end "
+            "elec mg-manual dime 9 9 9 glen 5 5 5 gcent mol 1 mol 1 lpbe bcfl map "
+            "ion 1 0.1 2 ion -1 0.1 2 pdie 1 sdie 1 chgm spl2 srfm spl2 swin 0.3 temp 300 "
+            "usemap diel 1 usemap kappa 1 usemap charge 1 usemap pot 1 end quit")
+    inp = nosh.parse(text)
+    assert nosh.check_runnable(inp) == []
+    assert [m[0] for m in inp.maps] == ["diel", "kappa", "charge", "pot"]
+    assert inp.maps[0][2] == ["x.dx", "y.dx", "z.dx"]
+    assert inp.elecs[1].usemaps == [("diel", 1), ("kappa", 1), ("charge", 1), ("pot", 1)]
 
 
 @pytest.mark.parametrize("bad", ["read mol pqr a.pqr end", "elec mg-manual dime 1 2 end quit",
