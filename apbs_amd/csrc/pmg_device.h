@@ -40,8 +40,10 @@ int gsrb7_launches(int nu);
 // between x and tmp (fine-size scratch).  Only interior points are written,
 // so x and tmp must hold the same boundary values (zero in the V-cycle).
 // The result is bitwise that of gsrb7 and ends in x.
-// tile_shape: rows*100 + planes of a workgroup's block (416, 48, 816, 88,
-// 84, 1616, 832); <= 0 takes APBS_PMG_FUSED_TILE or the default 816.
+// tile_shape: width*10000 + rows*100 + planes of a workgroup's block (rows
+// and planes 416, 48, 44, 68, 816, 88, 84, 1616, 832; width 64, 80, 96, or 0
+// for the width that fits nx best); <= 0 takes APBS_PMG_FUSED_TILE or the
+// default 48.
 template <class T>
 void gsrb7_fused(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D,
                  const T* fc, T* x, T* tmp, int nu, int iadjoint, int tile_shape = 0);
@@ -49,6 +51,23 @@ int gsrb7_fused_launches(int nu);
 template <class T>
 void gsrb27(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
             T* x, int nu, int iadjoint);
+// The same 8-colour sweeps with one launch per plane parity (the four colours
+// of the even or odd planes), ping-ponging between x and xs (level-size
+// scratch).  Only interior points are written, so x and xs must hold the same
+// boundary values (zero on the coarse levels).  Bitwise the result of gsrb27,
+// ending in x.  tile: 8, 16 or 32 (square tiles of a plane); other values
+// take gsrb27_planes_tile.
+template <class T>
+void gsrb27_planes(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D,
+                   const T* fc, T* x, T* xs, int nu, int iadjoint, int tile = 0);
+int gsrb27_planes_tile(int nx, int ny, int nz);
+// Whether the plane-parity form is the faster one for a level: it trades a
+// quarter of the launches for four dependent colour passes per workgroup,
+// which pays on levels up to about 49^2 points per plane (measured on MI355X:
+// 41^3 22 vs 29 us per FP64 sweep, 21^3 16 vs 22, 11^3 14 vs 22; at 81^3 the
+// per-colour launches are faster, 63 vs 75).
+bool gsrb27_use_planes(int nx, int ny, int nz);
+int gsrb27_planes_launches(int nz, int nu);
 
 // ------------------------------------------------------- residual / matvec
 template <class T>
@@ -124,7 +143,9 @@ void galerkin(hipStream_t s, int st, int nxf, int nyf, int nzf, const Ops<double
 // diagonal D, then invert it in place (Gauss-Jordan, SPD, one workgroup).
 void dense_assemble(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* D,
                     double* M, int N);
-void dense_invert(hipStream_t s, double* M, double* work, int N);
+// form 0: registers for N <= 64 (one barrier per pivot), else LDS / global;
+// form 1: never the register form
+void dense_invert(hipStream_t s, double* M, double* work, int N, int form = 0);
 // x(interior) = Minv * f(interior); boundary untouched
 template <class T>
 void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int N, const T* f,

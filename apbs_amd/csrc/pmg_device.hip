@@ -10,6 +10,7 @@
 
 #include <cmath>
 #include <cstdlib>
+#include <utility>
 
 namespace pmg {
 
@@ -71,10 +72,10 @@ __global__ void __launch_bounds__(kBlock)
 // ring and the whole tile row of plane k is written.  Each value is the
 // in-place sweep's value bit for bit (same terms, same order, no FMA).
 // Points outside the interior are not written.
-constexpr int kTX = 64, kFT = 256, kXCD = 8;
+constexpr int kFT = 256, kXCD = 8;
 
-// kTY rows and kKC planes per workgroup.
-template <class T, int kTY, int kKC>
+// kTX columns (even), kTY rows and kKC planes per workgroup.
+template <class T, int kTY, int kKC, int kTX>
 __global__ void __launch_bounds__(kFT)
     k_gsrb7_fused(int nx, int ny, int nz, const T* __restrict__ oE, const T* __restrict__ oN,
                   const T* __restrict__ uC, const T* __restrict__ D, const T* __restrict__ fc,
@@ -308,6 +309,91 @@ __global__ void __launch_bounds__(kBlock)
   const size_t p = IX(i, j, k, nx, ny);
   P27<T> s = part27<T>(A.a, x, p, nx, nxy);
   x[p] = (fc[p] + (s.o + s.u + s.d)) / D[p];
+}
+
+// The four colours of one plane parity of that sweep in one launch: colours
+// 4ck .. 4ck+3 (4ck+3 .. 4ck for the adjoint sweep).  Planes of one parity do
+// not touch each other, so a workgroup owns a TX x TY tile of one plane.  It
+// loads the plane's values over the tile and a 4-point margin into LDS and
+// runs the four colours there in order: colour s over the tile grown by 3 - s
+// points, so the margin holds every value the tile's later colours read.
+// Margin points are computed by several workgroups, identically, and only the
+// tile is written.  In-plane values come from src, the neighbouring planes
+// from nbr, and the result goes to dst, so no workgroup reads what another
+// writes.  The arithmetic is part27's, term for term.
+template <class T, int TX, int TY>
+__global__ void __launch_bounds__(kBlock)
+    k_gs27_plane(int nx, int ny, int nz, Ops27<T> A, const T* __restrict__ D,
+                 const T* __restrict__ fc, const T* __restrict__ src, const T* __restrict__ nbr,
+                 T* __restrict__ dst, int ck, int adj) {
+  constexpr int M = 4, LX = TX + 2 * M, LY = TY + 2 * M;
+  __shared__ T sx[LX * LY];
+  const T *oE = A.a[1], *oN = A.a[2], *uC = A.a[3], *oNE = A.a[4], *oNW = A.a[5], *uE = A.a[6],
+          *uW = A.a[7], *uN = A.a[8], *uS = A.a[9], *uNE = A.a[10], *uNW = A.a[11],
+          *uSE = A.a[12], *uSW = A.a[13];
+  const int tilesx = (nx - 2 + TX - 1) / TX;
+  const int tx0 = 1 + (int)(blockIdx.x % tilesx) * TX;
+  const int ty0 = 1 + (int)(blockIdx.x / tilesx) * TY;
+  const int k = (ck ? 1 : 2) + 2 * (int)blockIdx.y;
+  const size_t nxy = (size_t)nx * ny;
+  const size_t kp = (size_t)k * nxy;
+  const int bx = tx0 - M, by = ty0 - M;  // grid coordinates of sx[0]
+  for (int t = threadIdx.x; t < LX * LY; t += kBlock) {
+    const int i = bx + t % LX, j = by + t / LX;
+    sx[t] = (i >= 0 && i < nx && j >= 0 && j < ny) ? src[kp + (size_t)j * nx + i] : T(0);
+  }
+  __syncthreads();
+  for (int s = 0; s < 4; s++) {
+    const int q = adj ? 3 - s : s;  // in-plane colour: i parity (q & 1), j parity (q >> 1)
+    const int pi = q & 1, pj = q >> 1;
+    const int g = 3 - s;
+    int ilo = max(1, tx0 - g), jlo = max(1, ty0 - g);
+    const int ihi = min(nx - 2, tx0 + TX - 1 + g), jhi = min(ny - 2, ty0 + TY - 1 + g);
+    // colour parity 0 is the even indices (first interior point 2), 1 the odd
+    ilo += (ilo & 1) != pi;
+    jlo += (jlo & 1) != pj;
+    const int ni = ilo <= ihi ? (ihi - ilo) / 2 + 1 : 0;
+    const int nj = jlo <= jhi ? (jhi - jlo) / 2 + 1 : 0;
+    for (int t = threadIdx.x; t < ni * nj; t += kBlock) {
+      const int i = ilo + 2 * (t % ni), j = jlo + 2 * (t / ni);
+      const int e = (j - by) * LX + (i - bx);
+      const size_t p = kp + (size_t)j * nx + i;
+      T o = oN[p] * sx[e + LX];
+      o += oN[p - nx] * sx[e - LX];
+      o += oE[p] * sx[e + 1];
+      o += oE[p - 1] * sx[e - 1];
+      o += oNE[p] * sx[e + LX + 1];
+      o += oNW[p] * sx[e + LX - 1];
+      o += oNW[p - nx + 1] * sx[e - LX + 1];
+      o += oNE[p - nx - 1] * sx[e - LX - 1];
+      const size_t qa = p + nxy;
+      T u = uC[p] * nbr[qa];
+      u += uN[p] * nbr[qa + nx];
+      u += uS[p] * nbr[qa - nx];
+      u += uE[p] * nbr[qa + 1];
+      u += uW[p] * nbr[qa - 1];
+      u += uNE[p] * nbr[qa + nx + 1];
+      u += uNW[p] * nbr[qa + nx - 1];
+      u += uSE[p] * nbr[qa - nx + 1];
+      u += uSW[p] * nbr[qa - nx - 1];
+      const size_t m = p - nxy;
+      T d = uC[m] * nbr[m];
+      d += uS[m + nx] * nbr[m + nx];
+      d += uN[m - nx] * nbr[m - nx];
+      d += uW[m + 1] * nbr[m + 1];
+      d += uE[m - 1] * nbr[m - 1];
+      d += uSW[m + nx + 1] * nbr[m + nx + 1];
+      d += uSE[m + nx - 1] * nbr[m + nx - 1];
+      d += uNW[m - nx + 1] * nbr[m - nx + 1];
+      d += uNE[m - nx - 1] * nbr[m - nx - 1];
+      sx[e] = (fc[p] + (o + u + d)) / D[p];
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < TX * TY; t += kBlock) {
+    const int i = tx0 + t % TX, j = ty0 + t / TX;
+    if (i <= nx - 2 && j <= ny - 2) dst[kp + (size_t)j * nx + i] = sx[(j - by) * LX + (i - bx)];
+  }
 }
 
 template <class T>
@@ -867,12 +953,113 @@ __global__ void __launch_bounds__(kBlock)
   out.a[s][pc] = (s == 0) ? sum : -sum;
 }
 
+// The same sum with every loop unrolled at compile time for one stencil
+// width ST and one stored direction S: the pruned loop bounds, both
+// prolongation weights and the stored array of each fine offset become
+// constants, so a term is one load and one multiply-add.  Terms whose fine
+// entry is a structural zero (the diagonal offsets of a 7-point operator) add
+// +0.0 to a partial sum that starts at +0.0 and so cannot be -0.0; they are
+// left out without changing a bit.
+namespace gal {
+constexpr int kFwd[14][3] = {{0, 0, 0},  {1, 0, 0},  {0, 1, 0},  {0, 0, 1}, {1, 1, 0},
+                             {-1, 1, 0}, {1, 0, 1},  {-1, 0, 1}, {0, 1, 1}, {0, -1, 1},
+                             {1, 1, 1},  {-1, 1, 1}, {1, -1, 1}, {-1, -1, 1}};
+__host__ __device__ constexpr int iabs(int v) { return v < 0 ? -v : v; }
+__host__ __device__ constexpr int imax(int a, int b) { return a > b ? a : b; }
+__host__ __device__ constexpr int imin(int a, int b) { return a < b ? a : b; }
+// stored direction of offset (sx, sy, sz): +s forward, -s backward
+__host__ __device__ constexpr int ent(int sx, int sy, int sz) {
+  for (int s = 0; s < 14; s++) {
+    if (kFwd[s][0] == sx && kFwd[s][1] == sy && kFwd[s][2] == sz) return s;
+    if (kFwd[s][0] == -sx && kFwd[s][1] == -sy && kFwd[s][2] == -sz) return -s;
+  }
+  return 99;
+}
+__host__ __device__ constexpr double pw(int dx, int dy, int dz) {
+  return (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1)
+             ? 0.0
+             : (iabs(dx) + iabs(dy) + iabs(dz) == 0
+                    ? 1.0
+                    : (iabs(dx) + iabs(dy) + iabs(dz) == 1
+                           ? 0.5
+                           : (iabs(dx) + iabs(dy) + iabs(dz) == 2 ? 0.25 : 0.125)));
+}
+
+template <int ST, int S>
+__device__ __forceinline__ double coef(const Ops27<double>& A, size_t pf, int nx, size_t nxy) {
+  constexpr int FX = kFwd[S][0], FY = kFwd[S][1], FZ = kFwd[S][2];
+  double sum = 0.0;
+#pragma unroll
+  for (int d = 0; d < 27; d++) {
+    const int dz = d / 9 - 1, dy = (d / 3) % 3 - 1, dx = d % 3 - 1;
+    const int ox = 2 * FX - dx, oy = 2 * FY - dy, oz = 2 * FZ - dz;
+    const long f = (long)dx + (long)dy * nx + (long)dz * (long)nxy;
+    double tt = 0.0;
+#pragma unroll
+    for (int sz = -1; sz <= 1; sz++)
+#pragma unroll
+      for (int sy = -1; sy <= 1; sy++)
+#pragma unroll
+        for (int sx = -1; sx <= 1; sx++) {
+          if (sx < imax(-1, ox - 1) || sx > imin(1, ox + 1) || sy < imax(-1, oy - 1) ||
+              sy > imin(1, oy + 1) || sz < imax(-1, oz - 1) || sz > imin(1, oz + 1))
+            continue;
+          if (ST == 7 && iabs(sx) + iabs(sy) + iabs(sz) > 1) continue;
+          const double p2 = pw(sx - ox, sy - oy, sz - oz);
+          const int e = ent(sx, sy, sz);
+          double v;
+          if (e == 0)
+            v = A.a[0][pf + f];
+          else if (e > 0)
+            v = -A.a[e][pf + f];
+          else
+            v = -A.a[-e][pf + f + (long)sx + (long)sy * nx + (long)sz * (long)nxy];
+          tt += v * p2;
+        }
+    sum += pw(dx, dy, dz) * tt;
+  }
+  return sum;
+}
+
+template <int ST, int... S>
+__device__ __forceinline__ double coef_switch(int s, const Ops27<double>& A, size_t pf, int nx,
+                                              size_t nxy, std::integer_sequence<int, S...>) {
+  double r = 0.0;
+  ((s == S ? (r = coef<ST, S>(A, pf, nx, nxy), 0) : 0), ...);
+  return r;
+}
+}  // namespace gal
+
+template <int ST, int WAVES>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
+    k_galerkin_unrolled(int nxf, int nyf, int nzf, Ops27<double> Af, int nxc, int nyc, int nzc,
+                        OutPtrs out) {
+  const long ni = nxc - 2, nj = nyc - 2, nk = nzc - 2;
+  const long nq = ni * nj * nk;
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long w = t / 64;
+  const int s = (int)(w % 14);
+  const long q = (w / 14) * 64 + (t % 64);
+  if (q >= nq) return;
+  const int I = 1 + (int)(q % ni), J = 1 + (int)((q / ni) % nj), K = 1 + (int)(q / (ni * nj));
+  const int I2 = I + c_fwd[s][0], J2 = J + c_fwd[s][1], K2 = K + c_fwd[s][2];
+  const size_t pc = IX(I, J, K, nxc, nyc);
+  if (I2 < 1 || I2 > nxc - 2 || J2 < 1 || J2 > nyc - 2 || K2 < 1 || K2 > nzc - 2) {
+    out.a[s][pc] = 0.0;
+    return;
+  }
+  const size_t nxy = (size_t)nxf * nyf;
+  const size_t pf = IX(2 * I, 2 * J, 2 * K, nxf, nyf);
+  const double sum =
+      gal::coef_switch<ST>(s, Af, pf, nxf, nxy, std::make_integer_sequence<int, 14>{});
+  out.a[s][pc] = (s == 0) ? sum : -sum;
+}
+
 // Small levels (too few coarse points to fill the chip one thread each): one
 // 32-lane group per (coarse interior point, stored coefficient); lane d < 27
 // computes tt_d, lane 0 then adds p1_d * tt_d for d = 0..26 in order, the
 // same sum as k_galerkin_point's.
 constexpr int kGalGroups = kBlock / 32;
-constexpr long kGalPointMin = 16384;
 
 __global__ void __launch_bounds__(kBlock)
     k_galerkin_group(int st, int nxf, int nyf, int nzf, Ops27<double> Af, int nxc, int nyc,
@@ -1014,6 +1201,45 @@ __global__ void __launch_bounds__(1024) k_dense_invert_lds(double* Mg, int N) {
   for (int e = tid; e < N * N; e += 1024) Mg[e] = M[e];
 }
 
+// The same Gauss-Jordan inverse for N <= 64 with the matrix in registers:
+// thread t holds 16 entries of row t / 4.  Each pivot step publishes the
+// pivot row and column through double-buffered LDS (one barrier per pivot)
+// and scales the pivot row by the pivot's reciprocal.
+__global__ void __launch_bounds__(256) k_dense_invert_reg(double* Mg, int N) {
+  __shared__ double colb[2][64], rowb[2][64];
+  const int t = threadIdx.x, r = t >> 2, cb = t & 3, c0 = cb * 16;
+  double m[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++)
+    m[q] = (r < N && c0 + q < N) ? Mg[(size_t)r * N + c0 + q] : 0.0;
+  for (int k = 0; k < N; k++) {
+    const int b = k & 1;
+    if (r < N && (k >> 4) == cb) {
+      double v = 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; q++)
+        if (q == (k & 15)) v = m[q];
+      colb[b][r] = v;
+    }
+    if (r == k) {
+#pragma unroll
+      for (int q = 0; q < 16; q++) rowb[b][c0 + q] = m[q];
+    }
+    __syncthreads();
+    const double inv = 1.0 / rowb[b][k];
+    const double ci = r < N ? colb[b][r] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int j = c0 + q;
+      const double pj = ((j == k) ? 1.0 : rowb[b][j]) * inv;
+      m[q] = (r == k) ? pj : ((j == k) ? 0.0 : m[q]) - ci * pj;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 16; q++)
+    if (r < N && c0 + q < N) Mg[(size_t)r * N + c0 + q] = m[q];
+}
+
 template <class T>
 __global__ void k_dense_solve(int nx, int ny, int nz, const double* Minv, int N, const T* f,
                               T* x) {
@@ -1047,13 +1273,20 @@ Ops27<T> to27(const Ops<T>& A) {
 
 // =================================================================== wrappers
 
-template <class T, int TY, int KC>
+template <class T, int TY, int KC, int TX>
 void fused_sweep(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
                  const T* in, T* out, int first) {
-  const int nb = ((nx - 2 + kTX - 1) / kTX) * ((ny - 2 + TY - 1) / TY) * ((nz - 2 + KC - 1) / KC);
+  const int nb = ((nx - 2 + TX - 1) / TX) * ((ny - 2 + TY - 1) / TY) * ((nz - 2 + KC - 1) / KC);
   const unsigned g = (unsigned)(((nb + kXCD - 1) / kXCD) * kXCD);
-  k_gsrb7_fused<T, TY, KC><<<g, kFT, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, in, out,
-                                             first);
+  k_gsrb7_fused<T, TY, KC, TX><<<g, kFT, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, in,
+                                                 out, first);
+}
+
+static bool valid_tile(int v) {
+  const int w = v / 10000, rp = v % 10000;
+  const bool shape = rp == 416 || rp == 48 || rp == 816 || rp == 88 || rp == 84 || rp == 1616 ||
+                     rp == 832 || rp == 44 || rp == 68;
+  return shape && (w == 0 || w == 64 || w == 80 || w == 96);
 }
 
 // tile shape of the fused sweep: rows x planes per workgroup
@@ -1064,10 +1297,39 @@ int fused_tile() {
     // precisions at 161^3 and 193^3 (73 / 54 us and 152 / 77 us per sweep)
     if (!e) return 48;
     const int v = atoi(e);
-    return (v == 416 || v == 48 || v == 816 || v == 88 || v == 84 || v == 1616 || v == 832) ? v
-                                                                                          : 48;
+    return valid_tile(v) ? v : 48;
   }();
   return t;
+}
+
+// Tile width for an x extent: the one of 64 / 80 / 96 columns that leaves
+// the fewest idle columns in the last tile (64 on a tie).
+int fused_width(int nx) {
+  int best = 64, waste = 1 << 30;
+  for (int w : {64, 80, 96}) {
+    const int m = ((nx - 2 + w - 1) / w) * w - (nx - 2);
+    if (m < waste) {
+      waste = m;
+      best = w;
+    }
+  }
+  return best;
+}
+
+template <class T, int TX>
+static void fused_sweep_w(hipStream_t s, int rp, int nx, int ny, int nz, const Ops<T>& A,
+                          const T* D, const T* fc, const T* in, T* out, int first) {
+  switch (rp) {
+    case 416: fused_sweep<T, 4, 16, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+    case 48: fused_sweep<T, 4, 8, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+    case 44: fused_sweep<T, 4, 4, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+    case 68: fused_sweep<T, 6, 8, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+    case 88: fused_sweep<T, 8, 8, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+    case 84: fused_sweep<T, 8, 4, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+    case 1616: fused_sweep<T, 16, 16, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+    case 832: fused_sweep<T, 8, 32, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+    default: fused_sweep<T, 8, 16, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+  }
 }
 
 template <class T>
@@ -1076,17 +1338,16 @@ void gsrb7_fused(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T
   const int first = iadjoint ? 0 : 1;
   T* in = x;
   T* out = tmp;
-  const int tile = tile_shape > 0 ? tile_shape : fused_tile();
+  const int tile = tile_shape > 0 && valid_tile(tile_shape) ? tile_shape : fused_tile();
+  const int rp = tile % 10000;
+  const int w = tile / 10000 ? tile / 10000 : fused_width(nx);
   for (int it = 0; it < nu; it++) {
-    switch (tile) {
-      case 416: fused_sweep<T, 4, 16>(s, nx, ny, nz, A, D, fc, in, out, first); break;
-      case 48: fused_sweep<T, 4, 8>(s, nx, ny, nz, A, D, fc, in, out, first); break;
-      case 88: fused_sweep<T, 8, 8>(s, nx, ny, nz, A, D, fc, in, out, first); break;
-      case 84: fused_sweep<T, 8, 4>(s, nx, ny, nz, A, D, fc, in, out, first); break;
-      case 1616: fused_sweep<T, 16, 16>(s, nx, ny, nz, A, D, fc, in, out, first); break;
-      case 832: fused_sweep<T, 8, 32>(s, nx, ny, nz, A, D, fc, in, out, first); break;
-      default: fused_sweep<T, 8, 16>(s, nx, ny, nz, A, D, fc, in, out, first); break;
-    }
+    if (w == 80)
+      fused_sweep_w<T, 80>(s, rp, nx, ny, nz, A, D, fc, in, out, first);
+    else if (w == 96)
+      fused_sweep_w<T, 96>(s, rp, nx, ny, nz, A, D, fc, in, out, first);
+    else
+      fused_sweep_w<T, 64>(s, rp, nx, ny, nz, A, D, fc, in, out, first);
     T* t = in;
     in = out;
     out = t;
@@ -1126,6 +1387,56 @@ void gsrb27(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, 
       k_gsrb27<T><<<g, kBlock, 0, s>>>(nx, ny, nz, o, D, fc, x, c);
     }
 }
+
+// Workgroups of a plane-parity launch with TX x TY tiles.
+static long gs27_groups(int nx, int ny, int nz, int tx, int ty) {
+  return (long)((nx - 2 + tx - 1) / tx) * ((ny - 2 + ty - 1) / ty) * ((nz - 2) / 2);
+}
+
+int gsrb27_planes_tile(int nx, int ny, int nz) {
+  if (gs27_groups(nx, ny, nz, 32, 32) >= 256) return 32;
+  if (gs27_groups(nx, ny, nz, 16, 16) >= 64) return 16;
+  return 8;
+}
+
+template <class T>
+void gsrb27_planes(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D,
+                   const T* fc, T* x, T* xs, int nu, int iadjoint, int tile) {
+  if (nx < 3 || ny < 3 || nz < 3) return;
+  const Ops27<T> o = to27(A);
+  if (tile != 8 && tile != 16 && tile != 32) tile = gsrb27_planes_tile(nx, ny, nz);
+  T* in = x;
+  T* out = xs;
+  for (int it = 0; it < nu; it++) {
+    const int first = iadjoint ? 1 : 0;
+    for (int h = 0; h < 2; h++) {
+      const int ck = h == 0 ? first : 1 - first;
+      const int k0 = ck ? 1 : 2;
+      if (k0 > nz - 2) continue;
+      const int nk = (nz - 2 - k0) / 2 + 1;
+      const T* nb = h == 0 ? in : out;
+      const unsigned tiles = (unsigned)(((nx - 2 + tile - 1) / tile) * ((ny - 2 + tile - 1) / tile));
+      const dim3 g(tiles, (unsigned)nk);
+      if (tile == 32)
+        k_gs27_plane<T, 32, 32><<<g, kBlock, 0, s>>>(nx, ny, nz, o, D, fc, in, nb, out, ck,
+                                                     iadjoint);
+      else if (tile == 16)
+        k_gs27_plane<T, 16, 16><<<g, kBlock, 0, s>>>(nx, ny, nz, o, D, fc, in, nb, out, ck,
+                                                     iadjoint);
+      else
+        k_gs27_plane<T, 8, 8><<<g, kBlock, 0, s>>>(nx, ny, nz, o, D, fc, in, nb, out, ck,
+                                                   iadjoint);
+    }
+    T* t = in;
+    in = out;
+    out = t;
+  }
+  if (in != x) copy_interior<T>(s, nx, ny, nz, in, x);
+}
+
+bool gsrb27_use_planes(int nx, int ny, int nz) { return nx <= 49 && ny <= 49 && nz >= 3; }
+
+int gsrb27_planes_launches(int nz, int nu) { return (nz > 3 ? 2 : 1) * nu + (nu & 1); }
 
 template <class T>
 void resid(hipStream_t s, int st, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
@@ -1245,11 +1556,23 @@ void galerkin(hipStream_t s, int st, int nxf, int nyf, int nzf, const Ops<double
   OutPtrs o;
   for (int k = 0; k < 14; k++) o.a[k] = ac[k];
   const long nq = (long)(nxc - 2) * (nyc - 2) * (nzc - 2);
-  // Group form below kGalPointMin coarse points (measured on the 161^3
-  // hierarchy: 6859 points 143 us grouped vs 187 us per thread; 59319
-  // points 1.12 ms grouped vs 0.42 ms per thread).
-  if (form == 0) form = nq < kGalPointMin ? 2 : 1;
-  if (form == 2) {
+  // Form 0 is the unrolled kernel on every level (measured on MI355X, per
+  // call: 161^3 7-point fine level 118 us unrolled vs 1055 us per thread;
+  // 81^3 27-point 127 vs 408; 41^3 21 vs 171 (147 grouped); 21^3 17.5 vs
+  // 21 grouped).  Forms 1 and 2 stay for the tests.
+  if (form == 0) form = 3;
+  if (form == 3 || form == 4) {
+    const long tot = ((nq + 63) / 64) * 64 * 14;
+    const unsigned nb = (unsigned)((tot + kBlock - 1) / kBlock);
+    if (st == 7 && form == 3)
+      k_galerkin_unrolled<7, 1><<<nb, kBlock, 0, s>>>(nxf, nyf, nzf, to27(Af), nxc, nyc, nzc, o);
+    else if (st == 7)
+      k_galerkin_unrolled<7, 4><<<nb, kBlock, 0, s>>>(nxf, nyf, nzf, to27(Af), nxc, nyc, nzc, o);
+    else if (form == 3)
+      k_galerkin_unrolled<27, 1><<<nb, kBlock, 0, s>>>(nxf, nyf, nzf, to27(Af), nxc, nyc, nzc, o);
+    else
+      k_galerkin_unrolled<27, 4><<<nb, kBlock, 0, s>>>(nxf, nyf, nzf, to27(Af), nxc, nyc, nzc, o);
+  } else if (form == 2) {
     const long groups = nq * 14;
     k_galerkin_group<<<(unsigned)((groups + kGalGroups - 1) / kGalGroups), kBlock, 0, s>>>(
         st, nxf, nyf, nzf, to27(Af), nxc, nyc, nzc, o);
@@ -1265,7 +1588,11 @@ void dense_assemble(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A,
   k_dense_assemble<<<(N + kBlock - 1) / kBlock, kBlock, 0, s>>>(nx, ny, nz, to27(A), D, M, N);
 }
 
-void dense_invert(hipStream_t s, double* M, double* work, int N) {
+void dense_invert(hipStream_t s, double* M, double* work, int N, int form) {
+  if (N <= 64 && form != 1) {
+    k_dense_invert_reg<<<1, 256, 0, s>>>(M, N);
+    return;
+  }
   const size_t lds = ((size_t)N * N + 2 * N) * sizeof(double);
   if (lds <= 64 * 1024 && (size_t)N * N <= (size_t)kInvPer * 1024) {
     k_dense_invert_lds<<<1, 1024, lds, s>>>(M, N);
@@ -1295,6 +1622,8 @@ void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int 
                          int);                                                                  \
   template void gsrb27<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*, T*, int, \
                           int);                                                                 \
+  template void gsrb27_planes<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*,  \
+                                 T*, T*, int, int, int);                                        \
   template void resid<T>(hipStream_t, int, int, int, int, const Ops<T>&, const T*, const T*,     \
                          const T*, T*);                                                         \
   template void restrict3<T>(hipStream_t, int, int, int, int, int, int, const T*, T*);          \

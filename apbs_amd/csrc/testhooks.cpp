@@ -83,6 +83,86 @@ double fused_sweeps_us(int n, int sweeps, int tile) {
   return 1e3 * ms / sweeps;
 }
 
+// The 27-point 8-colour sweeps alone on an n^3 coarse-level operator: mode 0
+// one launch per colour, mode 1 one per plane parity (tile by level size),
+// 8 / 16 / 32 one per plane parity with that tile.  One untimed call, then
+// one timed call of `sweeps` sweeps; microseconds per sweep.
+template <class T>
+double gs27_sweeps_us(int n, int sweeps, int mode) {
+  const size_t N = (size_t)n * n * n;
+  std::vector<T> h(N);
+  T* a[14];
+  T *D, *fc, *x, *xs;
+  for (int s = 0; s < 14; s++) {
+    TCHECK(hipMalloc(&a[s], N * sizeof(T)));
+    for (size_t i = 0; i < N; i++) h[i] = s == 0 ? T(26) : T(1);
+    TCHECK(hipMemcpy(a[s], h.data(), N * sizeof(T), hipMemcpyHostToDevice));
+  }
+  T** bufs[4] = {&D, &fc, &x, &xs};
+  const T val[4] = {T(26), T(1), T(0), T(0)};
+  for (int b = 0; b < 4; b++) {
+    TCHECK(hipMalloc(bufs[b], N * sizeof(T)));
+    for (size_t i = 0; i < N; i++) h[i] = val[b];
+    TCHECK(hipMemcpy(*bufs[b], h.data(), N * sizeof(T), hipMemcpyHostToDevice));
+  }
+  pmg::Ops<T> ops{};
+  for (int s = 0; s < 14; s++) ops.a[s] = a[s];
+  auto run = [&](int nu) {
+    if (mode == 0)
+      pmg::gsrb27<T>(0, n, n, n, ops, D, fc, x, nu, 0);
+    else
+      pmg::gsrb27_planes<T>(0, n, n, n, ops, D, fc, x, xs, nu, 0, mode);
+  };
+  run(2);
+  hipEvent_t e0, e1;
+  TCHECK(hipEventCreate(&e0));
+  TCHECK(hipEventCreate(&e1));
+  TCHECK(hipEventRecord(e0, 0));
+  run(sweeps);
+  TCHECK(hipEventRecord(e1, 0));
+  TCHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  TCHECK(hipEventElapsedTime(&ms, e0, e1));
+  TCHECK(hipEventDestroy(e0));
+  TCHECK(hipEventDestroy(e1));
+  for (int s = 0; s < 14; s++) TCHECK(hipFree(a[s]));
+  for (int b = 0; b < 4; b++) TCHECK(hipFree(*bufs[b]));
+  return 1e3 * ms / sweeps;
+}
+
+// Galerkin coarse operator of a synthetic nf^3 operator (stencil st) in the
+// given kernel form: one untimed call, then reps timed calls; microseconds
+// per call.
+double galerkin_us(int nf, int st, int form, int reps) {
+  const int nc = (nf - 1) / 2 + 1;
+  const size_t Nf = (size_t)nf * nf * nf, Nc = (size_t)nc * nc * nc;
+  const int na = st == 7 ? 4 : 14;
+  std::vector<double> h(Nf);
+  pmg::Ops<double> ops{};
+  for (int s = 0; s < 14; s++) ops.a[s] = nullptr;
+  for (int s = 0; s < na; s++) {
+    for (size_t i = 0; i < Nf; i++) h[i] = s == 0 ? 26.0 : 1.0 + 1e-3 * (double)(i % 7);
+    ops.a[s] = up(h.data(), Nf);
+  }
+  double* ac[14];
+  for (int s = 0; s < 14; s++) ac[s] = up(nullptr, Nc);
+  pmg::galerkin(0, st, nf, nf, nf, ops, nc, nc, nc, ac, form);
+  hipEvent_t e0, e1;
+  TCHECK(hipEventCreate(&e0));
+  TCHECK(hipEventCreate(&e1));
+  TCHECK(hipEventRecord(e0, 0));
+  for (int r = 0; r < reps; r++) pmg::galerkin(0, st, nf, nf, nf, ops, nc, nc, nc, ac, form);
+  TCHECK(hipEventRecord(e1, 0));
+  TCHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  TCHECK(hipEventElapsedTime(&ms, e0, e1));
+  TCHECK(hipEventDestroy(e0));
+  TCHECK(hipEventDestroy(e1));
+  for (int s = 0; s < na; s++) TCHECK(hipFree((void*)ops.a[s]));
+  for (int s = 0; s < 14; s++) TCHECK(hipFree(ac[s]));
+  return 1e3 * ms / reps;
+}
+
 extern "C" {
 
 // Calibration for hardware byte counters: reps copies of n doubles (n*8
@@ -109,6 +189,44 @@ int pmgtest_fused_sweeps(int n, int word, int sweeps, int tile, double* us_per_s
 
 int gsrb_impl(int nx, int ny, int nz, int st, const double* a, const double* cc, const double* fc,
               double* x, int nu, int iadjoint, int tile);
+
+// Inverse of a dense N x N matrix (row-major) in the given form of
+// dense_invert (0 by size, 1 without the register form)
+int pmgtest_dense_invert(int N, const double* M, double* out, int form) {
+  if (N <= 0 || N > 4096 || form < 0 || form > 1) return 0;
+  const size_t nn = (size_t)N * N;
+  double *dM = up(M, nn), *dw = up(nullptr, (size_t)N);
+  pmg::dense_invert(0, dM, dw, N, form);
+  TCHECK(hipDeviceSynchronize());
+  down(out, dM, nn);
+  TCHECK(hipFree(dM));
+  TCHECK(hipFree(dw));
+  return 1;
+}
+
+// Average microseconds per Galerkin call on a synthetic nf^3 level (see
+// galerkin_us); form as pmgtest_galerkin_form
+int pmgtest_galerkin_time(int nf, int st, int form, int reps, double* us) {
+  if (nf < 5 || (nf & 1) == 0 || (st != 7 && st != 27) || reps <= 0 || form < 0 || form > 4) return 0;
+  *us = galerkin_us(nf, st, form, reps);
+  return 1;
+}
+
+// Average microseconds per 27-point sweep on an n^3 level (see gs27_sweeps_us)
+int pmgtest_gs27_sweeps(int n, int word, int sweeps, int mode, double* us_per_sweep) {
+  if (n < 3 || sweeps <= 0 || (word != 4 && word != 8)) return 0;
+  *us_per_sweep = word == 8 ? gs27_sweeps_us<double>(n, sweeps, mode)
+                            : gs27_sweeps_us<float>(n, sweeps, mode);
+  return 1;
+}
+
+// 27-point sweeps with the kernel form forced: mode 0 one launch per colour,
+// 1 per plane parity (tile by level size), 8 / 16 / 32 per plane parity with
+// that tile
+int pmgtest_gsrb27_mode(int nx, int ny, int nz, const double* a, const double* cc,
+                        const double* fc, double* x, int nu, int iadjoint, int mode) {
+  return gsrb_impl(nx, ny, nz, 27, a, cc, fc, x, nu, iadjoint, mode);
+}
 
 // x <- nu sweeps (7-point red-black or 27-point 8-colour), D = oC + cc
 int pmgtest_gsrb(int nx, int ny, int nz, int st, const double* a, const double* cc,
@@ -137,8 +255,16 @@ int gsrb_impl(int nx, int ny, int nz, int st, const double* a, const double* cc,
     pmg::gsrb7_fused<double>(0, nx, ny, nz, A.ops, dD, dfc, dx, dt, nu, iadjoint, tile);
     TCHECK(hipDeviceSynchronize());
     TCHECK(hipFree(dt));
-  } else
+  } else if (tile == 0) {
     pmg::gsrb27<double>(0, nx, ny, nz, A.ops, dD, dfc, dx, nu, iadjoint);
+  } else {
+    // the plane-parity sweeps; the partner buffer carries the boundary of x
+    double* dt = up(x, n);
+    pmg::gsrb27_planes<double>(0, nx, ny, nz, A.ops, dD, dfc, dx, dt, nu, iadjoint,
+                               tile == 1 ? 0 : tile);
+    TCHECK(hipDeviceSynchronize());
+    TCHECK(hipFree(dt));
+  }
   TCHECK(hipDeviceSynchronize());
   down(x, dx, n);
   TCHECK(hipFree(dcc));
@@ -189,7 +315,7 @@ int pmgtest_prolong(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf, const 
 
 int pmgtest_galerkin_form(int nxf, int nyf, int nzf, int st, const double* af, int nxc, int nyc,
                           int nzc, double* ac, int form) {
-  if (form < 0 || form > 2) return 0;
+  if (form < 0 || form > 4) return 0;
   const size_t nf = (size_t)nxf * nyf * nzf, nc = (size_t)nxc * nyc * nzc;
   OpsHolder A(af, nf, st);
   double* out[14];

@@ -636,6 +636,9 @@ struct DLevel {
   // FP32 copies for the mixed-precision correction cycles
   float* af[14] = {};
   float *Df = nullptr, *xfl = nullptr, *w0f = nullptr;
+  // coarse-level ping-pong partners of the 27-point smoother (zero boundary)
+  double* xs = nullptr;
+  float* xsf = nullptr;
 
   pmg::Ops<double> ops() const {
     pmg::Ops<double> o;
@@ -704,6 +707,10 @@ struct PmgDev {
   // fine-level smoother: 0 fused out-of-place sweep, 1 one launch per colour
   // (environment APBS_PMG_SMOOTHER=colour selects 1)
   int smoother = 0;
+  // coarse-level smoother: 0 one launch per plane parity on the levels where
+  // that is faster (gsrb27_use_planes), 1 one launch per colour everywhere
+  // (environment APBS_PMG_SMOOTHER27=colour selects 1)
+  int smoother27 = 0;
   int mirror = 1;
   double stats[8] = {};
   pmg::Geo geo{};
@@ -882,6 +889,10 @@ void free_hierarchy(PmgDev* d) {
     dfree(L.Df);
     dfree(L.xfl);
     dfree(L.w0f);
+    dfree(L.xs);
+    dfree(L.xsf);
+    L.xs = nullptr;
+    L.xsf = nullptr;
   }
   dfree(d->w1);
   dfree(d->w1f);
@@ -954,9 +965,9 @@ size_t pmgint::footprint_bytes(int nx, int ny, int nz, int nlev, bool newton, bo
   for (int l = 0; l < nt; l++) {
     const double m = (double)dims[l][0] * dims[l][1] * dims[l][2];
     const int na = l == 0 ? 4 : 14;
-    h += (na + 4 + (l > 0 ? 1 : 0)) * blk(8 * m);     // ops, cc, fc, D, w0 (+x)
+    h += (na + 4 + (l > 0 ? 2 : 0)) * blk(8 * m);     // ops, cc, fc, D, w0 (+x, xs)
     if (newton) h += 2 * blk(8 * m);                 // cprime, xtmp
-    if (mixed) h += (na - 1 + 3) * blk(4 * m);       // af, Df, xfl, w0f
+    if (mixed) h += (na - 1 + 3 + (l > 0 ? 1 : 0)) * blk(4 * m);  // af, Df, xfl, w0f (+xsf)
   }
   h += 3 * blk(8 * n);                               // w1, F, xs64
   if (newton) h += 4 * blk(8 * n);                   // wn1..3, nlw
@@ -1000,6 +1011,10 @@ void alloc_hierarchy(PmgDev* d, int nlev, bool newton, bool mixed) {
     pmg::fill<double>(d->s, L.fc, L.n, 0.0);
     pmg::fill<double>(d->s, L.D, L.n, 0.0);
     L.x = l == 0 ? d->u : dalloc<double>(L.n);
+    if (l > 0) {
+      L.xs = dalloc<double>(L.n);
+      pmg::fill<double>(d->s, L.xs, L.n, 0.0);
+    }
     L.w0 = dalloc<double>(L.n);
     pmg::fill<double>(d->s, L.w0, L.n, 0.0);
     if (l > 0) pmg::fill<double>(d->s, L.x, L.n, 0.0);
@@ -1018,6 +1033,10 @@ void alloc_hierarchy(PmgDev* d, int nlev, bool newton, bool mixed) {
       pmg::fill<float>(d->s, L.Df, L.n, 0.0f);
       L.xfl = dalloc<float>(L.n);
       L.w0f = dalloc<float>(L.n);
+      if (l > 0) {
+        L.xsf = dalloc<float>(L.n);
+        pmg::fill<float>(d->s, L.xsf, L.n, 0.0f);
+      }
       pmg::fill<float>(d->s, L.xfl, L.n, 0.0f);
       pmg::fill<float>(d->s, L.w0f, L.n, 0.0f);
     }
@@ -1102,6 +1121,7 @@ struct CycleArgs {
   const T* Dl[24];
   pmg::Ops<T> A[24];
   T* w0[24];
+  T* xsl[24];  // coarse-level smoother partner buffers
   const T* f;
   T* w1;
   T* xs;  // fine-size smoother partner buffer
@@ -1127,6 +1147,13 @@ void vcycle(PmgDev* d, CycleArgs<T>& c) {
       tend(d, e0, 0, pmg::gsrb7_launches(2));
     }
   };
+  auto coarse_smooth = [&](int l, int adj) {
+    if (d->smoother27 == 0 && pmg::gsrb27_use_planes(L[l].nx, L[l].ny, L[l].nz))
+      pmg::gsrb27_planes<T>(s, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l],
+                            c.xsl[l], 2, adj);
+    else
+      pmg::gsrb27<T>(s, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l], 2, adj);
+  };
   const int ecyc = tbegin(d);
   fine_smooth(0);
   pmg::resid<T>(s, 7, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], c.w1);
@@ -1135,7 +1162,7 @@ void vcycle(PmgDev* d, CycleArgs<T>& c) {
                       c.w0[l]);
     if (l != nt - 1) {
       pmg::fill<T>(s, c.X[l], L[l].n, T(0));
-      pmg::gsrb27<T>(s, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l], 2, 0);
+      coarse_smooth(l, 0);
       pmg::resid<T>(s, 27, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l], c.w1);
     }
   }
@@ -1150,7 +1177,7 @@ void vcycle(PmgDev* d, CycleArgs<T>& c) {
     if (l == 0)
       fine_smooth(1);
     else
-      pmg::gsrb27<T>(s, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l], 2, 1);
+      coarse_smooth(l, 1);
   }
   tend(d, ecyc, 1, 1);
 }
@@ -1190,6 +1217,7 @@ CycleArgs<double> args64(PmgDev* d, double* const* X, const double* f) {
     c.Dl[l] = d->L[l].D;
     c.A[l] = d->L[l].ops();
     c.w0[l] = d->L[l].w0;
+    c.xsl[l] = d->L[l].xs;
   }
   c.f = f;
   c.w1 = d->w1;
@@ -1238,6 +1266,7 @@ int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int i
     c.Dl[l] = d->L[l].Df;
     c.A[l] = d->L[l].opsf();
     c.w0[l] = d->L[l].w0f;
+    c.xsl[l] = d->L[l].xsf;
   }
   c.w1 = d->w1f;
   c.xs = d->xs32;
@@ -1398,6 +1427,8 @@ extern "C" int Vpmg_ctor2(Vpmg* thee, Vpmgp* pmgp, Vpbe* pbe, int focusFlag, Vpm
   thee->dev = d;
   d->s = pmgpool::stream_acquire();
   if (const char* sm = getenv("APBS_PMG_SMOOTHER")) d->smoother = strcmp(sm, "colour") == 0 ? 1 : 0;
+  if (const char* sm = getenv("APBS_PMG_SMOOTHER27"))
+    d->smoother27 = strcmp(sm, "colour") == 0 ? 1 : 0;
   if (const char* gr = getenv("APBS_PMG_GRAPHS")) d->graphs = strcmp(gr, "0") == 0 ? 0 : 1;
   d->nx = pmgp->nx;
   d->ny = pmgp->ny;

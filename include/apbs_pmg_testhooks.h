@@ -28,7 +28,9 @@ int pmgtest_prolong(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf, const 
 int pmgtest_galerkin(int nxf, int nyf, int nzf, int stencil, const double *af, int nxc, int nyc,
                      int nzc, double *ac);
 /* The same with the kernel form forced: 0 by level size, 1 one thread per
- * coefficient, 2 one 32-lane group per coefficient (both give the same bits) */
+ * coefficient, 2 one 32-lane group per coefficient, 3 one thread per
+ * coefficient with the loops unrolled at compile time (all give the same
+ * bits) */
 int pmgtest_galerkin_form(int nxf, int nyf, int nzf, int stencil, const double *af, int nxc,
                           int nyc, int nzc, double *ac, int form);
 /* VbuildA_fv (src/pmgc/buildAd.c:105-281) */
@@ -45,6 +47,22 @@ int pmgtest_nonlinear(int nx, int ny, int nz, const double *a, const double *cc,
  * (rows*100 + planes: 416, 48, 816, 88, 84, 1616, 832). */
 int pmgtest_gsrb_tile(int nx, int ny, int nz, const double *a, const double *cc,
                       const double *fc, double *x, int nu, int iadjoint, int tile);
+/* pmgtest_gsrb (27-point) with the kernel form forced: mode 0 one launch per
+ * colour, 1 one launch per plane parity (tile by level size), 8 / 16 / 32 one
+ * launch per plane parity with that square tile.  All give the same bits. */
+int pmgtest_gsrb27_mode(int nx, int ny, int nz, const double *a, const double *cc,
+                        const double *fc, double *x, int nu, int iadjoint, int mode);
+/* No reference counterpart: the 27-point sweeps alone on a synthetic n^3
+ * coarse-level operator, word 8 or 4, mode as above; average microseconds
+ * per sweep in *us_per_sweep. */
+int pmgtest_gs27_sweeps(int n, int word, int sweeps, int mode, double *us_per_sweep);
+/* No reference counterpart (the reference factors its coarsest operator in
+ * band form, mgsubd.c:244-255): the dense inverse of the coarsest level's
+ * operator, form 0 (by size: registers for N <= 64) or 1 (LDS / global). */
+int pmgtest_dense_invert(int N, const double *M, double *out, int form);
+/* No reference counterpart: microseconds per Galerkin call (form as above)
+ * on a synthetic nf^3 operator of stencil st (nf odd). */
+int pmgtest_galerkin_time(int nf, int st, int form, int reps, double *us);
 /* No reference counterpart: reps device copies of n doubles, the known-size
  * calibration for hardware byte counters (FETCH_SIZE / WRITE_SIZE). */
 int pmgtest_stream_copy(size_t n, int reps);

@@ -27,7 +27,7 @@ def declared_symbols():
             names.add(name)
     # keep only names that look like API functions
     return sorted(n for n in names
-                  if re.match(r"^(Vpmg|Vpmgp|Vpbe|Valist|MGparm|apbs_pmg|pmgtest|PmgServer)_", n))
+                  if re.match(r"^(Vpmg|Vpmgp|Vpbe|Valist|Vgrid|MGparm|apbs_pmg|pmgtest|PmgServer)_", n))
 
 
 @pytest.fixture(scope="module")

@@ -71,7 +71,31 @@ def test_gsrb_bitwise(L, O, st, n3, adj):
     assert np.array_equal(xg, xo)
 
 
-@pytest.mark.parametrize("tile", [416, 48, 816, 88, 84, 1616, 832])
+@pytest.mark.parametrize("n3", [(17, 17, 17), (9, 11, 13), (45, 43, 41), (5, 5, 4),
+                                (70, 21, 6), (81, 81, 9), (3, 3, 3)])
+@pytest.mark.parametrize("mode", [1, 8, 16, 32])
+def test_gsrb27_plane_sweeps_bitwise(L, O, n3, mode):
+    """The plane-parity 27-point sweep (one launch per plane parity, tiles
+    with recomputed margins) gives the oracle's 8-colour sweep bit for bit:
+    every tile size, partial tiles, forward and adjoint, even and odd sweep
+    counts."""
+    rng = np.random.default_rng(sum(n3) + mode)
+    nx, ny, nz = n3
+    n = nx * ny * nz
+    a = rand_op(rng, n3, 27)
+    cc, fc = rng.uniform(0, 1, n), rng.normal(size=n)
+    x0 = np.zeros(n)
+    x0.reshape(nz, ny, nx)[1:-1, 1:-1, 1:-1] = rng.normal(size=(nz - 2, ny - 2, nx - 2))
+    for adj in (0, 1):
+        for nu in (2, 3):
+            xg, xo = x0.copy(), x0.copy()
+            L.pmgtest_gsrb27_mode(nx, ny, nz, P(a), P(cc), P(fc), P(xg), nu, adj, mode)
+            O.orc_gsrb(nx, ny, nz, 27, P(a), P(cc), P(fc), P(xo), nu, adj, 1)
+            assert np.array_equal(xg, xo), (n3, mode, adj, nu)
+
+
+@pytest.mark.parametrize("tile", [416, 48, 816, 88, 84, 1616, 832, 44, 68, 640048, 800048,
+                                  960048, 800044, 800068, 960416, 800816])
 def test_fused_sweep_tiles_bitwise(L, O, tile):
     """Every tile shape of the fused red-black sweep gives the oracle's
     sweep bit for bit (partial tiles in every direction)."""
@@ -124,14 +148,16 @@ def test_transfers_bitwise(L, O):
     assert np.array_equal(interior(fg, f3), interior(fo, f3))
 
 
-@pytest.mark.parametrize("form", [0, 1, 2])
+@pytest.mark.parametrize("form", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("st,f3,c3", [(7, (17, 17, 17), (9, 9, 9)),
                                       (27, (17, 17, 17), (9, 9, 9)),
+                                      (7, (65, 57, 61), (33, 29, 31)),
                                       (27, (65, 57, 61), (33, 29, 31))])
 def test_galerkin_bitwise(L, O, st, f3, c3, form):
-    """Both kernel forms (one thread, one 32-lane group per coefficient) and
-    the size-based choice; the 33x29x31 level has more coarse points than the
-    group form's cut-off, so form 0 takes the per-thread kernel there."""
+    """Every kernel form (one thread, one 32-lane group, one thread with the
+    loops unrolled at compile time per coefficient) and the size-based
+    choice; the 33x29x31 level has more coarse points than the group form's
+    cut-off, so form 0 takes a per-thread kernel there."""
     rng = np.random.default_rng(5 + st)
     nf, nc = np.prod(f3), np.prod(c3)
     af = rand_op(rng, f3, st)
@@ -188,3 +214,24 @@ def test_nonlinear_terms(L, O):
     scale = np.abs(ri).max()
     assert np.max(np.abs(rgi - ri)) <= 8 * np.finfo(float).eps * scale
     assert np.allclose(dg, do, rtol=4 * np.finfo(float).eps, atol=0)
+
+
+@pytest.mark.parametrize("N", [64, 27, 8, 125])
+def test_dense_inverse_forms(L, N):
+    """The coarsest level's dense inverse: the register form (N <= 64) and
+    the LDS / global form agree to 1e-13 relative and invert an SPD matrix
+    like the coarsest 27-point operator (diagonally dominant, symmetric)."""
+    rng = np.random.default_rng(N)
+    A = rng.uniform(-1, 0, (N, N))
+    A = 0.5 * (A + A.T)
+    np.fill_diagonal(A, 0.0)
+    A[np.diag_indices(N)] = -A.sum(axis=1) + rng.uniform(0.5, 2.0, N)
+    A = np.ascontiguousarray(A)
+    outs = []
+    for form in (0, 1):
+        out = np.zeros((N, N))
+        assert L.pmgtest_dense_invert(N, P(A), P(out), form) == 1
+        assert np.abs(A @ out - np.eye(N)).max() < 1e-12
+        outs.append(out)
+    assert np.abs(outs[0] - outs[1]).max() <= 1e-13 * np.abs(outs[1]).max()
+
