@@ -586,14 +586,28 @@ __global__ void __launch_bounds__(kB)
   const int kmax = imin2(g.nz - 1, (int)floor((posz + zr) / g.hz));
   const int ni = imax - imin + 1, nj = jmax - jmin + 1, nk = kmax - kmin + 1;
   if (ni <= 0 || nj <= 0 || nk <= 0) return;
+  // lane e of the box walks e, e + 64, ...: the (i, j, k) offsets advance by
+  // 64 % ni columns and 64 / ni rows per step, carried without divisions
   const int nbox = ni * nj * nk;
+  int ii = lane % ni, jj = (lane / ni) % nj, kk = lane / (ni * nj);
+  const int di = 64 % ni, dr = 64 / ni;
   for (int e = lane; e < nbox; e += 64) {
-    const int i = imin + e % ni, j = jmin + (e / ni) % nj, k = kmin + e / (ni * nj);
+    const int i = imin + ii, j = jmin + jj, k = kmin + kk;
     const double fi = i, fj = j, fk = k;
     const double dx2 = (posx - g.hx * fi) * (posx - g.hx * fi);
     const double dy2 = (posy - g.hy * fj) * (posy - g.hy * fj);
     const double dz2 = (posz - g.hz * fk) * (posz - g.hz * fk);
     if ((dz2 + dy2 + dx2) <= rtot2) arr[IX(i, j, k, g.nx, g.ny)] = val;
+    ii += di;
+    jj += dr;
+    if (ii >= ni) {
+      ii -= ni;
+      jj++;
+    }
+    while (jj >= nj) {
+      jj -= nj;
+      kk++;
+    }
   }
 }
 

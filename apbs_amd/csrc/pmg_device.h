@@ -61,12 +61,22 @@ template <class T>
 void gsrb27_planes(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D,
                    const T* fc, T* x, T* xs, int nu, int iadjoint, int tile = 0);
 int gsrb27_planes_tile(int nx, int ny, int nz);
-// Whether the plane-parity form is the faster one for a level: it trades a
-// quarter of the launches for four dependent colour passes per workgroup,
-// which pays on levels up to about 49^2 points per plane (measured on MI355X:
-// 41^3 22 vs 29 us per FP64 sweep, 21^3 16 vs 22, 11^3 14 vs 22; at 81^3 the
-// per-colour launches are faster, 63 vs 75).
+// The same 8-colour sweeps in place with the two colours of each (j, k)
+// parity (c, c ^ 1) in one launch, one wavefront per row: 4 launches per
+// sweep.  Rows of up to 128 interior points (nx <= 130).  Bitwise gsrb27.
+template <class T>
+void gsrb27_pairs(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D,
+                  const T* fc, T* x, int nu, int iadjoint);
+// Whether the plane-parity form is the one for a level: it trades a quarter
+// of the launches for four dependent colour passes per workgroup, which pays
+// on levels up to about 49^2 points per plane (measured on MI355X, FP64 us
+// per sweep: 41^3 22 against 29 per colour, 21^3 16 against 22, 11^3 14
+// against 22; at 81^3 the per-colour launches are faster, 63 against 75).
 bool gsrb27_use_planes(int nx, int ny, int nz);
+// Whether the colour-pair form is the one for a level that is not
+// plane-parity: rows of up to 128 interior points (81^3: 40 against 62 us per
+// FP64 sweep, 28 against 40 FP32; 97^3: 79 against 87; 65^3: 23 against 34).
+bool gsrb27_use_pairs(int nx);
 int gsrb27_planes_launches(int nz, int nu);
 
 // ------------------------------------------------------- residual / matvec

@@ -311,6 +311,81 @@ __global__ void __launch_bounds__(kBlock)
   x[p] = (fc[p] + (s.o + s.u + s.d)) / D[p];
 }
 
+// Two colours of that sweep in one launch: A = c and B = c ^ 1, which differ
+// only in the parity of i and follow each other in both sweep orders.  An
+// A point and a B point are neighbours only along i within one row, and the
+// rows of one (j, k) parity do not touch each other, so a wavefront owns a
+// row: lane t updates the A point iA0 + 2t, writes it, and takes the new A
+// values beside its B point iB0 + 2t from the neighbouring lanes.  Rows of up
+// to 128 interior points.  The arithmetic is part27's, term for term.
+template <class T>
+__device__ inline T lane_value(T v, int src) {
+  return __shfl(v, src < 0 ? 0 : (src > 63 ? 63 : src), 64);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_gsrb27_pair(int nx, int ny, int nz, Ops27<T> A, const T* __restrict__ D,
+                  const T* __restrict__ fc, T* x, int pa, int cj, int ck) {
+  const T *oE = A.a[1], *oN = A.a[2], *uC = A.a[3], *oNE = A.a[4], *oNW = A.a[5], *uE = A.a[6],
+          *uW = A.a[7], *uN = A.a[8], *uS = A.a[9], *uNE = A.a[10], *uNW = A.a[11],
+          *uSE = A.a[12], *uSW = A.a[13];
+  const int j0 = cj ? 1 : 2, k0 = ck ? 1 : 2;
+  const int nj = (ny - 2 - j0) / 2 + 1;
+  const int row = (int)((blockIdx.x * (unsigned)kBlock + threadIdx.x) / 64);
+  const int lane = (int)(threadIdx.x % 64);
+  const int nk = (nz - 2 - k0) / 2 + 1;
+  if (row >= nj * nk) return;  // uniform per wavefront
+  const int j = j0 + 2 * (row % nj), k = k0 + 2 * (row / nj);
+  const size_t nxy = (size_t)nx * ny;
+  const size_t rowp = (size_t)k * nxy + (size_t)j * nx;
+  const int iA = (pa ? 1 : 2) + 2 * lane, iB = (pa ? 2 : 1) + 2 * lane;
+  T a = 0;
+  if (iA <= nx - 2) {
+    const size_t p = rowp + iA;
+    P27<T> r = part27<T>(A.a, x, p, nx, nxy);
+    a = (fc[p] + (r.o + r.u + r.d)) / D[p];
+    x[p] = a;
+  }
+  // new A values at iB - 1 and iB + 1: lanes t - 1 and t when A is even,
+  // t and t + 1 when A is odd
+  const T am = lane_value(a, pa ? lane : lane - 1);
+  const T ap = lane_value(a, pa ? lane + 1 : lane);
+  if (iB > nx - 2) return;
+  const size_t p = rowp + iB;
+  const T xm = (iB - 1 >= 1) ? am : x[p - 1];
+  const T xp = (iB + 1 <= nx - 2) ? ap : x[p + 1];
+  T o = oN[p] * x[p + nx];
+  o += oN[p - nx] * x[p - nx];
+  o += oE[p] * xp;
+  o += oE[p - 1] * xm;
+  o += oNE[p] * x[p + nx + 1];
+  o += oNW[p] * x[p + nx - 1];
+  o += oNW[p - nx + 1] * x[p - nx + 1];
+  o += oNE[p - nx - 1] * x[p - nx - 1];
+  const size_t q = p + nxy;
+  T u = uC[p] * x[q];
+  u += uN[p] * x[q + nx];
+  u += uS[p] * x[q - nx];
+  u += uE[p] * x[q + 1];
+  u += uW[p] * x[q - 1];
+  u += uNE[p] * x[q + nx + 1];
+  u += uNW[p] * x[q + nx - 1];
+  u += uSE[p] * x[q - nx + 1];
+  u += uSW[p] * x[q - nx - 1];
+  const size_t m = p - nxy;
+  T d = uC[m] * x[m];
+  d += uS[m + nx] * x[m + nx];
+  d += uN[m - nx] * x[m - nx];
+  d += uW[m + 1] * x[m + 1];
+  d += uE[m - 1] * x[m - 1];
+  d += uSW[m + nx + 1] * x[m + nx + 1];
+  d += uSE[m + nx - 1] * x[m + nx - 1];
+  d += uNW[m - nx + 1] * x[m - nx + 1];
+  d += uNE[m - nx - 1] * x[m - nx - 1];
+  x[p] = (fc[p] + (o + u + d)) / D[p];
+}
+
 // The four colours of one plane parity of that sweep in one launch: colours
 // 4ck .. 4ck+3 (4ck+3 .. 4ck for the adjoint sweep).  Planes of one parity do
 // not touch each other, so a workgroup owns a TX x TY tile of one plane.  It
@@ -321,8 +396,8 @@ __global__ void __launch_bounds__(kBlock)
 // tile is written.  In-plane values come from src, the neighbouring planes
 // from nbr, and the result goes to dst, so no workgroup reads what another
 // writes.  The arithmetic is part27's, term for term.
-template <class T, int TX, int TY>
-__global__ void __launch_bounds__(kBlock)
+template <class T, int TX, int TY, int NTH>
+__global__ void __launch_bounds__(NTH)
     k_gs27_plane(int nx, int ny, int nz, Ops27<T> A, const T* __restrict__ D,
                  const T* __restrict__ fc, const T* __restrict__ src, const T* __restrict__ nbr,
                  T* __restrict__ dst, int ck, int adj) {
@@ -338,7 +413,7 @@ __global__ void __launch_bounds__(kBlock)
   const size_t nxy = (size_t)nx * ny;
   const size_t kp = (size_t)k * nxy;
   const int bx = tx0 - M, by = ty0 - M;  // grid coordinates of sx[0]
-  for (int t = threadIdx.x; t < LX * LY; t += kBlock) {
+  for (int t = threadIdx.x; t < LX * LY; t += NTH) {
     const int i = bx + t % LX, j = by + t / LX;
     sx[t] = (i >= 0 && i < nx && j >= 0 && j < ny) ? src[kp + (size_t)j * nx + i] : T(0);
   }
@@ -354,7 +429,7 @@ __global__ void __launch_bounds__(kBlock)
     jlo += (jlo & 1) != pj;
     const int ni = ilo <= ihi ? (ihi - ilo) / 2 + 1 : 0;
     const int nj = jlo <= jhi ? (jhi - jlo) / 2 + 1 : 0;
-    for (int t = threadIdx.x; t < ni * nj; t += kBlock) {
+    for (int t = threadIdx.x; t < ni * nj; t += NTH) {
       const int i = ilo + 2 * (t % ni), j = jlo + 2 * (t / ni);
       const int e = (j - by) * LX + (i - bx);
       const size_t p = kp + (size_t)j * nx + i;
@@ -390,7 +465,7 @@ __global__ void __launch_bounds__(kBlock)
     }
     __syncthreads();
   }
-  for (int t = threadIdx.x; t < TX * TY; t += kBlock) {
+  for (int t = threadIdx.x; t < TX * TY; t += NTH) {
     const int i = tx0 + t % TX, j = ty0 + t / TX;
     if (i <= nx - 2 && j <= ny - 2) dst[kp + (size_t)j * nx + i] = sx[(j - by) * LX + (i - bx)];
   }
@@ -1388,6 +1463,23 @@ void gsrb27(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, 
     }
 }
 
+template <class T>
+void gsrb27_pairs(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D,
+                  const T* fc, T* x, int nu, int iadjoint) {
+  const Ops27<T> o = to27(A);
+  for (int it = 0; it < nu; it++)
+    for (int h = 0; h < 4; h++) {
+      // forward: (0,1) (2,3) (4,5) (6,7); adjoint: (7,6) (5,4) (3,2) (1,0)
+      const int c = iadjoint ? 7 - 2 * h : 2 * h;
+      const int pa = c & 1, cj = (c >> 1) & 1, ck = (c >> 2) & 1;
+      const int j0 = cj ? 1 : 2, k0 = ck ? 1 : 2;
+      if (j0 > ny - 2 || k0 > nz - 2) continue;
+      const long rows = (long)((ny - 2 - j0) / 2 + 1) * ((nz - 2 - k0) / 2 + 1);
+      const unsigned g = (unsigned)((rows * 64 + kBlock - 1) / kBlock);
+      k_gsrb27_pair<T><<<g, kBlock, 0, s>>>(nx, ny, nz, o, D, fc, x, pa, cj, ck);
+    }
+}
+
 // Workgroups of a plane-parity launch with TX x TY tiles.
 static long gs27_groups(int nx, int ny, int nz, int tx, int ty) {
   return (long)((nx - 2 + tx - 1) / tx) * ((ny - 2 + ty - 1) / ty) * ((nz - 2) / 2);
@@ -1418,14 +1510,14 @@ void gsrb27_planes(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const
       const unsigned tiles = (unsigned)(((nx - 2 + tile - 1) / tile) * ((ny - 2 + tile - 1) / tile));
       const dim3 g(tiles, (unsigned)nk);
       if (tile == 32)
-        k_gs27_plane<T, 32, 32><<<g, kBlock, 0, s>>>(nx, ny, nz, o, D, fc, in, nb, out, ck,
-                                                     iadjoint);
+        k_gs27_plane<T, 32, 32, kBlock><<<g, kBlock, 0, s>>>(nx, ny, nz, o, D, fc, in, nb, out,
+                                                             ck, iadjoint);
       else if (tile == 16)
-        k_gs27_plane<T, 16, 16><<<g, kBlock, 0, s>>>(nx, ny, nz, o, D, fc, in, nb, out, ck,
-                                                     iadjoint);
+        k_gs27_plane<T, 16, 16, kBlock><<<g, kBlock, 0, s>>>(nx, ny, nz, o, D, fc, in, nb, out,
+                                                             ck, iadjoint);
       else
-        k_gs27_plane<T, 8, 8><<<g, kBlock, 0, s>>>(nx, ny, nz, o, D, fc, in, nb, out, ck,
-                                                   iadjoint);
+        k_gs27_plane<T, 8, 8, kBlock><<<g, kBlock, 0, s>>>(nx, ny, nz, o, D, fc, in, nb, out, ck,
+                                                           iadjoint);
     }
     T* t = in;
     in = out;
@@ -1435,6 +1527,8 @@ void gsrb27_planes(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const
 }
 
 bool gsrb27_use_planes(int nx, int ny, int nz) { return nx <= 49 && ny <= 49 && nz >= 3; }
+
+bool gsrb27_use_pairs(int nx) { return nx >= 3 && nx <= 130; }
 
 int gsrb27_planes_launches(int nz, int nu) { return (nz > 3 ? 2 : 1) * nu + (nu & 1); }
 
@@ -1624,6 +1718,8 @@ void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int 
                           int);                                                                 \
   template void gsrb27_planes<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*,  \
                                  T*, T*, int, int, int);                                        \
+  template void gsrb27_pairs<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*,   \
+                                T*, int, int);                                                  \
   template void resid<T>(hipStream_t, int, int, int, int, const Ops<T>&, const T*, const T*,     \
                          const T*, T*);                                                         \
   template void restrict3<T>(hipStream_t, int, int, int, int, int, int, const T*, T*);          \

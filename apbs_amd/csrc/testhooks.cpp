@@ -110,6 +110,8 @@ double gs27_sweeps_us(int n, int sweeps, int mode) {
   auto run = [&](int nu) {
     if (mode == 0)
       pmg::gsrb27<T>(0, n, n, n, ops, D, fc, x, nu, 0);
+    else if (mode == 2)
+      pmg::gsrb27_pairs<T>(0, n, n, n, ops, D, fc, x, nu, 0);
     else
       pmg::gsrb27_planes<T>(0, n, n, n, ops, D, fc, x, xs, nu, 0, mode);
   };
@@ -221,8 +223,8 @@ int pmgtest_gs27_sweeps(int n, int word, int sweeps, int mode, double* us_per_sw
 }
 
 // 27-point sweeps with the kernel form forced: mode 0 one launch per colour,
-// 1 per plane parity (tile by level size), 8 / 16 / 32 per plane parity with
-// that tile
+// 1 per plane parity (tile by level size), 2 per colour pair (rows of up to
+// 128 interior points), 8 / 16 / 32 per plane parity with that tile
 int pmgtest_gsrb27_mode(int nx, int ny, int nz, const double* a, const double* cc,
                         const double* fc, double* x, int nu, int iadjoint, int mode) {
   return gsrb_impl(nx, ny, nz, 27, a, cc, fc, x, nu, iadjoint, mode);
@@ -257,6 +259,9 @@ int gsrb_impl(int nx, int ny, int nz, int st, const double* a, const double* cc,
     TCHECK(hipFree(dt));
   } else if (tile == 0) {
     pmg::gsrb27<double>(0, nx, ny, nz, A.ops, dD, dfc, dx, nu, iadjoint);
+  } else if (tile == 2) {
+    if (nx > 130) return 0;
+    pmg::gsrb27_pairs<double>(0, nx, ny, nz, A.ops, dD, dfc, dx, nu, iadjoint);
   } else {
     // the plane-parity sweeps; the partner buffer carries the boundary of x
     double* dt = up(x, n);

@@ -707,9 +707,9 @@ struct PmgDev {
   // fine-level smoother: 0 fused out-of-place sweep, 1 one launch per colour
   // (environment APBS_PMG_SMOOTHER=colour selects 1)
   int smoother = 0;
-  // coarse-level smoother: 0 one launch per plane parity on the levels where
-  // that is faster (gsrb27_use_planes), 1 one launch per colour everywhere
-  // (environment APBS_PMG_SMOOTHER27=colour selects 1)
+  // coarse-level smoother: 0 one launch per plane parity or per colour pair
+  // on the levels where each is faster (gsrb27_use_planes, gsrb27_use_pairs),
+  // 1 one launch per colour everywhere (APBS_PMG_SMOOTHER27=colour selects 1)
   int smoother27 = 0;
   int mirror = 1;
   double stats[8] = {};
@@ -1151,6 +1151,9 @@ void vcycle(PmgDev* d, CycleArgs<T>& c) {
     if (d->smoother27 == 0 && pmg::gsrb27_use_planes(L[l].nx, L[l].ny, L[l].nz))
       pmg::gsrb27_planes<T>(s, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l],
                             c.xsl[l], 2, adj);
+    else if (d->smoother27 == 0 && pmg::gsrb27_use_pairs(L[l].nx))
+      pmg::gsrb27_pairs<T>(s, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l], 2,
+                           adj);
     else
       pmg::gsrb27<T>(s, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l], 2, adj);
   };

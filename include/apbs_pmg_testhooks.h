@@ -48,8 +48,9 @@ int pmgtest_nonlinear(int nx, int ny, int nz, const double *a, const double *cc,
 int pmgtest_gsrb_tile(int nx, int ny, int nz, const double *a, const double *cc,
                       const double *fc, double *x, int nu, int iadjoint, int tile);
 /* pmgtest_gsrb (27-point) with the kernel form forced: mode 0 one launch per
- * colour, 1 one launch per plane parity (tile by level size), 8 / 16 / 32 one
- * launch per plane parity with that square tile.  All give the same bits. */
+ * colour, 1 one launch per plane parity (tile by level size), 2 one launch
+ * per colour pair (nx <= 130, else returns 0), 8 / 16 / 32 one launch per
+ * plane parity with that square tile.  All give the same bits. */
 int pmgtest_gsrb27_mode(int nx, int ny, int nz, const double *a, const double *cc,
                         const double *fc, double *x, int nu, int iadjoint, int mode);
 /* No reference counterpart: the 27-point sweeps alone on a synthetic n^3

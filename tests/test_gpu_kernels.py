@@ -72,13 +72,14 @@ def test_gsrb_bitwise(L, O, st, n3, adj):
 
 
 @pytest.mark.parametrize("n3", [(17, 17, 17), (9, 11, 13), (45, 43, 41), (5, 5, 4),
-                                (70, 21, 6), (81, 81, 9), (3, 3, 3)])
-@pytest.mark.parametrize("mode", [1, 8, 16, 32])
+                                (70, 21, 6), (81, 81, 9), (3, 3, 3), (130, 7, 6), (129, 6, 7)])
+@pytest.mark.parametrize("mode", [1, 2, 8, 16, 32])
 def test_gsrb27_plane_sweeps_bitwise(L, O, n3, mode):
     """The plane-parity 27-point sweep (one launch per plane parity, tiles
-    with recomputed margins) gives the oracle's 8-colour sweep bit for bit:
-    every tile size, partial tiles, forward and adjoint, even and odd sweep
-    counts."""
+    with recomputed margins) and the colour-pair sweep (mode 2, one
+    wavefront per row) give the oracle's 8-colour sweep bit for bit: every
+    tile size, partial tiles, forward and adjoint, even and odd sweep
+    counts, rows of 1 to 128 interior points."""
     rng = np.random.default_rng(sum(n3) + mode)
     nx, ny, nz = n3
     n = nx * ny * nz
