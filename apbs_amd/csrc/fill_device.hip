@@ -714,17 +714,28 @@ __global__ void __launch_bounds__(kB)
   const int c = blockIdx.z;
   const int per = (A.n + nch - 1) / nch;
   const int a0 = c * per, a1 = min(A.n, a0 + per);
-  double val = 0.0;
-  for (int a = a0; a < a1; a++) {
+  // One atom's term, as the reference writes it.
+  auto term = [&](int a) {
     const double* ap = A.pos + 3 * a;
     const double q = ec * A.chg[a];
     const double dist = sqrt((px - ap[0]) * (px - ap[0]) + (py - ap[1]) * (py - ap[1]) +
                              (pz - ap[2]) * (pz - ap[2]));
     if (xkappa > VSMALL)
-      val += pre1 * (q / dist) * exp(-xkappa * (dist - A.rad[a])) / (1 + xkappa * A.rad[a]);
-    else
-      val += pre1 * (q / dist);
+      return pre1 * (q / dist) * exp(-xkappa * (dist - A.rad[a])) / (1 + xkappa * A.rad[a]);
+    return pre1 * (q / dist);
+  };
+  // four independent terms in flight (their dependency chains interleave),
+  // added in atom order
+  double val = 0.0;
+  int a = a0;
+  for (; a + 3 < a1; a += 4) {
+    const double t0 = term(a), t1 = term(a + 1), t2 = term(a + 2), t3 = term(a + 3);
+    val += t0;
+    val += t1;
+    val += t2;
+    val += t3;
   }
+  for (; a < a1; a++) val += term(a);
   part[((size_t)c * 6 + f) * mx + t] = val;
 }
 

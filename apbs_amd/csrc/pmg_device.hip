@@ -74,9 +74,9 @@ __global__ void __launch_bounds__(kBlock)
 // Points outside the interior are not written.
 constexpr int kFT = 256, kXCD = 8;
 
-// kTX columns (even), kTY rows and kKC planes per workgroup.
-template <class T, int kTY, int kKC, int kTX>
-__global__ void __launch_bounds__(kFT)
+// kTX columns (even), kTY rows and kKC planes per workgroup of kNT threads.
+template <class T, int kTY, int kKC, int kTX, int kNT>
+__global__ void __launch_bounds__(kNT)
     k_gsrb7_fused(int nx, int ny, int nz, const T* __restrict__ oE, const T* __restrict__ oN,
                   const T* __restrict__ uC, const T* __restrict__ D, const T* __restrict__ fc,
                   const T* __restrict__ xi, T* __restrict__ xo, int first) {
@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(kFT)
   // point in kFR rounds.  loadF issues every round's loads at once into
   // registers; storeF computes the values (acc7's terms and order) into the
   // plane's LDS slot.
-  constexpr int kFR = (FN + kFT - 1) / kFT;
+  constexpr int kFR = (FN + kNT - 1) / kNT;
   struct FRegs {
     T v[kFR][14];
     int slot[kFR];  // LDS index in the ring plane
@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(kFT)
 #pragma unroll
     for (int u = 0; u < kFR; u++) {
       R.kind[u] = 0;
-      const int f = tid + u * kFT;
+      const int f = tid + u * kNT;
       if (f >= FN) continue;
       const int row = f / FPR, c = f % FPR;
       const int j = j0 - 1 + row;
@@ -166,12 +166,12 @@ __global__ void __launch_bounds__(kFT)
   // F and one S point).  The eight operands of the S point of plane kk are
   // loaded one plane ahead, together with stageF(kk), so each coefficient
   // line is fetched once for both colours while it is still cached.
-  constexpr int kSP = (kTX / 2 * kTY + kFT - 1) / kFT;
+  constexpr int kSP = (kTX / 2 * kTY + kNT - 1) / kNT;
   T cur[kSP][8], nxt[kSP][8];
   auto loadS = [&](int kk, T (&r)[kSP][8]) {
 #pragma unroll
     for (int u = 0; u < kSP; u++) {
-      const int sidx = tid + u * kFT;
+      const int sidx = tid + u * kNT;
       if (sidx >= kTX / 2 * kTY) continue;
       const int j = j0 + sidx / (kTX / 2);
       const int is = i0 + 2 * (sidx % (kTX / 2)) + ((1 - F - i0 - j - kk) & 1);
@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(kFT)
     const T* rp = ring[(k + 1) % 3];
 #pragma unroll
     for (int u = 0; u < kSP; u++) {
-      const int sidx = tid + u * kFT;
+      const int sidx = tid + u * kNT;
       if (sidx >= kTX / 2 * kTY) continue;
       const int jj = sidx / (kTX / 2);
       const int j = j0 + jj;
@@ -1348,33 +1348,43 @@ Ops27<T> to27(const Ops<T>& A) {
 
 // =================================================================== wrappers
 
-template <class T, int TY, int KC, int TX>
+template <class T, int TY, int KC, int TX, int NT = kFT>
 void fused_sweep(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
                  const T* in, T* out, int first) {
   const int nb = ((nx - 2 + TX - 1) / TX) * ((ny - 2 + TY - 1) / TY) * ((nz - 2 + KC - 1) / KC);
   const unsigned g = (unsigned)(((nb + kXCD - 1) / kXCD) * kXCD);
-  k_gsrb7_fused<T, TY, KC, TX><<<g, kFT, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, in,
-                                                 out, first);
+  k_gsrb7_fused<T, TY, KC, TX, NT><<<g, NT, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc,
+                                                    in, out, first);
 }
 
 static bool valid_tile(int v) {
   const int w = v / 10000, rp = v % 10000;
   const bool shape = rp == 416 || rp == 48 || rp == 816 || rp == 88 || rp == 84 || rp == 1616 ||
-                     rp == 832 || rp == 44 || rp == 68;
+                     rp == 832 || rp == 44 || rp == 68 || rp == 5088 || rp == 5084 ||
+                     rp == 5816;
   return shape && (w == 0 || w == 64 || w == 80 || w == 96);
 }
 
 // tile shape of the fused sweep: rows x planes per workgroup
-int fused_tile() {
+// An APBS_PMG_FUSED_TILE shape, or 0.
+static int fused_tile_env() {
   static int t = [] {
     const char* e = getenv("APBS_PMG_FUSED_TILE");
-    // 4 rows x 8 planes: fastest or within 6% of the fastest shape in both
-    // precisions at 161^3 and 193^3 (73 / 54 us and 152 / 77 us per sweep)
-    if (!e) return 48;
+    if (!e) return 0;
     const int v = atoi(e);
-    return valid_tile(v) ? v : 48;
+    return valid_tile(v) ? v : 0;
   }();
   return t;
+}
+
+// Default shape for a grid.  4 rows x 8 planes (256 threads) up to about
+// 5.5M points, where the fine arrays sit in the Infinity Cache: at 161^3 it
+// is the fastest (65 / 50 us per FP64 / FP32 sweep at 80 columns).  Above,
+// 8 rows x 8 planes with 512 threads (one F and one S round each, a quarter
+// less halo): at 193^3 138 / 71 us against 153 / 77.
+int fused_tile(int nx, int ny, int nz) {
+  if (const int t = fused_tile_env()) return t;
+  return (long)nx * ny * nz > 5500000L ? 5088 : 48;
 }
 
 // Tile width for an x extent: the one of 64 / 80 / 96 columns that leaves
@@ -1398,6 +1408,10 @@ static void fused_sweep_w(hipStream_t s, int rp, int nx, int ny, int nz, const O
     case 416: fused_sweep<T, 4, 16, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
     case 48: fused_sweep<T, 4, 8, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
     case 44: fused_sweep<T, 4, 4, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+    // 512-thread workgroups: 8 or 12 rows with one F round and one S round
+    case 5088: fused_sweep<T, 8, 8, TX, 512>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+    case 5084: fused_sweep<T, 8, 4, TX, 512>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+    case 5816: fused_sweep<T, 8, 16, TX, 512>(s, nx, ny, nz, A, D, fc, in, out, first); break;
     case 68: fused_sweep<T, 6, 8, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
     case 88: fused_sweep<T, 8, 8, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
     case 84: fused_sweep<T, 8, 4, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
@@ -1413,7 +1427,8 @@ void gsrb7_fused(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T
   const int first = iadjoint ? 0 : 1;
   T* in = x;
   T* out = tmp;
-  const int tile = tile_shape > 0 && valid_tile(tile_shape) ? tile_shape : fused_tile();
+  const int tile =
+      tile_shape > 0 && valid_tile(tile_shape) ? tile_shape : fused_tile(nx, ny, nz);
   const int rp = tile % 10000;
   const int w = tile / 10000 ? tile / 10000 : fused_width(nx);
   for (int it = 0; it < nu; it++) {

@@ -96,7 +96,8 @@ def test_gsrb27_plane_sweeps_bitwise(L, O, n3, mode):
 
 
 @pytest.mark.parametrize("tile", [416, 48, 816, 88, 84, 1616, 832, 44, 68, 640048, 800048,
-                                  960048, 800044, 800068, 960416, 800816])
+                                  960048, 800044, 800068, 960416, 800816, 5088, 645084,
+                                  805816])
 def test_fused_sweep_tiles_bitwise(L, O, tile):
     """Every tile shape of the fused red-black sweep gives the oracle's
     sweep bit for bit (partial tiles in every direction)."""
