@@ -92,9 +92,11 @@ void dcvec(hipStream_t s, size_t n, const double* cc, const double* x, double* o
            const Ions& ions);
 
 // ------------------------------------------------------------- transfers
+// out = P^T in on the coarse interior; zero (optional, same coarse size) is
+// set to 0 on the same points, for the coarse solution the cycle starts from.
 template <class T>
 void restrict3(hipStream_t s, int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
-               const T* in, T* out);
+               const T* in, T* out, T* zero = nullptr);
 // xf += damp[0] * P xc  (interior of the fine grid)
 template <class T>
 void prolong_add(hipStream_t s, int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,

@@ -617,7 +617,7 @@ __global__ void __launch_bounds__(kBlock)
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_restrict(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc, const T* __restrict__ in,
-               T* __restrict__ out) {
+               T* __restrict__ out, T* __restrict__ zero) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   const int K = blockIdx.y + 1;
   const int I = 1 + t % (nxc - 2), J = 1 + t / (nxc - 2);
@@ -656,6 +656,7 @@ __global__ void __launch_bounds__(kBlock)
   d += W8 * in[d0 + sx - sy];
   d += W8 * in[d0 - sx - sy];
   out[IX(I, J, K, nxc, nyc)] = o + u + d;
+  if (zero) zero[IX(I, J, K, nxc, nyc)] = T(0);
 }
 
 // Trilinear prolongation and correction x_f += damp * P x_c (matvecd.c:935-1076
@@ -1570,9 +1571,9 @@ void dcvec(hipStream_t s, size_t n, const double* cc, const double* x, double* o
 
 template <class T>
 void restrict3(hipStream_t s, int nxf, int nyf, int nzf, int nxc, int nyc, int nzc, const T* in,
-               T* out) {
+               T* out, T* zero) {
   k_restrict<T><<<grid_interior(nxc, nyc, nzc), kBlock, 0, s>>>(nxf, nyf, nzf, nxc, nyc, nzc, in,
-                                                                out);
+                                                                out, zero);
 }
 
 template <class T>
@@ -1737,7 +1738,7 @@ void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int 
                                 T*, int, int);                                                  \
   template void resid<T>(hipStream_t, int, int, int, int, const Ops<T>&, const T*, const T*,     \
                          const T*, T*);                                                         \
-  template void restrict3<T>(hipStream_t, int, int, int, int, int, int, const T*, T*);          \
+  template void restrict3<T>(hipStream_t, int, int, int, int, int, int, const T*, T*, T*);      \
   template void prolong_add<T>(hipStream_t, int, int, int, int, int, int, const T*, T*,         \
                                const double*);                                                  \
   template void nrm1<T>(hipStream_t, int, int, int, const T*, double*, double*);                \

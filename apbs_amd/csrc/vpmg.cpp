@@ -1161,10 +1161,11 @@ void vcycle(PmgDev* d, CycleArgs<T>& c) {
   fine_smooth(0);
   pmg::resid<T>(s, 7, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], c.w1);
   for (int l = 1; l < nt; l++) {
+    // the coarse solution starts at zero (interior cleared with the
+    // restriction; the boundary layer is never written)
     pmg::restrict3<T>(s, L[l - 1].nx, L[l - 1].ny, L[l - 1].nz, L[l].nx, L[l].ny, L[l].nz, c.w1,
-                      c.w0[l]);
+                      c.w0[l], l != nt - 1 ? c.X[l] : nullptr);
     if (l != nt - 1) {
-      pmg::fill<T>(s, c.X[l], L[l].n, T(0));
       coarse_smooth(l, 0);
       pmg::resid<T>(s, 27, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l], c.w1);
     }
@@ -1255,12 +1256,13 @@ int mvcs64(PmgDev* d, double* const* X, const double* f, int istop, int itmax, d
 // Mixed precision: FP32 V-cycles on the correction equation inside an FP64
 // defect-correction loop (fine residual, update and stopping test in FP64).
 int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int itmax,
-                double errtol, double* relres, int* outer) {
+                double errtol, double* relres, int* outer, bool x_zero) {
   const DLevel& F = d->L[0];
-  double rsden = 1.0;
-  if (istop == 1) {
+  double rsden = 1.0, fnrm = -1.0;
+  if (istop == 1 || x_zero) {
     pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, f, d->part, d->scal);
-    rsden = read_scalar(d, d->scal);
+    fnrm = read_scalar(d, d->scal);
+    if (istop == 1) rsden = fnrm;
   }
   if (rsden == 0.0) rsden = 1.0;
   CycleArgs<float> c;
@@ -1277,11 +1279,21 @@ int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int i
   c.f = d->L[0].w0f;
   const pmg::Ops<double> A0 = F.ops();
   int it = 0;
-  double rsnrm = rsden;
-  pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, A0, F.D, f, X64[0], d->w1, d->part, d->scal);
-  rsnrm = read_scalar(d, d->scal);
+  double rsnrm;
+  // the first residual: f itself when the solution starts at zero (f + A 0
+  // rounds to f), else computed
+  const double* r = d->w1;
+  if (x_zero) {
+    r = f;
+    rsnrm = fnrm;
+  } else {
+    pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, A0, F.D, f, X64[0], d->w1, d->part,
+                             d->scal);
+    rsnrm = read_scalar(d, d->scal);
+  }
   while (it < itmax && (rsnrm / rsden) > errtol) {
-    pmg::convert<double, float>(d->s, F.n, d->w1, d->L[0].w0f);
+    pmg::convert<double, float>(d->s, F.n, r, d->L[0].w0f);
+    r = d->w1;
     pmg::fill<float>(d->s, d->L[0].xfl, F.n, 0.0f);
     run_vcycle<float>(d, c);
     // X64 += (double) e
@@ -1338,16 +1350,16 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
                              L[l - 1].cprime, L[l].cprime);
     set_diagonals(d, true);
     const double errtol_s = std::min(0.9 * xnorm_old, 1.0 * pow(xnorm_old, 2.0));
-    for (int l = 0; l < d->ntot; l++) {
-      X[l] = L[l].xtmp;
-      pmg::fill<double>(s, X[l], L[l].n, 0.0);
-    }
+    // the V-cycle clears the coarse corrections itself; the fine one starts
+    // at zero
+    for (int l = 0; l < d->ntot; l++) X[l] = L[l].xtmp;
+    pmg::fill<double>(s, X[0], L[0].n, 0.0);
     double rr = 0;
     int its;
     if (d->precision == 1) {
       sync_fp32(d, iters == 1);
       int outer = 0;
-      mixed_solve(d, X, d->F, 0, 1000, errtol_s, &rr, &outer);
+      mixed_solve(d, X, d->F, 0, 1000, errtol_s, &rr, &outer, true);
       its = outer;
     } else {
       its = mvcs64(d, X, d->F, 0, 1000, errtol_s, &rr);
@@ -2041,7 +2053,7 @@ extern "C" int Vpmg_solve(Vpmg* thee) {
     if (d->precision == 1) {
       sync_fp32(d, true);
       const double tol = d->mixed_tol > 0 ? d->mixed_tol : p->errtol;
-      iters = mixed_solve(d, X, d->L[0].fc, 1, p->itmax * 4, tol, &relres, &outer);
+      iters = mixed_solve(d, X, d->L[0].fc, 1, p->itmax * 4, tol, &relres, &outer, true);
     } else {
       iters = mvcs64(d, X, d->L[0].fc, 1, p->itmax, p->errtol, &relres);
     }
