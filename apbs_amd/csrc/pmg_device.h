@@ -133,7 +133,10 @@ void copy_interior(hipStream_t s, int nx, int ny, int nz, const T* x, T* y);
 // y = a * x over the whole array
 void scale(hipStream_t s, size_t n, double a, const double* x, double* y);
 template <class T>
-void add_diag(hipStream_t s, size_t n, const T* oC, const T* cc, T* D);
+void add_diag(hipStream_t s, size_t n, const T* oC, const T* cc, T* D, float* Df = nullptr);
+// y += a * (double) x on the interior (x FP32, y FP64)
+void axpy_interior_f32(hipStream_t s, int nx, int ny, int nz, double a, const float* x,
+                       double* y);
 template <class Tin, class Tout>
 void convert(hipStream_t s, size_t n, const Tin* in, Tout* out);
 void set_faces(hipStream_t s, int nx, int ny, int nz, double* u, const double* gx,

@@ -531,6 +531,7 @@ __global__ void __launch_bounds__(kBlock)
   int t0, t1;
   part_range(n, t0, t1);
   double acc = 0.0;
+#pragma unroll 4
   for (int t = t0 + threadIdx.x; t < t1; t += kBlock) {
     const int k = 1 + t / nij, rem = t - (k - 1) * nij;
     const int j = 1 + rem / ni, i = 1 + rem - (j - 1) * ni;
@@ -764,9 +765,25 @@ __global__ void __launch_bounds__(kBlock)
 
 template <class T>
 __global__ void k_add_diag(size_t n, const T* __restrict__ oC, const T* __restrict__ cc,
-                           T* __restrict__ D) {
-  for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (size_t)gridDim.x * kBlock)
-    D[t] = oC[t] + cc[t];
+                           T* __restrict__ D, float* __restrict__ Df) {
+  for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (size_t)gridDim.x * kBlock) {
+    const T v = oC[t] + cc[t];
+    D[t] = v;
+    if (Df) Df[t] = (float)v;
+  }
+}
+
+// y += a * (double) x on the interior: an FP32 correction added to an FP64
+// solution, the same bits as converting x first and calling axpy_interior
+__global__ void __launch_bounds__(kBlock)
+    k_axpy_int_f32(int nx, int ny, int nz, double a, const float* __restrict__ x,
+                   double* __restrict__ y) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int k = blockIdx.y + 1;
+  const int i = 1 + t % (nx - 2), j = 1 + t / (nx - 2);
+  if (j > ny - 2) return;
+  const size_t p = IX(i, j, k, nx, ny);
+  y[p] += a * (double)x[p];
 }
 
 __global__ void k_scale(size_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
@@ -1632,8 +1649,13 @@ void copy_interior(hipStream_t s, int nx, int ny, int nz, const T* x, T* y) {
 }
 
 template <class T>
-void add_diag(hipStream_t s, size_t n, const T* oC, const T* cc, T* D) {
-  k_add_diag<T><<<nblocks(n), kBlock, 0, s>>>(n, oC, cc, D);
+void add_diag(hipStream_t s, size_t n, const T* oC, const T* cc, T* D, float* Df) {
+  k_add_diag<T><<<nblocks(n), kBlock, 0, s>>>(n, oC, cc, D, Df);
+}
+
+void axpy_interior_f32(hipStream_t s, int nx, int ny, int nz, double a, const float* x,
+                       double* y) {
+  k_axpy_int_f32<<<grid_interior(nx, ny, nz), kBlock, 0, s>>>(nx, ny, nz, a, x, y);
 }
 
 void scale(hipStream_t s, size_t n, double a, const double* x, double* y) {
@@ -1749,7 +1771,7 @@ void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int 
   template void fill<T>(hipStream_t, T*, size_t, T);                                            \
   template void axpy_interior<T>(hipStream_t, int, int, int, double, const T*, T*);             \
   template void copy_interior<T>(hipStream_t, int, int, int, const T*, T*);                     \
-  template void add_diag<T>(hipStream_t, size_t, const T*, const T*, T*);                       \
+  template void add_diag<T>(hipStream_t, size_t, const T*, const T*, T*, float*);               \
   template void dense_solve<T>(hipStream_t, int, int, int, const double*, int, const T*, T*);  \
   template void gsrb7_fused<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*, T*, \
                                T*, int, int, int);

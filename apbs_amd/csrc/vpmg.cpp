@@ -685,7 +685,6 @@ struct PmgDev {
   float* w1f = nullptr;
   double* xs64 = nullptr;  // fine-level ping-pong partners of the smoother (zero boundary)
   float* xs32 = nullptr;
-  double* ecorr = nullptr;  // FP64 image of the FP32 correction (mixed solves)
   double *part = nullptr, *scal = nullptr;
   double* hscal = nullptr;  // pinned
   double *dense = nullptr, *dwork = nullptr;
@@ -898,7 +897,6 @@ void free_hierarchy(PmgDev* d) {
   dfree(d->w1f);
   dfree(d->xs64);
   dfree(d->xs32);
-  dfree(d->ecorr);
   dfree(d->F);
   dfree(d->wn1);
   dfree(d->wn2);
@@ -971,7 +969,7 @@ size_t pmgint::footprint_bytes(int nx, int ny, int nz, int nlev, bool newton, bo
   }
   h += 3 * blk(8 * n);                               // w1, F, xs64
   if (newton) h += 4 * blk(8 * n);                   // wn1..3, nlw
-  if (mixed) h += 2 * blk(4 * n) + blk(8 * n);       // w1f, xs32, ecorr
+  if (mixed) h += 2 * blk(4 * n);                    // w1f, xs32
   const double dn = (double)(dims[nt - 1][0] - 2) * (dims[nt - 1][1] - 2) * (dims[nt - 1][2] - 2);
   h += blk(8 * dn * dn) + blk(8 * dn);
   return (size_t)(b + std::max(fill, h));
@@ -1050,8 +1048,6 @@ void alloc_hierarchy(PmgDev* d, int nlev, bool newton, bool mixed) {
     pmg::fill<float>(d->s, d->w1f, d->n, 0.0f);
     d->xs32 = dalloc<float>(d->n);
     pmg::fill<float>(d->s, d->xs32, d->n, 0.0f);
-    d->ecorr = dalloc<double>(d->n);
-    pmg::fill<double>(d->s, d->ecorr, d->n, 0.0);
   }
   d->F = dalloc<double>(d->n);
   pmg::fill<double>(d->s, d->F, d->n, 0.0);
@@ -1088,26 +1084,28 @@ void build_operators(PmgDev* d) {
 
 // D_l = oC_l + c_l for the given per-level Helmholtz arrays, then the dense
 // inverse of the coarsest level.
+// D_l = oC_l + c_l (and, for a mixed-precision solve, its FP32 image Df_l in
+// the same pass), then the dense inverse of the coarsest level.
 void set_diagonals(PmgDev* d, bool use_cprime) {
+  const bool f32 = d->precision == 1 && d->h_mixed;
   for (int l = 0; l < d->ntot; l++) {
     DLevel& L = d->L[l];
-    pmg::add_diag<double>(d->s, L.n, L.a[0], use_cprime ? L.cprime : L.cc, L.D);
+    pmg::add_diag<double>(d->s, L.n, L.a[0], use_cprime ? L.cprime : L.cc, L.D,
+                          f32 ? L.Df : nullptr);
   }
   const DLevel& C = d->L[d->ntot - 1];
   pmg::dense_assemble(d->s, C.nx, C.ny, C.nz, C.ops(), C.D, d->dense, d->dn);
   pmg::dense_invert(d->s, d->dense, d->dwork, d->dn);
 }
 
-// FP32 images of the level operators (ops: the stencil arrays, which do not
-// change within a solve) and of the diagonals D (which change with every
-// Newton Jacobian).
-void sync_fp32(PmgDev* d, bool ops) {
+// FP32 images of the level operators (the stencil arrays, which do not
+// change within a solve); set_diagonals writes the FP32 diagonals, which
+// change with every Newton Jacobian.
+void sync_fp32_ops(PmgDev* d) {
   for (int l = 0; l < d->ntot; l++) {
     DLevel& L = d->L[l];
     const int na = l == 0 ? 4 : 14;
-    if (ops)
-      for (int s = 1; s < na; s++) pmg::convert<double, float>(d->s, L.n, L.a[s], L.af[s]);
-    pmg::convert<double, float>(d->s, L.n, L.D, L.Df);
+    for (int s = 1; s < na; s++) pmg::convert<double, float>(d->s, L.n, L.a[s], L.af[s]);
   }
 }
 
@@ -1296,9 +1294,8 @@ int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int i
     r = d->w1;
     pmg::fill<float>(d->s, d->L[0].xfl, F.n, 0.0f);
     run_vcycle<float>(d, c);
-    // X64 += (double) e
-    pmg::convert<float, double>(d->s, F.n, d->L[0].xfl, d->ecorr);
-    pmg::axpy_interior<double>(d->s, F.nx, F.ny, F.nz, 1.0, d->ecorr, X64[0]);
+    // X64 += (double) e, in one pass
+    pmg::axpy_interior_f32(d->s, F.nx, F.ny, F.nz, 1.0, d->L[0].xfl, X64[0]);
     it++;
     pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, A0, F.D, f, X64[0], d->w1, d->part,
                              d->scal);
@@ -1357,7 +1354,7 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
     double rr = 0;
     int its;
     if (d->precision == 1) {
-      sync_fp32(d, iters == 1);
+      if (iters == 1) sync_fp32_ops(d);
       int outer = 0;
       mixed_solve(d, X, d->F, 0, 1000, errtol_s, &rr, &outer, true);
       its = outer;
@@ -2051,7 +2048,7 @@ extern "C" int Vpmg_solve(Vpmg* thee) {
     HIPCHECK(hipStreamSynchronize(d->s));
     d->stats[6] = now() - tb;
     if (d->precision == 1) {
-      sync_fp32(d, true);
+      sync_fp32_ops(d);
       const double tol = d->mixed_tol > 0 ? d->mixed_tol : p->errtol;
       iters = mixed_solve(d, X, d->L[0].fc, 1, p->itmax * 4, tol, &relres, &outer, true);
     } else {
