@@ -134,6 +134,17 @@ void copy_interior(hipStream_t s, int nx, int ny, int nz, const T* x, T* y);
 void scale(hipStream_t s, size_t n, double a, const double* x, double* y);
 template <class T>
 void add_diag(hipStream_t s, size_t n, const T* oC, const T* cc, T* D, float* Df = nullptr);
+// add_diag of up to 24 levels in one launch; off[l] is the first element of
+// level l in the concatenation, off[nl] the total; Df[l] may be null.
+struct DiagLevels {
+  int nl;
+  size_t off[25];
+  const double* oC[24];
+  const double* c[24];
+  double* D[24];
+  float* Df[24];
+};
+void add_diag_levels(hipStream_t s, const DiagLevels& L);
 // y += a * (double) x on the interior (x FP32, y FP64)
 void axpy_interior_f32(hipStream_t s, int nx, int ny, int nz, double a, const float* x,
                        double* y);

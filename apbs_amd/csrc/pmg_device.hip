@@ -773,6 +773,21 @@ __global__ void k_add_diag(size_t n, const T* __restrict__ oC, const T* __restri
   }
 }
 
+// D_l = oC_l + c_l on every level in one launch (and Df_l = (float) D_l
+// where Df_l is given): the same values as one add_diag per level.
+__global__ void __launch_bounds__(kBlock) k_add_diag_levels(DiagLevels L) {
+  const size_t tot = L.off[L.nl];
+  for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < tot;
+       t += (size_t)gridDim.x * kBlock) {
+    int l = 0;
+    while (t >= L.off[l + 1]) l++;
+    const size_t i = t - L.off[l];
+    const double v = L.oC[l][i] + L.c[l][i];
+    L.D[l][i] = v;
+    if (L.Df[l]) L.Df[l][i] = (float)v;
+  }
+}
+
 // y += a * (double) x on the interior: an FP32 correction added to an FP64
 // solution, the same bits as converting x first and calling axpy_interior
 __global__ void __launch_bounds__(kBlock)
@@ -1651,6 +1666,10 @@ void copy_interior(hipStream_t s, int nx, int ny, int nz, const T* x, T* y) {
 template <class T>
 void add_diag(hipStream_t s, size_t n, const T* oC, const T* cc, T* D, float* Df) {
   k_add_diag<T><<<nblocks(n), kBlock, 0, s>>>(n, oC, cc, D, Df);
+}
+
+void add_diag_levels(hipStream_t s, const DiagLevels& L) {
+  k_add_diag_levels<<<nblocks(L.off[L.nl]), kBlock, 0, s>>>(L);
 }
 
 void axpy_interior_f32(hipStream_t s, int nx, int ny, int nz, double a, const float* x,

@@ -1088,11 +1088,18 @@ void build_operators(PmgDev* d) {
 // the same pass), then the dense inverse of the coarsest level.
 void set_diagonals(PmgDev* d, bool use_cprime) {
   const bool f32 = d->precision == 1 && d->h_mixed;
+  pmg::DiagLevels dl{};
+  dl.nl = d->ntot;
+  dl.off[0] = 0;
   for (int l = 0; l < d->ntot; l++) {
     DLevel& L = d->L[l];
-    pmg::add_diag<double>(d->s, L.n, L.a[0], use_cprime ? L.cprime : L.cc, L.D,
-                          f32 ? L.Df : nullptr);
+    dl.oC[l] = L.a[0];
+    dl.c[l] = use_cprime ? L.cprime : L.cc;
+    dl.D[l] = L.D;
+    dl.Df[l] = f32 ? L.Df : nullptr;
+    dl.off[l + 1] = dl.off[l] + L.n;
   }
+  pmg::add_diag_levels(d->s, dl);
   const DLevel& C = d->L[d->ntot - 1];
   pmg::dense_assemble(d->s, C.nx, C.ny, C.nz, C.ops(), C.D, d->dense, d->dn);
   pmg::dense_invert(d->s, d->dense, d->dwork, d->dn);
