@@ -78,7 +78,6 @@ bool gsrb27_use_planes(int nx, int ny, int nz);
 // plane-parity: rows of up to 128 interior points (81^3: 40 against 62 us per
 // FP64 sweep, 28 against 40 FP32; 97^3: 79 against 87; 65^3: 23 against 34).
 bool gsrb27_use_pairs(int nx);
-int gsrb27_planes_launches(int nz, int nu);
 
 // ------------------------------------------------------- residual / matvec
 template <class T>

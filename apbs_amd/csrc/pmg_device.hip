@@ -1578,7 +1578,6 @@ bool gsrb27_use_planes(int nx, int ny, int nz) { return nx <= 49 && ny <= 49 && 
 
 bool gsrb27_use_pairs(int nx) { return nx >= 3 && nx <= 130; }
 
-int gsrb27_planes_launches(int nz, int nu) { return (nz > 3 ? 2 : 1) * nu + (nu & 1); }
 
 template <class T>
 void resid(hipStream_t s, int st, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
