@@ -117,6 +117,7 @@ void run_job(Job& j, PmgResult& res) {
     if (!pmg) break;
     if (r.precision) Vpmg_setPrecision(pmg, r.precision, r.mixed_tol);
     Vpmg_setHostMirror(pmg, 0);
+    pmgint::set_side_stream(pmg, false);
     if (!Vpmg_fillco(pmg, (Vsurf_Meth)r.srfm, r.swin, (Vchrg_Meth)r.chgm, 0, nullptr, 0, nullptr,
                      0, nullptr, 0, nullptr, 0, nullptr, 0, nullptr))
       break;

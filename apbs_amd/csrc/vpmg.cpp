@@ -710,6 +710,8 @@ struct PmgDev {
   // on the levels where each is faster (gsrb27_use_planes, gsrb27_use_pairs),
   // 1 one launch per colour everywhere (APBS_PMG_SMOOTHER27=colour selects 1)
   int smoother27 = 0;
+  // fillco's MDH boundary on a second stream (pmgint::set_side_stream)
+  int side = 1;
   int mirror = 1;
   double stats[8] = {};
   pmg::Geo geo{};
@@ -1834,6 +1836,42 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
   const pmg::Geo g = geo_of(thee);
   d->geo = g;
   const pmg::AtomsDev A = atoms_dev(d);
+  // The MDH boundary (FP64-bound, independent of the charge and coefficient
+  // maps) runs on a second stream while those are built; the object's
+  // stream waits for it at the boundary section.  The guard joins and frees
+  // on every return path.
+  struct SideBoundary {
+    hipStream_t s2 = nullptr;
+    hipEvent_t done = nullptr;
+    double* part = nullptr;
+    ~SideBoundary() {
+      if (s2) {
+        HIPCHECK(hipStreamSynchronize(s2));
+        pmgpool::stream_release(s2);
+      }
+      if (done) HIPCHECK(hipEventDestroy(done));
+      dfree(part);
+    }
+  } side;
+  const double pre1 =
+      ((Vunit_ec) / (4 * VPI * Vunit_eps0 * epsw * Vunit_kb * pbe->T)) * (1.0e10);
+  if (p->bcfl == BCFL_MDH && d->side) {
+    hipEvent_t ready;
+    HIPCHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    HIPCHECK(hipEventRecord(ready, d->s));  // xf, yf, zf uploaded
+    side.s2 = pmgpool::stream_acquire();
+    HIPCHECK(hipStreamWaitEvent(side.s2, ready, 0));
+    HIPCHECK(hipEventDestroy(ready));
+    pmg::fill<double>(side.s2, d->gx, 2 * (size_t)d->ny * d->nz, 0.0);
+    pmg::fill<double>(side.s2, d->gy, 2 * (size_t)d->nx * d->nz, 0.0);
+    pmg::fill<double>(side.s2, d->gz, 2 * (size_t)d->nx * d->ny, 0.0);
+    // bcflnew (vpmg.c:3289-3367)
+    side.part = dalloc<double>(pmg::bc_mdh_scratch(g, d->natoms));
+    pmg::bc_mdh(side.s2, g, d->xf, d->yf, d->zf, A, pre1, pbe->xkappa, side.part, d->gx, d->gy,
+                d->gz);
+    HIPCHECK(hipEventCreateWithFlags(&side.done, hipEventDisableTiming));
+    HIPCHECK(hipEventRecord(side.done, side.s2));
+  }
   Valist* al = pbe->alist;
   std::vector<double> hpos(3 * (size_t)d->natoms), hrad(d->natoms);
   double rmax = 0;
@@ -1934,13 +1972,14 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
     pmg::fill<double>(d->s, d->epsy, d->n, epsp);
     pmg::fill<double>(d->s, d->epsz, d->n, epsp);
   }
-  if (p->bcfl != BCFL_FOCUS) {
+  if (side.done) {
+    HIPCHECK(hipStreamWaitEvent(d->s, side.done, 0));  // MDH on the second stream
+  } else if (p->bcfl != BCFL_FOCUS) {
     pmg::fill<double>(d->s, d->gx, 2 * (size_t)d->ny * d->nz, 0.0);
     pmg::fill<double>(d->s, d->gy, 2 * (size_t)d->nx * d->nz, 0.0);
     pmg::fill<double>(d->s, d->gz, 2 * (size_t)d->nx * d->ny, 0.0);
     if (p->bcfl == BCFL_MDH) {
-      const double pre1 =
-          ((Vunit_ec) / (4 * VPI * Vunit_eps0 * epsw * Vunit_kb * pbe->T)) * (1.0e10);
+      // bcflnew (vpmg.c:3289-3367)
       double* part = dalloc<double>(pmg::bc_mdh_scratch(g, d->natoms));
       pmg::bc_mdh(d->s, g, d->xf, d->yf, d->zf, A, pre1, pbe->xkappa, part, d->gx, d->gy, d->gz);
       HIPCHECK(hipStreamSynchronize(d->s));
@@ -2550,6 +2589,10 @@ extern "C" int Vpmg_setPrecision(Vpmg* thee, int mode, double tol) {
   d->precision = mode;
   d->mixed_tol = tol;
   return 1;
+}
+
+void pmgint::set_side_stream(Vpmg* pmg, bool on) {
+  if (pmg) DEV(pmg)->side = on ? 1 : 0;
 }
 
 extern "C" int Vpmg_setHostMirror(Vpmg* thee, int on) {
