@@ -133,6 +133,25 @@ void copy_interior(hipStream_t s, int nx, int ny, int nz, const T* x, T* y);
 void scale(hipStream_t s, size_t n, double a, const double* x, double* y);
 template <class T>
 void add_diag(hipStream_t s, size_t n, const T* oC, const T* cc, T* D, float* Df = nullptr);
+// Arrays cleared together on one stream (sizes multiples of 4 bytes): add()
+// queues an array, flush() clears the queued ones in one launch; a full
+// queue and the destructor flush.
+constexpr int kZeroMax = 48;
+struct ZeroArgs {
+  int n;
+  size_t off[kZeroMax + 1];
+  unsigned* p[kZeroMax];
+};
+struct ZeroList {
+  hipStream_t s;
+  ZeroArgs z{};
+  explicit ZeroList(hipStream_t st) : s(st) {}
+  ~ZeroList() { flush(); }
+  ZeroList(const ZeroList&) = delete;
+  ZeroList& operator=(const ZeroList&) = delete;
+  void add(void* ptr, size_t bytes);
+  void flush();
+};
 // add_diag of up to 24 levels in one launch; off[l] is the first element of
 // level l in the concatenation, off[nl] the total; Df[l] may be null.
 struct DiagLevels {

@@ -773,6 +773,17 @@ __global__ void k_add_diag(size_t n, const T* __restrict__ oC, const T* __restri
   }
 }
 
+// Zero up to kZeroMax arrays in one launch (4-byte words).
+__global__ void __launch_bounds__(kBlock) k_zero_many(ZeroArgs Z) {
+  const size_t tot = Z.off[Z.n];
+  for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < tot;
+       t += (size_t)gridDim.x * kBlock) {
+    int a = 0;
+    while (t >= Z.off[a + 1]) a++;
+    Z.p[a][t - Z.off[a]] = 0u;
+  }
+}
+
 // D_l = oC_l + c_l on every level in one launch (and Df_l = (float) D_l
 // where Df_l is given): the same values as one add_diag per level.
 __global__ void __launch_bounds__(kBlock) k_add_diag_levels(DiagLevels L) {
@@ -1665,6 +1676,20 @@ void copy_interior(hipStream_t s, int nx, int ny, int nz, const T* x, T* y) {
 template <class T>
 void add_diag(hipStream_t s, size_t n, const T* oC, const T* cc, T* D, float* Df) {
   k_add_diag<T><<<nblocks(n), kBlock, 0, s>>>(n, oC, cc, D, Df);
+}
+
+void ZeroList::add(void* ptr, size_t bytes) {
+  if (!ptr || bytes < 4) return;
+  if (z.n == kZeroMax) flush();
+  z.p[z.n] = (unsigned*)ptr;
+  z.off[z.n + 1] = z.off[z.n] + bytes / 4;
+  z.n++;
+}
+
+void ZeroList::flush() {
+  if (z.n > 0) k_zero_many<<<nblocks(z.off[z.n]), kBlock, 0, s>>>(z);
+  z.n = 0;
+  z.off[0] = 0;
 }
 
 void add_diag_levels(hipStream_t s, const DiagLevels& L) {

@@ -987,6 +987,7 @@ void alloc_hierarchy(PmgDev* d, int nlev, bool newton, bool mixed) {
   // its interiors are rewritten before they are read.
   if (d->ntot > 0 && d->nlev_ref == nlev && d->h_newton == newton && d->h_mixed == mixed) return;
   free_hierarchy(d);
+  pmg::ZeroList z(d->s);
   d->h_newton = newton;
   d->h_mixed = mixed;
   int dims[24][3];
@@ -1002,66 +1003,66 @@ void alloc_hierarchy(PmgDev* d, int nlev, bool newton, bool mixed) {
     const int na = l == 0 ? 4 : 14;
     for (int s = 0; s < na; s++) {
       L.a[s] = dalloc<double>(L.n);
-      pmg::fill<double>(d->s, L.a[s], L.n, 0.0);
+      z.add(L.a[s], (L.n) * sizeof(double));
     }
     L.cc = dalloc<double>(L.n);
     L.fc = dalloc<double>(L.n);
     L.D = dalloc<double>(L.n);
-    pmg::fill<double>(d->s, L.cc, L.n, 0.0);
-    pmg::fill<double>(d->s, L.fc, L.n, 0.0);
-    pmg::fill<double>(d->s, L.D, L.n, 0.0);
+    z.add(L.cc, (L.n) * sizeof(double));
+    z.add(L.fc, (L.n) * sizeof(double));
+    z.add(L.D, (L.n) * sizeof(double));
     L.x = l == 0 ? d->u : dalloc<double>(L.n);
     if (l > 0) {
       L.xs = dalloc<double>(L.n);
-      pmg::fill<double>(d->s, L.xs, L.n, 0.0);
+      z.add(L.xs, (L.n) * sizeof(double));
     }
     L.w0 = dalloc<double>(L.n);
-    pmg::fill<double>(d->s, L.w0, L.n, 0.0);
-    if (l > 0) pmg::fill<double>(d->s, L.x, L.n, 0.0);
+    z.add(L.w0, (L.n) * sizeof(double));
+    if (l > 0) z.add(L.x, (L.n) * sizeof(double));
     if (newton) {
       L.cprime = dalloc<double>(L.n);
-      pmg::fill<double>(d->s, L.cprime, L.n, 0.0);
+      z.add(L.cprime, (L.n) * sizeof(double));
       L.xtmp = dalloc<double>(L.n);
-      pmg::fill<double>(d->s, L.xtmp, L.n, 0.0);
+      z.add(L.xtmp, (L.n) * sizeof(double));
     }
     if (mixed) {
       for (int s = 1; s < na; s++) {  // oC is folded into Df
         L.af[s] = dalloc<float>(L.n);
-        pmg::fill<float>(d->s, L.af[s], L.n, 0.0f);
+        z.add(L.af[s], (L.n) * sizeof(float));
       }
       L.Df = dalloc<float>(L.n);
-      pmg::fill<float>(d->s, L.Df, L.n, 0.0f);
+      z.add(L.Df, (L.n) * sizeof(float));
       L.xfl = dalloc<float>(L.n);
       L.w0f = dalloc<float>(L.n);
       if (l > 0) {
         L.xsf = dalloc<float>(L.n);
-        pmg::fill<float>(d->s, L.xsf, L.n, 0.0f);
+        z.add(L.xsf, (L.n) * sizeof(float));
       }
-      pmg::fill<float>(d->s, L.xfl, L.n, 0.0f);
-      pmg::fill<float>(d->s, L.w0f, L.n, 0.0f);
+      z.add(L.xfl, (L.n) * sizeof(float));
+      z.add(L.w0f, (L.n) * sizeof(float));
     }
   }
   d->w1 = dalloc<double>(d->n);
-  pmg::fill<double>(d->s, d->w1, d->n, 0.0);
+  z.add(d->w1, (d->n) * sizeof(double));
   d->xs64 = dalloc<double>(d->n);
-  pmg::fill<double>(d->s, d->xs64, d->n, 0.0);
+  z.add(d->xs64, (d->n) * sizeof(double));
   if (mixed) {
     d->w1f = dalloc<float>(d->n);
-    pmg::fill<float>(d->s, d->w1f, d->n, 0.0f);
+    z.add(d->w1f, (d->n) * sizeof(float));
     d->xs32 = dalloc<float>(d->n);
-    pmg::fill<float>(d->s, d->xs32, d->n, 0.0f);
+    z.add(d->xs32, (d->n) * sizeof(float));
   }
   d->F = dalloc<double>(d->n);
-  pmg::fill<double>(d->s, d->F, d->n, 0.0);
+  z.add(d->F, (d->n) * sizeof(double));
   if (newton) {
     d->wn1 = dalloc<double>(d->n);
     d->wn2 = dalloc<double>(d->n);
     d->wn3 = dalloc<double>(d->n);
     d->nlw = dalloc<double>(d->n);
-    pmg::fill<double>(d->s, d->nlw, d->n, 0.0);
-    pmg::fill<double>(d->s, d->wn1, d->n, 0.0);
-    pmg::fill<double>(d->s, d->wn2, d->n, 0.0);
-    pmg::fill<double>(d->s, d->wn3, d->n, 0.0);
+    z.add(d->nlw, (d->n) * sizeof(double));
+    z.add(d->wn1, (d->n) * sizeof(double));
+    z.add(d->wn2, (d->n) * sizeof(double));
+    z.add(d->wn3, (d->n) * sizeof(double));
   }
   const DLevel& C = d->L[d->ntot - 1];
   d->dn = (C.nx - 2) * (C.ny - 2) * (C.nz - 2);
@@ -1475,13 +1476,16 @@ extern "C" int Vpmg_ctor2(Vpmg* thee, Vpmgp* pmgp, Vpbe* pbe, int focusFlag, Vpm
   // device memory is recycled between objects: start every array at zero,
   // as the reference's zeroed allocations do (the boundary values of
   // bcfl zero, for one, are never written)
-  for (double* a : {d->epsx, d->epsy, d->epsz, d->kappa, d->charge, d->u, d->pvec, d->ccf})
-    pmg::fill<double>(d->s, a, n, 0.0);
-  pmg::fill<double>(d->s, d->gx, 2 * (size_t)d->ny * d->nz, 0.0);
-  pmg::fill<double>(d->s, d->gy, 2 * (size_t)d->nx * d->nz, 0.0);
-  pmg::fill<double>(d->s, d->gz, 2 * (size_t)d->nx * d->ny, 0.0);
-  pmg::fill<double>(d->s, d->part, 2 * pmg::kParts, 0.0);
-  pmg::fill<double>(d->s, d->scal, 32, 0.0);
+  {
+    pmg::ZeroList z(d->s);
+    for (double* a : {d->epsx, d->epsy, d->epsz, d->kappa, d->charge, d->u, d->pvec, d->ccf})
+      z.add(a, n * sizeof(double));
+    z.add(d->gx, 2 * (size_t)d->ny * d->nz * sizeof(double));
+    z.add(d->gy, 2 * (size_t)d->nx * d->nz * sizeof(double));
+    z.add(d->gz, 2 * (size_t)d->nx * d->ny * sizeof(double));
+    z.add(d->part, 2 * pmg::kParts * sizeof(double));
+    z.add(d->scal, 32 * sizeof(double));
+  }
   d->hscal = pmgpool::pinned_scalars();
   Valist* al = pbe->alist;
   d->natoms = al->number;
@@ -1496,8 +1500,11 @@ extern "C" int Vpmg_ctor2(Vpmg* thee, Vpmgp* pmgp, Vpbe* pbe, int focusFlag, Vpm
   d->achg = dalloc<double>(d->natoms);
   d->apart = dalloc<double>(d->natoms);
   d->atmp = dalloc<double>(d->natoms);
-  pmg::fill<double>(d->s, d->apart, d->natoms, 0.0);
-  pmg::fill<double>(d->s, d->atmp, d->natoms, 0.0);
+  {
+    pmg::ZeroList z(d->s);
+    z.add(d->apart, (size_t)d->natoms * sizeof(double));
+    z.add(d->atmp, (size_t)d->natoms * sizeof(double));
+  }
   HIPCHECK(hipMemcpyAsync(d->apos, pos.data(), pos.size() * sizeof(double), hipMemcpyHostToDevice, d->s));
   HIPCHECK(hipMemcpyAsync(d->arad, rad.data(), rad.size() * sizeof(double), hipMemcpyHostToDevice, d->s));
   HIPCHECK(hipMemcpyAsync(d->achg, chg.data(), chg.size() * sizeof(double), hipMemcpyHostToDevice, d->s));
