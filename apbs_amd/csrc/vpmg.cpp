@@ -725,8 +725,11 @@ struct PmgDev {
   std::vector<Mark> marks;
   int ev_next = 0;    // next free event of the pool
   // V-cycles captured as HIP graphs, keyed by precision, solution and
-  // right-hand side (environment APBS_PMG_GRAPHS=0 launches eagerly)
-  int graphs = 1;
+  // right-hand side, when the environment sets APBS_PMG_GRAPHS=1.  Eager
+  // launches are the default: on MI355X the replays were no faster (161^3
+  // npbe mixed: 61.2 single-request and 81.4 served solves/s replayed,
+  // 61.8 and 84.4 eager), and each new hierarchy pays the capture.
+  int graphs = 0;
   struct CycleGraph {
     int word;
     const void *f, *x;
@@ -1451,7 +1454,7 @@ extern "C" int Vpmg_ctor2(Vpmg* thee, Vpmgp* pmgp, Vpbe* pbe, int focusFlag, Vpm
   if (const char* sm = getenv("APBS_PMG_SMOOTHER")) d->smoother = strcmp(sm, "colour") == 0 ? 1 : 0;
   if (const char* sm = getenv("APBS_PMG_SMOOTHER27"))
     d->smoother27 = strcmp(sm, "colour") == 0 ? 1 : 0;
-  if (const char* gr = getenv("APBS_PMG_GRAPHS")) d->graphs = strcmp(gr, "0") == 0 ? 0 : 1;
+  if (const char* gr = getenv("APBS_PMG_GRAPHS")) d->graphs = strcmp(gr, "1") == 0 ? 1 : 0;
   d->nx = pmgp->nx;
   d->ny = pmgp->ny;
   d->nz = pmgp->nz;
