@@ -1359,17 +1359,26 @@ __global__ void __launch_bounds__(256) k_dense_invert_reg(double* Mg, int N) {
     if (r < N && c0 + q < N) Mg[(size_t)r * N + c0 + q] = m[q];
 }
 
+// x_r = sum_c Minv[r][c] f_c over the interior unknowns, N <= kDenseSmall:
+// one workgroup gathers f into LDS, then each row sums in column order.
+constexpr int kDenseSmall = 96;
+
 template <class T>
-__global__ void k_dense_solve(int nx, int ny, int nz, const double* Minv, int N, const T* f,
-                              T* x) {
-  const int r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= N) return;
+__global__ void __launch_bounds__(128) k_dense_solve(int nx, int ny, int nz, const double* Minv,
+                                                     int N, const T* f, T* x) {
+  __shared__ double fv[kDenseSmall];
   const int ni = nx - 2, nj = ny - 2;
-  double s = 0.0;
-  for (int c = 0; c < N; c++) {
+  for (int c = threadIdx.x; c < N; c += blockDim.x) {
     const int i = c % ni + 1, j = (c / ni) % nj + 1, k = c / (ni * nj) + 1;
-    s += Minv[(size_t)r * N + c] * (double)f[IX(i, j, k, nx, ny)];
+    fv[c] = (double)f[IX(i, j, k, nx, ny)];
   }
+  __syncthreads();
+  const int r = threadIdx.x;
+  if (r >= N) return;
+  const double* row = Minv + (size_t)r * N;
+  double s = 0.0;
+#pragma unroll 8
+  for (int c = 0; c < N; c++) s += row[c] * fv[c];
   const int i = r % ni + 1, j = (r / ni) % nj + 1, k = r / (ni * nj) + 1;
   x[IX(i, j, k, nx, ny)] = (T)s;
 }
@@ -1785,8 +1794,8 @@ void dense_invert(hipStream_t s, double* M, double* work, int N, int form) {
 template <class T>
 void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int N, const T* f,
                  T* x) {
-  if (((size_t)N * N + N) * sizeof(double) <= 64 * 1024)
-    k_dense_solve<T><<<(N + kBlock - 1) / kBlock, kBlock, 0, s>>>(nx, ny, nz, Minv, N, f, x);
+  if (((size_t)N * N + N) * sizeof(double) <= 64 * 1024 && N <= kDenseSmall)
+    k_dense_solve<T><<<1, 128, 0, s>>>(nx, ny, nz, Minv, N, f, x);
   else
     k_dense_solve_cols<T><<<(N + kBlock - 1) / kBlock, kBlock, 0, s>>>(nx, ny, nz, Minv, N, f, x);
 }

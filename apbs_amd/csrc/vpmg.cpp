@@ -710,8 +710,14 @@ struct PmgDev {
   // on the levels where each is faster (gsrb27_use_planes, gsrb27_use_pairs),
   // 1 one launch per colour everywhere (APBS_PMG_SMOOTHER27=colour selects 1)
   int smoother27 = 0;
-  // fillco's MDH boundary on a second stream (pmgint::set_side_stream)
+  // fillco's MDH boundary, and the coarsest level's dense inverse, on second
+  // streams (pmgint::set_side_stream)
   int side = 1;
+  // the dense inverse's stream: set_diagonals forks it after the diagonals
+  // are formed, the V-cycle joins it before the coarsest solve
+  hipStream_t cs = nullptr;
+  hipEvent_t cs_fork = nullptr, cs_join = nullptr;
+  bool cs_pending = false;
   int mirror = 1;
   double stats[8] = {};
   pmg::Geo geo{};
@@ -875,6 +881,8 @@ void free_graphs(PmgDev* d) {
 }
 
 void free_hierarchy(PmgDev* d) {
+  if (d->cs) HIPCHECK(hipStreamSynchronize(d->cs));
+  d->cs_pending = false;
   free_graphs(d);
   for (int l = 0; l < d->ntot; l++) {
     DLevel& L = d->L[l];
@@ -1107,8 +1115,33 @@ void set_diagonals(PmgDev* d, bool use_cprime) {
   }
   pmg::add_diag_levels(d->s, dl);
   const DLevel& C = d->L[d->ntot - 1];
-  pmg::dense_assemble(d->s, C.nx, C.ny, C.nz, C.ops(), C.D, d->dense, d->dn);
-  pmg::dense_invert(d->s, d->dense, d->dwork, d->dn);
+  // The inverse is first read by the coarsest solve, after the downward
+  // half of the next V-cycle; a one-workgroup kernel, it runs on its own
+  // stream meanwhile.  Captured cycles keep it on the object's stream.
+  hipStream_t s = d->s;
+  if (d->side && !d->graphs) {
+    if (!d->cs) {
+      d->cs = pmgpool::stream_acquire();
+      HIPCHECK(hipEventCreateWithFlags(&d->cs_fork, hipEventDisableTiming));
+      HIPCHECK(hipEventCreateWithFlags(&d->cs_join, hipEventDisableTiming));
+    }
+    HIPCHECK(hipEventRecord(d->cs_fork, d->s));
+    HIPCHECK(hipStreamWaitEvent(d->cs, d->cs_fork, 0));
+    s = d->cs;
+  }
+  pmg::dense_assemble(s, C.nx, C.ny, C.nz, C.ops(), C.D, d->dense, d->dn);
+  pmg::dense_invert(s, d->dense, d->dwork, d->dn);
+  if (s != d->s) {
+    HIPCHECK(hipEventRecord(d->cs_join, s));
+    d->cs_pending = true;
+  }
+}
+
+// Join the dense inverse's stream before the first read of the inverse.
+void join_dense(PmgDev* d) {
+  if (!d->cs_pending) return;
+  HIPCHECK(hipStreamWaitEvent(d->s, d->cs_join, 0));
+  d->cs_pending = false;
 }
 
 // FP32 images of the level operators (the stencil arrays, which do not
@@ -1182,6 +1215,7 @@ void vcycle(PmgDev* d, CycleArgs<T>& c) {
     }
   }
   const DLevel& C = L[nt - 1];
+  join_dense(d);
   pmg::dense_solve<T>(s, C.nx, C.ny, C.nz, d->dense, d->dn, c.w0[nt - 1], c.X[nt - 1]);
   for (int l = nt - 2; l >= 0; l--) {
     const DLevel& Cl = L[l + 1];
@@ -1551,6 +1585,11 @@ extern "C" void Vpmg_dtor2(Vpmg* thee) {
   if (d) {
     HIPCHECK(hipStreamSynchronize(d->s));
     free_hierarchy(d);
+    if (d->cs) {
+      HIPCHECK(hipEventDestroy(d->cs_fork));
+      HIPCHECK(hipEventDestroy(d->cs_join));
+      pmgpool::stream_release(d->cs);
+    }
     dfree(d->epsx);
     dfree(d->epsy);
     dfree(d->epsz);
