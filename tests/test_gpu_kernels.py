@@ -98,11 +98,12 @@ def test_gsrb27_plane_sweeps_bitwise(L, O, n3, mode):
 @pytest.mark.parametrize("tile", [416, 48, 816, 88, 84, 1616, 832, 44, 68, 640048, 800048,
                                   960048, 800044, 800068, 960416, 800816, 5088, 645084,
                                   805816])
-def test_fused_sweep_tiles_bitwise(L, O, tile):
+@pytest.mark.parametrize("n3", [(133, 21, 38), (133, 21, 41)])
+def test_fused_sweep_tiles_bitwise(L, O, tile, n3):
     """Every tile shape of the fused red-black sweep gives the oracle's
-    sweep bit for bit (partial tiles in every direction)."""
+    sweep bit for bit (partial tiles in every direction; 41 planes leave an
+    odd number of planes in the last chunk)."""
     rng = np.random.default_rng(tile)
-    n3 = (133, 21, 38)
     nx, ny, nz = n3
     n = nx * ny * nz
     a = rand_op(rng, n3, 7)
