@@ -701,27 +701,46 @@ __device__ inline int face_count(const Geo& g, int f) {
 // the chunk, whose sequential partial sum goes to part[chunk][face][t], and
 // k_bc_mdh_sum adds the partials in chunk order.  With one chunk the sum is
 // the reference's atom-order sum.
+// Per-atom factors of the MDH term: q = ec * charge, the screening
+// denominator 1 + kappa * radius and its correctly rounded reciprocal.
+__global__ void __launch_bounds__(kB) k_bc_mdh_atoms(AtomsDev A, double xkappa, double* fac) {
+  const int a = blockIdx.x * kB + threadIdx.x;
+  if (a >= A.n) return;
+  const double ec = 1.6021773e-19;  // Vunit_ec, src/generic/vunit.h:80
+  const double den = 1 + xkappa * A.rad[a];
+  fac[3 * a] = ec * A.chg[a];
+  fac[3 * a + 1] = den;
+  fac[3 * a + 2] = 1.0 / den;
+}
+
 __global__ void __launch_bounds__(kB)
-    k_bc_mdh(Geo g, const double* xf, const double* yf, const double* zf, AtomsDev A, double pre1,
-             double xkappa, int nch, int mx, double* part) {
+    k_bc_mdh(Geo g, const double* xf, const double* yf, const double* zf, AtomsDev A,
+             const double* fac, double pre1, double xkappa, int nch, int mx, double* part) {
   const int f = blockIdx.y;
   const int t = blockIdx.x * kB + threadIdx.x;
   if (t >= face_count(g, f)) return;
   int i, j, k;
   face_point(g, f, t, i, j, k);
   const double px = xf[i], py = yf[j], pz = zf[k];
-  const double ec = 1.6021773e-19;  // Vunit_ec, src/generic/vunit.h:80
   const int c = blockIdx.z;
   const int per = (A.n + nch - 1) / nch;
   const int a0 = c * per, a1 = min(A.n, a0 + per);
-  // One atom's term, as the reference writes it.
+  // One atom's term, as the reference writes it:
+  //   pre1 * (q / dist) * exp(-kappa * (dist - radius)) / (1 + kappa * radius).
+  // The last division is q0 = v * y, then q0 + (v - q0 * den) * y with two
+  // FMAs: y the correctly rounded 1 / den, this is the correctly rounded
+  // quotient v / den (Markstein's theorem; no underflow at these magnitudes).
   auto term = [&](int a) {
     const double* ap = A.pos + 3 * a;
-    const double q = ec * A.chg[a];
+    const double q = fac[3 * a];
     const double dist = sqrt((px - ap[0]) * (px - ap[0]) + (py - ap[1]) * (py - ap[1]) +
                              (pz - ap[2]) * (pz - ap[2]));
-    if (xkappa > VSMALL)
-      return pre1 * (q / dist) * exp(-xkappa * (dist - A.rad[a])) / (1 + xkappa * A.rad[a]);
+    if (xkappa > VSMALL) {
+      const double v = pre1 * (q / dist) * exp(-xkappa * (dist - A.rad[a]));
+      const double den = fac[3 * a + 1], y = fac[3 * a + 2];
+      const double q0 = v * y;
+      return fma(fma(-q0, den, v), y, q0);
+    }
     return pre1 * (q / dist);
   };
   // four independent terms in flight (their dependency chains interleave),
@@ -1421,7 +1440,7 @@ size_t bc_mdh_scratch(const Geo& g, int natoms) {
   size_t mx = (size_t)g.ny * g.nz;
   if ((size_t)g.nx * g.nz > mx) mx = (size_t)g.nx * g.nz;
   if ((size_t)g.nx * g.ny > mx) mx = (size_t)g.nx * g.ny;
-  return (size_t)bc_mdh_chunks(natoms) * 6 * mx;
+  return (size_t)bc_mdh_chunks(natoms) * 6 * mx + 3 * (size_t)(natoms > 0 ? natoms : 1);
 }
 
 void bc_mdh(hipStream_t s, const Geo& g, const double* xf, const double* yf, const double* zf,
@@ -1431,8 +1450,11 @@ void bc_mdh(hipStream_t s, const Geo& g, const double* xf, const double* yf, con
   if (g.nx * g.nz > mx) mx = g.nx * g.nz;
   if (g.nx * g.ny > mx) mx = g.nx * g.ny;
   const int nch = bc_mdh_chunks(A.n);
-  k_bc_mdh<<<dim3((mx + kB - 1) / kB, 6, nch), kB, 0, s>>>(g, xf, yf, zf, A, pre1, xkappa, nch, mx,
-                                                           part);
+  // the per-atom factors follow the partial sums in the scratch buffer
+  double* fac = part + (size_t)nch * 6 * mx;
+  if (A.n > 0) k_bc_mdh_atoms<<<(A.n + kB - 1) / kB, kB, 0, s>>>(A, xkappa, fac);
+  k_bc_mdh<<<dim3((mx + kB - 1) / kB, 6, nch), kB, 0, s>>>(g, xf, yf, zf, A, fac, pre1, xkappa,
+                                                           nch, mx, part);
   k_bc_mdh_sum<<<dim3((mx + kB - 1) / kB, 6), kB, 0, s>>>(g, nch, mx, part, gx, gy, gz);
 }
 
