@@ -168,6 +168,15 @@ void axpy_interior_f32(hipStream_t s, int nx, int ny, int nz, double a, const fl
                        double* y);
 template <class Tin, class Tout>
 void convert(hipStream_t s, size_t n, const Tin* in, Tout* out);
+// FP32 images of up to kConvMax FP64 arrays of one length in one launch.
+constexpr int kConvMax = 16;
+struct ConvArgs {
+  int m;
+  size_t n;
+  const double* in[kConvMax];
+  float* out[kConvMax];
+};
+void convert_many(hipStream_t s, const ConvArgs& c);
 void set_faces(hipStream_t s, int nx, int ny, int nz, double* u, const double* gx,
                const double* gy, const double* gz);
 

@@ -8,6 +8,7 @@
 // in-place lexicographic sweep (gsd.c:230-327), whose dependencies are serial.
 #include "pmg_device.h"
 
+#include <cstdio>
 #include <cmath>
 #include <cstdlib>
 #include <utility>
@@ -821,6 +822,15 @@ template <class Tin, class Tout>
 __global__ void k_convert(size_t n, const Tin* __restrict__ in, Tout* __restrict__ out) {
   for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (size_t)gridDim.x * kBlock)
     out[t] = (Tout)in[t];
+}
+
+// array blockIdx.y of the list
+__global__ void __launch_bounds__(kBlock) k_convert_many(ConvArgs C) {
+  const double* __restrict__ in = C.in[blockIdx.y];
+  float* __restrict__ out = C.out[blockIdx.y];
+  for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < C.n;
+       t += (size_t)gridDim.x * kBlock)
+    out[t] = (float)in[t];
 }
 
 // VfboundPMG (mikpckd.c:209-256): x faces, then y faces, then z faces, so
@@ -1712,6 +1722,15 @@ void axpy_interior_f32(hipStream_t s, int nx, int ny, int nz, double a, const fl
 
 void scale(hipStream_t s, size_t n, double a, const double* x, double* y) {
   k_scale<<<nblocks(n), kBlock, 0, s>>>(n, a, x, y);
+}
+
+void convert_many(hipStream_t s, const ConvArgs& c) {
+  if (c.m <= 0 || c.n == 0) return;
+  if (c.m > kConvMax) {
+    fprintf(stderr, "convert_many: %d arrays, at most %d\n", c.m, kConvMax);
+    abort();
+  }
+  k_convert_many<<<dim3(nblocks(c.n), (unsigned)c.m), kBlock, 0, s>>>(c);
 }
 
 template <class Tin, class Tout>

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <utility>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -1150,8 +1151,14 @@ void join_dense(PmgDev* d) {
 void sync_fp32_ops(PmgDev* d) {
   for (int l = 0; l < d->ntot; l++) {
     DLevel& L = d->L[l];
-    const int na = l == 0 ? 4 : 14;
-    for (int s = 1; s < na; s++) pmg::convert<double, float>(d->s, L.n, L.a[s], L.af[s]);
+    pmg::ConvArgs c{};
+    c.m = l == 0 ? 3 : 13;
+    c.n = L.n;
+    for (int s = 1; s <= c.m; s++) {
+      c.in[s - 1] = L.a[s];
+      c.out[s - 1] = L.af[s];
+    }
+    pmg::convert_many(d->s, c);
   }
 }
 
@@ -1422,8 +1429,11 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
         } else if (xnorm_new > xnorm_med) {
           break;
         }
-        pmg::copy_interior<double>(s, nx, ny, nz, d->wn1, d->wn2);
-        pmg::copy_interior<double>(s, nx, ny, nz, d->F, d->wn3);
+        // keep the candidate and its residual (wn2, wn3) by exchanging the
+        // buffers: every one of them has a zero boundary layer and is written
+        // on the interior only
+        std::swap(d->wn1, d->wn2);
+        std::swap(d->F, d->wn3);
         xnorm_med = xnorm_new;
         damp = damp / 2.0;
         pmg::xpay_interior(s, nx, ny, nz, d->u, damp, xtmp, d->wn1);
@@ -1432,7 +1442,7 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
         iter_d++;
       }
       pmg::copy_interior<double>(s, nx, ny, nz, d->wn2, d->u);
-      pmg::copy_interior<double>(s, nx, ny, nz, d->wn3, d->F);
+      std::swap(d->F, d->wn3);
       xnorm_new = xnorm_med;
       xnorm_old = xnorm_new;
       if ((iter_d - 1) == 0) idamp = 0;
