@@ -104,16 +104,19 @@ void prolong_add(hipStream_t s, int nxc, int nyc, int nzc, int nxf, int nyf, int
 // ------------------------------------------------------------ reductions
 // Each writes one double to out (device), deterministically.
 template <class T>
-void nrm1(hipStream_t s, int nx, int ny, int nz, const T* x, double* part, double* out);
+void nrm1(hipStream_t s, int nx, int ny, int nz, const T* x, double* part, double* out,
+          double* hout = nullptr);
 // r = f + sum a x - D x on the 7-point level (stored when r != nullptr) and
 // out[0] = ||r||_1 over the interior, bitwise equal to resid + nrm1.
 template <class T>
 void resid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D,
-                 const T* fc, const T* x, T* r, double* part, double* out);
+                 const T* fc, const T* x, T* r, double* part, double* out, double* hout = nullptr);
 // nresid7 followed by nrm1 of its result, in one pass.
 void nresid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* cc,
                   const double* fc, const double* x, double* r, const Ions& ions, double* part,
-                  double* out);
+                  double* out, double* hout = nullptr);
+// (hout: host-mapped pinned memory that also receives out[0], so the host
+// reads the norm after a stream synchronization without a copy)
 // out[0] = <x, w0> / <x, A x>   (Hackbusch-Reusken damping, 27-point A)
 template <class T>
 void damp_factor(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D,

@@ -570,7 +570,9 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // out[0] = sum(part[0..kParts-1]); ratio: out[0] = sum(a)/sum(b)
-__global__ void __launch_bounds__(kBlock) k_finish(const double* part, double* out, int ratio) {
+// (and hout[0] = out[0] when hout is given)
+__global__ void __launch_bounds__(kBlock) k_finish(const double* part, double* out, int ratio,
+                                                   double* hout) {
   __shared__ double sh[kBlock];
   double a = 0.0, b = 0.0;
   for (int t = threadIdx.x; t < kParts; t += kBlock) {
@@ -587,6 +589,9 @@ __global__ void __launch_bounds__(kBlock) k_finish(const double* part, double* o
     } else {
       out[0] = sa;
     }
+    // a system-scope store: the value reaches host memory past the GPU caches
+    if (hout) __hip_atomic_store(hout, ratio ? sa / sb : sa, __ATOMIC_RELEASE,
+                                 __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -1645,31 +1650,32 @@ void prolong_add(hipStream_t s, int nxc, int nyc, int nzc, int nxf, int nyf, int
 }
 
 template <class T>
-void nrm1(hipStream_t s, int nx, int ny, int nz, const T* x, double* part, double* out) {
+void nrm1(hipStream_t s, int nx, int ny, int nz, const T* x, double* part, double* out,
+          double* hout) {
   k_nrm1_part<T><<<kParts, kBlock, 0, s>>>(nx, ny, nz, x, part);
-  k_finish<<<1, kBlock, 0, s>>>(part, out, 0);
+  k_finish<<<1, kBlock, 0, s>>>(part, out, 0, hout);
 }
 
 template <class T>
 void resid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
-                 const T* x, T* r, double* part, double* out) {
+                 const T* x, T* r, double* part, double* out, double* hout) {
   k_resid7_nrm<T><<<kParts, kBlock, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, x, r, part);
-  k_finish<<<1, kBlock, 0, s>>>(part, out, 0);
+  k_finish<<<1, kBlock, 0, s>>>(part, out, 0, hout);
 }
 
 void nresid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* cc,
                   const double* fc, const double* x, double* r, const Ions& ions, double* part,
-                  double* out) {
+                  double* out, double* hout) {
   k_nresid7_nrm<<<kParts, kBlock, 0, s>>>(nx, ny, nz, A.a[0], A.a[1], A.a[2], A.a[3], cc, fc, x, r,
                                           ions, part);
-  k_finish<<<1, kBlock, 0, s>>>(part, out, 0);
+  k_finish<<<1, kBlock, 0, s>>>(part, out, 0, hout);
 }
 
 template <class T>
 void damp_factor(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* x,
                  const T* w0, double* part, double* out) {
   k_damp_part<T><<<kParts, kBlock, 0, s>>>(nx, ny, nz, to27(A), D, x, w0, part);
-  k_finish<<<1, kBlock, 0, s>>>(part, out, 1);
+  k_finish<<<1, kBlock, 0, s>>>(part, out, 1, nullptr);
 }
 
 template <class T>
@@ -1834,9 +1840,9 @@ void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int 
   template void restrict3<T>(hipStream_t, int, int, int, int, int, int, const T*, T*, T*);      \
   template void prolong_add<T>(hipStream_t, int, int, int, int, int, int, const T*, T*,         \
                                const double*);                                                  \
-  template void nrm1<T>(hipStream_t, int, int, int, const T*, double*, double*);                \
+  template void nrm1<T>(hipStream_t, int, int, int, const T*, double*, double*, double*);       \
   template void resid7_nrm1<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*,   \
-                               const T*, T*, double*, double*);                                 \
+                               const T*, T*, double*, double*, double*);                        \
   template void damp_factor<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*,   \
                                const T*, double*, double*);                                     \
   template void fill<T>(hipStream_t, T*, size_t, T);                                            \

@@ -755,6 +755,13 @@ double read_scalar(PmgDev* d, const double* dptr) {
   return d->hscal[0];
 }
 
+// The norm a reduction launched with hout = d->hscal wrote to host-mapped
+// memory (visible once the stream has drained).
+double read_norm(PmgDev* d) {
+  HIPCHECK(hipStreamSynchronize(d->s));
+  return d->hscal[0];
+}
+
 int tbegin(PmgDev* d) {
   if (!d->timing) return -1;
   const int e = d->ev_next;
@@ -1287,8 +1294,8 @@ int mvcs64(PmgDev* d, double* const* X, const double* f, int istop, int itmax, d
   const DLevel& F = d->L[0];
   double rsden = 1.0;
   if (istop == 1) {
-    pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, f, d->part, d->scal);
-    rsden = read_scalar(d, d->scal);
+    pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, f, d->part, d->scal, d->hscal);
+    rsden = read_norm(d);
   }
   if (rsden == 0.0) rsden = 1.0;
   CycleArgs<double> c = args64(d, X, f);
@@ -1298,8 +1305,8 @@ int mvcs64(PmgDev* d, double* const* X, const double* f, int istop, int itmax, d
     run_vcycle<double>(d, c);
     iters++;
     pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, c.A[0], c.Dl[0], f, X[0], nullptr, d->part,
-                             d->scal);
-    rsnrm = read_scalar(d, d->scal);
+                             d->scal, d->hscal);
+    rsnrm = read_norm(d);
   } while (iters < itmax && (rsnrm / rsden) > errtol);
   if (relres) *relres = rsnrm / rsden;
   return iters;
@@ -1312,8 +1319,8 @@ int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int i
   const DLevel& F = d->L[0];
   double rsden = 1.0, fnrm = -1.0;
   if (istop == 1 || x_zero) {
-    pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, f, d->part, d->scal);
-    fnrm = read_scalar(d, d->scal);
+    pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, f, d->part, d->scal, d->hscal);
+    fnrm = read_norm(d);
     if (istop == 1) rsden = fnrm;
   }
   if (rsden == 0.0) rsden = 1.0;
@@ -1340,8 +1347,8 @@ int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int i
     rsnrm = fnrm;
   } else {
     pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, A0, F.D, f, X64[0], d->w1, d->part,
-                             d->scal);
-    rsnrm = read_scalar(d, d->scal);
+                             d->scal, d->hscal);
+    rsnrm = read_norm(d);
   }
   while (it < itmax && (rsnrm / rsden) > errtol) {
     pmg::convert<double, float>(d->s, F.n, r, d->L[0].w0f);
@@ -1352,8 +1359,8 @@ int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int i
     pmg::axpy_interior_f32(d->s, F.nx, F.ny, F.nz, 1.0, d->L[0].xfl, X64[0]);
     it++;
     pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, A0, F.D, f, X64[0], d->w1, d->part,
-                             d->scal);
-    rsnrm = read_scalar(d, d->scal);
+                             d->scal, d->hscal);
+    rsnrm = read_norm(d);
   }
   if (relres) *relres = rsnrm / rsden;
   if (outer) *outer = it;
@@ -1382,11 +1389,11 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
   hipStream_t s = d->s;
   // rsden = ||F(0)||1
   pmg::fill<double>(s, d->wn1, d->n, 0.0);
-  pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->wn2, I, d->part, d->scal);
-  double rsden = read_scalar(d, d->scal);
+  pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->wn2, I, d->part, d->scal, d->hscal);
+  double rsden = read_norm(d);
   if (rsden == 0.0) rsden = 1.0;
-  pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->u, d->F, I, d->part, d->scal);
-  double xnorm_old = read_scalar(d, d->scal);
+  pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->u, d->F, I, d->part, d->scal, d->hscal);
+  double xnorm_old = read_norm(d);
   double xnorm_new = xnorm_old, xnorm_med = 0.0, rsnrm = rsden;
   int idamp = 1, iters = 0, inner = 0;
   const double errtol = thee->pmgp->errtol;
@@ -1419,8 +1426,8 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
     double* xtmp = L[0].xtmp;
     if (idamp == 1) {
       pmg::xpay_interior(s, nx, ny, nz, d->u, 1.0, xtmp, d->wn1);
-      pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->F, I, d->part, d->scal);
-      xnorm_new = read_scalar(d, d->scal);
+      pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->F, I, d->part, d->scal, d->hscal);
+      xnorm_new = read_norm(d);
       double damp = 1.0;
       int iter_d = 0, mode = 0;
       while (iter_d < 10) {
@@ -1437,8 +1444,8 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
         xnorm_med = xnorm_new;
         damp = damp / 2.0;
         pmg::xpay_interior(s, nx, ny, nz, d->u, damp, xtmp, d->wn1);
-        pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->F, I, d->part, d->scal);
-        xnorm_new = read_scalar(d, d->scal);
+        pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->F, I, d->part, d->scal, d->hscal);
+        xnorm_new = read_norm(d);
         iter_d++;
       }
       pmg::copy_interior<double>(s, nx, ny, nz, d->wn2, d->u);
@@ -1448,8 +1455,8 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
       if ((iter_d - 1) == 0) idamp = 0;
     } else {
       pmg::axpy_interior<double>(s, nx, ny, nz, 1.0, xtmp, d->u);
-      pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->u, d->F, I, d->part, d->scal);
-      xnorm_new = read_scalar(d, d->scal);
+      pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->u, d->F, I, d->part, d->scal, d->hscal);
+      xnorm_new = read_norm(d);
       xnorm_old = xnorm_new;
     }
     rsnrm = xnorm_new;
