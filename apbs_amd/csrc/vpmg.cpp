@@ -1104,10 +1104,9 @@ void build_operators(PmgDev* d) {
   }
 }
 
-// D_l = oC_l + c_l for the given per-level Helmholtz arrays, then the dense
-// inverse of the coarsest level.
-// D_l = oC_l + c_l (and, for a mixed-precision solve, its FP32 image Df_l in
-// the same pass), then the dense inverse of the coarsest level.
+// D_l = oC_l + c_l for the given per-level Helmholtz arrays (and, for a
+// mixed-precision solve, its FP32 image Df_l in the same pass), then the
+// dense inverse of the coarsest level.
 void set_diagonals(PmgDev* d, bool use_cprime) {
   const bool f32 = d->precision == 1 && d->h_mixed;
   pmg::DiagLevels dl{};
