@@ -201,8 +201,11 @@ void galerkin(hipStream_t s, int st, int nxf, int nyf, int nzf, const Ops<double
 void dense_assemble(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* D,
                     double* M, int N);
 // form 0: registers for N <= 64 (one barrier per pivot), else LDS / global;
-// form 1: never the register form
-void dense_invert(hipStream_t s, double* M, double* work, int N, int form = 0);
+// form 1: never the register form.  flag (host-mapped, may be null) receives
+// the 1-based index of a pivot that is not positive and finite (the operator
+// is not positive definite); it is left alone otherwise.
+void dense_invert(hipStream_t s, double* M, double* work, int N, int form = 0,
+                  double* flag = nullptr);
 // x(interior) = Minv * f(interior); boundary untouched
 template <class T>
 void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int N, const T* f,

@@ -9,6 +9,7 @@
 #include "pmg_device.h"
 
 #include <cstdio>
+#include <cfloat>
 #include <cmath>
 #include <cstdlib>
 #include <utility>
@@ -1260,9 +1261,19 @@ __global__ void __launch_bounds__(kBlock) k_gj_col(const double* M, double* col,
   if (i < N) col[i] = M[(size_t)i * N + k];
 }
 
-__global__ void __launch_bounds__(kBlock) k_gj_row(double* M, const double* col, int N, int k) {
+// A pivot that is not positive and finite means the operator is not positive
+// definite (Vdpbfa's leading-minor test, mlinpckd.c:194-233): its 1-based
+// index goes to *flag (host-mapped memory) with a system-scope store.
+__device__ inline void flag_pivot(double piv, int k, double* flag) {
+  if (flag && !(piv > 0.0 && piv <= DBL_MAX))
+    __hip_atomic_store(flag, (double)(k + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__global__ void __launch_bounds__(kBlock) k_gj_row(double* M, const double* col, int N, int k,
+                                                   double* flag) {
   const int j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= N) return;
+  if (j == k) flag_pivot(col[k], k, flag);
   const double v = (j == k) ? 1.0 : M[(size_t)k * N + j];
   M[(size_t)k * N + j] = v / col[k];
 }
@@ -1300,7 +1311,7 @@ __global__ void k_dense_solve_cols(int nx, int ny, int nz, const double* Minv, i
 // with k_gj_elim's expression; each thread's (i, j) pairs are decoded once.
 constexpr int kInvPer = 8;  // elements per thread: 89 * 89 <= 8 * 1024
 
-__global__ void __launch_bounds__(1024) k_dense_invert_lds(double* Mg, int N) {
+__global__ void __launch_bounds__(1024) k_dense_invert_lds(double* Mg, int N, double* flag) {
   extern __shared__ double sm[];
   double* M = sm;
   double* col = sm + (size_t)N * N;
@@ -1314,7 +1325,9 @@ __global__ void __launch_bounds__(1024) k_dense_invert_lds(double* Mg, int N) {
   }
   for (int e = tid; e < N * N; e += 1024) M[e] = Mg[e];
   __syncthreads();
+  int bad = -1;
   for (int k = 0; k < N; k++) {
+    if (tid == 0 && bad < 0 && !(M[k * N + k] > 0.0 && M[k * N + k] <= DBL_MAX)) bad = k;
     if (tid < N) {
       col[tid] = M[tid * N + k];
       const double v = (tid == k) ? 1.0 : M[k * N + tid];
@@ -1333,19 +1346,21 @@ __global__ void __launch_bounds__(1024) k_dense_invert_lds(double* Mg, int N) {
     __syncthreads();
   }
   for (int e = tid; e < N * N; e += 1024) Mg[e] = M[e];
+  if (tid == 0 && bad >= 0) flag_pivot(-1.0, bad, flag);
 }
 
 // The same Gauss-Jordan inverse for N <= 64 with the matrix in registers:
 // thread t holds 16 entries of row t / 4.  Each pivot step publishes the
 // pivot row and column through double-buffered LDS (one barrier per pivot)
 // and scales the pivot row by the pivot's reciprocal.
-__global__ void __launch_bounds__(256) k_dense_invert_reg(double* Mg, int N) {
+__global__ void __launch_bounds__(256) k_dense_invert_reg(double* Mg, int N, double* flag) {
   __shared__ double colb[2][64], rowb[2][64];
   const int t = threadIdx.x, r = t >> 2, cb = t & 3, c0 = cb * 16;
   double m[16];
 #pragma unroll
   for (int q = 0; q < 16; q++)
     m[q] = (r < N && c0 + q < N) ? Mg[(size_t)r * N + c0 + q] : 0.0;
+  int bad = -1;
   for (int k = 0; k < N; k++) {
     const int b = k & 1;
     if (r < N && (k >> 4) == cb) {
@@ -1360,6 +1375,7 @@ __global__ void __launch_bounds__(256) k_dense_invert_reg(double* Mg, int N) {
       for (int q = 0; q < 16; q++) rowb[b][c0 + q] = m[q];
     }
     __syncthreads();
+    if (t == 0 && bad < 0 && !(rowb[b][k] > 0.0 && rowb[b][k] <= DBL_MAX)) bad = k;
     const double inv = 1.0 / rowb[b][k];
     const double ci = r < N ? colb[b][r] : 0.0;
 #pragma unroll
@@ -1372,6 +1388,7 @@ __global__ void __launch_bounds__(256) k_dense_invert_reg(double* Mg, int N) {
 #pragma unroll
   for (int q = 0; q < 16; q++)
     if (r < N && c0 + q < N) Mg[(size_t)r * N + c0 + q] = m[q];
+  if (t == 0 && bad >= 0) flag_pivot(-1.0, bad, flag);
 }
 
 // x_r = sum_c Minv[r][c] f_c over the interior unknowns, N <= kDenseSmall:
@@ -1797,21 +1814,21 @@ void dense_assemble(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A,
   k_dense_assemble<<<(N + kBlock - 1) / kBlock, kBlock, 0, s>>>(nx, ny, nz, to27(A), D, M, N);
 }
 
-void dense_invert(hipStream_t s, double* M, double* work, int N, int form) {
+void dense_invert(hipStream_t s, double* M, double* work, int N, int form, double* flag) {
   if (N <= 64 && form != 1) {
-    k_dense_invert_reg<<<1, 256, 0, s>>>(M, N);
+    k_dense_invert_reg<<<1, 256, 0, s>>>(M, N, flag);
     return;
   }
   const size_t lds = ((size_t)N * N + 2 * N) * sizeof(double);
   if (lds <= 64 * 1024 && (size_t)N * N <= (size_t)kInvPer * 1024) {
-    k_dense_invert_lds<<<1, 1024, lds, s>>>(M, N);
+    k_dense_invert_lds<<<1, 1024, lds, s>>>(M, N, flag);
     return;
   }
   const unsigned gn = (unsigned)((N + kBlock - 1) / kBlock);
   const unsigned gnn = (unsigned)(((size_t)N * N + kBlock - 1) / kBlock);
   for (int k = 0; k < N; k++) {
     k_gj_col<<<gn, kBlock, 0, s>>>(M, work, N, k);
-    k_gj_row<<<gn, kBlock, 0, s>>>(M, work, N, k);
+    k_gj_row<<<gn, kBlock, 0, s>>>(M, work, N, k, flag);
     k_gj_elim<<<gnn, kBlock, 0, s>>>(M, work, N, k);
   }
 }

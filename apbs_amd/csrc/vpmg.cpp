@@ -720,6 +720,7 @@ struct PmgDev {
   hipEvent_t cs_fork = nullptr, cs_join = nullptr;
   bool cs_pending = false;
   int mirror = 1;
+  bool failed = false;  // the last solve stopped on a failure (solve_failed)
   double stats[8] = {};
   pmg::Geo geo{};
   // kernel timing: HIP event pairs recorded on the object's stream around
@@ -760,6 +761,31 @@ double read_scalar(PmgDev* d, const double* dptr) {
 double read_norm(PmgDev* d) {
   HIPCHECK(hipStreamSynchronize(d->s));
   return d->hscal[0];
+}
+
+// hscal slot the coarsest-level inverse flags a pivot in (0 = none): the
+// 1-based index of a pivot that was not positive and finite.
+constexpr int kPivotFlag = 8;
+
+// Whether the solve has failed, checked after every norm read: a norm that is
+// not finite (the iteration diverged or met inf/NaN; the reference's loop test
+// (rsnrm/rsden) > errtol is false for NaN and would stop as if converged), or
+// a pivot the dense inverse of the coarsest operator flagged (the operator is
+// not positive definite: the reference's Vdpbfa test, buildBd.c:106-114).
+// The inverse is joined into the object's stream before the coarsest solve,
+// so a flag it raised is visible at the next norm read.  Prints once.
+bool solve_failed(PmgDev* d, double nrm) {
+  const double piv = *(volatile double*)(d->hscal + kPivotFlag);
+  if (piv == 0.0 && std::isfinite(nrm)) return false;
+  if (!d->failed) {
+    if (piv != 0.0)
+      fprintf(stderr,
+              "Vpmg_solve: coarsest-level operator (%d unknowns) not positive definite: "
+              "pivot %d is not positive\n", d->dn, (int)piv);
+    if (!std::isfinite(nrm)) fprintf(stderr, "Vpmg_solve: residual norm is not finite\n");
+  }
+  d->failed = true;
+  return true;
 }
 
 int tbegin(PmgDev* d) {
@@ -1137,7 +1163,7 @@ void set_diagonals(PmgDev* d, bool use_cprime) {
     s = d->cs;
   }
   pmg::dense_assemble(s, C.nx, C.ny, C.nz, C.ops(), C.D, d->dense, d->dn);
-  pmg::dense_invert(s, d->dense, d->dwork, d->dn);
+  pmg::dense_invert(s, d->dense, d->dwork, d->dn, 0, d->hscal + kPivotFlag);
   if (s != d->s) {
     HIPCHECK(hipEventRecord(d->cs_join, s));
     d->cs_pending = true;
@@ -1287,7 +1313,8 @@ CycleArgs<double> args64(PmgDev* d, double* const* X, const double* f) {
   return c;
 }
 
-// Vmvcs with istop 0 (absolute) or 1 (relative to ||f||1); returns cycles
+// Vmvcs with istop 0 (absolute) or 1 (relative to ||f||1); returns cycles,
+// or -1 when the solve failed (solve_failed)
 int mvcs64(PmgDev* d, double* const* X, const double* f, int istop, int itmax, double errtol,
            double* relres) {
   const DLevel& F = d->L[0];
@@ -1295,6 +1322,7 @@ int mvcs64(PmgDev* d, double* const* X, const double* f, int istop, int itmax, d
   if (istop == 1) {
     pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, f, d->part, d->scal, d->hscal);
     rsden = read_norm(d);
+    if (solve_failed(d, rsden)) return -1;
   }
   if (rsden == 0.0) rsden = 1.0;
   CycleArgs<double> c = args64(d, X, f);
@@ -1306,6 +1334,7 @@ int mvcs64(PmgDev* d, double* const* X, const double* f, int istop, int itmax, d
     pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, c.A[0], c.Dl[0], f, X[0], nullptr, d->part,
                              d->scal, d->hscal);
     rsnrm = read_norm(d);
+    if (solve_failed(d, rsnrm)) return -1;
   } while (iters < itmax && (rsnrm / rsden) > errtol);
   if (relres) *relres = rsnrm / rsden;
   return iters;
@@ -1313,6 +1342,7 @@ int mvcs64(PmgDev* d, double* const* X, const double* f, int istop, int itmax, d
 
 // Mixed precision: FP32 V-cycles on the correction equation inside an FP64
 // defect-correction loop (fine residual, update and stopping test in FP64).
+// Returns the outer iterations, or -1 when the solve failed (solve_failed).
 int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int itmax,
                 double errtol, double* relres, int* outer, bool x_zero) {
   const DLevel& F = d->L[0];
@@ -1320,6 +1350,7 @@ int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int i
   if (istop == 1 || x_zero) {
     pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, f, d->part, d->scal, d->hscal);
     fnrm = read_norm(d);
+    if (solve_failed(d, fnrm)) return -1;
     if (istop == 1) rsden = fnrm;
   }
   if (rsden == 0.0) rsden = 1.0;
@@ -1348,6 +1379,7 @@ int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int i
     pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, A0, F.D, f, X64[0], d->w1, d->part,
                              d->scal, d->hscal);
     rsnrm = read_norm(d);
+    if (solve_failed(d, rsnrm)) return -1;
   }
   while (it < itmax && (rsnrm / rsden) > errtol) {
     pmg::convert<double, float>(d->s, F.n, r, d->L[0].w0f);
@@ -1360,6 +1392,7 @@ int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int i
     pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, A0, F.D, f, X64[0], d->w1, d->part,
                              d->scal, d->hscal);
     rsnrm = read_norm(d);
+    if (solve_failed(d, rsnrm)) return -1;
   }
   if (relres) *relres = rsnrm / rsden;
   if (outer) *outer = it;
@@ -1377,8 +1410,9 @@ pmg::Ions ions_scaled(Vpbe* pbe) {
   return I;
 }
 
-// Vnewton (newtond.c:162-548) with Vgetjac (:550-598)
-void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out) {
+// Vnewton (newtond.c:162-548) with Vgetjac (:550-598); false when the solve
+// failed (solve_failed)
+bool newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out) {
   PmgDev* d = DEV(thee);
   DLevel* L = d->L;
   const DLevel& F = L[0];
@@ -1390,9 +1424,11 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
   pmg::fill<double>(s, d->wn1, d->n, 0.0);
   pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->wn2, I, d->part, d->scal, d->hscal);
   double rsden = read_norm(d);
+  if (solve_failed(d, rsden)) return false;
   if (rsden == 0.0) rsden = 1.0;
   pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->u, d->F, I, d->part, d->scal, d->hscal);
   double xnorm_old = read_norm(d);
+  if (solve_failed(d, xnorm_old)) return false;
   double xnorm_new = xnorm_old, xnorm_med = 0.0, rsnrm = rsden;
   int idamp = 1, iters = 0, inner = 0;
   const double errtol = thee->pmgp->errtol;
@@ -1416,17 +1452,18 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
     if (d->precision == 1) {
       if (iters == 1) sync_fp32_ops(d);
       int outer = 0;
-      mixed_solve(d, X, d->F, 0, 1000, errtol_s, &rr, &outer, true);
-      its = outer;
+      its = mixed_solve(d, X, d->F, 0, 1000, errtol_s, &rr, &outer, true) < 0 ? -1 : outer;
     } else {
       its = mvcs64(d, X, d->F, 0, 1000, errtol_s, &rr);
     }
+    if (its < 0) return false;
     inner += its;
     double* xtmp = L[0].xtmp;
     if (idamp == 1) {
       pmg::xpay_interior(s, nx, ny, nz, d->u, 1.0, xtmp, d->wn1);
       pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->F, I, d->part, d->scal, d->hscal);
       xnorm_new = read_norm(d);
+      if (solve_failed(d, xnorm_new)) return false;
       double damp = 1.0;
       int iter_d = 0, mode = 0;
       while (iter_d < 10) {
@@ -1445,6 +1482,7 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
         pmg::xpay_interior(s, nx, ny, nz, d->u, damp, xtmp, d->wn1);
         pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->F, I, d->part, d->scal, d->hscal);
         xnorm_new = read_norm(d);
+        if (solve_failed(d, xnorm_new)) return false;
         iter_d++;
       }
       pmg::copy_interior<double>(s, nx, ny, nz, d->wn2, d->u);
@@ -1456,6 +1494,7 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
       pmg::axpy_interior<double>(s, nx, ny, nz, 1.0, xtmp, d->u);
       pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->u, d->F, I, d->part, d->scal, d->hscal);
       xnorm_new = read_norm(d);
+      if (solve_failed(d, xnorm_new)) return false;
       xnorm_old = xnorm_new;
     }
     rsnrm = xnorm_new;
@@ -1465,6 +1504,7 @@ void newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out
   *relres_out = rsnrm / rsden;
   *iters_out = iters;
   *inner_out = inner;
+  return true;
 }
 
 void ensure_forces(Vpmg* thee);
@@ -2145,13 +2185,18 @@ extern "C" int Vpmg_solve(Vpmg* thee) {
   else
     pmg::fill<double>(d->s, d->ccf, d->n, 0.0);
   alloc_hierarchy(d, p->nlev, newton, d->precision == 1);
+  // no inverse of an earlier solve is in flight (every solve ends with both
+  // streams drained), so the pivot flag can be cleared from the host
+  d->hscal[kPivotFlag] = 0.0;
+  d->failed = false;
   const double tb = now();
   build_operators(d);
   pmg::fill<double>(d->s, d->u, d->n, 0.0);
   double relres = 0;
   int iters = 0, inner = 0, outer = 0;
+  bool ok = true;
   if (newton) {
-    newton_solve(thee, &relres, &iters, &inner);
+    ok = newton_solve(thee, &relres, &iters, &inner);
   } else {
     set_diagonals(d, false);
     double* X[24];
@@ -2165,6 +2210,19 @@ extern "C" int Vpmg_solve(Vpmg* thee) {
     } else {
       iters = mvcs64(d, X, d->L[0].fc, 1, p->itmax, p->errtol, &relres);
     }
+    ok = iters >= 0;
+  }
+  if (!ok) {
+    // every kernel has been launched normally; drain both streams so the
+    // object stays usable (a later fillco / solve / dtor)
+    if (d->cs) HIPCHECK(hipStreamSynchronize(d->cs));
+    d->cs_pending = false;
+    HIPCHECK(hipStreamSynchronize(d->s));
+    tcollect(d);
+    d->stats[4] = now() - t0;
+    d->forces_valid = false;
+    fprintf(stderr, "Vpmg_solve: solve failed; the potential is not valid\n");
+    return 0;
   }
   pmg::set_faces(d->s, d->nx, d->ny, d->nz, d->u, d->gx, d->gy, d->gz);
   if (d->mirror)

@@ -164,29 +164,34 @@ def run_stream(server, specs, rate, seed=0, precision=1, want_potential=False):
                     precision=precision, want_potential=want_potential, tag=i)
             for i, s in enumerate(specs)]
     results = {}
+    lock = threading.Lock()
     done = threading.Event()
 
     def collector():
-        while not done.is_set() or len(results) < len(reqs):
-            for r in server.poll():
-                results[r["tag"]] = r
-            if len(results) >= len(reqs):
-                break
+        # stops once submission has ended (whatever happened to it) and the
+        # main thread has taken over the last poll after draining
+        while not done.is_set():
+            got = server.poll()
+            with lock:
+                for r in got:
+                    results[r["tag"]] = r
             time.sleep(0.002)
 
-    th = threading.Thread(target=collector)
+    th = threading.Thread(target=collector, daemon=True)
     th.start()
     t0 = time.perf_counter()
     due = t0
-    for req, g in zip(reqs, gaps):
-        due += g
-        d = due - time.perf_counter()
-        if d > 0:
-            time.sleep(d)
-        server.submit(req)
-    done.set()
-    server.drain()
-    th.join()
+    try:
+        for req, g in zip(reqs, gaps):
+            due += g
+            d = due - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            server.submit(req)
+    finally:
+        done.set()
+        th.join()
+        server.drain()
     for r in server.poll():
         results[r["tag"]] = r
     wall = max(r["t_done"] for r in results.values()) - min(r["t_submit"] for r in results.values())

@@ -41,7 +41,8 @@ def globule(n, semi_axes, seed, min_dist=2.5, net_charge=None):
         sites = _fcc_sites(semi_axes, d)
         if len(sites) >= 1.05 * n or (d <= min_dist and len(sites) >= n):
             break
-        d *= 0.98
+        # shrink by 2 %, trying d = min_dist itself before going below it
+        d = max(0.98 * d, min_dist) if d > min_dist else 0.98 * d
     if len(sites) < n:
         raise RuntimeError("ellipsoid too small")
     pick = np.sort(rng.choice(len(sites), size=n, replace=False))
@@ -101,3 +102,19 @@ def make(name, natoms=None):
     n = natoms or c["natoms"]
     xyz, q, r, _ = globule(n, c["axes"], c["seed"], net_charge=c["net"])
     return xyz, q, r
+
+
+def min_separation(xyz):
+    """The smallest distance between two atoms (A)."""
+    from scipy.spatial import cKDTree
+    d, _ = cKDTree(xyz).query(xyz, k=2)
+    return float(d[:, 1].min())
+
+
+def describe(name):
+    """What the generator achieved for a named workload: atoms, net charge and
+    the measured minimum pair distance (the requested 2.5 A cannot be met at
+    every workload's density; the generator then packs as loosely as fits)."""
+    xyz, q, r = make(name)
+    return {"atoms": len(q), "net_charge": round(float(q.sum()), 6),
+            "min_separation_A": round(min_separation(xyz), 4), "requested_separation_A": 2.5}

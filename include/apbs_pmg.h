@@ -334,8 +334,13 @@ int Vpmg_ctor2(Vpmg *thee, Vpmgp *parms, Vpbe *pbe, int focusFlag, Vpmg *pmgOLD,
                MGparm *mgparm, PBEparm_calcEnergy energyFlag);
 void Vpmg_dtor(Vpmg **thee);                    /* src/mg/vpmg.h:260 */
 void Vpmg_dtor2(Vpmg *thee);                    /* src/mg/vpmg.h:269 */
-/* Vpmg_fillco (src/mg/vpmg.h:281-298, vpmg.c:5250-5415).  External maps
- * (use*Map != 0) are not supported and return 0. */
+/* Vpmg_fillco (src/mg/vpmg.h:281-298, vpmg.c:5250-5415).  External maps are
+ * supported as in the reference: dielectric x/y/z (sampled at the shifted
+ * face points), kappa (scaled by its maximum), charge (times zmagic) and the
+ * potential map of bcfl map.  Returns 0 (with the reference's message) when a
+ * grid point falls off a map in use, when a kappa map holds a negative value,
+ * when coefficient maps are used without all three dielectric maps or (with
+ * salt) without a kappa map, or when a map in use is NULL or has no data. */
 int Vpmg_fillco(Vpmg *thee, Vsurf_Meth surfMeth, double splineWin, Vchrg_Meth chargeMeth,
                 int useDielXMap, Vgrid *dielXMap, int useDielYMap, Vgrid *dielYMap,
                 int useDielZMap, Vgrid *dielZMap, int useKappaMap, Vgrid *kappaMap,

@@ -88,3 +88,4 @@ def test_focus353_mixed_against_fp64():
     print(f"focus353: fp64 {e64:.6f} kJ/mol ({t64:.1f} s, {st64}), mixed {emx:.6f} "
           f"({tmx:.1f} s, {stmx}); energy rel {rel:.2e}, potential L2 {l2:.2e} Linf {linf:.2e}")
     assert rel < 1e-6
+    assert l2 < 1e-8 and linf < 1e-8

@@ -140,6 +140,15 @@ def test_write_pot_dx_matches_potential(tmp_path):
     text = text.replace("write pot gz potential", "write pot dx pot\n    write charge dx chg")
     inp = nosh.parse(text, base=os.path.join(EX, "born"))
     r = run.Runner(inp, out=io.StringIO(), outdir=str(tmp_path))
+    # keep the potential of every calculation that writes, as the solver holds it
+    solved = []
+    write = r._write
+
+    def keep(calc, e):
+        solved.append(calc.array("u").copy())
+        return write(calc, e)
+
+    r._write = keep
     r.run()
     u, origin, h = dx.read_dx(str(tmp_path / "pot.dx"))
     q, _, _ = dx.read_dx(str(tmp_path / "chg.dx"))
@@ -148,6 +157,12 @@ def test_write_pot_dx_matches_potential(tmp_path):
     assert np.allclose(h, (12.0 / 64,) * 3)
     mol = driver.Molecule.from_pqr(os.path.join(EX, "born", "ion.pqr"))
     assert np.allclose(origin, mol.center - 6.0)
-    assert np.isfinite(u).all() and u.max() > 0
+    # the file holds the first ELEC block's finest potential to the format's
+    # 7 significant digits (%12.6e), point for point
+    assert len(solved) == 2
+    want = solved[0]
+    assert u.shape == want.shape
+    assert np.all(np.abs(u - want) <= 5.01e-7 * np.abs(want) + 1e-300), \
+        np.abs(u - want).max()
     # total charge on the grid is the ion's charge (+1 e)
     assert abs(q.sum() * (12.0 / 64) ** 3 - 1.0) < 1e-5   # values carry 7 digits

@@ -1,0 +1,56 @@
+"""Solver failure is reported, not hidden (the reference's Vdpbfa leading-minor
+test, buildBd.c:106-114, and its loop test, which a NaN norm would pass).
+
+A negative dielectric map makes every level's operator negative definite.
+Gauss-Seidel still runs normally on it (it is the same iteration for A and
+-A), so every kernel launches and exits as usual; the dense inverse of the
+coarsest operator then meets a negative pivot, flags it, and Vpmg_solve
+returns 0 with a message.  The same object and a fresh one stay usable.
+"""
+import numpy as np
+import pytest
+
+from apbs_amd import driver, synth
+
+pytestmark = pytest.mark.gpu
+
+DIME, H = 33, 0.5
+
+
+def calc(diel, precision, npbe):
+    atoms = synth.globule(30, (4.0,) * 3, 5, net_charge=1)[:3]
+    mol = driver.Molecule(*atoms)
+    c = mol.center
+    origin = c - 10.0
+    shape = (43, 43, 43)
+    maps = {k: driver.Map(np.full(shape, diel), origin, (0.5,) * 3)
+            for k in ("dielx", "diely", "dielz")}
+    ions = [(1, 0.0, 2.0), (-1, 0.0, 2.0)]
+    if npbe:
+        maps["kappa"] = driver.Map(np.ones(shape), origin, (0.5,) * 3)
+        ions = [(1, 0.15, 2.0), (-1, 0.15, 2.0)]
+    pbe = dict(pdie=2.0, sdie=78.54, srfm="spl2", chgm="spl2", swin=0.3, srad=1.4, sdens=10.0,
+               temp=298.15, bcfl="mdh", ions=ions, npbe=npbe, maps=maps)
+    g = driver.Calc(mol, driver.manual_mgparm([DIME] * 3, c, grid=[H] * 3), pbe,
+                    precision=precision)
+    g._keep = (mol, maps)
+    return g
+
+
+@pytest.mark.parametrize("precision,npbe", [(0, False), (1, False), (0, True), (1, True)])
+def test_negative_dielectric_is_reported(precision, npbe, capfd):
+    bad = calc(-5.0, precision, npbe)
+    bad.fillco()
+    with pytest.raises(RuntimeError):
+        bad.solve()
+    err = capfd.readouterr().err
+    assert "not positive definite" in err, err
+    # the device and the library stay usable: the same object solves again
+    # after a valid fill is impossible (its maps are fixed), so a fresh one
+    good = calc(20.0, precision, npbe)
+    good.fillco()
+    good.solve()
+    st = good.stats()
+    assert np.isfinite(good.energy(1)) and st["relres"] <= 1e-6, st
+    # the failed object can still be released and its pool memory reused
+    bad.release()

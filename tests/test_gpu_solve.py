@@ -211,6 +211,33 @@ def test_mixed_precision_energy(npbe):
     assert linf < 1e-6
 
 
+@pytest.mark.parametrize("npbe", [False, True])
+def test_mixed_precision_against_oracle(npbe):
+    """Mixed precision held to the oracle directly: the 65^3 workload (lpbe and
+    its npbe form) against the oracle's reference-schedule solve at etol
+    1e-12.  Converged (FP64 defect correction to 1e-11, or Newton etol 1e-12
+    with FP32 inner cycles): energy and potential L2 / Linf within 1e-9.  At
+    the default etol (1e-6): energy within 1e-6, potential L2 / Linf within
+    1e-4."""
+    atoms = synth.make("lpbe65")
+    pbe = dict(synth.CONFIGS["lpbe65"]["pbe"], npbe=npbe)
+    c = od.mol_center(atoms[0])
+    eo, o = od.run_manual(atoms, [65] * 3, c, dict(pbe, etol=1e-12), grid=[0.5] * 3)
+    uo = o.array("u")
+    mol = driver.Molecule(*atoms)
+    tight = dict(pbe, etol=1e-12) if npbe else pbe
+    et, gt = driver.run_manual(mol, [65] * 3, c, tight, grid=[0.5] * 3, precision=1,
+                               mixed_tol=1e-11)
+    l2t, linft = pot_err(gt.array("u"), uo)
+    ed, gd = driver.run_manual(mol, [65] * 3, c, pbe, grid=[0.5] * 3, precision=1)
+    l2d, linfd = pot_err(gd.array("u"), uo)
+    print(f"npbe={npbe}: oracle {eo:.12f}; mixed converged {et:.12f} rel {rel(et, eo):.2e} "
+          f"L2 {l2t:.2e} Linf {linft:.2e} ({gt.stats()}); mixed etol 1e-6 {ed:.12f} "
+          f"rel {rel(ed, eo):.2e} L2 {l2d:.2e} Linf {linfd:.2e} ({gd.stats()})")
+    assert rel(et, eo) < 1e-9 and l2t < 1e-9 and linft < 1e-9
+    assert rel(ed, eo) < 1e-6 and l2d < 1e-4 and linfd < 1e-4
+
+
 def test_c_caller_matches_python(tmp_path):
     """tests/c_caller.c runs the routines.c sequence through the C header
     alone; its energy and force equal the Python driver's bitwise."""
