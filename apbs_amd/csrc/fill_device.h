@@ -191,6 +191,22 @@ void energy_qm(hipStream_t s, size_t n, int nonlin, const double* pvec, const do
 void energy_qf_point(hipStream_t s, const Geo& g, const AtomsDev& A, const double* u,
                      double* part, double* out);
 
+// The library's one statically allocated device buffer, for entry points that
+// must not allocate (Vpmg_energyNoAlloc).  Layout in doubles: [0,
+// kScratchParts) the energy reductions' partial sums (one per block),
+// [kScratchParts, kScratchParts + 8) finished sums, the int at double
+// kScratchParts + 8 the chop counter, from kScratchAtomOff one partial per
+// atom (at most kScratchAtoms atoms).  Every caller on a device shares that
+// device's copy, so calls that overlap on one device race.
+constexpr size_t kScratchParts = 1024;
+constexpr size_t kScratchAtomOff = kScratchParts + 64;
+constexpr size_t kScratchAtoms = (size_t)1 << 17;
+constexpr size_t kScratchDoubles = kScratchAtomOff + kScratchAtoms;
+// The device address of the current device's copy (looked up with
+// hipGetSymbolAddress on the first call per device, then cached); nullptr
+// when the lookup fails.
+double* scratch_address();
+
 // ------------------------------------------------------------------ forces
 // Per-atom force components (kT/A), out[3*a..3*a+2]; rc[a] = 1/0 like the
 // reference's return codes.

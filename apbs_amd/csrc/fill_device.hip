@@ -8,6 +8,8 @@
 #include "fill_device.h"
 
 #include <cmath>
+#include <cstdio>
+#include <mutex>
 #include <vector>
 
 namespace pmg {
@@ -15,6 +17,7 @@ namespace {
 
 constexpr int kB = 256;
 constexpr int kParts = 1024;
+static_assert(kScratchParts == kParts, "the static buffer holds one partial per reduction block");
 constexpr double VSMALL = 1.0e-9;
 constexpr double VPMGSMALL = 1e-12;
 
@@ -993,6 +996,8 @@ __global__ void __launch_bounds__(kB)
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+__global__ void k_zero_int(int* p) { *p = 0; }
+
 // one thread per atom, partial per atom, then fixed-order sum
 __global__ void __launch_bounds__(kB)
     k_e_qf_point(Geo g, AtomsDev A, const double* u, double* per_atom) {
@@ -1369,6 +1374,11 @@ __global__ void k_kappa_scale(size_t n, double kappamax, double* kappa) {
 
 // ================================================================ wrappers
 
+// The no-allocation entry points' buffer (layout in fill_device.h).  In host
+// code the name `scratch` is a host-side stand-in, not a device address: only
+// the address scratch_address() looks up is passed to kernels and copies.
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 static dim3 grid_all(const Geo& g) {
   return dim3((unsigned)(((long)g.nx * g.ny + kB - 1) / kB), (unsigned)g.nz);
 }
@@ -1497,7 +1507,7 @@ void energy_diel(hipStream_t s, const Geo& g, const double* pvec, const double* 
 void energy_qm(hipStream_t s, size_t n, int nonlin, const double* pvec, const double* kappa,
                const double* u, double zks2, double zkappa2, const IonBulk& ions, double* part,
                double* out, int* nchop) {
-  hipMemsetAsync(nchop, 0, sizeof(int), s);
+  k_zero_int<<<1, 1, 0, s>>>(nchop);
   k_e_qm<<<kParts, kB, 0, s>>>(n, nonlin, pvec, kappa, u, zks2, zkappa2, ions, part, nchop);
   k_fin<<<1, kB, 0, s>>>(part, out);
 }
@@ -1591,6 +1601,26 @@ void map_sample(hipStream_t s, const Geo& g, const MapDev& m, const double* xf, 
                 double* out, double* part, int* bad) {
   k_map_sample<<<grid_all(g), kB, 0, s>>>(g, m, xf, yf, zf, ox, oy, oz, mode, scale, out, part,
                                           bad);
+}
+
+double* scratch_address() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static double* addr[kMaxDevices] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "scratch_address: hipGetSymbolAddress failed on device %d: %s\n", dev,
+              hipGetErrorString(e));
+      return nullptr;
+    }
+    addr[dev] = (double*)p;
+  }
+  return addr[dev];
 }
 
 void kappa_scale(hipStream_t s, size_t n, double kappamax, double* kappa) {

@@ -2452,37 +2452,50 @@ static void ext_energy(Vpmg* thee, Vpmg* old, PBEparm_calcEnergy, double* partMi
 
 // ================================================================ energies
 
-extern "C" double Vpmg_qfEnergy(Vpmg* thee, int extFlag) {
+// Device buffers an energy reduction works in: partial sums (kParts), the
+// finished sum, the per-atom partials of the point-charge energy and the chop
+// counter.  The public energies use the object's buffers (and a pool block
+// for the counter); Vpmg_energyNoAlloc uses the library's static buffer.
+struct EnergyBufs {
+  double* part;
+  double* out;
+  double* apart;
+  int* nchop;  // nullptr: a pool block per call
+};
+
+EnergyBufs object_bufs(PmgDev* d) { return EnergyBufs{d->part, d->scal, d->atmp, nullptr}; }
+
+static double qf_energy(Vpmg* thee, int extFlag, const EnergyBufs& b) {
   PmgDev* d = DEV(thee);
   Vpmgp* p = thee->pmgp;
   double e;
   // Vpmg_qfEnergy (vpmg.c:1687-1702): the volume sum for a charge map or spl2
   if (thee->useChargeMap || thee->chargeMeth == VCM_BSPL2) {  // vpmg.c:1861-1891
-    pmg::energy_qf_volume(d->s, d->n, d->pvec, d->u, d->charge, d->part, d->scal);
-    e = read_scalar(d, d->scal);
+    pmg::energy_qf_volume(d->s, d->n, d->pvec, d->u, d->charge, b.part, b.out);
+    e = read_scalar(d, b.out);
     e = e * p->hx * p->hy * p->hzed / thee->pbe->zmagic;
     if (extFlag == 1) e += thee->extQfEnergy;
   } else {  // Vpmg_qfEnergyPoint (vpmg.c:1704-1789)
-    pmg::energy_qf_point(d->s, d->geo, atoms_dev(d), d->u, d->atmp, d->scal);
-    e = read_scalar(d, d->scal);
+    pmg::energy_qf_point(d->s, d->geo, atoms_dev(d), d->u, b.apart, b.out);
+    e = read_scalar(d, b.out);
     if (extFlag) e += thee->extQfEnergy;
   }
   return e;
 }
 
-extern "C" double Vpmg_dielEnergy(Vpmg* thee, int extFlag) {
+static double diel_energy(Vpmg* thee, int extFlag, const EnergyBufs& b) {
   PmgDev* d = DEV(thee);
   Vpmgp* p = thee->pmgp;
   if (!thee->filled) return 0.0;
-  pmg::energy_diel(d->s, d->geo, d->pvec, d->u, d->epsx, d->epsy, d->epsz, d->part, d->scal);
-  double e = read_scalar(d, d->scal);
+  pmg::energy_diel(d->s, d->geo, d->pvec, d->u, d->epsx, d->epsy, d->epsz, b.part, b.out);
+  double e = read_scalar(d, b.out);
   e = 0.5 * e * p->hx * p->hy * p->hzed;
   e = e / thee->pbe->zmagic;
   if (extFlag == 1) e += thee->extDiEnergy;
   return e;
 }
 
-extern "C" double Vpmg_qmEnergy(Vpmg* thee, int extFlag) {
+static double qm_energy(Vpmg* thee, int extFlag, const EnergyBufs& b) {
   PmgDev* d = DEV(thee);
   Vpmgp* p = thee->pmgp;
   Vpbe* pbe = thee->pbe;
@@ -2494,14 +2507,14 @@ extern "C" double Vpmg_qmEnergy(Vpmg* thee, int extFlag) {
     I.q[i] = pbe->ionQ[i];
     I.c[i] = pbe->ionConc[i];
   }
-  int* nchop = dalloc<int>(1);
+  int* nchop = b.nchop ? b.nchop : dalloc<int>(1);
   pmg::energy_qm(d->s, d->n, p->nonlin ? 1 : 0, d->pvec, d->kappa, d->u, zks2, pbe->zkappa2, I,
-                 d->part, d->scal, nchop);
-  double e = read_scalar(d, d->scal);
+                 b.part, b.out, nchop);
+  double e = read_scalar(d, b.out);
   int hchop = 0;
   HIPCHECK(hipMemcpyAsync(&hchop, nchop, sizeof(int), hipMemcpyDeviceToHost, d->s));
   HIPCHECK(hipStreamSynchronize(d->s));
-  dfree(nchop);
+  if (!b.nchop) dfree(nchop);
   if (p->nonlin && hchop > 0) {
     // the reference aborts here (vpmg.c:1468-1472)
     fprintf(stderr, "Vpmg_qmEnergy: chopped EXP %d times; large potential values\n", hchop);
@@ -2514,14 +2527,59 @@ extern "C" double Vpmg_qmEnergy(Vpmg* thee, int extFlag) {
   return e;
 }
 
-extern "C" double Vpmg_energy(Vpmg* thee, int extFlag) {
+// Vpmg_energy (vpmg.c:1248-1277)
+static double total_energy(Vpmg* thee, int extFlag, const EnergyBufs& b) {
   if ((thee->pmgp->nonlin) && (thee->pbe->bulkIonicStrength > 0.)) {
-    double qm = Vpmg_qmEnergy(thee, extFlag);
-    double qf = Vpmg_qfEnergy(thee, extFlag);
-    double di = Vpmg_dielEnergy(thee, extFlag);
+    double qm = qm_energy(thee, extFlag, b);
+    double qf = qf_energy(thee, extFlag, b);
+    double di = diel_energy(thee, extFlag, b);
     return qf - di - qm;
   }
-  return 0.5 * Vpmg_qfEnergy(thee, extFlag);
+  return 0.5 * qf_energy(thee, extFlag, b);
+}
+
+extern "C" double Vpmg_qfEnergy(Vpmg* thee, int extFlag) {
+  return qf_energy(thee, extFlag, object_bufs(DEV(thee)));
+}
+
+extern "C" double Vpmg_dielEnergy(Vpmg* thee, int extFlag) {
+  return diel_energy(thee, extFlag, object_bufs(DEV(thee)));
+}
+
+extern "C" double Vpmg_qmEnergy(Vpmg* thee, int extFlag) {
+  return qm_energy(thee, extFlag, object_bufs(DEV(thee)));
+}
+
+extern "C" double Vpmg_energy(Vpmg* thee, int extFlag) {
+  return total_energy(thee, extFlag, object_bufs(DEV(thee)));
+}
+
+// Vpmg_energy's value computed without any allocation: the reductions work in
+// the library's static device buffer (pmg::scratch_address) and the result is
+// read through the object's pinned scalars.  The point-charge energy (chgm
+// spl0 without a charge map) keeps one partial per atom there, so it is
+// refused above pmg::kScratchAtoms atoms.  Calls on one device share the
+// buffer: calls that overlap on one device race.
+extern "C" int Vpmg_energyNoAlloc(Vpmg* thee, int extFlag, double* energy) {
+  if (!thee || !thee->dev || !energy) {
+    fprintf(stderr, "Vpmg_energyNoAlloc: null argument\n");
+    return 0;
+  }
+  PmgDev* d = DEV(thee);
+  const bool point = !(thee->useChargeMap || thee->chargeMeth == VCM_BSPL2);
+  if (point && (size_t)d->natoms > pmg::kScratchAtoms) {
+    fprintf(stderr,
+            "Vpmg_energyNoAlloc: the point-charge energy of %d atoms needs one partial per "
+            "atom; the static buffer holds %zu (use Vpmg_energy)\n",
+            d->natoms, pmg::kScratchAtoms);
+    return 0;
+  }
+  double* sc = pmg::scratch_address();
+  if (!sc) return 0;
+  const EnergyBufs b{sc, sc + pmg::kScratchParts, sc + pmg::kScratchAtomOff,
+                     reinterpret_cast<int*>(sc + pmg::kScratchParts + 8)};
+  *energy = total_energy(thee, extFlag, b);
+  return std::isfinite(*energy) ? 1 : 0;
 }
 
 // Vpmg_qfAtomEnergy (vpmg.c:1791-1859): trilinear u at the atom

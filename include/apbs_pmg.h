@@ -394,6 +394,17 @@ int Vpmg_copyArray(Vpmg *thee, int which, double *out, size_t n);
  * point by one smoother launch. */
 int Vpmg_setKernelTiming(Vpmg *thee, int on);
 int Vpmg_getKernelTiming(Vpmg *thee, double *out, int n);
+/* Vpmg_energy(thee, extFlag) computed without allocating any memory, for
+ * callers that cannot allocate (tight serving loops, code that must not call
+ * hipMalloc).  No reference counterpart.  Writes the energy (kT) to *energy and
+ * returns 1; returns 0 with a message on a null argument, when the energy is
+ * not finite (the exponential of the nonlinear term was chopped, as
+ * Vpmg_energy reports with NAN), or for the point-charge energy (chgm spl0
+ * without a charge map) of more than 131072 atoms, the per-atom partials the
+ * library's static device buffer holds.  Its reductions use that buffer
+ * (about 1 MB per device, held until the process ends) instead of the
+ * object's buffers: calls that overlap on one device race. */
+int Vpmg_energyNoAlloc(Vpmg *thee, int extFlag, double *energy);
 /* hipDeviceSynchronize(); 1 on success. */
 int apbs_pmg_synchronize(void);
 /* Library version string. */
