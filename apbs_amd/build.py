@@ -32,18 +32,23 @@ def _run(cmd):
 
 
 def build(verbose=False):
-    objs = []
+    """Compiles the translation units in parallel (one hipcc each), then links."""
+    from concurrent.futures import ThreadPoolExecutor
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
+    cmds, objs = [], []
     for src in SOURCES:
         obj = os.path.join(bdir, os.path.splitext(src)[0] + ".o")
         cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
             cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
-        out = _run(cmd)
-        if verbose and out:
-            print(out)
+        cmds.append(cmd)
         objs.append(obj)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "6"))))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for out in ex.map(_run, cmds):
+            if verbose and out:
+                print(out)
     tmp = OUT + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, OUT)

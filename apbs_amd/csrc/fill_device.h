@@ -13,14 +13,24 @@ struct Geo {
   double xmin, ymin, zmin, xmax, ymax, zmax;
 };
 
-// Uniform cell hash over points (atoms or SAS points).  Cells hold point
-// indices in ascending order, so a k-way merge over neighbouring cells visits
-// the points in index order.
+// Uniform cell hash over points (atoms or SAS points), indices ascending per
+// cell.
 struct Cells {
   double lo[3];
   double cs;       // cell edge (A)
   int dim[3];
   const int* start;  // dim0*dim1*dim2 + 1
+  const int* list;
+};
+
+// Per-tile atom lists for the order-sensitive gathers: the grid is cut into
+// kTile^3-point tiles (tdim per axis, x fastest); tile t lists, in ascending
+// atom index, every atom that can reach a point of the tile (a superset:
+// the kernels apply the reference's own tests).
+constexpr int kTile = 8;
+struct Tiles {
+  int tdim[3];
+  const int* start;  // tdim0*tdim1*tdim2 + 1
   const int* list;
 };
 
@@ -37,7 +47,7 @@ struct AtomsDev {
 // scaled by zmagic/(hx hy hz) per atom order of the reference.
 // mode: 0 trilinear (fillcoChargeSpline1), 1 cubic B-spline (fillcoChargeSpline2),
 // 2 5th-order B-spline (chgm spl4, fillcoPermanentMultipole with charges only).
-void charge_map(hipStream_t s, const Geo& g, const AtomsDev& A, const Cells& C, int mode,
+void charge_map(hipStream_t s, const Geo& g, const AtomsDev& A, const Tiles& T, int mode,
                 double zmagic, int bcfl_lenient, double* charge);
 
 // ------------------------------------------------------------- coefficients
@@ -45,7 +55,7 @@ void charge_map(hipStream_t s, const Geo& g, const AtomsDev& A, const Cells& C, 
 // linear map to epsp..epsw and the ion mask.
 // order 2: srfm spl2 (cubic window); 3 / 4: srfm spl3 / spl4 (polynomial
 // windows), coef = kSplCoefHost doubles per atom from spline_coefficients.
-void coef_spline(hipStream_t s, const Geo& g, const AtomsDev& A, const Cells& C, double win,
+void coef_spline(hipStream_t s, const Geo& g, const AtomsDev& A, const Tiles& T, double win,
                  double irad, double ionmask, double epsw, double epsp, double* kappa,
                  double* epsx, double* epsy, double* epsz, int order = 2,
                  const double* coef = nullptr);
@@ -73,15 +83,34 @@ int sas_compact(hipStream_t s, const AtomsDev& A, const double* ref, int nref, d
                 const unsigned char* flags, int* scratch, double** xyz_out,
                 double* (*alloc)(size_t));
 
-// markSphere (vpmg.c:6444-6491) for a set of spheres: mode 0 atoms (radius
-// r_a + add, strict on-mesh test, dielectric), mode 1 atoms (r_a + add,
-// inflated test, ion accessibility), mode 2 points of radius add (SAS).
-// which: 0/1/2 = x/y/z shifted grid, 3 = unshifted.
-void mark_spheres(hipStream_t s, const Geo& g, int npts, const double* pos, const double* rad,
-                  double add, int mode, int which, double val, double* arr);
-// 9-point harmonic smoothing (vpmg.c:4500-4628); a1..a3 are the unsmoothed maps.
-void mol_smooth(hipStream_t s, const Geo& g, const double* a1, const double* a2, const double* a3,
-                double* epsx, double* epsy, double* epsz);
+// markSphere (vpmg.c:6444-6491) for a set of spheres, as flags: every grid
+// point of a sphere gets byte 1 in the flag map of each target.  Target t
+// (MarkTargets) names the grid (which: 0/1/2 = x/y/z shifted, 3 =
+// unshifted), the sphere test (mode 0: atoms, radius r_a + add, strict
+// on-mesh test (dielectric); mode 1: atoms, r_a + add, inflated test (ion
+// accessibility); mode 2: points of radius add (SAS), no mesh test) and its
+// flag map flags + t * (nx ny nz).  The flag maps must start at zero.
+struct MarkTarget {
+  int which, mode;
+  double add;
+};
+struct MarkTargets {
+  int nt;
+  MarkTarget t[4];
+};
+void mark_flags(hipStream_t s, const Geo& g, int npts, const double* pos, const double* rad,
+                const MarkTargets& T, unsigned char* flags);
+// The molecular-surface maps from the flags (fillcoCoefMolDiel vpmg.c:4335-4344
+// with NoSmooth :4346-4498 or Smooth :4500-4628, fillcoCoefMolIon :4237-4333):
+// the unsmoothed dielectric value at a shifted point is epsp inside an atom
+// sphere (atom flags, 3 maps) and not within a probe radius of an SAS point
+// (sas flags, 3 maps; null when the probe radius is 0), else epsw; smooth
+// selects the 9-point harmonic mean of those values.  kappa is 0 where the
+// kappa flag is set (null: nowhere), else ionmask.
+void mol_maps(hipStream_t s, const Geo& g, const unsigned char* atom_flags,
+              const unsigned char* sas_flags, const unsigned char* kappa_flags, double epsp,
+              double epsw, double ionmask, int smooth, double* epsx, double* epsy, double* epsz,
+              double* kappa);
 
 // ------------------------------------------------------------- boundaries
 // bcflnew (vpmg.c:3289-3367): Debye-Hueckel sum over all atoms on the faces.

@@ -30,48 +30,6 @@ inline unsigned nb(size_t n) {
   return (unsigned)(b == 0 ? 1 : b);
 }
 
-// ------------------------------------------------------- ordered cell merge
-struct Merge {
-  int cur[27], end[27];
-  int nc;
-};
-
-__device__ inline void merge_init(Merge& m, const Cells& C, double x, double y, double z) {
-  const int cx = (int)floor((x - C.lo[0]) / C.cs);
-  const int cy = (int)floor((y - C.lo[1]) / C.cs);
-  const int cz = (int)floor((z - C.lo[2]) / C.cs);
-  m.nc = 0;
-  for (int dz = -1; dz <= 1; dz++)
-    for (int dy = -1; dy <= 1; dy++)
-      for (int dx = -1; dx <= 1; dx++) {
-        const int a = cx + dx, b = cy + dy, c = cz + dz;
-        if (a < 0 || b < 0 || c < 0 || a >= C.dim[0] || b >= C.dim[1] || c >= C.dim[2]) continue;
-        const int id = a + C.dim[0] * (b + C.dim[1] * c);
-        const int s = C.start[id], e = C.start[id + 1];
-        if (s < e) {
-          m.cur[m.nc] = s;
-          m.end[m.nc] = e;
-          m.nc++;
-        }
-      }
-}
-
-// next point index in ascending order, -1 when exhausted
-__device__ inline int merge_next(Merge& m, const int* list) {
-  int best = -1, bi = -1;
-  for (int c = 0; c < m.nc; c++) {
-    if (m.cur[c] < m.end[c]) {
-      const int v = list[m.cur[c]];
-      if (best < 0 || v < best) {
-        best = v;
-        bi = c;
-      }
-    }
-  }
-  if (bi >= 0) m.cur[bi]++;
-  return best;
-}
-
 // ----------------------------------------------------------------- splines
 __device__ inline double bspline2(double x) {  // vpmg.c:5091-5104
   double m2m, m2, m3;
@@ -176,17 +134,31 @@ __device__ inline int imin2(int a, int b) { return a <= b ? a : b; }
 __device__ inline int imax2(int a, int b) { return a >= b ? a : b; }
 
 // ----------------------------------------------------------------- charges
+// Point r (0 or 1) of thread tid in tile `tile`: a workgroup of kB = 256
+// threads covers the tile's 8 x 8 x 8 points, two planes of 4 apart per thread.
+static_assert(kTile == 8 && kB == 256, "tile gathers: 2 points per thread");
+__device__ inline void tile_point(const Tiles& T, int tile, int tid, int r, int& i, int& j,
+                                  int& k) {
+  const int tx = tile % T.tdim[0], ty = (tile / T.tdim[0]) % T.tdim[1],
+            tz = tile / (T.tdim[0] * T.tdim[1]);
+  i = tx * kTile + tid % kTile;
+  j = ty * kTile + (tid / kTile) % kTile;
+  k = tz * kTile + tid / (kTile * kTile) + 4 * r;
+}
+
+// One workgroup per tile: every point sums the contributions of the tile's
+// atoms, in ascending atom index (the order of the reference's per-atom loop).
 __global__ void __launch_bounds__(kB)
-    k_charge(Geo g, AtomsDev A, Cells C, int mode, double zmagic, double* charge) {
-  const int t = blockIdx.x * kB + threadIdx.x;
-  const int k = blockIdx.y;
-  const int i = t % g.nx, j = t / g.nx;
-  if (j >= g.ny) return;
-  const double x = g.xmin + i * g.hx, y = g.ymin + j * g.hy, z = g.zmin + k * g.hz;
-  Merge m;
-  merge_init(m, C, x, y, z);
+    k_charge(Geo g, AtomsDev A, Tiles T, int mode, double zmagic, double* charge) {
+  const int tile = blockIdx.x;
+  const int q0 = T.start[tile], q1 = T.start[tile + 1];
+  for (int r = 0; r < 2; r++) {
+  int i, j, k;
+  tile_point(T, tile, threadIdx.x, r, i, j, k);
+  if (i >= g.nx || j >= g.ny || k >= g.nz) continue;
   double sum = 0.0;
-  for (int a = merge_next(m, C.list); a >= 0; a = merge_next(m, C.list)) {
+  for (int q = q0; q < q1; q++) {
+    const int a = T.list[q];
     const double* ap = A.pos + 3 * a;
     const double fi = (ap[0] - g.xmin) / g.hx, fj = (ap[1] - g.ymin) / g.hy,
                  fk = (ap[2] - g.zmin) / g.hz;
@@ -245,6 +217,7 @@ __global__ void __launch_bounds__(kB)
     }
   }
   charge[IX(i, j, k, g.nx, g.ny)] = sum;
+  }
 }
 
 // ------------------------------------------------------------ spline coefs
@@ -277,19 +250,20 @@ __device__ inline double poly_chi(const double* c, double sm, double sm2) {
 
 template <int kOrd>
 __global__ void __launch_bounds__(kB)
-    k_coef_spline(Geo g, AtomsDev A, Cells C, double win, double irad, double ionmask, double epsw,
+    k_coef_spline(Geo g, AtomsDev A, Tiles T, double win, double irad, double ionmask, double epsw,
                   double epsp, const double* pc, double* kappa, double* epsx, double* epsy,
                   double* epsz) {
-  const int t = blockIdx.x * kB + threadIdx.x;
-  const int k = blockIdx.y;
-  const int i = t % g.nx, j = t / g.nx;
-  if (j >= g.ny) return;
+  const int tile = blockIdx.x;
+  const int q0 = T.start[tile], q1 = T.start[tile + 1];
   const double w2i = 1.0 / (win * win), w3i = 1.0 / (win * win * win);
   const double hx = g.hx, hy = g.hy, hzed = g.hz;
+  for (int r = 0; r < 2; r++) {
+  int i, j, k;
+  tile_point(T, tile, threadIdx.x, r, i, j, k);
+  if (i >= g.nx || j >= g.ny || k >= g.nz) continue;
   double ka = 1.0, ex = 1.0, ey = 1.0, ez = 1.0;
-  Merge m;
-  merge_init(m, C, g.xmin + i * hx, g.ymin + j * hy, g.zmin + k * hzed);
-  for (int a = merge_next(m, C.list); a >= 0; a = merge_next(m, C.list)) {
+  for (int q = q0; q < q1; q++) {
+    const int a = T.list[q];
     const double* ap = A.pos + 3 * a;
     const double arad = A.rad[a];
     if ((ap[0] <= g.xmin) || (ap[0] >= g.xmax) || (ap[1] <= g.ymin) || (ap[1] >= g.ymax) ||
@@ -374,6 +348,7 @@ __global__ void __launch_bounds__(kB)
   epsx[p] = (epsw - epsp) * ex + epsp;
   epsy[p] = (epsw - epsp) * ey + epsp;
   epsz[p] = (epsw - epsp) * ez + epsp;
+  }
 }
 
 // ------------------------------------------------------------ SAS / markSphere
@@ -542,132 +517,189 @@ __global__ void __launch_bounds__(kB)
   }
 }
 
-// markSphere (vpmg.c:6444-6491) as a scatter: every grid point of a sphere
-// gets val.  Every writer of a point writes the same constant, so the result
-// does not depend on the order of the writes.
-// mode 0: atom spheres r_a + add with the strict on-mesh test (dielectric),
-// mode 1: atom spheres r_a + add with the inflated on-mesh test (ions),
-// mode 2: SAS points with radius add (no mesh test).
-// One 64-lane wavefront per sphere; the lanes stride over the sphere's
-// bounding box.  The point set is the reference loop's: its row skip
-// ((dx2 + dy2) > rtot2) only drops points that also fail the final test,
-// because rounding is monotone.
+// markSphere (vpmg.c:6444-6491) as a scatter of flags: every grid point of a
+// sphere gets byte 1 in its target's flag map.  Every writer of a byte writes
+// the same value, so the result does not depend on the order of the writes.
+// One 64-lane wavefront per sphere; for each target the lanes cover the
+// sphere's bounding box and test every point of it.  The point set is the
+// reference loop's: its row skip ((dx2 + dy2) > rtot2) only drops points
+// that also fail the final test, because rounding is monotone.
 constexpr int kMarkWaves = kB / 64;
 
 __global__ void __launch_bounds__(kB)
-    k_mark(Geo g, int npts, const double* pos, const double* rad, double add, int mode, double xs,
-           double ys, double zs, double val, double* arr) {
+    k_mark_flags(Geo g, int npts, const double* pos, const double* rad, MarkTargets T,
+                 unsigned char* flags) {
   const int a = blockIdx.x * kMarkWaves + threadIdx.x / 64;
   const int lane = threadIdx.x % 64;
   if (a >= npts) return;
   const double* tp = pos + 3 * a;
-  double rtot = add;
-  if (mode != 2) {
-    const double ar = rad[a];
-    if (mode == 0) {
-      if ((tp[0] <= g.xmin) || (tp[0] >= g.xmax) || (tp[1] <= g.ymin) || (tp[1] >= g.ymax) ||
-          (tp[2] <= g.zmin) || (tp[2] >= g.zmax))
-        return;
-      if (!(ar > VSMALL)) return;
-    } else {
-      if (!(ar > VSMALL)) return;
-      if ((tp[0] < (g.xmin - add - ar)) || (tp[0] > (g.xmax + add + ar)) ||
-          (tp[1] < (g.ymin - add - ar)) || (tp[1] > (g.ymax + add + ar)) ||
-          (tp[2] < (g.zmin - add - ar)) || (tp[2] > (g.zmax + add + ar)))
-        return;
+  const size_t n = (size_t)g.nx * g.ny * g.nz;
+  for (int ti = 0; ti < T.nt; ti++) {
+    const MarkTarget tg = T.t[ti];
+    const double add = tg.add;
+    double rtot = add;
+    if (tg.mode != 2) {
+      const double ar = rad[a];
+      if (tg.mode == 0) {
+        if ((tp[0] <= g.xmin) || (tp[0] >= g.xmax) || (tp[1] <= g.ymin) || (tp[1] >= g.ymax) ||
+            (tp[2] <= g.zmin) || (tp[2] >= g.zmax))
+          continue;
+        if (!(ar > VSMALL)) continue;
+      } else {
+        if (!(ar > VSMALL)) continue;
+        if ((tp[0] < (g.xmin - add - ar)) || (tp[0] > (g.xmax + add + ar)) ||
+            (tp[1] < (g.ymin - add - ar)) || (tp[1] > (g.ymax + add + ar)) ||
+            (tp[2] < (g.zmin - add - ar)) || (tp[2] > (g.zmax + add + ar)))
+          continue;
+      }
+      rtot = ar + add;
     }
-    rtot = ar + add;
-  }
-  const double posx = tp[0] - xs, posy = tp[1] - ys, posz = tp[2] - zs;
-  const double rtot2 = rtot * rtot;
-  const double xr = rtot + 0.5 * g.hx, yr = rtot + 0.5 * g.hy, zr = rtot + 0.5 * g.hz;
-  const int imin = imax2(0, (int)ceil((posx - xr) / g.hx));
-  const int jmin = imax2(0, (int)ceil((posy - yr) / g.hy));
-  const int kmin = imax2(0, (int)ceil((posz - zr) / g.hz));
-  const int imax = imin2(g.nx - 1, (int)floor((posx + xr) / g.hx));
-  const int jmax = imin2(g.ny - 1, (int)floor((posy + yr) / g.hy));
-  const int kmax = imin2(g.nz - 1, (int)floor((posz + zr) / g.hz));
-  const int ni = imax - imin + 1, nj = jmax - jmin + 1, nk = kmax - kmin + 1;
-  if (ni <= 0 || nj <= 0 || nk <= 0) return;
-  // lane e of the box walks e, e + 64, ...: the (i, j, k) offsets advance by
-  // 64 % ni columns and 64 / ni rows per step, carried without divisions
-  const int nbox = ni * nj * nk;
-  int ii = lane % ni, jj = (lane / ni) % nj, kk = lane / (ni * nj);
-  const int di = 64 % ni, dr = 64 / ni;
-  for (int e = lane; e < nbox; e += 64) {
-    const int i = imin + ii, j = jmin + jj, k = kmin + kk;
-    const double fi = i, fj = j, fk = k;
-    const double dx2 = (posx - g.hx * fi) * (posx - g.hx * fi);
-    const double dy2 = (posy - g.hy * fj) * (posy - g.hy * fj);
-    const double dz2 = (posz - g.hz * fk) * (posz - g.hz * fk);
-    if ((dz2 + dy2 + dx2) <= rtot2) arr[IX(i, j, k, g.nx, g.ny)] = val;
-    ii += di;
-    jj += dr;
-    if (ii >= ni) {
-      ii -= ni;
-      jj++;
+    const double xs = g.xmin + (tg.which == 0 ? 0.5 * g.hx : 0.0);
+    const double ys = g.ymin + (tg.which == 1 ? 0.5 * g.hy : 0.0);
+    const double zs = g.zmin + (tg.which == 2 ? 0.5 * g.hz : 0.0);
+    unsigned char* fl = flags + (size_t)ti * n;
+    const double posx = tp[0] - xs, posy = tp[1] - ys, posz = tp[2] - zs;
+    const double rtot2 = rtot * rtot;
+    const double xr = rtot + 0.5 * g.hx, yr = rtot + 0.5 * g.hy, zr = rtot + 0.5 * g.hz;
+    const int imin = imax2(0, (int)ceil((posx - xr) / g.hx));
+    const int jmin = imax2(0, (int)ceil((posy - yr) / g.hy));
+    const int kmin = imax2(0, (int)ceil((posz - zr) / g.hz));
+    const int imax = imin2(g.nx - 1, (int)floor((posx + xr) / g.hx));
+    const int jmax = imin2(g.ny - 1, (int)floor((posy + yr) / g.hy));
+    const int kmax = imin2(g.nz - 1, (int)floor((posz + zr) / g.hz));
+    const int ni = imax - imin + 1, nj = jmax - jmin + 1, nk = kmax - kmin + 1;
+    if (ni <= 0 || nj <= 0 || nk <= 0) continue;
+    if (ni <= 64 && nj <= 64 && nk <= 64) {
+      // Lane (ii, rr) = (lane % ni, lane / ni) keeps its column ii and walks
+      // the rows rr, rr + R, ... (R = 64 / ni rows per step).  The squared
+      // offsets of column ii, row jj and plane kk are each computed once by
+      // lane ii, jj or kk and fetched by shuffle; the sum keeps the
+      // reference's order (dz2 + dy2) + dx2.
+      const double fl_i = imin + lane, fl_j = jmin + lane, fl_k = kmin + lane;
+      const double dxl = (posx - g.hx * fl_i) * (posx - g.hx * fl_i);
+      const double dyl = (posy - g.hy * fl_j) * (posy - g.hy * fl_j);
+      const double dzl = (posz - g.hz * fl_k) * (posz - g.hz * fl_k);
+      const int R = 64 / ni;
+      const int ii = lane % ni, rr = lane / ni;
+      const bool col = rr < R;
+      const double dx2 = __shfl(dxl, ii, 64);
+      const int nrows = nj * nk;
+      int jj = rr % nj, kk = rr / nj;
+      for (int row0 = 0; row0 < nrows; row0 += R) {
+        const double dy2 = __shfl(dyl, jj < nj ? jj : 0, 64);
+        const double dz2 = __shfl(dzl, kk < nk ? kk : 0, 64);
+        if (col && row0 + rr < nrows && (dz2 + dy2 + dx2) <= rtot2)
+          fl[IX(imin + ii, jmin + jj, kmin + kk, g.nx, g.ny)] = 1;
+        jj += R;
+        while (jj >= nj) {
+          jj -= nj;
+          kk++;
+        }
+      }
+      continue;
     }
-    while (jj >= nj) {
-      jj -= nj;
-      kk++;
+    // wider boxes: lane e of the box walks e, e + 64, ...: the (i, j, k)
+    // offsets advance by 64 % ni columns and 64 / ni rows per step, carried
+    // without divisions
+    const int nbox = ni * nj * nk;
+    int ii = lane % ni, jj = (lane / ni) % nj, kk = lane / (ni * nj);
+    const int di = 64 % ni, dr = 64 / ni;
+    for (int e = lane; e < nbox; e += 64) {
+      const int i = imin + ii, j = jmin + jj, k = kmin + kk;
+      const double fi = i, fj = j, fk = k;
+      const double dx2 = (posx - g.hx * fi) * (posx - g.hx * fi);
+      const double dy2 = (posy - g.hy * fj) * (posy - g.hy * fj);
+      const double dz2 = (posz - g.hz * fk) * (posz - g.hz * fk);
+      if ((dz2 + dy2 + dx2) <= rtot2) fl[IX(i, j, k, g.nx, g.ny)] = 1;
+      ii += di;
+      jj += dr;
+      if (ii >= ni) {
+        ii -= ni;
+        jj++;
+      }
+      while (jj >= nj) {
+        jj -= nj;
+        kk++;
+      }
     }
   }
 }
 
+// The maps of the molecular surface from the flags, one grid point per thread.
+// The harmonic mean takes 1 / epsp or 1 / epsw for each neighbour value, the
+// same quotients as the reference's 1.0 / a (vpmg.c:4500-4628).
 __global__ void __launch_bounds__(kB)
-    k_mol_smooth(Geo g, const double* a1, const double* a2, const double* a3, double* epsx,
-                 double* epsy, double* epsz) {
+    k_mol_maps(Geo g, const unsigned char* fa, const unsigned char* fsas,
+               const unsigned char* fk, double epsp, double epsw, double ionmask, int smooth,
+               double* epsx, double* epsy, double* epsz, double* kappa) {
   const int t = blockIdx.x * kB + threadIdx.x;
   const int k = blockIdx.y;
   const int i = t % g.nx, j = t / g.nx;
   if (j >= g.ny) return;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const size_t n = (size_t)nx * ny * nz;
+  const size_t p = IX(i, j, k, nx, ny);
+  // unsmoothed value of map w at q: epsp inside an atom and not re-marked
+  auto solute = [&](int w, size_t q) {
+    return fa[w * n + q] != 0 && !(fsas && fsas[w * n + q] != 0);
+  };
+  kappa[p] = (fk && fk[p]) ? 0.0 : ionmask;
+  if (!smooth) {
+    epsx[p] = solute(0, p) ? epsp : epsw;
+    epsy[p] = solute(1, p) ? epsp : epsw;
+    epsz[p] = solute(2, p) ? epsp : epsw;
+    return;
+  }
+  const double rp = 1.0 / epsp, rw = 1.0 / epsw;
+  auto inv = [&](int w, int ii, int jj, int kk) {
+    return solute(w, IX(ii, jj, kk, nx, ny)) ? rp : rw;
+  };
   double frac;
   int np;
-  frac = 1.0 / a1[IX(i, j, k, nx, ny)];
-  frac += 1.0 / a2[IX(i, j, k, nx, ny)];
-  frac += 1.0 / a3[IX(i, j, k, nx, ny)];
+  frac = inv(0, i, j, k);
+  frac += inv(1, i, j, k);
+  frac += inv(2, i, j, k);
   np = 3;
-  if (j > 0) { frac += 1.0 / a2[IX(i, j - 1, k, nx, ny)]; np += 1; }
-  if (k > 0) { frac += 1.0 / a3[IX(i, j, k - 1, nx, ny)]; np += 1; }
+  if (j > 0) { frac += inv(1, i, j - 1, k); np += 1; }
+  if (k > 0) { frac += inv(2, i, j, k - 1); np += 1; }
   if (i < nx - 1) {
-    frac += 1.0 / a2[IX(i + 1, j, k, nx, ny)];
-    frac += 1.0 / a3[IX(i + 1, j, k, nx, ny)];
+    frac += inv(1, i + 1, j, k);
+    frac += inv(2, i + 1, j, k);
     np += 2;
-    if (j > 0) { frac += 1.0 / a2[IX(i + 1, j - 1, k, nx, ny)]; np += 1; }
-    if (k > 0) { frac += 1.0 / a3[IX(i + 1, j, k - 1, nx, ny)]; np += 1; }
+    if (j > 0) { frac += inv(1, i + 1, j - 1, k); np += 1; }
+    if (k > 0) { frac += inv(2, i + 1, j, k - 1); np += 1; }
   }
-  epsx[IX(i, j, k, nx, ny)] = np / frac;
+  epsx[p] = np / frac;
 
-  frac = 1.0 / a2[IX(i, j, k, nx, ny)];
-  frac += 1.0 / a1[IX(i, j, k, nx, ny)];
-  frac += 1.0 / a3[IX(i, j, k, nx, ny)];
+  frac = inv(1, i, j, k);
+  frac += inv(0, i, j, k);
+  frac += inv(2, i, j, k);
   np = 3;
-  if (i > 0) { frac += 1.0 / a1[IX(i - 1, j, k, nx, ny)]; np += 1; }
-  if (k > 0) { frac += 1.0 / a3[IX(i, j, k - 1, nx, ny)]; np += 1; }
+  if (i > 0) { frac += inv(0, i - 1, j, k); np += 1; }
+  if (k > 0) { frac += inv(2, i, j, k - 1); np += 1; }
   if (j < ny - 1) {
-    frac += 1.0 / a1[IX(i, j + 1, k, nx, ny)];
-    frac += 1.0 / a3[IX(i, j + 1, k, nx, ny)];
+    frac += inv(0, i, j + 1, k);
+    frac += inv(2, i, j + 1, k);
     np += 2;
-    if (i > 0) { frac += 1.0 / a1[IX(i - 1, j + 1, k, nx, ny)]; np += 1; }
-    if (k > 0) { frac += 1.0 / a3[IX(i, j + 1, k - 1, nx, ny)]; np += 1; }
+    if (i > 0) { frac += inv(0, i - 1, j + 1, k); np += 1; }
+    if (k > 0) { frac += inv(2, i, j + 1, k - 1); np += 1; }
   }
-  epsy[IX(i, j, k, nx, ny)] = np / frac;
+  epsy[p] = np / frac;
 
-  frac = 1.0 / a3[IX(i, j, k, nx, ny)];
-  frac += 1.0 / a1[IX(i, j, k, nx, ny)];
-  frac += 1.0 / a2[IX(i, j, k, nx, ny)];
+  frac = inv(2, i, j, k);
+  frac += inv(0, i, j, k);
+  frac += inv(1, i, j, k);
   np = 3;
-  if (i > 0) { frac += 1.0 / a1[IX(i - 1, j, k, nx, ny)]; np += 1; }
-  if (j > 0) { frac += 1.0 / a2[IX(i, j - 1, k, nx, ny)]; np += 1; }
+  if (i > 0) { frac += inv(0, i - 1, j, k); np += 1; }
+  if (j > 0) { frac += inv(1, i, j - 1, k); np += 1; }
   if (k < nz - 1) {
-    frac += 1.0 / a1[IX(i, j, k + 1, nx, ny)];
-    frac += 1.0 / a2[IX(i, j, k + 1, nx, ny)];
+    frac += inv(0, i, j, k + 1);
+    frac += inv(1, i, j, k + 1);
     np += 2;
-    if (i > 0) { frac += 1.0 / a1[IX(i - 1, j, k + 1, nx, ny)]; np += 1; }
-    if (j > 0) { frac += 1.0 / a2[IX(i, j - 1, k + 1, nx, ny)]; np += 1; }
+    if (i > 0) { frac += inv(0, i - 1, j, k + 1); np += 1; }
+    if (j > 0) { frac += inv(1, i, j - 1, k + 1); np += 1; }
   }
-  epsz[IX(i, j, k, nx, ny)] = np / frac;
+  epsz[p] = np / frac;
 }
 
 // -------------------------------------------------------------- boundaries
@@ -704,21 +736,39 @@ __device__ inline int face_count(const Geo& g, int f) {
 // the chunk, whose sequential partial sum goes to part[chunk][face][t], and
 // k_bc_mdh_sum adds the partials in chunk order.  With one chunk the sum is
 // the reference's atom-order sum.
-// Per-atom factors of the MDH term: q = ec * charge, the screening
-// denominator 1 + kappa * radius and its correctly rounded reciprocal.
-__global__ void __launch_bounds__(kB) k_bc_mdh_atoms(AtomsDev A, double xkappa, double* fac) {
+//
+// The reference's term pre1 * (q / d) * exp(-kappa (d - r)) / (1 + kappa r)
+// is evaluated as c_a * exp(-kappa d) / d with the per-atom factor
+// c_a = pre1 q exp(kappa r) / (1 + kappa r) (q = ec * charge), and 1 / d from
+// the hardware reciprocal square root refined by two Newton steps, so a term
+// costs one exponential and no division or square root.  Each term is within
+// a few ulp of the reference's expression.
+__global__ void __launch_bounds__(kB)
+    k_bc_mdh_atoms(AtomsDev A, double pre1, double xkappa, double* fac) {
   const int a = blockIdx.x * kB + threadIdx.x;
   if (a >= A.n) return;
   const double ec = 1.6021773e-19;  // Vunit_ec, src/generic/vunit.h:80
-  const double den = 1 + xkappa * A.rad[a];
-  fac[3 * a] = ec * A.chg[a];
-  fac[3 * a + 1] = den;
-  fac[3 * a + 2] = 1.0 / den;
+  const double q = ec * A.chg[a];
+  if (xkappa > VSMALL) {
+    const double r = A.rad[a];
+    fac[a] = pre1 * q * exp(xkappa * r) / (1 + xkappa * r);
+  } else {
+    fac[a] = pre1 * q;
+  }
+}
+
+// 1 / sqrt(d2) for d2 > 0: v_rsq_f64 and two Newton steps y += y (1 - d2 y^2) / 2
+__device__ inline double rsqrt_refined(double d2) {
+  double y = __builtin_amdgcn_rsq(d2);
+  double e = fma(-d2 * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-d2 * y, y, 1.0);
+  return fma(0.5 * y, e, y);
 }
 
 __global__ void __launch_bounds__(kB)
     k_bc_mdh(Geo g, const double* xf, const double* yf, const double* zf, AtomsDev A,
-             const double* fac, double pre1, double xkappa, int nch, int mx, double* part) {
+             const double* fac, double xkappa, int nch, int mx, double* part) {
   const int f = blockIdx.y;
   const int t = blockIdx.x * kB + threadIdx.x;
   if (t >= face_count(g, f)) return;
@@ -728,23 +778,14 @@ __global__ void __launch_bounds__(kB)
   const int c = blockIdx.z;
   const int per = (A.n + nch - 1) / nch;
   const int a0 = c * per, a1 = min(A.n, a0 + per);
-  // One atom's term, as the reference writes it:
-  //   pre1 * (q / dist) * exp(-kappa * (dist - radius)) / (1 + kappa * radius).
-  // The last division is q0 = v * y, then q0 + (v - q0 * den) * y with two
-  // FMAs: y the correctly rounded 1 / den, this is the correctly rounded
-  // quotient v / den (Markstein's theorem; no underflow at these magnitudes).
+  const bool screened = xkappa > VSMALL;
   auto term = [&](int a) {
     const double* ap = A.pos + 3 * a;
-    const double q = fac[3 * a];
-    const double dist = sqrt((px - ap[0]) * (px - ap[0]) + (py - ap[1]) * (py - ap[1]) +
-                             (pz - ap[2]) * (pz - ap[2]));
-    if (xkappa > VSMALL) {
-      const double v = pre1 * (q / dist) * exp(-xkappa * (dist - A.rad[a]));
-      const double den = fac[3 * a + 1], y = fac[3 * a + 2];
-      const double q0 = v * y;
-      return fma(fma(-q0, den, v), y, q0);
-    }
-    return pre1 * (q / dist);
+    const double dx = px - ap[0], dy = py - ap[1], dz = pz - ap[2];
+    const double d2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double y = rsqrt_refined(d2);
+    const double v = fac[a] * y;
+    return screened ? v * exp(-xkappa * (d2 * y)) : v;
   };
   // four independent terms in flight (their dependency chains interleave),
   // added in atom order
@@ -1383,23 +1424,27 @@ static dim3 grid_all(const Geo& g) {
   return dim3((unsigned)(((long)g.nx * g.ny + kB - 1) / kB), (unsigned)g.nz);
 }
 
-void charge_map(hipStream_t s, const Geo& g, const AtomsDev& A, const Cells& C, int mode,
-                double zmagic, int, double* charge) {
-  k_charge<<<grid_all(g), kB, 0, s>>>(g, A, C, mode, zmagic, charge);
+static unsigned n_tiles(const Tiles& T) {
+  return (unsigned)((long)T.tdim[0] * T.tdim[1] * T.tdim[2]);
 }
 
-void coef_spline(hipStream_t s, const Geo& g, const AtomsDev& A, const Cells& C, double win,
+void charge_map(hipStream_t s, const Geo& g, const AtomsDev& A, const Tiles& T, int mode,
+                double zmagic, int, double* charge) {
+  k_charge<<<n_tiles(T), kB, 0, s>>>(g, A, T, mode, zmagic, charge);
+}
+
+void coef_spline(hipStream_t s, const Geo& g, const AtomsDev& A, const Tiles& T, double win,
                  double irad, double ionmask, double epsw, double epsp, double* kappa,
                  double* epsx, double* epsy, double* epsz, int order, const double* coef) {
   if (order == 3)
-    k_coef_spline<3><<<grid_all(g), kB, 0, s>>>(g, A, C, win, irad, ionmask, epsw, epsp, coef,
-                                                kappa, epsx, epsy, epsz);
+    k_coef_spline<3><<<n_tiles(T), kB, 0, s>>>(g, A, T, win, irad, ionmask, epsw, epsp, coef,
+                                               kappa, epsx, epsy, epsz);
   else if (order == 4)
-    k_coef_spline<4><<<grid_all(g), kB, 0, s>>>(g, A, C, win, irad, ionmask, epsw, epsp, coef,
-                                                kappa, epsx, epsy, epsz);
+    k_coef_spline<4><<<n_tiles(T), kB, 0, s>>>(g, A, T, win, irad, ionmask, epsw, epsp, coef,
+                                               kappa, epsx, epsy, epsz);
   else
-    k_coef_spline<2><<<grid_all(g), kB, 0, s>>>(g, A, C, win, irad, ionmask, epsw, epsp, nullptr,
-                                                kappa, epsx, epsy, epsz);
+    k_coef_spline<2><<<n_tiles(T), kB, 0, s>>>(g, A, T, win, irad, ionmask, epsw, epsp, nullptr,
+                                               kappa, epsx, epsy, epsz);
 }
 
 void sas_flags(hipStream_t s, const AtomsDev& A, const Cells& C, const double* ref, int nref,
@@ -1429,19 +1474,18 @@ int sas_compact(hipStream_t s, const AtomsDev& A, const double* ref, int nref, d
   return tot;
 }
 
-void mark_spheres(hipStream_t s, const Geo& g, int npts, const double* pos, const double* rad,
-                  double add, int mode, int which, double val, double* arr) {
-  if (npts <= 0) return;
-  const double xs = g.xmin + (which == 0 ? 0.5 * g.hx : 0.0);
-  const double ys = g.ymin + (which == 1 ? 0.5 * g.hy : 0.0);
-  const double zs = g.zmin + (which == 2 ? 0.5 * g.hz : 0.0);
-  k_mark<<<(npts + kMarkWaves - 1) / kMarkWaves, kB, 0, s>>>(g, npts, pos, rad, add, mode, xs, ys,
-                                                              zs, val, arr);
+void mark_flags(hipStream_t s, const Geo& g, int npts, const double* pos, const double* rad,
+                const MarkTargets& T, unsigned char* flags) {
+  if (npts <= 0 || T.nt <= 0) return;
+  k_mark_flags<<<(npts + kMarkWaves - 1) / kMarkWaves, kB, 0, s>>>(g, npts, pos, rad, T, flags);
 }
 
-void mol_smooth(hipStream_t s, const Geo& g, const double* a1, const double* a2, const double* a3,
-                double* epsx, double* epsy, double* epsz) {
-  k_mol_smooth<<<grid_all(g), kB, 0, s>>>(g, a1, a2, a3, epsx, epsy, epsz);
+void mol_maps(hipStream_t s, const Geo& g, const unsigned char* atom_flags,
+              const unsigned char* sas_flags, const unsigned char* kappa_flags, double epsp,
+              double epsw, double ionmask, int smooth, double* epsx, double* epsy, double* epsz,
+              double* kappa) {
+  k_mol_maps<<<grid_all(g), kB, 0, s>>>(g, atom_flags, sas_flags, kappa_flags, epsp, epsw,
+                                        ionmask, smooth, epsx, epsy, epsz, kappa);
 }
 
 int bc_mdh_chunks(int natoms) { return natoms >= 4096 ? 16 : (natoms >= 512 ? natoms / 256 : 1); }
@@ -1450,7 +1494,7 @@ size_t bc_mdh_scratch(const Geo& g, int natoms) {
   size_t mx = (size_t)g.ny * g.nz;
   if ((size_t)g.nx * g.nz > mx) mx = (size_t)g.nx * g.nz;
   if ((size_t)g.nx * g.ny > mx) mx = (size_t)g.nx * g.ny;
-  return (size_t)bc_mdh_chunks(natoms) * 6 * mx + 3 * (size_t)(natoms > 0 ? natoms : 1);
+  return (size_t)bc_mdh_chunks(natoms) * 6 * mx + (size_t)(natoms > 0 ? natoms : 1);
 }
 
 void bc_mdh(hipStream_t s, const Geo& g, const double* xf, const double* yf, const double* zf,
@@ -1462,9 +1506,9 @@ void bc_mdh(hipStream_t s, const Geo& g, const double* xf, const double* yf, con
   const int nch = bc_mdh_chunks(A.n);
   // the per-atom factors follow the partial sums in the scratch buffer
   double* fac = part + (size_t)nch * 6 * mx;
-  if (A.n > 0) k_bc_mdh_atoms<<<(A.n + kB - 1) / kB, kB, 0, s>>>(A, xkappa, fac);
-  k_bc_mdh<<<dim3((mx + kB - 1) / kB, 6, nch), kB, 0, s>>>(g, xf, yf, zf, A, fac, pre1, xkappa,
-                                                           nch, mx, part);
+  if (A.n > 0) k_bc_mdh_atoms<<<(A.n + kB - 1) / kB, kB, 0, s>>>(A, pre1, xkappa, fac);
+  k_bc_mdh<<<dim3((mx + kB - 1) / kB, 6, nch), kB, 0, s>>>(g, xf, yf, zf, A, fac, xkappa, nch,
+                                                           mx, part);
   k_bc_mdh_sum<<<dim3((mx + kB - 1) / kB, 6), kB, 0, s>>>(g, nch, mx, part, gx, gy, gz);
 }
 

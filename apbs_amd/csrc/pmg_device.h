@@ -137,13 +137,17 @@ void scale(hipStream_t s, size_t n, double a, const double* x, double* y);
 template <class T>
 void add_diag(hipStream_t s, size_t n, const T* oC, const T* cc, T* D, float* Df = nullptr);
 // Arrays cleared together on one stream (sizes multiples of 4 bytes): add()
-// queues an array, flush() clears the queued ones in one launch; a full
-// queue and the destructor flush.
+// queues a whole array, add_shell() only the outer layer of a grid array (the
+// points with i, j or k on the boundary), flush() clears the queued ones in
+// one launch; a full queue and the destructor flush.
 constexpr int kZeroMax = 48;
 struct ZeroArgs {
   int n;
-  size_t off[kZeroMax + 1];
+  unsigned blk[kZeroMax + 1];  // first block of each array; blk[n] = blocks
   unsigned* p[kZeroMax];
+  size_t words[kZeroMax];      // whole arrays: 4-byte words to clear
+  int nx[kZeroMax], ny[kZeroMax], nz[kZeroMax];  // shells: grid shape (nx = 0: whole)
+  int wpe[kZeroMax];           // shells: 4-byte words per element
 };
 struct ZeroList {
   hipStream_t s;
@@ -153,6 +157,7 @@ struct ZeroList {
   ZeroList(const ZeroList&) = delete;
   ZeroList& operator=(const ZeroList&) = delete;
   void add(void* ptr, size_t bytes);
+  void add_shell(void* ptr, int nx, int ny, int nz, size_t elem_bytes);
   void flush();
 };
 // add_diag of up to 24 levels in one launch; off[l] is the first element of

@@ -780,14 +780,54 @@ __global__ void k_add_diag(size_t n, const T* __restrict__ oC, const T* __restri
   }
 }
 
-// Zero up to kZeroMax arrays in one launch (4-byte words).
+// Zero up to kZeroMax arrays in one launch.  Each array owns a contiguous
+// range of blocks (found once per block).  A whole array is cleared in 16-byte
+// stores, kZeroChunk words per block; a shell array one outer-layer element
+// per thread: the planes k = 0 and nz-1, then per inner plane the rows j = 0
+// and ny-1 and the ends i = 0 and nx-1 of the other rows.
+constexpr unsigned kZeroChunk = 4 * 4 * kBlock;
+
+__device__ inline size_t shell_index(int nx, int ny, int nz, size_t e) {
+  const size_t plane = (size_t)nx * ny;
+  if (e < plane) return e;
+  if (e < 2 * plane) return (size_t)(nz - 1) * plane + (e - plane);
+  const size_t r0 = e - 2 * plane;
+  const size_t per = 2 * (size_t)nx + 2 * (size_t)(ny - 2);
+  const size_t k = 1 + r0 / per;
+  const size_t r = r0 % per;
+  size_t i, j;
+  if (r < (size_t)nx) {
+    i = r;
+    j = 0;
+  } else if (r < 2 * (size_t)nx) {
+    i = r - nx;
+    j = ny - 1;
+  } else {
+    const size_t q = r - 2 * (size_t)nx;
+    j = 1 + q / 2;
+    i = (q & 1) ? nx - 1 : 0;
+  }
+  return (k * ny + j) * nx + i;
+}
+
 __global__ void __launch_bounds__(kBlock) k_zero_many(ZeroArgs Z) {
-  const size_t tot = Z.off[Z.n];
-  for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < tot;
-       t += (size_t)gridDim.x * kBlock) {
-    int a = 0;
-    while (t >= Z.off[a + 1]) a++;
-    Z.p[a][t - Z.off[a]] = 0u;
+  const unsigned b = blockIdx.x;
+  int a = 0;
+  while (a + 1 < Z.n && b >= Z.blk[a + 1]) a++;
+  const unsigned lb = b - Z.blk[a];
+  unsigned* base = Z.p[a];
+  if (Z.nx[a] == 0) {
+    const size_t w0 = (size_t)lb * kZeroChunk;
+    const size_t w1 = min(w0 + kZeroChunk, Z.words[a]);
+    const size_t v1 = w0 + ((w1 - w0) & ~(size_t)3);
+    uint4* v = reinterpret_cast<uint4*>(base + w0);
+    for (size_t t = threadIdx.x; w0 + 4 * t < v1; t += kBlock) v[t] = make_uint4(0u, 0u, 0u, 0u);
+    for (size_t w = v1 + threadIdx.x; w < w1; w += kBlock) base[w] = 0u;
+  } else {
+    const size_t e = (size_t)lb * kBlock + threadIdx.x;
+    if (e >= Z.words[a]) return;
+    const size_t p = shell_index(Z.nx[a], Z.ny[a], Z.nz[a], e) * Z.wpe[a];
+    for (int w = 0; w < Z.wpe[a]; w++) base[p + w] = 0u;
   }
 }
 
@@ -1724,14 +1764,35 @@ void ZeroList::add(void* ptr, size_t bytes) {
   if (!ptr || bytes < 4) return;
   if (z.n == kZeroMax) flush();
   z.p[z.n] = (unsigned*)ptr;
-  z.off[z.n + 1] = z.off[z.n] + bytes / 4;
+  z.words[z.n] = bytes / 4;
+  z.nx[z.n] = z.ny[z.n] = z.nz[z.n] = 0;
+  z.wpe[z.n] = 0;
+  z.blk[z.n + 1] = z.blk[z.n] + (unsigned)((bytes / 4 + kZeroChunk - 1) / kZeroChunk);
+  z.n++;
+}
+
+void ZeroList::add_shell(void* ptr, int nx, int ny, int nz, size_t elem_bytes) {
+  if (!ptr || nx < 1 || ny < 1 || nz < 1) return;
+  if (nx < 3 || ny < 3 || nz < 3) {  // every point is on the boundary
+    add(ptr, (size_t)nx * ny * nz * elem_bytes);
+    return;
+  }
+  if (z.n == kZeroMax) flush();
+  const size_t pts = 2 * (size_t)nx * ny + (size_t)(nz - 2) * (2 * (size_t)nx + 2 * (size_t)(ny - 2));
+  z.p[z.n] = (unsigned*)ptr;
+  z.words[z.n] = pts;
+  z.nx[z.n] = nx;
+  z.ny[z.n] = ny;
+  z.nz[z.n] = nz;
+  z.wpe[z.n] = (int)(elem_bytes / 4);
+  z.blk[z.n + 1] = z.blk[z.n] + (unsigned)((pts + kBlock - 1) / kBlock);
   z.n++;
 }
 
 void ZeroList::flush() {
-  if (z.n > 0) k_zero_many<<<nblocks(z.off[z.n]), kBlock, 0, s>>>(z);
+  if (z.n > 0 && z.blk[z.n] > 0) k_zero_many<<<z.blk[z.n], kBlock, 0, s>>>(z);
   z.n = 0;
-  z.off[0] = 0;
+  z.blk[0] = 0;
 }
 
 void add_diag_levels(hipStream_t s, const DiagLevels& L) {

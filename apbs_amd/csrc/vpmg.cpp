@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <limits>
 #include <vector>
 
 #include "../../include/apbs_pmg.h"
@@ -663,6 +664,19 @@ struct DevCells {
   }
 };
 
+// Tile lists (pmg::Tiles) on the device, with the host arrays they were
+// uploaded from (kept until release, after the kernels that read them).
+struct DevTiles {
+  int* start = nullptr;
+  int* list = nullptr;
+  pmg::Tiles t{};
+  std::vector<int> hstart, hlist;
+  void release() {
+    dfree(start);
+    dfree(list);
+  }
+};
+
 struct PmgDev {
   hipStream_t s = nullptr;
   int nx = 0, ny = 0, nz = 0;
@@ -721,6 +735,7 @@ struct PmgDev {
   bool cs_pending = false;
   int mirror = 1;
   bool failed = false;  // the last solve stopped on a failure (solve_failed)
+  bool poison = false;  // test hook: NaN-fill the interiors of a new hierarchy
   double stats[8] = {};
   pmg::Geo geo{};
   // kernel timing: HIP event pairs recorded on the object's stream around
@@ -862,6 +877,57 @@ DevCells build_cells(hipStream_t s, const double* hpos, int n, double reach) {
   dc.c.start = dc.start;
   dc.c.list = dc.list;
   return dc;
+}
+
+// Tile lists for the order-sensitive gathers (charge and spline maps): atom a
+// is listed, in ascending index, in every kTile^3 tile that meets the node box
+// [floor(c - R) - 1, ceil(c + R) + 1] per axis, c its position in nodes and R
+// its reach (A, reach[a] or reach[0] for all) in nodes.  The box contains the
+// points the kernels' own tests can accept, so the lists are supersets.
+DevTiles build_tiles(hipStream_t s, const pmg::Geo& g, const double* hpos, int n,
+                     const std::vector<double>& reach) {
+  DevTiles dt;
+  const int nn[3] = {g.nx, g.ny, g.nz};
+  const double h[3] = {g.hx, g.hy, g.hz}, lo[3] = {g.xmin, g.ymin, g.zmin};
+  for (int d = 0; d < 3; d++) dt.t.tdim[d] = (nn[d] + pmg::kTile - 1) / pmg::kTile;
+  const long nt = (long)dt.t.tdim[0] * dt.t.tdim[1] * dt.t.tdim[2];
+  dt.hstart.assign(nt + 1, 0);
+  std::vector<int> box(6 * (size_t)std::max(n, 1));
+  for (int a = 0; a < n; a++) {
+    const double r = reach.size() > 1 ? reach[a] : reach[0];
+    int* b = &box[6 * (size_t)a];
+    for (int d = 0; d < 3; d++) {
+      const double c = (hpos[3 * a + d] - lo[d]) / h[d], R = r / h[d];
+      const double blo = std::floor(c - R) - 1, bhi = std::ceil(c + R) + 1;
+      const int l = (int)std::max(0.0, std::min(blo, (double)nn[d])),
+                hgh = (int)std::max(-1.0, std::min(bhi, (double)(nn[d] - 1)));
+      b[2 * d] = l / pmg::kTile;
+      b[2 * d + 1] = l > hgh ? -1 : hgh / pmg::kTile;
+    }
+    for (int tz = b[4]; tz <= b[5]; tz++)
+      for (int ty = b[2]; ty <= b[3]; ty++)
+        for (int tx = b[0]; tx <= b[1]; tx++)
+          dt.hstart[tx + (long)dt.t.tdim[0] * (ty + (long)dt.t.tdim[1] * tz) + 1]++;
+  }
+  for (long t = 0; t < nt; t++) dt.hstart[t + 1] += dt.hstart[t];
+  dt.hlist.assign(std::max(dt.hstart[nt], 1), 0);
+  std::vector<int> fill(dt.hstart.begin(), dt.hstart.end() - 1);
+  for (int a = 0; a < n; a++) {
+    const int* b = &box[6 * (size_t)a];
+    for (int tz = b[4]; tz <= b[5]; tz++)
+      for (int ty = b[2]; ty <= b[3]; ty++)
+        for (int tx = b[0]; tx <= b[1]; tx++)
+          dt.hlist[fill[tx + (long)dt.t.tdim[0] * (ty + (long)dt.t.tdim[1] * tz)]++] = a;
+  }
+  dt.start = dalloc<int>(nt + 1);
+  dt.list = dalloc<int>(dt.hlist.size());
+  HIPCHECK(hipMemcpyAsync(dt.start, dt.hstart.data(), (nt + 1) * sizeof(int),
+                          hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemcpyAsync(dt.list, dt.hlist.data(), dt.hlist.size() * sizeof(int),
+                          hipMemcpyHostToDevice, s));
+  dt.t.start = dt.start;
+  dt.t.list = dt.list;
+  return dt;
 }
 
 // VaccSurf_refSphere (vacc.c:924-978); computed with the host libm, as the
@@ -1025,14 +1091,28 @@ size_t pmgint::footprint_bytes(int nx, int ny, int nz, int nlev, bool newton, bo
 namespace {
 
 void alloc_hierarchy(PmgDev* d, int nlev, bool newton, bool mixed) {
-  // Every array starts at zero, like the reference's zeroed workspace: the
-  // operator builders write interior entries only, and the 27-point stencil
-  // reads backward coefficients of the boundary layer (multiplied by the zero
-  // boundary of x).  A hierarchy of the same shape is reused by later solves;
-  // its interiors are rewritten before they are read.
+  // Every array's boundary layer starts at zero, like the reference's zeroed
+  // workspace: the operator builders and every solver kernel write interior
+  // entries only, and the stencils read coefficients and values of the
+  // boundary layer.  The interiors are written before they are read, which is
+  // also what lets a hierarchy of the same shape be reused by later solves,
+  // so a new one clears the outer layer of each array only (a recycled pool
+  // block's interior may hold anything).  With the poison test hook set, the
+  // interiors are first filled with NaN.
   if (d->ntot > 0 && d->nlev_ref == nlev && d->h_newton == newton && d->h_mixed == mixed) return;
   free_hierarchy(d);
-  pmg::ZeroList z(d->s);
+  struct Grid {
+    void* p;
+    int nx, ny, nz;
+    size_t elem;
+  };
+  std::vector<Grid> grids;
+  auto grid64 = [&](double* p, const DLevel& L) {
+    grids.push_back({p, L.nx, L.ny, L.nz, sizeof(double)});
+  };
+  auto grid32 = [&](float* p, const DLevel& L) {
+    grids.push_back({p, L.nx, L.ny, L.nz, sizeof(float)});
+  };
   d->h_newton = newton;
   d->h_mixed = mixed;
   int dims[24][3];
@@ -1048,66 +1128,79 @@ void alloc_hierarchy(PmgDev* d, int nlev, bool newton, bool mixed) {
     const int na = l == 0 ? 4 : 14;
     for (int s = 0; s < na; s++) {
       L.a[s] = dalloc<double>(L.n);
-      z.add(L.a[s], (L.n) * sizeof(double));
+      grid64(L.a[s], L);
     }
     L.cc = dalloc<double>(L.n);
     L.fc = dalloc<double>(L.n);
     L.D = dalloc<double>(L.n);
-    z.add(L.cc, (L.n) * sizeof(double));
-    z.add(L.fc, (L.n) * sizeof(double));
-    z.add(L.D, (L.n) * sizeof(double));
+    grid64(L.cc, L);
+    grid64(L.fc, L);
+    grid64(L.D, L);
     L.x = l == 0 ? d->u : dalloc<double>(L.n);
     if (l > 0) {
       L.xs = dalloc<double>(L.n);
-      z.add(L.xs, (L.n) * sizeof(double));
+      grid64(L.xs, L);
+      grid64(L.x, L);
     }
     L.w0 = dalloc<double>(L.n);
-    z.add(L.w0, (L.n) * sizeof(double));
-    if (l > 0) z.add(L.x, (L.n) * sizeof(double));
+    grid64(L.w0, L);
     if (newton) {
       L.cprime = dalloc<double>(L.n);
-      z.add(L.cprime, (L.n) * sizeof(double));
+      grid64(L.cprime, L);
       L.xtmp = dalloc<double>(L.n);
-      z.add(L.xtmp, (L.n) * sizeof(double));
+      grid64(L.xtmp, L);
     }
     if (mixed) {
       for (int s = 1; s < na; s++) {  // oC is folded into Df
         L.af[s] = dalloc<float>(L.n);
-        z.add(L.af[s], (L.n) * sizeof(float));
+        grid32(L.af[s], L);
       }
       L.Df = dalloc<float>(L.n);
-      z.add(L.Df, (L.n) * sizeof(float));
+      grid32(L.Df, L);
       L.xfl = dalloc<float>(L.n);
       L.w0f = dalloc<float>(L.n);
+      grid32(L.xfl, L);
+      grid32(L.w0f, L);
       if (l > 0) {
         L.xsf = dalloc<float>(L.n);
-        z.add(L.xsf, (L.n) * sizeof(float));
+        grid32(L.xsf, L);
       }
-      z.add(L.xfl, (L.n) * sizeof(float));
-      z.add(L.w0f, (L.n) * sizeof(float));
     }
   }
+  const DLevel& F = d->L[0];
   d->w1 = dalloc<double>(d->n);
-  z.add(d->w1, (d->n) * sizeof(double));
+  grid64(d->w1, F);
   d->xs64 = dalloc<double>(d->n);
-  z.add(d->xs64, (d->n) * sizeof(double));
+  grid64(d->xs64, F);
   if (mixed) {
     d->w1f = dalloc<float>(d->n);
-    z.add(d->w1f, (d->n) * sizeof(float));
+    grid32(d->w1f, F);
     d->xs32 = dalloc<float>(d->n);
-    z.add(d->xs32, (d->n) * sizeof(float));
+    grid32(d->xs32, F);
   }
   d->F = dalloc<double>(d->n);
-  z.add(d->F, (d->n) * sizeof(double));
+  grid64(d->F, F);
   if (newton) {
     d->wn1 = dalloc<double>(d->n);
     d->wn2 = dalloc<double>(d->n);
     d->wn3 = dalloc<double>(d->n);
     d->nlw = dalloc<double>(d->n);
-    z.add(d->nlw, (d->n) * sizeof(double));
-    z.add(d->wn1, (d->n) * sizeof(double));
-    z.add(d->wn2, (d->n) * sizeof(double));
-    z.add(d->wn3, (d->n) * sizeof(double));
+    grid64(d->nlw, F);
+    grid64(d->wn1, F);
+    grid64(d->wn2, F);
+    grid64(d->wn3, F);
+  }
+  if (d->poison)
+    for (const Grid& g : grids) {
+      const size_t n = (size_t)g.nx * g.ny * g.nz;
+      if (g.elem == sizeof(double))
+        pmg::fill<double>(d->s, (double*)g.p, n, std::numeric_limits<double>::quiet_NaN());
+      else
+        pmg::fill<float>(d->s, (float*)g.p, n, std::numeric_limits<float>::quiet_NaN());
+    }
+  {
+    pmg::ZeroList z(d->s);
+    for (const Grid& g : grids) z.add_shell(g.p, g.nx, g.ny, g.nz, g.elem);
   }
   const DLevel& C = d->L[d->ntot - 1];
   d->dn = (C.nx - 2) * (C.ny - 2) * (C.nz - 2);
@@ -1977,6 +2070,16 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
     HIPCHECK(hipEventCreateWithFlags(&side.done, hipEventDisableTiming));
     HIPCHECK(hipEventRecord(side.done, side.s2));
   }
+  struct TilesGuard {
+    hipStream_t s;
+    DevTiles t;
+    ~TilesGuard() {
+      if (t.start || t.list) {
+        HIPCHECK(hipStreamSynchronize(s));
+        t.release();
+      }
+    }
+  } charge_tiles{d->s, {}};
   Valist* al = pbe->alist;
   std::vector<double> hpos(3 * (size_t)d->natoms), hrad(d->natoms);
   double rmax = 0;
@@ -1994,11 +2097,11 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
       return 0;
     }
   } else {
-    const double reach = (chargeMeth == VCM_BSPL4 ? 3.0 : 2.0) * hmax + 1e-9;
-    DevCells cells = build_cells(d->s, hpos.data(), d->natoms, reach);
-    pmg::charge_map(d->s, g, A, cells.c, (int)chargeMeth, pbe->zmagic, 0, d->charge);
-    HIPCHECK(hipStreamSynchronize(d->s));
-    cells.release();
+    // the lists are released when fillco returns, after its stream has
+    // drained (no synchronization here: the surface maps queue behind)
+    const double reach = (chargeMeth == VCM_BSPL4 ? 3.0 : 2.0) * hmax;
+    charge_tiles.t = build_tiles(d->s, g, hpos.data(), d->natoms, {reach});
+    pmg::charge_map(d->s, g, A, charge_tiles.t.t, (int)chargeMeth, pbe->zmagic, 0, d->charge);
   }
   if (!islap && coef_maps) {
     if (!fill_coef_map(thee, g, ionstr)) return 0;
@@ -2009,20 +2112,35 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
       // fillcoCoefSpline (vpmg.c:4631), fillcoCoefSpline3 (:7525), fillcoCoefSpline4 (:7052)
       const int order = srfm_order(surfMeth);
       ensure_spline_coef(thee, order);
-      const double reach = std::max(irad + rmax + splineWin, rmax + splineWin) + 1.5 * hmax;
-      DevCells cells = build_cells(d->s, hpos.data(), d->natoms, reach);
-      pmg::coef_spline(d->s, g, A, cells.c, splineWin, irad, ionmask, epsw, epsp, d->kappa,
+      // the kernel's box: max(irad + r, r) + win + h/2 around the atom
+      std::vector<double> reach(std::max(d->natoms, 1));
+      for (int a = 0; a < d->natoms; a++)
+        reach[a] = std::max(irad + hrad[a] + splineWin, hrad[a] + splineWin) + 0.5 * hmax;
+      DevTiles tiles = build_tiles(d->s, g, hpos.data(), d->natoms, reach);
+      pmg::coef_spline(d->s, g, A, tiles.t, splineWin, irad, ionmask, epsw, epsp, d->kappa,
                        d->epsx, d->epsy, d->epsz, order, order == 2 ? nullptr : d->splcoef);
       HIPCHECK(hipStreamSynchronize(d->s));
-      cells.release();
+      tiles.release();
     } else {
-      // fillcoCoefMolDiel (vpmg.c:4335-4344) then fillcoCoefMolIon
+      // fillcoCoefMolDiel (vpmg.c:4335-4344) then fillcoCoefMolIon, as flag
+      // maps (byte per point): the atom spheres on the three shifted grids
+      // and, with salt, the ion-accessibility spheres on the unshifted one;
+      // the probe spheres at the SAS points, which re-mark the solvent
+      // value; then one pass forms the maps (and the harmonic mean for smol).
+      // Layout: [0, 3n) atom flags x, y, z; [3n, 4n) kappa; [4n, 7n) SAS.
       const double srad = pbe->solventRadius;
-      double* maps[3] = {d->epsx, d->epsy, d->epsz};
-      for (int w = 0; w < 3; w++) {
-        pmg::fill<double>(d->s, maps[w], d->n, epsw);
-        pmg::mark_spheres(d->s, g, d->natoms, d->apos, d->arad, srad, 0, w, epsp, maps[w]);
+      const bool ion_marks = !(ionstr < VPMGSMALL);
+      const size_t n = d->n;
+      const size_t fbytes = (7 * n + 3) & ~(size_t)3;
+      unsigned char* mflags = dalloc<unsigned char>(fbytes);
+      {
+        pmg::ZeroList z(d->s);
+        z.add(mflags, fbytes);
       }
+      pmg::MarkTargets T{};
+      for (int w = 0; w < 3; w++) T.t[T.nt++] = pmg::MarkTarget{w, 0, srad};
+      if (ion_marks) T.t[T.nt++] = pmg::MarkTarget{3, 1, irad};
+      pmg::mark_flags(d->s, g, d->natoms, d->apos, d->arad, T, mflags);
       if (srad > VSMALL) {
         // the accessibility object's reference sphere (vacc.c:146-189), read
         // from the Vpbe so a Vpbe built by the reference is honoured
@@ -2045,8 +2163,9 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
         double* sas = nullptr;
         int nsas = pmg::sas_compact(d->s, A, dref, nref, srad, flags, counter, &sas,
                                     [](size_t n) { return dalloc<double>(n); });
-        for (int w = 0; w < 3; w++)
-          pmg::mark_spheres(d->s, g, nsas, sas, nullptr, srad, 2, w, epsw, maps[w]);
+        pmg::MarkTargets S{};
+        for (int w = 0; w < 3; w++) S.t[S.nt++] = pmg::MarkTarget{w, 2, srad};
+        pmg::mark_flags(d->s, g, nsas, sas, nullptr, S, mflags + 4 * n);
         HIPCHECK(hipStreamSynchronize(d->s));
         dfree(flags);
         dfree(sas);
@@ -2054,22 +2173,11 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
         dfree(dref);
         cells.release();
       }
-      if (surfMeth == VSM_MOLSMOOTH) {
-        double* a1 = dalloc<double>(d->n);
-        double* a2 = dalloc<double>(d->n);
-        double* a3 = dalloc<double>(d->n);
-        HIPCHECK(hipMemcpyAsync(a1, d->epsx, d->n * sizeof(double), hipMemcpyDeviceToDevice, d->s));
-        HIPCHECK(hipMemcpyAsync(a2, d->epsy, d->n * sizeof(double), hipMemcpyDeviceToDevice, d->s));
-        HIPCHECK(hipMemcpyAsync(a3, d->epsz, d->n * sizeof(double), hipMemcpyDeviceToDevice, d->s));
-        pmg::mol_smooth(d->s, g, a1, a2, a3, d->epsx, d->epsy, d->epsz);
-        HIPCHECK(hipStreamSynchronize(d->s));
-        dfree(a1);
-        dfree(a2);
-        dfree(a3);
-      }
-      pmg::fill<double>(d->s, d->kappa, d->n, ionmask);
-      if (!(ionstr < VPMGSMALL))
-        pmg::mark_spheres(d->s, g, d->natoms, d->apos, d->arad, irad, 1, 3, 0.0, d->kappa);
+      pmg::mol_maps(d->s, g, mflags, srad > VSMALL ? mflags + 4 * n : nullptr,
+                    ion_marks ? mflags + 3 * n : nullptr, epsp, epsw, ionmask,
+                    surfMeth == VSM_MOLSMOOTH ? 1 : 0, d->epsx, d->epsy, d->epsz, d->kappa);
+      HIPCHECK(hipStreamSynchronize(d->s));
+      dfree(mflags);
     }
   } else {
     pmg::fill<double>(d->s, d->kappa, d->n, 0.0);
@@ -2774,6 +2882,16 @@ extern "C" int Vpmg_setPrecision(Vpmg* thee, int mode, double tol) {
 
 void pmgint::set_side_stream(Vpmg* pmg, bool on) {
   if (pmg) DEV(pmg)->side = on ? 1 : 0;
+}
+
+// Test hook: hierarchies this object allocates from now on have their
+// interiors filled with NaN before the boundary layers are cleared, so a
+// kernel that read an interior entry before writing it would spread NaN.
+extern "C" int pmgtest_poison_hierarchy(void* pmg, int on) {
+  Vpmg* thee = (Vpmg*)pmg;
+  if (!thee || !thee->dev) return 0;
+  DEV(thee)->poison = on != 0;
+  return 1;
 }
 
 extern "C" int Vpmg_setHostMirror(Vpmg* thee, int on) {

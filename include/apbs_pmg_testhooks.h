@@ -72,6 +72,11 @@ int pmgtest_stream_copy(size_t n, int reps);
  * launches; average microseconds per launch in *us_per_sweep.  The same
  * launches as the solver's fine level, for counter and timing runs. */
 int pmgtest_fused_sweeps(int n, int word, int sweeps, int tile, double *us_per_sweep);
+/* No reference counterpart: on != 0 makes the multigrid hierarchies a Vpmg
+ * (passed as void *) allocates from then on start with NaN in every array's
+ * interior (the boundary layers are cleared as usual), to show that every
+ * interior entry is written before it is read. */
+int pmgtest_poison_hierarchy(void *pmg, int on);
 #ifdef __cplusplus
 }
 #endif

@@ -3,9 +3,14 @@ test, buildBd.c:106-114, and its loop test, which a NaN norm would pass).
 
 A negative dielectric map makes every level's operator negative definite.
 Gauss-Seidel still runs normally on it (it is the same iteration for A and
--A), so every kernel launches and exits as usual; the dense inverse of the
-coarsest operator then meets a negative pivot, flags it, and Vpmg_solve
-returns 0 with a message.  The same object and a fresh one stay usable.
+-A), so every kernel launches and exits as usual.  In lpbe the dense inverse
+of the coarsest operator then meets a negative pivot, flags it, and
+Vpmg_solve returns 0 with that message.  In npbe the Newton Jacobian adds the
+ionic term N'(u) >= 0 to the diagonal; restricted to the coarse levels it
+outgrows the (negative) Galerkin stencil, so the coarsest operator can be
+positive definite while the fine iteration diverges: the solve then fails on
+the residual norm that is not finite.  Either way Vpmg_solve returns 0 with a
+message, and the same object and a fresh one stay usable.
 """
 import numpy as np
 import pytest
@@ -44,7 +49,10 @@ def test_negative_dielectric_is_reported(precision, npbe, capfd):
     with pytest.raises(RuntimeError):
         bad.solve()
     err = capfd.readouterr().err
-    assert "not positive definite" in err, err
+    if npbe:
+        assert "not positive definite" in err or "not finite" in err, err
+    else:
+        assert "not positive definite" in err, err
     # the device and the library stay usable: the same object solves again
     # after a valid fill is impossible (its maps are fixed), so a fresh one
     good = calc(20.0, precision, npbe)
