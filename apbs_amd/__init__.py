@@ -244,6 +244,8 @@ _SIGS = [
     ("Vpmg_setKernelTiming", ctypes.c_int, [ctypes.POINTER(Vpmg), ctypes.c_int]),
     ("Vpmg_getKernelTiming", ctypes.c_int, [ctypes.POINTER(Vpmg), _dp, ctypes.c_int]),
     ("Vpmg_energyNoAlloc", ctypes.c_int, [ctypes.POINTER(Vpmg), ctypes.c_int, _dp]),
+    ("Vpmg_solveBatch", ctypes.c_int,
+     [ctypes.POINTER(ctypes.POINTER(Vpmg)), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     ("apbs_pmg_synchronize", ctypes.c_int, []),
     ("PmgServer_ctor", ctypes.c_void_p, [ctypes.c_int, ctypes.c_size_t]),
     ("PmgServer_submit", ctypes.c_long, [ctypes.c_void_p, ctypes.POINTER(PmgRequest)]),

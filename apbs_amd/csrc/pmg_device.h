@@ -200,6 +200,88 @@ void buildA_fv(hipStream_t s, int nx, int ny, int nz, const double* xf, const do
 void galerkin(hipStream_t s, int st, int nxf, int nyf, int nzf, const Ops<double>& Af, int nxc,
               int nyc, int nzc, double* const ac[14], int form = 0);
 
+// ------------------------------------------------------------- batches
+// Up to kMaxBatch calculations of one grid shape share each launch of the
+// V-cycle kernels: a launch takes one argument set per calculation, and a
+// workgroup's calculation is its blockIdx.z (a flattened workgroup index for
+// the fused fine sweep, blockIdx.x or .y for the one-workgroup-per-matrix
+// kernels).  Every calculation's arithmetic is the same in a batch as alone;
+// the single-calculation wrappers above are batches of one.
+constexpr int kMaxBatch = 8;
+
+// One calculation's arrays on one level.  x is the solution (smoothers update
+// it), xs the partner buffer of the ping-pong sweeps (fused fine sweep and
+// plane-parity form), r the output of resid_batch.
+template <class T>
+struct LevelArgs {
+  Ops<T> A;
+  const T* D;
+  const T* f;
+  T* x;
+  T* xs;
+  T* r;
+};
+template <class T>
+void gsrb7_fused_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb,
+                       int nu, int iadjoint, int tile_shape = 0);
+template <class T>
+void gsrb27_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb, int nu,
+                  int iadjoint);
+template <class T>
+void gsrb27_pairs_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb,
+                        int nu, int iadjoint);
+template <class T>
+void gsrb27_planes_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb,
+                         int nu, int iadjoint, int tile = 0);
+template <class T>
+void resid_batch(hipStream_t s, int st, int nx, int ny, int nz, const LevelArgs<T>* c, int nb);
+// restriction: out = P^T in, zero cleared (optional); prolongation: out += damp[0] P in
+template <class T>
+struct TransferArgs {
+  const T* in;
+  T* out;
+  T* zero;
+  const double* damp;
+};
+template <class T>
+void restrict3_batch(hipStream_t s, int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
+                     const TransferArgs<T>* c, int nb);
+template <class T>
+void prolong_add_batch(hipStream_t s, int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
+                       const TransferArgs<T>* c, int nb);
+// damp_factor per calculation (part: its 2 * kParts partial sums)
+template <class T>
+struct DampArgs {
+  Ops<T> A;
+  const T* D;
+  const T* x;
+  const T* w0;
+  double* part;
+  double* out;
+};
+template <class T>
+void damp_factor_batch(hipStream_t s, int nx, int ny, int nz, const DampArgs<T>* c, int nb);
+// dense_solve per calculation (N interior unknowns, the same for all)
+template <class T>
+struct DenseArgs {
+  const double* Minv;
+  const T* f;
+  T* x;
+};
+template <class T>
+void dense_solve_batch(hipStream_t s, int nx, int ny, int nz, int N, const DenseArgs<T>* c,
+                       int nb);
+// dense_assemble then dense_invert per calculation (form and flag as there)
+struct InvertArgs {
+  Ops<double> A;
+  const double* D;
+  double* M;
+  double* work;
+  double* flag;
+};
+void dense_assemble_invert_batch(hipStream_t s, int nx, int ny, int nz, int N,
+                                 const InvertArgs* c, int nb, int form = 0);
+
 // ----------------------------------------------------- coarsest level (dense)
 // Assemble the dense (N x N) operator of the interior of a 27-point level with
 // diagonal D, then invert it in place (Gauss-Jordan, SPD, one workgroup).

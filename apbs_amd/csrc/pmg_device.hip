@@ -20,6 +20,13 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// Kernel-argument container of a batched launch: one argument set per
+// calculation (at most kMaxBatch).
+template <class P>
+struct Batch {
+  P v[kMaxBatch];
+};
+
 __host__ __device__ inline size_t IX(int i, int j, int k, int nx, int ny) {
   return ((size_t)k * ny + j) * nx + i;
 }
@@ -77,23 +84,24 @@ __global__ void __launch_bounds__(kBlock)
 constexpr int kFT = 256, kXCD = 8;
 
 // kTX columns (even), kTY rows and kKC planes per workgroup of kNT threads.
+// The arrays of one calculation in a fused-sweep launch.
+template <class T>
+struct SweepArgs {
+  const T *oE, *oN, *uC, *D, *fc, *xi;
+  T* xo;
+};
+
 template <class T, int kTY, int kKC, int kTX, int kNT>
-__global__ void __launch_bounds__(kNT)
-    k_gsrb7_fused(int nx, int ny, int nz, const T* __restrict__ oE, const T* __restrict__ oN,
-                  const T* __restrict__ uC, const T* __restrict__ D, const T* __restrict__ fc,
-                  const T* __restrict__ xi, T* __restrict__ xo, int first) {
+__device__ __forceinline__ void fused_body(int nx, int ny, int nz, int tile,
+                                           const T* __restrict__ oE, const T* __restrict__ oN,
+                                           const T* __restrict__ uC, const T* __restrict__ D,
+                                           const T* __restrict__ fc, const T* __restrict__ xi,
+                                           T* __restrict__ xo, int first) {
   constexpr int RX = kTX + 2, RY = kTY + 2, RN = RX * RY;
   constexpr int FPR = RX / 2, FN = FPR * RY;  // F points per region row / per plane
   __shared__ T ring[3][RN];
   const int tid = threadIdx.x;
-  // workgroups are dealt round-robin to the 8 XCDs: give XCD x the tiles
-  // [x*per, (x+1)*per) in z-major order, so tiles that share halo rows and
-  // planes share an L2
   const int nbx = (nx - 2 + kTX - 1) / kTX, nby = (ny - 2 + kTY - 1) / kTY;
-  const int nb = nbx * nby * ((nz - 2 + kKC - 1) / kKC);
-  const int per = (nb + kXCD - 1) / kXCD;
-  const int tile = (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
-  if (tile >= nb) return;
   const int i0 = 1 + (tile % nbx) * kTX, j0 = 1 + ((tile / nbx) % nby) * kTY;
   const int k0 = 1 + (tile / (nbx * nby)) * kKC;
   const int k1 = min(k0 + kKC, nz - 1);  // planes k0 .. k1-1
@@ -239,6 +247,23 @@ __global__ void __launch_bounds__(kNT)
   }
 }
 
+// Workgroups are dealt round-robin to the 8 XCDs: XCD x gets the (calculation,
+// tile) items [x*per, (x+1)*per) in calculation-major, z-major order, so tiles
+// that share halo rows and planes share an L2.
+template <class T, int kTY, int kKC, int kTX, int kNT>
+__global__ void __launch_bounds__(kNT)
+    k_gsrb7_fused(int nx, int ny, int nz, Batch<SweepArgs<T>> B, int nreq, int first) {
+  const int nbx = (nx - 2 + kTX - 1) / kTX, nby = (ny - 2 + kTY - 1) / kTY;
+  const int nb = nbx * nby * ((nz - 2 + kKC - 1) / kKC);
+  const int total = nb * nreq;
+  const int per = (total + kXCD - 1) / kXCD;
+  const int item = (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
+  if (item >= total) return;
+  const SweepArgs<T>& a = B.v[item / nb];
+  fused_body<T, kTY, kKC, kTX, kNT>(nx, ny, nz, item % nb, a.oE, a.oN, a.uC, a.D, a.fc, a.xi, a.xo,
+                                    first);
+}
+
 // ----------------------------------------------------------------- 27-point
 template <class T>
 struct P27 {
@@ -294,9 +319,9 @@ struct Ops27 {
 // Latency-bound on the small coarse levels: up to 128 VGPRs, so that all 54
 // operands of a point are in flight at once.
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-    k_gsrb27(int nx, int ny, int nz, Ops27<T> A, const T* __restrict__ D,
-             const T* __restrict__ fc, T* x, int c) {
+__device__ __forceinline__ void gsrb27_body(int nx, int ny, int nz, const Ops27<T>& A,
+                                            const T* __restrict__ D, const T* __restrict__ fc,
+                                            T* x, int c) {
   const int ci = c & 1, cj = (c >> 1) & 1, ck = (c >> 2) & 1;
   const int i0 = ci ? 1 : 2, j0 = cj ? 1 : 2, k0 = ck ? 1 : 2;
   const int ni = (nx - 2 - i0) / 2 + 1, nj = (ny - 2 - j0) / 2 + 1;
@@ -313,6 +338,24 @@ __global__ void __launch_bounds__(kBlock)
   x[p] = (fc[p] + (s.o + s.u + s.d)) / D[p];
 }
 
+// One calculation of a 27-point smoothing, residual or damping launch.
+template <class T>
+struct Args27 {
+  Ops27<T> A;
+  const T* D;
+  const T* f;
+  T* x;
+  T* y;  // plane form: the neighbour planes' values; residual: the output
+  T* z;  // plane form: the output
+};
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_gsrb27(int nx, int ny, int nz, Batch<Args27<T>> B, int c) {
+  const Args27<T>& a = B.v[blockIdx.z];
+  gsrb27_body<T>(nx, ny, nz, a.A, a.D, a.f, a.x, c);
+}
+
 // Two colours of that sweep in one launch: A = c and B = c ^ 1, which differ
 // only in the parity of i and follow each other in both sweep orders.  An
 // A point and a B point are neighbours only along i within one row, and the
@@ -326,9 +369,9 @@ __device__ inline T lane_value(T v, int src) {
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-    k_gsrb27_pair(int nx, int ny, int nz, Ops27<T> A, const T* __restrict__ D,
-                  const T* __restrict__ fc, T* x, int pa, int cj, int ck) {
+__device__ __forceinline__ void pair_body(int nx, int ny, int nz, const Ops27<T>& A,
+                                          const T* __restrict__ D, const T* __restrict__ fc, T* x,
+                                          int pa, int cj, int ck) {
   const T *oE = A.a[1], *oN = A.a[2], *uC = A.a[3], *oNE = A.a[4], *oNW = A.a[5], *uE = A.a[6],
           *uW = A.a[7], *uN = A.a[8], *uS = A.a[9], *uNE = A.a[10], *uNW = A.a[11],
           *uSE = A.a[12], *uSW = A.a[13];
@@ -388,6 +431,13 @@ __global__ void __launch_bounds__(kBlock)
   x[p] = (fc[p] + (o + u + d)) / D[p];
 }
 
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_gsrb27_pair(int nx, int ny, int nz, Batch<Args27<T>> B, int pa, int cj, int ck) {
+  const Args27<T>& a = B.v[blockIdx.z];
+  pair_body<T>(nx, ny, nz, a.A, a.D, a.f, a.x, pa, cj, ck);
+}
+
 // The four colours of one plane parity of that sweep in one launch: colours
 // 4ck .. 4ck+3 (4ck+3 .. 4ck for the adjoint sweep).  Planes of one parity do
 // not touch each other, so a workgroup owns a TX x TY tile of one plane.  It
@@ -399,10 +449,10 @@ __global__ void __launch_bounds__(kBlock)
 // from nbr, and the result goes to dst, so no workgroup reads what another
 // writes.  The arithmetic is part27's, term for term.
 template <class T, int TX, int TY, int NTH>
-__global__ void __launch_bounds__(NTH)
-    k_gs27_plane(int nx, int ny, int nz, Ops27<T> A, const T* __restrict__ D,
-                 const T* __restrict__ fc, const T* __restrict__ src, const T* __restrict__ nbr,
-                 T* __restrict__ dst, int ck, int adj) {
+__device__ __forceinline__ void plane_body(int nx, int ny, int nz, const Ops27<T>& A,
+                                           const T* __restrict__ D, const T* __restrict__ fc,
+                                           const T* __restrict__ src, const T* __restrict__ nbr,
+                                           T* __restrict__ dst, int ck, int adj) {
   constexpr int M = 4, LX = TX + 2 * M, LY = TY + 2 * M;
   __shared__ T sx[LX * LY];
   const T *oE = A.a[1], *oN = A.a[2], *uC = A.a[3], *oNE = A.a[4], *oNW = A.a[5], *uE = A.a[6],
@@ -473,11 +523,20 @@ __global__ void __launch_bounds__(NTH)
   }
 }
 
+// (x, y, z) of a calculation: the plane's own values, the neighbour planes'
+// values and the output.
+template <class T, int TX, int TY, int NTH>
+__global__ void __launch_bounds__(NTH)
+    k_gs27_plane(int nx, int ny, int nz, Batch<Args27<T>> B, int ck, int adj) {
+  const Args27<T>& a = B.v[blockIdx.z];
+  plane_body<T, TX, TY, NTH>(nx, ny, nz, a.A, a.D, a.f, a.x, a.y, a.z, ck, adj);
+}
+
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-    k_resid7(int nx, int ny, int nz, const T* __restrict__ oE, const T* __restrict__ oN,
-             const T* __restrict__ uC, const T* __restrict__ D, const T* __restrict__ fc,
-             const T* __restrict__ x, T* __restrict__ r) {
+__device__ __forceinline__ void resid7_body(int nx, int ny, int nz, const T* __restrict__ oE,
+                                            const T* __restrict__ oN, const T* __restrict__ uC,
+                                            const T* __restrict__ D, const T* __restrict__ fc,
+                                            const T* __restrict__ x, T* __restrict__ r) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   const int k = blockIdx.y + 1;
   const int i = 1 + t % (nx - 2), j = 1 + t / (nx - 2);
@@ -488,9 +547,15 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-    k_resid27(int nx, int ny, int nz, Ops27<T> A, const T* __restrict__ D,
-              const T* __restrict__ fc, const T* __restrict__ x, T* __restrict__ r) {
+__global__ void __launch_bounds__(kBlock) k_resid7(int nx, int ny, int nz, Batch<Args27<T>> B) {
+  const Args27<T>& a = B.v[blockIdx.z];
+  resid7_body<T>(nx, ny, nz, a.A.a[1], a.A.a[2], a.A.a[3], a.D, a.f, a.x, a.y);
+}
+
+template <class T>
+__device__ __forceinline__ void resid27_body(int nx, int ny, int nz, const Ops27<T>& A,
+                                             const T* __restrict__ D, const T* __restrict__ fc,
+                                             const T* __restrict__ x, T* __restrict__ r) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   const int k = blockIdx.y + 1;
   const int i = 1 + t % (nx - 2), j = 1 + t / (nx - 2);
@@ -499,6 +564,12 @@ __global__ void __launch_bounds__(kBlock)
   const size_t p = IX(i, j, k, nx, ny);
   P27<T> s = part27<T>(A.a, x, p, nx, nxy);
   r[p] = fc[p] + s.o + s.u + s.d - D[p] * x[p];
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_resid27(int nx, int ny, int nz, Batch<Args27<T>> B) {
+  const Args27<T>& a = B.v[blockIdx.z];
+  resid27_body<T>(nx, ny, nz, a.A, a.D, a.f, a.x, a.y);
 }
 
 // --------------------------------------------------------------- reductions
@@ -570,11 +641,20 @@ __global__ void __launch_bounds__(kBlock)
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// One calculation of a reduction's finishing launch.
+struct FinishArgs {
+  const double* part;
+  double* out;
+  double* hout;
+};
+
 // out[0] = sum(part[0..kParts-1]); ratio: out[0] = sum(a)/sum(b)
-// (and hout[0] = out[0] when hout is given)
-__global__ void __launch_bounds__(kBlock) k_finish(const double* part, double* out, int ratio,
-                                                   double* hout) {
+// (and hout[0] = out[0] when hout is given); calculation blockIdx.x
+__global__ void __launch_bounds__(kBlock) k_finish(Batch<FinishArgs> B, int ratio) {
   __shared__ double sh[kBlock];
+  const double* part = B.v[blockIdx.x].part;
+  double* out = B.v[blockIdx.x].out;
+  double* hout = B.v[blockIdx.x].hout;
   double a = 0.0, b = 0.0;
   for (int t = threadIdx.x; t < kParts; t += kBlock) {
     a += part[t];
@@ -596,10 +676,21 @@ __global__ void __launch_bounds__(kBlock) k_finish(const double* part, double* o
   }
 }
 
+// One calculation of a damping launch: the level's operator, the coarse
+// correction x and right-hand side w0, and its partial sums.
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-    k_damp_part(int nx, int ny, int nz, Ops27<T> A, const T* __restrict__ D,
-                const T* __restrict__ x, const T* __restrict__ w0, double* part) {
+struct DampPart {
+  Ops27<T> A;
+  const T* D;
+  const T* x;
+  const T* w0;
+  double* part;
+};
+
+template <class T>
+__device__ __forceinline__ void damp_body(int nx, int ny, int nz, const Ops27<T>& A,
+                                          const T* __restrict__ D, const T* __restrict__ x,
+                                          const T* __restrict__ w0, double* part) {
   __shared__ double sh[kBlock];
   const long ni = nx - 2, nij = ni * (ny - 2), n = nij * (nz - 2);
   const size_t nxy = (size_t)nx * ny;
@@ -620,12 +711,18 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_damp_part(int nx, int ny, int nz, Batch<DampPart<T>> B) {
+  const DampPart<T>& a = B.v[blockIdx.z];
+  damp_body<T>(nx, ny, nz, a.A, a.D, a.x, a.w0, a.part);
+}
+
 // ---------------------------------------------------------------- transfers
 // Restriction P^T (matvecd.c:801-913) with weights 1, 1/2, 1/4, 1/8.
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-    k_restrict(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc, const T* __restrict__ in,
-               T* __restrict__ out, T* __restrict__ zero) {
+__device__ __forceinline__ void restrict_body(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
+                                              const T* __restrict__ in, T* __restrict__ out,
+                                              T* __restrict__ zero) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   const int K = blockIdx.y + 1;
   const int I = 1 + t % (nxc - 2), J = 1 + t / (nxc - 2);
@@ -667,12 +764,19 @@ __global__ void __launch_bounds__(kBlock)
   if (zero) zero[IX(I, J, K, nxc, nyc)] = T(0);
 }
 
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_restrict(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc, Batch<TransferArgs<T>> B) {
+  const TransferArgs<T>& a = B.v[blockIdx.z];
+  restrict_body<T>(nxf, nyf, nzf, nxc, nyc, nzc, a.in, a.out, a.zero);
+}
+
 // Trilinear prolongation and correction x_f += damp * P x_c (matvecd.c:935-1076
 // followed by Vxaxpy, mgcsd.c:485-487).  Coarse boundary points carry weight 0.
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-    k_prolong_add(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf, const T* __restrict__ xc,
-                  T* __restrict__ xf, const double* __restrict__ damp) {
+__device__ __forceinline__ void prolong_body(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
+                                             const T* __restrict__ xc, T* __restrict__ xf,
+                                             const double* __restrict__ damp) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   const int k = blockIdx.y + 1;
   const int i = 1 + t % (nxf - 2), j = 1 + t / (nxf - 2);
@@ -726,6 +830,13 @@ __global__ void __launch_bounds__(kBlock)
   }
   const size_t p = IX(i, j, k, nxf, nyf);
   xf[p] = xf[p] + T(damp[0]) * v;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_prolong_add(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf, Batch<TransferArgs<T>> B) {
+  const TransferArgs<T>& a = B.v[blockIdx.z];
+  prolong_body<T>(nxc, nyc, nzc, nxf, nyf, nzf, a.in, a.out, a.damp);
 }
 
 // ------------------------------------------------------------- vector kit
@@ -1272,10 +1383,12 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ------------------------------------------------------------ dense coarse
-__global__ void k_dense_assemble(int nx, int ny, int nz, Ops27<double> A, const double* D,
-                                 double* M, int N) {
+__global__ void k_dense_assemble(int nx, int ny, int nz, Batch<InvertArgs> B, int N) {
   const int r = blockIdx.x * kBlock + threadIdx.x;
   if (r >= N) return;
+  const Ops<double>& A = B.v[blockIdx.y].A;
+  const double* D = B.v[blockIdx.y].D;
+  double* M = B.v[blockIdx.y].M;
   const int ni = nx - 2, nj = ny - 2;
   const int i = r % ni + 1, j = (r / ni) % nj + 1, k = r / (ni * nj) + 1;
   for (int c = 0; c < N; c++) M[(size_t)r * N + c] = 0.0;
@@ -1331,10 +1444,12 @@ __global__ void __launch_bounds__(kBlock) k_gj_elim(double* M, const double* col
 // x = Minv f reading Minv by columns (coalesced across rows); Minv is the
 // inverse of a symmetric operator.
 template <class T>
-__global__ void k_dense_solve_cols(int nx, int ny, int nz, const double* Minv, int N, const T* f,
-                                   T* x) {
+__global__ void k_dense_solve_cols(int nx, int ny, int nz, Batch<DenseArgs<T>> B, int N) {
   const int r = blockIdx.x * kBlock + threadIdx.x;
   if (r >= N) return;
+  const double* Minv = B.v[blockIdx.y].Minv;
+  const T* f = B.v[blockIdx.y].f;
+  T* x = B.v[blockIdx.y].x;
   const int ni = nx - 2, nj = ny - 2;
   double s = 0.0;
   for (int c = 0; c < N; c++) {
@@ -1351,7 +1466,9 @@ __global__ void k_dense_solve_cols(int nx, int ny, int nz, const double* Minv, i
 // with k_gj_elim's expression; each thread's (i, j) pairs are decoded once.
 constexpr int kInvPer = 8;  // elements per thread: 89 * 89 <= 8 * 1024
 
-__global__ void __launch_bounds__(1024) k_dense_invert_lds(double* Mg, int N, double* flag) {
+__global__ void __launch_bounds__(1024) k_dense_invert_lds(Batch<InvertArgs> B, int N) {
+  double* Mg = B.v[blockIdx.x].M;
+  double* flag = B.v[blockIdx.x].flag;
   extern __shared__ double sm[];
   double* M = sm;
   double* col = sm + (size_t)N * N;
@@ -1393,7 +1510,9 @@ __global__ void __launch_bounds__(1024) k_dense_invert_lds(double* Mg, int N, do
 // thread t holds 16 entries of row t / 4.  Each pivot step publishes the
 // pivot row and column through double-buffered LDS (one barrier per pivot)
 // and scales the pivot row by the pivot's reciprocal.
-__global__ void __launch_bounds__(256) k_dense_invert_reg(double* Mg, int N, double* flag) {
+__global__ void __launch_bounds__(256) k_dense_invert_reg(Batch<InvertArgs> B, int N) {
+  double* Mg = B.v[blockIdx.x].M;
+  double* flag = B.v[blockIdx.x].flag;
   __shared__ double colb[2][64], rowb[2][64];
   const int t = threadIdx.x, r = t >> 2, cb = t & 3, c0 = cb * 16;
   double m[16];
@@ -1436,9 +1555,12 @@ __global__ void __launch_bounds__(256) k_dense_invert_reg(double* Mg, int N, dou
 constexpr int kDenseSmall = 96;
 
 template <class T>
-__global__ void __launch_bounds__(128) k_dense_solve(int nx, int ny, int nz, const double* Minv,
-                                                     int N, const T* f, T* x) {
+__global__ void __launch_bounds__(128) k_dense_solve(int nx, int ny, int nz, Batch<DenseArgs<T>> B,
+                                                     int N) {
   __shared__ double fv[kDenseSmall];
+  const double* Minv = B.v[blockIdx.x].Minv;
+  const T* f = B.v[blockIdx.x].f;
+  T* x = B.v[blockIdx.x].x;
   const int ni = nx - 2, nj = ny - 2;
   for (int c = threadIdx.x; c < N; c += blockDim.x) {
     const int i = c % ni + 1, j = (c / ni) % nj + 1, k = c / (ni * nj) + 1;
@@ -1469,17 +1591,25 @@ Ops27<T> to27(const Ops<T>& A) {
   return o;
 }
 
+// A batched launch takes at most kMaxBatch calculations.
+inline void check_batch(int nb) {
+  if (nb < 1 || nb > kMaxBatch) {
+    fprintf(stderr, "pmg: a batched launch takes 1 to %d calculations, not %d\n", kMaxBatch, nb);
+    abort();
+  }
+}
+
 }  // namespace
 
 // =================================================================== wrappers
 
 template <class T, int TY, int KC, int TX, int NT = kFT>
-void fused_sweep(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
-                 const T* in, T* out, int first) {
-  const int nb = ((nx - 2 + TX - 1) / TX) * ((ny - 2 + TY - 1) / TY) * ((nz - 2 + KC - 1) / KC);
-  const unsigned g = (unsigned)(((nb + kXCD - 1) / kXCD) * kXCD);
-  k_gsrb7_fused<T, TY, KC, TX, NT><<<g, NT, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc,
-                                                    in, out, first);
+void fused_sweep(hipStream_t s, int nx, int ny, int nz, const Batch<SweepArgs<T>>& B, int nb,
+                 int first) {
+  const int tiles = ((nx - 2 + TX - 1) / TX) * ((ny - 2 + TY - 1) / TY) * ((nz - 2 + KC - 1) / KC);
+  const long total = (long)tiles * nb;
+  const unsigned g = (unsigned)(((total + kXCD - 1) / kXCD) * kXCD);
+  k_gsrb7_fused<T, TY, KC, TX, NT><<<g, NT, 0, s>>>(nx, ny, nz, B, nb, first);
 }
 
 static bool valid_tile(int v) {
@@ -1527,47 +1657,58 @@ int fused_width(int nx) {
 }
 
 template <class T, int TX>
-static void fused_sweep_w(hipStream_t s, int rp, int nx, int ny, int nz, const Ops<T>& A,
-                          const T* D, const T* fc, const T* in, T* out, int first) {
+static void fused_sweep_w(hipStream_t s, int rp, int nx, int ny, int nz,
+                          const Batch<SweepArgs<T>>& B, int nb, int first) {
   switch (rp) {
-    case 416: fused_sweep<T, 4, 16, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
-    case 48: fused_sweep<T, 4, 8, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
-    case 44: fused_sweep<T, 4, 4, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+    case 416: fused_sweep<T, 4, 16, TX>(s, nx, ny, nz, B, nb, first); break;
+    case 48: fused_sweep<T, 4, 8, TX>(s, nx, ny, nz, B, nb, first); break;
+    case 44: fused_sweep<T, 4, 4, TX>(s, nx, ny, nz, B, nb, first); break;
     // 512-thread workgroups: 8 or 12 rows with one F round and one S round
-    case 5088: fused_sweep<T, 8, 8, TX, 512>(s, nx, ny, nz, A, D, fc, in, out, first); break;
-    case 5084: fused_sweep<T, 8, 4, TX, 512>(s, nx, ny, nz, A, D, fc, in, out, first); break;
-    case 5816: fused_sweep<T, 8, 16, TX, 512>(s, nx, ny, nz, A, D, fc, in, out, first); break;
-    case 68: fused_sweep<T, 6, 8, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
-    case 88: fused_sweep<T, 8, 8, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
-    case 84: fused_sweep<T, 8, 4, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
-    case 1616: fused_sweep<T, 16, 16, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
-    case 832: fused_sweep<T, 8, 32, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
-    default: fused_sweep<T, 8, 16, TX>(s, nx, ny, nz, A, D, fc, in, out, first); break;
+    case 5088: fused_sweep<T, 8, 8, TX, 512>(s, nx, ny, nz, B, nb, first); break;
+    case 5084: fused_sweep<T, 8, 4, TX, 512>(s, nx, ny, nz, B, nb, first); break;
+    case 5816: fused_sweep<T, 8, 16, TX, 512>(s, nx, ny, nz, B, nb, first); break;
+    case 68: fused_sweep<T, 6, 8, TX>(s, nx, ny, nz, B, nb, first); break;
+    case 88: fused_sweep<T, 8, 8, TX>(s, nx, ny, nz, B, nb, first); break;
+    case 84: fused_sweep<T, 8, 4, TX>(s, nx, ny, nz, B, nb, first); break;
+    case 1616: fused_sweep<T, 16, 16, TX>(s, nx, ny, nz, B, nb, first); break;
+    case 832: fused_sweep<T, 8, 32, TX>(s, nx, ny, nz, B, nb, first); break;
+    default: fused_sweep<T, 8, 16, TX>(s, nx, ny, nz, B, nb, first); break;
   }
 }
 
 template <class T>
-void gsrb7_fused(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
-                 T* x, T* tmp, int nu, int iadjoint, int tile_shape) {
+void gsrb7_fused_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb,
+                       int nu, int iadjoint, int tile_shape) {
+  check_batch(nb);
   const int first = iadjoint ? 0 : 1;
-  T* in = x;
-  T* out = tmp;
   const int tile =
       tile_shape > 0 && valid_tile(tile_shape) ? tile_shape : fused_tile(nx, ny, nz);
   const int rp = tile % 10000;
   const int w = tile / 10000 ? tile / 10000 : fused_width(nx);
   for (int it = 0; it < nu; it++) {
+    // sweep it reads x (even it) or xs and writes the other one
+    Batch<SweepArgs<T>> B;
+    for (int b = 0; b < nb; b++) {
+      const LevelArgs<T>& a = c[b];
+      B.v[b] = SweepArgs<T>{a.A.a[1], a.A.a[2], a.A.a[3], a.D, a.f,
+                            (it & 1) ? a.xs : a.x, (it & 1) ? a.x : a.xs};
+    }
     if (w == 80)
-      fused_sweep_w<T, 80>(s, rp, nx, ny, nz, A, D, fc, in, out, first);
+      fused_sweep_w<T, 80>(s, rp, nx, ny, nz, B, nb, first);
     else if (w == 96)
-      fused_sweep_w<T, 96>(s, rp, nx, ny, nz, A, D, fc, in, out, first);
+      fused_sweep_w<T, 96>(s, rp, nx, ny, nz, B, nb, first);
     else
-      fused_sweep_w<T, 64>(s, rp, nx, ny, nz, A, D, fc, in, out, first);
-    T* t = in;
-    in = out;
-    out = t;
+      fused_sweep_w<T, 64>(s, rp, nx, ny, nz, B, nb, first);
   }
-  if (in != x) copy_interior<T>(s, nx, ny, nz, in, x);
+  if (nu & 1)
+    for (int b = 0; b < nb; b++) copy_interior<T>(s, nx, ny, nz, c[b].xs, c[b].x);
+}
+
+template <class T>
+void gsrb7_fused(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
+                 T* x, T* tmp, int nu, int iadjoint, int tile_shape) {
+  const LevelArgs<T> c{A, D, fc, x, tmp, nullptr};
+  gsrb7_fused_batch<T>(s, nx, ny, nz, &c, 1, nu, iadjoint, tile_shape);
 }
 
 int gsrb7_fused_launches(int nu) { return nu + (nu & 1); }
@@ -1589,35 +1730,59 @@ void gsrb7(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, c
 int gsrb7_launches(int nu) { return 2 * nu; }
 
 template <class T>
-void gsrb27(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc, T* x,
-            int nu, int iadjoint) {
-  const Ops27<T> o = to27(A);
+static Batch<Args27<T>> args27(const LevelArgs<T>* c, int nb) {
+  Batch<Args27<T>> B;
+  for (int b = 0; b < nb; b++)
+    B.v[b] = Args27<T>{to27(c[b].A), c[b].D, c[b].f, c[b].x, c[b].r, nullptr};
+  return B;
+}
+
+template <class T>
+void gsrb27_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb, int nu,
+                  int iadjoint) {
+  check_batch(nb);
+  const Batch<Args27<T>> B = args27(c, nb);
   for (int it = 0; it < nu; it++)
     for (int cc = 0; cc < 8; cc++) {
-      const int c = iadjoint ? 7 - cc : cc;
-      const int i0 = (c & 1) ? 1 : 2, j0 = ((c >> 1) & 1) ? 1 : 2, k0 = ((c >> 2) & 1) ? 1 : 2;
+      const int col = iadjoint ? 7 - cc : cc;
+      const int i0 = (col & 1) ? 1 : 2, j0 = ((col >> 1) & 1) ? 1 : 2, k0 = ((col >> 2) & 1) ? 1 : 2;
       const int ni = (nx - 2 - i0) / 2 + 1, nj = (ny - 2 - j0) / 2 + 1, nk = (nz - 2 - k0) / 2 + 1;
       if (ni <= 0 || nj <= 0 || nk <= 0) continue;
-      dim3 g((unsigned)(((long)ni * nj + kBlock - 1) / kBlock), (unsigned)nk);
-      k_gsrb27<T><<<g, kBlock, 0, s>>>(nx, ny, nz, o, D, fc, x, c);
+      dim3 g((unsigned)(((long)ni * nj + kBlock - 1) / kBlock), (unsigned)nk, (unsigned)nb);
+      k_gsrb27<T><<<g, kBlock, 0, s>>>(nx, ny, nz, B, col);
+    }
+}
+
+template <class T>
+void gsrb27(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc, T* x,
+            int nu, int iadjoint) {
+  const LevelArgs<T> c{A, D, fc, x, nullptr, nullptr};
+  gsrb27_batch<T>(s, nx, ny, nz, &c, 1, nu, iadjoint);
+}
+
+template <class T>
+void gsrb27_pairs_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb,
+                        int nu, int iadjoint) {
+  check_batch(nb);
+  const Batch<Args27<T>> B = args27(c, nb);
+  for (int it = 0; it < nu; it++)
+    for (int h = 0; h < 4; h++) {
+      // forward: (0,1) (2,3) (4,5) (6,7); adjoint: (7,6) (5,4) (3,2) (1,0)
+      const int col = iadjoint ? 7 - 2 * h : 2 * h;
+      const int pa = col & 1, cj = (col >> 1) & 1, ck = (col >> 2) & 1;
+      const int j0 = cj ? 1 : 2, k0 = ck ? 1 : 2;
+      if (j0 > ny - 2 || k0 > nz - 2) continue;
+      const long rows = (long)((ny - 2 - j0) / 2 + 1) * ((nz - 2 - k0) / 2 + 1);
+      const dim3 g((unsigned)((rows * 64 + kBlock - 1) / kBlock), 1, (unsigned)nb);
+      k_gsrb27_pair<T><<<g, kBlock, 0, s>>>(nx, ny, nz, B, pa, cj, ck);
     }
 }
 
 template <class T>
 void gsrb27_pairs(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D,
                   const T* fc, T* x, int nu, int iadjoint) {
-  const Ops27<T> o = to27(A);
-  for (int it = 0; it < nu; it++)
-    for (int h = 0; h < 4; h++) {
-      // forward: (0,1) (2,3) (4,5) (6,7); adjoint: (7,6) (5,4) (3,2) (1,0)
-      const int c = iadjoint ? 7 - 2 * h : 2 * h;
-      const int pa = c & 1, cj = (c >> 1) & 1, ck = (c >> 2) & 1;
-      const int j0 = cj ? 1 : 2, k0 = ck ? 1 : 2;
-      if (j0 > ny - 2 || k0 > nz - 2) continue;
-      const long rows = (long)((ny - 2 - j0) / 2 + 1) * ((nz - 2 - k0) / 2 + 1);
-      const unsigned g = (unsigned)((rows * 64 + kBlock - 1) / kBlock);
-      k_gsrb27_pair<T><<<g, kBlock, 0, s>>>(nx, ny, nz, o, D, fc, x, pa, cj, ck);
-    }
+  const LevelArgs<T> c{A, D, fc, x, nullptr, nullptr};
+  gsrb27_pairs_batch<T>(s, nx, ny, nz, &c, 1, nu, iadjoint);
 }
 
 // Workgroups of a plane-parity launch with TX x TY tiles.
@@ -1632,38 +1797,44 @@ int gsrb27_planes_tile(int nx, int ny, int nz) {
 }
 
 template <class T>
-void gsrb27_planes(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D,
-                   const T* fc, T* x, T* xs, int nu, int iadjoint, int tile) {
+void gsrb27_planes_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb,
+                         int nu, int iadjoint, int tile) {
+  check_batch(nb);
   if (nx < 3 || ny < 3 || nz < 3) return;
-  const Ops27<T> o = to27(A);
   if (tile != 8 && tile != 16 && tile != 32) tile = gsrb27_planes_tile(nx, ny, nz);
-  T* in = x;
-  T* out = xs;
   for (int it = 0; it < nu; it++) {
+    // sweep it reads x (even it) or xs and writes the other one
     const int first = iadjoint ? 1 : 0;
     for (int h = 0; h < 2; h++) {
       const int ck = h == 0 ? first : 1 - first;
       const int k0 = ck ? 1 : 2;
       if (k0 > nz - 2) continue;
       const int nk = (nz - 2 - k0) / 2 + 1;
-      const T* nb = h == 0 ? in : out;
+      Batch<Args27<T>> B;
+      for (int b = 0; b < nb; b++) {
+        T* in = (it & 1) ? c[b].xs : c[b].x;
+        T* out = (it & 1) ? c[b].x : c[b].xs;
+        B.v[b] = Args27<T>{to27(c[b].A), c[b].D, c[b].f, in, h == 0 ? in : out, out};
+      }
       const unsigned tiles = (unsigned)(((nx - 2 + tile - 1) / tile) * ((ny - 2 + tile - 1) / tile));
-      const dim3 g(tiles, (unsigned)nk);
+      const dim3 g(tiles, (unsigned)nk, (unsigned)nb);
       if (tile == 32)
-        k_gs27_plane<T, 32, 32, kBlock><<<g, kBlock, 0, s>>>(nx, ny, nz, o, D, fc, in, nb, out,
-                                                             ck, iadjoint);
+        k_gs27_plane<T, 32, 32, kBlock><<<g, kBlock, 0, s>>>(nx, ny, nz, B, ck, iadjoint);
       else if (tile == 16)
-        k_gs27_plane<T, 16, 16, kBlock><<<g, kBlock, 0, s>>>(nx, ny, nz, o, D, fc, in, nb, out,
-                                                             ck, iadjoint);
+        k_gs27_plane<T, 16, 16, kBlock><<<g, kBlock, 0, s>>>(nx, ny, nz, B, ck, iadjoint);
       else
-        k_gs27_plane<T, 8, 8, kBlock><<<g, kBlock, 0, s>>>(nx, ny, nz, o, D, fc, in, nb, out, ck,
-                                                           iadjoint);
+        k_gs27_plane<T, 8, 8, kBlock><<<g, kBlock, 0, s>>>(nx, ny, nz, B, ck, iadjoint);
     }
-    T* t = in;
-    in = out;
-    out = t;
   }
-  if (in != x) copy_interior<T>(s, nx, ny, nz, in, x);
+  if (nu & 1)
+    for (int b = 0; b < nb; b++) copy_interior<T>(s, nx, ny, nz, c[b].xs, c[b].x);
+}
+
+template <class T>
+void gsrb27_planes(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D,
+                   const T* fc, T* x, T* xs, int nu, int iadjoint, int tile) {
+  const LevelArgs<T> c{A, D, fc, x, xs, nullptr};
+  gsrb27_planes_batch<T>(s, nx, ny, nz, &c, 1, nu, iadjoint, tile);
 }
 
 bool gsrb27_use_planes(int nx, int ny, int nz) { return nx <= 49 && ny <= 49 && nz >= 3; }
@@ -1672,13 +1843,22 @@ bool gsrb27_use_pairs(int nx) { return nx >= 3 && nx <= 130; }
 
 
 template <class T>
+void resid_batch(hipStream_t s, int st, int nx, int ny, int nz, const LevelArgs<T>* c, int nb) {
+  check_batch(nb);
+  dim3 g = grid_interior(nx, ny, nz);
+  g.z = (unsigned)nb;
+  const Batch<Args27<T>> B = args27(c, nb);
+  if (st == 7)
+    k_resid7<T><<<g, kBlock, 0, s>>>(nx, ny, nz, B);
+  else
+    k_resid27<T><<<g, kBlock, 0, s>>>(nx, ny, nz, B);
+}
+
+template <class T>
 void resid(hipStream_t s, int st, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
            const T* x, T* r) {
-  dim3 g = grid_interior(nx, ny, nz);
-  if (st == 7)
-    k_resid7<T><<<g, kBlock, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, x, r);
-  else
-    k_resid27<T><<<g, kBlock, 0, s>>>(nx, ny, nz, to27(A), D, fc, x, r);
+  const LevelArgs<T> c{A, D, fc, const_cast<T*>(x), nullptr, r};
+  resid_batch<T>(s, st, nx, ny, nz, &c, 1);
 }
 
 void nresid7(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* cc,
@@ -1693,31 +1873,61 @@ void dcvec(hipStream_t s, size_t n, const double* cc, const double* x, double* o
 }
 
 template <class T>
+void restrict3_batch(hipStream_t s, int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
+                     const TransferArgs<T>* c, int nb) {
+  check_batch(nb);
+  Batch<TransferArgs<T>> B;
+  for (int b = 0; b < nb; b++) B.v[b] = c[b];
+  dim3 g = grid_interior(nxc, nyc, nzc);
+  g.z = (unsigned)nb;
+  k_restrict<T><<<g, kBlock, 0, s>>>(nxf, nyf, nzf, nxc, nyc, nzc, B);
+}
+
+template <class T>
 void restrict3(hipStream_t s, int nxf, int nyf, int nzf, int nxc, int nyc, int nzc, const T* in,
                T* out, T* zero) {
-  k_restrict<T><<<grid_interior(nxc, nyc, nzc), kBlock, 0, s>>>(nxf, nyf, nzf, nxc, nyc, nzc, in,
-                                                                out, zero);
+  const TransferArgs<T> c{in, out, zero, nullptr};
+  restrict3_batch<T>(s, nxf, nyf, nzf, nxc, nyc, nzc, &c, 1);
+}
+
+template <class T>
+void prolong_add_batch(hipStream_t s, int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
+                       const TransferArgs<T>* c, int nb) {
+  check_batch(nb);
+  Batch<TransferArgs<T>> B;
+  for (int b = 0; b < nb; b++) B.v[b] = c[b];
+  dim3 g = grid_interior(nxf, nyf, nzf);
+  g.z = (unsigned)nb;
+  k_prolong_add<T><<<g, kBlock, 0, s>>>(nxc, nyc, nzc, nxf, nyf, nzf, B);
 }
 
 template <class T>
 void prolong_add(hipStream_t s, int nxc, int nyc, int nzc, int nxf, int nyf, int nzf, const T* xc,
                  T* xf, const double* damp) {
-  k_prolong_add<T><<<grid_interior(nxf, nyf, nzf), kBlock, 0, s>>>(nxc, nyc, nzc, nxf, nyf, nzf,
-                                                                   xc, xf, damp);
+  const TransferArgs<T> c{xc, xf, nullptr, damp};
+  prolong_add_batch<T>(s, nxc, nyc, nzc, nxf, nyf, nzf, &c, 1);
 }
 
 template <class T>
 void nrm1(hipStream_t s, int nx, int ny, int nz, const T* x, double* part, double* out,
           double* hout) {
   k_nrm1_part<T><<<kParts, kBlock, 0, s>>>(nx, ny, nz, x, part);
-  k_finish<<<1, kBlock, 0, s>>>(part, out, 0, hout);
+  {
+    Batch<FinishArgs> F;
+    F.v[0] = FinishArgs{part, out, hout};
+    k_finish<<<1, kBlock, 0, s>>>(F, 0);
+  }
 }
 
 template <class T>
 void resid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
                  const T* x, T* r, double* part, double* out, double* hout) {
   k_resid7_nrm<T><<<kParts, kBlock, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, x, r, part);
-  k_finish<<<1, kBlock, 0, s>>>(part, out, 0, hout);
+  {
+    Batch<FinishArgs> F;
+    F.v[0] = FinishArgs{part, out, hout};
+    k_finish<<<1, kBlock, 0, s>>>(F, 0);
+  }
 }
 
 void nresid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* cc,
@@ -1725,14 +1935,31 @@ void nresid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, c
                   double* out, double* hout) {
   k_nresid7_nrm<<<kParts, kBlock, 0, s>>>(nx, ny, nz, A.a[0], A.a[1], A.a[2], A.a[3], cc, fc, x, r,
                                           ions, part);
-  k_finish<<<1, kBlock, 0, s>>>(part, out, 0, hout);
+  {
+    Batch<FinishArgs> F;
+    F.v[0] = FinishArgs{part, out, hout};
+    k_finish<<<1, kBlock, 0, s>>>(F, 0);
+  }
+}
+
+template <class T>
+void damp_factor_batch(hipStream_t s, int nx, int ny, int nz, const DampArgs<T>* c, int nb) {
+  check_batch(nb);
+  Batch<DampPart<T>> B;
+  Batch<FinishArgs> F;
+  for (int b = 0; b < nb; b++) {
+    B.v[b] = DampPart<T>{to27(c[b].A), c[b].D, c[b].x, c[b].w0, c[b].part};
+    F.v[b] = FinishArgs{c[b].part, c[b].out, nullptr};
+  }
+  k_damp_part<T><<<dim3(kParts, 1, nb), kBlock, 0, s>>>(nx, ny, nz, B);
+  k_finish<<<nb, kBlock, 0, s>>>(F, 1);
 }
 
 template <class T>
 void damp_factor(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* x,
                  const T* w0, double* part, double* out) {
-  k_damp_part<T><<<kParts, kBlock, 0, s>>>(nx, ny, nz, to27(A), D, x, w0, part);
-  k_finish<<<1, kBlock, 0, s>>>(part, out, 1, nullptr);
+  const DampArgs<T> c{A, D, x, w0, part, out};
+  damp_factor_batch<T>(s, nx, ny, nz, &c, 1);
 }
 
 template <class T>
@@ -1870,19 +2097,48 @@ void galerkin(hipStream_t s, int st, int nxf, int nyf, int nzf, const Ops<double
   }
 }
 
-void dense_assemble(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* D,
-                    double* M, int N) {
-  k_dense_assemble<<<(N + kBlock - 1) / kBlock, kBlock, 0, s>>>(nx, ny, nz, to27(A), D, M, N);
-}
-
-void dense_invert(hipStream_t s, double* M, double* work, int N, int form, double* flag) {
+void dense_assemble_invert_batch(hipStream_t s, int nx, int ny, int nz, int N,
+                                 const InvertArgs* c, int nb, int form) {
+  check_batch(nb);
+  Batch<InvertArgs> B;
+  for (int b = 0; b < nb; b++) B.v[b] = c[b];
+  k_dense_assemble<<<dim3((N + kBlock - 1) / kBlock, nb), kBlock, 0, s>>>(nx, ny, nz, B, N);
   if (N <= 64 && form != 1) {
-    k_dense_invert_reg<<<1, 256, 0, s>>>(M, N, flag);
+    k_dense_invert_reg<<<nb, 256, 0, s>>>(B, N);
     return;
   }
   const size_t lds = ((size_t)N * N + 2 * N) * sizeof(double);
   if (lds <= 64 * 1024 && (size_t)N * N <= (size_t)kInvPer * 1024) {
-    k_dense_invert_lds<<<1, 1024, lds, s>>>(M, N, flag);
+    k_dense_invert_lds<<<nb, 1024, lds, s>>>(B, N);
+    return;
+  }
+  const unsigned gn = (unsigned)((N + kBlock - 1) / kBlock);
+  const unsigned gnn = (unsigned)(((size_t)N * N + kBlock - 1) / kBlock);
+  for (int b = 0; b < nb; b++)
+    for (int k = 0; k < N; k++) {
+      k_gj_col<<<gn, kBlock, 0, s>>>(c[b].M, c[b].work, N, k);
+      k_gj_row<<<gn, kBlock, 0, s>>>(c[b].M, c[b].work, N, k, c[b].flag);
+      k_gj_elim<<<gnn, kBlock, 0, s>>>(c[b].M, c[b].work, N, k);
+    }
+}
+
+void dense_assemble(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* D,
+                    double* M, int N) {
+  Batch<InvertArgs> B;
+  B.v[0] = InvertArgs{A, D, M, nullptr, nullptr};
+  k_dense_assemble<<<dim3((N + kBlock - 1) / kBlock, 1), kBlock, 0, s>>>(nx, ny, nz, B, N);
+}
+
+void dense_invert(hipStream_t s, double* M, double* work, int N, int form, double* flag) {
+  Batch<InvertArgs> B;
+  B.v[0] = InvertArgs{Ops<double>{}, nullptr, M, work, flag};
+  if (N <= 64 && form != 1) {
+    k_dense_invert_reg<<<1, 256, 0, s>>>(B, N);
+    return;
+  }
+  const size_t lds = ((size_t)N * N + 2 * N) * sizeof(double);
+  if (lds <= 64 * 1024 && (size_t)N * N <= (size_t)kInvPer * 1024) {
+    k_dense_invert_lds<<<1, 1024, lds, s>>>(B, N);
     return;
   }
   const unsigned gn = (unsigned)((N + kBlock - 1) / kBlock);
@@ -1895,12 +2151,22 @@ void dense_invert(hipStream_t s, double* M, double* work, int N, int form, doubl
 }
 
 template <class T>
+void dense_solve_batch(hipStream_t s, int nx, int ny, int nz, int N, const DenseArgs<T>* c,
+                       int nb) {
+  check_batch(nb);
+  Batch<DenseArgs<T>> B;
+  for (int b = 0; b < nb; b++) B.v[b] = c[b];
+  if (((size_t)N * N + N) * sizeof(double) <= 64 * 1024 && N <= kDenseSmall)
+    k_dense_solve<T><<<nb, 128, 0, s>>>(nx, ny, nz, B, N);
+  else
+    k_dense_solve_cols<T><<<dim3((N + kBlock - 1) / kBlock, nb), kBlock, 0, s>>>(nx, ny, nz, B, N);
+}
+
+template <class T>
 void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int N, const T* f,
                  T* x) {
-  if (((size_t)N * N + N) * sizeof(double) <= 64 * 1024 && N <= kDenseSmall)
-    k_dense_solve<T><<<1, 128, 0, s>>>(nx, ny, nz, Minv, N, f, x);
-  else
-    k_dense_solve_cols<T><<<(N + kBlock - 1) / kBlock, kBlock, 0, s>>>(nx, ny, nz, Minv, N, f, x);
+  const DenseArgs<T> c{Minv, f, x};
+  dense_solve_batch<T>(s, nx, ny, nz, N, &c, 1);
 }
 
 // ------------------------------------------------------ explicit instances
@@ -1929,7 +2195,21 @@ void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int 
   template void add_diag<T>(hipStream_t, size_t, const T*, const T*, T*, float*);               \
   template void dense_solve<T>(hipStream_t, int, int, int, const double*, int, const T*, T*);  \
   template void gsrb7_fused<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*, T*, \
-                               T*, int, int, int);
+                               T*, int, int, int);                                              \
+  template void gsrb7_fused_batch<T>(hipStream_t, int, int, int, const LevelArgs<T>*, int, int,  \
+                                     int, int);                                                 \
+  template void gsrb27_batch<T>(hipStream_t, int, int, int, const LevelArgs<T>*, int, int, int); \
+  template void gsrb27_pairs_batch<T>(hipStream_t, int, int, int, const LevelArgs<T>*, int, int, \
+                                      int);                                                     \
+  template void gsrb27_planes_batch<T>(hipStream_t, int, int, int, const LevelArgs<T>*, int,     \
+                                       int, int, int);                                          \
+  template void resid_batch<T>(hipStream_t, int, int, int, int, const LevelArgs<T>*, int);       \
+  template void restrict3_batch<T>(hipStream_t, int, int, int, int, int, int,                     \
+                                   const TransferArgs<T>*, int);                                \
+  template void prolong_add_batch<T>(hipStream_t, int, int, int, int, int, int,                   \
+                                     const TransferArgs<T>*, int);                              \
+  template void damp_factor_batch<T>(hipStream_t, int, int, int, const DampArgs<T>*, int);       \
+  template void dense_solve_batch<T>(hipStream_t, int, int, int, int, const DenseArgs<T>*, int);
 INST(double)
 INST(float)
 #undef INST

@@ -1224,9 +1224,8 @@ void build_operators(PmgDev* d) {
 }
 
 // D_l = oC_l + c_l for the given per-level Helmholtz arrays (and, for a
-// mixed-precision solve, its FP32 image Df_l in the same pass), then the
-// dense inverse of the coarsest level.
-void set_diagonals(PmgDev* d, bool use_cprime) {
+// mixed-precision solve, its FP32 image Df_l in the same pass).
+void diag_levels(PmgDev* d, bool use_cprime) {
   const bool f32 = d->precision == 1 && d->h_mixed;
   pmg::DiagLevels dl{};
   dl.nl = d->ntot;
@@ -1240,6 +1239,16 @@ void set_diagonals(PmgDev* d, bool use_cprime) {
     dl.off[l + 1] = dl.off[l] + L.n;
   }
   pmg::add_diag_levels(d->s, dl);
+}
+
+pmg::InvertArgs invert_args(PmgDev* d) {
+  const DLevel& C = d->L[d->ntot - 1];
+  return pmg::InvertArgs{C.ops(), C.D, d->dense, d->dwork, d->hscal + kPivotFlag};
+}
+
+// The diagonals, then the dense inverse of the coarsest level.
+void set_diagonals(PmgDev* d, bool use_cprime) {
+  diag_levels(d, use_cprime);
   const DLevel& C = d->L[d->ntot - 1];
   // The inverse is first read by the coarsest solve, after the downward
   // half of the next V-cycle; a one-workgroup kernel, it runs on its own
@@ -1255,12 +1264,28 @@ void set_diagonals(PmgDev* d, bool use_cprime) {
     HIPCHECK(hipStreamWaitEvent(d->cs, d->cs_fork, 0));
     s = d->cs;
   }
-  pmg::dense_assemble(s, C.nx, C.ny, C.nz, C.ops(), C.D, d->dense, d->dn);
-  pmg::dense_invert(s, d->dense, d->dwork, d->dn, 0, d->hscal + kPivotFlag);
+  const pmg::InvertArgs ia = invert_args(d);
+  pmg::dense_assemble_invert_batch(s, C.nx, C.ny, C.nz, d->dn, &ia, 1);
   if (s != d->s) {
     HIPCHECK(hipEventRecord(d->cs_join, s));
     d->cs_pending = true;
   }
+}
+
+// set_diagonals for n calculations of one shape on the first one's stream:
+// the dense inverses of all n in one launch.
+void set_diagonals_batch(PmgDev* const* ds, int n, bool use_cprime) {
+  if (n == 1) {
+    set_diagonals(ds[0], use_cprime);
+    return;
+  }
+  pmg::InvertArgs ia[pmg::kMaxBatch];
+  for (int b = 0; b < n; b++) {
+    diag_levels(ds[b], use_cprime);
+    ia[b] = invert_args(ds[b]);
+  }
+  const DLevel& C = ds[0]->L[ds[0]->ntot - 1];
+  pmg::dense_assemble_invert_batch(ds[0]->s, C.nx, C.ny, C.nz, ds[0]->dn, ia, n);
 }
 
 // Join the dense inverse's stream before the first read of the inverse.
@@ -1290,7 +1315,7 @@ void sync_fp32_ops(PmgDev* d) {
 // ------------------------------------------------------------------ V-cycle
 // One V-cycle of Vmvcs (mgcsd.c:315-523) on per-level arrays.  X[l] are the
 // per-level solutions (X[0] fine), f the fine right-hand side, w1 a fine-size
-// scratch buffer, part/scal reduction scratch.
+// scratch buffer (the residual of every level on the way down).
 template <class T>
 struct CycleArgs {
   T* X[24];
@@ -1303,77 +1328,114 @@ struct CycleArgs {
   T* xs;  // fine-size smoother partner buffer
 };
 
+// The V-cycle of n calculations of one shape (n <= pmg::kMaxBatch), all on
+// the stream of the first: every stage is one launch over the batch, so the
+// coarse levels' small launches serve all n calculations.  Each calculation's
+// arithmetic is its own single V-cycle's.
 template <class T>
-void vcycle(PmgDev* d, CycleArgs<T>& c) {
-  hipStream_t s = d->s;
-  const int nt = d->ntot;
-  DLevel* L = d->L;
-  d->kt[4] = sizeof(T);
-  d->kt[5] = (double)L[0].n;
-  // algorithmic words per fine point per smoother launch: 7 for a full sweep
-  d->kt[6] = d->smoother == 0 ? 7.0 : 3.5;
+void vcycle_batch(PmgDev* const* ds, CycleArgs<T>* cs, int n) {
+  PmgDev* d0 = ds[0];
+  hipStream_t s = d0->s;
+  const int nt = d0->ntot;
+  DLevel* L = d0->L;  // level shapes (the same for every calculation)
+  if (n == 1) {
+    d0->kt[4] = sizeof(T);
+    d0->kt[5] = (double)L[0].n;
+    // algorithmic words per fine point per smoother launch: 7 for a full sweep
+    d0->kt[6] = d0->smoother == 0 ? 7.0 : 3.5;
+  }
+  pmg::LevelArgs<T> la[pmg::kMaxBatch];
+  pmg::TransferArgs<T> ta[pmg::kMaxBatch];
+  // the level-l arguments of every calculation (with_r: the residual goes to w1)
+  auto level = [&](int l, bool with_r) {
+    for (int b = 0; b < n; b++)
+      la[b] = pmg::LevelArgs<T>{cs[b].A[l], cs[b].Dl[l], l == 0 ? cs[b].f : cs[b].w0[l],
+                                cs[b].X[l], l == 0 ? cs[b].xs : cs[b].xsl[l],
+                                with_r ? cs[b].w1 : nullptr};
+  };
   auto fine_smooth = [&](int adj) {
-    const int e0 = tbegin(d);
-    if (d->smoother == 0) {
-      pmg::gsrb7_fused<T>(s, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], c.xs, 2,
-                          adj);
-      tend(d, e0, 0, pmg::gsrb7_fused_launches(2));
+    const int e0 = n == 1 ? tbegin(d0) : -1;
+    if (d0->smoother == 0) {
+      level(0, false);
+      pmg::gsrb7_fused_batch<T>(s, L[0].nx, L[0].ny, L[0].nz, la, n, 2, adj);
+      if (n == 1) tend(d0, e0, 0, pmg::gsrb7_fused_launches(2));
     } else {
-      pmg::gsrb7<T>(s, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], 2, adj);
-      tend(d, e0, 0, pmg::gsrb7_launches(2));
+      for (int b = 0; b < n; b++)
+        pmg::gsrb7<T>(s, L[0].nx, L[0].ny, L[0].nz, cs[b].A[0], cs[b].Dl[0], cs[b].f, cs[b].X[0],
+                      2, adj);
+      if (n == 1) tend(d0, e0, 0, pmg::gsrb7_launches(2));
     }
   };
   auto coarse_smooth = [&](int l, int adj) {
-    if (d->smoother27 == 0 && pmg::gsrb27_use_planes(L[l].nx, L[l].ny, L[l].nz))
-      pmg::gsrb27_planes<T>(s, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l],
-                            c.xsl[l], 2, adj);
-    else if (d->smoother27 == 0 && pmg::gsrb27_use_pairs(L[l].nx))
-      pmg::gsrb27_pairs<T>(s, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l], 2,
-                           adj);
+    level(l, false);
+    if (d0->smoother27 == 0 && pmg::gsrb27_use_planes(L[l].nx, L[l].ny, L[l].nz))
+      pmg::gsrb27_planes_batch<T>(s, L[l].nx, L[l].ny, L[l].nz, la, n, 2, adj);
+    else if (d0->smoother27 == 0 && pmg::gsrb27_use_pairs(L[l].nx))
+      pmg::gsrb27_pairs_batch<T>(s, L[l].nx, L[l].ny, L[l].nz, la, n, 2, adj);
     else
-      pmg::gsrb27<T>(s, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l], 2, adj);
+      pmg::gsrb27_batch<T>(s, L[l].nx, L[l].ny, L[l].nz, la, n, 2, adj);
   };
-  const int ecyc = tbegin(d);
+  const int ecyc = n == 1 ? tbegin(d0) : -1;
   fine_smooth(0);
-  pmg::resid<T>(s, 7, L[0].nx, L[0].ny, L[0].nz, c.A[0], c.Dl[0], c.f, c.X[0], c.w1);
+  level(0, true);
+  pmg::resid_batch<T>(s, 7, L[0].nx, L[0].ny, L[0].nz, la, n);
   for (int l = 1; l < nt; l++) {
     // the coarse solution starts at zero (interior cleared with the
     // restriction; the boundary layer is never written)
-    pmg::restrict3<T>(s, L[l - 1].nx, L[l - 1].ny, L[l - 1].nz, L[l].nx, L[l].ny, L[l].nz, c.w1,
-                      c.w0[l], l != nt - 1 ? c.X[l] : nullptr);
+    for (int b = 0; b < n; b++)
+      ta[b] = pmg::TransferArgs<T>{cs[b].w1, cs[b].w0[l], l != nt - 1 ? cs[b].X[l] : nullptr,
+                                   nullptr};
+    pmg::restrict3_batch<T>(s, L[l - 1].nx, L[l - 1].ny, L[l - 1].nz, L[l].nx, L[l].ny, L[l].nz,
+                            ta, n);
     if (l != nt - 1) {
       coarse_smooth(l, 0);
-      pmg::resid<T>(s, 27, L[l].nx, L[l].ny, L[l].nz, c.A[l], c.Dl[l], c.w0[l], c.X[l], c.w1);
+      level(l, true);
+      pmg::resid_batch<T>(s, 27, L[l].nx, L[l].ny, L[l].nz, la, n);
     }
   }
   const DLevel& C = L[nt - 1];
-  join_dense(d);
-  pmg::dense_solve<T>(s, C.nx, C.ny, C.nz, d->dense, d->dn, c.w0[nt - 1], c.X[nt - 1]);
+  pmg::DenseArgs<T> da[pmg::kMaxBatch];
+  for (int b = 0; b < n; b++) {
+    join_dense(ds[b]);
+    da[b] = pmg::DenseArgs<T>{ds[b]->dense, cs[b].w0[nt - 1], cs[b].X[nt - 1]};
+  }
+  pmg::dense_solve_batch<T>(s, C.nx, C.ny, C.nz, d0->dn, da, n);
+  pmg::DampArgs<T> pa[pmg::kMaxBatch];
   for (int l = nt - 2; l >= 0; l--) {
     const DLevel& Cl = L[l + 1];
-    pmg::damp_factor<T>(s, Cl.nx, Cl.ny, Cl.nz, c.A[l + 1], c.Dl[l + 1], c.X[l + 1], c.w0[l + 1],
-                        d->part, d->scal + 4);
-    pmg::prolong_add<T>(s, Cl.nx, Cl.ny, Cl.nz, L[l].nx, L[l].ny, L[l].nz, c.X[l + 1], c.X[l],
-                        d->scal + 4);
+    for (int b = 0; b < n; b++) {
+      pa[b] = pmg::DampArgs<T>{cs[b].A[l + 1], cs[b].Dl[l + 1], cs[b].X[l + 1], cs[b].w0[l + 1],
+                               ds[b]->part, ds[b]->scal + 4};
+      ta[b] = pmg::TransferArgs<T>{cs[b].X[l + 1], cs[b].X[l], nullptr, ds[b]->scal + 4};
+    }
+    pmg::damp_factor_batch<T>(s, Cl.nx, Cl.ny, Cl.nz, pa, n);
+    pmg::prolong_add_batch<T>(s, Cl.nx, Cl.ny, Cl.nz, L[l].nx, L[l].ny, L[l].nz, ta, n);
     if (l == 0)
       fine_smooth(1);
     else
       coarse_smooth(l, 1);
   }
-  tend(d, ecyc, 1, 1);
+  if (n == 1) tend(d0, ecyc, 1, 1);
+}
+
+template <class T>
+void vcycle(PmgDev* d, CycleArgs<T>& c) {
+  vcycle_batch<T>(&d, &c, 1);
 }
 
 // One V-cycle: replayed from a captured graph, or launched eagerly while
 // kernel timing is on (its events are host-side bookkeeping).  The cycle's
 // kernels depend only on device data, so the graph of a given (precision,
 // solution, right-hand side) is valid for every later cycle on this
-// hierarchy.
+// hierarchy.  Batches of several calculations are launched eagerly.
 template <class T>
-void run_vcycle(PmgDev* d, CycleArgs<T>& c) {
-  if (!d->graphs || d->timing) {
-    vcycle<T>(d, c);
+void run_vcycle(PmgDev* const* ds, CycleArgs<T>* cs, int n) {
+  PmgDev* d = ds[0];
+  if (n > 1 || !d->graphs || d->timing) {
+    vcycle_batch<T>(ds, cs, n);
     return;
   }
+  CycleArgs<T>& c = cs[0];
   for (auto& cg : d->cycle_graphs)
     if (cg.word == (int)sizeof(T) && cg.f == (const void*)c.f && cg.x == (const void*)c.X[0]) {
       HIPCHECK(hipGraphLaunch(cg.e, d->s));
@@ -1406,47 +1468,7 @@ CycleArgs<double> args64(PmgDev* d, double* const* X, const double* f) {
   return c;
 }
 
-// Vmvcs with istop 0 (absolute) or 1 (relative to ||f||1); returns cycles,
-// or -1 when the solve failed (solve_failed)
-int mvcs64(PmgDev* d, double* const* X, const double* f, int istop, int itmax, double errtol,
-           double* relres) {
-  const DLevel& F = d->L[0];
-  double rsden = 1.0;
-  if (istop == 1) {
-    pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, f, d->part, d->scal, d->hscal);
-    rsden = read_norm(d);
-    if (solve_failed(d, rsden)) return -1;
-  }
-  if (rsden == 0.0) rsden = 1.0;
-  CycleArgs<double> c = args64(d, X, f);
-  int iters = 0;
-  double rsnrm = rsden;
-  do {
-    run_vcycle<double>(d, c);
-    iters++;
-    pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, c.A[0], c.Dl[0], f, X[0], nullptr, d->part,
-                             d->scal, d->hscal);
-    rsnrm = read_norm(d);
-    if (solve_failed(d, rsnrm)) return -1;
-  } while (iters < itmax && (rsnrm / rsden) > errtol);
-  if (relres) *relres = rsnrm / rsden;
-  return iters;
-}
-
-// Mixed precision: FP32 V-cycles on the correction equation inside an FP64
-// defect-correction loop (fine residual, update and stopping test in FP64).
-// Returns the outer iterations, or -1 when the solve failed (solve_failed).
-int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int itmax,
-                double errtol, double* relres, int* outer, bool x_zero) {
-  const DLevel& F = d->L[0];
-  double rsden = 1.0, fnrm = -1.0;
-  if (istop == 1 || x_zero) {
-    pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, f, d->part, d->scal, d->hscal);
-    fnrm = read_norm(d);
-    if (solve_failed(d, fnrm)) return -1;
-    if (istop == 1) rsden = fnrm;
-  }
-  if (rsden == 0.0) rsden = 1.0;
+CycleArgs<float> args32(PmgDev* d) {
   CycleArgs<float> c;
   for (int l = 0; l < d->ntot; l++) {
     c.X[l] = d->L[l].xfl;
@@ -1459,37 +1481,163 @@ int mixed_solve(PmgDev* d, double* const* X64, const double* f, int istop, int i
   c.xs = d->xs32;
   // r (FP64) -> FP32 rhs stored in L[0].w0f
   c.f = d->L[0].w0f;
-  const pmg::Ops<double> A0 = F.ops();
-  int it = 0;
-  double rsnrm;
+  return c;
+}
+
+// Wait for the shared stream and check each active calculation's norm (read
+// from its host-mapped scalar); a failure (solve_failed) deactivates it.
+void read_norms(PmgDev* const* ds, int n, const bool* active, double* out, bool* failed) {
+  HIPCHECK(hipStreamSynchronize(ds[0]->s));
+  for (int b = 0; b < n; b++) {
+    if (!active[b]) continue;
+    out[b] = ds[b]->hscal[0];
+    if (solve_failed(ds[b], out[b])) failed[b] = true;
+  }
+}
+
+// Vmvcs with istop 0 (absolute) or 1 (relative to ||f||1) for n calculations
+// in lockstep: each runs V-cycles until its own stopping test holds.  its[b]
+// gets the cycles, or -1 when that solve failed (solve_failed).
+void mvcs64_batch(PmgDev* const* ds, int n, double* const* const* X, const double* const* f,
+                  int istop, int itmax, const double* errtol, double* relres, int* its) {
+  bool active[pmg::kMaxBatch], failed[pmg::kMaxBatch] = {};
+  double rsden[pmg::kMaxBatch], rsnrm[pmg::kMaxBatch];
+  for (int b = 0; b < n; b++) {
+    active[b] = true;
+    rsden[b] = 1.0;
+    its[b] = 0;
+  }
+  if (istop == 1) {
+    for (int b = 0; b < n; b++) {
+      const DLevel& F = ds[b]->L[0];
+      pmg::nrm1<double>(ds[b]->s, F.nx, F.ny, F.nz, f[b], ds[b]->part, ds[b]->scal,
+                        ds[b]->hscal);
+    }
+    read_norms(ds, n, active, rsden, failed);
+  }
+  for (int b = 0; b < n; b++) {
+    if (failed[b]) active[b] = false;
+    if (rsden[b] == 0.0) rsden[b] = 1.0;
+    rsnrm[b] = rsden[b];
+  }
+  CycleArgs<double> cs[pmg::kMaxBatch];
+  PmgDev* act[pmg::kMaxBatch];
+  for (;;) {
+    int m = 0;
+    for (int b = 0; b < n; b++)
+      if (active[b]) {
+        cs[m] = args64(ds[b], X[b], f[b]);
+        act[m++] = ds[b];
+      }
+    if (m == 0) break;
+    run_vcycle<double>(act, cs, m);
+    for (int b = 0; b < n; b++) {
+      if (!active[b]) continue;
+      its[b]++;
+      const DLevel& F = ds[b]->L[0];
+      pmg::resid7_nrm1<double>(ds[b]->s, F.nx, F.ny, F.nz, F.ops(), F.D, f[b], X[b][0], nullptr,
+                               ds[b]->part, ds[b]->scal, ds[b]->hscal);
+    }
+    read_norms(ds, n, active, rsnrm, failed);
+    for (int b = 0; b < n; b++) {
+      if (!active[b]) continue;
+      if (failed[b] || its[b] >= itmax || !((rsnrm[b] / rsden[b]) > errtol[b])) active[b] = false;
+    }
+  }
+  for (int b = 0; b < n; b++) {
+    if (failed[b]) {
+      its[b] = -1;
+      continue;
+    }
+    if (relres) relres[b] = rsnrm[b] / rsden[b];
+  }
+}
+
+// Mixed precision for n calculations in lockstep: FP32 V-cycles on the
+// correction equation inside an FP64 defect-correction loop (fine residual,
+// update and stopping test in FP64).  outer[b] gets the outer iterations, or
+// -1 when that solve failed (solve_failed).
+void mixed_batch(PmgDev* const* ds, int n, double* const* const* X64, const double* const* f,
+                 int istop, int itmax, const double* errtol, double* relres, int* outer,
+                 bool x_zero) {
+  bool active[pmg::kMaxBatch], failed[pmg::kMaxBatch] = {};
+  double rsden[pmg::kMaxBatch], fnrm[pmg::kMaxBatch], rsnrm[pmg::kMaxBatch];
+  const double* r[pmg::kMaxBatch];
+  int it[pmg::kMaxBatch];
+  for (int b = 0; b < n; b++) {
+    active[b] = true;
+    rsden[b] = 1.0;
+    fnrm[b] = -1.0;
+    it[b] = 0;
+  }
+  if (istop == 1 || x_zero) {
+    for (int b = 0; b < n; b++) {
+      const DLevel& F = ds[b]->L[0];
+      pmg::nrm1<double>(ds[b]->s, F.nx, F.ny, F.nz, f[b], ds[b]->part, ds[b]->scal,
+                        ds[b]->hscal);
+    }
+    read_norms(ds, n, active, fnrm, failed);
+    for (int b = 0; b < n; b++)
+      if (istop == 1) rsden[b] = fnrm[b];
+  }
+  for (int b = 0; b < n; b++) {
+    if (failed[b]) active[b] = false;
+    if (rsden[b] == 0.0) rsden[b] = 1.0;
+  }
   // the first residual: f itself when the solution starts at zero (f + A 0
   // rounds to f), else computed
-  const double* r = d->w1;
-  if (x_zero) {
-    r = f;
-    rsnrm = fnrm;
-  } else {
-    pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, A0, F.D, f, X64[0], d->w1, d->part,
-                             d->scal, d->hscal);
-    rsnrm = read_norm(d);
-    if (solve_failed(d, rsnrm)) return -1;
+  for (int b = 0; b < n; b++) {
+    r[b] = ds[b]->w1;
+    if (x_zero) {
+      r[b] = f[b];
+      rsnrm[b] = fnrm[b];
+    } else if (active[b]) {
+      const DLevel& F = ds[b]->L[0];
+      pmg::resid7_nrm1<double>(ds[b]->s, F.nx, F.ny, F.nz, F.ops(), F.D, f[b], X64[b][0],
+                               ds[b]->w1, ds[b]->part, ds[b]->scal, ds[b]->hscal);
+    }
   }
-  while (it < itmax && (rsnrm / rsden) > errtol) {
-    pmg::convert<double, float>(d->s, F.n, r, d->L[0].w0f);
-    r = d->w1;
-    pmg::fill<float>(d->s, d->L[0].xfl, F.n, 0.0f);
-    run_vcycle<float>(d, c);
-    // X64 += (double) e, in one pass
-    pmg::axpy_interior_f32(d->s, F.nx, F.ny, F.nz, 1.0, d->L[0].xfl, X64[0]);
-    it++;
-    pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, A0, F.D, f, X64[0], d->w1, d->part,
-                             d->scal, d->hscal);
-    rsnrm = read_norm(d);
-    if (solve_failed(d, rsnrm)) return -1;
+  if (!x_zero) read_norms(ds, n, active, rsnrm, failed);
+  CycleArgs<float> cs[pmg::kMaxBatch];
+  PmgDev* act[pmg::kMaxBatch];
+  for (;;) {
+    int m = 0;
+    for (int b = 0; b < n; b++) {
+      if (!active[b]) continue;
+      if (failed[b] || !(it[b] < itmax && (rsnrm[b] / rsden[b]) > errtol[b])) {
+        active[b] = false;
+        continue;
+      }
+      PmgDev* d = ds[b];
+      const DLevel& F = d->L[0];
+      pmg::convert<double, float>(d->s, F.n, r[b], d->L[0].w0f);
+      r[b] = d->w1;
+      pmg::fill<float>(d->s, d->L[0].xfl, F.n, 0.0f);
+      cs[m] = args32(d);
+      act[m++] = d;
+    }
+    if (m == 0) break;
+    run_vcycle<float>(act, cs, m);
+    for (int b = 0; b < n; b++) {
+      if (!active[b]) continue;
+      PmgDev* d = ds[b];
+      const DLevel& F = d->L[0];
+      // X64 += (double) e, in one pass
+      pmg::axpy_interior_f32(d->s, F.nx, F.ny, F.nz, 1.0, d->L[0].xfl, X64[b][0]);
+      it[b]++;
+      pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, F.ops(), F.D, f[b], X64[b][0], d->w1,
+                               d->part, d->scal, d->hscal);
+    }
+    read_norms(ds, n, active, rsnrm, failed);
   }
-  if (relres) *relres = rsnrm / rsden;
-  if (outer) *outer = it;
-  return it;
+  for (int b = 0; b < n; b++) {
+    if (failed[b]) {
+      outer[b] = -1;
+      continue;
+    }
+    if (relres) relres[b] = rsnrm[b] / rsden[b];
+    outer[b] = it[b];
+  }
 }
 
 pmg::Ions ions_scaled(Vpbe* pbe) {
@@ -1503,101 +1651,213 @@ pmg::Ions ions_scaled(Vpbe* pbe) {
   return I;
 }
 
-// Vnewton (newtond.c:162-548) with Vgetjac (:550-598); false when the solve
-// failed (solve_failed)
-bool newton_solve(Vpmg* thee, double* relres_out, int* iters_out, int* inner_out) {
-  PmgDev* d = DEV(thee);
-  DLevel* L = d->L;
-  const DLevel& F = L[0];
-  const int nx = F.nx, ny = F.ny, nz = F.nz;
-  const pmg::Ions I = ions_scaled(thee->pbe);
-  const pmg::Ops<double> A0 = F.ops();
-  hipStream_t s = d->s;
-  // rsden = ||F(0)||1
-  pmg::fill<double>(s, d->wn1, d->n, 0.0);
-  pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->wn2, I, d->part, d->scal, d->hscal);
-  double rsden = read_norm(d);
-  if (solve_failed(d, rsden)) return false;
-  if (rsden == 0.0) rsden = 1.0;
-  pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->u, d->F, I, d->part, d->scal, d->hscal);
-  double xnorm_old = read_norm(d);
-  if (solve_failed(d, xnorm_old)) return false;
-  double xnorm_new = xnorm_old, xnorm_med = 0.0, rsnrm = rsden;
-  int idamp = 1, iters = 0, inner = 0;
-  const double errtol = thee->pmgp->errtol;
-  const int itmax = thee->pmgp->itmax;
-  double* X[24];
+// Per-calculation state of Vnewton (newtond.c:162-548) in a lockstep batch.
+struct NewtonState {
+  pmg::Ions I;
+  double rsden = 1, xnorm_old = 0, xnorm_new = 0, xnorm_med = 0, rsnrm = 0, damp = 1;
+  double errtol = 0;
+  int itmax = 0;
+  int idamp = 1, iters = 0, inner = 0, iter_d = 0, mode = 0;
+  bool active = true, searching = false, failed = false;
+};
+
+// Vnewton with Vgetjac (:550-598) for n calculations of one shape in lockstep:
+// each follows its own damped inexact Newton path (its own step count, line
+// search and stopping test); the Jacobian inverses and the V-cycles of the
+// calculations still iterating are batched.  ok[b] is false when that solve
+// failed (solve_failed).
+void newton_batch(Vpmg* const* th, int n, double* relres, int* iters, int* inner, bool* ok) {
+  PmgDev* ds[pmg::kMaxBatch];
+  NewtonState st[pmg::kMaxBatch];
+  bool act[pmg::kMaxBatch];
+  double nv[pmg::kMaxBatch];
+  bool bad[pmg::kMaxBatch] = {};
+  for (int b = 0; b < n; b++) {
+    ds[b] = DEV(th[b]);
+    st[b].I = ions_scaled(th[b]->pbe);
+    st[b].errtol = th[b]->pmgp->errtol;
+    st[b].itmax = th[b]->pmgp->itmax;
+  }
+  auto nresid = [&](int b, const double* x, double* r) {
+    PmgDev* d = ds[b];
+    const DLevel& F = d->L[0];
+    pmg::nresid7_nrm1(d->s, F.nx, F.ny, F.nz, F.ops(), F.cc, F.fc, x, r, st[b].I, d->part, d->scal,
+                      d->hscal);
+  };
+  auto read = [&](bool NewtonState::*flag) {
+    for (int b = 0; b < n; b++) act[b] = st[b].*flag && !st[b].failed;
+    read_norms(ds, n, act, nv, bad);
+    for (int b = 0; b < n; b++)
+      if (bad[b]) st[b].failed = true;
+  };
+  // rsden = ||F(0)||1, then ||F(u)||1
+  for (int b = 0; b < n; b++) {
+    pmg::fill<double>(ds[b]->s, ds[b]->wn1, ds[b]->n, 0.0);
+    nresid(b, ds[b]->wn1, ds[b]->wn2);
+  }
+  read(&NewtonState::active);
+  for (int b = 0; b < n; b++) {
+    st[b].rsden = nv[b] == 0.0 ? 1.0 : nv[b];
+    nresid(b, ds[b]->u, ds[b]->F);
+  }
+  read(&NewtonState::active);
+  for (int b = 0; b < n; b++) {
+    st[b].xnorm_old = st[b].xnorm_new = nv[b];
+    st[b].rsnrm = st[b].rsden;
+    if (st[b].failed) st[b].active = false;
+  }
+  double* X[pmg::kMaxBatch][24];
   for (;;) {
-    iters++;
+    int m = 0;
+    PmgDev* ad[pmg::kMaxBatch];
+    int idx[pmg::kMaxBatch];
+    for (int b = 0; b < n; b++)
+      if (st[b].active) {
+        ad[m] = ds[b];
+        idx[m++] = b;
+      }
+    if (m == 0) break;
     // Jacobian diagonal on all levels
-    pmg::dcvec(s, d->n, F.cc, d->u, L[0].cprime, I);
-    for (int l = 1; l < d->ntot; l++)
-      pmg::restrict3<double>(s, L[l - 1].nx, L[l - 1].ny, L[l - 1].nz, L[l].nx, L[l].ny, L[l].nz,
-                             L[l - 1].cprime, L[l].cprime);
-    set_diagonals(d, true);
-    const double errtol_s = std::min(0.9 * xnorm_old, 1.0 * pow(xnorm_old, 2.0));
-    // the V-cycle clears the coarse corrections itself; the fine one starts
-    // at zero
-    for (int l = 0; l < d->ntot; l++) X[l] = L[l].xtmp;
-    pmg::fill<double>(s, X[0], L[0].n, 0.0);
-    double rr = 0;
-    int its;
-    if (d->precision == 1) {
-      if (iters == 1) sync_fp32_ops(d);
-      int outer = 0;
-      its = mixed_solve(d, X, d->F, 0, 1000, errtol_s, &rr, &outer, true) < 0 ? -1 : outer;
-    } else {
-      its = mvcs64(d, X, d->F, 0, 1000, errtol_s, &rr);
+    for (int q = 0; q < m; q++) {
+      PmgDev* d = ad[q];
+      DLevel* L = d->L;
+      st[idx[q]].iters++;
+      pmg::dcvec(d->s, d->n, L[0].cc, d->u, L[0].cprime, st[idx[q]].I);
+      for (int l = 1; l < d->ntot; l++)
+        pmg::restrict3<double>(d->s, L[l - 1].nx, L[l - 1].ny, L[l - 1].nz, L[l].nx, L[l].ny,
+                               L[l].nz, L[l - 1].cprime, L[l].cprime);
     }
-    if (its < 0) return false;
-    inner += its;
-    double* xtmp = L[0].xtmp;
-    if (idamp == 1) {
-      pmg::xpay_interior(s, nx, ny, nz, d->u, 1.0, xtmp, d->wn1);
-      pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->F, I, d->part, d->scal, d->hscal);
-      xnorm_new = read_norm(d);
-      if (solve_failed(d, xnorm_new)) return false;
-      double damp = 1.0;
-      int iter_d = 0, mode = 0;
-      while (iter_d < 10) {
-        if (mode == 0) {
-          if (xnorm_new < xnorm_old) mode = 1;
-        } else if (xnorm_new > xnorm_med) {
-          break;
+    set_diagonals_batch(ad, m, true);
+    // inner solve: the V-cycle clears the coarse corrections itself; the
+    // fine one starts at zero
+    double errs[pmg::kMaxBatch], rr[pmg::kMaxBatch];
+    int its[pmg::kMaxBatch];
+    double* const* Xp[pmg::kMaxBatch];
+    const double* fp[pmg::kMaxBatch];
+    for (int q = 0; q < m; q++) {
+      PmgDev* d = ad[q];
+      NewtonState& S = st[idx[q]];
+      errs[q] = std::min(0.9 * S.xnorm_old, 1.0 * pow(S.xnorm_old, 2.0));
+      for (int l = 0; l < d->ntot; l++) X[q][l] = d->L[l].xtmp;
+      pmg::fill<double>(d->s, X[q][0], d->L[0].n, 0.0);
+      Xp[q] = X[q];
+      fp[q] = d->F;
+      if (d->precision == 1 && S.iters == 1) sync_fp32_ops(d);
+    }
+    // calculations of one batch share the precision (batches are formed by
+    // shape and precision); each has its own inner tolerance
+    if (ad[0]->precision == 1)
+      mixed_batch(ad, m, Xp, fp, 0, 1000, errs, rr, its, true);
+    else
+      mvcs64_batch(ad, m, Xp, fp, 0, 1000, errs, rr, its);
+    for (int q = 0; q < m; q++) {
+      NewtonState& S = st[idx[q]];
+      if (its[q] < 0) {
+        S.failed = true;
+        S.active = false;
+        continue;
+      }
+      S.inner += its[q];
+    }
+    // first trial point (damped) or the full step
+    for (int q = 0; q < m; q++) {
+      const int b = idx[q];
+      NewtonState& S = st[b];
+      if (!S.active) continue;
+      PmgDev* d = ds[b];
+      const DLevel& F = d->L[0];
+      double* xtmp = F.xtmp;
+      if (S.idamp == 1) {
+        pmg::xpay_interior(d->s, F.nx, F.ny, F.nz, d->u, 1.0, xtmp, d->wn1);
+        nresid(b, d->wn1, d->F);
+      } else {
+        pmg::axpy_interior<double>(d->s, F.nx, F.ny, F.nz, 1.0, xtmp, d->u);
+        nresid(b, d->u, d->F);
+      }
+    }
+    read(&NewtonState::active);
+    for (int q = 0; q < m; q++) {
+      NewtonState& S = st[idx[q]];
+      if (!S.active) continue;
+      if (S.failed) {
+        S.active = false;
+        continue;
+      }
+      S.xnorm_new = nv[idx[q]];
+      if (S.idamp == 1) {
+        S.damp = 1.0;
+        S.iter_d = 0;
+        S.mode = 0;
+        S.searching = true;
+      } else {
+        S.xnorm_old = S.xnorm_new;
+      }
+    }
+    // the damping line search, one trial per round for every calculation
+    // still searching
+    for (;;) {
+      bool any = false;
+      for (int q = 0; q < m; q++) {
+        const int b = idx[q];
+        NewtonState& S = st[b];
+        if (!S.searching) continue;
+        bool stop = S.iter_d >= 10;
+        if (!stop) {
+          if (S.mode == 0) {
+            if (S.xnorm_new < S.xnorm_old) S.mode = 1;
+          } else if (S.xnorm_new > S.xnorm_med) {
+            stop = true;
+          }
+        }
+        PmgDev* d = ds[b];
+        const DLevel& F = d->L[0];
+        if (stop) {
+          pmg::copy_interior<double>(d->s, F.nx, F.ny, F.nz, d->wn2, d->u);
+          std::swap(d->F, d->wn3);
+          S.xnorm_new = S.xnorm_med;
+          S.xnorm_old = S.xnorm_new;
+          if ((S.iter_d - 1) == 0) S.idamp = 0;
+          S.searching = false;
+          continue;
         }
         // keep the candidate and its residual (wn2, wn3) by exchanging the
         // buffers: every one of them has a zero boundary layer and is written
         // on the interior only
         std::swap(d->wn1, d->wn2);
         std::swap(d->F, d->wn3);
-        xnorm_med = xnorm_new;
-        damp = damp / 2.0;
-        pmg::xpay_interior(s, nx, ny, nz, d->u, damp, xtmp, d->wn1);
-        pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->wn1, d->F, I, d->part, d->scal, d->hscal);
-        xnorm_new = read_norm(d);
-        if (solve_failed(d, xnorm_new)) return false;
-        iter_d++;
+        S.xnorm_med = S.xnorm_new;
+        S.damp = S.damp / 2.0;
+        pmg::xpay_interior(d->s, F.nx, F.ny, F.nz, d->u, S.damp, F.xtmp, d->wn1);
+        nresid(b, d->wn1, d->F);
+        any = true;
       }
-      pmg::copy_interior<double>(s, nx, ny, nz, d->wn2, d->u);
-      std::swap(d->F, d->wn3);
-      xnorm_new = xnorm_med;
-      xnorm_old = xnorm_new;
-      if ((iter_d - 1) == 0) idamp = 0;
-    } else {
-      pmg::axpy_interior<double>(s, nx, ny, nz, 1.0, xtmp, d->u);
-      pmg::nresid7_nrm1(s, nx, ny, nz, A0, F.cc, F.fc, d->u, d->F, I, d->part, d->scal, d->hscal);
-      xnorm_new = read_norm(d);
-      if (solve_failed(d, xnorm_new)) return false;
-      xnorm_old = xnorm_new;
+      if (!any) break;
+      read(&NewtonState::searching);
+      for (int q = 0; q < m; q++) {
+        NewtonState& S = st[idx[q]];
+        if (!S.searching) continue;
+        if (S.failed) {
+          S.searching = false;
+          S.active = false;
+          continue;
+        }
+        S.xnorm_new = nv[idx[q]];
+        S.iter_d++;
+      }
     }
-    rsnrm = xnorm_new;
-    if ((rsnrm / rsden) <= errtol) break;
-    if (iters >= itmax) break;
+    for (int q = 0; q < m; q++) {
+      NewtonState& S = st[idx[q]];
+      if (!S.active) continue;
+      S.rsnrm = S.xnorm_new;
+      if ((S.rsnrm / S.rsden) <= S.errtol || S.iters >= S.itmax) S.active = false;
+    }
   }
-  *relres_out = rsnrm / rsden;
-  *iters_out = iters;
-  *inner_out = inner;
-  return true;
+  for (int b = 0; b < n; b++) {
+    ok[b] = !st[b].failed;
+    relres[b] = st[b].rsnrm / st[b].rsden;
+    iters[b] = st[b].iters;
+    inner[b] = st[b].inner;
+  }
 }
 
 void ensure_forces(Vpmg* thee);
@@ -2276,75 +2536,164 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
 }
 
 // Vpmg_solve (vpmg.c:401-570)
-extern "C" int Vpmg_solve(Vpmg* thee) {
-  if (!thee || !thee->filled) return 0;
-  PmgDev* d = DEV(thee);
-  Vpmgp* p = thee->pmgp;
-  const double t0 = now();
-  const bool newton = (p->meth == VSOL_Newton);
-  if (!newton && p->meth != VSOL_MG) {
-    fprintf(stderr, "Vpmg_solve: solver method %d is not supported\n", p->meth);
-    return 0;
-  }
-  const double zkappa2 = thee->pbe->zkappa2;
-  // ccf = zkappa2 * kappa, or 0 (vpmg.c:437-447)
-  if (zkappa2 > VPMGSMALL)
-    pmg::scale(d->s, d->n, zkappa2, d->kappa, d->ccf);
-  else
-    pmg::fill<double>(d->s, d->ccf, d->n, 0.0);
-  alloc_hierarchy(d, p->nlev, newton, d->precision == 1);
-  // no inverse of an earlier solve is in flight (every solve ends with both
-  // streams drained), so the pivot flag can be cleared from the host
-  d->hscal[kPivotFlag] = 0.0;
-  d->failed = false;
-  const double tb = now();
-  build_operators(d);
-  pmg::fill<double>(d->s, d->u, d->n, 0.0);
-  double relres = 0;
-  int iters = 0, inner = 0, outer = 0;
-  bool ok = true;
-  if (newton) {
-    ok = newton_solve(thee, &relres, &iters, &inner);
-  } else {
-    set_diagonals(d, false);
-    double* X[24];
-    for (int l = 0; l < d->ntot; l++) X[l] = d->L[l].x;
-    HIPCHECK(hipStreamSynchronize(d->s));
-    d->stats[6] = now() - tb;
-    if (d->precision == 1) {
-      sync_fp32_ops(d);
-      const double tol = d->mixed_tol > 0 ? d->mixed_tol : p->errtol;
-      iters = mixed_solve(d, X, d->L[0].fc, 1, p->itmax * 4, tol, &relres, &outer, true);
-    } else {
-      iters = mvcs64(d, X, d->L[0].fc, 1, p->itmax, p->errtol, &relres);
+// Whether two filled calculations can share batched launches: the same grid,
+// hierarchy depth, method, precision and smoother forms.
+static bool same_batch(Vpmg* a, Vpmg* b) {
+  PmgDev* x = DEV(a);
+  PmgDev* y = DEV(b);
+  const Vpmgp* p = a->pmgp;
+  const Vpmgp* q = b->pmgp;
+  return x->nx == y->nx && x->ny == y->ny && x->nz == y->nz && p->nlev == q->nlev &&
+         p->meth == q->meth && x->precision == y->precision && x->smoother == y->smoother &&
+         x->smoother27 == y->smoother27;
+}
+
+// Vpmg_solve (vpmg.c:401-570) for n <= pmg::kMaxBatch calculations that
+// same_batch accepts, all on the first one's stream (the others' streams are
+// drained first and restored after): each calculation's preparation, then
+// the lockstep solve, then each one's Dirichlet faces and statistics.
+// Returns how many succeeded; ok[b] per calculation.
+static int solve_group(Vpmg* const* th, int n, int* ok) {
+  PmgDev* ds[pmg::kMaxBatch];
+  hipStream_t own[pmg::kMaxBatch];
+  for (int b = 0; b < n; b++) {
+    ds[b] = DEV(th[b]);
+    own[b] = ds[b]->s;
+    if (b > 0) {
+      HIPCHECK(hipStreamSynchronize(own[b]));
+      ds[b]->s = ds[0]->s;
     }
-    ok = iters >= 0;
   }
-  if (!ok) {
-    // every kernel has been launched normally; drain both streams so the
-    // object stays usable (a later fillco / solve / dtor)
-    if (d->cs) HIPCHECK(hipStreamSynchronize(d->cs));
-    d->cs_pending = false;
-    HIPCHECK(hipStreamSynchronize(d->s));
+  hipStream_t s = ds[0]->s;
+  const double t0 = now();
+  Vpmgp* p0 = th[0]->pmgp;
+  const bool newton = (p0->meth == VSOL_Newton);
+  for (int b = 0; b < n; b++) {
+    PmgDev* d = ds[b];
+    const double zkappa2 = th[b]->pbe->zkappa2;
+    // ccf = zkappa2 * kappa, or 0 (vpmg.c:437-447)
+    if (zkappa2 > VPMGSMALL)
+      pmg::scale(s, d->n, zkappa2, d->kappa, d->ccf);
+    else
+      pmg::fill<double>(s, d->ccf, d->n, 0.0);
+    alloc_hierarchy(d, th[b]->pmgp->nlev, newton, d->precision == 1);
+    // no inverse of an earlier solve is in flight (every solve ends with both
+    // streams drained), so the pivot flag can be cleared from the host
+    d->hscal[kPivotFlag] = 0.0;
+    d->failed = false;
+  }
+  const double tb = now();
+  for (int b = 0; b < n; b++) {
+    build_operators(ds[b]);
+    pmg::fill<double>(s, ds[b]->u, ds[b]->n, 0.0);
+  }
+  double relres[pmg::kMaxBatch] = {};
+  int iters[pmg::kMaxBatch] = {}, inner[pmg::kMaxBatch] = {}, outer[pmg::kMaxBatch] = {};
+  bool good[pmg::kMaxBatch];
+  if (newton) {
+    newton_batch(th, n, relres, iters, inner, good);
+  } else {
+    set_diagonals_batch(ds, n, false);
+    double* X[pmg::kMaxBatch][24];
+    double* const* Xp[pmg::kMaxBatch];
+    const double* fp[pmg::kMaxBatch];
+    double tol[pmg::kMaxBatch];
+    for (int b = 0; b < n; b++) {
+      for (int l = 0; l < ds[b]->ntot; l++) X[b][l] = ds[b]->L[l].x;
+      Xp[b] = X[b];
+      fp[b] = ds[b]->L[0].fc;
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+    for (int b = 0; b < n; b++) ds[b]->stats[6] = now() - tb;
+    if (ds[0]->precision == 1) {
+      for (int b = 0; b < n; b++) {
+        sync_fp32_ops(ds[b]);
+        tol[b] = ds[b]->mixed_tol > 0 ? ds[b]->mixed_tol : th[b]->pmgp->errtol;
+      }
+      mixed_batch(ds, n, Xp, fp, 1, p0->itmax * 4, tol, relres, outer, true);
+      for (int b = 0; b < n; b++) iters[b] = outer[b];
+    } else {
+      for (int b = 0; b < n; b++) tol[b] = th[b]->pmgp->errtol;
+      mvcs64_batch(ds, n, Xp, fp, 1, p0->itmax, tol, relres, iters);
+    }
+    for (int b = 0; b < n; b++) good[b] = iters[b] >= 0;
+  }
+  int nok = 0;
+  for (int b = 0; b < n; b++) {
+    PmgDev* d = ds[b];
+    ok[b] = good[b] ? 1 : 0;
+    if (!good[b]) {
+      // every kernel has been launched normally; drain both streams so the
+      // object stays usable (a later fillco / solve / dtor)
+      if (d->cs) HIPCHECK(hipStreamSynchronize(d->cs));
+      d->cs_pending = false;
+      d->forces_valid = false;
+      fprintf(stderr, "Vpmg_solve: solve failed; the potential is not valid\n");
+      continue;
+    }
+    nok++;
+    pmg::set_faces(s, d->nx, d->ny, d->nz, d->u, d->gx, d->gy, d->gz);
+    if (d->mirror)
+      HIPCHECK(hipMemcpyAsync(th[b]->u, d->u, d->n * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  HIPCHECK(hipStreamSynchronize(s));
+  for (int b = 0; b < n; b++) {
+    PmgDev* d = ds[b];
     tcollect(d);
     d->stats[4] = now() - t0;
     d->forces_valid = false;
-    fprintf(stderr, "Vpmg_solve: solve failed; the potential is not valid\n");
-    return 0;
+    if (good[b]) {
+      d->stats[0] = iters[b];
+      d->stats[1] = inner[b];
+      d->stats[2] = d->ntot;
+      d->stats[3] = relres[b];
+      d->stats[7] = outer[b];
+    }
+    d->s = own[b];
   }
-  pmg::set_faces(d->s, d->nx, d->ny, d->nz, d->u, d->gx, d->gy, d->gz);
-  if (d->mirror)
-    HIPCHECK(hipMemcpyAsync(thee->u, d->u, d->n * sizeof(double), hipMemcpyDeviceToHost, d->s));
-  HIPCHECK(hipStreamSynchronize(d->s));
-  tcollect(d);
-  d->stats[0] = iters;
-  d->stats[1] = inner;
-  d->stats[2] = d->ntot;
-  d->stats[3] = relres;
-  d->stats[4] = now() - t0;
-  d->stats[7] = outer;
-  d->forces_valid = false;
-  return 1;
+  return nok;
+}
+
+static bool solve_ready(Vpmg* thee) {
+  if (!thee || !thee->filled) return false;
+  const int meth = thee->pmgp->meth;
+  if (meth != VSOL_Newton && meth != VSOL_MG) {
+    fprintf(stderr, "Vpmg_solve: solver method %d is not supported\n", meth);
+    return false;
+  }
+  return true;
+}
+
+extern "C" int Vpmg_solve(Vpmg* thee) {
+  if (!solve_ready(thee)) return 0;
+  int ok = 0;
+  solve_group(&thee, 1, &ok);
+  return ok;
+}
+
+// Several calculations solved together: consecutive runs of up to
+// pmg::kMaxBatch calculations that same_batch accepts share every V-cycle
+// launch.  No reference counterpart.
+extern "C" int Vpmg_solveBatch(Vpmg** pmgs, int n, int* ok) {
+  if (!pmgs || n < 0) return 0;
+  int nok = 0;
+  int b = 0;
+  while (b < n) {
+    if (!solve_ready(pmgs[b])) {
+      if (ok) ok[b] = 0;
+      b++;
+      continue;
+    }
+    int e = b + 1;
+    while (e < n && e - b < pmg::kMaxBatch && solve_ready(pmgs[e]) && same_batch(pmgs[b], pmgs[e]))
+      e++;
+    int res[pmg::kMaxBatch];
+    nok += solve_group(pmgs + b, e - b, res);
+    if (ok)
+      for (int q = b; q < e; q++) ok[q] = res[q - b];
+    b = e;
+  }
+  return nok;
 }
 
 // ================================================================ focusing

@@ -14,7 +14,7 @@ from . import (BCFL_FOCUS, BCFL_MAP, BCFL_MDH, BCFL_SDH, BCFL_ZERO, MCT_AUTO, MC
                MCT_MANUAL, MCT_PARALLEL,
                NONLIN_LPBE, NONLIN_NPBE,
                PCE_NO, PCE_TOTAL, VCM_BSPL2, VCM_BSPL4, VCM_TRIL, VSM_MOL, VSM_MOLSMOOTH,
-               VSM_SPLINE, VSM_SPLINE3, VSM_SPLINE4, VSOL_MG, VSOL_Newton, MGparm, lib)
+               VSM_SPLINE, VSM_SPLINE3, VSM_SPLINE4, VSOL_MG, VSOL_Newton, MGparm, Vpmg, lib)
 
 SRFM = {"mol": VSM_MOL, "smol": VSM_MOLSMOOTH, "spl2": VSM_SPLINE, "spl3": VSM_SPLINE3,
         "spl4": VSM_SPLINE4}
@@ -267,6 +267,17 @@ class Calc:
 
     def __del__(self):
         self.release()
+
+
+def solve_batch(calcs):
+    """Vpmg_solveBatch over filled Calc objects: runs of calculations of one
+    shape, method and precision share their V-cycle launches.  Returns the
+    per-calculation Vpmg_solve results."""
+    n = len(calcs)
+    arr = (ctypes.POINTER(Vpmg) * n)(*[c.pmg for c in calcs])
+    ok = (ctypes.c_int * n)()
+    lib().Vpmg_solveBatch(arr, n, ok)
+    return list(ok)
 
 
 def _new_mgparm(kind):

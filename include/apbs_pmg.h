@@ -376,6 +376,14 @@ unsigned long int Vpmg_memChk(Vpmg *thee);     /* src/mg/vpmg.h:208 */
  * coarsest solve, Newton update and energy/force sums stay FP64.  tol is the
  * relative FP64 residual the mixed loop stops at (<= 0: pmgp->errtol). */
 int Vpmg_setPrecision(Vpmg *thee, int mode, double tol);
+/* Vpmg_solve for n calculations at once: consecutive runs of up to 8
+ * calculations with the same grid, level count, method (lpbe or npbe) and
+ * precision share every V-cycle launch (each stage is one kernel launch over
+ * the run) and the coarsest-level inverses; each calculation keeps its own
+ * Newton path, stopping test and results, bitwise those of Vpmg_solve.  The
+ * run uses the first calculation's stream.  ok (optional, n entries)
+ * receives each Vpmg_solve return value; returns the number solved. */
+int Vpmg_solveBatch(Vpmg **pmgs, int n, int *ok);
 /* Whether Vpmg_solve copies u back to the host array (default 1). */
 int Vpmg_setHostMirror(Vpmg *thee, int on);
 /* Solver statistics of the last solve: [0] V-cycles (linear) or Newton
