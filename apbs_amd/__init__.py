@@ -254,6 +254,9 @@ _SIGS = [
     ("PmgServer_stats", None, [ctypes.c_void_p, _dp]),
     ("PmgServer_dtor", None, [ctypes.POINTER(ctypes.c_void_p)]),
     ("PmgServer_footprint", ctypes.c_size_t, [ctypes.POINTER(PmgRequest)]),
+    ("PmgServer_setBatching", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    ("PmgServer_batchStats", None,
+     [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]),
     ("apbs_pmg_now", ctypes.c_double, []),
     ("apbs_pmg_poolStats", None, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
                                   ctypes.POINTER(ctypes.c_size_t), ctypes.c_int]),

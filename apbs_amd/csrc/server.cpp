@@ -1,11 +1,12 @@
 // server.cpp -- continuous-batching PB solve service (apbs_pmg_server.h).
 //
 // One worker thread per slot.  A worker takes the oldest queued request as
-// soon as its footprint fits in the unused HBM budget, runs the complete
-// mg-manual calculation through the public Vpmg entry points on the Vpmg's own
-// stream, posts the result and frees the footprint.  Workers never wait for
-// one another, so one request's kernels overlap with the others' on the
-// device.
+// soon as its footprint fits in the unused HBM budget, runs the mg-manual
+// calculation through the public Vpmg entry points on the Vpmg's own stream,
+// posts the result and frees the footprint.  The solves of calculations of
+// one shape are batched across requests (SolveGate), so their V-cycles share
+// kernel launches; everything else of a request overlaps with the others'
+// work on the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -67,9 +68,80 @@ size_t footprint_of(const PmgRequest& r) {
                                  r.precision == 1, r.natoms, nref);
 }
 
+// Cross-request batching of the solves.  A worker whose calculation is
+// filled waits here; a waiting worker leads a batch when fewer than
+// max_running batches run: it takes the oldest waiting calculations of its own
+// shape (grid, level count, method, precision), at most max_batch, and solves
+// them with Vpmg_solveBatch, which shares every V-cycle launch among them.
+// Calculations of one batch run in lockstep on one stream; batches of
+// different shapes and the other workers' setup overlap on the device.
+struct SolveGate {
+  struct Key {
+    int nx, ny, nz, nlev, meth, precision;
+    bool operator==(const Key& o) const {
+      return nx == o.nx && ny == o.ny && nz == o.nz && nlev == o.nlev && meth == o.meth &&
+             precision == o.precision;
+    }
+  };
+  struct Entry {
+    Vpmg* pmg;
+    Key key;
+    int ok = 0;
+    bool taken = false, done = false;
+  };
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Entry*> waiting;
+  int running = 0;
+  int max_batch = APBS_PMG_MAXBATCH;
+  int max_running = 2;
+  long batches = 0, batched = 0;
+
+  int solve(Vpmg* pmg, int precision) {
+    Entry e{pmg, Key{pmg->pmgp->nx, pmg->pmgp->ny, pmg->pmgp->nz, pmg->pmgp->nlev,
+                     (int)pmg->pmgp->meth, precision}};
+    std::unique_lock<std::mutex> lk(mu);
+    waiting.push_back(&e);
+    cv.notify_all();
+    for (;;) {
+      if (e.done) return e.ok;
+      if (!e.taken && running < max_running) {
+        std::vector<Entry*> batch;
+        for (auto it = waiting.begin(); it != waiting.end() && (int)batch.size() < max_batch;) {
+          if ((*it)->key == e.key) {
+            (*it)->taken = true;
+            batch.push_back(*it);
+            it = waiting.erase(it);
+          } else {
+            ++it;
+          }
+        }
+        running++;
+        batches++;
+        batched += (long)batch.size();
+        lk.unlock();
+        std::vector<Vpmg*> list;
+        for (Entry* x : batch) list.push_back(x->pmg);
+        std::vector<int> ok(batch.size(), 0);
+        Vpmg_solveBatch(list.data(), (int)list.size(), ok.data());
+        lk.lock();
+        running--;
+        for (size_t q = 0; q < batch.size(); q++) {
+          batch[q]->ok = ok[q];
+          batch[q]->done = true;
+        }
+        cv.notify_all();
+        continue;
+      }
+      cv.wait(lk);
+    }
+  }
+};
+
 // One mg-manual calculation (routines.c initMG / solveMG / setPartMG /
-// energyMG) through the public API.
-void run_job(Job& j, PmgResult& res) {
+// energyMG) through the public API; the solve goes through the gate when
+// batching is on.
+void run_job(Job& j, PmgResult& res, SolveGate* gate) {
   PmgRequest& r = j.req;
   res.status = 0;
   Valist* al = Valist_ctor();
@@ -121,7 +193,7 @@ void run_job(Job& j, PmgResult& res) {
     if (!Vpmg_fillco(pmg, (Vsurf_Meth)r.srfm, r.swin, (Vchrg_Meth)r.chgm, 0, nullptr, 0, nullptr,
                      0, nullptr, 0, nullptr, 0, nullptr, 0, nullptr))
       break;
-    if (!Vpmg_solve(pmg)) break;
+    if (!(gate ? gate->solve(pmg, r.precision) : Vpmg_solve(pmg))) break;
     double lo[3], hi[3];
     for (int k = 0; k < 3; k++) {
       lo[k] = m->center[k] - 0.5 * m->glen[k];
@@ -159,6 +231,8 @@ struct PmgServer {
   long next_id = 0, finished = 0;
   int running = 0, slots = 0;
   bool stop = false;
+  bool batching = true;
+  SolveGate gate;
 
   void work() {
     for (;;) {
@@ -185,7 +259,7 @@ struct PmgServer {
       res.footprint = j->footprint;
       res.t_submit = j->t_submit;
       res.t_start = apbs_pmg_now();
-      run_job(*j, res);
+      run_job(*j, res, batching ? &gate : nullptr);
       res.t_done = apbs_pmg_now();
       {
         std::lock_guard<std::mutex> lk(mu);
@@ -279,6 +353,22 @@ extern "C" void PmgServer_stats(PmgServer* s, double out[8]) {
   out[5] = (double)s->peak_admitted;
   out[6] = (double)peak;
   out[7] = (double)s->slots;
+}
+
+extern "C" int PmgServer_setBatching(PmgServer* s, int max_batch, int max_running) {
+  if (!s || max_batch < 1 || max_batch > APBS_PMG_MAXBATCH || max_running < 1) return 0;
+  std::lock_guard<std::mutex> lk(s->gate.mu);
+  s->batching = max_batch > 1;
+  s->gate.max_batch = max_batch;
+  s->gate.max_running = max_running;
+  return 1;
+}
+
+extern "C" void PmgServer_batchStats(PmgServer* s, long* batches, long* batched) {
+  if (!s) return;
+  std::lock_guard<std::mutex> lk(s->gate.mu);
+  if (batches) *batches = s->gate.batches;
+  if (batched) *batched = s->gate.batched;
 }
 
 extern "C" void PmgServer_dtor(PmgServer** sp) {

@@ -2536,6 +2536,8 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
 }
 
 // Vpmg_solve (vpmg.c:401-570)
+static_assert(APBS_PMG_MAXBATCH == pmg::kMaxBatch, "the public batch size is the kernels'");
+
 // Whether two filled calculations can share batched launches: the same grid,
 // hierarchy depth, method, precision and smoother forms.
 static bool same_batch(Vpmg* a, Vpmg* b) {
@@ -2864,7 +2866,9 @@ static void set_part_impl(Vpmg* thee, const double lo[3], const double hi[3], co
     }
   }
   upload_part(d);
-  HIPCHECK(hipMemcpyAsync(thee->pvec, d->pvec, d->n * sizeof(double), hipMemcpyDeviceToHost, d->s));
+  if (d->mirror)
+    HIPCHECK(hipMemcpyAsync(thee->pvec, d->pvec, d->n * sizeof(double), hipMemcpyDeviceToHost,
+                            d->s));
   HIPCHECK(hipStreamSynchronize(d->s));
   d->forces_valid = false;
 }
@@ -2883,7 +2887,8 @@ extern "C" void Vpmg_unsetPart(Vpmg* thee) {
     al->atoms[a].partID = 1.0;
   }
   upload_part(d);
-  for (size_t i = 0; i < d->n; i++) thee->pvec[i] = 1.0;
+  if (d->mirror)
+    for (size_t i = 0; i < d->n; i++) thee->pvec[i] = 1.0;
   HIPCHECK(hipStreamSynchronize(d->s));
   d->forces_valid = false;
 }

@@ -71,10 +71,16 @@ def _result(r):
 
 
 class Server:
-    def __init__(self, slots=8, hbm_budget=200 * 2 ** 30):
+    """PmgServer: `slots` requests in flight; their solves are batched across
+    requests of one shape, at most `max_batch` per batch and `max_running`
+    batches at once (max_batch 1: every request solved alone)."""
+
+    def __init__(self, slots=8, hbm_budget=200 * 2 ** 30, max_batch=8, max_running=2):
         self.h = lib().PmgServer_ctor(int(slots), int(hbm_budget))
         if not self.h:
             raise RuntimeError("PmgServer_ctor failed")
+        if not lib().PmgServer_setBatching(self.h, int(max_batch), int(max_running)):
+            raise ValueError("invalid batching parameters")
         self._keep = {}
 
     def submit(self, req):
@@ -102,9 +108,12 @@ class Server:
     def stats(self):
         o = np.zeros(8)
         lib().PmgServer_stats(self.h, _ptr(o))
+        nb, nr = ctypes.c_long(), ctypes.c_long()
+        lib().PmgServer_batchStats(self.h, ctypes.byref(nb), ctypes.byref(nr))
         return {"submitted": int(o[0]), "finished": int(o[1]), "running": int(o[2]),
                 "queued": int(o[3]), "admitted_bytes": o[4], "peak_admitted_bytes": o[5],
-                "peak_pool_bytes": o[6], "slots": int(o[7])}
+                "peak_pool_bytes": o[6], "slots": int(o[7]), "batches": nb.value,
+                "batched_requests": nr.value}
 
     def close(self):
         if self.h:

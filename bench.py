@@ -5,23 +5,29 @@ Workload (the metric's configuration): nonlinear PB, Newton-multigrid, grid
 charge -10 e, pdie 4, sdie 78.54, srfm smol, chgm spl2, bcfl mdh, 0.10 M and
 0.05 M monovalent salt.
 
-One step = one complete PB calculation on the device-resident Vpmg object:
-Vpmg_fillco (coefficient maps from the atom list) + Vpmg_solve (Newton
-multigrid to etol 1e-6) + Vpmg_energy.  The potential stays in HBM (no host
-mirror).  The atom list (120 KB) is uploaded inside fillco; that copy is part
-of the step.
+One request = one complete PB calculation: Vpmg_ctor + Vpmg_fillco
+(coefficient maps from the atom list) + Vpmg_solve (Newton multigrid to etol
+1e-6) + Vpmg_energy.  The potential stays in HBM (no host mirror).  The atom
+list (120 KB) is uploaded inside fillco; that copy is part of the request.
 
-Two paths are timed for each precision (FP64, and mixed: FP32 V-cycles in an
-FP64 defect-correction loop):
-  * single: one request at a time on one Vpmg (steps timed after warmup);
-  * served: --served requests submitted at once to the continuous-batching
-    service with --slots calculations in flight (closed loop).
-`value` is the served mixed-precision solves/s (p50/p99 are its request
-latencies); the single-request figures, the energy and potential error of
-mixed against FP64, the roofline of the dominant kernel (the fine-level
-red-black smoother, from HIP events recorded on the solver's stream during
-the single-request mixed run) and the oracle (the FP64 CPU restatement of the
-reference) timed on a bounded sample on the host are reported beside it.
+Paths, each in FP64 and in mixed precision (FP32 V-cycles in an FP64
+defect-correction loop):
+  * single: one request at a time on one Vpmg (--warmup, then --steps timed
+    requests; the Vpmg and its hierarchy are reused);
+  * served: the continuous-batching service (--slots requests in flight,
+    solves of ready requests batched up to 8 per launch), closed loop: --warmup
+    rounds, then exactly --steps timed rounds of --slots requests, all
+    submitted at once.  `value` is the served mixed-precision requests/s;
+    `steps` counts those rounds, `ms_per_step` is the time per round;
+  * served_stream: the served workload of binding triplets (512 requests,
+    dims 65..161, lpbe/npbe 50/50, 300-8000 atoms) with Poisson arrivals at
+    two rates below saturation: p50/p99 latency (completion - arrival),
+    solves/s and peak device memory.
+Beside them: the energy and potential error of mixed against FP64, the
+roofline of the dominant kernel (the fine-level red-black smoother, from HIP
+events recorded on the solver's stream during the single-request mixed run)
+and the oracle (the FP64 CPU restatement of the reference, built -O3, one
+host thread) timed on one complete calculation of the same workload.
 """
 import argparse
 import json
@@ -47,7 +53,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--slots", type=int, default=8, help="concurrent requests of the served path")
-    ap.add_argument("--served", type=int, default=48, help="requests in the served run (0: none)")
+    ap.add_argument("--served", type=int, default=1, help="run the served closed loop (0: skip)")
+    ap.add_argument("--stream", type=int, default=1,
+                    help="run the served stream of binding triplets at two rates (0: skip)")
     ap.add_argument("--pmc", action="store_true",
                     help="counter-collection workload: the fine level's fused sweeps in each "
                          "precision and a known-size copy kernel (run under rocprofv3 --pmc)")
@@ -177,59 +185,94 @@ def vcycle_bandwidth(kt, cfg, levels):
             "GBs": words * kt["word"] / (ms * 1e-3) / 1e9, "levels": dims}
 
 
-def cpu_baseline(cfg, newton_steps, sample_steps=3):
-    """The oracle (FP64 CPU restatement of the reference's pmgc path, one
-    host thread) on a bounded sample of the same workload: Vpmg_fillco in
-    full, then the first `sample_steps` Newton steps of the solve.  The solve
-    time is scaled to the full Newton step count of the FP64 GPU solve (the
-    same damped-inexact-Newton algorithm, one inner V-cycle per step at this
-    size)."""
+def cpu_baseline(cfg):
+    """The oracle (FP64 CPU restatement of the reference's pmgc path, built
+    -O3, one host thread) on one complete calculation of the workload:
+    Vpmg_fillco then the whole Newton solve, timed."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc_driver as od
     from apbs_amd import synth
     atoms = synth.make(WORKLOAD)
     c = od.mol_center(atoms[0])
-    pbe = dict(cfg["pbe"], newton_itmax=sample_steps)
     nlev, dime = od.check_nlev([cfg["dime"]] * 3)
-    p = od.make_params(dime, [cfg["h"]] * 3, c, nlev, pbe)
+    p = od.make_params(dime, [cfg["h"]] * 3, c, nlev, dict(cfg["pbe"]))
     o = od.OrcCalc(p, atoms)
     t = time.perf_counter()
     o.fillco()
     t_fill = time.perf_counter() - t
     t = time.perf_counter()
     o.solve()
-    t_part = time.perf_counter() - t
-    done = o.stats()["iters"]
-    t_solve = t_part * newton_steps / done
+    t_solve = time.perf_counter() - t
+    st = o.stats()
     total = t_fill + t_solve
     return {"value": round(1.0 / total, 5), "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"{WORKLOAD}: oracle fillco in full ({t_fill:.1f} s) + the first {done} "
-                      f"Newton steps ({t_part:.1f} s), solve time scaled to {newton_steps} "
-                      f"steps; one host thread"}
+            "sample": f"{WORKLOAD}: one complete calculation by the oracle (gcc -O3, one host "
+                      f"thread): fillco {t_fill:.1f} s + Newton solve {t_solve:.1f} s "
+                      f"({st['iters']} steps, reference schedule)"}
 
 
-def run_served(xyz, q, r, cfg, precision, nreq, slots, warmup):
-    """The served path: nreq requests of the workload submitted at once to a
-    PmgServer with `slots` concurrent calculations (closed loop); solves/s is
-    nreq over the time from the first submission to the last completion,
-    latency is completion minus submission per request."""
-    from apbs_amd import serve
-    pbe = dict(cfg["pbe"])
+def spec_of(xyz, q, r, cfg):
     from apbs_amd import driver
     center = driver.Molecule(xyz, q, r).center
-    spec = dict(xyz=xyz, q=q, r=r, dime=(cfg["dime"],) * 3, grid=[cfg["h"]] * 3, center=center,
-                pbe=pbe)
+    return dict(xyz=xyz, q=q, r=r, dime=(cfg["dime"],) * 3, grid=[cfg["h"]] * 3, center=center,
+                pbe=dict(cfg["pbe"]))
+
+
+def run_served(spec, precision, rounds, slots, warmup):
+    """The served path, closed loop: `warmup` rounds, then exactly `rounds`
+    timed rounds of `slots` identical requests submitted at once.  solves/s
+    is the timed requests over the time from the first submission to the
+    last completion; latency is completion minus submission per request."""
+    from apbs_amd import lib, serve
+    import ctypes
     srv = serve.Server(slots=slots)
-    serve.run_stream(srv, [spec] * max(warmup, slots), rate=0, precision=precision)
-    res, wall = serve.run_stream(srv, [spec] * nreq, rate=0, precision=precision)
+    serve.run_stream(srv, [spec] * (max(warmup, 1) * slots), rate=0, precision=precision)
+    z = ctypes.c_size_t()
+    lib().apbs_pmg_poolStats(ctypes.byref(z), ctypes.byref(z), ctypes.byref(z), 1)
+    b0 = srv.stats()
+    res, wall = serve.run_stream(srv, [spec] * (rounds * slots), rate=0, precision=precision)
     st = srv.stats()
     srv.close()
     summ = serve.latency_summary(res, wall)
+    summ["ms_per_round"] = wall / rounds * 1e3
     summ["energy_kT"] = res[0]["energy_kT"]
     summ["peak_admitted_GB"] = st["peak_admitted_bytes"] / 1e9
     summ["peak_pool_GB"] = st["peak_pool_bytes"] / 1e9
     summ["slots"] = slots
+    nb = st["batches"] - b0["batches"]
+    summ["batches"] = nb
+    summ["mean_batch"] = (st["batched_requests"] - b0["batched_requests"]) / max(nb, 1)
     return summ
+
+
+STREAM_REQUESTS = 512
+STREAM_RATES = (40.0, 80.0)
+
+
+def served_stream(rates=STREAM_RATES, n=STREAM_REQUESTS, slots=16):
+    """The served workload of binding triplets (serve.binding_stream: dims
+    65..161, lpbe/npbe 50/50, 300-8000 atoms, mixed precision) with Poisson
+    arrivals at each rate."""
+    from apbs_amd import lib, serve
+    import ctypes
+    specs = serve.binding_stream(n, seed=3)
+    out = []
+    for rate in rates:
+        z = ctypes.c_size_t()
+        lib().apbs_pmg_poolStats(ctypes.byref(z), ctypes.byref(z), ctypes.byref(z), 1)
+        srv = serve.Server(slots=slots)
+        res, wall = serve.run_stream(srv, specs, rate, seed=3, precision=1)
+        st = srv.stats()
+        srv.close()
+        summ = serve.latency_summary(res, wall)
+        summ.update(rate=rate, slots=slots, peak_pool_GB=round(st["peak_pool_bytes"] / 1e9, 3),
+                    peak_admitted_GB=round(st["peak_admitted_bytes"] / 1e9, 3),
+                    mean_batch=round(st["batched_requests"] / max(st["batches"], 1), 2))
+        out.append(summ)
+        print(f"# served stream at {rate}/s: {summ}", file=sys.stderr, flush=True)
+    return {"workload": f"{n} requests, binding triplets, dims 65/97/129/161, lpbe/npbe 50/50, "
+                        "300-8000 atoms, mixed precision, Poisson arrivals",
+            "rates": out}
 
 
 def main():
@@ -256,12 +299,6 @@ def main():
         print(f"# {mode}: {res['solves_per_s']:.2f} solves/s, p50 {res['p50_ms']:.2f} ms, "
               f"p99 {res['p99_ms']:.2f} ms, newton {res['newton_steps']}, "
               f"inner {res['inner_cycles']}, kernel {res['kernel']}", file=sys.stderr, flush=True)
-    served = {}
-    if a.served > 0:
-        for mode in modes:
-            served[mode] = run_served(xyz, q, r, cfg, 1 if mode == "mixed" else 0, a.served,
-                                      a.slots, a.warmup)
-            print(f"# served {mode}: {served[mode]}", file=sys.stderr, flush=True)
     head = "mixed" if "mixed" in out else modes[0]
     err = None
     if "mixed" in out and "fp64" in out:
@@ -273,19 +310,31 @@ def main():
                "potential_rel_L2": float(np.linalg.norm(d) / np.linalg.norm(u64)),
                "potential_rel_Linf": float(np.abs(d).max() / np.abs(u64).max())}
     kt = out[head]["kernel"]
+    for c in calcs.values():
+        c.release()
+    calcs.clear()
+    served = {}
+    if a.served:
+        spec = spec_of(xyz, q, r, cfg)
+        for mode in modes:
+            served[mode] = run_served(spec, 1 if mode == "mixed" else 0, a.steps, a.slots,
+                                      a.warmup)
+            print(f"# served {mode}: {served[mode]}", file=sys.stderr, flush=True)
+    stream = served_stream() if a.stream else None
     top = served.get(head)
     if top is not None:
-        value, ms, p50, p99 = (top["solves_per_s"], 1e3 / top["solves_per_s"], top["p50_ms"],
+        value, ms, p50, p99 = (top["solves_per_s"], top["ms_per_round"], top["p50_ms"],
                                top["p99_ms"])
     else:
         value, ms, p50, p99 = (out[head]["solves_per_s"], out[head]["ms_per_solve"],
                                out[head]["p50_ms"], out[head]["p99_ms"])
+    from apbs_amd import synth
     line = {
         "metric": "solves/s + p50/p99 latency, mixed vs FP64 PMG at 161^3 npbe; HBM GB/s",
         "value": round(value, 3),
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": a.served if top is not None else a.steps,
+        "steps": a.steps,
         "warmup": a.warmup,
         "ms_per_step": round(ms, 3),
         "p50_ms": round(p50, 3),
@@ -296,8 +345,11 @@ def main():
         "dtype": "f32+f64" if head == "mixed" else "f64",
         "data": "synthetic (seeded globule)",
         "config": {"workload": f"{a.workload}: npbe 161^3 at 0.4 A, 5000 atoms, smol, spl2, mdh",
-                   "path": f"{head}, served ({a.slots} slots)" if top is not None else head,
-                   "step": "one request: fillco + solve + energy"},
+                   "molecule": synth.describe(a.workload),
+                   "path": (f"{head}, served ({a.slots} slots, batched solves)"
+                            if top is not None else head),
+                   "step": (f"one round of {a.slots} requests (fillco + solve + energy each)"
+                            if top is not None else "one request: fillco + solve + energy")},
         "served": served,
         "speedup_served_mixed_over_fp64": (served["mixed"]["solves_per_s"]
                                            / out["fp64"]["solves_per_s"]
@@ -311,14 +363,9 @@ def main():
         "roofline": smoother_roofline(kt),
         "roofline_fp64": smoother_roofline(out["fp64"]["kernel"]) if "fp64" in out else None,
         "vcycle": vcycle_bandwidth(kt, cfg, out[head]["levels"]),
+        "served_stream": stream,
     }
-    for c in calcs.values():
-        c.release()
-    if not a.no_cpu_baseline:
-        steps = out["fp64"]["newton_steps"] if "fp64" in out else out[head]["newton_steps"]
-        line["cpu_baseline"] = cpu_baseline(cfg, steps)
-    else:
-        line["cpu_baseline"] = None
+    line["cpu_baseline"] = None if a.no_cpu_baseline else cpu_baseline(cfg)
     print(json.dumps(line), flush=True)
 
 

@@ -52,6 +52,8 @@ typedef enum { MCM_POINT = 0, MCM_MOLECULE = 1, MCM_FOCUS = 2 } MGparm_CentMeth;
 typedef enum { PCE_NO = 0, PCE_TOTAL = 1, PCE_COMPS = 2 } PBEparm_calcEnergy;
 
 #define APBS_PMG_MAXION 10   /* src/generic/vhal.h:377 MAXION */
+/* Calculations one batched launch holds (Vpmg_solveBatch, PmgServer). */
+#define APBS_PMG_MAXBATCH 8
 
 typedef struct sVmem Vmem;   /* opaque; unused by this library */
 /* struct sVgrid, field for field (src/mg/vgrid.h:85-104): a scalar map on a
@@ -384,7 +386,10 @@ int Vpmg_setPrecision(Vpmg *thee, int mode, double tol);
  * run uses the first calculation's stream.  ok (optional, n entries)
  * receives each Vpmg_solve return value; returns the number solved. */
 int Vpmg_solveBatch(Vpmg **pmgs, int n, int *ok);
-/* Whether Vpmg_solve copies u back to the host array (default 1). */
+/* Whether the host arrays callers dereference are kept current (default 1):
+ * Vpmg_solve copies u back to the host, Vpmg_setPart and Vpmg_unsetPart
+ * write pvec there.  With 0 both stay on the device only (Vpmg_copyArray
+ * reads them on demand). */
 int Vpmg_setHostMirror(Vpmg *thee, int on);
 /* Solver statistics of the last solve: [0] V-cycles (linear) or Newton
  * steps, [1] total inner V-cycles, [2] levels used, [3] final relative

@@ -8,9 +8,12 @@
  * admitted first-in first-out, whenever a slot is free and the estimated HBM
  * footprint of their multigrid hierarchy fits in the remaining budget.
  * Requests that arrive while others run join as soon as a slot and memory
- * free up, and their GPU work overlaps on the device.  Device memory is
- * recycled through the library's caching pool, so a finished request frees
- * nothing back to the driver.
+ * free up, and their GPU work overlaps on the device.  The solves of requests
+ * of one grid, level count, method and precision that are ready at the same
+ * time are batched (Vpmg_solveBatch): their V-cycles share every kernel
+ * launch, and each result is bitwise the request's result alone.  Device
+ * memory is recycled through the library's caching pool, so a finished
+ * request frees nothing back to the driver.
  */
 #ifndef APBS_PMG_SERVER_H
 #define APBS_PMG_SERVER_H
@@ -71,6 +74,16 @@ void PmgServer_drain(PmgServer *thee);
  * footprint now (bytes), [5] peak admitted footprint, [6] peak device bytes
  * held by the library's pool, [7] slots. */
 void PmgServer_stats(PmgServer *thee, double out[8]);
+/* Cross-request batching of the solves (default: max_batch 8, max_running
+ * 2).  A request whose coefficient maps are built waits for its solve; a
+ * waiting request's worker starts a batch whenever fewer than max_running
+ * batches run, with the oldest waiting requests of its shape (grid, level
+ * count, method, precision), at most max_batch (<= APBS_PMG_MAXBATCH).
+ * max_batch 1 solves every request alone on its own stream.  Returns 0 on
+ * invalid values. */
+int PmgServer_setBatching(PmgServer *thee, int max_batch, int max_running);
+/* Batches solved so far and the requests they held. */
+void PmgServer_batchStats(PmgServer *thee, long *batches, long *batched);
 /* Drains, then stops the workers and frees the service. */
 void PmgServer_dtor(PmgServer **thee);
 /* Estimated device bytes one request holds while it runs. */

@@ -1,10 +1,13 @@
 """The continuous-batching service (include/apbs_pmg_server.h) on the GPU.
 
 Every kernel of a calculation is deterministic (ordered gathers, fixed
-two-stage reductions, order-free scatters), so a calculation's result does
-not depend on what else runs on the device: the bar is bitwise equality of
-energy and potential between a request run alone and the same request run
-in a mix, and against the same calculation run directly through Vpmg.
+two-stage reductions, order-free scatters), and a batched solve runs each
+calculation's own arithmetic, so a calculation's result does not depend on
+what else runs on the device or shares its launches: the bar is bitwise
+equality of energy and potential between a request run alone (one slot, no
+batching) and the same request run in a mix (several slots, solves batched
+across requests), and against the same calculation run directly through
+Vpmg.
 """
 import numpy as np
 import pytest
@@ -30,7 +33,7 @@ def make_req(s, precision, tag=0):
 def test_alone_equals_in_mix(precision):
     specs = mix_specs()
     alone = []
-    srv = serve.Server(slots=1)
+    srv = serve.Server(slots=1, max_batch=1)
     for i, s in enumerate(specs):
         req = make_req(s, precision, i)
         srv.submit(req)
@@ -48,6 +51,8 @@ def test_alone_equals_in_mix(precision):
     st = srv.stats()
     srv.close()
     assert len(res) == len(specs) and st["finished"] == len(specs)
+    print(f"mix: {st['batches']} batches holding {st['batched_requests']} requests")
+    assert st["batched_requests"] == len(specs)
     overlap = 0
     for i, req in enumerate(reqs):
         assert res[i]["status"] == 1
