@@ -246,7 +246,7 @@ def run_served(spec, precision, rounds, slots, warmup):
 
 
 STREAM_REQUESTS = 512
-STREAM_RATES = (40.0, 80.0)
+STREAM_RATES = (25.0, 50.0)
 
 
 def served_stream(rates=STREAM_RATES, n=STREAM_REQUESTS, slots=16):
