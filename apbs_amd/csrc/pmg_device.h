@@ -86,9 +86,11 @@ void resid(hipStream_t s, int st, int nx, int ny, int nz, const Ops<T>& A, const
 // nonlinear residual on the fine 7-point level: f + sum a x - oC x - N(x)
 void nresid7(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* cc,
              const double* fc, const double* x, double* r, const Ions& ions);
-// cprime = N'(x) (Vdc_vecpmg), full array
+// cprime = N'(x) (Vdc_vecpmg), full array; with oC also the diagonal
+// D = oC + cprime and, with Df, its FP32 image (add_diag's values), in one pass
 void dcvec(hipStream_t s, size_t n, const double* cc, const double* x, double* out,
-           const Ions& ions);
+           const Ions& ions, const double* oC = nullptr, double* D = nullptr,
+           float* Df = nullptr);
 
 // ------------------------------------------------------------- transfers
 // out = P^T in on the coarse interior; zero (optional, same coarse size) is
@@ -106,11 +108,22 @@ void prolong_add(hipStream_t s, int nxc, int nyc, int nzc, int nxf, int nyf, int
 template <class T>
 void nrm1(hipStream_t s, int nx, int ny, int nz, const T* x, double* part, double* out,
           double* hout = nullptr);
-// r = f + sum a x - D x on the 7-point level (stored when r != nullptr) and
-// out[0] = ||r||_1 over the interior, bitwise equal to resid + nrm1.
+// r = f + sum a x - D x on the 7-point level (stored when r != nullptr;
+// r32, when given, receives (float) r on the interior) and out[0] = ||r||_1
+// over the interior, bitwise equal to resid + nrm1.
 template <class T>
 void resid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D,
-                 const T* fc, const T* x, T* r, double* part, double* out, double* hout = nullptr);
+                 const T* fc, const T* x, T* r, double* part, double* out, double* hout = nullptr,
+                 float* r32 = nullptr);
+// nrm1 of x and y = (float) x on the interior, in one pass
+void nrm1_to32(hipStream_t s, int nx, int ny, int nz, const double* x, float* y, double* part,
+               double* out, double* hout = nullptr);
+// y = x + a z (xpay_interior) and nresid7_nrm1 at y, in one pass, bitwise the
+// two calls (the boundary layers of x, z and y are zero)
+void nresid7_xpay_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A,
+                       const double* cc, const double* fc, const double* x, double a,
+                       const double* z, double* y, double* r, const Ions& ions, double* part,
+                       double* out, double* hout = nullptr);
 // nresid7 followed by nrm1 of its result, in one pass.
 void nresid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* cc,
                   const double* fc, const double* x, double* r, const Ions& ions, double* part,
@@ -171,9 +184,10 @@ struct DiagLevels {
   float* Df[24];
 };
 void add_diag_levels(hipStream_t s, const DiagLevels& L);
-// y += a * (double) x on the interior (x FP32, y FP64)
+// y += a * (double) x on the interior (x FP32, y FP64); assign: y starts at
+// zero (its interior is not read)
 void axpy_interior_f32(hipStream_t s, int nx, int ny, int nz, double a, const float* x,
-                       double* y);
+                       double* y, bool assign = false);
 template <class Tin, class Tout>
 void convert(hipStream_t s, size_t n, const Tin* in, Tout* out);
 // FP32 images of up to kConvMax FP64 arrays of one length in one launch.

@@ -622,7 +622,8 @@ template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_resid7_nrm(int nx, int ny, int nz, const T* __restrict__ oE, const T* __restrict__ oN,
                  const T* __restrict__ uC, const T* __restrict__ D, const T* __restrict__ fc,
-                 const T* __restrict__ x, T* __restrict__ r, double* part) {
+                 const T* __restrict__ x, T* __restrict__ r, float* __restrict__ r32,
+                 double* part) {
   __shared__ double sh[kBlock];
   const int ni = nx - 2, nij = ni * (ny - 2), n = nij * (nz - 2);
   const size_t nxy = (size_t)nx * ny;
@@ -635,7 +636,30 @@ __global__ void __launch_bounds__(kBlock)
     const size_t p = IX(i, j, k, nx, ny);
     const T v = acc7(fc[p], oE, oN, uC, x, p, nx, nxy) - D[p] * x[p];
     if (r) r[p] = v;
+    if (r32) r32[p] = (float)v;
     acc += fabs((double)v);
+  }
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// The L1 norm partials of x (k_nrm1_part's order) and y = (float) x on the
+// interior, in one pass.
+__global__ void __launch_bounds__(kBlock)
+    k_nrm1_to32(int nx, int ny, int nz, const double* __restrict__ x, float* __restrict__ y,
+                double* part) {
+  __shared__ double sh[kBlock];
+  const int ni = nx - 2, nij = ni * (ny - 2), n = nij * (nz - 2);
+  int t0, t1;
+  part_range(n, t0, t1);
+  double acc = 0.0;
+  for (int t = t0 + threadIdx.x; t < t1; t += kBlock) {
+    const int k = 1 + t / nij, rem = t - (k - 1) * nij;
+    const int j = 1 + rem / ni, i = 1 + rem - (j - 1) * ni;
+    const size_t p = IX(i, j, k, nx, ny);
+    const double v = x[p];
+    y[p] = (float)v;
+    acc += fabs(v);
   }
   double s = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -961,13 +985,16 @@ __global__ void __launch_bounds__(kBlock) k_add_diag_levels(DiagLevels L) {
 // solution, the same bits as converting x first and calling axpy_interior
 __global__ void __launch_bounds__(kBlock)
     k_axpy_int_f32(int nx, int ny, int nz, double a, const float* __restrict__ x,
-                   double* __restrict__ y) {
+                   double* __restrict__ y, int assign) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   const int k = blockIdx.y + 1;
   const int i = 1 + t % (nx - 2), j = 1 + t / (nx - 2);
   if (j > ny - 2) return;
   const size_t p = IX(i, j, k, nx, ny);
-  y[p] += a * (double)x[p];
+  // assign: y starts at zero, and 0.0 + v rounds like the update of a zero y
+  double v = assign ? 0.0 : y[p];
+  v += a * (double)x[p];
+  y[p] = v;
 }
 
 __global__ void k_scale(size_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
@@ -1074,8 +1101,56 @@ __global__ void __launch_bounds__(kBlock)
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// k_nresid7_nrm at the trial point y = x + a z (k_xpay_int's arithmetic,
+// evaluated for every stencil point it reads), storing y and the residual:
+// the same bits as k_xpay_int then k_nresid7_nrm.  The boundary layers of x
+// and z are zero, as y's is.
+__global__ void __launch_bounds__(kBlock)
+    k_nresid7_xpay(int nx, int ny, int nz, const double* __restrict__ oC,
+                   const double* __restrict__ oE, const double* __restrict__ oN,
+                   const double* __restrict__ uC, const double* __restrict__ cc,
+                   const double* __restrict__ fc, const double* __restrict__ x, double a,
+                   const double* __restrict__ z, double* __restrict__ y,
+                   double* __restrict__ r, Ions ions, double* part) {
+  __shared__ double sh[kBlock];
+  const int ni = nx - 2, nij = ni * (ny - 2), n = nij * (nz - 2);
+  const size_t nxy = (size_t)nx * ny;
+  int t0, t1;
+  part_range(n, t0, t1);
+  auto yv = [&](size_t q) {
+    double v = x[q];
+    v += a * z[q];
+    return v;
+  };
+  double acc = 0.0;
+  for (int t = t0 + threadIdx.x; t < t1; t += kBlock) {
+    const int k = 1 + t / nij, rem = t - (k - 1) * nij;
+    const int j = 1 + rem / ni, i = 1 + rem - (j - 1) * ni;
+    const size_t p = IX(i, j, k, nx, ny);
+    const double y0 = yv(p);
+    y[p] = y0;
+    const double w = ncvec(cc[p], y0, ions);
+    double s7 = fc[p];
+    s7 += oN[p] * yv(p + nx);
+    s7 += oN[p - nx] * yv(p - nx);
+    s7 += oE[p] * yv(p + 1);
+    s7 += oE[p - 1] * yv(p - 1);
+    s7 += uC[p - nxy] * yv(p - nxy);
+    s7 += uC[p] * yv(p + nxy);
+    const double v = s7 - oC[p] * y0 - w;
+    r[p] = v;
+    acc += fabs(v);
+  }
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Vdc_vecpmg (mypdec.c:366-433) at every point; with oC given also the
+// diagonal D = oC + N'(x) and, with Df, its FP32 image (k_add_diag_levels'
+// values for this level) in the same pass.
 __global__ void k_dcvec(size_t n, const double* __restrict__ cc, const double* __restrict__ x,
-                        double* __restrict__ out, Ions I) {
+                        double* __restrict__ out, Ions I, const double* __restrict__ oC,
+                        double* __restrict__ D, float* __restrict__ Df) {
   for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (size_t)gridDim.x * kBlock) {
     double o = 0.0;
     const double coef = cc[t], u = x[t];
@@ -1093,6 +1168,11 @@ __global__ void k_dcvec(size_t n, const double* __restrict__ cc, const double* _
       o += zcf2 * coef * exp(arg);
     }
     out[t] = o;
+    if (oC) {
+      const double v = oC[t] + o;
+      D[t] = v;
+      if (Df) Df[t] = (float)v;
+    }
   }
 }
 
@@ -1868,8 +1948,8 @@ void nresid7(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const 
 }
 
 void dcvec(hipStream_t s, size_t n, const double* cc, const double* x, double* out,
-           const Ions& ions) {
-  k_dcvec<<<nblocks(n), kBlock, 0, s>>>(n, cc, x, out, ions);
+           const Ions& ions, const double* oC, double* D, float* Df) {
+  k_dcvec<<<nblocks(n), kBlock, 0, s>>>(n, cc, x, out, ions, oC, D, Df);
 }
 
 template <class T>
@@ -1921,13 +2001,33 @@ void nrm1(hipStream_t s, int nx, int ny, int nz, const T* x, double* part, doubl
 
 template <class T>
 void resid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
-                 const T* x, T* r, double* part, double* out, double* hout) {
-  k_resid7_nrm<T><<<kParts, kBlock, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, x, r, part);
+                 const T* x, T* r, double* part, double* out, double* hout, float* r32) {
+  k_resid7_nrm<T><<<kParts, kBlock, 0, s>>>(nx, ny, nz, A.a[1], A.a[2], A.a[3], D, fc, x, r, r32,
+                                            part);
   {
     Batch<FinishArgs> F;
     F.v[0] = FinishArgs{part, out, hout};
     k_finish<<<1, kBlock, 0, s>>>(F, 0);
   }
+}
+
+void nrm1_to32(hipStream_t s, int nx, int ny, int nz, const double* x, float* y, double* part,
+               double* out, double* hout) {
+  k_nrm1_to32<<<kParts, kBlock, 0, s>>>(nx, ny, nz, x, y, part);
+  Batch<FinishArgs> F;
+  F.v[0] = FinishArgs{part, out, hout};
+  k_finish<<<1, kBlock, 0, s>>>(F, 0);
+}
+
+void nresid7_xpay_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A,
+                       const double* cc, const double* fc, const double* x, double a,
+                       const double* z, double* y, double* r, const Ions& ions, double* part,
+                       double* out, double* hout) {
+  k_nresid7_xpay<<<kParts, kBlock, 0, s>>>(nx, ny, nz, A.a[0], A.a[1], A.a[2], A.a[3], cc, fc, x,
+                                           a, z, y, r, ions, part);
+  Batch<FinishArgs> F;
+  F.v[0] = FinishArgs{part, out, hout};
+  k_finish<<<1, kBlock, 0, s>>>(F, 0);
 }
 
 void nresid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* cc,
@@ -2027,8 +2127,9 @@ void add_diag_levels(hipStream_t s, const DiagLevels& L) {
 }
 
 void axpy_interior_f32(hipStream_t s, int nx, int ny, int nz, double a, const float* x,
-                       double* y) {
-  k_axpy_int_f32<<<grid_interior(nx, ny, nz), kBlock, 0, s>>>(nx, ny, nz, a, x, y);
+                       double* y, bool assign) {
+  k_axpy_int_f32<<<grid_interior(nx, ny, nz), kBlock, 0, s>>>(nx, ny, nz, a, x, y,
+                                                               assign ? 1 : 0);
 }
 
 void scale(hipStream_t s, size_t n, double a, const double* x, double* y) {
@@ -2186,7 +2287,7 @@ void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int 
                                const double*);                                                  \
   template void nrm1<T>(hipStream_t, int, int, int, const T*, double*, double*, double*);       \
   template void resid7_nrm1<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*,   \
-                               const T*, T*, double*, double*, double*);                        \
+                               const T*, T*, double*, double*, double*, float*);                \
   template void damp_factor<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*,   \
                                const T*, double*, double*);                                     \
   template void fill<T>(hipStream_t, T*, size_t, T);                                            \

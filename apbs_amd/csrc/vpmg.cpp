@@ -1224,21 +1224,23 @@ void build_operators(PmgDev* d) {
 }
 
 // D_l = oC_l + c_l for the given per-level Helmholtz arrays (and, for a
-// mixed-precision solve, its FP32 image Df_l in the same pass).
-void diag_levels(PmgDev* d, bool use_cprime) {
+// mixed-precision solve, its FP32 image Df_l in the same pass), levels first
+// and up.
+void diag_levels(PmgDev* d, bool use_cprime, int first) {
   const bool f32 = d->precision == 1 && d->h_mixed;
   pmg::DiagLevels dl{};
-  dl.nl = d->ntot;
+  dl.nl = d->ntot - first;
   dl.off[0] = 0;
-  for (int l = 0; l < d->ntot; l++) {
+  for (int q = 0; q < dl.nl; q++) {
+    const int l = first + q;
     DLevel& L = d->L[l];
-    dl.oC[l] = L.a[0];
-    dl.c[l] = use_cprime ? L.cprime : L.cc;
-    dl.D[l] = L.D;
-    dl.Df[l] = f32 ? L.Df : nullptr;
-    dl.off[l + 1] = dl.off[l] + L.n;
+    dl.oC[q] = L.a[0];
+    dl.c[q] = use_cprime ? L.cprime : L.cc;
+    dl.D[q] = L.D;
+    dl.Df[q] = f32 ? L.Df : nullptr;
+    dl.off[q + 1] = dl.off[q] + L.n;
   }
-  pmg::add_diag_levels(d->s, dl);
+  if (dl.nl > 0) pmg::add_diag_levels(d->s, dl);
 }
 
 pmg::InvertArgs invert_args(PmgDev* d) {
@@ -1246,9 +1248,10 @@ pmg::InvertArgs invert_args(PmgDev* d) {
   return pmg::InvertArgs{C.ops(), C.D, d->dense, d->dwork, d->hscal + kPivotFlag};
 }
 
-// The diagonals, then the dense inverse of the coarsest level.
-void set_diagonals(PmgDev* d, bool use_cprime) {
-  diag_levels(d, use_cprime);
+// The diagonals (levels first and up; the Newton Jacobian forms the fine one
+// with N'(u)), then the dense inverse of the coarsest level.
+void set_diagonals(PmgDev* d, bool use_cprime, int first = 0) {
+  diag_levels(d, use_cprime, first);
   const DLevel& C = d->L[d->ntot - 1];
   // The inverse is first read by the coarsest solve, after the downward
   // half of the next V-cycle; a one-workgroup kernel, it runs on its own
@@ -1274,14 +1277,14 @@ void set_diagonals(PmgDev* d, bool use_cprime) {
 
 // set_diagonals for n calculations of one shape on the first one's stream:
 // the dense inverses of all n in one launch.
-void set_diagonals_batch(PmgDev* const* ds, int n, bool use_cprime) {
+void set_diagonals_batch(PmgDev* const* ds, int n, bool use_cprime, int first = 0) {
   if (n == 1) {
-    set_diagonals(ds[0], use_cprime);
+    set_diagonals(ds[0], use_cprime, first);
     return;
   }
   pmg::InvertArgs ia[pmg::kMaxBatch];
   for (int b = 0; b < n; b++) {
-    diag_levels(ds[b], use_cprime);
+    diag_levels(ds[b], use_cprime, first);
     ia[b] = invert_args(ds[b]);
   }
   const DLevel& C = ds[0]->L[ds[0]->ntot - 1];
@@ -1555,14 +1558,17 @@ void mvcs64_batch(PmgDev* const* ds, int n, double* const* const* X, const doubl
 
 // Mixed precision for n calculations in lockstep: FP32 V-cycles on the
 // correction equation inside an FP64 defect-correction loop (fine residual,
-// update and stopping test in FP64).  outer[b] gets the outer iterations, or
-// -1 when that solve failed (solve_failed).
+// update and stopping test in FP64).  Each residual is written straight to
+// the FP32 right-hand side of the next cycle (L[0].w0f), the values a
+// conversion of the FP64 residual would give.  With x_zero the solution
+// starts at zero and is not read: the first correction is assigned to it.
+// outer[b] gets the outer iterations, or -1 when that solve failed
+// (solve_failed).
 void mixed_batch(PmgDev* const* ds, int n, double* const* const* X64, const double* const* f,
                  int istop, int itmax, const double* errtol, double* relres, int* outer,
                  bool x_zero) {
   bool active[pmg::kMaxBatch], failed[pmg::kMaxBatch] = {};
   double rsden[pmg::kMaxBatch], fnrm[pmg::kMaxBatch], rsnrm[pmg::kMaxBatch];
-  const double* r[pmg::kMaxBatch];
   int it[pmg::kMaxBatch];
   for (int b = 0; b < n; b++) {
     active[b] = true;
@@ -1570,11 +1576,16 @@ void mixed_batch(PmgDev* const* ds, int n, double* const* const* X64, const doub
     fnrm[b] = -1.0;
     it[b] = 0;
   }
+  // the first residual: f itself when the solution starts at zero (f + A 0
+  // rounds to f, and its FP32 image comes with its norm), else computed
   if (istop == 1 || x_zero) {
     for (int b = 0; b < n; b++) {
-      const DLevel& F = ds[b]->L[0];
-      pmg::nrm1<double>(ds[b]->s, F.nx, F.ny, F.nz, f[b], ds[b]->part, ds[b]->scal,
-                        ds[b]->hscal);
+      PmgDev* d = ds[b];
+      const DLevel& F = d->L[0];
+      if (x_zero)
+        pmg::nrm1_to32(d->s, F.nx, F.ny, F.nz, f[b], d->L[0].w0f, d->part, d->scal, d->hscal);
+      else
+        pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, f[b], d->part, d->scal, d->hscal);
     }
     read_norms(ds, n, active, fnrm, failed);
     for (int b = 0; b < n; b++)
@@ -1584,17 +1595,14 @@ void mixed_batch(PmgDev* const* ds, int n, double* const* const* X64, const doub
     if (failed[b]) active[b] = false;
     if (rsden[b] == 0.0) rsden[b] = 1.0;
   }
-  // the first residual: f itself when the solution starts at zero (f + A 0
-  // rounds to f), else computed
   for (int b = 0; b < n; b++) {
-    r[b] = ds[b]->w1;
     if (x_zero) {
-      r[b] = f[b];
       rsnrm[b] = fnrm[b];
     } else if (active[b]) {
-      const DLevel& F = ds[b]->L[0];
-      pmg::resid7_nrm1<double>(ds[b]->s, F.nx, F.ny, F.nz, F.ops(), F.D, f[b], X64[b][0],
-                               ds[b]->w1, ds[b]->part, ds[b]->scal, ds[b]->hscal);
+      PmgDev* d = ds[b];
+      const DLevel& F = d->L[0];
+      pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, F.ops(), F.D, f[b], X64[b][0], nullptr,
+                               d->part, d->scal, d->hscal, d->L[0].w0f);
     }
   }
   if (!x_zero) read_norms(ds, n, active, rsnrm, failed);
@@ -1609,10 +1617,7 @@ void mixed_batch(PmgDev* const* ds, int n, double* const* const* X64, const doub
         continue;
       }
       PmgDev* d = ds[b];
-      const DLevel& F = d->L[0];
-      pmg::convert<double, float>(d->s, F.n, r[b], d->L[0].w0f);
-      r[b] = d->w1;
-      pmg::fill<float>(d->s, d->L[0].xfl, F.n, 0.0f);
+      pmg::fill<float>(d->s, d->L[0].xfl, d->L[0].n, 0.0f);
       cs[m] = args32(d);
       act[m++] = d;
     }
@@ -1622,15 +1627,20 @@ void mixed_batch(PmgDev* const* ds, int n, double* const* const* X64, const doub
       if (!active[b]) continue;
       PmgDev* d = ds[b];
       const DLevel& F = d->L[0];
-      // X64 += (double) e, in one pass
-      pmg::axpy_interior_f32(d->s, F.nx, F.ny, F.nz, 1.0, d->L[0].xfl, X64[b][0]);
+      // X64 += (double) e in one pass (X64 = 0 + (double) e the first time
+      // when it starts at zero)
+      pmg::axpy_interior_f32(d->s, F.nx, F.ny, F.nz, 1.0, d->L[0].xfl, X64[b][0],
+                             x_zero && it[b] == 0);
       it[b]++;
-      pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, F.ops(), F.D, f[b], X64[b][0], d->w1,
-                               d->part, d->scal, d->hscal);
+      pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, F.ops(), F.D, f[b], X64[b][0], nullptr,
+                               d->part, d->scal, d->hscal, d->L[0].w0f);
     }
     read_norms(ds, n, active, rsnrm, failed);
   }
   for (int b = 0; b < n; b++) {
+    // a solution that started at zero and took no correction is zero
+    if (x_zero && it[b] == 0)
+      pmg::fill<double>(ds[b]->s, X64[b][0], ds[b]->L[0].n, 0.0);
     if (failed[b]) {
       outer[b] = -1;
       continue;
@@ -1684,6 +1694,13 @@ void newton_batch(Vpmg* const* th, int n, double* relres, int* iters, int* inner
     pmg::nresid7_nrm1(d->s, F.nx, F.ny, F.nz, F.ops(), F.cc, F.fc, x, r, st[b].I, d->part, d->scal,
                       d->hscal);
   };
+  // the trial point wn1 = u + a xtmp and its nonlinear residual in F
+  auto nresid_at = [&](int b, double a, const double* xtmp) {
+    PmgDev* d = ds[b];
+    const DLevel& F = d->L[0];
+    pmg::nresid7_xpay_nrm1(d->s, F.nx, F.ny, F.nz, F.ops(), F.cc, F.fc, d->u, a, xtmp, d->wn1,
+                           d->F, st[b].I, d->part, d->scal, d->hscal);
+  };
   auto read = [&](bool NewtonState::*flag) {
     for (int b = 0; b < n; b++) act[b] = st[b].*flag && !st[b].failed;
     read_norms(ds, n, act, nv, bad);
@@ -1722,12 +1739,14 @@ void newton_batch(Vpmg* const* th, int n, double* relres, int* iters, int* inner
       PmgDev* d = ad[q];
       DLevel* L = d->L;
       st[idx[q]].iters++;
-      pmg::dcvec(d->s, d->n, L[0].cc, d->u, L[0].cprime, st[idx[q]].I);
+      // cprime and the fine diagonal oC + cprime (and its FP32 image) in one pass
+      pmg::dcvec(d->s, d->n, L[0].cc, d->u, L[0].cprime, st[idx[q]].I, L[0].a[0], L[0].D,
+                 d->precision == 1 && d->h_mixed ? L[0].Df : nullptr);
       for (int l = 1; l < d->ntot; l++)
         pmg::restrict3<double>(d->s, L[l - 1].nx, L[l - 1].ny, L[l - 1].nz, L[l].nx, L[l].ny,
                                L[l].nz, L[l - 1].cprime, L[l].cprime);
     }
-    set_diagonals_batch(ad, m, true);
+    set_diagonals_batch(ad, m, true, 1);
     // inner solve: the V-cycle clears the coarse corrections itself; the
     // fine one starts at zero
     double errs[pmg::kMaxBatch], rr[pmg::kMaxBatch];
@@ -1739,7 +1758,9 @@ void newton_batch(Vpmg* const* th, int n, double* relres, int* iters, int* inner
       NewtonState& S = st[idx[q]];
       errs[q] = std::min(0.9 * S.xnorm_old, 1.0 * pow(S.xnorm_old, 2.0));
       for (int l = 0; l < d->ntot; l++) X[q][l] = d->L[l].xtmp;
-      pmg::fill<double>(d->s, X[q][0], d->L[0].n, 0.0);
+      // the FP64 cycles start from a zero fine correction; the mixed solve
+      // assigns its first correction (x_zero) instead of adding it to zero
+      if (d->precision != 1) pmg::fill<double>(d->s, X[q][0], d->L[0].n, 0.0);
       Xp[q] = X[q];
       fp[q] = d->F;
       if (d->precision == 1 && S.iters == 1) sync_fp32_ops(d);
@@ -1768,8 +1789,7 @@ void newton_batch(Vpmg* const* th, int n, double* relres, int* iters, int* inner
       const DLevel& F = d->L[0];
       double* xtmp = F.xtmp;
       if (S.idamp == 1) {
-        pmg::xpay_interior(d->s, F.nx, F.ny, F.nz, d->u, 1.0, xtmp, d->wn1);
-        nresid(b, d->wn1, d->F);
+        nresid_at(b, 1.0, xtmp);
       } else {
         pmg::axpy_interior<double>(d->s, F.nx, F.ny, F.nz, 1.0, xtmp, d->u);
         nresid(b, d->u, d->F);
@@ -1827,8 +1847,7 @@ void newton_batch(Vpmg* const* th, int n, double* relres, int* iters, int* inner
         std::swap(d->F, d->wn3);
         S.xnorm_med = S.xnorm_new;
         S.damp = S.damp / 2.0;
-        pmg::xpay_interior(d->s, F.nx, F.ny, F.nz, d->u, S.damp, F.xtmp, d->wn1);
-        nresid(b, d->wn1, d->F);
+        nresid_at(b, S.damp, F.xtmp);
         any = true;
       }
       if (!any) break;
