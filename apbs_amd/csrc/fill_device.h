@@ -125,9 +125,11 @@ void bc_sdh(hipStream_t s, const Geo& g, const double* xf, const double* yf, con
             const SdhMoments& M, double* gx, double* gy, double* gz);
 // Doubles of scratch bc_mdh needs for natoms atoms.
 size_t bc_mdh_scratch(const Geo& g, int natoms);
+// fp32_terms: each atom-point term in FP32 (hardware rsqrt and 2^x), summed
+// in FP64; otherwise FP64 terms within a few ulp of the reference's.
 void bc_mdh(hipStream_t s, const Geo& g, const double* xf, const double* yf, const double* zf,
             const AtomsDev& A, double pre1, double xkappa, double* part, double* gx, double* gy,
-            double* gz);
+            double* gz, bool fp32_terms = false);
 // --------------------------------------------------------------- maps
 // An external map on the device or the host (the fields of a Vgrid).
 struct MapDev {

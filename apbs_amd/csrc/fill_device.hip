@@ -802,6 +802,65 @@ __global__ void __launch_bounds__(kB)
   part[((size_t)c * 6 + f) * mx + t] = val;
 }
 
+// The same sum from FP32 terms, added in FP64 (the fast-boundary mode of the
+// mixed-precision solve).  Atom and boundary coordinates are taken relative
+// to the box centre before rounding to FP32, so a distance carries an
+// absolute error of a few 1e-6 A; 1 / d is the hardware reciprocal square
+// root and exp(-kappa d) the hardware 2^x of -kappa log2(e) d, each about
+// one ulp.  A term is within about 1e-6 relative of the FP64 term (the
+// exponent's argument, at most a few tens, dominates); the partial sums and
+// their order are those of k_bc_mdh.
+__global__ void __launch_bounds__(kB)
+    k_bc_mdh_atoms32(AtomsDev A, double pre1, double xkappa, double cx, double cy, double cz,
+                     float4* at) {
+  const int a = blockIdx.x * kB + threadIdx.x;
+  if (a >= A.n) return;
+  const double ec = 1.6021773e-19;  // Vunit_ec, src/generic/vunit.h:80
+  const double q = ec * A.chg[a];
+  double fac = pre1 * q;
+  if (xkappa > VSMALL) {
+    const double r = A.rad[a];
+    fac = pre1 * q * exp(xkappa * r) / (1 + xkappa * r);
+  }
+  const double* ap = A.pos + 3 * a;
+  at[a] = make_float4((float)(ap[0] - cx), (float)(ap[1] - cy), (float)(ap[2] - cz), (float)fac);
+}
+
+__global__ void __launch_bounds__(kB)
+    k_bc_mdh32(Geo g, const double* xf, const double* yf, const double* zf, double cx, double cy,
+               double cz, int natoms, const float4* at, float nk2, int nch, int mx,
+               double* part) {
+  const int f = blockIdx.y;
+  const int t = blockIdx.x * kB + threadIdx.x;
+  if (t >= face_count(g, f)) return;
+  int i, j, k;
+  face_point(g, f, t, i, j, k);
+  const float px = (float)(xf[i] - cx), py = (float)(yf[j] - cy), pz = (float)(zf[k] - cz);
+  const int c = blockIdx.z;
+  const int per = (natoms + nch - 1) / nch;
+  const int a0 = c * per, a1 = min(natoms, a0 + per);
+  const bool screened = nk2 != 0.0f;
+  auto term = [&](int a) {
+    const float4 q = at[a];
+    const float dx = px - q.x, dy = py - q.y, dz = pz - q.z;
+    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float y = __builtin_amdgcn_rsqf(d2);
+    const float v = q.w * y;
+    return screened ? v * __builtin_amdgcn_exp2f(nk2 * (d2 * y)) : v;
+  };
+  double val = 0.0;
+  int a = a0;
+  for (; a + 3 < a1; a += 4) {
+    const float t0 = term(a), t1 = term(a + 1), t2 = term(a + 2), t3 = term(a + 3);
+    val += (double)t0;
+    val += (double)t1;
+    val += (double)t2;
+    val += (double)t3;
+  }
+  for (; a < a1; a++) val += (double)term(a);
+  part[((size_t)c * 6 + f) * mx + t] = val;
+}
+
 __global__ void __launch_bounds__(kB)
     k_bc_mdh_sum(Geo g, int nch, int mx, const double* part, double* gx, double* gy, double* gz) {
   const int f = blockIdx.y;
@@ -1494,21 +1553,33 @@ size_t bc_mdh_scratch(const Geo& g, int natoms) {
   size_t mx = (size_t)g.ny * g.nz;
   if ((size_t)g.nx * g.nz > mx) mx = (size_t)g.nx * g.nz;
   if ((size_t)g.nx * g.ny > mx) mx = (size_t)g.nx * g.ny;
-  return (size_t)bc_mdh_chunks(natoms) * 6 * mx + (size_t)(natoms > 0 ? natoms : 1);
+  // partial sums, then the per-atom factors (FP64) or float4 atoms (two doubles each)
+  return (size_t)bc_mdh_chunks(natoms) * 6 * mx + 2 * (size_t)(natoms > 0 ? natoms : 1);
 }
 
 void bc_mdh(hipStream_t s, const Geo& g, const double* xf, const double* yf, const double* zf,
             const AtomsDev& A, double pre1, double xkappa, double* part, double* gx, double* gy,
-            double* gz) {
+            double* gz, bool fp32_terms) {
   int mx = g.ny * g.nz;
   if (g.nx * g.nz > mx) mx = g.nx * g.nz;
   if (g.nx * g.ny > mx) mx = g.nx * g.ny;
   const int nch = bc_mdh_chunks(A.n);
-  // the per-atom factors follow the partial sums in the scratch buffer
+  // the per-atom data follow the partial sums in the scratch buffer (6 * mx
+  // * nch doubles: an even count, so the float4 array is 16-byte aligned)
   double* fac = part + (size_t)nch * 6 * mx;
-  if (A.n > 0) k_bc_mdh_atoms<<<(A.n + kB - 1) / kB, kB, 0, s>>>(A, pre1, xkappa, fac);
-  k_bc_mdh<<<dim3((mx + kB - 1) / kB, 6, nch), kB, 0, s>>>(g, xf, yf, zf, A, fac, xkappa, nch,
-                                                           mx, part);
+  const dim3 grid((mx + kB - 1) / kB, 6, nch);
+  if (fp32_terms) {
+    const double cx = 0.5 * (g.xmin + g.xmax), cy = 0.5 * (g.ymin + g.ymax),
+                 cz = 0.5 * (g.zmin + g.zmax);
+    float4* at = reinterpret_cast<float4*>(fac);
+    if (A.n > 0)
+      k_bc_mdh_atoms32<<<(A.n + kB - 1) / kB, kB, 0, s>>>(A, pre1, xkappa, cx, cy, cz, at);
+    const float nk2 = xkappa > VSMALL ? (float)(-xkappa * 1.4426950408889634) : 0.0f;
+    k_bc_mdh32<<<grid, kB, 0, s>>>(g, xf, yf, zf, cx, cy, cz, A.n, at, nk2, nch, mx, part);
+  } else {
+    if (A.n > 0) k_bc_mdh_atoms<<<(A.n + kB - 1) / kB, kB, 0, s>>>(A, pre1, xkappa, fac);
+    k_bc_mdh<<<grid, kB, 0, s>>>(g, xf, yf, zf, A, fac, xkappa, nch, mx, part);
+  }
   k_bc_mdh_sum<<<dim3((mx + kB - 1) / kB, 6), kB, 0, s>>>(g, nch, mx, part, gx, gy, gz);
 }
 

@@ -65,7 +65,7 @@ size_t footprint_of(const PmgRequest& r) {
     nref = (int)std::ceil(4.0 * kVPI * mr * mr * r.sdens);
   }
   return pmgint::footprint_bytes(r.dime[0], r.dime[1], r.dime[2], nlev, r.npbe != 0,
-                                 r.precision == 1, r.natoms, nref);
+                                 r.precision != 0, r.natoms, nref);
 }
 
 // Cross-request batching of the solves.  A worker whose calculation is
@@ -193,7 +193,8 @@ void run_job(Job& j, PmgResult& res, SolveGate* gate) {
     if (!Vpmg_fillco(pmg, (Vsurf_Meth)r.srfm, r.swin, (Vchrg_Meth)r.chgm, 0, nullptr, 0, nullptr,
                      0, nullptr, 0, nullptr, 0, nullptr, 0, nullptr))
       break;
-    if (!(gate ? gate->solve(pmg, r.precision) : Vpmg_solve(pmg))) break;
+    // modes 1 and 2 solve alike (2 differs in fillco only)
+    if (!(gate ? gate->solve(pmg, r.precision ? 1 : 0) : Vpmg_solve(pmg))) break;
     double lo[3], hi[3];
     for (int k = 0; k < 3; k++) {
       lo[k] = m->center[k] - 0.5 * m->glen[k];
@@ -299,7 +300,9 @@ extern "C" PmgServer* PmgServer_ctor(int slots, size_t hbm_budget) {
 }
 
 extern "C" long PmgServer_submit(PmgServer* s, const PmgRequest* req) {
-  if (!s || !req || req->natoms < 0 || req->nion < 0 || req->nion > APBS_PMG_MAXION) return -1;
+  if (!s || !req || req->natoms < 0 || req->nion < 0 || req->nion > APBS_PMG_MAXION ||
+      req->precision < 0 || req->precision > 2)
+    return -1;
   for (int k = 0; k < 3; k++)
     if (req->dime[k] < 5) return -1;
   Job* j = new Job();

@@ -718,6 +718,8 @@ struct PmgDev {
   // options / stats
   int precision = 0;
   double mixed_tol = -1.0;
+  // precision mode 2: the MDH boundary from FP32 terms (set before fillco)
+  bool fast_boundary = false;
   // fine-level smoother: 0 fused out-of-place sweep, 1 one launch per colour
   // (environment APBS_PMG_SMOOTHER=colour selects 1)
   int smoother = 0;
@@ -2345,7 +2347,7 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
     // bcflnew (vpmg.c:3289-3367)
     side.part = dalloc<double>(pmg::bc_mdh_scratch(g, d->natoms));
     pmg::bc_mdh(side.s2, g, d->xf, d->yf, d->zf, A, pre1, pbe->xkappa, side.part, d->gx, d->gy,
-                d->gz);
+                d->gz, d->fast_boundary);
     HIPCHECK(hipEventCreateWithFlags(&side.done, hipEventDisableTiming));
     HIPCHECK(hipEventRecord(side.done, side.s2));
   }
@@ -2473,7 +2475,8 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
     if (p->bcfl == BCFL_MDH) {
       // bcflnew (vpmg.c:3289-3367)
       double* part = dalloc<double>(pmg::bc_mdh_scratch(g, d->natoms));
-      pmg::bc_mdh(d->s, g, d->xf, d->yf, d->zf, A, pre1, pbe->xkappa, part, d->gx, d->gy, d->gz);
+      pmg::bc_mdh(d->s, g, d->xf, d->yf, d->zf, A, pre1, pbe->xkappa, part, d->gx, d->gy, d->gz,
+                  d->fast_boundary);
       HIPCHECK(hipStreamSynchronize(d->s));
       dfree(part);
     } else if (p->bcfl == BCFL_SDH) {
@@ -3246,9 +3249,10 @@ extern "C" unsigned long int Vpmg_memChk(Vpmg* thee) {
 // ============================================================= extensions
 
 extern "C" int Vpmg_setPrecision(Vpmg* thee, int mode, double tol) {
-  if (!thee || (mode != 0 && mode != 1)) return 0;
+  if (!thee || mode < 0 || mode > 2) return 0;
   PmgDev* d = DEV(thee);
-  d->precision = mode;
+  d->precision = mode ? 1 : 0;
+  d->fast_boundary = mode == 2;
   d->mixed_tol = tol;
   return 1;
 }

@@ -158,10 +158,16 @@ class Calc:
         if not self.pmg:
             raise RuntimeError("Vpmg_ctor failed")
         if precision:
-            L.Vpmg_setPrecision(self.pmg, precision, mixed_tol)
+            self.set_precision(precision, mixed_tol)
         L.Vpmg_setHostMirror(self.pmg, 1 if mirror else 0)
         self.srfm = srfm
         self.chgm = CHGM[pbe["chgm"]]
+
+    def set_precision(self, precision, mixed_tol=-1.0):
+        """Vpmg_setPrecision: 0 FP64, 1 mixed, 2 mixed with the MDH boundary
+        from FP32 terms (takes effect at the next fillco)."""
+        if not lib().Vpmg_setPrecision(self.pmg, int(precision), float(mixed_tol)):
+            raise ValueError(f"precision {precision}: 0 FP64, 1 mixed, 2 mixed + FP32 MDH terms")
 
     def fillco(self):
         """Vpmg_fillco; pbe["maps"] may name Map objects under MAP_KEYS

@@ -375,8 +375,13 @@ unsigned long int Vpmg_memChk(Vpmg *thee);     /* src/mg/vpmg.h:208 */
 /* Precision of the multigrid solve: 0 = FP64 throughout (default, the
  * reference arithmetic); 1 = FP32 V-cycles (smoother, transfers, coarse
  * operators) inside an FP64 defect-correction loop whose fine residual,
- * coarsest solve, Newton update and energy/force sums stay FP64.  tol is the
- * relative FP64 residual the mixed loop stops at (<= 0: pmgp->errtol). */
+ * coarsest solve, Newton update and energy/force sums stay FP64; 2 = mode 1,
+ * and a later Vpmg_fillco evaluates each atom's term of the MDH boundary
+ * (bcfl mdh) in FP32 and sums the terms in FP64 (boundary values within
+ * about 1e-6 relative of the FP64 ones; the problem solved changes by that
+ * much, so the result no longer equals the FP64 solve to the solver
+ * tolerance).  tol is the relative FP64 residual the mixed loop stops at
+ * (<= 0: pmgp->errtol).  Returns 0 for another mode. */
 int Vpmg_setPrecision(Vpmg *thee, int mode, double tol);
 /* Vpmg_solve for n calculations at once: consecutive runs of up to 8
  * calculations with the same grid, level count, method (lpbe or npbe) and

@@ -40,7 +40,7 @@ typedef struct PmgRequest {
     double pdie, sdie, srad, swin, sdens, temp;
     int srfm, chgm, bcfl;   /* Vsurf_Meth, Vchrg_Meth, Vbcfl */
     double etol;            /* <= 0: 1e-6 */
-    int precision;          /* 0 FP64, 1 mixed (Vpmg_setPrecision) */
+    int precision;          /* 0 FP64, 1 mixed, 2 mixed + FP32 MDH terms (Vpmg_setPrecision) */
     double mixed_tol;       /* <= 0: etol */
     double *potential;      /* optional host buffer of dime[0]*dime[1]*dime[2] doubles,
                                filled before the result is posted */

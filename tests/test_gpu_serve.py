@@ -29,7 +29,7 @@ def make_req(s, precision, tag=0):
                          precision=precision, want_potential=True, tag=tag)
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_alone_equals_in_mix(precision):
     specs = mix_specs()
     alone = []
