@@ -174,6 +174,32 @@ def smoother_roofline(kt):
             "bytes_per_launch": bytes_per_launch, "avg_launch_us": per_launch_ms * 1e3}
 
 
+LARGE_GRID = 193
+LARGE_SWEEPS = 20
+
+
+def large_grid_roofline(n=LARGE_GRID, sweeps=LARGE_SWEEPS):
+    """The fused sweep on a grid whose fine arrays do not fit the 256 MB
+    Infinity Cache (193^3: 7.2M points, 201 MB per FP32 sweep, 403 MB per
+    FP64 sweep): back-to-back launches on a synthetic operator, timed with
+    HIP events around the launches (pmgtest_fused_sweeps), 7 words per point."""
+    from apbs_amd import lib
+    import ctypes
+    L = lib()
+    L.pmgtest_fused_sweeps.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_double)]
+    us = ctypes.c_double()
+    out = {}
+    for word, name in ((4, "fp32"), (8, "fp64")):
+        if L.pmgtest_fused_sweeps(n, word, sweeps, 0, ctypes.byref(us)) != 1:
+            raise RuntimeError("pmgtest_fused_sweeps failed")
+        b = 7 * word * n ** 3
+        gbs = b / (us.value * 1e-6) / 1e9
+        out[name] = {"grid": n, "avg_launch_us": round(us.value, 2), "bytes_per_launch": b,
+                     "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def vcycle_bandwidth(kt, cfg, levels):
     """Achieved GB/s per V-cycle against the algorithmic V-cycle bytes
     w * (44 N0 + 106 * sum of the intermediate levels' points); the coarsest
@@ -373,6 +399,7 @@ def main():
         "roofline": smoother_roofline(kt),
         "roofline_fp64": smoother_roofline(out["fp64"]["kernel"]) if "fp64" in out else None,
         "vcycle": vcycle_bandwidth(kt, cfg, out[head]["levels"]),
+        "roofline_large_grid": large_grid_roofline(),
         "served_stream": stream,
     }
     line["cpu_baseline"] = None if a.no_cpu_baseline else cpu_baseline(cfg)
