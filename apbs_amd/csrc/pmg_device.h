@@ -124,6 +124,15 @@ void nresid7_xpay_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>&
                        const double* cc, const double* fc, const double* x, double a,
                        const double* z, double* y, double* r, const Ions& ions, double* part,
                        double* out, double* hout = nullptr);
+// resid7_nrm1 of the Jacobian system at z (r stored only as its FP32 image
+// r32, when given) and nresid7_xpay_nrm1 at y = x + a z, in one pass, bitwise
+// the two calls; part holds 2 * kParts partial sums, out / hout receive the
+// first norm, out2 / hout2 the second
+void jres_trial_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A,
+                     const double* D, const double* cc, const double* fc, const double* fz,
+                     const double* x, double a, const double* z, double* y, double* rn,
+                     float* r32, const Ions& ions, double* part, double* out, double* hout,
+                     double* out2, double* hout2);
 // nresid7 followed by nrm1 of its result, in one pass.
 void nresid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* cc,
                   const double* fc, const double* x, double* r, const Ions& ions, double* part,

@@ -1145,6 +1145,62 @@ __global__ void __launch_bounds__(kBlock)
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// The end of a Newton step's inner cycle and the step's first trial point in
+// one pass.  With z the correction the cycle produced (its boundary layer
+// zero): the inner residual fz + J z of the Jacobian system (J's diagonal D),
+// its FP32 image in r32 and its L1 partials in part[0, kParts)
+// (k_resid7_nrm<double>'s values and order); and y = x + a z with the
+// nonlinear residual at y in rn, its L1 partials in part[kParts, 2 kParts)
+// (k_nresid7_xpay's values and order).  The two stencils share their operator
+// loads.
+__global__ void __launch_bounds__(kBlock)
+    k_jres_trial(int nx, int ny, int nz, const double* __restrict__ oC,
+                 const double* __restrict__ oE, const double* __restrict__ oN,
+                 const double* __restrict__ uC, const double* __restrict__ D,
+                 const double* __restrict__ cc, const double* __restrict__ fc,
+                 const double* __restrict__ fz, const double* __restrict__ x, double a,
+                 const double* __restrict__ z, double* __restrict__ y, double* __restrict__ rn,
+                 float* __restrict__ r32, Ions ions, double* part) {
+  __shared__ double sh[kBlock];
+  const int ni = nx - 2, nij = ni * (ny - 2), n = nij * (nz - 2);
+  const size_t nxy = (size_t)nx * ny;
+  int t0, t1;
+  part_range(n, t0, t1);
+  auto yv = [&](size_t q) {
+    double v = x[q];
+    v += a * z[q];
+    return v;
+  };
+  double acc_j = 0.0, acc_n = 0.0;
+  for (int t = t0 + threadIdx.x; t < t1; t += kBlock) {
+    const int k = 1 + t / nij, rem = t - (k - 1) * nij;
+    const int j = 1 + rem / ni, i = 1 + rem - (j - 1) * ni;
+    const size_t p = IX(i, j, k, nx, ny);
+    const double vj = acc7(fz[p], oE, oN, uC, z, p, nx, nxy) - D[p] * z[p];
+    if (r32) r32[p] = (float)vj;
+    acc_j += fabs(vj);
+    const double y0 = yv(p);
+    y[p] = y0;
+    const double w = ncvec(cc[p], y0, ions);
+    double s7 = fc[p];
+    s7 += oN[p] * yv(p + nx);
+    s7 += oN[p - nx] * yv(p - nx);
+    s7 += oE[p] * yv(p + 1);
+    s7 += oE[p - 1] * yv(p - 1);
+    s7 += uC[p - nxy] * yv(p - nxy);
+    s7 += uC[p] * yv(p + nxy);
+    const double v = s7 - oC[p] * y0 - w;
+    rn[p] = v;
+    acc_n += fabs(v);
+  }
+  const double sj = block_sum(acc_j, sh);
+  const double sn = block_sum(acc_n, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = sj;
+    part[kParts + blockIdx.x] = sn;
+  }
+}
+
 // Vdc_vecpmg (mypdec.c:366-433) at every point; with oC given also the
 // diagonal D = oC + N'(x) and, with Df, its FP32 image (k_add_diag_levels'
 // values for this level) in the same pass.
@@ -2028,6 +2084,19 @@ void nresid7_xpay_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>&
   Batch<FinishArgs> F;
   F.v[0] = FinishArgs{part, out, hout};
   k_finish<<<1, kBlock, 0, s>>>(F, 0);
+}
+
+void jres_trial_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A,
+                     const double* D, const double* cc, const double* fc, const double* fz,
+                     const double* x, double a, const double* z, double* y, double* rn,
+                     float* r32, const Ions& ions, double* part, double* out, double* hout,
+                     double* out2, double* hout2) {
+  k_jres_trial<<<kParts, kBlock, 0, s>>>(nx, ny, nz, A.a[0], A.a[1], A.a[2], A.a[3], D, cc, fc,
+                                         fz, x, a, z, y, rn, r32, ions, part);
+  Batch<FinishArgs> F;
+  F.v[0] = FinishArgs{part, out, hout};
+  F.v[1] = FinishArgs{part + kParts, out2, hout2};
+  k_finish<<<2, kBlock, 0, s>>>(F, 0);
 }
 
 void nresid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* cc,

@@ -1500,11 +1500,51 @@ void read_norms(PmgDev* const* ds, int n, const bool* active, double* out, bool*
   }
 }
 
+// A Newton step's first trial point, formed with the inner solve's residual
+// (jres_trial_nrm1): y = u + z for the correction z the cycle produced, rn the
+// nonlinear residual at y.  After the inner solve, ok[b] says whether
+// calculation b ran a cycle (so y and rn belong to its final correction) and
+// nrm[b] holds ||rn||1.
+struct Trials {
+  const double* u[pmg::kMaxBatch];
+  double* y[pmg::kMaxBatch];
+  double* rn[pmg::kMaxBatch];
+  const pmg::Ions* I[pmg::kMaxBatch];
+  double nrm[pmg::kMaxBatch];
+  bool ok[pmg::kMaxBatch];
+};
+
+// The inner residual f + J z of calculation b's fine level (its L1 norm to
+// hscal[0]; its FP32 image to r32 when given) and, with trials, its trial
+// point and nonlinear residual (norm to hscal[1]) in the same pass.
+void inner_resid(PmgDev* d, const double* f, const double* z, float* r32, Trials* tr, int b) {
+  const DLevel& F = d->L[0];
+  if (tr) {
+    pmg::jres_trial_nrm1(d->s, F.nx, F.ny, F.nz, F.ops(), F.D, F.cc, F.fc, f, tr->u[b], 1.0, z,
+                         tr->y[b], tr->rn[b], r32, *tr->I[b], d->part, d->scal, d->hscal,
+                         d->scal + 8, d->hscal + 1);
+    tr->ok[b] = true;
+  } else {
+    pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, F.ops(), F.D, f, z, nullptr, d->part,
+                             d->scal, d->hscal, r32);
+  }
+}
+
+// The trial norms of the calculations that computed a trial this round
+// (read after the stream synchronization of read_norms).
+void read_trials(PmgDev* const* ds, int n, const bool* ran, Trials* tr) {
+  if (!tr) return;
+  for (int b = 0; b < n; b++)
+    if (ran[b]) tr->nrm[b] = ds[b]->hscal[1];
+}
+
 // Vmvcs with istop 0 (absolute) or 1 (relative to ||f||1) for n calculations
 // in lockstep: each runs V-cycles until its own stopping test holds.  its[b]
-// gets the cycles, or -1 when that solve failed (solve_failed).
+// gets the cycles, or -1 when that solve failed (solve_failed).  With tr,
+// each cycle's residual also forms the Newton trial point (Trials).
 void mvcs64_batch(PmgDev* const* ds, int n, double* const* const* X, const double* const* f,
-                  int istop, int itmax, const double* errtol, double* relres, int* its) {
+                  int istop, int itmax, const double* errtol, double* relres, int* its,
+                  Trials* tr = nullptr) {
   bool active[pmg::kMaxBatch], failed[pmg::kMaxBatch] = {};
   double rsden[pmg::kMaxBatch], rsnrm[pmg::kMaxBatch];
   for (int b = 0; b < n; b++) {
@@ -1539,11 +1579,10 @@ void mvcs64_batch(PmgDev* const* ds, int n, double* const* const* X, const doubl
     for (int b = 0; b < n; b++) {
       if (!active[b]) continue;
       its[b]++;
-      const DLevel& F = ds[b]->L[0];
-      pmg::resid7_nrm1<double>(ds[b]->s, F.nx, F.ny, F.nz, F.ops(), F.D, f[b], X[b][0], nullptr,
-                               ds[b]->part, ds[b]->scal, ds[b]->hscal);
+      inner_resid(ds[b], f[b], X[b][0], nullptr, tr, b);
     }
     read_norms(ds, n, active, rsnrm, failed);
+    read_trials(ds, n, active, tr);
     for (int b = 0; b < n; b++) {
       if (!active[b]) continue;
       if (failed[b] || its[b] >= itmax || !((rsnrm[b] / rsden[b]) > errtol[b])) active[b] = false;
@@ -1568,7 +1607,7 @@ void mvcs64_batch(PmgDev* const* ds, int n, double* const* const* X, const doubl
 // (solve_failed).
 void mixed_batch(PmgDev* const* ds, int n, double* const* const* X64, const double* const* f,
                  int istop, int itmax, const double* errtol, double* relres, int* outer,
-                 bool x_zero) {
+                 bool x_zero, Trials* tr = nullptr) {
   bool active[pmg::kMaxBatch], failed[pmg::kMaxBatch] = {};
   double rsden[pmg::kMaxBatch], fnrm[pmg::kMaxBatch], rsnrm[pmg::kMaxBatch];
   int it[pmg::kMaxBatch];
@@ -1634,10 +1673,10 @@ void mixed_batch(PmgDev* const* ds, int n, double* const* const* X64, const doub
       pmg::axpy_interior_f32(d->s, F.nx, F.ny, F.nz, 1.0, d->L[0].xfl, X64[b][0],
                              x_zero && it[b] == 0);
       it[b]++;
-      pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, F.ops(), F.D, f[b], X64[b][0], nullptr,
-                               d->part, d->scal, d->hscal, d->L[0].w0f);
+      inner_resid(d, f[b], X64[b][0], d->L[0].w0f, tr, b);
     }
     read_norms(ds, n, active, rsnrm, failed);
+    read_trials(ds, n, active, tr);
   }
   for (int b = 0; b < n; b++) {
     // a solution that started at zero and took no correction is zero
@@ -1767,12 +1806,23 @@ void newton_batch(Vpmg* const* th, int n, double* relres, int* iters, int* inner
       fp[q] = d->F;
       if (d->precision == 1 && S.iters == 1) sync_fp32_ops(d);
     }
+    // the inner solve's last residual also forms each step's first trial
+    // point u + xtmp (wn1) and its nonlinear residual (wn3)
+    Trials tr;
+    for (int q = 0; q < m; q++) {
+      PmgDev* d = ad[q];
+      tr.u[q] = d->u;
+      tr.y[q] = d->wn1;
+      tr.rn[q] = d->wn3;
+      tr.I[q] = &st[idx[q]].I;
+      tr.ok[q] = false;
+    }
     // calculations of one batch share the precision (batches are formed by
     // shape and precision); each has its own inner tolerance
     if (ad[0]->precision == 1)
-      mixed_batch(ad, m, Xp, fp, 0, 1000, errs, rr, its, true);
+      mixed_batch(ad, m, Xp, fp, 0, 1000, errs, rr, its, true, &tr);
     else
-      mvcs64_batch(ad, m, Xp, fp, 0, 1000, errs, rr, its);
+      mvcs64_batch(ad, m, Xp, fp, 0, 1000, errs, rr, its, &tr);
     for (int q = 0; q < m; q++) {
       NewtonState& S = st[idx[q]];
       if (its[q] < 0) {
@@ -1782,7 +1832,11 @@ void newton_batch(Vpmg* const* th, int n, double* relres, int* iters, int* inner
       }
       S.inner += its[q];
     }
-    // first trial point (damped) or the full step
+    // first trial point (damped) or the full step: u + xtmp and its residual
+    // as the inner solve left them in wn1 and wn3 (the buffers are exchanged:
+    // all have a zero boundary layer and are written on the interior only),
+    // or formed here when the inner solve ran no cycle
+    bool formed[pmg::kMaxBatch] = {}, need_read = false;
     for (int q = 0; q < m; q++) {
       const int b = idx[q];
       NewtonState& S = st[b];
@@ -1790,6 +1844,18 @@ void newton_batch(Vpmg* const* th, int n, double* relres, int* iters, int* inner
       PmgDev* d = ds[b];
       const DLevel& F = d->L[0];
       double* xtmp = F.xtmp;
+      if (tr.ok[q]) {
+        formed[q] = true;
+        std::swap(d->F, d->wn3);
+        if (S.idamp != 1) {
+          std::swap(d->u, d->wn1);
+          d->L[0].x = d->u;  // the fine level's solution is the potential
+        }
+        nv[b] = tr.nrm[q];
+        if (solve_failed(d, nv[b])) S.failed = true;
+        continue;
+      }
+      need_read = true;
       if (S.idamp == 1) {
         nresid_at(b, 1.0, xtmp);
       } else {
@@ -1797,7 +1863,13 @@ void newton_batch(Vpmg* const* th, int n, double* relres, int* iters, int* inner
         nresid(b, d->u, d->F);
       }
     }
-    read(&NewtonState::active);
+    if (need_read) {
+      double keep[pmg::kMaxBatch];
+      for (int q = 0; q < m; q++) keep[q] = nv[idx[q]];
+      read(&NewtonState::active);
+      for (int q = 0; q < m; q++)
+        if (formed[q]) nv[idx[q]] = keep[q];
+    }
     for (int q = 0; q < m; q++) {
       NewtonState& S = st[idx[q]];
       if (!S.active) continue;
