@@ -521,7 +521,7 @@ __global__ void __launch_bounds__(kB)
 // sphere gets byte 1 in its target's flag map.  Every writer of a byte writes
 // the same value, so the result does not depend on the order of the writes.
 // One 64-lane wavefront per sphere; for each target the lanes cover the
-// sphere's bounding box and test every point of it.  The point set is the
+// sphere's bounding box column by column and test every point of it.  The point set is the
 // reference loop's: its row skip ((dx2 + dy2) > rtot2) only drops points
 // that also fail the final test, because rounding is monotone.
 constexpr int kMarkWaves = kB / 64;
@@ -569,57 +569,22 @@ __global__ void __launch_bounds__(kB)
     const int kmax = imin2(g.nz - 1, (int)floor((posz + zr) / g.hz));
     const int ni = imax - imin + 1, nj = jmax - jmin + 1, nk = kmax - kmin + 1;
     if (ni <= 0 || nj <= 0 || nk <= 0) continue;
-    if (ni <= 64 && nj <= 64 && nk <= 64) {
-      // Lane (ii, rr) = (lane % ni, lane / ni) keeps its column ii and walks
-      // the rows rr, rr + R, ... (R = 64 / ni rows per step).  The squared
-      // offsets of column ii, row jj and plane kk are each computed once by
-      // lane ii, jj or kk and fetched by shuffle; the sum keeps the
-      // reference's order (dz2 + dy2) + dx2.
-      const double fl_i = imin + lane, fl_j = jmin + lane, fl_k = kmin + lane;
-      const double dxl = (posx - g.hx * fl_i) * (posx - g.hx * fl_i);
-      const double dyl = (posy - g.hy * fl_j) * (posy - g.hy * fl_j);
-      const double dzl = (posz - g.hz * fl_k) * (posz - g.hz * fl_k);
-      const int R = 64 / ni;
-      const int ii = lane % ni, rr = lane / ni;
-      const bool col = rr < R;
-      const double dx2 = __shfl(dxl, ii, 64);
-      const int nrows = nj * nk;
-      int jj = rr % nj, kk = rr / nj;
-      for (int row0 = 0; row0 < nrows; row0 += R) {
-        const double dy2 = __shfl(dyl, jj < nj ? jj : 0, 64);
-        const double dz2 = __shfl(dzl, kk < nk ? kk : 0, 64);
-        if (col && row0 + rr < nrows && (dz2 + dy2 + dx2) <= rtot2)
-          fl[IX(imin + ii, jmin + jj, kmin + kk, g.nx, g.ny)] = 1;
-        jj += R;
-        while (jj >= nj) {
-          jj -= nj;
-          kk++;
-        }
-      }
-      continue;
-    }
-    // wider boxes: lane e of the box walks e, e + 64, ...: the (i, j, k)
-    // offsets advance by 64 % ni columns and 64 / ni rows per step, carried
-    // without divisions
-    const int nbox = ni * nj * nk;
-    int ii = lane % ni, jj = (lane / ni) % nj, kk = lane / (ni * nj);
-    const int di = 64 % ni, dr = 64 / ni;
-    for (int e = lane; e < nbox; e += 64) {
-      const int i = imin + ii, j = jmin + jj, k = kmin + kk;
-      const double fi = i, fj = j, fk = k;
+    // Lane e of a panel of 64 (i, j) columns of the box keeps its column and
+    // walks the planes k: its column's squared offsets are formed once, the
+    // plane's per step, and the sum keeps the reference's order
+    // (dz2 + dy2) + dx2.
+    const int ncol = ni * nj;
+    const size_t nxy = (size_t)g.nx * g.ny;
+    for (int e = lane; e < ncol; e += 64) {
+      const int ii = e % ni, jj = e / ni;
+      const double fi = imin + ii, fj = jmin + jj;
       const double dx2 = (posx - g.hx * fi) * (posx - g.hx * fi);
       const double dy2 = (posy - g.hy * fj) * (posy - g.hy * fj);
-      const double dz2 = (posz - g.hz * fk) * (posz - g.hz * fk);
-      if ((dz2 + dy2 + dx2) <= rtot2) fl[IX(i, j, k, g.nx, g.ny)] = 1;
-      ii += di;
-      jj += dr;
-      if (ii >= ni) {
-        ii -= ni;
-        jj++;
-      }
-      while (jj >= nj) {
-        jj -= nj;
-        kk++;
+      size_t q = IX(imin + ii, jmin + jj, kmin, g.nx, g.ny);
+      for (int kk = 0; kk < nk; kk++, q += nxy) {
+        const double fk = kmin + kk;
+        const double dz2 = (posz - g.hz * fk) * (posz - g.hz * fk);
+        if ((dz2 + dy2 + dx2) <= rtot2) fl[q] = 1;
       }
     }
   }
