@@ -94,7 +94,7 @@ struct SolveGate {
   std::deque<Entry*> waiting;
   int running = 0;
   int max_batch = APBS_PMG_MAXBATCH;
-  int max_running = 2;
+  int max_running = 1;  // set to the slot count by PmgServer_ctor
   long batches = 0, batched = 0;
 
   int solve(Vpmg* pmg, int precision) {
@@ -295,6 +295,7 @@ extern "C" PmgServer* PmgServer_ctor(int slots, size_t hbm_budget) {
   PmgServer* s = new PmgServer();
   s->budget = hbm_budget;
   s->slots = slots;
+  s->gate.max_running = slots;  // no limit: a ready solve never waits for a batch
   for (int i = 0; i < slots; i++) s->workers.emplace_back([s] { s->work(); });
   return s;
 }

@@ -73,12 +73,15 @@ def _result(r):
 class Server:
     """PmgServer: `slots` requests in flight; their solves are batched across
     requests of one shape, at most `max_batch` per batch and `max_running`
-    batches at once (max_batch 1: every request solved alone)."""
+    batches at once (default: the slot count; max_batch 1: every request
+    solved alone)."""
 
-    def __init__(self, slots=8, hbm_budget=200 * 2 ** 30, max_batch=8, max_running=2):
+    def __init__(self, slots=8, hbm_budget=200 * 2 ** 30, max_batch=8, max_running=None):
         self.h = lib().PmgServer_ctor(int(slots), int(hbm_budget))
         if not self.h:
             raise RuntimeError("PmgServer_ctor failed")
+        if max_running is None:
+            max_running = slots
         if not lib().PmgServer_setBatching(self.h, int(max_batch), int(max_running)):
             raise ValueError("invalid batching parameters")
         self._keep = {}

@@ -75,12 +75,12 @@ void PmgServer_drain(PmgServer *thee);
  * held by the library's pool, [7] slots. */
 void PmgServer_stats(PmgServer *thee, double out[8]);
 /* Cross-request batching of the solves (default: max_batch 8, max_running
- * 2).  A request whose coefficient maps are built waits for its solve; a
- * waiting request's worker starts a batch whenever fewer than max_running
- * batches run, with the oldest waiting requests of its shape (grid, level
- * count, method, precision), at most max_batch (<= APBS_PMG_MAXBATCH).
- * max_batch 1 solves every request alone on its own stream.  Returns 0 on
- * invalid values. */
+ * the slot count, so no ready solve waits).  A request whose coefficient maps
+ * are built waits for its solve; a waiting request's worker starts a batch
+ * whenever fewer than max_running batches run, with the oldest waiting
+ * requests of its shape (grid, level count, method, precision), at most
+ * max_batch (<= APBS_PMG_MAXBATCH).  max_batch 1 solves every request alone
+ * on its own stream.  Returns 0 on invalid values. */
 int PmgServer_setBatching(PmgServer *thee, int max_batch, int max_running);
 /* Batches solved so far and the requests they held. */
 void PmgServer_batchStats(PmgServer *thee, long *batches, long *batched);

@@ -42,7 +42,9 @@ def test_alone_equals_in_mix(precision):
         assert r["status"] == 1
         alone.append((r["energy_kT"], req.potential.copy()))
     srv.close()
-    srv = serve.Server(slots=4)
+    # one batch at a time, so requests that finish their setup meanwhile
+    # wait at the gate and are solved together
+    srv = serve.Server(slots=4, max_running=1)
     reqs = [make_req(s, precision, i) for i, s in enumerate(specs)]
     for req in reqs:
         srv.submit(req)
