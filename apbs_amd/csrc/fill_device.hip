@@ -1477,23 +1477,32 @@ void sas_flags(hipStream_t s, const AtomsDev& A, const Cells& C, const double* r
   k_sas_flags<<<(unsigned)A.n, kB, 0, s>>>(A, C, ref, nref, probe, flags);
 }
 
+static void sas_check(hipError_t e) {
+  if (e != hipSuccess) {
+    fprintf(stderr, "sas_compact: %s\n", hipGetErrorString(e));
+    abort();
+  }
+}
+
 int sas_compact(hipStream_t s, const AtomsDev& A, const double* ref, int nref, double probe,
                 const unsigned char* flags, int* scratch, double** xyz_out,
                 double* (*alloc)(size_t)) {
   const long n = (long)A.n * nref;
   k_sas_count<<<kSasParts, kB, 0, s>>>(flags, n, scratch);
   std::vector<int> cnt(kSasParts), off(kSasParts);
-  hipMemcpyAsync(cnt.data(), scratch, kSasParts * sizeof(int), hipMemcpyDeviceToHost, s);
-  hipStreamSynchronize(s);
+  sas_check(hipMemcpyAsync(cnt.data(), scratch, kSasParts * sizeof(int), hipMemcpyDeviceToHost,
+                           s));
+  sas_check(hipStreamSynchronize(s));
   int tot = 0;
   for (int b = 0; b < kSasParts; b++) {
     off[b] = tot;
     tot += cnt[b];
   }
   double* xyz = alloc(3 * (size_t)(tot > 0 ? tot : 1));
-  hipMemcpyAsync(scratch, off.data(), kSasParts * sizeof(int), hipMemcpyHostToDevice, s);
+  sas_check(hipMemcpyAsync(scratch, off.data(), kSasParts * sizeof(int), hipMemcpyHostToDevice,
+                           s));
   k_sas_compact<<<kSasParts, kB, 0, s>>>(A, ref, nref, probe, flags, scratch, xyz);
-  hipStreamSynchronize(s);
+  sas_check(hipStreamSynchronize(s));
   *xyz_out = xyz;
   return tot;
 }

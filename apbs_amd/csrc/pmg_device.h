@@ -126,13 +126,14 @@ void nresid7_xpay_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>&
                        double* out, double* hout = nullptr);
 // resid7_nrm1 of the Jacobian system at z (r stored only as its FP32 image
 // r32, when given) and nresid7_xpay_nrm1 at y = x + a z, in one pass, bitwise
-// the two calls; part holds 2 * kParts partial sums, out / hout receive the
-// first norm, out2 / hout2 the second
+// the two calls; with zf given, z is read as (double) zf (z is not read);
+// rn32, when given, receives (float) rn; part holds 2 * kParts partial sums,
+// out / hout receive the first norm, out2 / hout2 the second
 void jres_trial_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A,
                      const double* D, const double* cc, const double* fc, const double* fz,
-                     const double* x, double a, const double* z, double* y, double* rn,
-                     float* r32, const Ions& ions, double* part, double* out, double* hout,
-                     double* out2, double* hout2);
+                     const double* x, double a, const double* z, const float* zf, double* y,
+                     double* rn, float* r32, float* rn32, const Ions& ions, double* part,
+                     double* out, double* hout, double* out2, double* hout2);
 // nresid7 followed by nrm1 of its result, in one pass.
 void nresid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* cc,
                   const double* fc, const double* x, double* r, const Ions& ions, double* part,
@@ -244,9 +245,11 @@ struct LevelArgs {
   T* xs;
   T* r;
 };
+// zero_x: every x starts at zero (its contents need not be cleared: the first
+// sweep does not read them)
 template <class T>
 void gsrb7_fused_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb,
-                       int nu, int iadjoint, int tile_shape = 0);
+                       int nu, int iadjoint, int tile_shape = 0, bool zero_x = false);
 template <class T>
 void gsrb27_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb, int nu,
                   int iadjoint);

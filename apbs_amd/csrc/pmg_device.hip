@@ -91,7 +91,10 @@ struct SweepArgs {
   T* xo;
 };
 
-template <class T, int kTY, int kKC, int kTX, int kNT>
+// kZeroX: xi is zero everywhere (a solve of the correction equation from a
+// zero start); its values are not read, and the sweep's values are those of
+// reading the zeros.
+template <class T, int kTY, int kKC, int kTX, int kNT, bool kZeroX>
 __device__ __forceinline__ void fused_body(int nx, int ny, int nz, int tile,
                                            const T* __restrict__ oE, const T* __restrict__ oN,
                                            const T* __restrict__ uC, const T* __restrict__ D,
@@ -135,21 +138,21 @@ __device__ __forceinline__ void fused_body(int nx, int ny, int nz, int tile,
         R.kind[u] = 2;
         R.v[u][0] = fc[p];
         R.v[u][1] = oN[p];
-        R.v[u][2] = xi[p + nx];
+        R.v[u][2] = kZeroX ? T(0) : xi[p + nx];
         R.v[u][3] = oN[p - nx];
-        R.v[u][4] = xi[p - nx];
+        R.v[u][4] = kZeroX ? T(0) : xi[p - nx];
         R.v[u][5] = oE[p];
-        R.v[u][6] = xi[p + 1];
+        R.v[u][6] = kZeroX ? T(0) : xi[p + 1];
         R.v[u][7] = oE[p - 1];
-        R.v[u][8] = xi[p - 1];
+        R.v[u][8] = kZeroX ? T(0) : xi[p - 1];
         R.v[u][9] = uC[p - nxy];
-        R.v[u][10] = xi[p - nxy];
+        R.v[u][10] = kZeroX ? T(0) : xi[p - nxy];
         R.v[u][11] = uC[p];
-        R.v[u][12] = xi[p + nxy];
+        R.v[u][12] = kZeroX ? T(0) : xi[p + nxy];
         R.v[u][13] = D[p];
       } else {
         R.kind[u] = 1;
-        R.v[u][0] = xi[p];
+        R.v[u][0] = kZeroX ? T(0) : xi[p];
       }
     }
   };
@@ -250,7 +253,7 @@ __device__ __forceinline__ void fused_body(int nx, int ny, int nz, int tile,
 // Workgroups are dealt round-robin to the 8 XCDs: XCD x gets the (calculation,
 // tile) items [x*per, (x+1)*per) in calculation-major, z-major order, so tiles
 // that share halo rows and planes share an L2.
-template <class T, int kTY, int kKC, int kTX, int kNT>
+template <class T, int kTY, int kKC, int kTX, int kNT, bool kZeroX = false>
 __global__ void __launch_bounds__(kNT)
     k_gsrb7_fused(int nx, int ny, int nz, Batch<SweepArgs<T>> B, int nreq, int first) {
   const int nbx = (nx - 2 + kTX - 1) / kTX, nby = (ny - 2 + kTY - 1) / kTY;
@@ -260,8 +263,8 @@ __global__ void __launch_bounds__(kNT)
   const int item = (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
   if (item >= total) return;
   const SweepArgs<T>& a = B.v[item / nb];
-  fused_body<T, kTY, kKC, kTX, kNT>(nx, ny, nz, item % nb, a.oE, a.oN, a.uC, a.D, a.fc, a.xi, a.xo,
-                                    first);
+  fused_body<T, kTY, kKC, kTX, kNT, kZeroX>(nx, ny, nz, item % nb, a.oE, a.oN, a.uC, a.D, a.fc,
+                                            a.xi, a.xo, first);
 }
 
 // ----------------------------------------------------------------- 27-point
@@ -1147,28 +1150,31 @@ __global__ void __launch_bounds__(kBlock)
 
 // The end of a Newton step's inner cycle and the step's first trial point in
 // one pass.  With z the correction the cycle produced (its boundary layer
-// zero): the inner residual fz + J z of the Jacobian system (J's diagonal D),
-// its FP32 image in r32 and its L1 partials in part[0, kParts)
-// (k_resid7_nrm<double>'s values and order); and y = x + a z with the
-// nonlinear residual at y in rn, its L1 partials in part[kParts, 2 kParts)
-// (k_nresid7_xpay's values and order).  The two stencils share their operator
-// loads.
+// zero; read as (double) zf when zf is given, the values an FP32 correction
+// assigned to z would have): the inner residual fz + J z of the Jacobian
+// system (J's diagonal D), its FP32 image in r32 and its L1 partials in
+// part[0, kParts) (k_resid7_nrm<double>'s values and order); and y = x + a z
+// with the nonlinear residual at y in rn (its FP32 image in rn32 when given),
+// its L1 partials in part[kParts, 2 kParts) (k_nresid7_xpay's values and
+// order).  The two stencils share their operator loads.
 __global__ void __launch_bounds__(kBlock)
     k_jres_trial(int nx, int ny, int nz, const double* __restrict__ oC,
                  const double* __restrict__ oE, const double* __restrict__ oN,
                  const double* __restrict__ uC, const double* __restrict__ D,
                  const double* __restrict__ cc, const double* __restrict__ fc,
                  const double* __restrict__ fz, const double* __restrict__ x, double a,
-                 const double* __restrict__ z, double* __restrict__ y, double* __restrict__ rn,
-                 float* __restrict__ r32, Ions ions, double* part) {
+                 const double* __restrict__ z, const float* __restrict__ zf,
+                 double* __restrict__ y, double* __restrict__ rn, float* __restrict__ r32,
+                 float* __restrict__ rn32, Ions ions, double* part) {
   __shared__ double sh[kBlock];
   const int ni = nx - 2, nij = ni * (ny - 2), n = nij * (nz - 2);
   const size_t nxy = (size_t)nx * ny;
   int t0, t1;
   part_range(n, t0, t1);
+  auto zv = [&](size_t q) -> double { return zf ? (double)zf[q] : z[q]; };
   auto yv = [&](size_t q) {
     double v = x[q];
-    v += a * z[q];
+    v += a * zv(q);
     return v;
   };
   double acc_j = 0.0, acc_n = 0.0;
@@ -1176,7 +1182,15 @@ __global__ void __launch_bounds__(kBlock)
     const int k = 1 + t / nij, rem = t - (k - 1) * nij;
     const int j = 1 + rem / ni, i = 1 + rem - (j - 1) * ni;
     const size_t p = IX(i, j, k, nx, ny);
-    const double vj = acc7(fz[p], oE, oN, uC, z, p, nx, nxy) - D[p] * z[p];
+    // acc7's terms and order at z
+    double sj = fz[p];
+    sj += oN[p] * zv(p + nx);
+    sj += oN[p - nx] * zv(p - nx);
+    sj += oE[p] * zv(p + 1);
+    sj += oE[p - 1] * zv(p - 1);
+    sj += uC[p - nxy] * zv(p - nxy);
+    sj += uC[p] * zv(p + nxy);
+    const double vj = sj - D[p] * zv(p);
     if (r32) r32[p] = (float)vj;
     acc_j += fabs(vj);
     const double y0 = yv(p);
@@ -1191,13 +1205,14 @@ __global__ void __launch_bounds__(kBlock)
     s7 += uC[p] * yv(p + nxy);
     const double v = s7 - oC[p] * y0 - w;
     rn[p] = v;
+    if (rn32) rn32[p] = (float)v;
     acc_n += fabs(v);
   }
-  const double sj = block_sum(acc_j, sh);
-  const double sn = block_sum(acc_n, sh);
+  const double bj = block_sum(acc_j, sh);
+  const double bn = block_sum(acc_n, sh);
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = sj;
-    part[kParts + blockIdx.x] = sn;
+    part[blockIdx.x] = bj;
+    part[kParts + blockIdx.x] = bn;
   }
 }
 
@@ -1739,14 +1754,19 @@ inline void check_batch(int nb) {
 
 // =================================================================== wrappers
 
-template <class T, int TY, int KC, int TX, int NT = kFT>
+template <class T, int TY, int KC, int TX, int NT = kFT, bool ZX = false>
 void fused_sweep(hipStream_t s, int nx, int ny, int nz, const Batch<SweepArgs<T>>& B, int nb,
                  int first) {
   const int tiles = ((nx - 2 + TX - 1) / TX) * ((ny - 2 + TY - 1) / TY) * ((nz - 2 + KC - 1) / KC);
   const long total = (long)tiles * nb;
   const unsigned g = (unsigned)(((total + kXCD - 1) / kXCD) * kXCD);
-  k_gsrb7_fused<T, TY, KC, TX, NT><<<g, NT, 0, s>>>(nx, ny, nz, B, nb, first);
+  k_gsrb7_fused<T, TY, KC, TX, NT, ZX><<<g, NT, 0, s>>>(nx, ny, nz, B, nb, first);
 }
+
+// The default shapes (4 rows x 8 planes, 8 rows x 8 planes with 512
+// threads) also come in the zero-start form; other shapes start from a
+// cleared x.
+static bool zero_start_shape(int rp) { return rp == 48 || rp == 5088; }
 
 static bool valid_tile(int v) {
   const int w = v / 10000, rp = v % 10000;
@@ -1794,7 +1814,14 @@ int fused_width(int nx) {
 
 template <class T, int TX>
 static void fused_sweep_w(hipStream_t s, int rp, int nx, int ny, int nz,
-                          const Batch<SweepArgs<T>>& B, int nb, int first) {
+                          const Batch<SweepArgs<T>>& B, int nb, int first, bool zero_x) {
+  if (zero_x) {
+    if (rp == 48)
+      fused_sweep<T, 4, 8, TX, kFT, true>(s, nx, ny, nz, B, nb, first);
+    else
+      fused_sweep<T, 8, 8, TX, 512, true>(s, nx, ny, nz, B, nb, first);
+    return;
+  }
   switch (rp) {
     case 416: fused_sweep<T, 4, 16, TX>(s, nx, ny, nz, B, nb, first); break;
     case 48: fused_sweep<T, 4, 8, TX>(s, nx, ny, nz, B, nb, first); break;
@@ -1814,13 +1841,17 @@ static void fused_sweep_w(hipStream_t s, int rp, int nx, int ny, int nz,
 
 template <class T>
 void gsrb7_fused_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb,
-                       int nu, int iadjoint, int tile_shape) {
+                       int nu, int iadjoint, int tile_shape, bool zero_x) {
   check_batch(nb);
   const int first = iadjoint ? 0 : 1;
   const int tile =
       tile_shape > 0 && valid_tile(tile_shape) ? tile_shape : fused_tile(nx, ny, nz);
   const int rp = tile % 10000;
   const int w = tile / 10000 ? tile / 10000 : fused_width(nx);
+  if (zero_x && !zero_start_shape(rp)) {
+    for (int b = 0; b < nb; b++) fill<T>(s, c[b].x, (size_t)nx * ny * nz, T(0));
+    zero_x = false;
+  }
   for (int it = 0; it < nu; it++) {
     // sweep it reads x (even it) or xs and writes the other one
     Batch<SweepArgs<T>> B;
@@ -1829,12 +1860,13 @@ void gsrb7_fused_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>
       B.v[b] = SweepArgs<T>{a.A.a[1], a.A.a[2], a.A.a[3], a.D, a.f,
                             (it & 1) ? a.xs : a.x, (it & 1) ? a.x : a.xs};
     }
+    const bool zx = zero_x && it == 0;
     if (w == 80)
-      fused_sweep_w<T, 80>(s, rp, nx, ny, nz, B, nb, first);
+      fused_sweep_w<T, 80>(s, rp, nx, ny, nz, B, nb, first, zx);
     else if (w == 96)
-      fused_sweep_w<T, 96>(s, rp, nx, ny, nz, B, nb, first);
+      fused_sweep_w<T, 96>(s, rp, nx, ny, nz, B, nb, first, zx);
     else
-      fused_sweep_w<T, 64>(s, rp, nx, ny, nz, B, nb, first);
+      fused_sweep_w<T, 64>(s, rp, nx, ny, nz, B, nb, first, zx);
   }
   if (nu & 1)
     for (int b = 0; b < nb; b++) copy_interior<T>(s, nx, ny, nz, c[b].xs, c[b].x);
@@ -2088,11 +2120,11 @@ void nresid7_xpay_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>&
 
 void jres_trial_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A,
                      const double* D, const double* cc, const double* fc, const double* fz,
-                     const double* x, double a, const double* z, double* y, double* rn,
-                     float* r32, const Ions& ions, double* part, double* out, double* hout,
-                     double* out2, double* hout2) {
+                     const double* x, double a, const double* z, const float* zf, double* y,
+                     double* rn, float* r32, float* rn32, const Ions& ions, double* part,
+                     double* out, double* hout, double* out2, double* hout2) {
   k_jres_trial<<<kParts, kBlock, 0, s>>>(nx, ny, nz, A.a[0], A.a[1], A.a[2], A.a[3], D, cc, fc,
-                                         fz, x, a, z, y, rn, r32, ions, part);
+                                         fz, x, a, z, zf, y, rn, r32, rn32, ions, part);
   Batch<FinishArgs> F;
   F.v[0] = FinishArgs{part, out, hout};
   F.v[1] = FinishArgs{part + kParts, out2, hout2};
@@ -2367,7 +2399,7 @@ void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int 
   template void gsrb7_fused<T>(hipStream_t, int, int, int, const Ops<T>&, const T*, const T*, T*, \
                                T*, int, int, int);                                              \
   template void gsrb7_fused_batch<T>(hipStream_t, int, int, int, const LevelArgs<T>*, int, int,  \
-                                     int, int);                                                 \
+                                     int, int, bool);                                           \
   template void gsrb27_batch<T>(hipStream_t, int, int, int, const LevelArgs<T>*, int, int, int); \
   template void gsrb27_pairs_batch<T>(hipStream_t, int, int, int, const LevelArgs<T>*, int, int, \
                                       int);                                                     \

@@ -1331,6 +1331,9 @@ struct CycleArgs {
   const T* f;
   T* w1;
   T* xs;  // fine-size smoother partner buffer
+  // the fine solution starts at zero (its contents are not read and need not
+  // be cleared)
+  bool zero_x = false;
 };
 
 // The V-cycle of n calculations of one shape (n <= pmg::kMaxBatch), all on
@@ -1358,16 +1361,20 @@ void vcycle_batch(PmgDev* const* ds, CycleArgs<T>* cs, int n) {
                                 cs[b].X[l], l == 0 ? cs[b].xs : cs[b].xsl[l],
                                 with_r ? cs[b].w1 : nullptr};
   };
-  auto fine_smooth = [&](int adj) {
-    const int e0 = n == 1 ? tbegin(d0) : -1;
+  // (the smoother timing covers full sweeps only: a zero-start pair's first
+  // sweep reads no solution)
+  auto fine_smooth = [&](int adj, bool zero_x) {
+    const int e0 = n == 1 && !zero_x ? tbegin(d0) : -1;
     if (d0->smoother == 0) {
       level(0, false);
-      pmg::gsrb7_fused_batch<T>(s, L[0].nx, L[0].ny, L[0].nz, la, n, 2, adj);
+      pmg::gsrb7_fused_batch<T>(s, L[0].nx, L[0].ny, L[0].nz, la, n, 2, adj, 0, zero_x);
       if (n == 1) tend(d0, e0, 0, pmg::gsrb7_fused_launches(2));
     } else {
-      for (int b = 0; b < n; b++)
+      for (int b = 0; b < n; b++) {
+        if (zero_x) pmg::fill<T>(s, cs[b].X[0], L[0].n, T(0));
         pmg::gsrb7<T>(s, L[0].nx, L[0].ny, L[0].nz, cs[b].A[0], cs[b].Dl[0], cs[b].f, cs[b].X[0],
                       2, adj);
+      }
       if (n == 1) tend(d0, e0, 0, pmg::gsrb7_launches(2));
     }
   };
@@ -1381,7 +1388,8 @@ void vcycle_batch(PmgDev* const* ds, CycleArgs<T>* cs, int n) {
       pmg::gsrb27_batch<T>(s, L[l].nx, L[l].ny, L[l].nz, la, n, 2, adj);
   };
   const int ecyc = n == 1 ? tbegin(d0) : -1;
-  fine_smooth(0);
+  // calculations of a batch start alike (every mixed-precision cycle from zero)
+  fine_smooth(0, cs[0].zero_x);
   level(0, true);
   pmg::resid_batch<T>(s, 7, L[0].nx, L[0].ny, L[0].nz, la, n);
   for (int l = 1; l < nt; l++) {
@@ -1416,7 +1424,7 @@ void vcycle_batch(PmgDev* const* ds, CycleArgs<T>* cs, int n) {
     pmg::damp_factor_batch<T>(s, Cl.nx, Cl.ny, Cl.nz, pa, n);
     pmg::prolong_add_batch<T>(s, Cl.nx, Cl.ny, Cl.nz, L[l].nx, L[l].ny, L[l].nz, ta, n);
     if (l == 0)
-      fine_smooth(1);
+      fine_smooth(1, false);
     else
       coarse_smooth(l, 1);
   }
@@ -1505,24 +1513,32 @@ void read_norms(PmgDev* const* ds, int n, const bool* active, double* out, bool*
 // nonlinear residual at y.  After the inner solve, ok[b] says whether
 // calculation b ran a cycle (so y and rn belong to its final correction) and
 // nrm[b] holds ||rn||1.
+// rn32[b], when set, receives the FP32 image of rn.  A mixed-precision solve
+// that starts from zero reads its first correction straight from the FP32
+// cycle's solution; x_pending[b] then says that z itself was not written
+// (it equals that FP32 solution converted).
 struct Trials {
   const double* u[pmg::kMaxBatch];
   double* y[pmg::kMaxBatch];
   double* rn[pmg::kMaxBatch];
+  float* rn32[pmg::kMaxBatch];
   const pmg::Ions* I[pmg::kMaxBatch];
   double nrm[pmg::kMaxBatch];
   bool ok[pmg::kMaxBatch];
+  bool x_pending[pmg::kMaxBatch];
 };
 
 // The inner residual f + J z of calculation b's fine level (its L1 norm to
 // hscal[0]; its FP32 image to r32 when given) and, with trials, its trial
-// point and nonlinear residual (norm to hscal[1]) in the same pass.
-void inner_resid(PmgDev* d, const double* f, const double* z, float* r32, Trials* tr, int b) {
+// point and nonlinear residual (norm to hscal[1]) in the same pass.  With zf
+// given, z is read as (double) zf.
+void inner_resid(PmgDev* d, const double* f, const double* z, float* r32, Trials* tr, int b,
+                 const float* zf = nullptr) {
   const DLevel& F = d->L[0];
   if (tr) {
     pmg::jres_trial_nrm1(d->s, F.nx, F.ny, F.nz, F.ops(), F.D, F.cc, F.fc, f, tr->u[b], 1.0, z,
-                         tr->y[b], tr->rn[b], r32, *tr->I[b], d->part, d->scal, d->hscal,
-                         d->scal + 8, d->hscal + 1);
+                         zf, tr->y[b], tr->rn[b], r32, tr->rn32[b], *tr->I[b], d->part, d->scal,
+                         d->hscal, d->scal + 8, d->hscal + 1);
     tr->ok[b] = true;
   } else {
     pmg::resid7_nrm1<double>(d->s, F.nx, F.ny, F.nz, F.ops(), F.D, f, z, nullptr, d->part,
@@ -1607,7 +1623,7 @@ void mvcs64_batch(PmgDev* const* ds, int n, double* const* const* X, const doubl
 // (solve_failed).
 void mixed_batch(PmgDev* const* ds, int n, double* const* const* X64, const double* const* f,
                  int istop, int itmax, const double* errtol, double* relres, int* outer,
-                 bool x_zero, Trials* tr = nullptr) {
+                 bool x_zero, Trials* tr = nullptr, const double* known_fnrm = nullptr) {
   bool active[pmg::kMaxBatch], failed[pmg::kMaxBatch] = {};
   double rsden[pmg::kMaxBatch], fnrm[pmg::kMaxBatch], rsnrm[pmg::kMaxBatch];
   int it[pmg::kMaxBatch];
@@ -1618,17 +1634,24 @@ void mixed_batch(PmgDev* const* ds, int n, double* const* const* X64, const doub
     it[b] = 0;
   }
   // the first residual: f itself when the solution starts at zero (f + A 0
-  // rounds to f, and its FP32 image comes with its norm), else computed
+  // rounds to f, and its FP32 image comes with its norm), else computed; a
+  // known norm (known_fnrm[b] >= 0) comes with the image already in w0f
   if (istop == 1 || x_zero) {
+    bool launched[pmg::kMaxBatch] = {}, any = false;
     for (int b = 0; b < n; b++) {
       PmgDev* d = ds[b];
       const DLevel& F = d->L[0];
+      if (x_zero && known_fnrm && known_fnrm[b] >= 0.0) {
+        fnrm[b] = known_fnrm[b];
+        continue;
+      }
+      launched[b] = any = true;
       if (x_zero)
         pmg::nrm1_to32(d->s, F.nx, F.ny, F.nz, f[b], d->L[0].w0f, d->part, d->scal, d->hscal);
       else
         pmg::nrm1<double>(d->s, F.nx, F.ny, F.nz, f[b], d->part, d->scal, d->hscal);
     }
-    read_norms(ds, n, active, fnrm, failed);
+    if (any) read_norms(ds, n, launched, fnrm, failed);
     for (int b = 0; b < n; b++)
       if (istop == 1) rsden[b] = fnrm[b];
   }
@@ -1649,6 +1672,7 @@ void mixed_batch(PmgDev* const* ds, int n, double* const* const* X64, const doub
   if (!x_zero) read_norms(ds, n, active, rsnrm, failed);
   CycleArgs<float> cs[pmg::kMaxBatch];
   PmgDev* act[pmg::kMaxBatch];
+  bool pending[pmg::kMaxBatch] = {};  // X64 not written: it is (double) xfl
   for (;;) {
     int m = 0;
     for (int b = 0; b < n; b++) {
@@ -1658,8 +1682,15 @@ void mixed_batch(PmgDev* const* ds, int n, double* const* const* X64, const doub
         continue;
       }
       PmgDev* d = ds[b];
-      pmg::fill<float>(d->s, d->L[0].xfl, d->L[0].n, 0.0f);
+      const DLevel& F = d->L[0];
+      if (pending[b]) {
+        pmg::axpy_interior_f32(d->s, F.nx, F.ny, F.nz, 1.0, F.xfl, X64[b][0], true);
+        pending[b] = false;
+      }
+      // every cycle solves for the correction from zero: the cycle's first
+      // sweep does not read the fine solution, which is not cleared
       cs[m] = args32(d);
+      cs[m].zero_x = true;
       act[m++] = d;
     }
     if (m == 0) break;
@@ -1669,15 +1700,21 @@ void mixed_batch(PmgDev* const* ds, int n, double* const* const* X64, const doub
       PmgDev* d = ds[b];
       const DLevel& F = d->L[0];
       // X64 += (double) e in one pass (X64 = 0 + (double) e the first time
-      // when it starts at zero)
-      pmg::axpy_interior_f32(d->s, F.nx, F.ny, F.nz, 1.0, d->L[0].xfl, X64[b][0],
-                             x_zero && it[b] == 0);
+      // when it starts at zero); with trials, the first correction of a zero
+      // start is read from e itself
+      const bool direct = tr && x_zero && it[b] == 0;
+      if (!direct)
+        pmg::axpy_interior_f32(d->s, F.nx, F.ny, F.nz, 1.0, F.xfl, X64[b][0],
+                               x_zero && it[b] == 0);
       it[b]++;
-      inner_resid(d, f[b], X64[b][0], d->L[0].w0f, tr, b);
+      inner_resid(d, f[b], X64[b][0], F.w0f, tr, b, direct ? F.xfl : nullptr);
+      pending[b] = direct;
     }
     read_norms(ds, n, active, rsnrm, failed);
     read_trials(ds, n, active, tr);
   }
+  if (tr)
+    for (int b = 0; b < n; b++) tr->x_pending[b] = pending[b];
   for (int b = 0; b < n; b++) {
     // a solution that started at zero and took no correction is zero
     if (x_zero && it[b] == 0)
@@ -1710,6 +1747,9 @@ struct NewtonState {
   int itmax = 0;
   int idamp = 1, iters = 0, inner = 0, iter_d = 0, mode = 0;
   bool active = true, searching = false, failed = false;
+  // the FP32 image of F is in the fine partner buffer xs32 (written with F by
+  // the step's trial kernel); its norm is xnorm_new
+  bool f32 = false;
 };
 
 // Vnewton with Vgetjac (:550-598) for n calculations of one shape in lockstep:
@@ -1808,21 +1848,42 @@ void newton_batch(Vpmg* const* th, int n, double* relres, int* iters, int* inner
     }
     // the inner solve's last residual also forms each step's first trial
     // point u + xtmp (wn1) and its nonlinear residual (wn3)
+    // (mixed precision, full steps: with the FP32 image of the residual,
+    // which the next step's inner solve takes as its right-hand side)
     Trials tr;
+    double known[pmg::kMaxBatch];
     for (int q = 0; q < m; q++) {
       PmgDev* d = ad[q];
+      NewtonState& S = st[idx[q]];
+      const bool mixed = d->precision == 1 && d->h_mixed;
+      known[q] = -1.0;
+      if (mixed && S.f32) {
+        // F's FP32 image becomes the right-hand side of the FP32 cycles
+        std::swap(d->L[0].w0f, d->xs32);
+        known[q] = S.xnorm_new;
+      }
+      S.f32 = false;
       tr.u[q] = d->u;
       tr.y[q] = d->wn1;
       tr.rn[q] = d->wn3;
+      tr.rn32[q] = mixed && S.idamp != 1 ? d->xs32 : nullptr;
       tr.I[q] = &st[idx[q]].I;
       tr.ok[q] = false;
+      tr.x_pending[q] = false;
     }
     // calculations of one batch share the precision (batches are formed by
     // shape and precision); each has its own inner tolerance
     if (ad[0]->precision == 1)
-      mixed_batch(ad, m, Xp, fp, 0, 1000, errs, rr, its, true, &tr);
+      mixed_batch(ad, m, Xp, fp, 0, 1000, errs, rr, its, true, &tr, known);
     else
       mvcs64_batch(ad, m, Xp, fp, 0, 1000, errs, rr, its, &tr);
+    // the damping search reads the correction itself
+    for (int q = 0; q < m; q++) {
+      PmgDev* d = ad[q];
+      const DLevel& F = d->L[0];
+      if (tr.x_pending[q] && st[idx[q]].idamp == 1)
+        pmg::axpy_interior_f32(d->s, F.nx, F.ny, F.nz, 1.0, F.xfl, F.xtmp, true);
+    }
     for (int q = 0; q < m; q++) {
       NewtonState& S = st[idx[q]];
       if (its[q] < 0) {
@@ -1850,6 +1911,7 @@ void newton_batch(Vpmg* const* th, int n, double* relres, int* iters, int* inner
         if (S.idamp != 1) {
           std::swap(d->u, d->wn1);
           d->L[0].x = d->u;  // the fine level's solution is the potential
+          S.f32 = tr.rn32[q] != nullptr;
         }
         nv[b] = tr.nrm[q];
         if (solve_failed(d, nv[b])) S.failed = true;

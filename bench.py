@@ -161,7 +161,9 @@ def pmc_workload(cfg):
 def smoother_roofline(kt):
     """Fine-level red-black smoother: algorithmic bytes 7 words per point per
     full sweep (D = oC + cc, oE, oN, uC, f, x in, x out); the library reports
-    the words per point one launch moves (7 for the fused sweep kernel)."""
+    the words per point one launch moves (7 for the fused sweep kernel) and
+    times only such sweeps (a mixed-precision cycle's first sweep starts from
+    a zero correction and reads no x: that pair is not timed)."""
     if kt["smoother_launches"] == 0:
         return None
     per_launch_ms = kt["smoother_ms"] / kt["smoother_launches"]
