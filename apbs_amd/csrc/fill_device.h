@@ -106,11 +106,12 @@ void mark_flags(hipStream_t s, const Geo& g, int npts, const double* pos, const 
 // sphere (atom flags, 3 maps) and not within a probe radius of an SAS point
 // (sas flags, 3 maps; null when the probe radius is 0), else epsw; smooth
 // selects the 9-point harmonic mean of those values.  kappa is 0 where the
-// kappa flag is set (null: nowhere), else ionmask.
+// kappa flag is set (null: nowhere), else ionmask.  bits: nx*ny*nz bytes of
+// scratch (the solute bits of every point).
 void mol_maps(hipStream_t s, const Geo& g, const unsigned char* atom_flags,
               const unsigned char* sas_flags, const unsigned char* kappa_flags, double epsp,
               double epsw, double ionmask, int smooth, double* epsx, double* epsy, double* epsz,
-              double* kappa);
+              double* kappa, unsigned char* bits);
 
 // ------------------------------------------------------------- boundaries
 // bcflnew (vpmg.c:3289-3367): Debye-Hueckel sum over all atoms on the faces.

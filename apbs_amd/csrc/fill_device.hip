@@ -590,79 +590,103 @@ __global__ void __launch_bounds__(kB)
   }
 }
 
-// The maps of the molecular surface from the flags, one grid point per thread.
-// The harmonic mean takes 1 / epsp or 1 / epsw for each neighbour value, the
-// same quotients as the reference's 1.0 / a (vpmg.c:4500-4628).
+// The solute bits of every grid point: bit w (map w = x, y, z shifted grid)
+// is set inside an atom sphere (atom flags) and not within a probe radius of
+// an SAS point (sas flags, absent when the probe radius is 0), where the
+// unsmoothed dielectric value is epsp.
 __global__ void __launch_bounds__(kB)
-    k_mol_maps(Geo g, const unsigned char* fa, const unsigned char* fsas,
-               const unsigned char* fk, double epsp, double epsw, double ionmask, int smooth,
-               double* epsx, double* epsy, double* epsz, double* kappa) {
+    k_solute_bits(Geo g, const unsigned char* fa, const unsigned char* fsas,
+                  unsigned char* bits) {
+  const int t = blockIdx.x * kB + threadIdx.x;
+  const int k = blockIdx.y;
+  const int i = t % g.nx, j = t / g.nx;
+  if (j >= g.ny) return;
+  const size_t n = (size_t)g.nx * g.ny * g.nz;
+  const size_t p = IX(i, j, k, g.nx, g.ny);
+  unsigned char b = 0;
+  for (int w = 0; w < 3; w++)
+    if (fa[w * n + p] != 0 && !(fsas && fsas[w * n + p] != 0)) b |= (unsigned char)(1 << w);
+  bits[p] = b;
+}
+
+// The maps of the molecular surface from the solute bits, one grid point per
+// thread.  The harmonic mean takes 1 / epsp or 1 / epsw for each neighbour
+// value, the same quotients as the reference's 1.0 / a (vpmg.c:4500-4628),
+// summed in the reference's order; the bits of the 13 points the three means
+// read are loaded once.
+__global__ void __launch_bounds__(kB)
+    k_mol_maps(Geo g, const unsigned char* __restrict__ sb, const unsigned char* fk, double epsp,
+               double epsw, double ionmask, int smooth, double* epsx, double* epsy, double* epsz,
+               double* kappa) {
   const int t = blockIdx.x * kB + threadIdx.x;
   const int k = blockIdx.y;
   const int i = t % g.nx, j = t / g.nx;
   if (j >= g.ny) return;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const size_t n = (size_t)nx * ny * nz;
+  const size_t nxy = (size_t)nx * ny;
   const size_t p = IX(i, j, k, nx, ny);
-  // unsmoothed value of map w at q: epsp inside an atom and not re-marked
-  auto solute = [&](int w, size_t q) {
-    return fa[w * n + q] != 0 && !(fsas && fsas[w * n + q] != 0);
-  };
   kappa[p] = (fk && fk[p]) ? 0.0 : ionmask;
+  const int c = sb[p];
   if (!smooth) {
-    epsx[p] = solute(0, p) ? epsp : epsw;
-    epsy[p] = solute(1, p) ? epsp : epsw;
-    epsz[p] = solute(2, p) ? epsp : epsw;
+    epsx[p] = (c & 1) ? epsp : epsw;
+    epsy[p] = (c & 2) ? epsp : epsw;
+    epsz[p] = (c & 4) ? epsp : epsw;
     return;
   }
+  const bool xlo = i > 0, xhi = i < nx - 1, ylo = j > 0, yhi = j < ny - 1, zlo = k > 0,
+             zhi = k < nz - 1;
+  const int xm = xlo ? sb[p - 1] : 0, xp = xhi ? sb[p + 1] : 0;
+  const int ym = ylo ? sb[p - nx] : 0, yp = yhi ? sb[p + nx] : 0;
+  const int zm = zlo ? sb[p - nxy] : 0, zp = zhi ? sb[p + nxy] : 0;
+  const int xp_ym = xhi && ylo ? sb[p + 1 - nx] : 0, xp_zm = xhi && zlo ? sb[p + 1 - nxy] : 0;
+  const int xm_yp = xlo && yhi ? sb[p - 1 + nx] : 0, yp_zm = yhi && zlo ? sb[p + nx - nxy] : 0;
+  const int xm_zp = xlo && zhi ? sb[p - 1 + nxy] : 0, ym_zp = ylo && zhi ? sb[p - nx + nxy] : 0;
   const double rp = 1.0 / epsp, rw = 1.0 / epsw;
-  auto inv = [&](int w, int ii, int jj, int kk) {
-    return solute(w, IX(ii, jj, kk, nx, ny)) ? rp : rw;
-  };
+  auto inv = [&](int w, int bits) { return ((bits >> w) & 1) ? rp : rw; };
   double frac;
   int np;
-  frac = inv(0, i, j, k);
-  frac += inv(1, i, j, k);
-  frac += inv(2, i, j, k);
+  frac = inv(0, c);
+  frac += inv(1, c);
+  frac += inv(2, c);
   np = 3;
-  if (j > 0) { frac += inv(1, i, j - 1, k); np += 1; }
-  if (k > 0) { frac += inv(2, i, j, k - 1); np += 1; }
-  if (i < nx - 1) {
-    frac += inv(1, i + 1, j, k);
-    frac += inv(2, i + 1, j, k);
+  if (ylo) { frac += inv(1, ym); np += 1; }
+  if (zlo) { frac += inv(2, zm); np += 1; }
+  if (xhi) {
+    frac += inv(1, xp);
+    frac += inv(2, xp);
     np += 2;
-    if (j > 0) { frac += inv(1, i + 1, j - 1, k); np += 1; }
-    if (k > 0) { frac += inv(2, i + 1, j, k - 1); np += 1; }
+    if (ylo) { frac += inv(1, xp_ym); np += 1; }
+    if (zlo) { frac += inv(2, xp_zm); np += 1; }
   }
   epsx[p] = np / frac;
 
-  frac = inv(1, i, j, k);
-  frac += inv(0, i, j, k);
-  frac += inv(2, i, j, k);
+  frac = inv(1, c);
+  frac += inv(0, c);
+  frac += inv(2, c);
   np = 3;
-  if (i > 0) { frac += inv(0, i - 1, j, k); np += 1; }
-  if (k > 0) { frac += inv(2, i, j, k - 1); np += 1; }
-  if (j < ny - 1) {
-    frac += inv(0, i, j + 1, k);
-    frac += inv(2, i, j + 1, k);
+  if (xlo) { frac += inv(0, xm); np += 1; }
+  if (zlo) { frac += inv(2, zm); np += 1; }
+  if (yhi) {
+    frac += inv(0, yp);
+    frac += inv(2, yp);
     np += 2;
-    if (i > 0) { frac += inv(0, i - 1, j + 1, k); np += 1; }
-    if (k > 0) { frac += inv(2, i, j + 1, k - 1); np += 1; }
+    if (xlo) { frac += inv(0, xm_yp); np += 1; }
+    if (zlo) { frac += inv(2, yp_zm); np += 1; }
   }
   epsy[p] = np / frac;
 
-  frac = inv(2, i, j, k);
-  frac += inv(0, i, j, k);
-  frac += inv(1, i, j, k);
+  frac = inv(2, c);
+  frac += inv(0, c);
+  frac += inv(1, c);
   np = 3;
-  if (i > 0) { frac += inv(0, i - 1, j, k); np += 1; }
-  if (j > 0) { frac += inv(1, i, j - 1, k); np += 1; }
-  if (k < nz - 1) {
-    frac += inv(0, i, j, k + 1);
-    frac += inv(1, i, j, k + 1);
+  if (xlo) { frac += inv(0, xm); np += 1; }
+  if (ylo) { frac += inv(1, ym); np += 1; }
+  if (zhi) {
+    frac += inv(0, zp);
+    frac += inv(1, zp);
     np += 2;
-    if (i > 0) { frac += inv(0, i - 1, j, k + 1); np += 1; }
-    if (j > 0) { frac += inv(1, i, j - 1, k + 1); np += 1; }
+    if (xlo) { frac += inv(0, xm_zp); np += 1; }
+    if (ylo) { frac += inv(1, ym_zp); np += 1; }
   }
   epsz[p] = np / frac;
 }
@@ -1516,9 +1540,10 @@ void mark_flags(hipStream_t s, const Geo& g, int npts, const double* pos, const 
 void mol_maps(hipStream_t s, const Geo& g, const unsigned char* atom_flags,
               const unsigned char* sas_flags, const unsigned char* kappa_flags, double epsp,
               double epsw, double ionmask, int smooth, double* epsx, double* epsy, double* epsz,
-              double* kappa) {
-  k_mol_maps<<<grid_all(g), kB, 0, s>>>(g, atom_flags, sas_flags, kappa_flags, epsp, epsw,
-                                        ionmask, smooth, epsx, epsy, epsz, kappa);
+              double* kappa, unsigned char* bits) {
+  k_solute_bits<<<grid_all(g), kB, 0, s>>>(g, atom_flags, sas_flags, bits);
+  k_mol_maps<<<grid_all(g), kB, 0, s>>>(g, bits, kappa_flags, epsp, epsw, ionmask, smooth, epsx,
+                                        epsy, epsz, kappa);
 }
 
 int bc_mdh_chunks(int natoms) { return natoms >= 4096 ? 16 : (natoms >= 512 ? natoms / 256 : 1); }

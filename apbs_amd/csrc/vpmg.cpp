@@ -1067,10 +1067,10 @@ size_t pmgint::footprint_bytes(int nx, int ny, int nz, int nlev, bool newton, bo
   b += blk(16.0 * ny * nz) + blk(16.0 * nx * nz) + blk(16.0 * nx * ny);
   b += 3 * blk(8.0 * std::max(nx, std::max(ny, nz))) + blk(16.0 * pmg::kParts) + blk(256);
   b += 5 * blk(8.0 * std::max(natoms, 1) * 3) + 4 * blk(24.0 * std::max(natoms, 1));
-  // Vpmg_fillco temporaries (released before the solve): SAS flags and
-  // points, or the three smoothing copies
+  // Vpmg_fillco temporaries (released before the solve): the surface flag
+  // maps (8 bytes per point), SAS flags and points; or the spline tiles
   const double sas = (double)natoms * nref;
-  const double fill = std::max(blk(sas) + blk(24 * sas), 3 * blk(8 * n)) + (64 << 20);
+  const double fill = std::max(blk(8 * n) + blk(sas) + blk(24 * sas), 3 * blk(8 * n)) + (64 << 20);
   // hierarchy
   int dims[24][3];
   const int nt = plan_levels(nx, ny, nz, nlev, dims);
@@ -2542,12 +2542,13 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
       // and, with salt, the ion-accessibility spheres on the unshifted one;
       // the probe spheres at the SAS points, which re-mark the solvent
       // value; then one pass forms the maps (and the harmonic mean for smol).
-      // Layout: [0, 3n) atom flags x, y, z; [3n, 4n) kappa; [4n, 7n) SAS.
+      // Layout: [0, 3n) atom flags x, y, z; [3n, 4n) kappa; [4n, 7n) SAS;
+      // [7n, 8n) the solute bits mol_maps forms (written whole, not cleared).
       const double srad = pbe->solventRadius;
       const bool ion_marks = !(ionstr < VPMGSMALL);
       const size_t n = d->n;
       const size_t fbytes = (7 * n + 3) & ~(size_t)3;
-      unsigned char* mflags = dalloc<unsigned char>(fbytes);
+      unsigned char* mflags = dalloc<unsigned char>(fbytes + n);
       {
         pmg::ZeroList z(d->s);
         z.add(mflags, fbytes);
@@ -2590,7 +2591,8 @@ extern "C" int Vpmg_fillco(Vpmg* thee, Vsurf_Meth surfMeth, double splineWin,
       }
       pmg::mol_maps(d->s, g, mflags, srad > VSMALL ? mflags + 4 * n : nullptr,
                     ion_marks ? mflags + 3 * n : nullptr, epsp, epsw, ionmask,
-                    surfMeth == VSM_MOLSMOOTH ? 1 : 0, d->epsx, d->epsy, d->epsz, d->kappa);
+                    surfMeth == VSM_MOLSMOOTH ? 1 : 0, d->epsx, d->epsy, d->epsz, d->kappa,
+                    mflags + fbytes);
       HIPCHECK(hipStreamSynchronize(d->s));
       dfree(mflags);
     }
