@@ -356,10 +356,13 @@ __global__ void __launch_bounds__(kB)
 // ivdwAccExclus, vacc.c:78-128, 866-922): a point is kept unless another atom
 // b has |pt - b|^2 < (r_b + probe)^2.  One workgroup per atom: the atoms that
 // can exclude any point of this atom's sphere (|a - b| < r_a + r_b + 2 probe,
-// plus a margin) are gathered once into LDS, then every point is tested
-// against that list.  The test is a boolean "any", so the list order does not
-// matter; the arithmetic of each test is the oracle's.
-constexpr int kSasCap = 1024;
+// plus a margin) are gathered once into LDS and ordered nearest first, then
+// every point is tested against that list until one excludes it (most points
+// are buried, and the nearest atoms exclude them first).  The test is a
+// boolean "any", so the list order does not change a flag; the arithmetic of
+// each test is the oracle's.  A list longer than kSasCap falls back to the
+// cell list in global memory.
+constexpr int kSasCap = 256;
 
 __device__ inline bool sas_free_global(const AtomsDev& A, const Cells& C, int self, double px,
                                        double py, double pz, double probe) {
@@ -388,7 +391,8 @@ __device__ inline bool sas_free_global(const AtomsDev& A, const Cells& C, int se
 __global__ void __launch_bounds__(kB)
     k_sas_flags(AtomsDev A, Cells C, const double* ref, int nref, double probe,
                 unsigned char* flags) {
-  __shared__ double sx[kSasCap], sy[kSasCap], sz[kSasCap], sr2[kSasCap];
+  __shared__ double sx[kSasCap], sy[kSasCap], sz[kSasCap], sr2[kSasCap], sd[kSasCap];
+  __shared__ double tx[kSasCap], ty[kSasCap], tz[kSasCap], tr2[kSasCap];
   __shared__ int cnt;
   const int a = blockIdx.x;
   const int tid = threadIdx.x;
@@ -425,6 +429,7 @@ __global__ void __launch_bounds__(kB)
             sy[slot] = bp[1];
             sz[slot] = bp[2];
             sr2[slot] = rr * rr;
+            sd[slot] = d2;
           }
         }
       }
@@ -432,6 +437,19 @@ __global__ void __launch_bounds__(kB)
   }
   __syncthreads();
   const int n = cnt;
+  if (n <= kSasCap) {
+    // nearest first: entry e goes to its rank by (distance, slot)
+    for (int e = tid; e < n; e += kB) {
+      const double de = sd[e];
+      int r = 0;
+      for (int f = 0; f < n; f++) r += (sd[f] < de || (sd[f] == de && f < e)) ? 1 : 0;
+      tx[r] = sx[e];
+      ty[r] = sy[e];
+      tz[r] = sz[e];
+      tr2[r] = sr2[e];
+    }
+    __syncthreads();
+  }
   const double rad = arad + probe;
   for (int q = tid; q < nref; q += kB) {
     const double px = rad * (ref[3 * q]) + ax;
@@ -440,9 +458,9 @@ __global__ void __launch_bounds__(kB)
     unsigned char ok = 1;
     if (n <= kSasCap) {
       for (int b = 0; b < n; b++) {
-        const double d2 = (px - sx[b]) * (px - sx[b]) + (py - sy[b]) * (py - sy[b]) +
-                          (pz - sz[b]) * (pz - sz[b]);
-        if (d2 < sr2[b]) {
+        const double d2 = (px - tx[b]) * (px - tx[b]) + (py - ty[b]) * (py - ty[b]) +
+                          (pz - tz[b]) * (pz - tz[b]);
+        if (d2 < tr2[b]) {
           ok = 0;
           break;
         }
