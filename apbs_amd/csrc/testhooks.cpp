@@ -242,6 +242,26 @@ int pmgtest_gsrb_tile(int nx, int ny, int nz, const double* a, const double* cc,
   return gsrb_impl(nx, ny, nz, 7, a, cc, fc, x, nu, iadjoint, tile);
 }
 
+// Two fused 7-point sweeps from a zero start (the zero_x form of the V-cycle's
+// first smoothing): x's boundary layer must be zero, its interior is ignored
+// (the caller may fill it with anything); tile as pmgtest_gsrb_tile (0: the
+// default shape).  The result is written back to x.
+int pmgtest_gsrb_zero(int nx, int ny, int nz, const double* a, const double* cc,
+                      const double* fc, double* x, int iadjoint, int tile) {
+  const size_t n = (size_t)nx * ny * nz;
+  OpsHolder A(a, n, 7);
+  double *dcc = up(cc, n), *dfc = up(fc, n), *dx = up(x, n), *dD = up(nullptr, n);
+  pmg::add_diag<double>(0, n, A.ops.a[0], dcc, dD);
+  double* dt = up(nullptr, n);  // the partner: zero boundary layer
+  pmg::fill<double>(0, dt, n, 0.0);
+  const pmg::LevelArgs<double> c{A.ops, dD, dfc, dx, dt, nullptr};
+  pmg::gsrb7_fused_batch<double>(0, nx, ny, nz, &c, 1, 2, iadjoint, tile, true);
+  TCHECK(hipDeviceSynchronize());
+  down(x, dx, n);
+  for (double* p : {dcc, dfc, dx, dD, dt}) TCHECK(hipFree(p));
+  return 1;
+}
+
 }  // extern "C"
 
 int gsrb_impl(int nx, int ny, int nz, int st, const double* a, const double* cc, const double* fc,

@@ -47,6 +47,12 @@ int pmgtest_nonlinear(int nx, int ny, int nz, const double *a, const double *cc,
  * (rows*100 + planes: 416, 48, 816, 88, 84, 1616, 832). */
 int pmgtest_gsrb_tile(int nx, int ny, int nz, const double *a, const double *cc,
                       const double *fc, double *x, int nu, int iadjoint, int tile);
+/* Two fused 7-point sweeps from a zero start (the first smoothing of a
+ * mixed-precision correction cycle): x's boundary layer must be zero and its
+ * interior is not read; tile as pmgtest_gsrb_tile (0: the default shape; a
+ * shape without the zero-start form clears x first).  Result in x. */
+int pmgtest_gsrb_zero(int nx, int ny, int nz, const double *a, const double *cc,
+                      const double *fc, double *x, int iadjoint, int tile);
 /* pmgtest_gsrb (27-point) with the kernel form forced: mode 0 one launch per
  * colour, 1 one launch per plane parity (tile by level size), 2 one launch
  * per colour pair (nx <= 130, else returns 0), 8 / 16 / 32 one launch per

@@ -117,6 +117,27 @@ def test_fused_sweep_tiles_bitwise(L, O, tile, n3):
         assert np.array_equal(xg, xo), (tile, adj)
 
 
+@pytest.mark.parametrize("n3", [(17, 19, 21), (161, 23, 29), (133, 21, 38)])
+@pytest.mark.parametrize("tile", [0, 5088, 800048, 416])
+def test_fused_zero_start_bitwise(L, O, n3, tile):
+    """The zero-start form of the fused sweep pair (the first smoothing of a
+    mixed-precision correction cycle) ignores the interior of x (NaN here)
+    and gives the oracle's two sweeps from x = 0, in the default shapes and,
+    by clearing x first, in a shape without that form (416)."""
+    rng = np.random.default_rng(sum(n3) + tile)
+    nx, ny, nz = n3
+    n = nx * ny * nz
+    a = rand_op(rng, n3, 7)
+    cc, fc = rng.uniform(0, 1, n), rng.normal(size=n)
+    for adj in (0, 1):
+        xg = np.zeros(n)
+        xg.reshape(nz, ny, nx)[1:-1, 1:-1, 1:-1] = np.nan
+        xo = np.zeros(n)
+        assert L.pmgtest_gsrb_zero(nx, ny, nz, P(a), P(cc), P(fc), P(xg), adj, tile) == 1
+        O.orc_gsrb(nx, ny, nz, 7, P(a), P(cc), P(fc), P(xo), 2, adj, 1)
+        assert np.array_equal(xg, xo), (n3, tile, adj)
+
+
 @pytest.mark.parametrize("st", [7, 27])
 def test_resid_bitwise(L, O, st):
     rng = np.random.default_rng(11)
