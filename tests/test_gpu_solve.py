@@ -290,6 +290,34 @@ def test_graph_replay_equals_eager(npbe, precision, monkeypatch):
     assert res["1"][0][0] == res["1"][2][0]
 
 
+@pytest.mark.parametrize("env", ["APBS_PMG_SMOOTHER", "APBS_PMG_SMOOTHER27"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_smoother_forms_give_the_same_bits(env, precision, monkeypatch):
+    """The one-launch-per-colour smoother forms (fine level, or every 27-point
+    level) give bitwise the default forms' npbe solve, in FP64 and mixed
+    precision (whose cycles start from zero: the per-colour fine form clears
+    the correction, the fused form's first sweep does not read it)."""
+    atoms = small_mol(300, 9.0, 23, net=-3)
+    pbe = pbe_of("smol", "spl2", npbe=True)
+    mol = driver.Molecule(*atoms)
+    c = od.mol_center(atoms[0])
+    res = {}
+    for form in ("colour", "default"):
+        if form == "colour":
+            monkeypatch.setenv(env, "colour")
+        else:
+            monkeypatch.delenv(env, raising=False)
+        calc = driver.Calc(mol, driver.manual_mgparm([49] * 3, c, grid=[0.45] * 3), pbe,
+                           precision=precision)
+        calc.fillco()
+        calc.solve()
+        res[form] = (calc.energy(1), calc.array("u").copy(), calc.stats()["iters"])
+        calc.release()
+    assert res["colour"][0] == res["default"][0]
+    assert np.array_equal(res["colour"][1], res["default"][1])
+    assert res["colour"][2] == res["default"][2]
+
+
 @pytest.mark.parametrize("srfm,key", [("mol", "actin_mol_auto"), ("smol", "actin_smol_auto")])
 def test_actin_dimer_golden(srfm, key):
     """examples/actin-dimer/apbs-{s}mol-auto.in on the GPU: mg-auto at 161^3,
