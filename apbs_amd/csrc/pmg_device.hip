@@ -799,7 +799,11 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Trilinear prolongation and correction x_f += damp * P x_c (matvecd.c:935-1076
-// followed by Vxaxpy, mgcsd.c:485-487).  Coarse boundary points carry weight 0.
+// followed by Vxaxpy, mgcsd.c:485-487).  The reference gives coarse boundary
+// points weight 0; the coarse solution's boundary layer is zero (never
+// written), so here they are read and add +0: the same terms in the same
+// order, W * x_c with W = 1/2^(number of odd fine indices), without the
+// per-term boundary tests.
 template <class T>
 __device__ __forceinline__ void prolong_body(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
                                              const T* __restrict__ xc, T* __restrict__ xf,
@@ -808,52 +812,24 @@ __device__ __forceinline__ void prolong_body(int nxc, int nyc, int nzc, int nxf,
   const int k = blockIdx.y + 1;
   const int i = 1 + t % (nxf - 2), j = 1 + t / (nxf - 2);
   if (j > nyf - 2) return;
-  const int ii = i >> 1, jj = j >> 1, kk = k >> 1;
   const int bx = i & 1, by = j & 1, bz = k & 1;
-  const T W2 = T(0.5), W4 = T(0.25), W8 = T(0.125);
-  auto in = [&](int a, int b, int c) -> T {
-    return (a > 0 && a < nxc - 1 && b > 0 && b < nyc - 1 && c > 0 && c < nzc - 1) ? T(1.0) : T(0.0);
-  };
-  auto X = [&](int a, int b, int c) -> T { return xc[IX(a, b, c, nxc, nyc)]; };
-  T v;
-  const int sel = bx | (by << 1) | (bz << 2);
-  switch (sel) {
-    case 0:
-      v = X(ii, jj, kk);
-      break;
-    case 1:
-      v = W2 * in(ii, jj, kk) * X(ii, jj, kk) + W2 * in(ii + 1, jj, kk) * X(ii + 1, jj, kk);
-      break;
-    case 2:
-      v = W2 * in(ii, jj, kk) * X(ii, jj, kk) + W2 * in(ii, jj + 1, kk) * X(ii, jj + 1, kk);
-      break;
-    case 4:
-      v = W2 * in(ii, jj, kk) * X(ii, jj, kk) + W2 * in(ii, jj, kk + 1) * X(ii, jj, kk + 1);
-      break;
-    case 3:
-      v = W4 * in(ii, jj, kk) * X(ii, jj, kk) + W4 * in(ii + 1, jj, kk) * X(ii + 1, jj, kk) +
-          W4 * in(ii, jj + 1, kk) * X(ii, jj + 1, kk) +
-          W4 * in(ii + 1, jj + 1, kk) * X(ii + 1, jj + 1, kk);
-      break;
-    case 5:
-      v = W4 * in(ii, jj, kk) * X(ii, jj, kk) + W4 * in(ii + 1, jj, kk) * X(ii + 1, jj, kk) +
-          W4 * in(ii, jj, kk + 1) * X(ii, jj, kk + 1) +
-          W4 * in(ii + 1, jj, kk + 1) * X(ii + 1, jj, kk + 1);
-      break;
-    case 6:
-      v = W4 * in(ii, jj, kk) * X(ii, jj, kk) + W4 * in(ii, jj + 1, kk) * X(ii, jj + 1, kk) +
-          W4 * in(ii, jj, kk + 1) * X(ii, jj, kk + 1) +
-          W4 * in(ii, jj + 1, kk + 1) * X(ii, jj + 1, kk + 1);
-      break;
-    default:
-      v = W8 * in(ii, jj, kk) * X(ii, jj, kk) + W8 * in(ii + 1, jj, kk) * X(ii + 1, jj, kk) +
-          W8 * in(ii, jj + 1, kk) * X(ii, jj + 1, kk) +
-          W8 * in(ii + 1, jj + 1, kk) * X(ii + 1, jj + 1, kk) +
-          W8 * in(ii, jj, kk + 1) * X(ii, jj, kk + 1) +
-          W8 * in(ii + 1, jj, kk + 1) * X(ii + 1, jj, kk + 1) +
-          W8 * in(ii, jj + 1, kk + 1) * X(ii, jj + 1, kk + 1) +
-          W8 * in(ii + 1, jj + 1, kk + 1) * X(ii + 1, jj + 1, kk + 1);
-      break;
+  const T W = (bx ? T(0.5) : T(1.0)) * (by ? T(0.5) : T(1.0)) * (bz ? T(0.5) : T(1.0));
+  const size_t sy = nxc, sz = (size_t)nxc * nyc;
+  const size_t c0 = IX(i >> 1, j >> 1, k >> 1, nxc, nyc);
+  // terms in the order (0,0,0), (1,0,0), (0,1,0), (1,1,0), then the same at kk + 1
+  T v = W * xc[c0];
+  if (bx) v += W * xc[c0 + 1];
+  if (by) {
+    v += W * xc[c0 + sy];
+    if (bx) v += W * xc[c0 + sy + 1];
+  }
+  if (bz) {
+    v += W * xc[c0 + sz];
+    if (bx) v += W * xc[c0 + sz + 1];
+    if (by) {
+      v += W * xc[c0 + sz + sy];
+      if (bx) v += W * xc[c0 + sz + sy + 1];
+    }
   }
   const size_t p = IX(i, j, k, nxf, nyf);
   xf[p] = xf[p] + T(damp[0]) * v;
