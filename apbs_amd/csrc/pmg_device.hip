@@ -637,7 +637,19 @@ __global__ void __launch_bounds__(kBlock)
     const int k = 1 + t / nij, rem = t - (k - 1) * nij;
     const int j = 1 + rem / ni, i = 1 + rem - (j - 1) * ni;
     const size_t p = IX(i, j, k, nx, ny);
-    const T v = acc7(fc[p], oE, oN, uC, x, p, nx, nxy) - D[p] * x[p];
+    // every operand first; acc7's terms and order
+    const T x0 = x[p], xjp = x[p + nx], xjm = x[p - nx], xip = x[p + 1], xim = x[p - 1];
+    const T xkm = x[p - nxy], xkp = x[p + nxy];
+    const T vN = oN[p], vNm = oN[p - nx], vE = oE[p], vEm = oE[p - 1];
+    const T vUm = uC[p - nxy], vU = uC[p], vD = D[p];
+    T sv = fc[p];
+    sv += vN * xjp;
+    sv += vNm * xjm;
+    sv += vE * xip;
+    sv += vEm * xim;
+    sv += vUm * xkm;
+    sv += vU * xkp;
+    const T v = sv - vD * x0;
     if (r) r[p] = v;
     if (r32) r32[p] = (float)v;
     acc += fabs((double)v);
@@ -1071,8 +1083,20 @@ __global__ void __launch_bounds__(kBlock)
     const int k = 1 + t / nij, rem = t - (k - 1) * nij;
     const int j = 1 + rem / ni, i = 1 + rem - (j - 1) * ni;
     const size_t p = IX(i, j, k, nx, ny);
-    const double w = ncvec(cc[p], x[p], ions);
-    const double v = acc7(fc[p], oE, oN, uC, x, p, nx, nxy) - oC[p] * x[p] - w;
+    // every operand first; acc7's terms and order
+    const double x0 = x[p], xjp = x[p + nx], xjm = x[p - nx], xip = x[p + 1], xim = x[p - 1];
+    const double xkm = x[p - nxy], xkp = x[p + nxy];
+    const double vN = oN[p], vNm = oN[p - nx], vE = oE[p], vEm = oE[p - 1];
+    const double vUm = uC[p - nxy], vU = uC[p], vC = oC[p], vcc = cc[p], vfc = fc[p];
+    const double w = ncvec(vcc, x0, ions);
+    double s7 = vfc;
+    s7 += vN * xjp;
+    s7 += vNm * xjm;
+    s7 += vE * xip;
+    s7 += vEm * xim;
+    s7 += vUm * xkm;
+    s7 += vU * xkp;
+    const double v = s7 - vC * x0 - w;
     r[p] = v;
     acc += fabs(v);
   }
@@ -1096,29 +1120,40 @@ __global__ void __launch_bounds__(kBlock)
   const size_t nxy = (size_t)nx * ny;
   int t0, t1;
   part_range(n, t0, t1);
-  auto yv = [&](size_t q) {
-    double v = x[q];
-    v += a * z[q];
-    return v;
-  };
   double acc = 0.0;
   for (int t = t0 + threadIdx.x; t < t1; t += kBlock) {
     const int k = 1 + t / nij, rem = t - (k - 1) * nij;
     const int j = 1 + rem / ni, i = 1 + rem - (j - 1) * ni;
     const size_t p = IX(i, j, k, nx, ny);
-    const double y0 = yv(p);
-    y[p] = y0;
-    const double w = ncvec(cc[p], y0, ions);
-    double s7 = fc[p];
-    s7 += oN[p] * yv(p + nx);
-    s7 += oN[p - nx] * yv(p - nx);
-    s7 += oE[p] * yv(p + 1);
-    s7 += oE[p - 1] * yv(p - 1);
-    s7 += uC[p - nxy] * yv(p - nxy);
-    s7 += uC[p] * yv(p + nxy);
-    const double v = s7 - oC[p] * y0 - w;
-    r[p] = v;
+    // every operand first: the point and its neighbours p, +j, -j, +i, -i, -k, +k
+    const size_t q[7] = {p, p + nx, p - nx, p + 1, p - 1, p - nxy, p + nxy};
+    double xc[7], zc[7];
+#pragma unroll
+    for (int u = 0; u < 7; u++) {
+      xc[u] = x[q[u]];
+      zc[u] = z[q[u]];
+    }
+    const double vN = oN[p], vNm = oN[p - nx], vE = oE[p], vEm = oE[p - 1];
+    const double vUm = uC[p - nxy], vU = uC[p], vC = oC[p], vcc = cc[p], vfc = fc[p];
+    double yc[7];
+#pragma unroll
+    for (int u = 0; u < 7; u++) {
+      double v = xc[u];
+      v += a * zc[u];
+      yc[u] = v;
+    }
+    const double w = ncvec(vcc, yc[0], ions);
+    double s7 = vfc;
+    s7 += vN * yc[1];
+    s7 += vNm * yc[2];
+    s7 += vE * yc[3];
+    s7 += vEm * yc[4];
+    s7 += vUm * yc[5];
+    s7 += vU * yc[6];
+    const double v = s7 - vC * yc[0] - w;
     acc += fabs(v);
+    y[p] = yc[0];
+    r[p] = v;
   }
   double s = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -1133,6 +1168,9 @@ __global__ void __launch_bounds__(kBlock)
 // with the nonlinear residual at y in rn (its FP32 image in rn32 when given),
 // its L1 partials in part[kParts, 2 kParts) (k_nresid7_xpay's values and
 // order).  The two stencils share their operator loads.
+// Every operand of a point is loaded before the arithmetic (the two stencils
+// read z and x at the same seven points); kZF selects z from zf.
+template <bool kZF>
 __global__ void __launch_bounds__(kBlock)
     k_jres_trial(int nx, int ny, int nz, const double* __restrict__ oC,
                  const double* __restrict__ oE, const double* __restrict__ oN,
@@ -1147,42 +1185,53 @@ __global__ void __launch_bounds__(kBlock)
   const size_t nxy = (size_t)nx * ny;
   int t0, t1;
   part_range(n, t0, t1);
-  auto zv = [&](size_t q) -> double { return zf ? (double)zf[q] : z[q]; };
-  auto yv = [&](size_t q) {
-    double v = x[q];
-    v += a * zv(q);
-    return v;
-  };
   double acc_j = 0.0, acc_n = 0.0;
   for (int t = t0 + threadIdx.x; t < t1; t += kBlock) {
     const int k = 1 + t / nij, rem = t - (k - 1) * nij;
     const int j = 1 + rem / ni, i = 1 + rem - (j - 1) * ni;
     const size_t p = IX(i, j, k, nx, ny);
+    // the point and its neighbours in acc7's order: p, +j, -j, +i, -i, -k, +k
+    const size_t q[7] = {p, p + nx, p - nx, p + 1, p - 1, p - nxy, p + nxy};
+    double zc[7], xc[7];
+#pragma unroll
+    for (int u = 0; u < 7; u++) {
+      zc[u] = kZF ? (double)zf[q[u]] : z[q[u]];
+      xc[u] = x[q[u]];
+    }
+    const double vfz = fz[p], vN = oN[p], vNm = oN[p - nx], vE = oE[p], vEm = oE[p - 1];
+    const double vUm = uC[p - nxy], vU = uC[p], vD = D[p], vC = oC[p], vcc = cc[p], vfc = fc[p];
     // acc7's terms and order at z
-    double sj = fz[p];
-    sj += oN[p] * zv(p + nx);
-    sj += oN[p - nx] * zv(p - nx);
-    sj += oE[p] * zv(p + 1);
-    sj += oE[p - 1] * zv(p - 1);
-    sj += uC[p - nxy] * zv(p - nxy);
-    sj += uC[p] * zv(p + nxy);
-    const double vj = sj - D[p] * zv(p);
-    if (r32) r32[p] = (float)vj;
+    double sj = vfz;
+    sj += vN * zc[1];
+    sj += vNm * zc[2];
+    sj += vE * zc[3];
+    sj += vEm * zc[4];
+    sj += vUm * zc[5];
+    sj += vU * zc[6];
+    const double vj = sj - vD * zc[0];
     acc_j += fabs(vj);
-    const double y0 = yv(p);
-    y[p] = y0;
-    const double w = ncvec(cc[p], y0, ions);
-    double s7 = fc[p];
-    s7 += oN[p] * yv(p + nx);
-    s7 += oN[p - nx] * yv(p - nx);
-    s7 += oE[p] * yv(p + 1);
-    s7 += oE[p - 1] * yv(p - 1);
-    s7 += uC[p - nxy] * yv(p - nxy);
-    s7 += uC[p] * yv(p + nxy);
-    const double v = s7 - oC[p] * y0 - w;
+    // y = x + a z at the seven points (k_xpay_int's arithmetic)
+    double yc[7];
+#pragma unroll
+    for (int u = 0; u < 7; u++) {
+      double v = xc[u];
+      v += a * zc[u];
+      yc[u] = v;
+    }
+    const double w = ncvec(vcc, yc[0], ions);
+    double s7 = vfc;
+    s7 += vN * yc[1];
+    s7 += vNm * yc[2];
+    s7 += vE * yc[3];
+    s7 += vEm * yc[4];
+    s7 += vUm * yc[5];
+    s7 += vU * yc[6];
+    const double v = s7 - vC * yc[0] - w;
+    acc_n += fabs(v);
+    if (r32) r32[p] = (float)vj;
+    y[p] = yc[0];
     rn[p] = v;
     if (rn32) rn32[p] = (float)v;
-    acc_n += fabs(v);
   }
   const double bj = block_sum(acc_j, sh);
   const double bn = block_sum(acc_n, sh);
@@ -2099,8 +2148,14 @@ void jres_trial_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A
                      const double* x, double a, const double* z, const float* zf, double* y,
                      double* rn, float* r32, float* rn32, const Ions& ions, double* part,
                      double* out, double* hout, double* out2, double* hout2) {
-  k_jres_trial<<<kParts, kBlock, 0, s>>>(nx, ny, nz, A.a[0], A.a[1], A.a[2], A.a[3], D, cc, fc,
-                                         fz, x, a, z, zf, y, rn, r32, rn32, ions, part);
+  if (zf)
+    k_jres_trial<true><<<kParts, kBlock, 0, s>>>(nx, ny, nz, A.a[0], A.a[1], A.a[2], A.a[3], D, cc,
+                                                 fc, fz, x, a, z, zf, y, rn, r32, rn32, ions,
+                                                 part);
+  else
+    k_jres_trial<false><<<kParts, kBlock, 0, s>>>(nx, ny, nz, A.a[0], A.a[1], A.a[2], A.a[3], D,
+                                                  cc, fc, fz, x, a, z, zf, y, rn, r32, rn32, ions,
+                                                  part);
   Batch<FinishArgs> F;
   F.v[0] = FinishArgs{part, out, hout};
   F.v[1] = FinishArgs{part + kParts, out2, hout2};
