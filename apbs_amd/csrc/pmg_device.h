@@ -49,6 +49,8 @@ template <class T>
 void gsrb7_fused(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D,
                  const T* fc, T* x, T* tmp, int nu, int iadjoint, int tile_shape = 0);
 int gsrb7_fused_launches(int nu);
+// whether the default fused sweep of a grid runs in the vector form (word 4 or 8)
+bool fused_vector(int nx, int ny, int nz, int word);
 template <class T>
 void gsrb27(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, const T* fc,
             T* x, int nu, int iadjoint);

@@ -267,6 +267,202 @@ __global__ void __launch_bounds__(kNT)
                                             a.xi, a.xo, first);
 }
 
+// The vector form of the fused sweep.  The scalar form above loads one
+// 4- or 8-byte word per lane and operand with the lanes two words apart, so
+// the texture addresser (four lanes per clock) and the L1 limit it long
+// before memory does.  Here a lane owns a run of kV consecutive elements of
+// one row (16 bytes: 4 floats or 2 doubles, both colours) and loads every
+// operand as a 16-byte vector; the coefficient vectors that the F step of
+// plane k loads are the S step's operands of plane k one step later, so the
+// S step loads nothing from memory.  Values and their order are the scalar
+// form's (acc7's terms, no FMA).
+//
+// Lanes: region row r (0 .. kTY+1, row j = j0-1+r) and run c of that row
+// (columns i0-1+kV*c ..); a run is moved inwards to [1, nx-1-kV] when it would
+// cross the first or last interior column, so every load stays inside the
+// row and runs near the grid edge may overlap (they then write the same
+// values).  The boundary columns 0 and nx-1 come from the x(p-1) / x(p+1)
+// vectors of the runs beside them.
+template <class T, int N>
+__device__ __forceinline__ void vld16(T (&v)[N], const T* p) {
+  static_assert(sizeof(T) * N == 16, "16-byte vectors");
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  u4 w;
+  __builtin_memcpy(&w, p, 16);
+  __builtin_memcpy(v, &w, 16);
+}
+
+template <class T, int kTY, int kKC, int kTX, int kNT, bool kZeroX>
+__device__ __forceinline__ void vfused_body(int nx, int ny, int nz, int tile,
+                                            const T* __restrict__ oE, const T* __restrict__ oN,
+                                            const T* __restrict__ uC, const T* __restrict__ D,
+                                            const T* __restrict__ fc, const T* __restrict__ xi,
+                                            T* __restrict__ xo, int first) {
+  constexpr int kV = 16 / (int)sizeof(T), kH = kV / 2;
+  constexpr int RX = kTX + 2, RY = kTY + 2, RN = RX * RY;
+  constexpr int NSEG = (RX + kV - 1) / kV;
+  static_assert(RY * NSEG <= kNT, "one lane per run");
+  __shared__ T ring[3][RN];
+  const int tid = threadIdx.x;
+  const int nbx = (nx - 2 + kTX - 1) / kTX, nby = (ny - 2 + kTY - 1) / kTY;
+  const int i0 = 1 + (tile % nbx) * kTX, j0 = 1 + ((tile / nbx) % nby) * kTY;
+  const int k0 = 1 + (tile / (nbx * nby)) * kKC;
+  const int k1 = min(k0 + kKC, nz - 1);  // planes k0 .. k1-1
+  const size_t nxy = (size_t)nx * ny;
+  const int F = first;
+
+  const bool lane = tid < RY * NSEG;
+  const int r = lane ? tid / NSEG : 0, c = lane ? tid % NSEG : 0;
+  const int j = j0 - 1 + r;
+  const bool rowin = lane && j <= ny - 1;
+  const bool rowint = j >= 1 && j <= ny - 2;
+  const bool srow = lane && r >= 1 && r <= kTY && j <= ny - 2;
+  const int jr = min(j, ny - 1), jc = min(max(j, 1), ny - 2);
+  const int is = i0 - 1 + kV * c;
+  const int ib = min(max(is, 1), nx - 1 - kV);
+  const int chi = min(i0 + kTX - 1, nx - 2);  // last column this tile writes
+  // a lane holds, per plane: fc, oN, oN(-nx), oE, oE(-1), uC(-nxy), uC, D and
+  // x(+nx), x(-nx), x(+1), x(-1), x(+nxy), x(-nxy), each at columns ib ..
+  struct Set {
+    T a[8][kV];
+    T x[6][kV];
+  };
+  auto load = [&](int kk, Set& S) {
+    const int kc = min(max(kk, 1), nz - 2);
+    const size_t q = ((size_t)kc * ny + jc) * nx + ib;   // interior row / plane
+    const size_t pr = ((size_t)kk * ny + jr) * nx + ib;  // the lane's own row and plane
+    vld16(S.a[0], fc + q);
+    vld16(S.a[1], oN + q);
+    vld16(S.a[2], oN + q - nx);
+    vld16(S.a[3], oE + q);
+    vld16(S.a[4], oE + q - 1);
+    vld16(S.a[5], uC + q - nxy);
+    vld16(S.a[6], uC + q);
+    vld16(S.a[7], D + q);
+    if (!kZeroX) {
+      vld16(S.x[0], xi + q + nx);
+      vld16(S.x[1], xi + q - nx);
+      vld16(S.x[2], xi + pr + 1);
+      vld16(S.x[3], xi + pr - 1);
+      vld16(S.x[4], xi + q + nxy);
+      vld16(S.x[5], xi + q - nxy);
+    } else {
+#pragma unroll
+      for (int v = 0; v < 6; v++)
+#pragma unroll
+        for (int e = 0; e < kV; e++) S.x[v][e] = T(0);
+    }
+  };
+  // New F values of plane kk into the ring (and fv: the lane's F elements
+  // e0, e0+2, ..., with e0 the first element of F colour)
+  auto stageF = [&](int kk, const Set& S, T (&fv)[kH]) {
+    if (!rowin) return;
+    T* rg = ring[kk % 3] + r * RX - (i0 - 1);
+    const bool inner = rowint && kk >= 1 && kk <= nz - 2;
+    const int e0 = (F - ib - j - kk) & 1;
+#pragma unroll
+    for (int m = 0; m < kH; m++) {
+      // (selects between values, not addresses, so the vectors stay in registers)
+      auto pk = [&](const T(&v)[kV]) {
+        const T lo = v[2 * m], hi = v[2 * m + 1];
+        return e0 ? hi : lo;
+      };
+      T val;
+      if (inner) {
+        T acc = pk(S.a[0]);
+        acc += pk(S.a[1]) * pk(S.x[0]);
+        acc += pk(S.a[2]) * pk(S.x[1]);
+        acc += pk(S.a[3]) * pk(S.x[2]);
+        acc += pk(S.a[4]) * pk(S.x[3]);
+        acc += pk(S.a[5]) * pk(S.x[5]);
+        acc += pk(S.a[6]) * pk(S.x[4]);
+        val = acc / pk(S.a[7]);
+      } else {
+        // a boundary row or plane keeps its values: x at element e is
+        // x(p-1)[e+1], or x(p+1)[e-1] for the last element
+        const T xe = 2 * m + 2 < kV ? S.x[3][(2 * m + 2) % kV] : S.x[2][2 * m];
+        const T xo1 = S.x[3][2 * m + 1];
+        val = e0 ? xe : xo1;
+      }
+      fv[m] = val;
+      const int col = ib + e0 + 2 * m;
+      if (col >= i0 - 1 && col <= i0 + kTX) rg[col] = val;
+    }
+    // boundary columns 0 and nx-1 when they are in the region and of F colour
+    if (is <= 0 && ((j + kk) & 1) == F) rg[0] = S.x[3][0];
+    if (ib == nx - 1 - kV && nx - 1 <= i0 + kTX && ((nx - 1 + j + kk) & 1) == F)
+      rg[nx - 1] = S.x[2][kV - 1];
+  };
+  // S values of plane k from the coefficient vectors of plane k and the ring;
+  // both colours of the lane's tile columns are written
+  auto stageS = [&](int k, const Set& S, const T(&fv)[kH]) {
+    if (!srow) return;
+    const int off = r * RX - (i0 - 1);
+    const T* rm = ring[(k - 1) % 3] + off;
+    const T* r0 = ring[k % 3] + off;
+    const T* rp = ring[(k + 1) % 3] + off;
+    const int e0 = (F - ib - j - k) & 1;
+    T* row = xo + ((size_t)k * ny + j) * nx;
+#pragma unroll
+    for (int m = 0; m < kH; m++) {
+      const int cf = ib + e0 + 2 * m;
+      if (cf >= i0 && cf <= chi) row[cf] = fv[m];
+      const int cs = ib + (1 - e0) + 2 * m;
+      if (cs >= i0 && cs <= chi) {
+        auto pk = [&](const T(&v)[kV]) {
+          const T lo = v[2 * m], hi = v[2 * m + 1];
+          return e0 ? lo : hi;
+        };
+        T acc = pk(S.a[0]);
+        acc += pk(S.a[1]) * r0[cs + RX];
+        acc += pk(S.a[2]) * r0[cs - RX];
+        acc += pk(S.a[3]) * r0[cs + 1];
+        acc += pk(S.a[4]) * r0[cs - 1];
+        acc += pk(S.a[5]) * rm[cs];
+        acc += pk(S.a[6]) * rp[cs];
+        row[cs] = acc / pk(S.a[7]);
+      }
+    }
+  };
+  // one plane step: F(k+1) from n (loaded one step earlier), then S(k) from
+  // c, whose registers then take the loads of plane k+2
+  auto step = [&](int k, Set& cur, T(&fc_)[kH], const Set& nxt, T(&fn)[kH]) {
+    stageF(k + 1, nxt, fn);
+    __syncthreads();
+    stageS(k, cur, fc_);
+    if (k + 2 <= k1) load(k + 2, cur);
+    __syncthreads();
+  };
+
+  Set A, B;
+  T fa[kH], fb[kH];
+  load(k0 - 1, A);
+  stageF(k0 - 1, A, fa);
+  load(k0, A);
+  stageF(k0, A, fa);
+  load(k0 + 1, B);
+  int k = k0;
+  for (; k + 1 < k1; k += 2) {
+    step(k, A, fa, B, fb);
+    step(k + 1, B, fb, A, fa);
+  }
+  if (k < k1) step(k, A, fa, B, fb);
+}
+
+template <class T, int kTY, int kKC, int kTX, int kNT, bool kZeroX = false>
+__global__ void __launch_bounds__(kNT)
+    k_gsrb7_vfused(int nx, int ny, int nz, Batch<SweepArgs<T>> B, int nreq, int first) {
+  const int nbx = (nx - 2 + kTX - 1) / kTX, nby = (ny - 2 + kTY - 1) / kTY;
+  const int nb = nbx * nby * ((nz - 2 + kKC - 1) / kKC);
+  const int total = nb * nreq;
+  const int per = (total + kXCD - 1) / kXCD;
+  const int item = (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
+  if (item >= total) return;
+  const SweepArgs<T>& a = B.v[item / nb];
+  vfused_body<T, kTY, kKC, kTX, kNT, kZeroX>(nx, ny, nz, item % nb, a.oE, a.oN, a.uC, a.D, a.fc,
+                                             a.xi, a.xo, first);
+}
+
 // ----------------------------------------------------------------- 27-point
 template <class T>
 struct P27 {
@@ -1788,16 +1984,33 @@ void fused_sweep(hipStream_t s, int nx, int ny, int nz, const Batch<SweepArgs<T>
   k_gsrb7_fused<T, TY, KC, TX, NT, ZX><<<g, NT, 0, s>>>(nx, ny, nz, B, nb, first);
 }
 
+// The vector form: one lane per 16-byte run of a region row, the workgroup
+// rounded up to whole wavefronts.
+template <class T, int TY, int KC, int TX, bool ZX = false>
+void vfused_sweep(hipStream_t s, int nx, int ny, int nz, const Batch<SweepArgs<T>>& B, int nb,
+                  int first) {
+  constexpr int kV = 16 / (int)sizeof(T);
+  constexpr int NT = (((TY + 2) * ((TX + 2 + kV - 1) / kV) + 63) / 64) * 64;
+  const int tiles = ((nx - 2 + TX - 1) / TX) * ((ny - 2 + TY - 1) / TY) * ((nz - 2 + KC - 1) / KC);
+  const long total = (long)tiles * nb;
+  const unsigned g = (unsigned)(((total + kXCD - 1) / kXCD) * kXCD);
+  k_gsrb7_vfused<T, TY, KC, TX, NT, ZX><<<g, NT, 0, s>>>(nx, ny, nz, B, nb, first);
+}
+
+// The vector-form shapes (70TP: TY rows x P planes) need rows of at least
+// 16 bytes plus the two boundary points.
+static bool vector_shape(int rp) { return rp == 7048 || rp == 7088 || rp == 7068; }
+
 // The default shapes (4 rows x 8 planes, 8 rows x 8 planes with 512
-// threads) also come in the zero-start form; other shapes start from a
-// cleared x.
-static bool zero_start_shape(int rp) { return rp == 48 || rp == 5088; }
+// threads) and the vector shapes also come in the zero-start form; other
+// shapes start from a cleared x.
+static bool zero_start_shape(int rp) { return rp == 48 || rp == 5088 || vector_shape(rp); }
 
 static bool valid_tile(int v) {
   const int w = v / 10000, rp = v % 10000;
   const bool shape = rp == 416 || rp == 48 || rp == 816 || rp == 88 || rp == 84 || rp == 1616 ||
                      rp == 832 || rp == 44 || rp == 68 || rp == 5088 || rp == 5084 ||
-                     rp == 5816;
+                     rp == 5816 || vector_shape(rp);
   return shape && (w == 0 || w == 64 || w == 80 || w == 96);
 }
 
@@ -1813,14 +2026,23 @@ static int fused_tile_env() {
   return t;
 }
 
-// Default shape for a grid.  4 rows x 8 planes (256 threads) up to about
-// 5.5M points, where the fine arrays sit in the Infinity Cache: at 161^3 it
-// is the fastest (65 / 50 us per FP64 / FP32 sweep at 80 columns).  Above,
-// 8 rows x 8 planes with 512 threads (one F and one S round each, a quarter
-// less halo): at 193^3 138 / 71 us against 153 / 77.
-int fused_tile(int nx, int ny, int nz) {
+// Default shape for a grid and word size (standalone sweeps, us per sweep
+// at 161^3 / 193^3).  FP32: the vector form, 4 rows x 8 planes up to about
+// 5.5M points (36.6 against 50.4 for the scalar 4 x 8), 8 rows above (52.0
+// against 70.8 for the scalar 8 x 8 with 512 threads).  FP64: the vector
+// form 4 x 8 up to 5.5M points (58.8 against 65.3); above, where the arrays
+// no longer fit the Infinity Cache, the scalar 8 x 8 with 512 threads
+// (138.7; the vector forms are no faster).
+int fused_tile(int nx, int ny, int nz, int word) {
   if (const int t = fused_tile_env()) return t;
-  return (long)nx * ny * nz > 5500000L ? 5088 : 48;
+  const bool large = (long)nx * ny * nz > 5500000L;
+  if (word == 4) return large ? 7088 : 7048;
+  return large ? 5088 : 7048;
+}
+
+// Whether the default fused sweep of a grid runs in the vector form.
+bool fused_vector(int nx, int ny, int nz, int word) {
+  return vector_shape(fused_tile(nx, ny, nz, word) % 10000) && nx >= 16 / word + 2;
 }
 
 // Tile width for an x extent: the one of 64 / 80 / 96 columns that leaves
@@ -1843,11 +2065,20 @@ static void fused_sweep_w(hipStream_t s, int rp, int nx, int ny, int nz,
   if (zero_x) {
     if (rp == 48)
       fused_sweep<T, 4, 8, TX, kFT, true>(s, nx, ny, nz, B, nb, first);
+    else if (rp == 7048)
+      vfused_sweep<T, 4, 8, TX, true>(s, nx, ny, nz, B, nb, first);
+    else if (rp == 7068)
+      vfused_sweep<T, 6, 8, TX, true>(s, nx, ny, nz, B, nb, first);
+    else if (rp == 7088)
+      vfused_sweep<T, 8, 8, TX, true>(s, nx, ny, nz, B, nb, first);
     else
       fused_sweep<T, 8, 8, TX, 512, true>(s, nx, ny, nz, B, nb, first);
     return;
   }
   switch (rp) {
+    case 7048: vfused_sweep<T, 4, 8, TX>(s, nx, ny, nz, B, nb, first); break;
+    case 7068: vfused_sweep<T, 6, 8, TX>(s, nx, ny, nz, B, nb, first); break;
+    case 7088: vfused_sweep<T, 8, 8, TX>(s, nx, ny, nz, B, nb, first); break;
     case 416: fused_sweep<T, 4, 16, TX>(s, nx, ny, nz, B, nb, first); break;
     case 48: fused_sweep<T, 4, 8, TX>(s, nx, ny, nz, B, nb, first); break;
     case 44: fused_sweep<T, 4, 4, TX>(s, nx, ny, nz, B, nb, first); break;
@@ -1870,9 +2101,11 @@ void gsrb7_fused_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>
   check_batch(nb);
   const int first = iadjoint ? 0 : 1;
   const int tile =
-      tile_shape > 0 && valid_tile(tile_shape) ? tile_shape : fused_tile(nx, ny, nz);
-  const int rp = tile % 10000;
+      tile_shape > 0 && valid_tile(tile_shape) ? tile_shape
+                                               : fused_tile(nx, ny, nz, (int)sizeof(T));
+  int rp = tile % 10000;
   const int w = tile / 10000 ? tile / 10000 : fused_width(nx);
+  if (vector_shape(rp) && nx < 16 / (int)sizeof(T) + 2) rp = 48;
   if (zero_x && !zero_start_shape(rp)) {
     for (int b = 0; b < nb; b++) fill<T>(s, c[b].x, (size_t)nx * ny * nz, T(0));
     zero_x = false;

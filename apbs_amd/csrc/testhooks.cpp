@@ -409,4 +409,33 @@ int pmgtest_nonlinear(int nx, int ny, int nz, const double* a, const double* cc,
   return 1;
 }
 
+// Fused FP32 7-point sweeps (nu of them, or the zero-start pair when zero is
+// set, x's interior then ignored) on the given operator arrays: oE, oN, uC,
+// D, fc and x are nx*ny*nz floats; tile as pmgtest_gsrb_tile.  The partner
+// buffer carries x's boundary.  Result in x.
+int pmgtest_fused32(int nx, int ny, int nz, const float* oE, const float* oN, const float* uC,
+                    const float* D, const float* fc, float* x, int nu, int iadjoint, int tile,
+                    int zero) {
+  if (nx < 3 || ny < 3 || nz < 3 || (!zero && nu < 1)) return 0;
+  const size_t n = (size_t)nx * ny * nz;
+  auto upf = [&](const float* h) {
+    float* d = nullptr;
+    TCHECK(hipMalloc(&d, n * sizeof(float)));
+    TCHECK(hipMemcpy(d, h, n * sizeof(float), hipMemcpyHostToDevice));
+    return d;
+  };
+  float *dE = upf(oE), *dN = upf(oN), *dU = upf(uC), *dD = upf(D), *df = upf(fc), *dx = upf(x),
+        *dt = upf(x);
+  pmg::Ops<float> ops{};
+  ops.a[1] = dE;
+  ops.a[2] = dN;
+  ops.a[3] = dU;
+  const pmg::LevelArgs<float> c{ops, dD, df, dx, dt, nullptr};
+  pmg::gsrb7_fused_batch<float>(0, nx, ny, nz, &c, 1, zero ? 2 : nu, iadjoint, tile, zero != 0);
+  TCHECK(hipDeviceSynchronize());
+  TCHECK(hipMemcpy(x, dx, n * sizeof(float), hipMemcpyDeviceToHost));
+  for (float* q : {dE, dN, dU, dD, df, dx, dt}) TCHECK(hipFree(q));
+  return 1;
+}
+
 }  // extern "C"

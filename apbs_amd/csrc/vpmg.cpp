@@ -762,7 +762,8 @@ struct PmgDev {
     hipGraphExec_t e;
   };
   std::vector<CycleGraph> cycle_graphs;
-  double kt[7] = {};  // ms0, launches0, ms1, count1, word bytes, fine points, words/pt/launch
+  // ms0, launches0, ms1, count1, word bytes, fine points, words/pt/launch, vector form
+  double kt[8] = {};
 };
 
 PmgDev* DEV(Vpmg* p) { return (PmgDev*)p->dev; }
@@ -1351,6 +1352,7 @@ void vcycle_batch(PmgDev* const* ds, CycleArgs<T>* cs, int n) {
     d0->kt[5] = (double)L[0].n;
     // algorithmic words per fine point per smoother launch: 7 for a full sweep
     d0->kt[6] = d0->smoother == 0 ? 7.0 : 3.5;
+    d0->kt[7] = d0->smoother == 0 && pmg::fused_vector(L[0].nx, L[0].ny, L[0].nz, (int)sizeof(T));
   }
   pmg::LevelArgs<T> la[pmg::kMaxBatch];
   pmg::TransferArgs<T> ta[pmg::kMaxBatch];
@@ -3434,7 +3436,7 @@ extern "C" int Vpmg_setKernelTiming(Vpmg* thee, int on) {
 extern "C" int Vpmg_getKernelTiming(Vpmg* thee, double* out, int n) {
   if (!thee) return 0;
   PmgDev* d = DEV(thee);
-  int m = std::min(n, 7);
+  int m = std::min(n, 8);
   for (int i = 0; i < m; i++) out[i] = d->kt[i];
   return m;
 }

@@ -252,11 +252,12 @@ class Calc:
         lib().Vpmg_setKernelTiming(self.pmg, 1 if on else 0)
 
     def kernel_times(self):
-        out = np.zeros(7)
-        lib().Vpmg_getKernelTiming(self.pmg, _dp(out), 7)
+        out = np.zeros(8)
+        lib().Vpmg_getKernelTiming(self.pmg, _dp(out), 8)
         return {"smoother_ms": float(out[0]), "smoother_launches": int(out[1]),
                 "vcycle_ms": float(out[2]), "vcycles": int(out[3]), "word": int(out[4]),
-                "points": int(out[5]), "words_per_point": float(out[6])}
+                "points": int(out[5]), "words_per_point": float(out[6]),
+                "vector_form": bool(out[7])}
 
     def release(self):
         L = lib()

@@ -171,7 +171,8 @@ def smoother_roofline(kt):
     achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kt["word"]),
-            "kernel": "%s<%s>" % ("k_gsrb7_fused" if kt["words_per_point"] == 7 else "k_gsrb7",
+            "kernel": "%s<%s>" % ("k_gsrb7_vfused" if kt.get("vector_form") else
+                                  "k_gsrb7_fused" if kt["words_per_point"] == 7 else "k_gsrb7",
                                   "float" if kt["word"] == 4 else "double"),
             "bytes_per_launch": bytes_per_launch, "avg_launch_us": per_launch_ms * 1e3}
 

@@ -405,11 +405,12 @@ int Vpmg_getStats(Vpmg *thee, double *out, int n);
  * 4 kappa, 5 charge, 6 pvec, 7 gxcf (2*ny*nz), 8 gycf, 9 gzcf. */
 int Vpmg_copyArray(Vpmg *thee, int which, double *out, size_t n);
 /* Kernel timing with HIP events on the object's stream: on != 0 starts (and
- * zeroes) the accumulators, 0 stops.  Vpmg_getKernelTiming writes up to 7
+ * zeroes) the accumulators, 0 stops.  Vpmg_getKernelTiming writes up to 8
  * values: [0] fine-level smoother time (ms), [1] its kernel launches,
  * [2] V-cycle time (ms), [3] V-cycles, [4] word size of the last cycle (4 or
  * 8 bytes), [5] fine-level points, [6] algorithmic words moved per fine
- * point by one smoother launch. */
+ * point by one smoother launch, [7] 1 when the fused sweep ran in its vector
+ * form (16-byte loads), else 0. */
 int Vpmg_setKernelTiming(Vpmg *thee, int on);
 int Vpmg_getKernelTiming(Vpmg *thee, double *out, int n);
 /* Vpmg_energy(thee, extFlag) computed without allocating any memory, for

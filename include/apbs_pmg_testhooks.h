@@ -53,6 +53,13 @@ int pmgtest_gsrb_tile(int nx, int ny, int nz, const double *a, const double *cc,
  * shape without the zero-start form clears x first).  Result in x. */
 int pmgtest_gsrb_zero(int nx, int ny, int nz, const double *a, const double *cc,
                       const double *fc, double *x, int iadjoint, int tile);
+/* Fused FP32 7-point sweeps on explicit operator arrays (oE, oN, uC, the
+ * diagonal D, fc and x: nx*ny*nz floats each): nu sweeps, or with zero set the
+ * zero-start pair (x's interior not read); tile as pmgtest_gsrb_tile.  Result
+ * in x; returns 0 on bad sizes. */
+int pmgtest_fused32(int nx, int ny, int nz, const float *oE, const float *oN, const float *uC,
+                    const float *D, const float *fc, float *x, int nu, int iadjoint, int tile,
+                    int zero);
 /* pmgtest_gsrb (27-point) with the kernel form forced: mode 0 one launch per
  * colour, 1 one launch per plane parity (tile by level size), 2 one launch
  * per colour pair (nx <= 130, else returns 0), 8 / 16 / 32 one launch per

@@ -97,7 +97,7 @@ def test_gsrb27_plane_sweeps_bitwise(L, O, n3, mode):
 
 @pytest.mark.parametrize("tile", [416, 48, 816, 88, 84, 1616, 832, 44, 68, 640048, 800048,
                                   960048, 800044, 800068, 960416, 800816, 5088, 645084,
-                                  805816])
+                                  805816, 7048, 7068, 7088, 647048, 807088, 967068])
 @pytest.mark.parametrize("n3", [(133, 21, 38), (133, 21, 41)])
 def test_fused_sweep_tiles_bitwise(L, O, tile, n3):
     """Every tile shape of the fused red-black sweep gives the oracle's
@@ -118,7 +118,7 @@ def test_fused_sweep_tiles_bitwise(L, O, tile, n3):
 
 
 @pytest.mark.parametrize("n3", [(17, 19, 21), (161, 23, 29), (133, 21, 38)])
-@pytest.mark.parametrize("tile", [0, 5088, 800048, 416])
+@pytest.mark.parametrize("tile", [0, 5088, 800048, 416, 7048, 807088])
 def test_fused_zero_start_bitwise(L, O, n3, tile):
     """The zero-start form of the fused sweep pair (the first smoothing of a
     mixed-precision correction cycle) ignores the interior of x (NaN here)
@@ -136,6 +136,95 @@ def test_fused_zero_start_bitwise(L, O, n3, tile):
         assert L.pmgtest_gsrb_zero(nx, ny, nz, P(a), P(cc), P(fc), P(xg), adj, tile) == 1
         O.orc_gsrb(nx, ny, nz, 7, P(a), P(cc), P(fc), P(xo), 2, adj, 1)
         assert np.array_equal(xg, xo), (n3, tile, adj)
+
+
+def rb_sweeps(oE, oN, uC, D, fc, x, first, nu):
+    """In-place red-black sweeps in the arrays' own precision, colour
+    `first` ((i + j + k) & 1) first, acc7's terms in its order with every
+    product and sum rounded (numpy does not contract)."""
+    x = x.copy()
+    nz, ny, nx = x.shape
+    kk, jj, ii = np.meshgrid(np.arange(1, nz - 1), np.arange(1, ny - 1), np.arange(1, nx - 1),
+                             indexing="ij")
+    par = (ii + jj + kk) & 1
+
+    def sh(a, dk, dj, di):
+        return a[1 + dk:nz - 1 + dk, 1 + dj:ny - 1 + dj, 1 + di:nx - 1 + di]
+
+    for _ in range(nu):
+        for colour in (first, 1 - first):
+            acc = sh(fc, 0, 0, 0).copy()
+            acc = acc + sh(oN, 0, 0, 0) * sh(x, 0, 1, 0)
+            acc = acc + sh(oN, 0, -1, 0) * sh(x, 0, -1, 0)
+            acc = acc + sh(oE, 0, 0, 0) * sh(x, 0, 0, 1)
+            acc = acc + sh(oE, 0, 0, -1) * sh(x, 0, 0, -1)
+            acc = acc + sh(uC, -1, 0, 0) * sh(x, -1, 0, 0)
+            acc = acc + sh(uC, 0, 0, 0) * sh(x, 1, 0, 0)
+            new = acc / sh(D, 0, 0, 0)
+            xi = x[1:-1, 1:-1, 1:-1]
+            xi[par == colour] = new[par == colour]
+    return x
+
+
+def test_rb_sweeps_reference_is_the_oracle(O):
+    """The numpy sweep used for the FP32 kernels, run in FP64, gives the
+    oracle's sweeps bit for bit (so it follows the reference's colour order
+    and term order)."""
+    rng = np.random.default_rng(5)
+    n3 = (13, 11, 9)
+    nx, ny, nz = n3
+    n = nx * ny * nz
+    a = rand_op(rng, n3, 7)
+    cc, fc = rng.uniform(0, 1, n), rng.normal(size=n)
+    x0 = rng.normal(size=n)
+    for adj in (0, 1):
+        xo = x0.copy()
+        O.orc_gsrb(nx, ny, nz, 7, P(a), P(cc), P(fc), P(xo), 2, adj, 1)
+        A = a.reshape(4, nz, ny, nx)
+        D = A[0] + cc.reshape(nz, ny, nx)
+        xr = rb_sweeps(A[1], A[2], A[3], D, fc.reshape(nz, ny, nx), x0.reshape(nz, ny, nx),
+                       0 if adj else 1, 2)
+        assert np.array_equal(xr.ravel(), xo), adj
+
+
+fp = ctypes.POINTER(ctypes.c_float)
+
+
+def F32(a):
+    return a.ctypes.data_as(fp)
+
+
+@pytest.mark.parametrize("n3", [(133, 21, 38), (133, 21, 41), (17, 19, 21), (161, 23, 29),
+                                (7, 9, 11), (5, 8, 9)])
+@pytest.mark.parametrize("tile", [0, 48, 5088, 7048, 7068, 7088, 647048, 967088])
+def test_fused_fp32_bitwise(L, n3, tile):
+    """The FP32 fused sweeps (scalar and vector forms, full and zero-start)
+    against the numpy FP32 sweeps, bit for bit, with a non-zero boundary (a
+    vector form needs rows of 6 points; narrower grids take the scalar
+    form)."""
+    rng = np.random.default_rng(sum(n3) * 7 + tile)
+    nx, ny, nz = n3
+    sh = (nz, ny, nx)
+    f32 = np.float32
+    oE, oN, uC = (rng.uniform(0.1, 1.0, sh).astype(f32) for _ in range(3))
+    D = rng.uniform(6.0, 12.0, sh).astype(f32)
+    fc = rng.normal(size=sh).astype(f32)
+    x0 = rng.normal(size=sh).astype(f32)
+    for adj in (0, 1):
+        for nu in (1, 2, 3):
+            xg = x0.copy()
+            assert L.pmgtest_fused32(nx, ny, nz, F32(oE), F32(oN), F32(uC), F32(D), F32(fc),
+                                     F32(xg), nu, adj, tile, 0) == 1
+            xr = rb_sweeps(oE, oN, uC, D, fc, x0, 0 if adj else 1, nu)
+            assert np.array_equal(xg, xr), (tile, adj, nu)
+        # the zero-start pair: zero boundary, interior not read
+        xz = np.zeros(sh, f32)
+        xg = xz.copy()
+        xg[1:-1, 1:-1, 1:-1] = np.nan
+        assert L.pmgtest_fused32(nx, ny, nz, F32(oE), F32(oN), F32(uC), F32(D), F32(fc),
+                                 F32(xg), 2, adj, tile, 1) == 1
+        xr = rb_sweeps(oE, oN, uC, D, fc, xz, 0 if adj else 1, 2)
+        assert np.array_equal(xg, xr), (tile, adj, "zero start")
 
 
 @pytest.mark.parametrize("st", [7, 27])
