@@ -292,6 +292,15 @@ __device__ __forceinline__ void vld16(T (&v)[N], const T* p) {
   __builtin_memcpy(v, &w, 16);
 }
 
+template <class T, int N>
+__device__ __forceinline__ void vst16(T* p, const T (&v)[N]) {
+  static_assert(sizeof(T) * N == 16, "16-byte vectors");
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  u4 w;
+  __builtin_memcpy(&w, v, 16);
+  __builtin_memcpy(p, &w, 16);
+}
+
 template <class T, int kTY, int kKC, int kTX, int kNT, bool kZeroX>
 __device__ __forceinline__ void vfused_body(int nx, int ny, int nz, int tile,
                                             const T* __restrict__ oE, const T* __restrict__ oN,
@@ -403,12 +412,16 @@ __device__ __forceinline__ void vfused_body(int nx, int ny, int nz, int tile,
     const T* rp = ring[(k + 1) % 3] + off;
     const int e0 = (F - ib - j - k) & 1;
     T* row = xo + ((size_t)k * ny + j) * nx;
+    // a run wholly inside the tile's columns is written with one 16-byte
+    // store; the runs at the tile's edges element by element
+    const bool full = ib >= i0 && ib + kV - 1 <= chi;
+    T out[kV];
 #pragma unroll
     for (int m = 0; m < kH; m++) {
       const int cf = ib + e0 + 2 * m;
-      if (cf >= i0 && cf <= chi) row[cf] = fv[m];
       const int cs = ib + (1 - e0) + 2 * m;
-      if (cs >= i0 && cs <= chi) {
+      T sv = T(0);
+      if (full || (cs >= i0 && cs <= chi)) {
         auto pk = [&](const T(&v)[kV]) {
           const T lo = v[2 * m], hi = v[2 * m + 1];
           return e0 ? lo : hi;
@@ -420,9 +433,17 @@ __device__ __forceinline__ void vfused_body(int nx, int ny, int nz, int tile,
         acc += pk(S.a[4]) * r0[cs - 1];
         acc += pk(S.a[5]) * rm[cs];
         acc += pk(S.a[6]) * rp[cs];
-        row[cs] = acc / pk(S.a[7]);
+        sv = acc / pk(S.a[7]);
+      }
+      const T f = fv[m];
+      out[2 * m] = e0 ? sv : f;
+      out[2 * m + 1] = e0 ? f : sv;
+      if (!full) {
+        if (cf >= i0 && cf <= chi) row[cf] = f;
+        if (cs >= i0 && cs <= chi) row[cs] = sv;
       }
     }
+    if (full) vst16(row + ib, out);
   };
   // one plane step: F(k+1) from n (loaded one step earlier), then S(k) from
   // c, whose registers then take the loads of plane k+2
