@@ -40,11 +40,14 @@ int gsrb7_launches(int nu);
 // between x and tmp (fine-size scratch).  Only interior points are written,
 // so x and tmp must hold the same boundary values (zero in the V-cycle).
 // The result is bitwise that of gsrb7 and ends in x.
-// tile_shape: width*10000 + rows*100 + planes of a workgroup's block (rows
-// and planes 416, 48, 44, 68, 816, 88, 84, 1616, 832 with 256 threads, or
-// 5088, 5084, 5816 for 8 rows with 512 threads; width 64, 80, 96, or 0 for
-// the width that fits nx best); <= 0 takes APBS_PMG_FUSED_TILE, else 48 up
-// to 5.5M points and 5088 above.
+// tile_shape: width*10000 + rows*100 + planes of a workgroup's block for the
+// scalar form (rows and planes 416, 48, 44, 68, 816, 88, 84, 1616, 832 with
+// 256 threads, or 5088, 5084, 5816 for 8 rows with 512 threads), or
+// width*10000 + a vector-form code (7048, 7088: 4 or 8 rows x 8 planes;
+// 7410, 7412, 7416: 4 rows x 10, 12, 16 planes; 7816: 8 rows x 16 planes;
+// one lane per 16-byte run); width 64, 80, 96, or 0 for the width that fits
+// nx best; <= 0 takes APBS_PMG_FUSED_TILE, else the vector shape measured
+// fastest for the grid's size and the word size.
 template <class T>
 void gsrb7_fused(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D,
                  const T* fc, T* x, T* tmp, int nu, int iadjoint, int tile_shape = 0);

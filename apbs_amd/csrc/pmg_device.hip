@@ -2018,9 +2018,30 @@ void vfused_sweep(hipStream_t s, int nx, int ny, int nz, const Batch<SweepArgs<T
   k_gsrb7_vfused<T, TY, KC, TX, NT, ZX><<<g, NT, 0, s>>>(nx, ny, nz, B, nb, first);
 }
 
-// The vector-form shapes (70TP: TY rows x P planes) need rows of at least
-// 16 bytes plus the two boundary points.
-static bool vector_shape(int rp) { return rp == 7048 || rp == 7088 || rp == 7068; }
+// The vector-form shapes (70TP or 7TPP: TY rows x P planes) need rows of at
+// least 16 bytes plus the two boundary points.
+static bool vector_shape(int rp) {
+  switch (rp) {
+    case 7048: case 7088: case 7410: case 7412: case 7416: case 7816:
+      return true;
+    default:
+      return false;
+  }
+}
+
+// Launches the vector form of shape rp (vector_shape(rp) holds).
+template <class T, int TX, bool ZX>
+void vfused_dispatch(hipStream_t s, int rp, int nx, int ny, int nz, const Batch<SweepArgs<T>>& B,
+                     int nb, int first) {
+  switch (rp) {
+    case 7048: vfused_sweep<T, 4, 8, TX, ZX>(s, nx, ny, nz, B, nb, first); break;
+    case 7088: vfused_sweep<T, 8, 8, TX, ZX>(s, nx, ny, nz, B, nb, first); break;
+    case 7410: vfused_sweep<T, 4, 10, TX, ZX>(s, nx, ny, nz, B, nb, first); break;
+    case 7412: vfused_sweep<T, 4, 12, TX, ZX>(s, nx, ny, nz, B, nb, first); break;
+    case 7416: vfused_sweep<T, 4, 16, TX, ZX>(s, nx, ny, nz, B, nb, first); break;
+    default: vfused_sweep<T, 8, 16, TX, ZX>(s, nx, ny, nz, B, nb, first); break;
+  }
+}
 
 // The default shapes (4 rows x 8 planes, 8 rows x 8 planes with 512
 // threads) and the vector shapes also come in the zero-start form; other
@@ -2047,18 +2068,26 @@ static int fused_tile_env() {
   return t;
 }
 
-// Default shape for a grid and word size (standalone sweeps, us per sweep
-// at 161^3 / 193^3).  FP32: the vector form, 4 rows x 8 planes up to about
-// 5.5M points (36.6 against 50.4 for the scalar 4 x 8), 8 rows above (52.0
-// against 70.8 for the scalar 8 x 8 with 512 threads).  FP64: the vector
-// form 4 x 8 up to 5.5M points (58.8 against 65.3); above, where the arrays
-// no longer fit the Infinity Cache, the scalar 8 x 8 with 512 threads
-// (138.7; the vector forms are no faster).
+// Default shape for a grid and word size: the vector form, with the rows x
+// planes per workgroup that were fastest in standalone sweeps (us per sweep;
+// the best shape moves with how the workgroups fill the chip):
+//   FP32: up to 2.5M points 4 x 8 (65^3 12.2, 97^3 15.0, 129^3 20.3); up to
+//         5.5M 4 x 10 (161^3 29.7 against 36.5 for 4 x 8 and 50.2 for the
+//         scalar form); above 8 x 16 (193^3 49.3, 353^3 421 against 70.8 and
+//         573 for the scalar 8 x 8 with 512 threads).
+//   FP64: up to 0.5M points 4 x 8 (65^3 12.9); up to 1M 4 x 10 (97^3 18.7);
+//         up to 3M 4 x 12 (129^3 31.8); up to 5.5M 4 x 10 (161^3 48.9 against
+//         65.4 scalar); up to 20M 4 x 12 (193^3 125.4 against 137.9); above
+//         4 x 16 (353^3 822 against 931).
 int fused_tile(int nx, int ny, int nz, int word) {
   if (const int t = fused_tile_env()) return t;
-  const bool large = (long)nx * ny * nz > 5500000L;
-  if (word == 4) return large ? 7088 : 7048;
-  return large ? 5088 : 7048;
+  const long n = (long)nx * ny * nz;
+  if (word == 4) return n <= 2500000L ? 7048 : n <= 5500000L ? 7410 : 7816;
+  if (n <= 500000L) return 7048;
+  if (n <= 1000000L) return 7410;
+  if (n <= 3000000L) return 7412;
+  if (n <= 5500000L) return 7410;
+  return n <= 20000000L ? 7412 : 7416;
 }
 
 // Whether the default fused sweep of a grid runs in the vector form.
@@ -2086,20 +2115,17 @@ static void fused_sweep_w(hipStream_t s, int rp, int nx, int ny, int nz,
   if (zero_x) {
     if (rp == 48)
       fused_sweep<T, 4, 8, TX, kFT, true>(s, nx, ny, nz, B, nb, first);
-    else if (rp == 7048)
-      vfused_sweep<T, 4, 8, TX, true>(s, nx, ny, nz, B, nb, first);
-    else if (rp == 7068)
-      vfused_sweep<T, 6, 8, TX, true>(s, nx, ny, nz, B, nb, first);
-    else if (rp == 7088)
-      vfused_sweep<T, 8, 8, TX, true>(s, nx, ny, nz, B, nb, first);
+    else if (vector_shape(rp))
+      vfused_dispatch<T, TX, true>(s, rp, nx, ny, nz, B, nb, first);
     else
       fused_sweep<T, 8, 8, TX, 512, true>(s, nx, ny, nz, B, nb, first);
     return;
   }
+  if (vector_shape(rp)) {
+    vfused_dispatch<T, TX, false>(s, rp, nx, ny, nz, B, nb, first);
+    return;
+  }
   switch (rp) {
-    case 7048: vfused_sweep<T, 4, 8, TX>(s, nx, ny, nz, B, nb, first); break;
-    case 7068: vfused_sweep<T, 6, 8, TX>(s, nx, ny, nz, B, nb, first); break;
-    case 7088: vfused_sweep<T, 8, 8, TX>(s, nx, ny, nz, B, nb, first); break;
     case 416: fused_sweep<T, 4, 16, TX>(s, nx, ny, nz, B, nb, first); break;
     case 48: fused_sweep<T, 4, 8, TX>(s, nx, ny, nz, B, nb, first); break;
     case 44: fused_sweep<T, 4, 4, TX>(s, nx, ny, nz, B, nb, first); break;
