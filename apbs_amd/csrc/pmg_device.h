@@ -261,6 +261,11 @@ void gsrb27_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, 
 template <class T>
 void gsrb27_pairs_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb,
                         int nu, int iadjoint);
+// The same with the kernel chosen: form 0 the vector kernel (16-byte runs per
+// lane) where the row allows it, 1 the one-point-per-lane kernel.
+template <class T>
+void gsrb27_pairs_form(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb,
+                       int nu, int iadjoint, int form);
 template <class T>
 void gsrb27_planes_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb,
                          int nu, int iadjoint, int tile = 0);

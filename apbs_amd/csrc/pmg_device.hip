@@ -8,6 +8,8 @@
 // in-place lexicographic sweep (gsd.c:230-327), whose dependencies are serial.
 #include "pmg_device.h"
 
+#include <type_traits>
+
 #include <cstdio>
 #include <cfloat>
 #include <cmath>
@@ -286,19 +288,21 @@ __global__ void __launch_bounds__(kNT)
 template <class T, int N>
 __device__ __forceinline__ void vld16(T (&v)[N], const T* p) {
   static_assert(sizeof(T) * N == 16, "16-byte vectors");
-  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-  u4 w;
-  __builtin_memcpy(&w, p, 16);
-  __builtin_memcpy(v, &w, 16);
+  typedef T vT __attribute__((ext_vector_type(N)));
+  vT a;
+  __builtin_memcpy(&a, p, 16);
+#pragma unroll
+  for (int i = 0; i < N; i++) v[i] = a[i];
 }
 
 template <class T, int N>
 __device__ __forceinline__ void vst16(T* p, const T (&v)[N]) {
   static_assert(sizeof(T) * N == 16, "16-byte vectors");
-  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-  u4 w;
-  __builtin_memcpy(&w, v, 16);
-  __builtin_memcpy(p, &w, 16);
+  typedef T vT __attribute__((ext_vector_type(N)));
+  vT a;
+#pragma unroll
+  for (int i = 0; i < N; i++) a[i] = v[i];
+  __builtin_memcpy(p, &a, 16);
 }
 
 template <class T, int kTY, int kKC, int kTX, int kNT, bool kZeroX>
@@ -656,6 +660,176 @@ __global__ void __launch_bounds__(kBlock)
     k_gsrb27_pair(int nx, int ny, int nz, Batch<Args27<T>> B, int pa, int cj, int ck) {
   const Args27<T>& a = B.v[blockIdx.z];
   pair_body<T>(nx, ny, nz, a.A, a.D, a.f, a.x, pa, cj, ck);
+}
+
+// The vector form of the colour-pair launch.  In the form above a lane loads
+// one word per operand, with the lanes two words apart, so its 54 loads per
+// point keep the texture addresser (four lanes per clock) busy for most of
+// the launch.  Here a lane owns a run of kV consecutive points of a row (16
+// bytes: 4 floats or 2 doubles, both colours): each of the 26 coefficient
+// terms, f and D is one 16-byte load, and each of the 9 neighbouring rows of
+// x one window of kV + 2 values.  The A points of the run are updated first
+// (their row neighbours are B points, still old) and written to a row buffer
+// in LDS; after a barrier the B points take their new A neighbours from it.
+// The last run of a row is moved inwards so that every load stays inside its
+// row; overlapping runs compute and write the same values.  part27's terms in
+// its order, so the results are the scalar form's bit for bit.
+template <class T, int N>
+__device__ __forceinline__ void vldw(T (&w)[N + 2], const T* p) {
+  static_assert(sizeof(T) * N == 16, "16-byte vectors");
+  typedef T vT __attribute__((ext_vector_type(N)));
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  vT a;
+  v2 b;
+  __builtin_memcpy(&a, p, 16);
+  __builtin_memcpy(&b, p + N, 2 * sizeof(T));
+#pragma unroll
+  for (int i = 0; i < N; i++) w[i] = a[i];
+  w[N] = b[0];
+  w[N + 1] = b[1];
+}
+
+// R runs per row, RB rows per workgroup; LDS holds RB rows of new A values.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_gsrb27_vpair(int nx, int ny, int nz, Batch<Args27<T>> B, int pa, int cj, int ck, int R,
+                   int RB) {
+  constexpr int kV = 16 / (int)sizeof(T), kH = kV / 2;
+  extern __shared__ unsigned char vpair_smem[];
+  T* na = reinterpret_cast<T*>(vpair_smem);
+  const Args27<T>& ar = B.v[blockIdx.z];
+  const T *oE = ar.A.a[1], *oN = ar.A.a[2], *uC = ar.A.a[3], *oNE = ar.A.a[4],
+          *oNW = ar.A.a[5], *uE = ar.A.a[6], *uW = ar.A.a[7], *uN = ar.A.a[8], *uS = ar.A.a[9],
+          *uNE = ar.A.a[10], *uNW = ar.A.a[11], *uSE = ar.A.a[12], *uSW = ar.A.a[13];
+  const T *D = ar.D, *fc = ar.f;
+  T* x = ar.x;
+  const int j0 = cj ? 1 : 2, k0 = ck ? 1 : 2;
+  const int nj = (ny - 2 - j0) / 2 + 1, nk = (nz - 2 - k0) / 2 + 1;
+  const int lr = (int)threadIdx.x / R, run = (int)threadIdx.x % R;
+  const int row = (int)blockIdx.x * RB + lr;
+  const bool act = lr < RB && row < nj * nk;
+  const int rr = act ? row : 0;  // idle lanes load row 0 of the class (unused)
+  const int j = j0 + 2 * (rr % nj), k = k0 + 2 * (rr / nj);
+  const int ib = min(1 + kV * run, nx - 1 - kV);
+  const size_t nxy = (size_t)nx * ny;
+  const size_t p = (size_t)k * nxy + (size_t)j * nx + ib;
+  const size_t q = p + nxy, m = p - nxy;
+  T* nrow = na + (size_t)(act ? lr : 0) * nx;
+
+  // coefficient vectors (columns ib .. ib+kV-1 of each term)
+  T c[28][kV];
+  vld16(c[0], oN + p);
+  vld16(c[1], oN + p - nx);
+  vld16(c[2], oE + p);
+  vld16(c[3], oE + p - 1);
+  vld16(c[4], oNE + p);
+  vld16(c[5], oNW + p);
+  vld16(c[6], oNW + p - nx + 1);
+  vld16(c[7], oNE + p - nx - 1);
+  vld16(c[8], uC + p);
+  vld16(c[9], uN + p);
+  vld16(c[10], uS + p);
+  vld16(c[11], uE + p);
+  vld16(c[12], uW + p);
+  vld16(c[13], uNE + p);
+  vld16(c[14], uNW + p);
+  vld16(c[15], uSE + p);
+  vld16(c[16], uSW + p);
+  vld16(c[17], uC + m);
+  vld16(c[18], uS + m + nx);
+  vld16(c[19], uN + m - nx);
+  vld16(c[20], uW + m + 1);
+  vld16(c[21], uE + m - 1);
+  vld16(c[22], uSW + m + nx + 1);
+  vld16(c[23], uSE + m + nx - 1);
+  vld16(c[24], uNW + m - nx + 1);
+  vld16(c[25], uNE + m - nx - 1);
+  vld16(c[26], fc + p);
+  vld16(c[27], D + p);
+  // x windows (columns ib-1 .. ib+kV) of the 9 rows: 0 (j,k), 1 (j+1,k),
+  // 2 (j-1,k), 3 (j,k+1), 4 (j+1,k+1), 5 (j-1,k+1), 6 (j,k-1), 7 (j+1,k-1),
+  // 8 (j-1,k-1)
+  T w[9][kV + 2];
+  vldw<T, kV>(w[0], x + p - 1);
+  vldw<T, kV>(w[1], x + p + nx - 1);
+  vldw<T, kV>(w[2], x + p - nx - 1);
+  vldw<T, kV>(w[3], x + q - 1);
+  vldw<T, kV>(w[4], x + q + nx - 1);
+  vldw<T, kV>(w[5], x + q - nx - 1);
+  vldw<T, kV>(w[6], x + m - 1);
+  vldw<T, kV>(w[7], x + m + nx - 1);
+  vldw<T, kV>(w[8], x + m - nx - 1);
+
+  // the value of element e0 + 2*mm (e0 = 0 or 1): selects between values
+  auto point = [&](int e0, int mm, T xm, T xp) {
+    auto pc = [&](int t) {
+      const T lo = c[t][2 * mm], hi = c[t][2 * mm + 1];
+      return e0 ? hi : lo;
+    };
+    auto pw = [&](int r, int d) {
+      const T lo = w[r][2 * mm + d], hi = w[r][2 * mm + 1 + d];
+      return e0 ? hi : lo;
+    };
+    T o = pc(0) * pw(1, 1);
+    o += pc(1) * pw(2, 1);
+    o += pc(2) * xp;
+    o += pc(3) * xm;
+    o += pc(4) * pw(1, 2);
+    o += pc(5) * pw(1, 0);
+    o += pc(6) * pw(2, 2);
+    o += pc(7) * pw(2, 0);
+    T u = pc(8) * pw(3, 1);
+    u += pc(9) * pw(4, 1);
+    u += pc(10) * pw(5, 1);
+    u += pc(11) * pw(3, 2);
+    u += pc(12) * pw(3, 0);
+    u += pc(13) * pw(4, 2);
+    u += pc(14) * pw(4, 0);
+    u += pc(15) * pw(5, 2);
+    u += pc(16) * pw(5, 0);
+    T d = pc(17) * pw(6, 1);
+    d += pc(18) * pw(7, 1);
+    d += pc(19) * pw(8, 1);
+    d += pc(20) * pw(6, 2);
+    d += pc(21) * pw(6, 0);
+    d += pc(22) * pw(7, 2);
+    d += pc(23) * pw(7, 0);
+    d += pc(24) * pw(8, 2);
+    d += pc(25) * pw(8, 0);
+    return (pc(26) + (o + u + d)) / pc(27);
+  };
+
+  // A points (column parity pa): row neighbours are old B values.  The
+  // element pairs are handled with constant indices (the calls below, not a
+  // loop), so every selected value is a register, never an address.
+  const int eA = (pa - ib) & 1, eB = eA ^ 1;
+  T av[kH];
+  auto stageA = [&](auto mmc) {
+    constexpr int mm = decltype(mmc)::value;
+    const T xm = eA ? w[0][2 * mm + 1] : w[0][2 * mm];
+    const T xp = eA ? w[0][2 * mm + 3] : w[0][2 * mm + 2];
+    av[mm] = point(eA, mm, xm, xp);
+    if (act) nrow[ib + eA + 2 * mm] = av[mm];
+  };
+  stageA(std::integral_constant<int, 0>{});
+  if constexpr (kH > 1) stageA(std::integral_constant<int, 1>{});
+  __syncthreads();
+  // B points: new A neighbours from the row buffer, boundary columns old
+  T out[kV];
+  auto stageB = [&](auto mmc) {
+    constexpr int mm = decltype(mmc)::value;
+    const int col = ib + eB + 2 * mm;
+    const T om = eB ? w[0][2 * mm + 1] : w[0][2 * mm];
+    const T op = eB ? w[0][2 * mm + 3] : w[0][2 * mm + 2];
+    const T xm = col - 1 >= 1 ? nrow[col - 1] : om;
+    const T xp = col + 1 <= nx - 2 ? nrow[col + 1] : op;
+    const T bv = point(eB, mm, xm, xp);
+    out[2 * mm] = eA ? bv : av[mm];
+    out[2 * mm + 1] = eA ? av[mm] : bv;
+  };
+  stageB(std::integral_constant<int, 0>{});
+  if constexpr (kH > 1) stageB(std::integral_constant<int, 1>{});
+  if (act) vst16(x + p, out);
 }
 
 // The four colours of one plane parity of that sweep in one launch: colours
@@ -2233,11 +2407,35 @@ void gsrb27(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, 
   gsrb27_batch<T>(s, nx, ny, nz, &c, 1, nu, iadjoint);
 }
 
+// The colour-pair form: the vector kernel (form 0) where a row holds at
+// least one 16-byte run plus its boundary points and fits one workgroup, and
+// in FP32 from 80 points a row; else (or with form 1) the one-point-per-lane
+// kernel.  Per sweep (4 launches), vector against one point per lane, us:
+// FP32 81^3 23.1 / 28.1, 97^3 31.0 / 40.6, 129^3 55.1 / 78.7, but 65^3 19.9 /
+// 16.6 (a wavefront per 4 rows leaves too few wavefronts to hide latency);
+// FP64 65^3 20.8 / 23.3, 81^3 35.4 / 40.3, 97^3 45.6 / 79.6, 129^3 95.2 / 162.7.
 template <class T>
-void gsrb27_pairs_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb,
-                        int nu, int iadjoint) {
+void gsrb27_pairs_form(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb,
+                       int nu, int iadjoint, int form) {
   check_batch(nb);
   const Batch<Args27<T>> B = args27(c, nb);
+  constexpr int kV = 16 / (int)sizeof(T);
+  const int R = (nx - 2 + kV - 1) / kV;
+  const bool vec = form == 0 && nx >= kV + 2 && R <= kBlock && (sizeof(T) == 8 || nx >= 80);
+  // workgroup size for the vector kernel: whole rows, fewest idle lanes
+  int NT = kBlock;
+  if (vec) {
+    int idle = 1 << 30;
+    for (int t = 64; t <= kBlock; t += 64) {
+      if (t < R) continue;
+      const int id = t - (t / R) * R;
+      if (id * kBlock < idle * t) {  // compare idle fractions
+        idle = id;
+        NT = t;
+      }
+    }
+  }
+  const int RB = vec ? NT / R : 0;
   for (int it = 0; it < nu; it++)
     for (int h = 0; h < 4; h++) {
       // forward: (0,1) (2,3) (4,5) (6,7); adjoint: (7,6) (5,4) (3,2) (1,0)
@@ -2246,9 +2444,21 @@ void gsrb27_pairs_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T
       const int j0 = cj ? 1 : 2, k0 = ck ? 1 : 2;
       if (j0 > ny - 2 || k0 > nz - 2) continue;
       const long rows = (long)((ny - 2 - j0) / 2 + 1) * ((nz - 2 - k0) / 2 + 1);
-      const dim3 g((unsigned)((rows * 64 + kBlock - 1) / kBlock), 1, (unsigned)nb);
-      k_gsrb27_pair<T><<<g, kBlock, 0, s>>>(nx, ny, nz, B, pa, cj, ck);
+      if (vec) {
+        const dim3 g((unsigned)((rows + RB - 1) / RB), 1, (unsigned)nb);
+        k_gsrb27_vpair<T><<<g, NT, (size_t)RB * nx * sizeof(T), s>>>(nx, ny, nz, B, pa, cj, ck,
+                                                                      R, RB);
+      } else {
+        const dim3 g((unsigned)((rows * 64 + kBlock - 1) / kBlock), 1, (unsigned)nb);
+        k_gsrb27_pair<T><<<g, kBlock, 0, s>>>(nx, ny, nz, B, pa, cj, ck);
+      }
     }
+}
+
+template <class T>
+void gsrb27_pairs_batch(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb,
+                        int nu, int iadjoint) {
+  gsrb27_pairs_form<T>(s, nx, ny, nz, c, nb, nu, iadjoint, 0);
 }
 
 template <class T>
@@ -2714,6 +2924,8 @@ void dense_solve(hipStream_t s, int nx, int ny, int nz, const double* Minv, int 
   template void gsrb27_batch<T>(hipStream_t, int, int, int, const LevelArgs<T>*, int, int, int); \
   template void gsrb27_pairs_batch<T>(hipStream_t, int, int, int, const LevelArgs<T>*, int, int, \
                                       int);                                                     \
+  template void gsrb27_pairs_form<T>(hipStream_t, int, int, int, const LevelArgs<T>*, int, int,  \
+                                     int, int);                                                 \
   template void gsrb27_planes_batch<T>(hipStream_t, int, int, int, const LevelArgs<T>*, int,     \
                                        int, int, int);                                          \
   template void resid_batch<T>(hipStream_t, int, int, int, int, const LevelArgs<T>*, int);       \

@@ -224,7 +224,9 @@ int pmgtest_gs27_sweeps(int n, int word, int sweeps, int mode, double* us_per_sw
 
 // 27-point sweeps with the kernel form forced: mode 0 one launch per colour,
 // 1 per plane parity (tile by level size), 2 per colour pair (rows of up to
-// 128 interior points), 8 / 16 / 32 per plane parity with that tile
+// 128 interior points; the vector kernel where the row allows it), 3 per
+// colour pair with one point per lane, 8 / 16 / 32 per plane parity with
+// that tile
 int pmgtest_gsrb27_mode(int nx, int ny, int nz, const double* a, const double* cc,
                         const double* fc, double* x, int nu, int iadjoint, int mode) {
   return gsrb_impl(nx, ny, nz, 27, a, cc, fc, x, nu, iadjoint, mode);
@@ -279,9 +281,10 @@ int gsrb_impl(int nx, int ny, int nz, int st, const double* a, const double* cc,
     TCHECK(hipFree(dt));
   } else if (tile == 0) {
     pmg::gsrb27<double>(0, nx, ny, nz, A.ops, dD, dfc, dx, nu, iadjoint);
-  } else if (tile == 2) {
+  } else if (tile == 2 || tile == 3) {
     if (nx > 130) return 0;
-    pmg::gsrb27_pairs<double>(0, nx, ny, nz, A.ops, dD, dfc, dx, nu, iadjoint);
+    const pmg::LevelArgs<double> c{A.ops, dD, dfc, dx, nullptr, nullptr};
+    pmg::gsrb27_pairs_form<double>(0, nx, ny, nz, &c, 1, nu, iadjoint, tile - 2);
   } else {
     // the plane-parity sweeps; the partner buffer carries the boundary of x
     double* dt = up(x, n);
@@ -435,6 +438,35 @@ int pmgtest_fused32(int nx, int ny, int nz, const float* oE, const float* oN, co
   TCHECK(hipDeviceSynchronize());
   TCHECK(hipMemcpy(x, dx, n * sizeof(float), hipMemcpyDeviceToHost));
   for (float* q : {dE, dN, dU, dD, df, dx, dt}) TCHECK(hipFree(q));
+  return 1;
+}
+
+// FP32 27-point sweeps in a given form on explicit arrays (see the header)
+int pmgtest_gsrb27_32(int nx, int ny, int nz, const float* a, const float* D, const float* fc,
+                      float* x, int nu, int iadjoint, int mode) {
+  if (nx < 3 || ny < 3 || nz < 3 || nu < 1) return 0;
+  if ((mode == 2 || mode == 3) && nx > 130) return 0;
+  const size_t n = (size_t)nx * ny * nz;
+  auto upf = [&](const float* h) {
+    float* d = nullptr;
+    TCHECK(hipMalloc(&d, n * sizeof(float)));
+    TCHECK(hipMemcpy(d, h, n * sizeof(float), hipMemcpyHostToDevice));
+    return d;
+  };
+  pmg::Ops<float> ops{};
+  for (int s = 0; s < 14; s++) ops.a[s] = upf(a + (size_t)s * n);
+  float *dD = upf(D), *df = upf(fc), *dx = upf(x), *dt = upf(x);
+  const pmg::LevelArgs<float> c{ops, dD, df, dx, dt, nullptr};
+  if (mode == 0)
+    pmg::gsrb27_batch<float>(0, nx, ny, nz, &c, 1, nu, iadjoint);
+  else if (mode == 2 || mode == 3)
+    pmg::gsrb27_pairs_form<float>(0, nx, ny, nz, &c, 1, nu, iadjoint, mode - 2);
+  else
+    pmg::gsrb27_planes_batch<float>(0, nx, ny, nz, &c, 1, nu, iadjoint, mode == 1 ? 0 : mode);
+  TCHECK(hipDeviceSynchronize());
+  TCHECK(hipMemcpy(x, dx, n * sizeof(float), hipMemcpyDeviceToHost));
+  for (int s = 0; s < 14; s++) TCHECK(hipFree((void*)ops.a[s]));
+  for (float* q : {dD, df, dx, dt}) TCHECK(hipFree(q));
   return 1;
 }
 

@@ -62,10 +62,16 @@ int pmgtest_fused32(int nx, int ny, int nz, const float *oE, const float *oN, co
                     int zero);
 /* pmgtest_gsrb (27-point) with the kernel form forced: mode 0 one launch per
  * colour, 1 one launch per plane parity (tile by level size), 2 one launch
- * per colour pair (nx <= 130, else returns 0), 8 / 16 / 32 one launch per
- * plane parity with that square tile.  All give the same bits. */
+ * per colour pair (nx <= 130, else returns 0; the vector kernel where the
+ * row allows it), 3 the colour pair with one point per lane, 8 / 16 / 32 one
+ * launch per plane parity with that square tile.  All give the same bits. */
 int pmgtest_gsrb27_mode(int nx, int ny, int nz, const double *a, const double *cc,
                         const double *fc, double *x, int nu, int iadjoint, int mode);
+/* FP32 27-point sweeps in a given form (mode as pmgtest_gsrb27_mode) on
+ * explicit arrays: a holds the 14 stencil arrays (oC first, unused), D the
+ * diagonal; every array nx*ny*nz floats.  Result in x; 0 on bad sizes. */
+int pmgtest_gsrb27_32(int nx, int ny, int nz, const float *a, const float *D, const float *fc,
+                      float *x, int nu, int iadjoint, int mode);
 /* No reference counterpart: the 27-point sweeps alone on a synthetic n^3
  * coarse-level operator, word 8 or 4, mode as above; average microseconds
  * per sweep in *us_per_sweep. */

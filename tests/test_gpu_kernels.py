@@ -73,11 +73,12 @@ def test_gsrb_bitwise(L, O, st, n3, adj):
 
 @pytest.mark.parametrize("n3", [(17, 17, 17), (9, 11, 13), (45, 43, 41), (5, 5, 4),
                                 (70, 21, 6), (81, 81, 9), (3, 3, 3), (130, 7, 6), (129, 6, 7)])
-@pytest.mark.parametrize("mode", [1, 2, 8, 16, 32])
+@pytest.mark.parametrize("mode", [1, 2, 3, 8, 16, 32])
 def test_gsrb27_plane_sweeps_bitwise(L, O, n3, mode):
     """The plane-parity 27-point sweep (one launch per plane parity, tiles
-    with recomputed margins) and the colour-pair sweep (mode 2, one
-    wavefront per row) give the oracle's 8-colour sweep bit for bit: every
+    with recomputed margins) and the colour-pair sweep (mode 2 the vector
+    kernel, 16-byte runs per lane; mode 3 one wavefront per row) give the
+    oracle's 8-colour sweep bit for bit: every
     tile size, partial tiles, forward and adjoint, even and odd sweep
     counts, rows of 1 to 128 interior points."""
     rng = np.random.default_rng(sum(n3) + mode)
@@ -137,6 +138,32 @@ def test_fused_zero_start_bitwise(L, O, n3, tile):
         assert L.pmgtest_gsrb_zero(nx, ny, nz, P(a), P(cc), P(fc), P(xg), adj, tile) == 1
         O.orc_gsrb(nx, ny, nz, 7, P(a), P(cc), P(fc), P(xo), 2, adj, 1)
         assert np.array_equal(xg, xo), (n3, tile, adj)
+
+
+@pytest.mark.parametrize("n3", [(81, 81, 9), (45, 43, 41), (17, 17, 17), (9, 11, 13),
+                                (7, 6, 5), (130, 7, 6), (129, 6, 7)])
+@pytest.mark.parametrize("mode", [2, 3, 1])
+def test_gsrb27_fp32_forms_bitwise(L, n3, mode):
+    """FP32 27-point sweeps: the colour-pair kernels (vector and one point
+    per lane) and the plane-parity sweep give the per-colour kernel's bits
+    (that kernel is part27 term for term, and in FP64 it is the oracle's
+    8-colour sweep bit for bit), with a non-zero boundary."""
+    rng = np.random.default_rng(sum(n3) * 3 + mode)
+    nx, ny, nz = n3
+    sh = (nz, ny, nx)
+    f32 = np.float32
+    a = rng.uniform(0.05, 0.5, (14,) + sh).astype(f32)
+    D = rng.uniform(8.0, 16.0, sh).astype(f32)
+    fc = rng.normal(size=sh).astype(f32)
+    x0 = rng.normal(size=sh).astype(f32)
+    for adj in (0, 1):
+        for nu in (2, 3):
+            ref, got = x0.copy(), x0.copy()
+            assert L.pmgtest_gsrb27_32(nx, ny, nz, F32(a), F32(D), F32(fc), F32(ref), nu, adj,
+                                       0) == 1
+            assert L.pmgtest_gsrb27_32(nx, ny, nz, F32(a), F32(D), F32(fc), F32(got), nu, adj,
+                                       mode) == 1
+            assert np.array_equal(got, ref), (n3, mode, adj, nu)
 
 
 def rb_sweeps(oE, oN, uC, D, fc, x, first, nu):
