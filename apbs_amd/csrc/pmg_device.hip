@@ -946,6 +946,55 @@ __global__ void __launch_bounds__(kBlock) k_resid7(int nx, int ny, int nz, Batch
   resid7_body<T>(nx, ny, nz, a.A.a[1], a.A.a[2], a.A.a[3], a.D, a.f, a.x, a.y);
 }
 
+// The 7-point residual with one lane per 16-byte run of a row (4 floats or
+// 2 doubles): every operand is one 16-byte load, x of the row a window of
+// kV + 2 values, r one 16-byte store.  The last run of a row is moved inwards
+// so that every load stays inside its row (overlapping runs write the same
+// values).  acc7's terms and order, so the values are k_resid7's bit for bit.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_resid7v(int nx, int ny, int nz, Batch<Args27<T>> B) {
+  constexpr int kV = 16 / (int)sizeof(T);
+  const Args27<T>& a = B.v[blockIdx.z];
+  const T *oE = a.A.a[1], *oN = a.A.a[2], *uC = a.A.a[3], *D = a.D, *fc = a.f, *x = a.x;
+  T* r = a.y;
+  const int R = (nx - 2 + kV - 1) / kV;
+  const int t = (int)(blockIdx.x * kBlock + threadIdx.x);
+  const int run = t % R, row = t / R;
+  if (row >= (ny - 2) * (nz - 2)) return;
+  const int j = 1 + row % (ny - 2), k = 1 + row / (ny - 2);
+  const int ib = min(1 + kV * run, nx - 1 - kV);
+  const size_t nxy = (size_t)nx * ny;
+  const size_t p = (size_t)k * nxy + (size_t)j * nx + ib;
+  T vf[kV], vN[kV], vNm[kV], vE[kV], vEm[kV], vUm[kV], vU[kV], vD[kV];
+  T xn[kV], xs[kV], xu[kV], xd[kV], xw[kV + 2];
+  vld16(vf, fc + p);
+  vld16(vN, oN + p);
+  vld16(vNm, oN + p - nx);
+  vld16(vE, oE + p);
+  vld16(vEm, oE + p - 1);
+  vld16(vUm, uC + p - nxy);
+  vld16(vU, uC + p);
+  vld16(vD, D + p);
+  vld16(xn, x + p + nx);
+  vld16(xs, x + p - nx);
+  vld16(xu, x + p + nxy);
+  vld16(xd, x + p - nxy);
+  vldw<T, kV>(xw, x + p - 1);
+  T out[kV];
+#pragma unroll
+  for (int e = 0; e < kV; e++) {
+    T sv = vf[e];
+    sv += vN[e] * xn[e];
+    sv += vNm[e] * xs[e];
+    sv += vE[e] * xw[e + 2];
+    sv += vEm[e] * xw[e];
+    sv += vUm[e] * xd[e];
+    sv += vU[e] * xu[e];
+    out[e] = sv - vD[e] * xw[e + 1];
+  }
+  vst16(r + p, out);
+}
+
 template <class T>
 __device__ __forceinline__ void resid27_body(int nx, int ny, int nz, const Ops27<T>& A,
                                              const T* __restrict__ D, const T* __restrict__ fc,
@@ -2531,7 +2580,12 @@ void resid_batch(hipStream_t s, int st, int nx, int ny, int nz, const LevelArgs<
   dim3 g = grid_interior(nx, ny, nz);
   g.z = (unsigned)nb;
   const Batch<Args27<T>> B = args27(c, nb);
-  if (st == 7)
+  constexpr int kV = 16 / (int)sizeof(T);
+  if (st == 7 && nx >= kV + 2) {
+    const long lanes = (long)((nx - 2 + kV - 1) / kV) * (ny - 2) * (nz - 2);
+    const dim3 gv((unsigned)((lanes + kBlock - 1) / kBlock), 1, (unsigned)nb);
+    k_resid7v<T><<<gv, kBlock, 0, s>>>(nx, ny, nz, B);
+  } else if (st == 7)
     k_resid7<T><<<g, kBlock, 0, s>>>(nx, ny, nz, B);
   else
     k_resid27<T><<<g, kBlock, 0, s>>>(nx, ny, nz, B);

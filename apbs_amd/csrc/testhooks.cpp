@@ -470,4 +470,29 @@ int pmgtest_gsrb27_32(int nx, int ny, int nz, const float* a, const float* D, co
   return 1;
 }
 
+// FP32 7-point residual r = f + sum(a x) - D x on explicit arrays (see the
+// header); r's boundary is left as the caller passed it
+int pmgtest_resid32(int nx, int ny, int nz, const float* oE, const float* oN, const float* uC,
+                    const float* D, const float* fc, const float* x, float* r) {
+  if (nx < 3 || ny < 3 || nz < 3) return 0;
+  const size_t n = (size_t)nx * ny * nz;
+  auto upf = [&](const float* h) {
+    float* d = nullptr;
+    TCHECK(hipMalloc(&d, n * sizeof(float)));
+    TCHECK(hipMemcpy(d, h, n * sizeof(float), hipMemcpyHostToDevice));
+    return d;
+  };
+  float *dE = upf(oE), *dN = upf(oN), *dU = upf(uC), *dD = upf(D), *df = upf(fc), *dx = upf(x),
+        *dr = upf(r);
+  pmg::Ops<float> ops{};
+  ops.a[1] = dE;
+  ops.a[2] = dN;
+  ops.a[3] = dU;
+  pmg::resid<float>(0, 7, nx, ny, nz, ops, dD, df, dx, dr);
+  TCHECK(hipDeviceSynchronize());
+  TCHECK(hipMemcpy(r, dr, n * sizeof(float), hipMemcpyDeviceToHost));
+  for (float* q : {dE, dN, dU, dD, df, dx, dr}) TCHECK(hipFree(q));
+  return 1;
+}
+
 }  // extern "C"

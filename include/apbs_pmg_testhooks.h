@@ -72,6 +72,10 @@ int pmgtest_gsrb27_mode(int nx, int ny, int nz, const double *a, const double *c
  * diagonal; every array nx*ny*nz floats.  Result in x; 0 on bad sizes. */
 int pmgtest_gsrb27_32(int nx, int ny, int nz, const float *a, const float *D, const float *fc,
                       float *x, int nu, int iadjoint, int mode);
+/* FP32 7-point residual r = fc + sum(a x) - D x on the interior (oE, oN, uC,
+ * D, fc, x, r: nx*ny*nz floats; r's boundary is kept).  0 on bad sizes. */
+int pmgtest_resid32(int nx, int ny, int nz, const float *oE, const float *oN, const float *uC,
+                    const float *D, const float *fc, const float *x, float *r);
 /* No reference counterpart: the 27-point sweeps alone on a synthetic n^3
  * coarse-level operator, word 8 or 4, mode as above; average microseconds
  * per sweep in *us_per_sweep. */

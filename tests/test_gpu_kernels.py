@@ -140,6 +140,38 @@ def test_fused_zero_start_bitwise(L, O, n3, tile):
         assert np.array_equal(xg, xo), (n3, tile, adj)
 
 
+@pytest.mark.parametrize("n3", [(161, 23, 9), (81, 81, 9), (17, 19, 21), (7, 9, 11), (5, 8, 9),
+                                (70, 6, 5)])
+def test_resid7_fp32_bitwise(L, n3):
+    """The FP32 7-point residual (vector form from 6 points a row, one point
+    per thread below) against numpy FP32 in acc7's order, bit for bit; r's
+    boundary untouched."""
+    rng = np.random.default_rng(sum(n3))
+    nx, ny, nz = n3
+    sh = (nz, ny, nx)
+    f32 = np.float32
+    oE, oN, uC = (rng.uniform(0.1, 1.0, sh).astype(f32) for _ in range(3))
+    D = rng.uniform(6.0, 12.0, sh).astype(f32)
+    fc, x = rng.normal(size=sh).astype(f32), rng.normal(size=sh).astype(f32)
+    r = np.full(sh, 7.0, f32)
+    assert L.pmgtest_resid32(nx, ny, nz, F32(oE), F32(oN), F32(uC), F32(D), F32(fc), F32(x),
+                             F32(r)) == 1
+
+    def s_(a, dk, dj, di):
+        return a[1 + dk:nz - 1 + dk, 1 + dj:ny - 1 + dj, 1 + di:nx - 1 + di]
+
+    acc = s_(fc, 0, 0, 0).copy()
+    acc = acc + s_(oN, 0, 0, 0) * s_(x, 0, 1, 0)
+    acc = acc + s_(oN, 0, -1, 0) * s_(x, 0, -1, 0)
+    acc = acc + s_(oE, 0, 0, 0) * s_(x, 0, 0, 1)
+    acc = acc + s_(oE, 0, 0, -1) * s_(x, 0, 0, -1)
+    acc = acc + s_(uC, -1, 0, 0) * s_(x, -1, 0, 0)
+    acc = acc + s_(uC, 0, 0, 0) * s_(x, 1, 0, 0)
+    ref = np.full(sh, 7.0, f32)
+    ref[1:-1, 1:-1, 1:-1] = acc - s_(D, 0, 0, 0) * s_(x, 0, 0, 0)
+    assert np.array_equal(r, ref), n3
+
+
 @pytest.mark.parametrize("n3", [(81, 81, 9), (45, 43, 41), (17, 17, 17), (9, 11, 13),
                                 (7, 6, 5), (130, 7, 6), (129, 6, 7)])
 @pytest.mark.parametrize("mode", [2, 3, 1])
@@ -255,10 +287,12 @@ def test_fused_fp32_bitwise(L, n3, tile):
         assert np.array_equal(xg, xr), (tile, adj, "zero start")
 
 
-@pytest.mark.parametrize("st", [7, 27])
-def test_resid_bitwise(L, O, st):
+@pytest.mark.parametrize("st,n3", [(7, (21, 17, 19)), (27, (21, 17, 19)), (7, (161, 9, 7)),
+                                  (7, (5, 6, 5)), (7, (70, 5, 4))])
+def test_resid_bitwise(L, O, st, n3):
+    """The residual (for st 7 the vector form: 2-point runs) against the
+    oracle, bit for bit, on the interior."""
     rng = np.random.default_rng(11)
-    n3 = (21, 17, 19)
     nx, ny, nz = n3
     n = nx * ny * nz
     a = rand_op(rng, n3, st)
