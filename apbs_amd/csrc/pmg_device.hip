@@ -287,22 +287,22 @@ __global__ void __launch_bounds__(kNT)
 // vectors of the runs beside them.
 template <class T, int N>
 __device__ __forceinline__ void vld16(T (&v)[N], const T* p) {
-  static_assert(sizeof(T) * N == 16, "16-byte vectors");
+  static_assert(sizeof(T) * N == 16 || sizeof(T) * N == 8, "8- or 16-byte vectors");
   typedef T vT __attribute__((ext_vector_type(N)));
   vT a;
-  __builtin_memcpy(&a, p, 16);
+  __builtin_memcpy(&a, p, sizeof(T) * N);
 #pragma unroll
   for (int i = 0; i < N; i++) v[i] = a[i];
 }
 
 template <class T, int N>
 __device__ __forceinline__ void vst16(T* p, const T (&v)[N]) {
-  static_assert(sizeof(T) * N == 16, "16-byte vectors");
+  static_assert(sizeof(T) * N == 16 || sizeof(T) * N == 8, "8- or 16-byte vectors");
   typedef T vT __attribute__((ext_vector_type(N)));
   vT a;
 #pragma unroll
   for (int i = 0; i < N; i++) a[i] = v[i];
-  __builtin_memcpy(p, &a, 16);
+  __builtin_memcpy(p, &a, sizeof(T) * N);
 }
 
 template <class T, int kTY, int kKC, int kTX, int kNT, bool kZeroX>
@@ -676,12 +676,12 @@ __global__ void __launch_bounds__(kBlock)
 // its order, so the results are the scalar form's bit for bit.
 template <class T, int N>
 __device__ __forceinline__ void vldw(T (&w)[N + 2], const T* p) {
-  static_assert(sizeof(T) * N == 16, "16-byte vectors");
+  static_assert(sizeof(T) * N == 16 || sizeof(T) * N == 8, "8- or 16-byte vectors");
   typedef T vT __attribute__((ext_vector_type(N)));
   typedef T v2 __attribute__((ext_vector_type(2)));
   vT a;
   v2 b;
-  __builtin_memcpy(&a, p, 16);
+  __builtin_memcpy(&a, p, sizeof(T) * N);
   __builtin_memcpy(&b, p + N, 2 * sizeof(T));
 #pragma unroll
   for (int i = 0; i < N; i++) w[i] = a[i];
@@ -690,11 +690,11 @@ __device__ __forceinline__ void vldw(T (&w)[N + 2], const T* p) {
 }
 
 // R runs per row, RB rows per workgroup; LDS holds RB rows of new A values.
-template <class T>
+template <class T, int kV>
 __global__ void __launch_bounds__(kBlock)
     k_gsrb27_vpair(int nx, int ny, int nz, Batch<Args27<T>> B, int pa, int cj, int ck, int R,
                    int RB) {
-  constexpr int kV = 16 / (int)sizeof(T), kH = kV / 2;
+  constexpr int kH = kV / 2;
   extern __shared__ unsigned char vpair_smem[];
   T* na = reinterpret_cast<T*>(vpair_smem);
   const Args27<T>& ar = B.v[blockIdx.z];
@@ -2456,35 +2456,22 @@ void gsrb27(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T* D, 
   gsrb27_batch<T>(s, nx, ny, nz, &c, 1, nu, iadjoint);
 }
 
-// The colour-pair form: the vector kernel (form 0) where a row holds at
-// least one 16-byte run plus its boundary points and fits one workgroup, and
-// in FP32 from 80 points a row; else (or with form 1) the one-point-per-lane
-// kernel.  Per sweep (4 launches), vector against one point per lane, us:
-// FP32 81^3 23.1 / 28.1, 97^3 31.0 / 40.6, 129^3 55.1 / 78.7, but 65^3 19.9 /
-// 16.6 (a wavefront per 4 rows leaves too few wavefronts to hide latency);
-// FP64 65^3 20.8 / 23.3, 81^3 35.4 / 40.3, 97^3 45.6 / 79.6, 129^3 95.2 / 162.7.
-template <class T>
-void gsrb27_pairs_form(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb,
-                       int nu, int iadjoint, int form) {
-  check_batch(nb);
-  const Batch<Args27<T>> B = args27(c, nb);
-  constexpr int kV = 16 / (int)sizeof(T);
+// The colour-pair sweeps with the vector kernel of kV-element runs.
+template <class T, int kV>
+static void vpair_launch(hipStream_t s, int nx, int ny, int nz, const Batch<Args27<T>>& B,
+                         int nb, int nu, int iadjoint) {
   const int R = (nx - 2 + kV - 1) / kV;
-  const bool vec = form == 0 && nx >= kV + 2 && R <= kBlock && (sizeof(T) == 8 || nx >= 80);
-  // workgroup size for the vector kernel: whole rows, fewest idle lanes
-  int NT = kBlock;
-  if (vec) {
-    int idle = 1 << 30;
-    for (int t = 64; t <= kBlock; t += 64) {
-      if (t < R) continue;
-      const int id = t - (t / R) * R;
-      if (id * kBlock < idle * t) {  // compare idle fractions
-        idle = id;
-        NT = t;
-      }
+  // workgroup size: whole rows, fewest idle lanes
+  int NT = kBlock, idle = 1 << 30;
+  for (int t = 64; t <= kBlock; t += 64) {
+    if (t < R) continue;
+    const int id = t - (t / R) * R;
+    if (id * kBlock < idle * t) {  // compare idle fractions
+      idle = id;
+      NT = t;
     }
   }
-  const int RB = vec ? NT / R : 0;
+  const int RB = NT / R;
   for (int it = 0; it < nu; it++)
     for (int h = 0; h < 4; h++) {
       // forward: (0,1) (2,3) (4,5) (6,7); adjoint: (7,6) (5,4) (3,2) (1,0)
@@ -2493,14 +2480,49 @@ void gsrb27_pairs_form(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>
       const int j0 = cj ? 1 : 2, k0 = ck ? 1 : 2;
       if (j0 > ny - 2 || k0 > nz - 2) continue;
       const long rows = (long)((ny - 2 - j0) / 2 + 1) * ((nz - 2 - k0) / 2 + 1);
-      if (vec) {
-        const dim3 g((unsigned)((rows + RB - 1) / RB), 1, (unsigned)nb);
-        k_gsrb27_vpair<T><<<g, NT, (size_t)RB * nx * sizeof(T), s>>>(nx, ny, nz, B, pa, cj, ck,
-                                                                      R, RB);
-      } else {
-        const dim3 g((unsigned)((rows * 64 + kBlock - 1) / kBlock), 1, (unsigned)nb);
-        k_gsrb27_pair<T><<<g, kBlock, 0, s>>>(nx, ny, nz, B, pa, cj, ck);
-      }
+      const dim3 g((unsigned)((rows + RB - 1) / RB), 1, (unsigned)nb);
+      k_gsrb27_vpair<T, kV><<<g, NT, (size_t)RB * nx * sizeof(T), s>>>(nx, ny, nz, B, pa, cj, ck,
+                                                                       R, RB);
+    }
+}
+
+// The colour-pair form.  Form 0 (the default): the vector kernel where a row
+// holds at least one run plus its boundary points and fits one workgroup,
+// with 16-byte runs in FP64 and 8-byte runs in FP32 (16-byte runs for rows
+// of 80 to 90 points), else the one-point-per-lane kernel; form 1 that
+// kernel; form 2 the vector kernel with 8-byte runs (FP32).  Per sweep (4
+// launches), us, FP32 8-byte runs / 16-byte runs / one point per lane: 57^3
+// 14.9 / - / 16.4, 65^3 15.3 / 19.9 / 16.6, 81^3 23.7 / 23.1 / 28.1, 97^3
+// 29.2 / 31.0 / 40.1, 113^3 43.1 / 45.0 / 57.4, 129^3 53.2 / 54.9 / 75.1;
+// FP64 16-byte runs / one point per lane: 65^3 20.8 / 23.3, 81^3 35.4 /
+// 40.3, 97^3 45.6 / 79.6, 129^3 95.2 / 162.7.
+template <class T>
+void gsrb27_pairs_form(hipStream_t s, int nx, int ny, int nz, const LevelArgs<T>* c, int nb,
+                       int nu, int iadjoint, int form) {
+  check_batch(nb);
+  const Batch<Args27<T>> B = args27(c, nb);
+  constexpr int kV = 16 / (int)sizeof(T), kV8 = 8 / (int)sizeof(T);
+  const bool wide = sizeof(T) == 8 || (nx >= 80 && nx <= 90);
+  if (form == 0 && wide && nx >= kV + 2 && (nx - 2 + kV - 1) / kV <= kBlock) {
+    vpair_launch<T, kV>(s, nx, ny, nz, B, nb, nu, iadjoint);
+    return;
+  }
+  if constexpr (kV8 >= 2) {
+    if ((form == 2 || (form == 0 && !wide)) && nx >= kV8 + 2 &&
+        (nx - 2 + kV8 - 1) / kV8 <= kBlock) {
+      vpair_launch<T, kV8>(s, nx, ny, nz, B, nb, nu, iadjoint);
+      return;
+    }
+  }
+  for (int it = 0; it < nu; it++)
+    for (int h = 0; h < 4; h++) {
+      const int col = iadjoint ? 7 - 2 * h : 2 * h;
+      const int pa = col & 1, cj = (col >> 1) & 1, ck = (col >> 2) & 1;
+      const int j0 = cj ? 1 : 2, k0 = ck ? 1 : 2;
+      if (j0 > ny - 2 || k0 > nz - 2) continue;
+      const long rows = (long)((ny - 2 - j0) / 2 + 1) * ((nz - 2 - k0) / 2 + 1);
+      const dim3 g((unsigned)((rows * 64 + kBlock - 1) / kBlock), 1, (unsigned)nb);
+      k_gsrb27_pair<T><<<g, kBlock, 0, s>>>(nx, ny, nz, B, pa, cj, ck);
     }
 }
 

@@ -110,8 +110,10 @@ double gs27_sweeps_us(int n, int sweeps, int mode) {
   auto run = [&](int nu) {
     if (mode == 0)
       pmg::gsrb27<T>(0, n, n, n, ops, D, fc, x, nu, 0);
-    else if (mode == 2)
-      pmg::gsrb27_pairs<T>(0, n, n, n, ops, D, fc, x, nu, 0);
+    else if (mode >= 2 && mode <= 4) {
+      const pmg::LevelArgs<T> c{ops, D, fc, x, nullptr, nullptr};
+      pmg::gsrb27_pairs_form<T>(0, n, n, n, &c, 1, nu, 0, mode - 2);
+    }
     else
       pmg::gsrb27_planes<T>(0, n, n, n, ops, D, fc, x, xs, nu, 0, mode);
   };
@@ -445,7 +447,7 @@ int pmgtest_fused32(int nx, int ny, int nz, const float* oE, const float* oN, co
 int pmgtest_gsrb27_32(int nx, int ny, int nz, const float* a, const float* D, const float* fc,
                       float* x, int nu, int iadjoint, int mode) {
   if (nx < 3 || ny < 3 || nz < 3 || nu < 1) return 0;
-  if ((mode == 2 || mode == 3) && nx > 130) return 0;
+  if (mode >= 2 && mode <= 4 && nx > 130) return 0;
   const size_t n = (size_t)nx * ny * nz;
   auto upf = [&](const float* h) {
     float* d = nullptr;
@@ -459,7 +461,7 @@ int pmgtest_gsrb27_32(int nx, int ny, int nz, const float* a, const float* D, co
   const pmg::LevelArgs<float> c{ops, dD, df, dx, dt, nullptr};
   if (mode == 0)
     pmg::gsrb27_batch<float>(0, nx, ny, nz, &c, 1, nu, iadjoint);
-  else if (mode == 2 || mode == 3)
+  else if (mode >= 2 && mode <= 4)
     pmg::gsrb27_pairs_form<float>(0, nx, ny, nz, &c, 1, nu, iadjoint, mode - 2);
   else
     pmg::gsrb27_planes_batch<float>(0, nx, ny, nz, &c, 1, nu, iadjoint, mode == 1 ? 0 : mode);

@@ -67,7 +67,8 @@ int pmgtest_fused32(int nx, int ny, int nz, const float *oE, const float *oN, co
  * launch per plane parity with that square tile.  All give the same bits. */
 int pmgtest_gsrb27_mode(int nx, int ny, int nz, const double *a, const double *cc,
                         const double *fc, double *x, int nu, int iadjoint, int mode);
-/* FP32 27-point sweeps in a given form (mode as pmgtest_gsrb27_mode) on
+/* FP32 27-point sweeps in a given form (mode as pmgtest_gsrb27_mode, and 4
+ * the colour-pair vector kernel with 8-byte runs) on
  * explicit arrays: a holds the 14 stencil arrays (oC first, unused), D the
  * diagonal; every array nx*ny*nz floats.  Result in x; 0 on bad sizes. */
 int pmgtest_gsrb27_32(int nx, int ny, int nz, const float *a, const float *D, const float *fc,

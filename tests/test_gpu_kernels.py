@@ -174,7 +174,7 @@ def test_resid7_fp32_bitwise(L, n3):
 
 @pytest.mark.parametrize("n3", [(81, 81, 9), (45, 43, 41), (17, 17, 17), (9, 11, 13),
                                 (7, 6, 5), (130, 7, 6), (129, 6, 7)])
-@pytest.mark.parametrize("mode", [2, 3, 1])
+@pytest.mark.parametrize("mode", [2, 3, 4, 1])
 def test_gsrb27_fp32_forms_bitwise(L, n3, mode):
     """FP32 27-point sweeps: the colour-pair kernels (vector and one point
     per lane) and the plane-parity sweep give the per-colour kernel's bits
