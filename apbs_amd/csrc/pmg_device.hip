@@ -2266,8 +2266,8 @@ void vfused_dispatch(hipStream_t s, int rp, int nx, int ny, int nz, const Batch<
   }
 }
 
-// The default shapes (4 rows x 8 planes, 8 rows x 8 planes with 512
-// threads) and the vector shapes also come in the zero-start form; other
+// The scalar shapes 4 rows x 8 planes and 8 rows x 8 planes with 512
+// threads and every vector shape also come in the zero-start form; other
 // shapes start from a cleared x.
 static bool zero_start_shape(int rp) { return rp == 48 || rp == 5088 || vector_shape(rp); }
 
