@@ -2296,21 +2296,29 @@ static int fused_tile_env() {
 // the best shape moves with how the workgroups fill the chip):
 //   FP32: up to 2.5M points 4 x 8 (65^3 12.2, 97^3 15.0, 129^3 20.3); up to
 //         5.5M 4 x 10 (161^3 29.7 against 36.5 for 4 x 8 and 50.2 for the
-//         scalar form); above 8 x 16 (193^3 49.3, 353^3 421 against 70.8 and
-//         573 for the scalar 8 x 8 with 512 threads).
+//         scalar form); up to 20M 8 x 16 (193^3 49.3 against 70.8 for the
+//         scalar 8 x 8 with 512 threads); above 8 x 16 in 96-column tiles
+//         (353^3 403, against 421 in 64-column tiles and 573 scalar).
 //   FP64: up to 0.5M points 4 x 8 (65^3 12.9); up to 1M 4 x 10 (97^3 18.7);
 //         up to 3M 4 x 12 (129^3 31.8); up to 5.5M 4 x 10 (161^3 48.9 against
 //         65.4 scalar); up to 20M 4 x 12 (193^3 125.4 against 137.9); above
-//         4 x 16 (353^3 822 against 931).
+//         4 x 12 in 80-column tiles (353^3 786, against 822 for 4 x 16 in
+//         64-column tiles and 931 scalar).
+// The column width otherwise follows fused_width; at 353^3 the widths above
+// leave fewer idle lanes (a run is 16 bytes, the workgroup whole wavefronts).
 int fused_tile(int nx, int ny, int nz, int word) {
   if (const int t = fused_tile_env()) return t;
   const long n = (long)nx * ny * nz;
-  if (word == 4) return n <= 2500000L ? 7048 : n <= 5500000L ? 7410 : 7816;
+  if (word == 4) {
+    if (n <= 2500000L) return 7048;
+    if (n <= 5500000L) return 7410;
+    return n <= 20000000L ? 7816 : 967816;
+  }
   if (n <= 500000L) return 7048;
   if (n <= 1000000L) return 7410;
   if (n <= 3000000L) return 7412;
   if (n <= 5500000L) return 7410;
-  return n <= 20000000L ? 7412 : 7416;
+  return n <= 20000000L ? 7412 : 807412;
 }
 
 // Whether the default fused sweep of a grid runs in the vector form.

@@ -99,7 +99,7 @@ def test_gsrb27_plane_sweeps_bitwise(L, O, n3, mode):
 @pytest.mark.parametrize("tile", [416, 48, 816, 88, 84, 1616, 832, 44, 68, 640048, 800048,
                                   960048, 800044, 800068, 960416, 800816, 5088, 645084,
                                   805816, 7048, 7088, 647048, 807088, 7410, 807412, 967416,
-                                  647410, 807816])
+                                  647410, 807816, 807412, 967816])
 @pytest.mark.parametrize("n3", [(133, 21, 38), (133, 21, 41)])
 def test_fused_sweep_tiles_bitwise(L, O, tile, n3):
     """Every tile shape of the fused red-black sweep gives the oracle's
@@ -256,7 +256,7 @@ def F32(a):
 
 @pytest.mark.parametrize("n3", [(133, 21, 38), (133, 21, 41), (17, 19, 21), (161, 23, 29),
                                 (7, 9, 11), (5, 8, 9)])
-@pytest.mark.parametrize("tile", [0, 48, 5088, 7048, 7088, 7410, 7816, 647412, 967416])
+@pytest.mark.parametrize("tile", [0, 48, 5088, 7048, 7088, 7410, 7816, 647412, 967416, 967816])
 def test_fused_fp32_bitwise(L, n3, tile):
     """The FP32 fused sweeps (scalar and vector forms, full and zero-start)
     against the numpy FP32 sweeps, bit for bit, with a non-zero boundary (a
