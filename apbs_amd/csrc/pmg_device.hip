@@ -1128,14 +1128,16 @@ struct FinishArgs {
 };
 
 // out[0] = sum(part[0..kParts-1]); ratio: out[0] = sum(a)/sum(b)
-// (and hout[0] = out[0] when hout is given); calculation blockIdx.x
-__global__ void __launch_bounds__(kBlock) k_finish(Batch<FinishArgs> B, int ratio) {
+// (and hout[0] = out[0] when hout is given); calculation blockIdx.x.  Only
+// the first nparts partial sums are read: the others are taken as +0.0, which
+// leaves the sum's bits as they are (no partial sum is -0.0).
+__global__ void __launch_bounds__(kBlock) k_finish(Batch<FinishArgs> B, int ratio, int nparts) {
   __shared__ double sh[kBlock];
   const double* part = B.v[blockIdx.x].part;
   double* out = B.v[blockIdx.x].out;
   double* hout = B.v[blockIdx.x].hout;
   double a = 0.0, b = 0.0;
-  for (int t = threadIdx.x; t < kParts; t += kBlock) {
+  for (int t = threadIdx.x; t < nparts; t += kBlock) {
     a += part[t];
     if (ratio) b += part[kParts + t];
   }
@@ -2682,7 +2684,7 @@ void nrm1(hipStream_t s, int nx, int ny, int nz, const T* x, double* part, doubl
   {
     Batch<FinishArgs> F;
     F.v[0] = FinishArgs{part, out, hout};
-    k_finish<<<1, kBlock, 0, s>>>(F, 0);
+    k_finish<<<1, kBlock, 0, s>>>(F, 0, kParts);
   }
 }
 
@@ -2694,7 +2696,7 @@ void resid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<T>& A, const T
   {
     Batch<FinishArgs> F;
     F.v[0] = FinishArgs{part, out, hout};
-    k_finish<<<1, kBlock, 0, s>>>(F, 0);
+    k_finish<<<1, kBlock, 0, s>>>(F, 0, kParts);
   }
 }
 
@@ -2703,7 +2705,7 @@ void nrm1_to32(hipStream_t s, int nx, int ny, int nz, const double* x, float* y,
   k_nrm1_to32<<<kParts, kBlock, 0, s>>>(nx, ny, nz, x, y, part);
   Batch<FinishArgs> F;
   F.v[0] = FinishArgs{part, out, hout};
-  k_finish<<<1, kBlock, 0, s>>>(F, 0);
+  k_finish<<<1, kBlock, 0, s>>>(F, 0, kParts);
 }
 
 void nresid7_xpay_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A,
@@ -2714,7 +2716,7 @@ void nresid7_xpay_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>&
                                            a, z, y, r, ions, part);
   Batch<FinishArgs> F;
   F.v[0] = FinishArgs{part, out, hout};
-  k_finish<<<1, kBlock, 0, s>>>(F, 0);
+  k_finish<<<1, kBlock, 0, s>>>(F, 0, kParts);
 }
 
 void jres_trial_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A,
@@ -2733,7 +2735,7 @@ void jres_trial_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A
   Batch<FinishArgs> F;
   F.v[0] = FinishArgs{part, out, hout};
   F.v[1] = FinishArgs{part + kParts, out2, hout2};
-  k_finish<<<2, kBlock, 0, s>>>(F, 0);
+  k_finish<<<2, kBlock, 0, s>>>(F, 0, kParts);
 }
 
 void nresid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, const double* cc,
@@ -2744,7 +2746,7 @@ void nresid7_nrm1(hipStream_t s, int nx, int ny, int nz, const Ops<double>& A, c
   {
     Batch<FinishArgs> F;
     F.v[0] = FinishArgs{part, out, hout};
-    k_finish<<<1, kBlock, 0, s>>>(F, 0);
+    k_finish<<<1, kBlock, 0, s>>>(F, 0, kParts);
   }
 }
 
@@ -2757,8 +2759,12 @@ void damp_factor_batch(hipStream_t s, int nx, int ny, int nz, const DampArgs<T>*
     B.v[b] = DampPart<T>{to27(c[b].A), c[b].D, c[b].x, c[b].w0, c[b].part};
     F.v[b] = FinishArgs{c[b].part, c[b].out, nullptr};
   }
-  k_damp_part<T><<<dim3(kParts, 1, nb), kBlock, 0, s>>>(nx, ny, nz, B);
-  k_finish<<<nb, kBlock, 0, s>>>(F, 1);
+  // a level of fewer than kParts * kBlock interior points leaves the blocks
+  // past its last point empty (partial sums +0.0): they are not launched
+  const long n = (long)(nx - 2) * (ny - 2) * (nz - 2);
+  const int np = (int)std::min<long>(kParts, std::max<long>(1, (n + kBlock - 1) / kBlock));
+  k_damp_part<T><<<dim3(np, 1, nb), kBlock, 0, s>>>(nx, ny, nz, B);
+  k_finish<<<nb, kBlock, 0, s>>>(F, 1, np);
 }
 
 template <class T>
