@@ -2003,13 +2003,17 @@ __global__ void __launch_bounds__(kBlock)
 // ------------------------------------------------------------ dense coarse
 __global__ void k_dense_assemble(int nx, int ny, int nz, Batch<InvertArgs> B, int N) {
   const int r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= N) return;
   const Ops<double>& A = B.v[blockIdx.y].A;
   const double* D = B.v[blockIdx.y].D;
   double* M = B.v[blockIdx.y].M;
+  // the workgroup's rows are cleared with consecutive stores first
+  const int r0 = blockIdx.x * kBlock, rows = min(kBlock, N - r0);
+  double* Mb = M + (size_t)r0 * N;
+  for (int t = threadIdx.x; t < rows * N; t += kBlock) Mb[t] = 0.0;
+  __syncthreads();
+  if (r >= N) return;
   const int ni = nx - 2, nj = ny - 2;
   const int i = r % ni + 1, j = (r / ni) % nj + 1, k = r / (ni * nj) + 1;
-  for (int c = 0; c < N; c++) M[(size_t)r * N + c] = 0.0;
   M[(size_t)r * N + r] = D[IX(i, j, k, nx, ny)];
   // couplings to all 26 neighbours: forward ones from this point's
   // coefficients, backward ones from the neighbour's
